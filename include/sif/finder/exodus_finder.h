@@ -1,0 +1,101 @@
+/*
+ * The exodus spherical void finder. Entry points, extras and variants of
+ * include/sif/finder/exodus_finder.h:40-407, :560-723 (sif v0.5.0).
+ *
+ * The grid must be a density contrast and the mesh must span the same box.
+ * The finder smooths on the GPU (Stockham c2r with the top-hat and the CIC
+ * window fused), scans and sorts the candidates on the GPU, grows spheres one
+ * wavefront per candidate, and commits in candidate order on the host, which
+ * makes the catalogue independent of batching and launch shapes.
+ */
+#ifndef SIF_FINDER_EXODUS_FINDER_H
+#define SIF_FINDER_EXODUS_FINDER_H
+
+#include "sif/core/macros.h"
+#include "sif/structures/catalogue.h"
+#include "sif/structures/chain_mesh.h"
+#include "sif/structures/field.h"
+#include "sif/structures/grid.h"
+
+SIF_EXTERN_C_BEGIN
+
+/* exodus_finder.h:40-106 */
+typedef struct {
+  const char* checkpoint;
+  const sif_catalogue_t* prior;
+  int keep_checkpoint;
+  int save_memory;
+} sif_finder_exodus_extras_t;
+
+/* exodus_finder.h:124-130 */
+typedef enum {
+  SIF_EXODUS_EXCURSION_SET = 0,
+  SIF_EXODUS_EXCURSION_PEAK,
+  SIF_EXODUS_LAGRANGIAN_OVERLAP,
+  SIF__EXODUS_N_VARIANTS
+} sif_exodus_variant_t;
+
+/* exodus_finder.h:147 */
+SIF_API SIF_NODISCARD int sif_finder_exodus_preset(
+  sif_exodus_variant_t variant, sif_real* overlap_fraction, sif_option* opt);
+SIF_API const char* sif_finder_exodus_variant_name(sif_exodus_variant_t variant);
+SIF_API SIF_NODISCARD int sif_finder_exodus_variant_from_name(
+  const char* name, sif_exodus_variant_t* variant);
+
+/* exodus_finder.h:184 */
+SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus(sif_grid_t* grid,
+  const sif_chain_mesh_t* mesh, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_exodus_variant_t variant,
+  const sif_finder_exodus_extras_t* extras);
+
+/* exodus_finder.h:300. Box runs with SIF_FINDER_SEEDS_CELLS on their nodes
+ * (excursion-set, lagrangian-overlap); other seeds and rules return
+ * SIF_ERR_UNSUPPORTED in this build. */
+SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid,
+  const sif_chain_mesh_t* mesh, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_real overlap_fraction, sif_option opt,
+  const sif_finder_exodus_extras_t* extras);
+
+#define SIF_FINDER_MESH_PARTICLES_PER_CELL 30.0
+#define SIF_FINDER_MESH_MAX_CELLS          1024u
+
+/* exodus_finder.h:654-685: mesh resolution at about 30 tracers per cell,
+ * floored so the widest search sphere fits, clamped to [8, 1024]. */
+static inline uint32_t sif_finder_suggest_mesh_cells(uint64_t n_particles,
+  sif_real box_length, sif_real max_radius, sif_real volume) {
+  if (n_particles == 0 || !(box_length > 0.0f) || !isfinite(box_length) ||
+      !(volume >= 0.0f) || !isfinite(volume))
+    return 0;
+  const double side = (double)box_length;
+  const double as_box = volume > 0.0f
+                          ? (double)n_particles * side * side * side / volume
+                          : (double)n_particles;
+  double cells = cbrt(as_box / SIF_FINDER_MESH_PARTICLES_PER_CELL);
+  if (max_radius > 0.0f) {
+    const double reach = 2.0 * (double)max_radius;
+    if (reach >= side)
+      return 0;
+    const double least = side / (side - reach) + 1.0;
+    if (cells < least)
+      cells = least;
+  }
+  if (cells < 8.0)
+    cells = 8.0;
+  if (cells > (double)SIF_FINDER_MESH_MAX_CELLS)
+    cells = (double)SIF_FINDER_MESH_MAX_CELLS;
+  return (uint32_t)cells;
+}
+
+typedef enum {
+  SIF_FINDER_UNITS_PHYSICAL = 0,
+  SIF_FINDER_UNITS_MPS
+} sif_finder_units_t;
+
+/* exodus_finder.h:719 */
+SIF_API SIF_NODISCARD uint32_t sif_finder_grid_cells_from_size(
+  uint64_t n_particles, sif_real box_length, sif_real cell_size,
+  sif_finder_units_t units, sif_real volume);
+
+SIF_EXTERN_C_END
+
+#endif

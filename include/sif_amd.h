@@ -1,0 +1,137 @@
+/*
+ * sif-amd device-level C ABI: the stages of the exodus path on data that is
+ * already resident in HBM. Plain pointers and sizes only; every pointer named
+ * d_* (or documented as device) is device memory on the current HIP device.
+ * Each call runs on the library stream and returns once its results are
+ * complete. Errors: a negative SIF_ERR_* status, with sif_error_message().
+ *
+ * These are the entry points the host-struct API of include/sif/ (the drop-in
+ * for sif v0.5.0) is built on, exported so that callers holding device
+ * buffers -- the bench, a torch autograd function, tests -- skip the host
+ * round trip. Each cites the reference function whose work it does.
+ */
+#ifndef SIF_AMD_H
+#define SIF_AMD_H
+
+#include <stdint.h>
+
+#include "sif/core/macros.h"
+#include "sif/structures/catalogue.h"
+
+SIF_EXTERN_C_BEGIN
+
+/* --- plumbing (no reference counterpart: device memory for callers without
+ * a framework of their own) --- */
+SIF_API int sif_amd_device_count(void);
+SIF_API int sif_amd_malloc(void** d_ptr, uint64_t bytes);
+SIF_API int sif_amd_free(void* d_ptr);
+SIF_API int sif_amd_memcpy_h2d(void* d_dst, const void* src, uint64_t bytes);
+SIF_API int sif_amd_memcpy_d2h(void* dst, const void* d_src, uint64_t bytes);
+SIF_API int sif_amd_memset(void* d_dst, int value, uint64_t bytes);
+SIF_API int sif_amd_synchronize(void);
+
+/* Per-stage device time (HIP events on the library stream). Stage order:
+ * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit. */
+#define SIF_AMD_TIMING_STAGES 9
+SIF_API int sif_amd_timing_enable(int on);
+SIF_API void sif_amd_timing_reset(void);
+SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);
+
+/* --- field --- */
+/* sif__field_check_in_box (src/structures/field.c:889-935). */
+SIF_API int sif_amd_check_in_box(const float* d_x, const float* d_y,
+  const float* d_z, uint64_t n, float box, uint64_t* n_bad, uint64_t* n_edge,
+  uint64_t* first_bad);
+
+/* --- grid --- */
+/* Bytes of device scratch sif_amd_grid_assign_cic() needs. */
+SIF_API uint64_t sif_amd_cic_work_bytes(uint32_t n_cells);
+/* grid_compute_cic for CIC (src/structures/grid.c:679-746): density per unit
+ * volume into d_values (n_cells^3 floats). d_w may be NULL. */
+SIF_API int sif_amd_grid_assign_cic(const float* d_x, const float* d_y,
+  const float* d_z, const float* d_w, uint64_t n, uint32_t n_cells, float box,
+  float* d_values, void* d_work);
+/* The deterministic mean of a grid (65536-cell blocks, double). */
+SIF_API int sif_amd_grid_mean(const float* d_values, uint64_t total_cells,
+  double* mean);
+/* sif_grid_to_density_contrast (grid.c:1177-1244), in place. */
+SIF_API int sif_amd_grid_to_density_contrast(float* d_values,
+  uint64_t total_cells, double* mean_out);
+
+/* --- chain mesh --- */
+/* chain_mesh_create + canonical order (src/structures/chain_mesh.c:680-832,
+ * :364-542): d_offsets has n_cells^3 + 1 entries; d_w, d_mw,
+ * d_original_indices and d_cell_weights may be NULL. */
+SIF_API int sif_amd_chain_mesh_build(const float* d_x, const float* d_y,
+  const float* d_z, const float* d_w, uint64_t n, uint32_t n_cells, float box,
+  uint64_t* d_offsets, float* d_mx, float* d_my, float* d_mz, float* d_mw,
+  uint64_t* d_original_indices, float* d_cell_weights, double* total_weight);
+
+/* --- FFT smoothing --- */
+#define SIF_AMD_FILTER_NONE     0
+#define SIF_AMD_FILTER_TOP_HAT  1
+#define SIF_AMD_FILTER_GAUSSIAN 2
+/* Bytes of an [n][n][n/2+1] complex64 spectrum (also the work size). */
+SIF_API uint64_t sif_amd_spectrum_bytes(uint32_t n);
+/* sif__fft_grid_forward (src/math/fft.c:790-818). */
+SIF_API int sif_amd_fft_r2c(const float* d_values, uint32_t n, float* d_spectrum,
+  void* d_work);
+/* sif__fft_apply_filter_smoothed + sif__fft_grid_backward (fft.c:703-888):
+ * window_order 0 applies no window correction, else 1/W (mode 0) or W/S
+ * (mode 1) of order window_order. */
+SIF_API int sif_amd_fft_filter_c2r(const float* d_spectrum, uint32_t n,
+  float box, int filter, float r, float sigma, int window_order,
+  int window_mode, float* d_out, void* d_work);
+
+/* --- candidates --- */
+/* sif__finder_scan_candidates (src/finder/utils.c:130-213): cells with
+ * delta <= threshold and a clear mask bit (d_mask_words may be NULL), sorted
+ * (delta, flat). Writes min(count, capacity) entries; *count is the total. */
+SIF_API int sif_amd_scan_candidates(const float* d_delta,
+  const uint64_t* d_mask_words, uint64_t total_cells, float threshold,
+  float* d_out_delta, uint32_t* d_out_flat, uint64_t capacity, uint64_t* count);
+
+/* --- sphere growth --- */
+/* sif__exodus_find_radius (src/finder/exodus_rescale.c:642-1027), unweighted,
+ * one result per centre: radius (or -1), reason (0 ok, 1 empty, 2 beyond the
+ * search sphere, 3 below the rung, 4 degenerate, 5 refinement overflow) and
+ * the enclosed count at the crossing. d_n_in may be NULL. */
+SIF_API int sif_amd_find_radius(const uint64_t* d_offsets, const float* d_mx,
+  const float* d_my, const float* d_mz, uint32_t mesh_cells, float box,
+  const float* d_cx, const float* d_cy, const float* d_cz, uint32_t n,
+  float rmin, float r_search, float threshold, float vol_factor,
+  float* d_radius, uint32_t* d_reason, uint64_t* d_n_in);
+
+/* --- the whole finder on device-resident data --- */
+
+/* Per-rung counters of a run, as sif__finder_log_radius prints them
+ * (src/finder/utils.c:776-808). */
+typedef struct {
+  float radius;
+  uint64_t candidates, masked, proxy, overlap, rescale_fail, accepted;
+  uint64_t evaluated; /* growths the GPU ran for this rung */
+} sif_amd_rung_stats_t;
+
+/* sif__exodus_run (src/finder/exodus_finder.c:1585-2492) for a periodic box,
+ * SIF_FINDER_SEEDS_CELLS on their nodes, unweighted: the grid d_delta (a
+ * density contrast, n_cells^3, left as the last rung's smoothing unless
+ * restore) and the mesh arrays are device memory. n_in (optional, host,
+ * capacity of the result) receives each void's enclosed count. stats
+ * (optional, host, n_radii entries) receives the per-rung counters in ladder
+ * order (largest first). */
+SIF_API sif_catalogue_t* sif_amd_exodus_device(float* d_delta,
+  uint32_t n_cells, float box, const uint64_t* d_offsets, const float* d_mx,
+  const float* d_my, const float* d_mz, uint32_t mesh_cells,
+  uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_real overlap_fraction, sif_option options,
+  int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats);
+
+/* Per-rung smoothed fields as the finder scans them, for parity checks:
+ * rung i (descending ladder) into d_fields + i * n_cells^3. */
+SIF_API int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells,
+  float box, const sif_real* radii, uint32_t n_radii, sif_option options,
+  float* d_fields);
+
+SIF_EXTERN_C_END
+
+#endif

@@ -1,0 +1,359 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes binding of the oracle (liboracle.so), the
+CPU restatement of the sif v0.5.0 exodus path, and of the reference's own
+quicksort harness (_ref/libref_sort.so) when it has been built.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as
+the checker or the timed CPU baseline. The product (sif_amd/) never imports
+it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_SORT_PATH = os.path.join(HERE, "_ref", "libref_sort.so")
+
+
+def build():
+    """Compiles the oracle (and the reference sort harness when the
+    reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+if not os.path.exists(LIB_PATH):
+    build()
+
+lib = C.CDLL(LIB_PATH)
+P = C.c_void_p
+u32, u64, f32, f64, i32 = C.c_uint32, C.c_uint64, C.c_float, C.c_double, C.c_int
+
+
+class Candidate(C.Structure):
+    _fields_ = [("delta", f32), ("refined", u32), ("flat_idx", u64)]
+
+
+class Template(C.Structure):
+    _fields_ = [("dx", P), ("dy", P), ("dz", P), ("type", P), ("min_d2", P),
+                ("max_d2", P), ("count", u32)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("n_cells", u32), ("box_length", f32), ("cell_length", f32),
+                ("offsets", P), ("x", P), ("y", P), ("z", P)]
+
+
+class Catalogue(C.Structure):
+    _fields_ = [("cx", C.POINTER(f32)), ("cy", C.POINTER(f32)), ("cz", C.POINTER(f32)),
+                ("r", C.POINTER(f32)), ("n_in", C.POINTER(u64)), ("rung", C.POINTER(u32)),
+                ("n", u64), ("capacity", u64)]
+
+
+class RungStats(C.Structure):
+    _fields_ = [("candidates", u64), ("masked", u64), ("proxy", u64), ("overlap", u64),
+                ("witness", u64), ("rescale_fail", u64), ("accepted", u64)]
+
+
+def _sig(name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_sig("orc_box_fold", f32, f32, f32)
+_sig("orc_check_in_box", u64, P, P, P, u64, f32, C.POINTER(u64), C.POINTER(u64))
+_sig("orc_cic_float", None, P, P, P, P, u64, u32, f32, u64, P)
+_sig("orc_cic_fixed", None, P, P, P, P, u64, u32, f32, P)
+_sig("orc_cic_fixed_density", None, P, u32, f32, P, C.POINTER(f64))
+_sig("orc_cic_brute_double", None, P, P, P, P, u64, u32, f32, P)
+_sig("orc_density_contrast", i32, P, u64, C.POINTER(f64))
+_sig("orc_density_contrast_with_mean", None, P, u64, f64)
+_sig("orc_chain_mesh", None, P, P, P, P, u64, u32, f32, P, P, P, P, P, P, P,
+     C.POINTER(f64))
+_sig("orc_filter_lut_size", u32, u32)
+_sig("orc_filter_lut", None, i32, f32, f32, f32, u32, P)
+_sig("orc_window_axis", None, u32, i32, i32, P)
+_sig("orc_window_corner_gain", f64, i32, i32, i32)
+_sig("orc_mode_factor", f32, P, P, u32, u32, u32, u32)
+_sig("orc_fft_r2c", i32, P, u32, P)
+_sig("orc_fft_c2r", i32, P, u32, P)
+_sig("orc_scan_candidates", u64, P, P, u64, f32, P)
+_sig("orc_sort_candidates_ref", None, P, u64)
+_sig("orc_sort_candidates_flat", None, P, u64)
+_sig("orc_template_build", i32, C.POINTER(Template), u32, f32, f32, f32)
+_sig("orc_template_free", None, C.POINTER(Template))
+_sig("orc_find_radius", f32, C.POINTER(Mesh), f32, f32, f32, f32, f32, f32, f32,
+     C.POINTER(Template), u32, C.POINTER(C.c_uint8), C.POINTER(u64))
+_sig("orc_witness", i32, C.POINTER(Mesh), f32, f32, f32, f32, f32, f32, f32,
+     C.POINTER(Template), i32)
+_sig("orc_brute_force_radius", f64, P, P, P, u64, f32, f64, f64, f64, f64, f64, f64)
+_sig("orc_overlap_quick", i32, P, u32, u32, u32, u32, u32)
+_sig("orc_mark_sphere", None, P, f32, f32, f32, f32, u32, f32)
+_sig("orc_exodus_run", i32, C.POINTER(P), u32, f32, C.POINTER(Mesh), u64, P, u32, f32,
+     f32, i32, u32, i32, i32, u32, C.POINTER(Catalogue), C.POINTER(RungStats))
+_sig("orc_catalogue_free", None, C.POINTER(Catalogue))
+_sig("orc_exodus_pipeline", i32, P, P, P, u64, u32, f32, u32, P, u32, f32, u32,
+     C.POINTER(Catalogue), P)
+_sig("orc_num_threads", i32)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def f32a(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def check_in_box(x, y, z, box):
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    edge, first = C.c_uint64(), C.c_uint64()
+    bad = lib.orc_check_in_box(_p(x), _p(y), _p(z), len(x), box, C.byref(edge), C.byref(first))
+    return bad, edge.value, first.value
+
+
+def cic_float(x, y, z, n_cells, box, w=None, tile=0):
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    w = None if w is None else f32a(w)
+    out = np.empty(n_cells ** 3, np.float32)
+    lib.orc_cic_float(_p(x), _p(y), _p(z), _p(w), len(x), n_cells, box, tile, _p(out))
+    return out
+
+
+def cic_fixed(x, y, z, n_cells, box, w=None):
+    """(density float32, fixed-point int64 accumulators, fixed-point mean)."""
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    w = None if w is None else f32a(w)
+    acc = np.empty(n_cells ** 3, np.int64)
+    lib.orc_cic_fixed(_p(x), _p(y), _p(z), _p(w), len(x), n_cells, box, _p(acc))
+    out = np.empty(n_cells ** 3, np.float32)
+    mean = C.c_double()
+    lib.orc_cic_fixed_density(_p(acc), n_cells, box, _p(out), C.byref(mean))
+    return out, acc, mean.value
+
+
+def cic_brute(x, y, z, n_cells, box, w=None):
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    w = None if w is None else f32a(w)
+    out = np.empty(n_cells ** 3, np.float64)
+    lib.orc_cic_brute_double(_p(x), _p(y), _p(z), _p(w), len(x), n_cells, box, _p(out))
+    return out
+
+
+def density_contrast(values):
+    v = np.array(values, dtype=np.float32, copy=True)
+    mean = C.c_double()
+    st = lib.orc_density_contrast(_p(v), v.size, C.byref(mean))
+    if st != 0:
+        raise ValueError(f"mean density {mean.value} not positive")
+    return v, mean.value
+
+
+def density_contrast_with_mean(values, mean):
+    v = np.array(values, dtype=np.float32, copy=True)
+    lib.orc_density_contrast_with_mean(_p(v), v.size, mean)
+    return v
+
+
+def chain_mesh(x, y, z, n_cells, box, w=None):
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    w = None if w is None else f32a(w)
+    n = len(x)
+    cells = n_cells ** 3
+    offsets = np.empty(cells + 1, np.uint64)
+    mx, my, mz = (np.empty(n, np.float32) for _ in range(3))
+    mw = np.empty(n, np.float32) if w is not None else None
+    cw = np.empty(cells, np.float32) if w is not None else None
+    orig = np.empty(n, np.uint64)
+    tw = C.c_double()
+    lib.orc_chain_mesh(_p(x), _p(y), _p(z), _p(w), n, n_cells, box, _p(offsets), _p(mx),
+                       _p(my), _p(mz), _p(mw), _p(orig), _p(cw), C.byref(tw))
+    return dict(offsets=offsets, x=mx, y=my, z=mz, w=mw, orig=orig, cell_weights=cw,
+                total_weight=tw.value)
+
+
+def filter_lut(filt, r, sigma, box, n):
+    lut = np.empty(lib.orc_filter_lut_size(n), np.float32)
+    lib.orc_filter_lut(filt, r, sigma, box, n, _p(lut))
+    return lut
+
+
+def window_axis(n, order=2, mode=0):
+    a = np.empty(n, np.float64)
+    lib.orc_window_axis(n, order, mode, _p(a))
+    return a
+
+
+def corner_gain(order, mode, interlaced=0):
+    return lib.orc_window_corner_gain(order, mode, interlaced)
+
+
+def mode_factors(lut, axis, n):
+    """The per-mode smoothing factor over the half spectrum, as the reference
+    applies it (fft.c:753-780), vectorised here in the same float/double
+    steps: (float)((double)lut[k2] * (axis[ix] * axis[iy]) * axis[iz])."""
+    h = n // 2
+    idx = np.arange(n)
+    k = np.where(idx > h, idx - n, idx)
+    kz = np.arange(h + 1)
+    k2 = (k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    s = lut[k2].astype(np.float32)
+    if axis is None:
+        return s
+    cxy = axis[:, None] * axis[None, :]
+    return (s.astype(np.float64) * cxy[:, :, None] * axis[None, None, :h + 1]).astype(np.float32)
+
+
+def fft_r2c(values, n):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    spec = np.empty((n, n, n // 2 + 1, 2), np.float64)
+    if lib.orc_fft_r2c(_p(v), n, _p(spec)) != 0:
+        raise ValueError("n must be a power of two")
+    return spec[..., 0] + 1j * spec[..., 1]
+
+
+def fft_c2r(spec, n):
+    s = np.empty((n, n, n // 2 + 1, 2), np.float64)
+    s[..., 0] = spec.real
+    s[..., 1] = spec.imag
+    out = np.empty(n ** 3, np.float64)
+    if lib.orc_fft_c2r(_p(s), n, _p(out)) != 0:
+        raise ValueError("n must be a power of two")
+    return out
+
+
+def scan_candidates(delta, threshold, mask=None, tie_rule=0):
+    d = f32a(delta)
+    buf = (Candidate * max(d.size, 1))()
+    k = lib.orc_scan_candidates(_p(d), _p(mask), d.size, threshold, buf)
+    if tie_rule == 0:
+        lib.orc_sort_candidates_ref(buf, k)
+    else:
+        lib.orc_sort_candidates_flat(buf, k)
+    arr = np.ctypeslib.as_array(buf)[:k]
+    return arr["delta"].copy(), arr["flat_idx"].copy()
+
+
+def sort_candidates_ref(delta, flat):
+    buf = (Candidate * max(len(delta), 1))()
+    arr = np.ctypeslib.as_array(buf)
+    arr["delta"][:len(delta)] = delta
+    arr["flat_idx"][:len(delta)] = flat
+    lib.orc_sort_candidates_ref(buf, len(delta))
+    return arr["delta"][:len(delta)].copy(), arr["flat_idx"][:len(delta)].copy()
+
+
+def ref_sort_available():
+    return os.path.exists(REF_SORT_PATH)
+
+
+def sort_candidates_reference_build(delta, flat):
+    """The same sort through the reference's own sort.h (oracle/_ref)."""
+    ref = C.CDLL(REF_SORT_PATH)
+    ref.ref_sort_candidates.argtypes = [P, u64]
+    ref.ref_sort_candidates.restype = None
+    buf = (Candidate * max(len(delta), 1))()
+    arr = np.ctypeslib.as_array(buf)
+    arr["delta"][:len(delta)] = delta
+    arr["flat_idx"][:len(delta)] = flat
+    ref.ref_sort_candidates(buf, len(delta))
+    return arr["delta"][:len(delta)].copy(), arr["flat_idx"][:len(delta)].copy()
+
+
+class OracleMesh:
+    def __init__(self, m, n_cells, box):
+        self.keep = m
+        self.s = Mesh(n_cells, box, np.float32(box) / np.float32(n_cells),
+                      m["offsets"].ctypes.data, m["x"].ctypes.data, m["y"].ctypes.data,
+                      m["z"].ctypes.data)
+
+
+class OracleTemplate:
+    def __init__(self, mesh_cells, cell_length, rmin, r_search):
+        self.t = Template()
+        st = lib.orc_template_build(C.byref(self.t), mesh_cells, cell_length, rmin, r_search)
+        if st != 0:
+            raise ValueError("search sphere wider than the mesh")
+
+    def arrays(self):
+        n = self.t.count
+        g = lambda p, ct: np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), (n,)).copy()
+        return dict(dx=g(self.t.dx, C.c_int32), dy=g(self.t.dy, C.c_int32),
+                    dz=g(self.t.dz, C.c_int32), type=g(self.t.type, C.c_uint8),
+                    min_d2=g(self.t.min_d2, C.c_float), max_d2=g(self.t.max_d2, C.c_float))
+
+    def __del__(self):
+        lib.orc_template_free(C.byref(self.t))
+
+
+def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, window=8):
+    tpl = OracleTemplate(mesh.s.n_cells, mesh.s.cell_length, rmin, r_search)
+    out = np.empty(len(centres), np.float32)
+    reason = np.empty(len(centres), np.uint8)
+    nin = np.zeros(len(centres), np.uint64)
+    for i, (x, y, z) in enumerate(np.asarray(centres, np.float32)):
+        rs = C.c_uint8()
+        ni = C.c_uint64()
+        out[i] = lib.orc_find_radius(C.byref(mesh.s), x, y, z, r_search, threshold,
+                                     vol_factor, rmin, C.byref(tpl.t), window, C.byref(rs),
+                                     C.byref(ni))
+        reason[i] = rs.value
+        nin[i] = ni.value
+    return out, reason, nin
+
+
+def brute_force_radius(x, y, z, box, centre, rung, r_search, threshold):
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    return lib.orc_brute_force_radius(_p(x), _p(y), _p(z), len(x), box, float(centre[0]),
+                                      float(centre[1]), float(centre[2]), rung, r_search,
+                                      threshold)
+
+
+def _catalogue(cat):
+    n = cat.n
+    if n == 0:
+        return np.zeros((0, 4), np.float32), np.zeros(0, np.uint64), np.zeros(0, np.uint32)
+    g = lambda p: np.ctypeslib.as_array(p, (n,)).copy()
+    out = np.stack([g(cat.cx), g(cat.cy), g(cat.cz), g(cat.r)], axis=1)
+    return out, g(cat.n_in), g(cat.rung)
+
+
+def exodus_run(fields, n_cells, box, mesh, n_particles, radii_desc, threshold,
+               overlap=0.0, lagrangian=False, search_opt=0, tie_rule=0, witness=True,
+               window=8):
+    """Sequential exodus commit on given per-rung smoothed fields."""
+    fields = [np.ascontiguousarray(f, dtype=np.float32) for f in fields]
+    ptrs = (P * len(fields))(*[f.ctypes.data for f in fields])
+    radii = np.ascontiguousarray(radii_desc, dtype=np.float32)
+    cat = Catalogue()
+    stats = (RungStats * len(radii))()
+    st = lib.orc_exodus_run(ptrs, n_cells, box, C.byref(mesh.s), n_particles,
+                            radii.ctypes.data, len(radii), threshold, overlap,
+                            1 if lagrangian else 0, search_opt, tie_rule,
+                            1 if witness else 0, window, C.byref(cat), stats)
+    if st != 0:
+        lib.orc_catalogue_free(C.byref(cat))
+        raise ValueError(f"oracle exodus failed ({st})")
+    out = _catalogue(cat)
+    lib.orc_catalogue_free(C.byref(cat))
+    return out + ([dict((k, getattr(s, k)) for k, _ in RungStats._fields_) for s in stats],)
+
+
+def exodus_pipeline(x, y, z, n_cells, box, mesh_cells, radii, threshold, max_rungs=0):
+    x, y, z = f32a(x), f32a(y), f32a(z)
+    radii = np.ascontiguousarray(radii, dtype=np.float32)
+    cat = Catalogue()
+    stage = np.zeros(6, np.float64)
+    st = lib.orc_exodus_pipeline(_p(x), _p(y), _p(z), len(x), n_cells, box, mesh_cells,
+                                 _p(radii), len(radii), threshold, max_rungs, C.byref(cat),
+                                 _p(stage))
+    if st != 0:
+        lib.orc_catalogue_free(C.byref(cat))
+        raise ValueError(f"oracle pipeline failed ({st})")
+    out = _catalogue(cat)
+    lib.orc_catalogue_free(C.byref(cat))
+    return out[0], out[1], stage
+

@@ -1,0 +1,308 @@
+// CIC deposit with deterministic fixed-point accumulation, and the density
+// contrast.
+//
+// Replaces the deposit of src/structures/grid.c:455-746 (tile/slab counting
+// sort and per-slab float accumulation) and sif_grid_to_density_contrast
+// (grid.c:1177-1244) of sif v0.5.0.
+//
+// The eight corner weights of a particle are computed with exactly the float
+// operations of grid.c:585-650. Each is rounded to an integer number of 2^-32
+// units and added with a 64-bit integer atomic, so the sum in every cell is
+// independent of launch shape and arrival order: the grid is bit-identical
+// from run to run and from one launch configuration to the next. (The
+// reference gets the same property from a fixed chunk/slab order; the bits of
+// the two differ by float rounding, within 1e-5 of the peak.)
+#include "common.h"
+#include "internal.h"
+#include "sif_amd.h"
+
+namespace sif_amd {
+
+__device__ __forceinline__ float box_fold(float v, float box) {
+  // src/structures/field_internal.h:58-64
+  if (v >= box)
+    v -= box;
+  else if (v < 0.0f)
+    v += box;
+  return v >= box ? 0.0f : v;
+}
+
+__device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+__device__ __forceinline__ int64_t to_fixed(float v) {
+  return (int64_t)__double2ll_rn((double)v * 4294967296.0);
+}
+
+// One thread per particle. Positions are read once (12 B, +4 with weights);
+// the eight fixed-point values go straight to the accumulators.
+__global__ void __launch_bounds__(256) cic_deposit_kernel(const float* __restrict__ xs,
+  const float* __restrict__ ys, const float* __restrict__ zs,
+  const float* __restrict__ ws, uint64_t n, uint32_t N, float idx, float box,
+  int64_t* __restrict__ acc) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += stride) {
+    const float cx = box_fold(xs[p], box) * idx;
+    const float cy = box_fold(ys[p], box) * idx;
+    const float cz = box_fold(zs[p], box) * idx;
+    const uint32_t rx = (uint32_t)cx, ry = (uint32_t)cy, rz = (uint32_t)cz;
+    const uint32_t ix = rx >= N ? 0u : rx;
+    const uint32_t iy = ry >= N ? 0u : ry;
+    const uint32_t iz = rz >= N ? 0u : rz;
+    const uint32_t ix1 = ix + 1 == N ? 0u : ix + 1;
+    const uint32_t iy1 = iy + 1 == N ? 0u : iy + 1;
+    const uint32_t iz1 = iz + 1 == N ? 0u : iz + 1;
+    const float tx = cx - (float)rx, ty = cy - (float)ry, tz = cz - (float)rz;
+    const float wx0 = 1.0f - tx, wx1 = tx;
+    const float wy0 = 1.0f - ty, wy1 = ty;
+    const float wz0 = 1.0f - tz, wz1 = tz;
+    const float w00 = wx0 * wy0, w01 = wx0 * wy1;
+    const float w10 = wx1 * wy0, w11 = wx1 * wy1;
+    const float pw = ws ? ws[p] : 1.0f;
+    const float mz0 = pw * wz0, mz1 = pw * wz1;
+    const uint64_t NN = (uint64_t)N * N;
+    const uint64_t a0 = (uint64_t)ix * NN, a1 = (uint64_t)ix1 * NN;
+    const uint64_t b0 = (uint64_t)iy * N, b1 = (uint64_t)iy1 * N;
+    atomic_add_i64(acc + a0 + b0 + iz, to_fixed(w00 * mz0));
+    atomic_add_i64(acc + a0 + b0 + iz1, to_fixed(w00 * mz1));
+    atomic_add_i64(acc + a0 + b1 + iz, to_fixed(w01 * mz0));
+    atomic_add_i64(acc + a0 + b1 + iz1, to_fixed(w01 * mz1));
+    atomic_add_i64(acc + a1 + b0 + iz, to_fixed(w10 * mz0));
+    atomic_add_i64(acc + a1 + b0 + iz1, to_fixed(w10 * mz1));
+    atomic_add_i64(acc + a1 + b1 + iz, to_fixed(w11 * mz0));
+    atomic_add_i64(acc + a1 + b1 + iz1, to_fixed(w11 * mz1));
+  }
+}
+
+// Fixed point -> density per unit volume (grid.c:735-738 divides by l^3).
+__global__ void cic_finish_kernel(const int64_t* __restrict__ acc, uint64_t total,
+  float inv_vol, float* __restrict__ out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += stride)
+    out[i] = (float)((double)acc[i] * 0x1p-32) * inv_vol;
+}
+
+// Deterministic mean: block b covers cells [b*MEAN_BLOCK, (b+1)*MEAN_BLOCK);
+// thread t sums cells b*MEAN_BLOCK + t + 256 j in j order, in double; the 256
+// partials fold in a fixed binary tree; the host adds block sums in order.
+constexpr uint32_t MEAN_BLOCK = 65536;
+
+__global__ void __launch_bounds__(256) mean_blocks_kernel(const float* __restrict__ v,
+  uint64_t total, double* __restrict__ block_sums) {
+  __shared__ double s[256];
+  const uint64_t base = (uint64_t)blockIdx.x * MEAN_BLOCK;
+  double acc = 0.0;
+  for (uint32_t j = 0; j < MEAN_BLOCK / 256; j++) {
+    const uint64_t i = base + threadIdx.x + 256ull * j;
+    if (i < total)
+      acc += (double)v[i];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride)
+      s[threadIdx.x] += s[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    block_sums[blockIdx.x] = s[0];
+}
+
+__global__ void contrast_kernel(float* __restrict__ v, uint64_t total, float inv) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += stride)
+    v[i] = v[i] * inv - 1.0f;
+}
+
+// Field validation of src/structures/field.c:889-935: counts outside the box
+// beyond 4 float ulps of its length, and those accepted only by that slack.
+__global__ void check_box_kernel(const float* __restrict__ xs,
+  const float* __restrict__ ys, const float* __restrict__ zs, uint64_t n,
+  float box, double tol, unsigned long long* __restrict__ out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long bad = 0, edge = 0, first = ~0ull;
+  const double L = (double)box;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const float p[3] = {xs[i], ys[i], zs[i]};
+    bool inside = true, ok = true;
+    for (int a = 0; a < 3; a++) {
+      if (!(p[a] >= 0.0f && p[a] < box))
+        inside = false;
+      const double d = (double)p[a];
+      if (!(d >= -tol && d <= L + tol))
+        ok = false;
+    }
+    if (inside)
+      continue;
+    if (ok) {
+      edge++;
+    } else {
+      bad++;
+      if (i < first)
+        first = i;
+    }
+  }
+  if (bad)
+    atomicAdd(&out[0], bad);
+  if (edge)
+    atomicAdd(&out[1], edge);
+  if (first != ~0ull)
+    atomicMin(&out[2], first);
+}
+
+namespace {
+DeviceBuffer g_mean_buf;
+DeviceBuffer g_box_buf;
+}  // namespace
+
+int grid_mean(const float* values, uint64_t total, double* mean) {
+  const uint64_t blocks = (total + MEAN_BLOCK - 1) / MEAN_BLOCK;
+  SIF_HIP(hipGetLastError());
+  int st = g_mean_buf.reserve(blocks * sizeof(double));
+  if (st != SIF_OK)
+    return st;
+  mean_blocks_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(
+    values, total, g_mean_buf.as<double>());
+  SIF_HIP(hipGetLastError());
+  double* host = new double[blocks];
+  hipError_t e = hipMemcpyAsync(host, g_mean_buf.ptr, blocks * sizeof(double),
+    hipMemcpyDeviceToHost, stream());
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(stream());
+  double sum = 0.0;
+  for (uint64_t b = 0; b < blocks; b++)
+    sum += host[b];
+  delete[] host;
+  SIF_HIP(e);
+  *mean = sum / (double)total;
+  return SIF_OK;
+}
+
+int density_contrast(float* values, uint64_t total, double mean) {
+  const float inv = (float)(1.0 / mean);
+  contrast_kernel<<<grid_for(total, 256), 256, 0, stream()>>>(values, total, inv);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+int check_in_box(const float* x, const float* y, const float* z, uint64_t n,
+  float box, uint64_t* n_bad, uint64_t* n_edge, uint64_t* first_bad) {
+  int st = g_box_buf.reserve(3 * sizeof(unsigned long long));
+  if (st != SIF_OK)
+    return st;
+  unsigned long long init[3] = {0, 0, ~0ull};
+  SIF_HIP(hipMemcpyAsync(
+    g_box_buf.ptr, init, sizeof(init), hipMemcpyHostToDevice, stream()));
+  const double tol = (double)box * 4.0 * (double)FLT_EPSILON;
+  if (n)
+    check_box_kernel<<<grid_for(n, 256) > 4096 ? 4096 : grid_for(n, 256), 256,
+      0, stream()>>>(x, y, z, n, box, tol, g_box_buf.as<unsigned long long>());
+  SIF_HIP(hipGetLastError());
+  unsigned long long out[3];
+  SIF_HIP(hipMemcpyAsync(
+    out, g_box_buf.ptr, sizeof(out), hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  if (n_bad)
+    *n_bad = out[0];
+  if (n_edge)
+    *n_edge = out[1];
+  if (first_bad)
+    *first_bad = out[2];
+  return SIF_OK;
+}
+
+int cic_deposit(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t N, float box, float* values, int64_t* acc) {
+  const uint64_t total = (uint64_t)N * N * N;
+  const float idx = (float)N / box;
+  const float cell = box / (float)N;
+  const float inv_vol = 1.0f / (float)(cell * cell * cell);
+  timing_begin(SIF_AMD_STAGE_CIC);
+  SIF_HIP(hipMemsetAsync(acc, 0, total * sizeof(int64_t), stream()));
+  if (n) {
+    unsigned g = grid_for(n, 256);
+    if (g > 65536u * 4)
+      g = 65536u * 4;
+    cic_deposit_kernel<<<g, 256, 0, stream()>>>(x, y, z, w, n, N, idx, box, acc);
+    SIF_HIP(hipGetLastError());
+  }
+  cic_finish_kernel<<<grid_for(total, 256), 256, 0, stream()>>>(
+    acc, total, inv_vol, values);
+  SIF_HIP(hipGetLastError());
+  timing_end(SIF_AMD_STAGE_CIC);
+  return SIF_OK;
+}
+
+}  // namespace sif_amd
+
+using namespace sif_amd;
+
+extern "C" {
+
+uint64_t sif_amd_cic_work_bytes(uint32_t n_cells) {
+  return (uint64_t)n_cells * n_cells * n_cells * sizeof(int64_t);
+}
+
+int sif_amd_check_in_box(const float* x, const float* y, const float* z,
+  uint64_t n, float box, uint64_t* n_bad, uint64_t* n_edge,
+  uint64_t* first_bad) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  return check_in_box(x, y, z, n, box, n_bad, n_edge, first_bad);
+}
+
+int sif_amd_grid_assign_cic(const float* x, const float* y, const float* z,
+  const float* w, uint64_t n, uint32_t n_cells, float box, float* values,
+  void* work) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!x || !y || !z || !values || !work || n_cells < 4 || !(box > 0.0f)) {
+    error_set(SIF_ERR_INVALID, "sif_amd_grid_assign_cic: invalid arguments");
+    return SIF_ERR_INVALID;
+  }
+  st = cic_deposit(x, y, z, w, n, n_cells, box, values, (int64_t*)work);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+int sif_amd_grid_mean(const float* values, uint64_t total_cells, double* mean) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  return grid_mean(values, total_cells, mean);
+}
+
+int sif_amd_grid_to_density_contrast(float* values, uint64_t total_cells,
+  double* mean_out) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  double mean = 0.0;
+  timing_begin(SIF_AMD_STAGE_CONTRAST);
+  st = grid_mean(values, total_cells, &mean);
+  if (st != SIF_OK)
+    return st;
+  if (mean_out)
+    *mean_out = mean;
+  if (!(mean > 0.0)) {
+    error_set(SIF_ERR_RANGE, "grid: mean density %g (expected > 0)", mean);
+    return SIF_ERR_RANGE;
+  }
+  st = density_contrast(values, total_cells, mean);
+  timing_end(SIF_AMD_STAGE_CONTRAST);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// sif-amd internal: HIP plumbing shared by the kernels and the host driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstdio>
+
+#include "sif/core/macros.h"
+
+namespace sif_amd {
+
+// The error record (logger.c of the reference keeps it thread-local too).
+void error_set(int status, const char* fmt, ...)
+  __attribute__((format(printf, 2, 3)));
+
+// Library stream, created by sif_system_init(); blocking with respect to the
+// null stream so work queued by other users of the device orders with it.
+hipStream_t stream();
+bool initialised();
+int ensure_init();  // lazy device init for the sif_amd_* device entry points
+
+// Per-stage device timing (HIP events on the library stream).
+void timing_begin(int stage);
+void timing_end(int stage);
+bool timing_active();
+void timing_add(int stage, double ms);  // host-side stages (the commit)
+
+// Checks a HIP call; on failure records SIF_ERR_ALLOC/INVALID and returns it.
+int hip_check(hipError_t e, const char* what);
+
+#define SIF_HIP(call)                                                          \
+  do {                                                                         \
+    int _s = ::sif_amd::hip_check((call), #call);                              \
+    if (_s != SIF_OK)                                                          \
+      return _s;                                                               \
+  } while (0)
+
+// Device scratch that grows on demand and is reused across calls.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t want);
+  void release();
+  ~DeviceBuffer() { release(); }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+inline unsigned grid_for(uint64_t n, unsigned block) {
+  uint64_t g = (n + block - 1) / block;
+  return (unsigned)(g > 0x7fffffffull ? 0x7fffffffull : (g ? g : 1));
+}
+
+}  // namespace sif_amd
+
+// Stage ids for the per-stage timers.
+enum {
+  SIF_AMD_STAGE_CIC = 0,
+  SIF_AMD_STAGE_CONTRAST,
+  SIF_AMD_STAGE_MESH,
+  SIF_AMD_STAGE_FFT_FORWARD,
+  SIF_AMD_STAGE_SMOOTH,
+  SIF_AMD_STAGE_SCAN,
+  SIF_AMD_STAGE_PREFILTER,
+  SIF_AMD_STAGE_GROW,
+  SIF_AMD_STAGE_COMMIT,
+  SIF_AMD_N_STAGES
+};

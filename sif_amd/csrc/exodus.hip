@@ -1,0 +1,659 @@
+// The exodus driver on device-resident data.
+//
+// Replaces sif__exodus_run (src/finder/exodus_finder.c:1585-2492) for a
+// periodic box with SIF_FINDER_SEEDS_CELLS on their nodes, and the helpers it
+// calls on the commit path: accept_void (656-683), sif__overlap_quick and
+// sif__overlap_exact (src/finder/utils.c:819-960), sif__mark_sphere
+// (utils.c:965-1033) and the void cell list (cell_linked_list.c:181-210).
+//
+// Per rung, on the GPU: the filtered c2r (top-hat at the rung, CIC window
+// divided out), the candidate scan and sort, and a prefilter of every
+// candidate against the catalogue as the rung begins (exodus_finder.c:
+// 871-944). The survivors come to the host in order. They are then evaluated
+// in speculative batches -- a batch's sphere growths run on the GPU against the
+// catalogue as the batch begins -- and committed one by one on the host, where
+// each candidate is tested again against the catalogue as it now stands, at
+// the rung radius and at its grown radius, exactly the tests of
+// exodus_finder.c:2181-2329. The catalogue is therefore that of a sequential
+// pass, whatever the batch sizes.
+//
+// The crossing witness (exodus_finder.c:2021-2035) is not run: it only refuses
+// candidates that the commit would refuse, so leaving it out costs GPU work
+// on those candidates and changes nothing in the catalogue.
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+#include "sif/structures/catalogue.h"
+#include "sif_amd.h"
+
+namespace sif_amd {
+
+namespace {
+
+constexpr int VOID_CELLS = 32;
+constexpr uint32_t BATCH_FLOOR = 512;
+constexpr uint32_t BATCH_MAX = 16384;
+
+inline uint32_t wrap(int32_t v, uint32_t n) {
+  const int32_t w = v % (int32_t)n;
+  return (uint32_t)(w < 0 ? w + (int32_t)n : w);
+}
+
+// Host mask of accepted-void cells, n^3 bits.
+struct Mask {
+  std::vector<uint64_t> words;
+  uint32_t n = 0;
+  bool dirty = true;
+  void init(uint32_t n_) {
+    n = n_;
+    words.assign(((uint64_t)n * n * n + 63) / 64, 0ull);
+    dirty = true;
+  }
+  bool get(uint64_t i) const { return (words[i >> 6] >> (i & 63)) & 1ull; }
+  void set(uint64_t i) { words[i >> 6] |= 1ull << (i & 63); }
+};
+
+// utils.c:819-848
+bool overlap_quick(const Mask& m, uint32_t ix, uint32_t iy, uint32_t iz, uint32_t r) {
+  const uint64_t n = m.n;
+  if (m.get(((uint64_t)wrap((int32_t)(ix + r), m.n) * n + iy) * n + iz))
+    return true;
+  if (m.get(((uint64_t)wrap((int32_t)ix - (int32_t)r, m.n) * n + iy) * n + iz))
+    return true;
+  if (m.get(((uint64_t)ix * n + wrap((int32_t)(iy + r), m.n)) * n + iz))
+    return true;
+  if (m.get(((uint64_t)ix * n + wrap((int32_t)iy - (int32_t)r, m.n)) * n + iz))
+    return true;
+  if (m.get(((uint64_t)ix * n + iy) * n + wrap((int32_t)(iz + r), m.n)))
+    return true;
+  if (m.get(((uint64_t)ix * n + iy) * n + wrap((int32_t)iz - (int32_t)r, m.n)))
+    return true;
+  return false;
+}
+
+// utils.c:965-1033, setting form: every cell whose node lies within r.
+void mark_sphere(Mask& m, float cx, float cy, float cz, float r, float l) {
+  const float r2 = r * r;
+  const float inv_l = 1.0f / l;
+  const float L = (float)m.n * l;
+  const float half = L * 0.5f;
+  const int32_t c = (int32_t)(r * inv_l) + 1;
+  const int32_t ox = (int32_t)(cx * inv_l), oy = (int32_t)(cy * inv_l),
+                oz = (int32_t)(cz * inv_l);
+  for (int32_t dx = -c; dx <= c; dx++) {
+    const int32_t gx = ox + dx;
+    float ddx = fabsf((float)gx * l - cx);
+    ddx = ddx > half ? L - ddx : ddx;
+    const float dx2 = ddx * ddx;
+    if (dx2 > r2)
+      continue;
+    const uint32_t wx = wrap(gx, m.n);
+    for (int32_t dy = -c; dy <= c; dy++) {
+      const int32_t gy = oy + dy;
+      float ddy = fabsf((float)gy * l - cy);
+      ddy = ddy > half ? L - ddy : ddy;
+      const float dxy2 = dx2 + ddy * ddy;
+      if (dxy2 > r2)
+        continue;
+      const uint32_t wy = wrap(gy, m.n);
+      for (int32_t dz = -c; dz <= c; dz++) {
+        const int32_t gz = oz + dz;
+        float ddz = fabsf((float)gz * l - cz);
+        ddz = ddz > half ? L - ddz : ddz;
+        if (dxy2 + ddz * ddz <= r2)
+          m.set(((uint64_t)wx * m.n + wy) * (uint64_t)m.n + wrap(gz, m.n));
+      }
+    }
+  }
+  m.dirty = true;
+}
+
+// The accepted voids with their 32^3 cell list (cell_linked_list.c:41-59,
+// :181-210; head push, periodic cell index).
+struct Voids {
+  std::vector<float> cx, cy, cz, r;
+  std::vector<uint64_t> n_in;
+  std::vector<int32_t> head, next;
+  float inv_l = 0.0f, box = 0.0f, max_r = 0.0f;
+  bool dirty = true;
+  void init(float box_) {
+    box = box_;
+    inv_l = (float)VOID_CELLS / box;
+    head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
+  }
+  uint32_t cell_of(float x, float y, float z) const {
+    auto ax = [&](float v) { return wrap((int32_t)floorf(v * inv_l), VOID_CELLS); };
+    return (ax(x) * VOID_CELLS + ax(y)) * VOID_CELLS + ax(z);
+  }
+  void add(float x, float y, float z, float rr, uint64_t nin) {
+    const int32_t k = (int32_t)cx.size();
+    cx.push_back(x);
+    cy.push_back(y);
+    cz.push_back(z);
+    r.push_back(rr);
+    n_in.push_back(nin);
+    const uint32_t c = cell_of(x, y, z);
+    next.push_back(head[c]);
+    head[c] = k;
+    if (rr > max_r)
+      max_r = rr;
+    dirty = true;
+  }
+  // utils.c:850-960
+  bool overlap_exact(float x, float y, float z, float rr, float f, float scale) const {
+    const float half = box * 0.5f;
+    const float sr = scale * (rr + max_r - f * (rr < max_r ? rr : max_r));
+    const int32_t lo[3] = {(int32_t)floorf((x - sr) * inv_l), (int32_t)floorf((y - sr) * inv_l),
+      (int32_t)floorf((z - sr) * inv_l)};
+    const int32_t hi[3] = {(int32_t)floorf((x + sr) * inv_l), (int32_t)floorf((y + sr) * inv_l),
+      (int32_t)floorf((z + sr) * inv_l)};
+    const bool interior = (x - sr >= 0.0f) && (x + sr < box) && (y - sr >= 0.0f) &&
+                          (y + sr < box) && (z - sr >= 0.0f) && (z + sr < box);
+    for (int32_t a = lo[0]; a <= hi[0]; a++) {
+      const uint32_t wa = wrap(a, VOID_CELLS);
+      for (int32_t b = lo[1]; b <= hi[1]; b++) {
+        const uint32_t wb = wrap(b, VOID_CELLS);
+        for (int32_t c = lo[2]; c <= hi[2]; c++) {
+          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wrap(c, VOID_CELLS);
+          for (int32_t i = head[cell]; i != -1; i = next[i]) {
+            const float ri = r[i];
+            const float mr = rr < ri ? rr : ri;
+            const float tot = scale * (rr + ri - f * mr);
+            const float t2 = tot * tot;
+            if (interior) {
+              const float dx = x - cx[i], dy = y - cy[i], dz = z - cz[i];
+              if (dx * dx + dy * dy + dz * dz < t2)
+                return true;
+            } else {
+              float dx = fabsf(x - cx[i]);
+              dx = dx > half ? box - dx : dx;
+              const float dx2 = dx * dx;
+              if (dx2 >= t2)
+                continue;
+              float dy = fabsf(y - cy[i]);
+              dy = dy > half ? box - dy : dy;
+              const float dy2 = dy * dy;
+              if (dx2 + dy2 >= t2)
+                continue;
+              float dz = fabsf(z - cz[i]);
+              dz = dz > half ? box - dz : dz;
+              if (dx2 + dy2 + dz * dz < t2)
+                return true;
+            }
+          }
+        }
+      }
+    }
+    return false;
+  }
+};
+
+struct DeviceState {
+  DeviceBuffer spec, work, mask, keys, flat, keep, surv;
+  DeviceBuffer bflat, bcx, bcy, bcz, brad, breason, bnin;
+  DeviceBuffer vcx, vcy, vcz, vr, vstart, vitems;
+};
+DeviceState g_dev;
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+    std::chrono::steady_clock::now().time_since_epoch())
+    .count();
+}
+
+int upload_voids(const Voids& V, VoidIndex* out) {
+  const uint32_t nv = (uint32_t)V.cx.size();
+  const uint32_t cells = VOID_CELLS * VOID_CELLS * VOID_CELLS;
+  std::vector<uint32_t> start(cells + 1, 0), items(nv ? nv : 1);
+  for (uint32_t i = 0; i < nv; i++)
+    start[V.cell_of(V.cx[i], V.cy[i], V.cz[i]) + 1]++;
+  for (uint32_t c = 0; c < cells; c++)
+    start[c + 1] += start[c];
+  std::vector<uint32_t> cur(start.begin(), start.end() - 1);
+  for (uint32_t i = 0; i < nv; i++)
+    items[cur[V.cell_of(V.cx[i], V.cy[i], V.cz[i])]++] = i;
+  const size_t fb = (size_t)(nv ? nv : 1) * sizeof(float);
+  int st = g_dev.vcx.reserve(fb);
+  if (st == SIF_OK) st = g_dev.vcy.reserve(fb);
+  if (st == SIF_OK) st = g_dev.vcz.reserve(fb);
+  if (st == SIF_OK) st = g_dev.vr.reserve(fb);
+  if (st == SIF_OK) st = g_dev.vstart.reserve((cells + 1) * sizeof(uint32_t));
+  if (st == SIF_OK) st = g_dev.vitems.reserve(items.size() * sizeof(uint32_t));
+  if (st != SIF_OK)
+    return st;
+  if (nv) {
+    SIF_HIP(hipMemcpyAsync(g_dev.vcx.ptr, V.cx.data(), nv * 4, hipMemcpyHostToDevice, stream()));
+    SIF_HIP(hipMemcpyAsync(g_dev.vcy.ptr, V.cy.data(), nv * 4, hipMemcpyHostToDevice, stream()));
+    SIF_HIP(hipMemcpyAsync(g_dev.vcz.ptr, V.cz.data(), nv * 4, hipMemcpyHostToDevice, stream()));
+    SIF_HIP(hipMemcpyAsync(g_dev.vr.ptr, V.r.data(), nv * 4, hipMemcpyHostToDevice, stream()));
+    SIF_HIP(hipMemcpyAsync(g_dev.vitems.ptr, items.data(), nv * 4, hipMemcpyHostToDevice, stream()));
+  }
+  SIF_HIP(hipMemcpyAsync(g_dev.vstart.ptr, start.data(), (cells + 1) * 4,
+    hipMemcpyHostToDevice, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  out->cx = g_dev.vcx.as<float>();
+  out->cy = g_dev.vcy.as<float>();
+  out->cz = g_dev.vcz.as<float>();
+  out->r = g_dev.vr.as<float>();
+  out->cell_start = g_dev.vstart.as<uint32_t>();
+  out->items = g_dev.vitems.as<uint32_t>();
+  out->inv_l = V.inv_l;
+  out->max_r = V.max_r;
+  return SIF_OK;
+}
+
+int cmp_desc(const void* a, const void* b) {
+  const float x = *(const float*)a, y = *(const float*)b;
+  return (x < y) - (x > y);
+}
+
+}  // namespace
+
+static int window_order_of(sif_option opt) {
+  return (opt & SIF_FINDER_KEEP_CIC_WINDOW) ? 0 : 2;
+}
+
+int exodus_prepare_spectrum(const float* d_delta, uint32_t n) {
+  const uint64_t sb = (uint64_t)n * n * (n / 2 + 1) * sizeof(float2);
+  int st = g_dev.spec.reserve(sb);
+  if (st == SIF_OK)
+    st = g_dev.work.reserve(sb);
+  if (st != SIF_OK)
+    return st;
+  return fft_r2c(d_delta, n, g_dev.spec.as<float2>(), g_dev.work.as<float2>());
+}
+
+int exodus_smooth(uint32_t n, float box, float radius, sif_option opt, float* d_out) {
+  return fft_filter_c2r(g_dev.spec.as<float2>(), n, box, SIF_AMD_FILTER_TOP_HAT, radius,
+    0.0f, window_order_of(opt), (opt & SIF_FINDER_ALIASING_WINDOW) ? 1 : 0, d_out,
+    g_dev.work.as<float2>());
+}
+
+int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
+  sif_amd_rung_stats_t* stats_out) {
+  const uint32_t n = R.n;
+  const uint64_t total = (uint64_t)n * n * n;
+  const float l = R.box / (float)n;
+  const float box_volume = R.box * R.box * R.box;
+  const float mean_density = (float)R.n_particles / box_volume;
+  const float vol = (4.0f / 3.0f) * SIF_PI * mean_density;
+  const float scale =
+    (R.opt & SIF_FINDER_LAGRANGIAN_OVERLAP) ? (float)cbrt(1.0 + (double)R.threshold) : 1.0f;
+  const float sf = sif__finder_search_factor(R.opt);
+
+  int st = exodus_prepare_spectrum(R.d_delta, n);
+  if (st != SIF_OK)
+    return st;
+
+  Mask mask;
+  mask.init(n);
+  Voids V;
+  V.init(R.box);
+  if (R.prior) {
+    for (uint64_t v = 0; v < R.prior->n_voids; v++) {
+      V.add(R.prior->cx[v], R.prior->cy[v], R.prior->cz[v], R.prior->radii[v], 0);
+      mark_sphere(mask, R.prior->cx[v], R.prior->cy[v], R.prior->cz[v],
+        scale * R.prior->radii[v], l);
+    }
+  }
+
+  st = g_dev.mask.reserve(mask.words.size() * sizeof(uint64_t));
+  if (st != SIF_OK)
+    return st;
+
+  std::vector<uint32_t> surv;
+  std::vector<uint32_t> bflat;
+  std::vector<float> brad;
+  std::vector<uint32_t> breason;
+  std::vector<uint64_t> bnin;
+  uint64_t cand_cap = 0;
+
+  for (size_t i = 0; i < R.radii.size() && st == SIF_OK; i++) {
+    const float radius = R.radii[i];
+    sif_amd_rung_stats_t S;
+    memset(&S, 0, sizeof(S));
+    S.radius = radius;
+
+    st = exodus_smooth(n, R.box, radius, R.opt, R.d_delta);
+    if (st != SIF_OK)
+      break;
+    if (mask.dirty) {
+      SIF_HIP(hipMemcpyAsync(g_dev.mask.ptr, mask.words.data(),
+        mask.words.size() * sizeof(uint64_t), hipMemcpyHostToDevice, stream()));
+      mask.dirty = false;
+    }
+
+    // scan and sort, growing the candidate buffers when they are short
+    uint64_t nc = 0;
+    for (;;) {
+      if (cand_cap == 0) {
+        cand_cap = total / 64 + 1024;
+        if ((st = g_dev.keys.reserve(cand_cap * 4)) != SIF_OK) break;
+        if ((st = g_dev.flat.reserve(cand_cap * 4)) != SIF_OK) break;
+        if ((st = g_dev.keep.reserve(cand_cap * 4)) != SIF_OK) break;
+        if ((st = g_dev.surv.reserve(cand_cap * 4)) != SIF_OK) break;
+      }
+      st = scan_candidates(R.d_delta, g_dev.mask.as<uint64_t>(), total, R.threshold,
+        g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap, &nc);
+      if (st != SIF_OK || nc <= cand_cap)
+        break;
+      cand_cap = (uint64_t)((double)nc * 1.2) + 1;
+      g_dev.keys.release();
+      g_dev.flat.release();
+      g_dev.keep.release();
+      g_dev.surv.release();
+      if ((st = g_dev.keys.reserve(cand_cap * 4)) != SIF_OK) break;
+      if ((st = g_dev.flat.reserve(cand_cap * 4)) != SIF_OK) break;
+      if ((st = g_dev.keep.reserve(cand_cap * 4)) != SIF_OK) break;
+      if ((st = g_dev.surv.reserve(cand_cap * 4)) != SIF_OK) break;
+    }
+    if (st != SIF_OK)
+      break;
+    S.candidates = nc;
+
+    const float rmin = 1.00f * radius;
+    float r_search = radius + fmaxf((sf - 1.0f) * radius, 3.0f * l);
+    if (i > 0 && R.radii[i - 1] > r_search)
+      r_search = R.radii[i - 1];
+    const int32_t proxy = (int32_t)(scale * radius * (1.0f - R.f) / l) - 1;
+
+    // prefilter against the catalogue as the rung begins
+    VoidIndex VI;
+    st = upload_voids(V, &VI);
+    if (st != SIF_OK)
+      break;
+    uint64_t pf[3] = {0, 0, 0};
+    timing_begin(SIF_AMD_STAGE_PREFILTER);
+    st = prefilter(g_dev.flat.as<uint32_t>(), (uint32_t)nc, n, l, R.box,
+      g_dev.mask.as<uint64_t>(), proxy, radius, R.f, scale, VI, (uint32_t)V.cx.size(),
+      g_dev.keep.as<uint32_t>(), pf);
+    if (st != SIF_OK)
+      break;
+    uint64_t ns = 0;
+    st = compact_u32(g_dev.flat.as<uint32_t>(), g_dev.keep.as<uint32_t>(), nc,
+      g_dev.surv.as<uint32_t>(), &ns);
+    if (st != SIF_OK)
+      break;
+    timing_end(SIF_AMD_STAGE_PREFILTER);
+    S.masked += pf[0];
+    S.proxy += pf[1];
+    S.overlap += pf[2];
+    surv.resize(ns);
+    if (ns)
+      SIF_HIP(hipMemcpyAsync(surv.data(), g_dev.surv.ptr, ns * 4, hipMemcpyDeviceToHost,
+        stream()));
+    SIF_HIP(hipStreamSynchronize(stream()));
+
+    // speculative batches, sequential commit
+    uint32_t cap = BATCH_FLOOR;
+    if ((st = g_dev.bflat.reserve(BATCH_MAX * 4)) != SIF_OK) break;
+    if ((st = g_dev.bcx.reserve(BATCH_MAX * 4)) != SIF_OK) break;
+    if ((st = g_dev.bcy.reserve(BATCH_MAX * 4)) != SIF_OK) break;
+    if ((st = g_dev.bcz.reserve(BATCH_MAX * 4)) != SIF_OK) break;
+    if ((st = g_dev.brad.reserve(BATCH_MAX * 4)) != SIF_OK) break;
+    if ((st = g_dev.breason.reserve(BATCH_MAX * 4)) != SIF_OK) break;
+    if ((st = g_dev.bnin.reserve(BATCH_MAX * 8)) != SIF_OK) break;
+    uint64_t k = 0;
+    const uint64_t nn = n;
+    auto node_of = [&](uint32_t flat, uint32_t* ix, uint32_t* iy, uint32_t* iz) {
+      *ix = (uint32_t)(flat / (nn * nn));
+      *iy = (uint32_t)((flat / nn) % nn);
+      *iz = (uint32_t)(flat % nn);
+    };
+    // the rung-radius tests against the catalogue as it stands: 0 pass,
+    // 1 masked, 2 proxy, 3 overlap
+    auto rung_tests = [&](uint32_t flat) -> int {
+      if (mask.get(flat))
+        return 1;
+      uint32_t ix, iy, iz;
+      node_of(flat, &ix, &iy, &iz);
+      if (proxy > 0 && overlap_quick(mask, ix, iy, iz, (uint32_t)proxy))
+        return 2;
+      if (V.overlap_exact((float)ix * l, (float)iy * l, (float)iz * l, radius, R.f, scale))
+        return 3;
+      return 0;
+    };
+    while (k < ns && st == SIF_OK) {
+      const double t0 = now_ms();
+      bflat.clear();
+      while (k < ns && bflat.size() < cap) {
+        const uint32_t flat = surv[k++];
+        const int why = rung_tests(flat);
+        if (why == 1)
+          S.masked++;
+        else if (why == 2)
+          S.proxy++;
+        else if (why == 3)
+          S.overlap++;
+        else
+          bflat.push_back(flat);
+      }
+      const uint32_t bc = (uint32_t)bflat.size();
+      if (bc == 0)
+        break;
+      const double t1 = now_ms();
+      if (timing_active())
+        timing_add(SIF_AMD_STAGE_COMMIT, t1 - t0);
+      SIF_HIP(hipMemcpyAsync(g_dev.bflat.ptr, bflat.data(), bc * 4, hipMemcpyHostToDevice,
+        stream()));
+      st = node_centres(g_dev.bflat.as<uint32_t>(), bc, n, l, g_dev.bcx.as<float>(),
+        g_dev.bcy.as<float>(), g_dev.bcz.as<float>());
+      if (st == SIF_OK)
+        st = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
+          g_dev.bcx.as<float>(), g_dev.bcy.as<float>(), g_dev.bcz.as<float>(), bc, rmin,
+          r_search, R.threshold, vol, g_dev.brad.as<float>(), g_dev.breason.as<uint32_t>(),
+          g_dev.bnin.as<uint64_t>());
+      if (st != SIF_OK)
+        break;
+      S.evaluated += bc;
+      brad.resize(bc);
+      breason.resize(bc);
+      bnin.resize(bc);
+      SIF_HIP(hipMemcpyAsync(brad.data(), g_dev.brad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
+      SIF_HIP(hipMemcpyAsync(breason.data(), g_dev.breason.ptr, bc * 4, hipMemcpyDeviceToHost,
+        stream()));
+      SIF_HIP(hipMemcpyAsync(bnin.data(), g_dev.bnin.ptr, bc * 8, hipMemcpyDeviceToHost, stream()));
+      SIF_HIP(hipStreamSynchronize(stream()));
+      const double t2 = now_ms();
+
+      uint64_t accepted = 0;
+      for (uint32_t b = 0; b < bc; b++) {
+        const uint32_t flat = bflat[b];
+        if (breason[b] == 5) {
+          error_set(SIF_ERR_RANGE,
+            "exodus: a search shell at rung %g holds more tracers in one histogram "
+            "bin than the refinement buffer takes",
+            (double)radius);
+          st = SIF_ERR_RANGE;
+          break;
+        }
+        const int why = rung_tests(flat);
+        if (why) {
+          if (why == 1)
+            S.masked++;
+          else if (why == 2)
+            S.proxy++;
+          else
+            S.overlap++;
+          continue;
+        }
+        const float r = brad[b];
+        if (r < 0.0f) {
+          S.rescale_fail++;
+          continue;
+        }
+        uint32_t ix, iy, iz;
+        node_of(flat, &ix, &iy, &iz);
+        const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+        const int32_t pole = (int32_t)(scale * r * (1.0f - R.f) / l) - 1;
+        if (pole > 0 && overlap_quick(mask, ix, iy, iz, (uint32_t)pole)) {
+          S.proxy++;
+          continue;
+        }
+        if (V.overlap_exact(cx, cy, cz, r, R.f, scale)) {
+          S.overlap++;
+          continue;
+        }
+        V.add(cx, cy, cz, r, bnin[b]);
+        mark_sphere(mask, cx, cy, cz, scale * r, l);
+        accepted++;
+      }
+      S.accepted += accepted;
+      if (timing_active())
+        timing_add(SIF_AMD_STAGE_COMMIT, now_ms() - t2);
+      // exodus_finder.c:2331-2349: widen after a quiet batch, back off after
+      // accepts
+      if (accepted == 0) {
+        cap += BATCH_FLOOR;
+        if (cap > BATCH_MAX)
+          cap = BATCH_MAX;
+      } else {
+        cap /= (uint32_t)(accepted + 1);
+        if (cap < BATCH_FLOOR)
+          cap = BATCH_FLOOR;
+      }
+    }
+    if (stats_out)
+      stats_out[i] = S;
+  }
+
+  if (R.restore && st == SIF_OK)
+    st = fft_filter_c2r(g_dev.spec.as<float2>(), n, R.box, SIF_AMD_FILTER_NONE, 0.0f, 0.0f,
+      0, 0, R.d_delta, g_dev.work.as<float2>());
+  if (st == SIF_OK)
+    SIF_HIP(hipStreamSynchronize(stream()));
+  if (st != SIF_OK)
+    return st;
+
+  const uint64_t nv = V.cx.size();
+  sif_catalogue_t* cat = sif_catalogue_alloc(nv ? nv : 1);
+  if (!cat) {
+    error_set(SIF_ERR_ALLOC, "exodus: cannot allocate a catalogue of %llu voids",
+      (unsigned long long)nv);
+    return SIF_ERR_ALLOC;
+  }
+  for (uint64_t v = 0; v < nv; v++)
+    sif_catalogue_append(cat, V.cx[v], V.cy[v], V.cz[v], V.r[v]);
+  if (n_in_out) {
+    *n_in_out = (uint64_t*)malloc(sizeof(uint64_t) * (nv ? nv : 1));
+    if (*n_in_out)
+      memcpy(*n_in_out, V.n_in.data(), sizeof(uint64_t) * nv);
+  }
+  *out_cat = cat;
+  return SIF_OK;
+}
+
+int sorted_ladder(const sif_real* radii, uint32_t n_radii, std::vector<float>& out) {
+  out.assign(radii, radii + n_radii);
+  qsort(out.data(), out.size(), sizeof(float), cmp_desc);
+  return SIF_OK;
+}
+
+// sif__finder_check_args (src/finder/utils.c:33-58)
+int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real f) {
+  if (!radii || n_radii == 0) {
+    error_set(SIF_ERR_INVALID, "radii: none given");
+    return SIF_ERR_INVALID;
+  }
+  for (uint32_t i = 0; i < n_radii; i++)
+    if (!(radii[i] > 0.0f) || !std::isfinite(radii[i])) {
+      error_set(SIF_ERR_INVALID, "radii[%u] = %g: not a positive radius", i, (double)radii[i]);
+      return SIF_ERR_INVALID;
+    }
+  if (!(threshold > -1.0f && threshold < 0.0f)) {
+    error_set(SIF_ERR_INVALID, "threshold %g: not a density contrast in (-1, 0)",
+      (double)threshold);
+    return SIF_ERR_INVALID;
+  }
+  if (!(f >= 0.0f && f <= 1.0f)) {
+    error_set(SIF_ERR_INVALID, "overlap_fraction %g: not in [0, 1]", (double)f);
+    return SIF_ERR_INVALID;
+  }
+  return SIF_OK;
+}
+
+// exodus_finder.c:1112-1138: what this build can run.
+int check_options(sif_option opt) {
+  const sif_option refine = opt & SIF__FINDER_REFINE_MASK;
+  const char* missing = (opt & SIF_FINDER_SEEDS_MINIMA)         ? "SIF_FINDER_SEEDS_MINIMA"
+                        : refine == SIF_FINDER_REFINE_GAUSSIAN  ? "SIF_FINDER_REFINE_GAUSSIAN"
+                        : refine == SIF_FINDER_REFINE_TRACERS   ? "SIF_FINDER_REFINE_TRACERS"
+                        : refine                                ? "SIF_FINDER_REFINE_HILL"
+                        : (opt & SIF_FINDER_EXTEND_REACH)       ? "SIF_FINDER_EXTEND_REACH"
+                        : (opt & SIF_FINDER_DISPLACE)           ? "SIF_FINDER_DISPLACE"
+                                                                : nullptr;
+  if (missing) {
+    error_set(SIF_ERR_UNSUPPORTED,
+      "options: %s is not in this build (cells on their nodes, no displacement)", missing);
+    return SIF_ERR_UNSUPPORTED;
+  }
+  return SIF_OK;
+}
+
+}  // namespace sif_amd
+
+using namespace sif_amd;
+
+extern "C" {
+
+sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float box,
+  const uint64_t* d_offsets, const float* d_mx, const float* d_my, const float* d_mz,
+  uint32_t mesh_cells, uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_real overlap_fraction, sif_option options, int restore,
+  uint64_t** n_in, sif_amd_rung_stats_t* stats) {
+  int st = ensure_init();
+  if (st == SIF_OK)
+    st = check_args(radii, n_radii, threshold, overlap_fraction);
+  if (st == SIF_OK)
+    st = check_options(options);
+  if (st == SIF_OK && (!d_delta || !d_offsets || !d_mx || !d_my || !d_mz ||
+                        n_particles == 0 || mesh_cells == 0 || !(box > 0.0f))) {
+    error_set(SIF_ERR_INVALID, "exodus: grid and mesh arrays are required");
+    st = SIF_ERR_INVALID;
+  }
+  if (st != SIF_OK)
+    return nullptr;
+  RunSpec R;
+  R.d_delta = d_delta;
+  R.n = n_cells;
+  R.box = box;
+  R.d_offsets = d_offsets;
+  R.d_mx = d_mx;
+  R.d_my = d_my;
+  R.d_mz = d_mz;
+  R.mesh_n = mesh_cells;
+  R.n_particles = n_particles;
+  sorted_ladder(radii, n_radii, R.radii);
+  R.threshold = threshold;
+  R.f = overlap_fraction;
+  R.opt = options;
+  R.restore = restore != 0;
+  R.prior = nullptr;
+  sif_catalogue_t* cat = nullptr;
+  st = exodus_run(R, &cat, n_in, stats);
+  return st == SIF_OK ? cat : nullptr;
+}
+
+int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells, float box,
+  const sif_real* radii, uint32_t n_radii, sif_option options, float* d_fields) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  std::vector<float> ladder;
+  sorted_ladder(radii, n_radii, ladder);
+  st = exodus_prepare_spectrum(d_delta, n_cells);
+  const uint64_t total = (uint64_t)n_cells * n_cells * n_cells;
+  for (uint32_t i = 0; i < n_radii && st == SIF_OK; i++)
+    st = exodus_smooth(n_cells, box, ladder[i], options, d_fields + i * total);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+}  // extern "C"

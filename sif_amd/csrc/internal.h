@@ -1,0 +1,84 @@
+// sif-amd internal: the stage functions the host driver strings together.
+// Every one enqueues on the library stream; the caller synchronises.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "sif/structures/catalogue.h"
+#include "sif_amd.h"
+
+namespace sif_amd {
+
+// cic.hip
+int check_in_box(const float* x, const float* y, const float* z, uint64_t n,
+  float box, uint64_t* n_bad, uint64_t* n_edge, uint64_t* first_bad);
+int cic_deposit(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t N, float box, float* values, int64_t* acc);
+int grid_mean(const float* values, uint64_t total, double* mean);
+int density_contrast(float* values, uint64_t total, double mean);
+
+// mesh.hip
+int mesh_build(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t nc, float box, uint64_t* offsets, float* mx, float* my,
+  float* mz, float* mw, uint64_t* orig, float* cw, double* total_weight);
+
+// fft.hip
+int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work);
+int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
+  float r, float sigma, int window_order, int window_mode, float* out,
+  float2* work);
+
+// scan.hip
+int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
+  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
+int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
+
+// grow.hip
+struct VoidIndex {
+  const float* cx;
+  const float* cy;
+  const float* cz;
+  const float* r;
+  const uint32_t* cell_start;  // 32^3 + 1 entries
+  const uint32_t* items;
+  float inv_l;  // (float)32 / box
+  float max_r;
+};
+int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
+  const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
+  const float* cz, uint32_t n, float rmin, float r_search, float threshold,
+  float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in);
+int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
+  float* cx, float* cy, float* cz);
+int prefilter(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
+  float box, const uint64_t* mask, int32_t proxy, float radius, float f,
+  float scale, const VoidIndex& V, uint32_t n_voids, uint32_t* keep,
+  uint64_t counts_out[3]);
+
+// primitives.hip compaction helper used by the driver
+int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,
+  uint32_t* out, uint64_t* count);
+
+// One exodus run on device-resident data (exodus.hip).
+struct RunSpec {
+  float* d_delta;
+  uint32_t n;
+  float box;
+  const uint64_t* d_offsets;
+  const float *d_mx, *d_my, *d_mz;
+  uint32_t mesh_n;
+  uint64_t n_particles;
+  std::vector<float> radii;  // descending
+  float threshold, f;
+  sif_option opt;
+  bool restore;
+  const sif_catalogue_t* prior;
+};
+int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
+  sif_amd_rung_stats_t* stats_out);
+int sorted_ladder(const sif_real* radii, uint32_t n_radii, std::vector<float>& out);
+int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real f);
+int check_options(sif_option opt);
+
+}  // namespace sif_amd

@@ -1,0 +1,306 @@
+// Device-wide prefix sums, stream compaction and a stable LSD radix sort,
+// shared by the chain mesh and the candidate scan. Everything here is integer
+// and order-preserving, so results do not depend on launch shapes.
+#include "common.h"
+#include "internal.h"
+#include "primitives.h"
+
+namespace sif_amd {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_THREADS * SCAN_ITEMS;
+
+// Block-wide exclusive scan of one value per thread (64-wide wavefronts).
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* total) {
+  __shared__ uint64_t wave_sums[SCAN_THREADS / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(x, off, 64);
+    if (lane >= off)
+      x += y;
+  }
+  if (lane == 63)
+    wave_sums[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int w = 0; w < SCAN_THREADS / 64; w++) {
+    if (w < wave)
+      before += wave_sums[w];
+    all += wave_sums[w];
+  }
+  __syncthreads();
+  if (total)
+    *total = all;
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) tile_sums_u32(const uint32_t* __restrict__ in,
+  uint64_t n, uint64_t* __restrict__ sums) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  uint64_t s = 0;
+  for (int j = 0; j < SCAN_ITEMS; j++)
+    if (base + j < n)
+      s += in[base + j];
+  uint64_t total;
+  block_exclusive_scan(s, &total);
+  if (threadIdx.x == 0)
+    sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) tile_sums_u64(const uint64_t* __restrict__ in,
+  uint64_t n, uint64_t* __restrict__ sums) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  uint64_t s = 0;
+  for (int j = 0; j < SCAN_ITEMS; j++)
+    if (base + j < n)
+      s += in[base + j];
+  uint64_t total;
+  block_exclusive_scan(s, &total);
+  if (threadIdx.x == 0)
+    sums[blockIdx.x] = total;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SCAN_THREADS) tile_scan(const T* __restrict__ in, uint64_t n,
+  const uint64_t* __restrict__ tile_offsets, uint64_t* __restrict__ out,
+  uint64_t* __restrict__ total_out) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  uint64_t v[SCAN_ITEMS];
+  uint64_t s = 0;
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    v[j] = base + j < n ? (uint64_t)in[base + j] : 0;
+    s += v[j];
+  }
+  uint64_t run = block_exclusive_scan(s, nullptr) + (tile_offsets ? tile_offsets[blockIdx.x] : 0);
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    if (base + j < n)
+      out[base + j] = run;
+    run += v[j];
+  }
+  // The last thread of the last tile ends on the grand total.
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1)
+    *total_out = run;
+}
+
+// Exclusive scan of 64-bit values in place-compatible form (out may not
+// alias in). Recursive over tiles.
+static int scan_u64_rec(const uint64_t* in, uint64_t n, uint64_t* out,
+  uint64_t* total_dev, DeviceBuffer* scratch, int depth);
+
+static int scan_tiles(uint64_t n_tiles, uint64_t** tile_offsets,
+  DeviceBuffer* scratch, int depth, const void* in, uint64_t n, bool is32) {
+  uint64_t* sums = scratch[depth].as<uint64_t>();
+  if (is32)
+    tile_sums_u32<<<(unsigned)n_tiles, SCAN_THREADS, 0, stream()>>>(
+      (const uint32_t*)in, n, sums);
+  else
+    tile_sums_u64<<<(unsigned)n_tiles, SCAN_THREADS, 0, stream()>>>(
+      (const uint64_t*)in, n, sums);
+  SIF_HIP(hipGetLastError());
+  uint64_t* offs = sums + n_tiles;
+  int st = scan_u64_rec(sums, n_tiles, offs, nullptr, scratch, depth + 1);
+  if (st != SIF_OK)
+    return st;
+  *tile_offsets = offs;
+  return SIF_OK;
+}
+
+static int scan_u64_rec(const uint64_t* in, uint64_t n, uint64_t* out,
+  uint64_t* total_dev, DeviceBuffer* scratch, int depth) {
+  const uint64_t n_tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (depth >= 4) {
+    error_set(SIF_ERR_RANGE, "scan: %llu elements is beyond the supported size",
+      (unsigned long long)n);
+    return SIF_ERR_RANGE;
+  }
+  int st = scratch[depth].reserve(2 * (n_tiles + 1) * sizeof(uint64_t));
+  if (st != SIF_OK)
+    return st;
+  uint64_t* offs = nullptr;
+  if (n_tiles > 1) {
+    st = scan_tiles(n_tiles, &offs, scratch, depth, in, n, false);
+    if (st != SIF_OK)
+      return st;
+  }
+  tile_scan<uint64_t><<<(unsigned)(n_tiles ? n_tiles : 1), SCAN_THREADS, 0, stream()>>>(
+    in, n, offs, out, total_dev);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+namespace {
+DeviceBuffer g_scan_scratch[4];
+}
+
+int exclusive_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out,
+  uint64_t* total_dev) {
+  const uint64_t n_tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  int st = g_scan_scratch[0].reserve(2 * (n_tiles + 1) * sizeof(uint64_t));
+  if (st != SIF_OK)
+    return st;
+  uint64_t* offs = nullptr;
+  if (n_tiles > 1) {
+    st = scan_tiles(n_tiles, &offs, g_scan_scratch, 0, in, n, true);
+    if (st != SIF_OK)
+      return st;
+  }
+  tile_scan<uint32_t><<<(unsigned)(n_tiles ? n_tiles : 1), SCAN_THREADS, 0, stream()>>>(
+    in, n, offs, out, total_dev);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+int exclusive_scan_u64(const uint64_t* in, uint64_t n, uint64_t* out,
+  uint64_t* total_dev) {
+  return scan_u64_rec(in, n, out, total_dev, g_scan_scratch, 0);
+}
+
+// --- stable LSD radix sort of (u32 key, u32 value), 4-bit digits ---
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr uint32_t RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_RADIX = 16;
+
+__global__ void __launch_bounds__(RS_THREADS) radix_hist(const uint32_t* __restrict__ keys,
+  uint32_t n, int shift, uint32_t n_tiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[RS_RADIX];
+  if (threadIdx.x < RS_RADIX)
+    h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * RS_TILE + threadIdx.x * RS_ITEMS;
+  for (int j = 0; j < RS_ITEMS; j++)
+    if (base + j < n)
+      atomicAdd(&h[(keys[base + j] >> shift) & (RS_RADIX - 1)], 1u);
+  __syncthreads();
+  if (threadIdx.x < RS_RADIX)
+    hist[(uint64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Each thread owns RS_ITEMS consecutive items, so block order = thread-major
+// then item order, and the rank of an item among equal digits follows it.
+__global__ void __launch_bounds__(RS_THREADS) radix_scatter(const uint32_t* __restrict__ keys,
+  const uint32_t* __restrict__ vals, uint32_t n, int shift, uint32_t n_tiles,
+  const uint64_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
+  uint32_t* __restrict__ vals_out) {
+  __shared__ uint32_t cnt[RS_RADIX][RS_THREADS];
+  const uint32_t base = blockIdx.x * RS_TILE + threadIdx.x * RS_ITEMS;
+  uint32_t k[RS_ITEMS], v[RS_ITEMS];
+  uint32_t local[RS_RADIX];
+  for (int d = 0; d < RS_RADIX; d++)
+    local[d] = 0;
+  for (int j = 0; j < RS_ITEMS; j++) {
+    if (base + j < n) {
+      k[j] = keys[base + j];
+      v[j] = vals[base + j];
+      local[(k[j] >> shift) & (RS_RADIX - 1)]++;
+    }
+  }
+  for (int d = 0; d < RS_RADIX; d++)
+    cnt[d][threadIdx.x] = local[d];
+  __syncthreads();
+  // exclusive scan along threads for each digit: 16 digits, each by 16 lanes
+  // of work; simple and fixed-order.
+  if (threadIdx.x < RS_RADIX) {
+    uint32_t run = 0;
+    for (int t = 0; t < RS_THREADS; t++) {
+      const uint32_t c = cnt[threadIdx.x][t];
+      cnt[threadIdx.x][t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  uint32_t pos[RS_RADIX];
+  for (int d = 0; d < RS_RADIX; d++)
+    pos[d] = (uint32_t)offsets[(uint64_t)d * n_tiles + blockIdx.x] + cnt[d][threadIdx.x];
+  for (int j = 0; j < RS_ITEMS; j++) {
+    if (base + j < n) {
+      const uint32_t d = (k[j] >> shift) & (RS_RADIX - 1);
+      keys_out[pos[d]] = k[j];
+      vals_out[pos[d]] = v[j];
+      pos[d]++;
+    }
+  }
+}
+
+namespace {
+DeviceBuffer g_rs_hist, g_rs_offs, g_rs_k, g_rs_v;
+}
+
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits) {
+  if (n <= 1)
+    return SIF_OK;
+  const uint32_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
+  const uint64_t hsize = (uint64_t)RS_RADIX * n_tiles;
+  int st = g_rs_hist.reserve(hsize * sizeof(uint32_t));
+  if (st == SIF_OK)
+    st = g_rs_offs.reserve((hsize + 1) * sizeof(uint64_t));
+  if (st == SIF_OK)
+    st = g_rs_k.reserve((uint64_t)n * sizeof(uint32_t));
+  if (st == SIF_OK)
+    st = g_rs_v.reserve((uint64_t)n * sizeof(uint32_t));
+  if (st != SIF_OK)
+    return st;
+  uint32_t *ka = keys, *va = vals, *kb = g_rs_k.as<uint32_t>(), *vb = g_rs_v.as<uint32_t>();
+  int passes = (bits + 3) / 4;
+  for (int p = 0; p < passes; p++) {
+    const int shift = 4 * p;
+    radix_hist<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, n, shift, n_tiles,
+      g_rs_hist.as<uint32_t>());
+    SIF_HIP(hipGetLastError());
+    st = exclusive_scan_u32(g_rs_hist.as<uint32_t>(), hsize, g_rs_offs.as<uint64_t>(), nullptr);
+    if (st != SIF_OK)
+      return st;
+    radix_scatter<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, va, n, shift, n_tiles,
+      g_rs_offs.as<uint64_t>(), kb, vb);
+    SIF_HIP(hipGetLastError());
+    uint32_t* t = ka;
+    ka = kb;
+    kb = t;
+    t = va;
+    va = vb;
+    vb = t;
+  }
+  if (ka != keys) {
+    SIF_HIP(hipMemcpyAsync(keys, ka, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
+    SIF_HIP(hipMemcpyAsync(vals, va, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
+  }
+  return SIF_OK;
+}
+
+// --- order-preserving compaction of u32 values by keep flags ---
+
+__global__ void compact_scatter(const uint32_t* __restrict__ in,
+  const uint32_t* __restrict__ keep, const uint64_t* __restrict__ pos, uint64_t n,
+  uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && keep[i])
+    out[pos[i]] = in[i];
+}
+
+namespace {
+DeviceBuffer g_cpos, g_ccount;
+}
+
+int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,
+  uint32_t* out, uint64_t* count) {
+  int st = g_cpos.reserve((n ? n : 1) * 8);
+  if (st == SIF_OK)
+    st = g_ccount.reserve(8);
+  if (st != SIF_OK)
+    return st;
+  st = exclusive_scan_u32(keep, n, g_cpos.as<uint64_t>(), g_ccount.as<uint64_t>());
+  if (st != SIF_OK)
+    return st;
+  if (n)
+    compact_scatter<<<grid_for(n, 256), 256, 0, stream()>>>(in, keep,
+      g_cpos.as<uint64_t>(), n, out);
+  SIF_HIP(hipGetLastError());
+  SIF_HIP(hipMemcpyAsync(count, g_ccount.ptr, 8, hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+}  // namespace sif_amd

@@ -1,0 +1,255 @@
+// sif-amd runtime: error record, system init, device memory helpers and the
+// per-stage timers.
+//
+// Replaces include/sif/utils/logger.h:297-349 (error record) and
+// include/sif/core/system.h (sif_system_init/finalise) of sif v0.5.0 for the
+// exodus path.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "common.h"
+#include "sif/core/system.h"
+#include "sif/utils/logger.h"
+#include "sif_amd.h"
+
+namespace sif_amd {
+
+namespace {
+thread_local int t_status = SIF_OK;
+thread_local char t_message[512] = "";
+
+std::mutex g_mutex;
+bool g_init = false;
+int g_device = 0;
+hipStream_t g_stream = nullptr;
+
+bool g_timing = false;
+hipEvent_t g_ev_start[SIF_AMD_N_STAGES];
+hipEvent_t g_ev_stop[SIF_AMD_N_STAGES];
+bool g_ev_made = false;
+bool g_open[SIF_AMD_N_STAGES];
+double g_ms[SIF_AMD_N_STAGES];
+uint64_t g_count[SIF_AMD_N_STAGES];
+}  // namespace
+
+void error_set(int status, const char* fmt, ...) {
+  t_status = status;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_message, sizeof(t_message), fmt, ap);
+  va_end(ap);
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess)
+    return SIF_OK;
+  const int st =
+    (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SIF_ERR_ALLOC
+                                                                 : SIF_ERR_INVALID;
+  error_set(st, "HIP: %s failed: %s", what, hipGetErrorString(e));
+  return st;
+}
+
+static int init_locked() {
+  if (g_init)
+    return SIF_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    error_set(SIF_ERR_UNSUPPORTED, "sif_system_init: no HIP device is visible");
+    return SIF_ERR_UNSUPPORTED;
+  }
+  SIF_HIP(hipGetDevice(&g_device));
+  SIF_HIP(hipStreamCreate(&g_stream));
+  g_init = true;
+  return SIF_OK;
+}
+
+int ensure_init() {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return init_locked();
+}
+
+bool initialised() { return g_init; }
+hipStream_t stream() { return g_stream; }
+
+int DeviceBuffer::reserve(size_t want) {
+  if (want <= bytes && ptr)
+    return SIF_OK;
+  release();
+  if (want == 0)
+    want = 256;
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    bytes = 0;
+    error_set(SIF_ERR_ALLOC, "device allocation of %zu bytes failed: %s", want,
+      hipGetErrorString(e));
+    return SIF_ERR_ALLOC;
+  }
+  bytes = want;
+  return SIF_OK;
+}
+
+void DeviceBuffer::release() {
+  if (ptr)
+    (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+void timing_begin(int s) {
+  if (!g_timing || !g_ev_made)
+    return;
+  (void)hipEventRecord(g_ev_start[s], g_stream);
+  g_open[s] = true;
+}
+
+void timing_end(int s) {
+  if (!g_timing || !g_ev_made || !g_open[s])
+    return;
+  (void)hipEventRecord(g_ev_stop[s], g_stream);
+  (void)hipEventSynchronize(g_ev_stop[s]);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, g_ev_start[s], g_ev_stop[s]);
+  g_ms[s] += ms;
+  g_count[s]++;
+  g_open[s] = false;
+}
+
+bool timing_active() { return g_timing; }
+
+void timing_add(int s, double ms) {
+  g_ms[s] += ms;
+  g_count[s]++;
+}
+
+}  // namespace sif_amd
+
+using namespace sif_amd;
+
+extern "C" {
+
+const char* sif_error_message(void) { return t_message; }
+int sif_error_status(void) { return t_status; }
+int sif_error_errno(void) { return 0; }
+void sif_error_clear(void) {
+  t_status = SIF_OK;
+  t_message[0] = '\0';
+}
+
+int sif_system_init(const sif_config_t* config) {
+  (void)config;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return init_locked();
+}
+
+void sif_system_finalise(void) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!g_init)
+    return;
+  if (g_ev_made) {
+    for (int s = 0; s < SIF_AMD_N_STAGES; s++) {
+      (void)hipEventDestroy(g_ev_start[s]);
+      (void)hipEventDestroy(g_ev_stop[s]);
+    }
+    g_ev_made = false;
+  }
+  (void)hipStreamSynchronize(g_stream);
+  (void)hipStreamDestroy(g_stream);
+  g_stream = nullptr;
+  g_init = false;
+}
+
+int sif_system_is_initialised(void) { return g_init ? 1 : 0; }
+
+int sif_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess)
+    return 0;
+  return n;
+}
+
+int sif_amd_malloc(void** ptr, uint64_t bytes) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMalloc(ptr, bytes ? bytes : 256));
+  return SIF_OK;
+}
+
+int sif_amd_free(void* ptr) {
+  if (ptr)
+    SIF_HIP(hipFree(ptr));
+  return SIF_OK;
+}
+
+int sif_amd_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  SIF_HIP(hipStreamSynchronize(g_stream));
+  return SIF_OK;
+}
+
+int sif_amd_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  SIF_HIP(hipStreamSynchronize(g_stream));
+  return SIF_OK;
+}
+
+int sif_amd_memset(void* dst, int value, uint64_t bytes) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemsetAsync(dst, value, bytes, g_stream));
+  SIF_HIP(hipStreamSynchronize(g_stream));
+  return SIF_OK;
+}
+
+int sif_amd_synchronize(void) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(g_stream));
+  return SIF_OK;
+}
+
+int sif_amd_timing_enable(int on) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (on && !g_ev_made) {
+    for (int s = 0; s < SIF_AMD_N_STAGES; s++) {
+      SIF_HIP(hipEventCreate(&g_ev_start[s]));
+      SIF_HIP(hipEventCreate(&g_ev_stop[s]));
+      g_open[s] = false;
+    }
+    g_ev_made = true;
+  }
+  g_timing = on != 0;
+  return SIF_OK;
+}
+
+void sif_amd_timing_reset(void) {
+  for (int s = 0; s < SIF_AMD_N_STAGES; s++) {
+    g_ms[s] = 0.0;
+    g_count[s] = 0;
+  }
+}
+
+int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages) {
+  for (int s = 0; s < n_stages && s < SIF_AMD_N_STAGES; s++) {
+    if (ms)
+      ms[s] = g_ms[s];
+    if (launches)
+      launches[s] = g_count[s];
+  }
+  return SIF_AMD_N_STAGES;
+}
+
+}  // extern "C"

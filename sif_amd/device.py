@@ -1,0 +1,147 @@
+"""Python entry points over the device-level C ABI (include/sif_amd.h): the
+stages of the exodus path on device-resident arrays. Each function calls the
+HIP library; there is no other implementation behind them.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import DeviceArray, check, lib
+
+
+def to_device(a, dtype=np.float32):
+    return DeviceArray.from_host(np.ascontiguousarray(a, dtype=dtype))
+
+
+def check_in_box(dx, dy, dz, n, box):
+    bad, edge, first = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(lib.sif_amd_check_in_box(dx.ptr, dy.ptr, dz.ptr, n, box, C.byref(bad),
+                                   C.byref(edge), C.byref(first)))
+    return bad.value, edge.value, first.value
+
+
+def cic(dx, dy, dz, n, n_cells, box, dw=None, out=None, work=None):
+    """CIC density per unit volume (grid.c:679-746) as a device array."""
+    total = n_cells ** 3
+    out = out or DeviceArray((total,), np.float32)
+    own_work = work is None
+    work = work or DeviceArray((total,), np.int64)
+    check(lib.sif_amd_grid_assign_cic(dx.ptr, dy.ptr, dz.ptr, dw.ptr if dw else None, n,
+                                      n_cells, box, out.ptr, work.ptr))
+    if own_work:
+        work.free()
+    return out
+
+
+def density_contrast(dvalues, total_cells):
+    mean = C.c_double()
+    check(lib.sif_amd_grid_to_density_contrast(dvalues.ptr, total_cells, C.byref(mean)))
+    return mean.value
+
+
+def grid_mean(dvalues, total_cells):
+    mean = C.c_double()
+    check(lib.sif_amd_grid_mean(dvalues.ptr, total_cells, C.byref(mean)))
+    return mean.value
+
+
+class Mesh:
+    """A chain mesh on the device (chain_mesh.h:40-108 layout, SoA arrays)."""
+
+    def __init__(self, dx, dy, dz, n, n_cells, box, dw=None, indices=True):
+        cells = n_cells ** 3
+        self.n_cells, self.box, self.n = n_cells, float(box), n
+        self.offsets = DeviceArray((cells + 1,), np.uint64)
+        self.x = DeviceArray((n,), np.float32)
+        self.y = DeviceArray((n,), np.float32)
+        self.z = DeviceArray((n,), np.float32)
+        self.w = DeviceArray((n,), np.float32) if dw else None
+        self.cell_weights = DeviceArray((cells,), np.float32) if dw else None
+        self.orig = DeviceArray((n,), np.uint64) if indices else None
+        tw = C.c_double()
+        check(lib.sif_amd_chain_mesh_build(
+            dx.ptr, dy.ptr, dz.ptr, dw.ptr if dw else None, n, n_cells, box,
+            self.offsets.ptr, self.x.ptr, self.y.ptr, self.z.ptr,
+            self.w.ptr if self.w else None, self.orig.ptr if self.orig else None,
+            self.cell_weights.ptr if self.cell_weights else None, C.byref(tw)))
+        self.total_weight = tw.value
+
+
+def spectrum(dvalues, n):
+    spec = DeviceArray((n, n, n // 2 + 1, 2), np.float32)
+    work = DeviceArray((n, n, n // 2 + 1, 2), np.float32)
+    check(lib.sif_amd_fft_r2c(dvalues.ptr, n, spec.ptr, work.ptr))
+    work.free()
+    return spec
+
+
+def smooth(dspec, n, box, filt, r, sigma=0.0, window_order=2, window_mode=0, out=None,
+           work=None):
+    out = out or DeviceArray((n ** 3,), np.float32)
+    own = work is None
+    work = work or DeviceArray((n, n, n // 2 + 1, 2), np.float32)
+    check(lib.sif_amd_fft_filter_c2r(dspec.ptr, n, box, filt, r, sigma, window_order,
+                                     window_mode, out.ptr, work.ptr))
+    if own:
+        work.free()
+    return out
+
+
+def scan_candidates(ddelta, total, threshold, dmask=None, capacity=None):
+    cap = capacity or total
+    d_delta = DeviceArray((max(cap, 1),), np.float32)
+    d_flat = DeviceArray((max(cap, 1),), np.uint32)
+    count = C.c_uint64()
+    check(lib.sif_amd_scan_candidates(ddelta.ptr, dmask.ptr if dmask else None, total,
+                                      threshold, d_delta.ptr, d_flat.ptr, cap,
+                                      C.byref(count)))
+    k = count.value
+    return d_delta.get()[:k], d_flat.get()[:k]
+
+
+def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor):
+    """Grows spheres at (k, 3) float32 centres; returns radius, reason, n_in."""
+    c = np.ascontiguousarray(centres, dtype=np.float32)
+    k = c.shape[0]
+    dcx, dcy, dcz = (to_device(c[:, a]) for a in range(3))
+    dr = DeviceArray((max(k, 1),), np.float32)
+    drs = DeviceArray((max(k, 1),), np.uint32)
+    dn = DeviceArray((max(k, 1),), np.uint64)
+    check(lib.sif_amd_find_radius(mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+                                  mesh.n_cells, mesh.box, dcx.ptr, dcy.ptr, dcz.ptr, k,
+                                  rmin, r_search, threshold, vol_factor, dr.ptr, drs.ptr,
+                                  dn.ptr))
+    return dr.get()[:k], drs.get()[:k], dn.get()[:k]
+
+
+def exodus(ddelta, n_cells, box, mesh, radii, threshold, overlap=0.0, options=0,
+           restore=True):
+    """The exodus finder on device data: (catalogue (n, 4), n_in, stats)."""
+    radii = np.ascontiguousarray(radii, dtype=np.float32)
+    nr = len(radii)
+    stats = (N.RungStats * nr)()
+    nin = C.POINTER(C.c_uint64)()
+    cat = lib.sif_amd_exodus_device(
+        ddelta.ptr, n_cells, box, mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+        mesh.n_cells, mesh.n, radii.ctypes.data_as(C.POINTER(C.c_float)), nr, threshold,
+        overlap, options, 1 if restore else 0, C.byref(nin), stats)
+    if not cat:
+        raise N.error_for(lib.sif_error_status())
+    out = N.catalogue_to_numpy(cat)
+    n_in = np.ctypeslib.as_array(nin, (max(len(out), 1),))[:len(out)].copy()
+    N.libc.free(C.cast(nin, C.c_void_p))
+    st = [dict(radius=s.radius, candidates=s.candidates, masked=s.masked, proxy=s.proxy,
+               overlap=s.overlap, rescale_fail=s.rescale_fail, accepted=s.accepted,
+               evaluated=s.evaluated) for s in stats]
+    return out, n_in, st
+
+
+def exodus_fields(ddelta, n_cells, box, radii, options=0):
+    radii = np.ascontiguousarray(radii, dtype=np.float32)
+    total = n_cells ** 3
+    d = DeviceArray((len(radii), total), np.float32)
+    check(lib.sif_amd_exodus_fields(ddelta.ptr, n_cells, box,
+                                    radii.ctypes.data_as(C.POINTER(C.c_float)),
+                                    len(radii), options, d.ptr))
+    return d.get()

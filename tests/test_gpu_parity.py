@@ -1,0 +1,305 @@
+"""GPU parity of every stage of the exodus path against the oracle, through
+the C ABI, on seeded Zel'dovich particle sets small enough for the oracle.
+
+Bars: integer and index work bit-exact (mesh, candidate order, counts);
+the fixed-point CIC bit-exact against its integer model and within 1e-5 of
+the peak against the textbook double deposit and the reference float order
+(tests/test_field_grid.c:812); the FP32 smoothed field against an FP64
+transform within the tolerance stated in the test; radii bit-exact against
+the oracle's restatement on the same centres, and within 1e-5 relative of
+the brute-force radius of tests/test_finders.c:433-465.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_SIDE, BOX, SEED = 32, 100.0, 11
+MESH_CELLS = 10
+THRESH = -0.8
+
+
+@pytest.fixture(scope="module")
+def particles():
+    from sif_amd import zeldovich
+    return zeldovich.generate(N_SIDE, BOX, SEED)
+
+
+@pytest.fixture(scope="module")
+def dev(native, particles):
+    from sif_amd import device as D
+    x, y, z = particles
+    return D, D.to_device(x), D.to_device(y), D.to_device(z)
+
+
+def test_check_in_box(dev, particles):
+    D, dx, dy, dz = dev
+    import oracle
+    x, y, z = particles
+    assert D.check_in_box(dx, dy, dz, len(x), BOX)[:2] == oracle.check_in_box(x, y, z, BOX)[:2]
+    bad = x.copy()
+    bad[7] = BOX * 1.01
+    bad[9] = -1.0
+    bad[11] = np.float32(BOX)  # on the face: accepted within tolerance
+    got = D.check_in_box(D.to_device(bad), dy, dz, len(x), BOX)
+    assert got == (2, 1, 7)
+    assert got == oracle.check_in_box(bad, y, z, BOX)
+
+
+@pytest.mark.parametrize("n_cells", [16, 32, 48])
+def test_cic_matches_fixed_point_model_bit_exact(dev, particles, n_cells):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    got = D.cic(dx, dy, dz, len(x), n_cells, BOX).get()
+    want, _, _ = oracle.cic_fixed(x, y, z, n_cells, BOX)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_cic_against_textbook_and_reference_order(dev, particles):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    n_cells = 32
+    got = D.cic(dx, dy, dz, len(x), n_cells, BOX).get().astype(np.float64)
+    brute = oracle.cic_brute(x, y, z, n_cells, BOX)
+    ref = oracle.cic_float(x, y, z, n_cells, BOX).astype(np.float64)
+    peak = brute.max()
+    assert np.abs(got - brute).max() <= 1e-5 * peak
+    assert np.abs(got - ref).max() <= 1e-5 * peak
+    # mass: every particle deposits 1 (test_field_grid.c:260-309)
+    cell = BOX / n_cells
+    assert abs(got.sum() * cell ** 3 - len(x)) <= 1e-4 * len(x)
+
+
+def test_cic_weighted_and_repeatable(dev, particles):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    w = np.random.default_rng(3).uniform(0.25, 4.0, len(x)).astype(np.float32)
+    dw = D.to_device(w)
+    a = D.cic(dx, dy, dz, len(x), 32, BOX, dw=dw).get()
+    b = D.cic(dx, dy, dz, len(x), 32, BOX, dw=dw).get()
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    want, _, _ = oracle.cic_fixed(x, y, z, 32, BOX, w=w)
+    np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
+
+
+def test_density_contrast(dev, particles):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    g = D.cic(dx, dy, dz, len(x), 32, BOX)
+    host = g.get()
+    mean = D.density_contrast(g, 32 ** 3)
+    assert abs(mean - host.astype(np.float64).mean()) <= 1e-12 * mean
+    want = oracle.density_contrast_with_mean(host, mean)
+    np.testing.assert_array_equal(g.get().view(np.uint32), want.view(np.uint32))
+    ref, ref_mean = oracle.density_contrast(host)
+    assert abs(ref_mean - mean) <= 1e-12 * mean
+
+
+@pytest.mark.parametrize("mesh_cells", [1, 7, 10, 16])
+def test_chain_mesh_bit_exact(dev, particles, mesh_cells):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    m = D.Mesh(dx, dy, dz, len(x), mesh_cells, BOX)
+    want = oracle.chain_mesh(x, y, z, mesh_cells, BOX)
+    np.testing.assert_array_equal(m.offsets.get(), want["offsets"])
+    np.testing.assert_array_equal(m.orig.get(), want["orig"])
+    for a in "xyz":
+        np.testing.assert_array_equal(getattr(m, a).get(), want[a])
+    assert m.total_weight == len(x)
+
+
+def test_chain_mesh_weighted(dev, particles):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    w = (np.random.default_rng(5).integers(1, 8, len(x)) * 0.25).astype(np.float32)
+    m = D.Mesh(dx, dy, dz, len(x), 10, BOX, dw=D.to_device(w))
+    want = oracle.chain_mesh(x, y, z, 10, BOX, w=w)
+    np.testing.assert_array_equal(m.w.get(), want["w"])
+    np.testing.assert_array_equal(m.cell_weights.get(), want["cell_weights"])
+    assert m.total_weight == want["total_weight"]
+
+
+def test_chain_mesh_edge_particles(native):
+    """Coordinates within rounding of the faces are folded (field_internal.h)."""
+    import oracle
+    from sif_amd import device as D
+    L = np.float32(BOX)
+    x = np.array([0.0, L, np.nextafter(L, 0), 50.0, 99.99, -1e-6], np.float32)
+    y = np.array([L, 0.0, 3.0, 50.0, 0.5, 2.0], np.float32)
+    z = np.array([1.0, 2.0, L, 50.0, 0.0, 0.0], np.float32)
+    m = D.Mesh(D.to_device(x), D.to_device(y), D.to_device(z), len(x), 4, BOX)
+    want = oracle.chain_mesh(x, y, z, 4, BOX)
+    np.testing.assert_array_equal(m.offsets.get(), want["offsets"])
+    np.testing.assert_array_equal(m.orig.get(), want["orig"])
+    np.testing.assert_array_equal(m.x.get(), want["x"])
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_fft_round_trip(native, n):
+    """test_fft.c:49-125: r2c then c2r returns the field to 1e-4."""
+    from sif_amd import device as D
+    rng = np.random.default_rng(n)
+    v = rng.standard_normal(n ** 3).astype(np.float32)
+    dv = D.to_device(v)
+    spec = D.spectrum(dv, n)
+    s = spec.get()
+    want = np.fft.rfftn(v.reshape(n, n, n).astype(np.float64))
+    got = s[..., 0] + 1j * s[..., 1]
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    back = D.smooth(spec, n, BOX, 0, 0.0, window_order=0).get()
+    assert np.abs(back - v).max() < 1e-4
+
+
+def test_top_hat_preserves_constant(native):
+    """test_fft.c:129-173."""
+    from sif_amd import device as D
+    n = 32
+    v = np.full(n ** 3, 3.25, np.float32)
+    spec = D.spectrum(D.to_device(v), n)
+    out = D.smooth(spec, n, BOX, 1, 7.0).get()
+    assert np.abs(out - 3.25).max() < 1e-4 * 3.25
+
+
+@pytest.mark.parametrize("radius,window", [(6.0, 2), (12.5, 2), (6.0, 0)])
+def test_smoothing_against_fp64(dev, particles, radius, window):
+    """The FP32 smoothed field against the same filter applied to an FP64
+    transform of the same contrast; the bar (1e-5 of the field's range) is the
+    FP32 transform's rounding at these sizes."""
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    n = 32
+    g = D.cic(dx, dy, dz, len(x), n, BOX)
+    D.density_contrast(g, n ** 3)
+    delta = g.get()
+    spec = D.spectrum(g, n)
+    got = D.smooth(spec, n, BOX, 1, radius, window_order=window).get()
+    lut = oracle.filter_lut(1, radius, 0.0, BOX, n)
+    axis = oracle.window_axis(n, 2, 0) if window else None
+    fac = oracle.mode_factors(lut, axis, n).astype(np.float64)
+    want = np.fft.irfftn(np.fft.rfftn(delta.reshape(n, n, n).astype(np.float64)) * fac,
+                         s=(n, n, n)).reshape(-1)
+    span = want.max() - want.min()
+    err = np.abs(got - want)
+    assert err.max() <= 1e-5 * span, (err.max(), span)
+
+
+def test_scan_candidates_match_oracle(dev, particles):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    n = 32
+    g = D.cic(dx, dy, dz, len(x), n, BOX)
+    D.density_contrast(g, n ** 3)
+    spec = D.spectrum(g, n)
+    f = D.smooth(spec, n, BOX, 1, 8.0)
+    field = f.get()
+    mask = np.zeros((n ** 3 + 63) // 64, np.uint64)
+    mask[3] = 0xFFFF0000FFFF0000
+    dmask = D.to_device(mask, np.uint64)
+    for t in (-0.2, -0.5):
+        gd, gf = D.scan_candidates(f, n ** 3, t, dmask=dmask)
+        od, of = oracle.scan_candidates(field, t, mask=mask, tie_rule=1)
+        np.testing.assert_array_equal(gf, of.astype(np.uint32))
+        np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32))
+        # no exact ties on this field: the reference quicksort gives the same
+        # order
+        rd, rf = oracle.scan_candidates(field, t, mask=mask, tie_rule=0)
+        if len(np.unique(od)) == len(od):
+            np.testing.assert_array_equal(rf, of)
+
+
+def _mesh_pair(dev, particles, mesh_cells):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    m = D.Mesh(dx, dy, dz, len(x), mesh_cells, BOX)
+    om = oracle.OracleMesh(oracle.chain_mesh(x, y, z, mesh_cells, BOX), mesh_cells, BOX)
+    return m, om
+
+
+@pytest.mark.parametrize("rung,factor", [(6.0, 1.5), (9.0, 2.0), (4.0, 1.25)])
+def test_find_radius_bit_exact_and_brute_force(dev, particles, rung, factor):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    m, om = _mesh_pair(dev, particles, MESH_CELLS)
+    rng = np.random.default_rng(int(rung * 10))
+    centres = rng.uniform(0, BOX, (300, 3)).astype(np.float32)
+    cell = BOX / 32
+    r_search = rung + max((factor - 1.0) * rung, 3.0 * cell)
+    vol = (np.float32(4.0) / np.float32(3.0)) * np.float32(3.14159265358979323846) * \
+        (np.float32(len(x)) / (np.float32(BOX) * np.float32(BOX) * np.float32(BOX)))
+    t = np.float32(THRESH)
+    gr, gre, gn = D.find_radius(m, centres, rung, r_search, t, vol)
+    orr, ore, on = oracle.find_radius(om, centres, rung, r_search, t, vol)
+    np.testing.assert_array_equal(gre, ore)
+    np.testing.assert_array_equal(gr.view(np.uint32), orr.view(np.uint32))
+    np.testing.assert_array_equal(gn[gr > 0], on[orr > 0])
+    found = np.nonzero(gr > 0)[0]
+    assert len(found) > 0
+    for i in found[:40]:
+        want = oracle.brute_force_radius(x, y, z, BOX, centres[i], rung, r_search, THRESH)
+        assert want > 0 and abs(gr[i] - want) <= 1e-5 * want
+
+
+def test_find_radius_window_invariance(dev, particles):
+    """The oracle's radius does not depend on the resolve window (the
+    reference's claim, exodus_rescale.c:384-390), so the GPU's choice of
+    window is free."""
+    import oracle
+    _, om = _mesh_pair(dev, particles, MESH_CELLS)
+    centres = np.random.default_rng(9).uniform(0, BOX, (100, 3)).astype(np.float32)
+    vol = (np.float32(4.0) / np.float32(3.0)) * np.float32(3.14159265358979323846) * \
+        (np.float32(N_SIDE ** 3) / (np.float32(BOX) ** 3))
+    base = oracle.find_radius(om, centres, 6.0, 9.375, THRESH, vol, window=1)[0]
+    for w in (2, 8, 64):
+        np.testing.assert_array_equal(
+            oracle.find_radius(om, centres, 6.0, 9.375, THRESH, vol, window=w)[0], base)
+
+
+def _exodus_setup(dev, particles, n=32):
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    g = D.cic(dx, dy, dz, len(x), n, BOX)
+    D.density_contrast(g, n ** 3)
+    return g
+
+
+@pytest.mark.parametrize("overlap,opts", [(0.0, 0), (0.0, 1 << 18), (0.5, 0)])
+def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap, opts):
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    n = 32
+    radii = np.array([14.0, 11.0, 9.0, 7.5, 6.25], np.float32)
+    g = _exodus_setup(dev, particles, n)
+    delta0 = g.get()
+    fields = D.exodus_fields(g, n, BOX, radii, opts)
+    m, om = _mesh_pair(dev, particles, MESH_CELLS)
+    cat, nin, stats = D.exodus(g, n, BOX, m, radii, THRESH, overlap=overlap, options=opts)
+    # the grid comes back restored to float precision
+    assert np.abs(g.get() - delta0).max() < 1e-4 * max(1.0, np.abs(delta0).max())
+    lag = bool(opts & (1 << 18))
+    for witness in (True, False):
+        ocat, onin, orung, ostats = oracle.exodus_run(
+            list(fields), n, BOX, om, len(x), radii, THRESH, overlap=overlap,
+            lagrangian=lag, tie_rule=1, witness=witness)
+        np.testing.assert_array_equal(cat.view(np.uint32), ocat.view(np.uint32))
+        np.testing.assert_array_equal(nin, onin)
+    assert len(cat) > 0
+    assert sum(s["accepted"] for s in stats) == len(cat)
+    # radii against brute force at the rung they came from
+    for (cx, cy, cz, r), k in zip(cat[:20], orung[:20]):
+        rung = float(radii[k])
+        rs = rung + max(0.5 * rung, 3.0 * BOX / n)
+        if k > 0:
+            rs = max(rs, float(radii[k - 1]))
+        want = oracle.brute_force_radius(x, y, z, BOX, (cx, cy, cz), rung, rs, THRESH)
+        assert abs(r - want) <= 1e-5 * want
