@@ -1,0 +1,193 @@
+"""exodus particles/s on one MI355X, with the HBM roofline of the dominant
+kernel and the oracle's CPU path timed beside it.
+
+A step is one full exodus catalogue of a seeded Zel'dovich particle set whose
+positions are already in HBM: CIC deposit, density contrast, chain mesh,
+forward FFT, then every rung of the radius ladder (top-hat smoothing with the
+CIC window divided out, candidate scan and sort, prefilter, sphere growth,
+ordered commit). Default workload: 512^3 particles in a 1000 Mpc/h box, 512^3
+grid, mesh cell 10 Mpc/h, threshold -0.8, a geometric 5% ladder from two grid
+cells to 40 Mpc/h.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain512|plain128]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # name: particles per side, box, grid cells per side, mesh cell, rmax, seed
+    "plain512": dict(n_side=512, box=1000.0, n_cells=512, mesh_cell=10.0, rmax=40.0, seed=2),
+    "plain128": dict(n_side=128, box=250.0, n_cells=128, mesh_cell=5.0, rmax=40.0, seed=1),
+}
+THRESHOLD = -0.8
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(cfg):
+    from sif_amd import zeldovich
+    c = CONFIGS[cfg]
+    t0 = time.time()
+    x, y, z = zeldovich.generate(c["n_side"], c["box"], c["seed"])
+    log(f"[bench] generated {c['n_side']}^3 Zel'dovich particles in {time.time() - t0:.1f}s, "
+        f"crc32 {zeldovich.crc32(x, y, z):08x}")
+    cell = c["box"] / c["n_cells"]
+    radii = zeldovich.ladder(2.0 * cell, c["rmax"], 0.05)
+    mesh_cells = int(round(c["box"] / c["mesh_cell"]))
+    return c, x, y, z, radii, mesh_cells
+
+
+class Step:
+    """Device buffers allocated once; run() is one catalogue."""
+
+    def __init__(self, c, x, y, z, radii, mesh_cells):
+        from sif_amd import device as D
+        from sif_amd._native import DeviceArray
+        self.D = D
+        self.c, self.radii, self.mesh_cells = c, radii, mesh_cells
+        self.n = len(x)
+        self.dx, self.dy, self.dz = D.to_device(x), D.to_device(y), D.to_device(z)
+        n3 = c["n_cells"] ** 3
+        self.grid = DeviceArray((n3,), np.float32)
+        self.work = DeviceArray((n3,), np.int64)
+
+    def run(self):
+        D, c = self.D, self.c
+        D.cic(self.dx, self.dy, self.dz, self.n, c["n_cells"], c["box"], out=self.grid,
+              work=self.work)
+        D.density_contrast(self.grid, c["n_cells"] ** 3)
+        mesh = D.Mesh(self.dx, self.dy, self.dz, self.n, self.mesh_cells, c["box"],
+                      indices=False)
+        cat, nin, stats = D.exodus(self.grid, c["n_cells"], c["box"], mesh, self.radii,
+                                   THRESHOLD, restore=False)
+        return cat, stats
+
+
+def roofline(c, n_particles, timing):
+    """Roofline of the kernel that dominates the step, from the per-stage HIP
+    event times (each stage listed is one kernel launch per call)."""
+    n3 = c["n_cells"] ** 3
+    modes = c["n_cells"] ** 2 * (c["n_cells"] // 2 + 1)
+    # algorithmic bytes per launch: each input read once, each output written once
+    algo = {
+        "cic": 12.0 * n_particles + 4.0 * n3,
+        "smooth": 8.0 * modes + 4.0 * n3,
+        "mesh": 24.0 * n_particles + 8.0 * ((c["box"] / c["mesh_cell"]) ** 3 + 1),
+    }
+    best = None
+    for name, by in algo.items():
+        ms, cnt = timing.get(name, (0.0, 0))
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt * 1e-3
+        share = ms
+        if best is None or share > best[3]:
+            best = (name, by, avg_s, share)
+    name, by, avg_s, _ = best
+    achieved = by / avg_s / 1e9
+    return {"bound": "hbm", "kernel_stage": name, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes": by, "avg_launch_ms": round(avg_s * 1e3, 4)}
+
+
+def cpu_baseline():
+    """The oracle's CPU path (the reference algorithm restated in C, OpenMP)
+    on the reference's own CPU-sized case: 128^3 particles, L = 250."""
+    import oracle
+    from sif_amd import zeldovich
+    c = CONFIGS["plain128"]
+    x, y, z = zeldovich.generate(c["n_side"], c["box"], c["seed"])
+    radii = zeldovich.ladder(2.0 * c["box"] / c["n_cells"], c["rmax"], 0.05)
+    t0 = time.time()
+    cat, _, stage = oracle.exodus_pipeline(x, y, z, c["n_cells"], c["box"],
+                                           int(round(c["box"] / c["mesh_cell"])), radii,
+                                           THRESHOLD)
+    dt = time.time() - t0
+    return {"value": round(len(x) / dt, 1), "unit": "particles/s",
+            "cores": oracle.lib.orc_num_threads(), "kind": "port",
+            "sample": f"128^3 Zel'dovich particles, L=250, 128^3 grid, mesh 50^3, "
+                      f"{len(radii)} rungs, full catalogue ({len(cat)} voids) in {dt:.1f}s; "
+                      f"stages cic/contrast/mesh/fft/smooth/finder "
+                      f"{'/'.join(f'{s:.2f}' for s in stage)} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="plain512", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU figures are not part of "
+              "it (see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from sif_amd import _native as N
+    if not N.has_gpu():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    c, x, y, z, radii, mesh_cells = workload(args.config)
+    step = Step(c, x, y, z, radii, mesh_cells)
+    log(f"[bench] ladder {len(radii)} rungs {radii[0]:.2f}..{radii[-1]:.2f}, mesh {mesh_cells}^3")
+    for w in range(args.warmup):
+        t0 = time.time()
+        cat, stats = step.run()
+        log(f"[bench] warmup {w}: {time.time() - t0:.2f}s, {len(cat)} voids")
+    N.check(N.lib.sif_amd_synchronize())
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        cat, stats = step.run()
+        log(f"[bench] step {s} done")
+    N.check(N.lib.sif_amd_synchronize())
+    dt = time.perf_counter() - t0
+    value = step.n * args.steps / dt
+
+    # per-stage device times in a separate, instrumented step
+    N.check(N.lib.sif_amd_timing_enable(1))
+    N.lib.sif_amd_timing_reset()
+    step.run()
+    timing = N.timing_read()
+    N.check(N.lib.sif_amd_timing_enable(0))
+    log("[bench] stage times (ms, launches): " +
+        ", ".join(f"{k} {v[0]:.1f}/{v[1]}" for k, v in timing.items()))
+    roof = roofline(c, step.n, timing)
+
+    cpu = None
+    if not args.no_cpu:
+        t1 = time.time()
+        cpu = cpu_baseline()
+        log(f"[bench] cpu baseline {cpu['value']:.3g} particles/s in {time.time() - t1:.1f}s")
+
+    out = {
+        "metric": "exodus particles/sec at 512^3 (plain vs batched/adjoint) + % HBM roofline",
+        "value": round(value, 1), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Zel'dovich/BBKS LCDM particle set)",
+        "config": {"workload": f"plain exodus, {c['n_side']}^3 particles, L={c['box']:g}, "
+                               f"grid {c['n_cells']}^3, mesh {mesh_cells}^3, t={THRESHOLD}, "
+                               f"{len(radii)} rungs {radii[-1]:.3f}-{radii[0]:.2f}",
+                   "voids": int(len(cat)), "parallelism": "single GPU"},
+        "stages_ms": {k: round(v[0], 3) for k, v in timing.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -102,6 +102,14 @@ SIF_API int sif_amd_find_radius(const uint64_t* d_offsets, const float* d_mx,
   float rmin, float r_search, float threshold, float vol_factor,
   float* d_radius, uint32_t* d_reason, uint64_t* d_n_in);
 
+/* sif__exodus_template_build (exodus_rescale.c:175-283), host only: the
+ * per-rung stencil the growth kernel walks, as (dx, dy, dz, class) and
+ * (min d^2, max d^2) per entry; *count is the total, entries past capacity
+ * are not written. */
+SIF_API int sif_amd_template(uint32_t mesh_cells, float box, float rmin,
+  float r_search, int32_t* cells4, float* bounds2, uint32_t capacity,
+  uint32_t* count);
+
 /* --- the whole finder on device-resident data --- */
 
 /* Per-rung counters of a run, as sif__finder_log_radius prints them

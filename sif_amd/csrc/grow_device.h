@@ -1,0 +1,519 @@
+// Device code of the sphere growth (grow.hip): one wavefront per candidate.
+// Kept in a header of its own so that the same text also compiles for the
+// host under a wavefront emulator (tests/emu/), where it is checked lane by
+// lane without a GPU.
+#pragma once
+
+namespace sif_amd {
+
+constexpr int GROW_WAVES = 4;
+constexpr int MAX_BINS = 2048;
+constexpr int REFINE_CAP = 1024;
+constexpr int SUB_BINS = 64;
+
+enum { CELL_CORE = 1, CELL_STRADDLES = 2, CELL_EDGE = 3, CELL_SHELL = 4 };
+
+enum {
+  R_OK = 0,
+  R_EMPTY = 1,
+  R_BEYOND = 2,
+  R_BELOW_RUNG = 3,
+  R_DEGENERATE = 4,
+  R_OVERFLOW = 5
+};
+
+struct GrowParams {
+  const uint64_t* offsets;
+  const float* mx;
+  const float* my;
+  const float* mz;
+  int32_t mesh_n;
+  float box;
+  float inv_l;
+  const int4* tcell;    // dx, dy, dz, type
+  const float2* tbound; // min_d2, max_d2
+  uint32_t tcount;
+  float rmin, r_search, threshold, vol_factor;
+  const float* cx;
+  const float* cy;
+  const float* cz;
+  uint32_t n_cand;
+  float* radius;
+  uint32_t* reason;
+  uint64_t* n_in;
+};
+
+struct WaveLDS {
+  uint32_t bins[MAX_BINS];
+  float buf[REFINE_CAP];
+  uint32_t t_start[64];
+  uint32_t t_scan[64];
+  float t_ex[64], t_ey[64], t_ez[64];
+  uint32_t t_type[64];
+  uint32_t sub[SUB_BINS];
+  uint32_t counter;
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(v, off, 64);
+    if (lane >= off)
+      v += y;
+  }
+  return v;
+}
+
+// Values every lane of the wave holds equally, moved to scalar registers so
+// that the compiler sees them as uniform. Every loop bound and branch below
+// that decides the wave's path goes through one of these: a condition the
+// compiler cannot prove uniform makes the loop divergent, and it may then
+// restructure it so that some lanes re-enter a cross-lane operation (a
+// shuffle) without the lane that supplies its value.
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni(uint64_t v) {
+  return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | (uint64_t)uni((uint32_t)v);
+}
+__device__ __forceinline__ float unif(float v) { return __int_as_float(uni(__float_as_int(v))); }
+
+// exodus_rescale.c:349-355, no contraction
+__device__ __forceinline__ float dist2(float px, float py, float pz, float cx,
+  float cy, float cz) {
+  const float dx = __fsub_rn(px, cx), dy = __fsub_rn(py, cy), dz = __fsub_rn(pz, cz);
+  return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+// exodus_rescale.c:360-369
+__device__ __forceinline__ uint32_t bin_of(float d2, float rc2, float inv_w, uint32_t nb) {
+  const int32_t b = (int32_t)__fmul_rn(__fsub_rn(d2, rc2), inv_w);
+  if (b < 0)
+    return 0;
+  if ((uint32_t)b >= nb)
+    return nb - 1;
+  return (uint32_t)b;
+}
+
+struct Query {
+  float cx, cy, cz;
+  int32_t ix, iy, iz;
+};
+
+// exodus_rescale.c:295-342: template entry -> particle range and the image
+// of the centre it is measured against.
+__device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
+  const int4 t, uint32_t* start, uint32_t* count, float* ex, float* ey, float* ez) {
+  const int32_t N = P.mesh_n;
+  const float L = P.box;
+  int32_t ix = q.ix + t.x;
+  float cx = q.cx;
+  if (ix < 0) {
+    ix += N;
+    cx = __fadd_rn(cx, L);
+  } else if (ix >= N) {
+    ix -= N;
+    cx = __fsub_rn(cx, L);
+  }
+  int32_t iy = q.iy + t.y;
+  float cy = q.cy;
+  if (iy < 0) {
+    iy += N;
+    cy = __fadd_rn(cy, L);
+  } else if (iy >= N) {
+    iy -= N;
+    cy = __fsub_rn(cy, L);
+  }
+  int32_t iz = q.iz + t.z;
+  float cz = q.cz;
+  if (iz < 0) {
+    iz += N;
+    cz = __fadd_rn(cz, L);
+  } else if (iz >= N) {
+    iz -= N;
+    cz = __fsub_rn(cz, L);
+  }
+  const uint64_t flat = ((uint64_t)ix * N + (uint64_t)iy) * N + (uint64_t)iz;
+  const uint64_t s = P.offsets[flat], e = P.offsets[flat + 1];
+  *start = (uint32_t)s;
+  *count = (uint32_t)(e - s);
+  *ex = cx;
+  *ey = cy;
+  *ez = cz;
+}
+
+// Walks the template 64 entries at a time. `want(entry, type, bound)` says
+// whether the entry is visited; `whole(type, count)` consumes a whole cell
+// without reading its particles (returns true when it did); `visit(d2, type)`
+// sees every particle of the remaining cells.
+template <class Want, class Whole, class Visit>
+__device__ __forceinline__ void for_particles(const GrowParams& P, const Query& q,
+  WaveLDS& S, int lane, Want want, Whole whole, Visit visit) {
+  for (uint32_t base = 0; base < P.tcount; base += 64) {
+    const uint32_t e = base + lane;
+    uint32_t cnt = 0, start = 0, type = 0;
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    if (e < P.tcount) {
+      const int4 t = P.tcell[e];
+      type = (uint32_t)t.w;
+      if (want(type, P.tbound[e])) {
+        query_cell(P, q, t, &start, &cnt, &ex, &ey, &ez);
+        if (cnt && whole(type, cnt))
+          cnt = 0;
+      }
+    }
+    const uint32_t incl = wave_incl_scan_u32(cnt, lane);
+    const uint32_t total = uni(__shfl(incl, 63, 64));
+    if (total == 0)
+      continue;
+    S.t_start[lane] = start;
+    S.t_scan[lane] = incl;
+    S.t_ex[lane] = ex;
+    S.t_ey[lane] = ey;
+    S.t_ez[lane] = ez;
+    S.t_type[lane] = type;
+    wave_sync();
+    for (uint32_t j = lane; j < total; j += 64) {
+      // first entry whose inclusive prefix exceeds j
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (S.t_scan[mid] > j)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      const uint32_t before = lo ? S.t_scan[lo - 1] : 0u;
+      const uint32_t p = S.t_start[lo] + (j - before);
+      const float d2 = dist2(P.mx[p], P.my[p], P.mz[p], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]);
+      visit(d2, S.t_type[lo]);
+    }
+    wave_sync();
+  }
+}
+
+// Ascending bitonic sort of buf[0..n) (n <= REFINE_CAP) by one wave.
+__device__ void wave_bitonic(float* buf, uint32_t n, int lane) {
+  uint32_t P2 = 1;
+  while (P2 < n)
+    P2 <<= 1;
+  for (uint32_t i = n + lane; i < P2; i += 64)
+    buf[i] = __int_as_float(0x7f800000);  // +inf pads sort last
+  wave_sync();
+  for (uint32_t k = 2; k <= P2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = lane; t < P2 / 2; t += 64) {
+        const uint32_t lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+        const uint32_t hi = lo + j;
+        const bool asc = (lo & k) == 0;
+        const float a = buf[lo], b = buf[hi];
+        if ((a > b) == asc) {
+          buf[lo] = b;
+          buf[hi] = a;
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
+  __shared__ WaveLDS lds[GROW_WAVES];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  WaveLDS& S = lds[w];
+
+  const float rs2 = __fmul_rn(P.r_search, P.r_search);
+  const float rc2 = __fmul_rn(P.rmin, P.rmin);
+  const float span = __fsub_rn(rs2, rc2);
+  const float rs3 = __fmul_rn(__fmul_rn(P.r_search, P.r_search), P.r_search);
+  const float rm3 = __fmul_rn(__fmul_rn(P.rmin, P.rmin), P.rmin);
+  uint32_t nb = MAX_BINS;
+  {
+    const float exp_shell = __fmul_rn(P.vol_factor, __fsub_rn(rs3, rm3));
+    const float want = __fdiv_rn(exp_shell, 16.0f);
+    if (!(want >= 64.0f))
+      nb = 64;
+    else if (want < (float)MAX_BINS)
+      nb = (uint32_t)want;
+  }
+  const float expected = __fmul_rn(__fmul_rn(__fmul_rn(P.vol_factor, P.r_search), P.r_search), P.r_search);
+  const float K = __fmul_rn(__fadd_rn(P.threshold, 1.0f), P.vol_factor);
+  const float K2 = __fmul_rn(K, K);
+  const float inv_w = __fdiv_rn((float)nb, span);
+  const float bin_w = __fdiv_rn(span, (float)nb);
+
+  // One candidate per wavefront, indexed from the block: the index, and so
+  // every branch on it, is uniform over the wave by construction.
+  const uint32_t c = blockIdx.x * GROW_WAVES + (uint32_t)uni(w);
+  if (c >= P.n_cand)
+    return;
+  float radius = -1.0f;
+  uint32_t reason = R_OK;
+  uint64_t n_in_out = 0;
+  {
+    Query q;
+    q.cx = P.cx[c];
+    q.cy = P.cy[c];
+    q.cz = P.cz[c];
+    q.ix = (int32_t)__fmul_rn(q.cx, P.inv_l);
+    q.iy = (int32_t)__fmul_rn(q.cy, P.inv_l);
+    q.iz = (int32_t)__fmul_rn(q.cz, P.inv_l);
+
+    // A centre outside [0, box] would index past the mesh: refused.
+    const bool inside = q.cx >= 0.0f && q.cx <= P.box && q.cy >= 0.0f &&
+                        q.cy <= P.box && q.cz >= 0.0f && q.cz <= P.box &&
+                        q.ix <= P.mesh_n && q.iy <= P.mesh_n && q.iz <= P.mesh_n;
+    if (!inside) {
+      reason = R_DEGENERATE;
+    } else if (!(span > 0.0f)) {
+      reason = R_DEGENERATE;
+    } else {
+      // 1. early refusal, O(surface): exodus_rescale.c:711-782
+      uint64_t within = 0;
+      for_particles(P, q, S, lane,
+        [](uint32_t, float2) { return true; },
+        [&](uint32_t type, uint32_t cnt) {
+          if (type == CELL_CORE || type == CELL_SHELL) {
+            within += cnt;
+            return true;
+          }
+          return false;
+        },
+        [&](float d2, uint32_t) { within += d2 <= rs2 ? 1u : 0u; });
+      within = uni(wave_sum_u64(within));
+      if (within == 0) {
+        reason = R_EMPTY;
+      } else if (__fsub_rn(__fdiv_rn((float)within, expected), 1.0f) <= P.threshold) {
+        reason = R_BEYOND;
+      }
+    }
+
+    if (reason == R_OK) {
+      // 2. histogram of the annulus, uniform in d^2: exodus_rescale.c:793-865
+      for (uint32_t b = lane; b < nb; b += 64)
+        S.bins[b] = 0;
+      wave_sync();
+      uint64_t n_core = 0, n_shell = 0;
+      for_particles(P, q, S, lane,
+        [](uint32_t type, float2) { return true; },
+        [&](uint32_t type, uint32_t cnt) {
+          if (type == CELL_CORE) {
+            n_core += cnt;
+            return true;
+          }
+          return false;
+        },
+        [&](float d2, uint32_t type) {
+          if (type == CELL_SHELL) {
+            atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
+            n_shell++;
+          } else {
+            n_core += d2 <= rc2 ? 1u : 0u;
+            if (d2 > rc2 && d2 <= rs2) {
+              atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
+              n_shell++;
+            }
+          }
+        });
+      wave_sync();
+      const uint64_t total = uni(wave_sum_u64(n_core) + wave_sum_u64(n_shell));
+      if (total == 0) {
+        reason = R_EMPTY;
+      } else if (__fsub_rn(__fdiv_rn((float)total, expected), 1.0f) <= P.threshold) {
+        reason = R_BEYOND;
+      } else {
+        // 3. skip flags: lane L owns bins [L*per, (L+1)*per), top down.
+        const uint32_t per = (nb + 63) / 64;
+        const uint32_t b0 = lane * per;
+        const uint32_t b1 = b0 + per < nb ? b0 + per : nb;
+        uint64_t mine = 0;
+        for (uint32_t b = b0; b < b1; b++)
+          mine += S.bins[b];
+        // bins above this lane's range: suffix over higher lanes
+        uint64_t suffix = mine;
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint64_t y = __shfl_down(suffix, off, 64);
+          if (lane + off < 64)
+            suffix += y;
+        }
+        uint64_t above = suffix - mine;
+        for (int32_t b = (int32_t)b1 - 1; b >= (int32_t)b0; b--) {
+          const uint32_t cnt = S.bins[b];
+          if (cnt == 0)
+            continue;
+          float hi = __fadd_rn(rc2, __fmul_rn((float)(b + 1), bin_w));
+          if (b == (int32_t)nb - 1)
+            hi = __fmul_rn(rs2, __fadd_rn(1.0f, 1e-6f));
+          const float n_min = (float)(total - above - cnt);
+          const float lhs = __fmul_rn(n_min, n_min);
+          const float rhs = __fmul_rn(__fmul_rn(__fmul_rn(K2, hi), hi), hi);
+          if (!(lhs > rhs))
+            S.bins[b] = cnt | 0x80000000u;  // a bin the walk must open
+          above += cnt;
+        }
+        wave_sync();
+
+        // 4. walk: next flagged bin below `limit`, resolve a window, repeat.
+        int32_t limit = (int32_t)nb;
+        while (reason == R_OK && radius < 0.0f) {
+          int best = -1;
+          for (int32_t b = (int32_t)b1 - 1; b >= (int32_t)b0; b--)
+            if (b < limit && (S.bins[b] & 0x80000000u)) {
+              best = b;
+              break;
+            }
+          best = uni(wave_max_i32(best));
+          if (best < 0) {
+            reason = R_BELOW_RUNG;
+            break;
+          }
+          // particles at or below the top of bin `best`
+          uint64_t abv = 0;
+          for (uint32_t b = (uint32_t)best + 1 + lane; b < nb; b += 64)
+            abv += S.bins[b] & 0x7fffffffu;
+          abv = uni(wave_sum_u64(abv));
+          const uint64_t cur_n = total - abv;
+          // window [best - W + 1, best]: as many bins as fit the buffer
+          uint32_t wcnt = 0;
+          if (best - lane >= 0)
+            wcnt = S.bins[best - lane] & 0x7fffffffu;
+          const uint32_t wscan = wave_incl_scan_u32(wcnt, lane);
+          const uint64_t fits = __ballot(best - lane >= 0 && wscan <= REFINE_CAP);
+          int W = uni(fits ? __popcll(fits) : 0);
+          const bool split = W == 0;  // the top bin alone overflows
+          if (split)
+            W = 1;
+          const int32_t blo = best - W + 1;
+          float hi = __fadd_rn(rc2, __fmul_rn((float)(best + 1), bin_w));
+          if (best == (int32_t)nb - 1)
+            hi = __fmul_rn(rs2, __fadd_rn(1.0f, 1e-6f));
+          const float lo = __fadd_rn(rc2, __fmul_rn((float)blo, bin_w));
+          const uint32_t win_total = uni(__shfl(wscan, W - 1, 64));
+
+          // sub-ranges of squared distance: 1 normally, SUB_BINS when split
+          const int n_sub = split ? SUB_BINS : 1;
+          const float inv_sub = n_sub > 1 ? __fdiv_rn((float)n_sub, __fsub_rn(hi, lo)) : 0.0f;
+          uint32_t sub_count_mine = 0;  // lane s holds the count of sub-range s
+          if (split) {
+            S.sub[lane] = 0;
+            wave_sync();
+            for_particles(P, q, S, lane,
+              [&](uint32_t type, float2 bd) {
+                return type != CELL_CORE && !(bd.y < lo) && !(bd.x > hi);
+              },
+              [](uint32_t, uint32_t) { return false; },
+              [&](float d2, uint32_t type) {
+                if (type != CELL_SHELL && !(d2 > rc2 && d2 <= rs2))
+                  return;
+                const uint32_t b = bin_of(d2, rc2, inv_w, nb);
+                if ((int32_t)b < blo || (int32_t)b > best)
+                  return;
+                int32_t sb = (int32_t)__fmul_rn(__fsub_rn(d2, lo), inv_sub);
+                sb = sb < 0 ? 0 : (sb >= SUB_BINS ? SUB_BINS - 1 : sb);
+                atomicAdd(&S.sub[sb], 1u);
+              });
+            wave_sync();
+            sub_count_mine = S.sub[lane];
+          }
+
+          uint64_t top_n = cur_n;  // particles at or below the top of the range
+          bool exhausted = false;
+          for (int s = n_sub - 1; s >= 0 && radius < 0.0f && !exhausted; s--) {
+            const uint32_t expect_cnt = uni(split ? __shfl(sub_count_mine, s, 64) : win_total);
+            if (expect_cnt == 0)
+              continue;
+            if (expect_cnt > (uint32_t)REFINE_CAP) {
+              reason = R_OVERFLOW;
+              break;
+            }
+            S.counter = 0;  // every lane stores the same zero
+            wave_sync();
+            for_particles(P, q, S, lane,
+              [&](uint32_t type, float2 bd) {
+                return type != CELL_CORE && !(bd.y < lo) && !(bd.x > hi);
+              },
+              [](uint32_t, uint32_t) { return false; },
+              [&](float d2, uint32_t type) {
+                if (type != CELL_SHELL && !(d2 > rc2 && d2 <= rs2))
+                  return;
+                const uint32_t b = bin_of(d2, rc2, inv_w, nb);
+                if ((int32_t)b < blo || (int32_t)b > best)
+                  return;
+                if (split) {
+                  int32_t sb = (int32_t)__fmul_rn(__fsub_rn(d2, lo), inv_sub);
+                  sb = sb < 0 ? 0 : (sb >= SUB_BINS ? SUB_BINS - 1 : sb);
+                  if (sb != s)
+                    return;
+                }
+                const uint32_t slot = atomicAdd(&S.counter, 1u);
+                if (slot < (uint32_t)REFINE_CAP)
+                  S.buf[slot] = d2;
+              });
+            wave_sync();
+            const uint32_t got = uni(S.counter);
+            if (got != expect_cnt) {
+              reason = R_OVERFLOW;  // counts disagree: never expected
+              break;
+            }
+            wave_bitonic(S.buf, got, lane);
+            // outside in: element k encloses top_n - got + k particles
+            int found = -1;
+            for (uint32_t k = lane; k < got; k += 64) {
+              const float d2 = S.buf[k];
+              if (d2 == 0.0f)
+                continue;
+              const uint64_t n_in = top_n - got + k;
+              const float rn = (float)n_in;
+              const float lhs = __fmul_rn(rn, rn);
+              const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
+              if (lhs <= rhs)
+                found = (int)k;
+            }
+            found = uni(wave_max_i32(found));
+            if (found >= 0) {
+              const float d2 = unif(S.buf[found]);
+              radius = __fsqrt_rn(d2);
+              n_in_out = top_n - got + (uint64_t)found;
+            }
+            top_n -= got;
+            wave_sync();
+          }
+          if (radius < 0.0f && reason == R_OK)
+            limit = blo;
+          wave_sync();
+        }
+      }
+    }
+  }
+  // Past every loop: only lane 0 writes the wave's result.
+  if (lane == 0) {
+    P.radius[c] = radius;
+    P.reason[c] = reason;
+    if (P.n_in)
+      P.n_in[c] = n_in_out;
+  }
+}
+
+
+}  // namespace sif_amd

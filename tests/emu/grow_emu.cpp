@@ -1,0 +1,87 @@
+// TEST INFRASTRUCTURE ONLY -- runs sif_amd/csrc/grow_device.h's grow_kernel on
+// the CPU under wave_emu.h, for the CPU test of the kernel's text against the
+// oracle (tests/test_grow_emulated.py). Built by tests/emu/Makefile.
+#include <thread>
+#include <vector>
+
+#include "wave_emu.h"
+
+thread_local emu_dim3 threadIdx;
+thread_local emu_dim3 blockIdx;
+emu_dim3 blockDim, gridDim;
+
+#include "grow_device.h"
+
+static EmuWave g_waves[sif_amd::GROW_WAVES];
+static std::atomic<int> g_stuck{0};
+
+EmuWave& emu_wave() { return g_waves[threadIdx.x >> 6]; }
+
+void EmuWave::barrier(const char* site) {
+  std::unique_lock<std::mutex> lk(m);
+  const uint64_t gen = generation;
+  if (++arrived == 64) {
+    arrived = 0;
+    generation++;
+    cv.notify_all();
+    return;
+  }
+  if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return generation != gen; })) {
+    std::fprintf(stderr,
+      "wave_emu: lanes of wave %u, block %u did not all reach %s (%d of 64 arrived)\n",
+      threadIdx.x >> 6, blockIdx.x, site, arrived);
+    g_stuck = 1;
+    std::_Exit(3);
+  }
+}
+
+extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const float* my,
+  const float* mz, int32_t mesh_n, float box, const int32_t* tdx, const int32_t* tdy,
+  const int32_t* tdz, const uint8_t* ttype, const float* tmin, const float* tmax,
+  uint32_t tcount, float rmin, float r_search, float threshold, float vol_factor,
+  const float* cx, const float* cy, const float* cz, uint32_t n, float* radius,
+  uint32_t* reason, uint64_t* n_in, uint32_t blocks) {
+  blocks = (n + sif_amd::GROW_WAVES - 1) / sif_amd::GROW_WAVES;
+  std::vector<int4> cells(tcount);
+  std::vector<float2> bounds(tcount);
+  for (uint32_t i = 0; i < tcount; i++) {
+    cells[i] = make_int4(tdx[i], tdy[i], tdz[i], ttype[i]);
+    bounds[i] = make_float2(tmin[i], tmax[i]);
+  }
+  sif_amd::GrowParams P;
+  P.offsets = offsets;
+  P.mx = mx;
+  P.my = my;
+  P.mz = mz;
+  P.mesh_n = mesh_n;
+  P.box = box;
+  P.inv_l = 1.0f / (box / (float)mesh_n);
+  P.tcell = cells.data();
+  P.tbound = bounds.data();
+  P.tcount = tcount;
+  P.rmin = rmin;
+  P.r_search = r_search;
+  P.threshold = threshold;
+  P.vol_factor = vol_factor;
+  P.cx = cx;
+  P.cy = cy;
+  P.cz = cz;
+  P.n_cand = n;
+  P.radius = radius;
+  P.reason = reason;
+  P.n_in = n_in;
+  blockDim.x = 64 * sif_amd::GROW_WAVES;
+  gridDim.x = blocks;
+  for (uint32_t b = 0; b < blocks; b++) {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < blockDim.x; t++)
+      th.emplace_back([&, t, b] {
+        threadIdx.x = t;
+        blockIdx.x = b;
+        sif_amd::grow_kernel(P);
+      });
+    for (auto& x : th)
+      x.join();
+  }
+  return g_stuck.load();
+}
