@@ -493,7 +493,9 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
             found = uni(wave_max_i32(found));
             if (found >= 0) {
               const float d2 = unif(S.buf[found]);
-              radius = __fsqrt_rn(d2);
+              // correctly rounded: HIP's __fsqrt_rn is the 1-ulp native
+              // instruction unless OCML rounded operations are enabled
+              radius = __builtin_sqrtf(d2);
               n_in_out = top_n - got + (uint64_t)found;
             }
             top_n -= got;
