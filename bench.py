@@ -26,6 +26,9 @@ CONFIGS = {
     # name: particles per side, box, grid cells per side, mesh cell, rmax, seed
     "plain512": dict(n_side=512, box=1000.0, n_cells=512, mesh_cell=10.0, rmax=40.0, seed=2),
     "plain128": dict(n_side=128, box=250.0, n_cells=128, mesh_cell=5.0, rmax=40.0, seed=1),
+    # the CPU baseline's bounded sample of plain512: one eighth of its volume
+    # at the same particle density, grid spacing, mesh cell and ladder
+    "sample256": dict(n_side=256, box=500.0, n_cells=256, mesh_cell=10.0, rmax=40.0, seed=2),
 }
 THRESHOLD = -0.8
 HBM_PEAK_GBS = 8000.0
@@ -101,12 +104,14 @@ def roofline(c, n_particles, timing):
             "traffic": None, "algorithmic_bytes": by, "avg_launch_ms": round(avg_s * 1e3, 4)}
 
 
-def cpu_baseline():
-    """The oracle's CPU path (the reference algorithm restated in C, OpenMP)
-    on the reference's own CPU-sized case: 128^3 particles, L = 250."""
+def cpu_baseline(name="sample256"):
+    """The oracle's CPU path (the reference algorithm restated in C, OpenMP,
+    FFT a plain double-precision radix-2 rather than FFTW) on a bounded sample
+    of the 512^3 workload: 256^3 particles in a 500 Mpc/h box, the same
+    density, grid spacing, mesh cell, threshold and ladder."""
     import oracle
     from sif_amd import zeldovich
-    c = CONFIGS["plain128"]
+    c = CONFIGS[name]
     x, y, z = zeldovich.generate(c["n_side"], c["box"], c["seed"])
     radii = zeldovich.ladder(2.0 * c["box"] / c["n_cells"], c["rmax"], 0.05)
     t0 = time.time()
@@ -116,7 +121,8 @@ def cpu_baseline():
     dt = time.time() - t0
     return {"value": round(len(x) / dt, 1), "unit": "particles/s",
             "cores": oracle.lib.orc_num_threads(), "kind": "port",
-            "sample": f"128^3 Zel'dovich particles, L=250, 128^3 grid, mesh 50^3, "
+            "sample": f"{c['n_side']}^3 Zel'dovich particles, L={c['box']:g}, {c['n_cells']}^3 grid, "
+                      f"mesh {int(round(c['box'] / c['mesh_cell']))}^3, "
                       f"{len(radii)} rungs, full catalogue ({len(cat)} voids) in {dt:.1f}s; "
                       f"stages cic/contrast/mesh/fft/smooth/finder "
                       f"{'/'.join(f'{s:.2f}' for s in stage)} s"}
@@ -159,9 +165,11 @@ def main():
     # per-stage device times in a separate, instrumented step
     N.check(N.lib.sif_amd_timing_enable(1))
     N.lib.sif_amd_timing_reset()
-    step.run()
+    _, stats = step.run()
     timing = N.timing_read()
     N.check(N.lib.sif_amd_timing_enable(0))
+    tot = {k: sum(s[k] for s in stats) for k in stats[0] if k != "radius"}
+    log("[bench] candidates over all rungs: " + ", ".join(f"{k} {v}" for k, v in tot.items()))
     log("[bench] stage times (ms, launches): " +
         ", ".join(f"{k} {v[0]:.1f}/{v[1]}" for k, v in timing.items()))
     roof = roofline(c, step.n, timing)

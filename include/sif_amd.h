@@ -31,8 +31,10 @@ SIF_API int sif_amd_memset(void* d_dst, int value, uint64_t bytes);
 SIF_API int sif_amd_synchronize(void);
 
 /* Per-stage device time (HIP events on the library stream). Stage order:
- * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit. */
-#define SIF_AMD_TIMING_STAGES 9
+ * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit,
+ * retest. commit and retest are host time: the ordered accept loop, and the
+ * re-tests that assemble each growth batch. */
+#define SIF_AMD_TIMING_STAGES 10
 SIF_API int sif_amd_timing_enable(int on);
 SIF_API void sif_amd_timing_reset(void);
 SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);

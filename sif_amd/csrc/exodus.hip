@@ -438,7 +438,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         break;
       const double t1 = now_ms();
       if (timing_active())
-        timing_add(SIF_AMD_STAGE_COMMIT, t1 - t0);
+        timing_add(SIF_AMD_STAGE_RETEST, t1 - t0);
       SIF_HIP(hipMemcpyAsync(g_dev.bflat.ptr, bflat.data(), bc * 4, hipMemcpyHostToDevice,
         stream()));
       st = node_centres(g_dev.bflat.as<uint32_t>(), bc, n, l, g_dev.bcx.as<float>(),
