@@ -32,9 +32,9 @@ SIF_API int sif_amd_synchronize(void);
 
 /* Per-stage device time (HIP events on the library stream). Stage order:
  * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit,
- * retest. commit and retest are host time: the ordered accept loop, and the
- * re-tests that assemble each growth batch. */
-#define SIF_AMD_TIMING_STAGES 10
+ * retest, sort. commit and retest are host time: the ordered accept loop,
+ * and the re-tests that assemble each growth batch. */
+#define SIF_AMD_TIMING_STAGES 11
 SIF_API int sif_amd_timing_enable(int on);
 SIF_API void sif_amd_timing_reset(void);
 SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);

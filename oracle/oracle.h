@@ -139,6 +139,12 @@ int orc_overlap_quick(const uint64_t* mask_words, uint32_t n_cells, uint32_t ix,
   uint32_t iy, uint32_t iz, uint32_t r);
 void orc_mark_sphere(uint64_t* mask_words, float cx, float cy, float cz,
   float r, uint32_t n_cells, float cell_length);
+/* utils.c:850-960 for nq queries against nv voids (reference 32^3 list,
+ * reference max_r given); out[q] = 1 on an overlap. */
+void orc_overlap_exact_many(const float* vcx, const float* vcy, const float* vcz,
+  const float* vr, uint64_t nv, float max_r, const float* qx, const float* qy,
+  const float* qz, const float* qr, uint64_t nq, float box, float f, float scale,
+  uint8_t* out);
 
 /* --- sequential exodus driver (box, cells on nodes, unweighted) --- */
 typedef struct {

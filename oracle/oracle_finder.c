@@ -669,6 +669,32 @@ static int overlap_exact(const orc_catalogue_t* cat, const void_index_t* vi,
   return 0;
 }
 
+/* utils.c:850-960 for many queries against one catalogue, its 32^3 list
+ * built as the reference builds it (insertion in catalogue order). */
+void orc_overlap_exact_many(const float* vcx, const float* vcy, const float* vcz,
+  const float* vr, uint64_t nv, float max_r, const float* qx, const float* qy,
+  const float* qz, const float* qr, uint64_t nq, float box, float f, float scale,
+  uint8_t* out) {
+  void_index_t* vi = calloc(1, sizeof(void_index_t));
+  for (int32_t c = 0; c < VOID_CELLS * VOID_CELLS * VOID_CELLS; c++)
+    vi->head[c] = -1;
+  vi->inv_l = (float)VOID_CELLS / box;
+  for (uint64_t k = 0; k < nv; k++)
+    vindex_insert(vi, k, vcx[k], vcy[k], vcz[k]);
+  orc_catalogue_t cat;
+  memset(&cat, 0, sizeof(cat));
+  cat.cx = (float*)vcx;
+  cat.cy = (float*)vcy;
+  cat.cz = (float*)vcz;
+  cat.r = (float*)vr;
+  cat.n = nv;
+  for (uint64_t q = 0; q < nq; q++)
+    out[q] = (uint8_t)overlap_exact(&cat, vi, qx[q], qy[q], qz[q], qr[q], max_r, box, f,
+      scale);
+  free(vi->next);
+  free(vi);
+}
+
 /* exodus_finder.c:779-845, with min_rj = 0 (no displacement) */
 static double collision_radius(const orc_catalogue_t* cat,
   const void_index_t* vi, float cx, float cy, float cz, float r_hi,

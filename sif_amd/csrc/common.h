@@ -66,5 +66,6 @@ enum {
   SIF_AMD_STAGE_GROW,
   SIF_AMD_STAGE_COMMIT,
   SIF_AMD_STAGE_RETEST,
+  SIF_AMD_STAGE_SORT,
   SIF_AMD_N_STAGES
 };

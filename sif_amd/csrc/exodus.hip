@@ -28,6 +28,8 @@
 
 #include "common.h"
 #include "internal.h"
+#include "mask.h"
+#include "overlap.h"
 #include "sif/structures/catalogue.h"
 #include "sif_amd.h"
 
@@ -35,83 +37,10 @@ namespace sif_amd {
 
 namespace {
 
-constexpr int VOID_CELLS = 32;
-constexpr uint32_t BATCH_FLOOR = 512;
-constexpr uint32_t BATCH_MAX = 16384;
-
-inline uint32_t wrap(int32_t v, uint32_t n) {
-  const int32_t w = v % (int32_t)n;
-  return (uint32_t)(w < 0 ? w + (int32_t)n : w);
-}
-
-// Host mask of accepted-void cells, n^3 bits.
-struct Mask {
-  std::vector<uint64_t> words;
-  uint32_t n = 0;
-  bool dirty = true;
-  void init(uint32_t n_) {
-    n = n_;
-    words.assign(((uint64_t)n * n * n + 63) / 64, 0ull);
-    dirty = true;
-  }
-  bool get(uint64_t i) const { return (words[i >> 6] >> (i & 63)) & 1ull; }
-  void set(uint64_t i) { words[i >> 6] |= 1ull << (i & 63); }
-};
-
-// utils.c:819-848
-bool overlap_quick(const Mask& m, uint32_t ix, uint32_t iy, uint32_t iz, uint32_t r) {
-  const uint64_t n = m.n;
-  if (m.get(((uint64_t)wrap((int32_t)(ix + r), m.n) * n + iy) * n + iz))
-    return true;
-  if (m.get(((uint64_t)wrap((int32_t)ix - (int32_t)r, m.n) * n + iy) * n + iz))
-    return true;
-  if (m.get(((uint64_t)ix * n + wrap((int32_t)(iy + r), m.n)) * n + iz))
-    return true;
-  if (m.get(((uint64_t)ix * n + wrap((int32_t)iy - (int32_t)r, m.n)) * n + iz))
-    return true;
-  if (m.get(((uint64_t)ix * n + iy) * n + wrap((int32_t)(iz + r), m.n)))
-    return true;
-  if (m.get(((uint64_t)ix * n + iy) * n + wrap((int32_t)iz - (int32_t)r, m.n)))
-    return true;
-  return false;
-}
-
-// utils.c:965-1033, setting form: every cell whose node lies within r.
-void mark_sphere(Mask& m, float cx, float cy, float cz, float r, float l) {
-  const float r2 = r * r;
-  const float inv_l = 1.0f / l;
-  const float L = (float)m.n * l;
-  const float half = L * 0.5f;
-  const int32_t c = (int32_t)(r * inv_l) + 1;
-  const int32_t ox = (int32_t)(cx * inv_l), oy = (int32_t)(cy * inv_l),
-                oz = (int32_t)(cz * inv_l);
-  for (int32_t dx = -c; dx <= c; dx++) {
-    const int32_t gx = ox + dx;
-    float ddx = fabsf((float)gx * l - cx);
-    ddx = ddx > half ? L - ddx : ddx;
-    const float dx2 = ddx * ddx;
-    if (dx2 > r2)
-      continue;
-    const uint32_t wx = wrap(gx, m.n);
-    for (int32_t dy = -c; dy <= c; dy++) {
-      const int32_t gy = oy + dy;
-      float ddy = fabsf((float)gy * l - cy);
-      ddy = ddy > half ? L - ddy : ddy;
-      const float dxy2 = dx2 + ddy * ddy;
-      if (dxy2 > r2)
-        continue;
-      const uint32_t wy = wrap(gy, m.n);
-      for (int32_t dz = -c; dz <= c; dz++) {
-        const int32_t gz = oz + dz;
-        float ddz = fabsf((float)gz * l - cz);
-        ddz = ddz > half ? L - ddz : ddz;
-        if (dxy2 + ddz * ddz <= r2)
-          m.set(((uint64_t)wx * m.n + wy) * (uint64_t)m.n + wrap(gz, m.n));
-      }
-    }
-  }
-  m.dirty = true;
-}
+// Growth batches: the reference's adaptive rule (exodus_finder.c:2331-2349)
+// with a floor that fills the GPU (about 2048 wavefronts resident).
+constexpr uint32_t BATCH_FLOOR = 2048;
+constexpr uint32_t BATCH_MAX = 32768;
 
 // The accepted voids with their 32^3 cell list (cell_linked_list.c:41-59,
 // :181-210; head push, periodic cell index).
@@ -127,8 +56,8 @@ struct Voids {
     head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
   }
   uint32_t cell_of(float x, float y, float z) const {
-    auto ax = [&](float v) { return wrap((int32_t)floorf(v * inv_l), VOID_CELLS); };
-    return (ax(x) * VOID_CELLS + ax(y)) * VOID_CELLS + ax(z);
+    return (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
+           void_axis_cell(z, inv_l);
   }
   void add(float x, float y, float z, float rr, uint64_t nin) {
     const int32_t k = (int32_t)cx.size();
@@ -155,11 +84,11 @@ struct Voids {
     const bool interior = (x - sr >= 0.0f) && (x + sr < box) && (y - sr >= 0.0f) &&
                           (y + sr < box) && (z - sr >= 0.0f) && (z + sr < box);
     for (int32_t a = lo[0]; a <= hi[0]; a++) {
-      const uint32_t wa = wrap(a, VOID_CELLS);
+      const uint32_t wa = void_wrap(a);
       for (int32_t b = lo[1]; b <= hi[1]; b++) {
-        const uint32_t wb = wrap(b, VOID_CELLS);
+        const uint32_t wb = void_wrap(b);
         for (int32_t c = lo[2]; c <= hi[2]; c++) {
-          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wrap(c, VOID_CELLS);
+          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + void_wrap(c);
           for (int32_t i = head[cell]; i != -1; i = next[i]) {
             const float ri = r[i];
             const float mr = rr < ri ? rr : ri;
@@ -193,9 +122,73 @@ struct Voids {
   }
 };
 
+// The voids accepted since the rung began, in a cell list of their own (the
+// device index holds the catalogue as the rung began). state() is the
+// overlap state of overlap.h over these voids alone.
+struct RungVoids {
+  std::vector<float> cx, cy, cz, r;
+  std::vector<int32_t> head, next;
+  std::vector<uint32_t> touched;
+  float inv_l = 0.0f, max_r = 0.0f;
+  void init(float box) {
+    inv_l = (float)VOID_CELLS / box;
+    head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
+  }
+  void clear() {
+    for (uint32_t c : touched)
+      head[c] = -1;
+    touched.clear();
+    cx.clear();
+    cy.clear();
+    cz.clear();
+    r.clear();
+    next.clear();
+    max_r = 0.0f;
+  }
+  void add(float x, float y, float z, float rr) {
+    const int32_t k = (int32_t)cx.size();
+    cx.push_back(x);
+    cy.push_back(y);
+    cz.push_back(z);
+    r.push_back(rr);
+    const uint32_t c =
+      (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
+      void_axis_cell(z, inv_l);
+    if (head[c] == -1)
+      touched.push_back(c);
+    next.push_back(head[c]);
+    head[c] = k;
+    if (rr > max_r)
+      max_r = rr;
+  }
+  int state(const OverlapQuery& q) const {
+    if (cx.empty())
+      return 0;
+    int32_t lo[3], hi[3];
+    superset_scan(q, max_r, inv_l, lo, hi);
+    int st = 0;
+    for (int32_t a = lo[0]; a <= hi[0]; a++) {
+      const uint32_t wa = void_wrap(a);
+      for (int32_t b = lo[1]; b <= hi[1]; b++) {
+        const uint32_t wb = void_wrap(b);
+        for (int32_t c = lo[2]; c <= hi[2]; c++) {
+          const uint32_t wc = void_wrap(c);
+          for (int32_t i = head[(wa * VOID_CELLS + wb) * VOID_CELLS + wc]; i != -1; i = next[i]) {
+            const int s = pair_state(q, wa, wb, wc, cx[i], cy[i], cz[i], r[i]);
+            if (s == 1)
+              return 1;
+            st |= s;
+          }
+        }
+      }
+    }
+    return st;
+  }
+};
+
 struct DeviceState {
-  DeviceBuffer spec, work, mask, keys, flat, keep, surv;
-  DeviceBuffer bflat, bcx, bcy, bcz, brad, breason, bnin;
+  DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys;
+  DeviceBuffer bflat, bcx, bcy, bcz, brad, breason, bnin, bstate;
   DeviceBuffer vcx, vcy, vcz, vr, vstart, vitems;
 };
 DeviceState g_dev;
@@ -206,23 +199,37 @@ double now_ms() {
     .count();
 }
 
+// Radius classes of the device index, by void-cell widths: a quarter, a
+// half, one, and more.
+uint32_t void_class(float r, float cell) {
+  return r < 0.25f * cell ? 0u : r < 0.5f * cell ? 1u : r < cell ? 2u : 3u;
+}
+
 int upload_voids(const Voids& V, VoidIndex* out) {
   const uint32_t nv = (uint32_t)V.cx.size();
   const uint32_t cells = VOID_CELLS * VOID_CELLS * VOID_CELLS;
-  std::vector<uint32_t> start(cells + 1, 0), items(nv ? nv : 1);
-  for (uint32_t i = 0; i < nv; i++)
-    start[V.cell_of(V.cx[i], V.cy[i], V.cz[i]) + 1]++;
-  for (uint32_t c = 0; c < cells; c++)
+  const uint32_t slots = VOID_CLASSES * cells;
+  const float cell_w = V.box / (float)VOID_CELLS;
+  std::vector<uint32_t> start(slots + 1, 0), items(nv ? nv : 1), slot(nv ? nv : 1);
+  float cmax[VOID_CLASSES] = {-1.0f, -1.0f, -1.0f, -1.0f};
+  for (uint32_t i = 0; i < nv; i++) {
+    const uint32_t cls = void_class(V.r[i], cell_w);
+    slot[i] = cls * cells + V.cell_of(V.cx[i], V.cy[i], V.cz[i]);
+    start[slot[i] + 1]++;
+    if (V.r[i] > cmax[cls])
+      cmax[cls] = V.r[i];
+  }
+  for (uint32_t c = 0; c < slots; c++)
     start[c + 1] += start[c];
   std::vector<uint32_t> cur(start.begin(), start.end() - 1);
   for (uint32_t i = 0; i < nv; i++)
-    items[cur[V.cell_of(V.cx[i], V.cy[i], V.cz[i])]++] = i;
+    items[cur[slot[i]]++] = i;
   const size_t fb = (size_t)(nv ? nv : 1) * sizeof(float);
   int st = g_dev.vcx.reserve(fb);
   if (st == SIF_OK) st = g_dev.vcy.reserve(fb);
   if (st == SIF_OK) st = g_dev.vcz.reserve(fb);
   if (st == SIF_OK) st = g_dev.vr.reserve(fb);
-  if (st == SIF_OK) st = g_dev.vstart.reserve((cells + 1) * sizeof(uint32_t));
+  if (st == SIF_OK) st = g_dev.vstart.reserve((slots + 1) * sizeof(uint32_t));
   if (st == SIF_OK) st = g_dev.vitems.reserve(items.size() * sizeof(uint32_t));
   if (st != SIF_OK)
     return st;
@@ -233,7 +240,7 @@ int upload_voids(const Voids& V, VoidIndex* out) {
     SIF_HIP(hipMemcpyAsync(g_dev.vr.ptr, V.r.data(), nv * 4, hipMemcpyHostToDevice, stream()));
     SIF_HIP(hipMemcpyAsync(g_dev.vitems.ptr, items.data(), nv * 4, hipMemcpyHostToDevice, stream()));
   }
-  SIF_HIP(hipMemcpyAsync(g_dev.vstart.ptr, start.data(), (cells + 1) * 4,
+  SIF_HIP(hipMemcpyAsync(g_dev.vstart.ptr, start.data(), (slots + 1) * 4,
     hipMemcpyHostToDevice, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
   out->cx = g_dev.vcx.as<float>();
@@ -242,8 +249,11 @@ int upload_voids(const Voids& V, VoidIndex* out) {
   out->r = g_dev.vr.as<float>();
   out->cell_start = g_dev.vstart.as<uint32_t>();
   out->items = g_dev.vitems.as<uint32_t>();
+  out->n = nv;
   out->inv_l = V.inv_l;
   out->max_r = V.max_r;
+  for (int c = 0; c < VOID_CLASSES; c++)
+    out->class_max_r[c] = cmax[c];
   return SIF_OK;
 }
 
@@ -294,6 +304,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   mask.init(n);
   Voids V;
   V.init(R.box);
+  RungVoids RV;
+  RV.init(R.box);
   if (R.prior) {
     for (uint64_t v = 0; v < R.prior->n_voids; v++) {
       V.add(R.prior->cx[v], R.prior->cy[v], R.prior->cz[v], R.prior->radii[v], 0);
@@ -311,6 +323,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   std::vector<float> brad;
   std::vector<uint32_t> breason;
   std::vector<uint64_t> bnin;
+  std::vector<uint32_t> bstate;
+  std::vector<uint8_t> bedge;
   uint64_t cand_cap = 0;
 
   for (size_t i = 0; i < R.radii.size() && st == SIF_OK; i++) {
@@ -328,17 +342,23 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       mask.dirty = false;
     }
 
-    // scan and sort, growing the candidate buffers when they are short
+    // the candidates in flat order, growing the buffers when they are short
     uint64_t nc = 0;
+    auto reserve_candidates = [&]() {
+      int e = g_dev.keys.reserve(cand_cap * 4);
+      if (e == SIF_OK) e = g_dev.flat.reserve(cand_cap * 4);
+      if (e == SIF_OK) e = g_dev.keep.reserve(cand_cap * 4);
+      if (e == SIF_OK) e = g_dev.surv.reserve(cand_cap * 4);
+      if (e == SIF_OK) e = g_dev.skeys.reserve(cand_cap * 4);
+      return e;
+    };
     for (;;) {
       if (cand_cap == 0) {
         cand_cap = total / 64 + 1024;
-        if ((st = g_dev.keys.reserve(cand_cap * 4)) != SIF_OK) break;
-        if ((st = g_dev.flat.reserve(cand_cap * 4)) != SIF_OK) break;
-        if ((st = g_dev.keep.reserve(cand_cap * 4)) != SIF_OK) break;
-        if ((st = g_dev.surv.reserve(cand_cap * 4)) != SIF_OK) break;
+        if ((st = reserve_candidates()) != SIF_OK)
+          break;
       }
-      st = scan_candidates(R.d_delta, g_dev.mask.as<uint64_t>(), total, R.threshold,
+      st = scan_candidates_flat(R.d_delta, g_dev.mask.as<uint64_t>(), total, R.threshold,
         g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap, &nc);
       if (st != SIF_OK || nc <= cand_cap)
         break;
@@ -347,10 +367,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       g_dev.flat.release();
       g_dev.keep.release();
       g_dev.surv.release();
-      if ((st = g_dev.keys.reserve(cand_cap * 4)) != SIF_OK) break;
-      if ((st = g_dev.flat.reserve(cand_cap * 4)) != SIF_OK) break;
-      if ((st = g_dev.keep.reserve(cand_cap * 4)) != SIF_OK) break;
-      if ((st = g_dev.surv.reserve(cand_cap * 4)) != SIF_OK) break;
+      g_dev.skeys.release();
+      if ((st = reserve_candidates()) != SIF_OK)
+        break;
     }
     if (st != SIF_OK)
       break;
@@ -362,24 +381,32 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       r_search = R.radii[i - 1];
     const int32_t proxy = (int32_t)(scale * radius * (1.0f - R.f) / l) - 1;
 
-    // prefilter against the catalogue as the rung begins
+    // prefilter against the catalogue as the rung begins, in flat order (the
+    // verdicts do not depend on the order; neighbouring candidates share
+    // void cells), then sort the survivors deepest first
     VoidIndex VI;
     st = upload_voids(V, &VI);
     if (st != SIF_OK)
       break;
+    RV.clear();
     uint64_t pf[3] = {0, 0, 0};
     timing_begin(SIF_AMD_STAGE_PREFILTER);
     st = prefilter(g_dev.flat.as<uint32_t>(), (uint32_t)nc, n, l, R.box,
-      g_dev.mask.as<uint64_t>(), proxy, radius, R.f, scale, VI, (uint32_t)V.cx.size(),
-      g_dev.keep.as<uint32_t>(), pf);
+      g_dev.mask.as<uint64_t>(), proxy, radius, R.f, scale, VI, g_dev.keep.as<uint32_t>(), pf);
     if (st != SIF_OK)
       break;
-    uint64_t ns = 0;
+    uint64_t ns = 0, nk = 0;
     st = compact_u32(g_dev.flat.as<uint32_t>(), g_dev.keep.as<uint32_t>(), nc,
       g_dev.surv.as<uint32_t>(), &ns);
+    if (st == SIF_OK)
+      st = compact_u32(g_dev.keys.as<uint32_t>(), g_dev.keep.as<uint32_t>(), nc,
+        g_dev.skeys.as<uint32_t>(), &nk);
     if (st != SIF_OK)
       break;
     timing_end(SIF_AMD_STAGE_PREFILTER);
+    st = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), ns);
+    if (st != SIF_OK)
+      break;
     S.masked += pf[0];
     S.proxy += pf[1];
     S.overlap += pf[2];
@@ -398,6 +425,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if ((st = g_dev.brad.reserve(BATCH_MAX * 4)) != SIF_OK) break;
     if ((st = g_dev.breason.reserve(BATCH_MAX * 4)) != SIF_OK) break;
     if ((st = g_dev.bnin.reserve(BATCH_MAX * 8)) != SIF_OK) break;
+    if ((st = g_dev.bstate.reserve(BATCH_MAX * 4)) != SIF_OK) break;
     uint64_t k = 0;
     const uint64_t nn = n;
     auto node_of = [&](uint32_t flat, uint32_t* ix, uint32_t* iy, uint32_t* iz) {
@@ -405,33 +433,54 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       *iy = (uint32_t)((flat / nn) % nn);
       *iz = (uint32_t)(flat % nn);
     };
+    // An exact overlap with the catalogue as it stands (utils.c:850-948),
+    // given the overlap state of the voids the rung began with: only the
+    // voids accepted since are scanned here, unless a state 2 sends the
+    // candidate to the reference's own scan of the whole catalogue.
+    auto exact_overlap = [&](float cx, float cy, float cz, float rr, uint32_t old_state) {
+      if (old_state == 1)
+        return true;
+      if (old_state == 0) {
+        OverlapQuery q;
+        overlap_query(q, cx, cy, cz, rr, V.max_r, R.box, R.f, scale, V.inv_l);
+        const int s = RV.state(q);
+        if (s != 2)
+          return s == 1;
+      }
+      return V.overlap_exact(cx, cy, cz, rr, R.f, scale);
+    };
     // the rung-radius tests against the catalogue as it stands: 0 pass,
     // 1 masked, 2 proxy, 3 overlap
-    auto rung_tests = [&](uint32_t flat) -> int {
+    auto rung_tests = [&](uint32_t flat, bool edge) -> int {
       if (mask.get(flat))
         return 1;
       uint32_t ix, iy, iz;
       node_of(flat, &ix, &iy, &iz);
       if (proxy > 0 && overlap_quick(mask, ix, iy, iz, (uint32_t)proxy))
         return 2;
-      if (V.overlap_exact((float)ix * l, (float)iy * l, (float)iz * l, radius, R.f, scale))
+      if (exact_overlap((float)ix * l, (float)iy * l, (float)iz * l, radius, edge ? 2u : 0u))
         return 3;
       return 0;
     };
     while (k < ns && st == SIF_OK) {
       const double t0 = now_ms();
       bflat.clear();
+      bedge.clear();
       while (k < ns && bflat.size() < cap) {
-        const uint32_t flat = surv[k++];
-        const int why = rung_tests(flat);
+        const uint32_t flat = surv[k] & 0x7fffffffu;
+        const bool edge = (surv[k] >> 31) != 0;
+        k++;
+        const int why = rung_tests(flat, edge);
         if (why == 1)
           S.masked++;
         else if (why == 2)
           S.proxy++;
         else if (why == 3)
           S.overlap++;
-        else
+        else {
           bflat.push_back(flat);
+          bedge.push_back(edge ? 1 : 0);
+        }
       }
       const uint32_t bc = (uint32_t)bflat.size();
       if (bc == 0)
@@ -448,16 +497,23 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           g_dev.bcx.as<float>(), g_dev.bcy.as<float>(), g_dev.bcz.as<float>(), bc, rmin,
           r_search, R.threshold, vol, g_dev.brad.as<float>(), g_dev.breason.as<uint32_t>(),
           g_dev.bnin.as<uint64_t>());
+      if (st == SIF_OK)
+        st = grown_overlap(g_dev.bcx.as<float>(), g_dev.bcy.as<float>(), g_dev.bcz.as<float>(),
+          g_dev.brad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
+          g_dev.bstate.as<uint32_t>());
       if (st != SIF_OK)
         break;
       S.evaluated += bc;
       brad.resize(bc);
       breason.resize(bc);
       bnin.resize(bc);
+      bstate.resize(bc);
       SIF_HIP(hipMemcpyAsync(brad.data(), g_dev.brad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
       SIF_HIP(hipMemcpyAsync(breason.data(), g_dev.breason.ptr, bc * 4, hipMemcpyDeviceToHost,
         stream()));
       SIF_HIP(hipMemcpyAsync(bnin.data(), g_dev.bnin.ptr, bc * 8, hipMemcpyDeviceToHost, stream()));
+      SIF_HIP(hipMemcpyAsync(bstate.data(), g_dev.bstate.ptr, bc * 4, hipMemcpyDeviceToHost,
+        stream()));
       SIF_HIP(hipStreamSynchronize(stream()));
       const double t2 = now_ms();
 
@@ -472,7 +528,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           st = SIF_ERR_RANGE;
           break;
         }
-        const int why = rung_tests(flat);
+        const int why = rung_tests(flat, bedge[b] != 0);
         if (why) {
           if (why == 1)
             S.masked++;
@@ -495,11 +551,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           S.proxy++;
           continue;
         }
-        if (V.overlap_exact(cx, cy, cz, r, R.f, scale)) {
+        if (exact_overlap(cx, cy, cz, r, bstate[b])) {
           S.overlap++;
           continue;
         }
         V.add(cx, cy, cz, r, bnin[b]);
+        RV.add(cx, cy, cz, r);
         mark_sphere(mask, cx, cy, cz, scale * r, l);
         accepted++;
       }

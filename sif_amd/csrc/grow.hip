@@ -27,12 +27,11 @@
 #include "internal.h"
 #include "sif_amd.h"
 #include "grow_device.h"
+#include "overlap.h"
 
 namespace sif_amd {
 
 // --- prefilter: the rung-radius tests of exodus_finder.c:871-944 ---
-
-constexpr int VOID_CELLS = 32;
 
 __device__ __forceinline__ uint32_t wrap_pbc(int32_t v, uint32_t n) {
   const int32_t w = v % (int32_t)n;
@@ -62,89 +61,99 @@ __device__ bool overlap_quick(const uint64_t* mask, uint32_t n, uint32_t ix,
   return false;
 }
 
-// utils.c:850-960, over a CSR void index instead of a linked list (the
-// answer is a yes/no over a set, so the visiting order does not matter).
-__device__ bool overlap_exact(const VoidIndex& V, float cx, float cy, float cz,
-  float r, float box, float f, float scale) {
-  const float half = __fmul_rn(box, 0.5f);
-  const float mr0 = r < V.max_r ? r : V.max_r;
-  const float sr = __fmul_rn(scale, __fsub_rn(__fadd_rn(r, V.max_r), __fmul_rn(f, mr0)));
-  const int32_t lo0 = (int32_t)floorf(__fmul_rn(__fsub_rn(cx, sr), V.inv_l));
-  const int32_t hi0 = (int32_t)floorf(__fmul_rn(__fadd_rn(cx, sr), V.inv_l));
-  const int32_t lo1 = (int32_t)floorf(__fmul_rn(__fsub_rn(cy, sr), V.inv_l));
-  const int32_t hi1 = (int32_t)floorf(__fmul_rn(__fadd_rn(cy, sr), V.inv_l));
-  const int32_t lo2 = (int32_t)floorf(__fmul_rn(__fsub_rn(cz, sr), V.inv_l));
-  const int32_t hi2 = (int32_t)floorf(__fmul_rn(__fadd_rn(cz, sr), V.inv_l));
-  const bool interior = (__fsub_rn(cx, sr) >= 0.0f) && (__fadd_rn(cx, sr) < box) &&
-                        (__fsub_rn(cy, sr) >= 0.0f) && (__fadd_rn(cy, sr) < box) &&
-                        (__fsub_rn(cz, sr) >= 0.0f) && (__fadd_rn(cz, sr) < box);
-  for (int32_t a = lo0; a <= hi0; a++) {
-    const uint32_t wa = wrap_pbc(a, VOID_CELLS);
-    for (int32_t b = lo1; b <= hi1; b++) {
-      const uint32_t wb = wrap_pbc(b, VOID_CELLS);
-      for (int32_t c = lo2; c <= hi2; c++) {
-        const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wrap_pbc(c, VOID_CELLS);
-        for (uint32_t k = V.cell_start[cell]; k < V.cell_start[cell + 1]; k++) {
-          const uint32_t i = V.items[k];
-          const float ri = V.r[i];
-          const float mr = r < ri ? r : ri;
-          const float tot = __fmul_rn(scale, __fsub_rn(__fadd_rn(r, ri), __fmul_rn(f, mr)));
-          const float t2 = __fmul_rn(tot, tot);
-          if (interior) {
-            const float dx = __fsub_rn(cx, V.cx[i]), dy = __fsub_rn(cy, V.cy[i]),
-                        dz = __fsub_rn(cz, V.cz[i]);
-            if (__fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz)) < t2)
-              return true;
-          } else {
-            float dx = fabsf(__fsub_rn(cx, V.cx[i]));
-            dx = dx > half ? __fsub_rn(box, dx) : dx;
-            const float dx2 = __fmul_rn(dx, dx);
-            if (dx2 >= t2)
-              continue;
-            float dy = fabsf(__fsub_rn(cy, V.cy[i]));
-            dy = dy > half ? __fsub_rn(box, dy) : dy;
-            const float dy2 = __fmul_rn(dy, dy);
-            if (__fadd_rn(dx2, dy2) >= t2)
-              continue;
-            float dz = fabsf(__fsub_rn(cz, V.cz[i]));
-            dz = dz > half ? __fsub_rn(box, dz) : dz;
-            if (__fadd_rn(__fadd_rn(dx2, dy2), __fmul_rn(dz, dz)) < t2)
-              return true;
+// The overlap state (overlap.h) of a query against the CSR void index: the
+// index lists each void under the reference's cell, so a scan of the
+// superset cells sees every void that could collide.
+__device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
+  int state = 0;
+  for (int cls = VOID_CLASSES - 1; cls >= 0; cls--) {
+    if (V.class_max_r[cls] < 0.0f)
+      continue;
+    int32_t lo[3], hi[3];
+    superset_scan(q, V.class_max_r[cls], V.inv_l, lo, hi);
+    const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+    for (int32_t a = lo[0]; a <= hi[0]; a++) {
+      const uint32_t wa = void_wrap(a);
+      for (int32_t b = lo[1]; b <= hi[1]; b++) {
+        const uint32_t wb = void_wrap(b);
+        for (int32_t c = lo[2]; c <= hi[2]; c++) {
+          const uint32_t wc = void_wrap(c);
+          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wc;
+          const uint32_t end = start[cell + 1];
+          for (uint32_t k = start[cell]; k < end; k++) {
+            const uint32_t i = V.items[k];
+            const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
+            if (s == 1)
+              return 1;
+            state |= s;
           }
         }
       }
     }
   }
-  return false;
+  return state;
 }
 
-// keep[k] = 0 refused (masked node, pole probe or exact overlap at the rung
-// radius, against the catalogue as the rung began), 1 kept.
-__global__ void prefilter_kernel(const uint32_t* __restrict__ flat, uint32_t n,
+// keep[k] = 0 refused (masked node, pole probe, or an overlap at the rung
+// radius with the catalogue as the rung began, decided as the reference
+// decides it), 1 kept. A kept candidate that some void collides with under
+// the periodic distance but outside the reference's scan (overlap state 2)
+// gets bit 31 of its flat index set: the host re-tests it against the whole
+// catalogue.
+__global__ void prefilter_kernel(uint32_t* __restrict__ flat, uint32_t n,
   uint32_t grid_n, float l, float box, const uint64_t* __restrict__ mask,
-  int32_t proxy, float radius, float f, float scale, VoidIndex V, uint32_t n_voids,
+  int32_t proxy, float radius, float f, float scale, VoidIndex V,
   uint32_t* __restrict__ keep, unsigned long long* __restrict__ counts) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n)
     return;
-  const uint64_t node = flat[k];
+  const uint32_t node = flat[k];
   uint32_t why = 0;  // 0 kept, 1 masked, 2 proxy, 3 overlap
+  bool edge = false;
   if (mask_bit(mask, node)) {
     why = 1;
   } else {
-    const uint64_t nn = grid_n;
-    const uint32_t ix = (uint32_t)(node / (nn * nn));
-    const uint32_t iy = (uint32_t)((node / nn) % nn);
-    const uint32_t iz = (uint32_t)(node % nn);
-    if (proxy > 0 && overlap_quick(mask, grid_n, ix, iy, iz, (uint32_t)proxy))
+    const uint32_t ix = node / (grid_n * grid_n);
+    const uint32_t iy = (node / grid_n) % grid_n;
+    const uint32_t iz = node % grid_n;
+    if (proxy > 0 && overlap_quick(mask, grid_n, ix, iy, iz, (uint32_t)proxy)) {
       why = 2;
-    else if (n_voids && overlap_exact(V, __fmul_rn((float)ix, l), __fmul_rn((float)iy, l),
-                          __fmul_rn((float)iz, l), radius, box, f, scale))
-      why = 3;
+    } else if (V.n) {
+      OverlapQuery q;
+      overlap_query(q, (float)ix * l, (float)iy * l, (float)iz * l, radius, V.max_r, box,
+        f, scale, V.inv_l);
+      const int s = overlap_state(V, q);
+      if (s == 1)
+        why = 3;
+      else
+        edge = s == 2;
+    }
   }
   keep[k] = why == 0 ? 1u : 0u;
+  if (edge)
+    flat[k] = node | 0x80000000u;
   if (why)
     atomicAdd(&counts[why - 1], 1ull);
+}
+
+// The overlap state of each grown sphere (radius >= 0) against the catalogue
+// as the rung began, for a reference max_r of the catalogue as the batch
+// began (exodus_finder.c:2302-2311, the test at the rescaled radius).
+__global__ void grown_overlap_kernel(const float* __restrict__ cx,
+  const float* __restrict__ cy, const float* __restrict__ cz,
+  const float* __restrict__ radius, uint32_t n, VoidIndex V, float max_r, float box,
+  float f, float scale, uint32_t* __restrict__ state) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n)
+    return;
+  const float r = radius[k];
+  int s = 0;
+  if (r >= 0.0f && V.n) {
+    OverlapQuery q;
+    overlap_query(q, cx[k], cy[k], cz[k], r, max_r, box, f, scale, V.inv_l);
+    s = overlap_state(V, q);
+  }
+  state[k] = (uint32_t)s;
 }
 
 // Candidate nodes -> centres (exodus_finder.c:386-405 for a centre on its
@@ -291,23 +300,33 @@ int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
   return SIF_OK;
 }
 
-int prefilter(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
+int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
-  const VoidIndex& V, uint32_t n_voids, uint32_t* keep, uint64_t counts_out[3]) {
+  const VoidIndex& V, uint32_t* keep, uint64_t counts_out[3]) {
   int st = g_pf_counts.reserve(3 * sizeof(unsigned long long));
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipMemsetAsync(g_pf_counts.ptr, 0, 3 * sizeof(unsigned long long), stream()));
   if (n)
     prefilter_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(flat, n, grid_n, l, box,
-      mask, proxy, radius, f, scale, V, n_voids, keep,
-      g_pf_counts.as<unsigned long long>());
+      mask, proxy, radius, f, scale, V, keep, g_pf_counts.as<unsigned long long>());
   SIF_HIP(hipGetLastError());
   unsigned long long c[3];
   SIF_HIP(hipMemcpyAsync(c, g_pf_counts.ptr, sizeof(c), hipMemcpyDeviceToHost, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
   for (int i = 0; i < 3; i++)
     counts_out[i] = c[i];
+  return SIF_OK;
+}
+
+int grown_overlap(const float* cx, const float* cy, const float* cz, const float* radius,
+  uint32_t n, const VoidIndex& V, float max_r, float box, float f, float scale,
+  uint32_t* state) {
+  if (n == 0)
+    return SIF_OK;
+  grown_overlap_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(cx, cy, cz, radius, n, V,
+    max_r, box, f, scale, state);
+  SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
 

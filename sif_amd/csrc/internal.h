@@ -32,18 +32,29 @@ int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
 // scan.hip
 int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
   float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
+// The same candidates in flat order, unsorted; then the sort on its own.
+int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
+  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
+int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n);
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
 
 // grow.hip
+// The catalogue as a rung begins, on the device: one CSR cell list per radius
+// class (a class is scanned with its own largest radius, so the many small
+// voids are not scanned at the reach of the few large ones), every list over
+// the reference's 32^3 cells.
+constexpr int VOID_CLASSES = 4;
 struct VoidIndex {
   const float* cx;
   const float* cy;
   const float* cz;
   const float* r;
-  const uint32_t* cell_start;  // 32^3 + 1 entries
-  const uint32_t* items;
-  float inv_l;  // (float)32 / box
-  float max_r;
+  const uint32_t* cell_start;  // VOID_CLASSES * 32^3 + 1 entries, class-major
+  const uint32_t* items;       // void ids, grouped by (class, reference cell)
+  uint32_t n;                  // voids
+  float inv_l;                 // (float)32 / box
+  float max_r;                 // largest radius in the index
+  float class_max_r[VOID_CLASSES];  // < 0 for an empty class
 };
 int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
@@ -51,10 +62,12 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in);
 int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
   float* cx, float* cy, float* cz);
-int prefilter(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
-  float box, const uint64_t* mask, int32_t proxy, float radius, float f,
-  float scale, const VoidIndex& V, uint32_t n_voids, uint32_t* keep,
-  uint64_t counts_out[3]);
+int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
+  const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
+  const VoidIndex& V, uint32_t* keep, uint64_t counts_out[3]);
+int grown_overlap(const float* cx, const float* cy, const float* cz, const float* radius,
+  uint32_t n, const VoidIndex& V, float max_r, float box, float f, float scale,
+  uint32_t* state);
 
 // primitives.hip compaction helper used by the driver
 int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,

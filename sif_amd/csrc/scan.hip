@@ -59,13 +59,14 @@ namespace {
 DeviceBuffer g_flags, g_pos, g_count;
 }
 
-// Writes the sorted candidates to keys/flat (device, capacity `cap`) and
-// their number to *count. keys are the order-preserving integer images of
-// the depths.
-int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
+// Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)
+// and their number to *count; keys are the order-preserving integer images of
+// the depths. When the count exceeds `cap` nothing is written and the caller
+// grows the buffers and calls again.
+int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
   float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
-  if (total >= 0xffffffffull) {
-    error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^32 - 1)",
+  if (total >= 0x80000000ull) {
+    error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^31 - 1)",
       (unsigned long long)total);
     return SIF_ERR_RANGE;
   }
@@ -90,16 +91,34 @@ int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
   SIF_HIP(hipMemcpyAsync(&n, g_count.ptr, 8, hipMemcpyDeviceToHost, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
   *count = n;
-  if (n > cap) {
-    timing_end(SIF_AMD_STAGE_SCAN);
-    return SIF_OK;  // caller grows and calls again
+  if (n <= cap) {
+    cand_scatter_kernel<<<g, 256, 0, stream()>>>(delta, g_flags.as<uint32_t>(),
+      g_pos.as<uint64_t>(), total, keys, flat);
+    SIF_HIP(hipGetLastError());
   }
-  cand_scatter_kernel<<<g, 256, 0, stream()>>>(delta, g_flags.as<uint32_t>(),
-    g_pos.as<uint64_t>(), total, keys, flat);
-  SIF_HIP(hipGetLastError());
-  st = radix_sort_u32(keys, flat, (uint32_t)n, 32);
   timing_end(SIF_AMD_STAGE_SCAN);
+  return SIF_OK;
+}
+
+// Deepest first; equal depths keep their order (a stable radix sort), so a
+// list in flat order comes out ordered by (depth, flat index). Bit 31 of a
+// flat value rides along untouched.
+int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n) {
+  if (n == 0)
+    return SIF_OK;
+  timing_begin(SIF_AMD_STAGE_SORT);
+  const int st = radix_sort_u32(keys, flat, (uint32_t)n, 32);
+  timing_end(SIF_AMD_STAGE_SORT);
   return st;
+}
+
+// The sorted candidates (sif__finder_scan_candidates + sort_candidates).
+int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
+  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
+  int st = scan_candidates_flat(delta, mask, total, t, keys, flat, cap, count);
+  if (st != SIF_OK || *count > cap)
+    return st;
+  return sort_candidates(keys, flat, *count);
 }
 
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out) {
