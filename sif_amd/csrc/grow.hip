@@ -100,40 +100,46 @@ __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
 // the periodic distance but outside the reference's scan (overlap state 2)
 // gets bit 31 of its flat index set: the host re-tests it against the whole
 // catalogue.
-__global__ void prefilter_kernel(uint32_t* __restrict__ flat, uint32_t n,
-  uint32_t grid_n, float l, float box, const uint64_t* __restrict__ mask,
+__global__ void __launch_bounds__(256) prefilter_kernel(uint32_t* __restrict__ flat,
+  uint32_t n, uint32_t grid_n, float l, float box, const uint64_t* __restrict__ mask,
   int32_t proxy, float radius, float f, float scale, VoidIndex V,
   uint32_t* __restrict__ keep, unsigned long long* __restrict__ counts) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n)
-    return;
-  const uint32_t node = flat[k];
   uint32_t why = 0;  // 0 kept, 1 masked, 2 proxy, 3 overlap
-  bool edge = false;
-  if (mask_bit(mask, node)) {
-    why = 1;
-  } else {
-    const uint32_t ix = node / (grid_n * grid_n);
-    const uint32_t iy = (node / grid_n) % grid_n;
-    const uint32_t iz = node % grid_n;
-    if (proxy > 0 && overlap_quick(mask, grid_n, ix, iy, iz, (uint32_t)proxy)) {
-      why = 2;
-    } else if (V.n) {
-      OverlapQuery q;
-      overlap_query(q, (float)ix * l, (float)iy * l, (float)iz * l, radius, V.max_r, box,
-        f, scale, V.inv_l);
-      const int s = overlap_state(V, q);
-      if (s == 1)
-        why = 3;
-      else
-        edge = s == 2;
+  if (k < n) {
+    const uint32_t node = flat[k];
+    bool edge = false;
+    if (mask_bit(mask, node)) {
+      why = 1;
+    } else {
+      const uint32_t ix = node / (grid_n * grid_n);
+      const uint32_t iy = (node / grid_n) % grid_n;
+      const uint32_t iz = node % grid_n;
+      if (proxy > 0 && overlap_quick(mask, grid_n, ix, iy, iz, (uint32_t)proxy)) {
+        why = 2;
+      } else if (V.n) {
+        OverlapQuery q;
+        overlap_query(q, (float)ix * l, (float)iy * l, (float)iz * l, radius, V.max_r, box,
+          f, scale, V.inv_l);
+        const int s = overlap_state(V, q);
+        if (s == 1)
+          why = 3;
+        else
+          edge = s == 2;
+      }
     }
+    keep[k] = why == 0 ? 1u : 0u;
+    if (edge)
+      flat[k] = node | 0x80000000u;
   }
-  keep[k] = why == 0 ? 1u : 0u;
-  if (edge)
-    flat[k] = node | 0x80000000u;
-  if (why)
-    atomicAdd(&counts[why - 1], 1ull);
+  // refusal counts, one atomic per wavefront and reason (same-address
+  // atomics per candidate serialise at the L2)
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t r = 1; r <= 3; r++) {
+    const uint64_t votes = __ballot(why == r);
+    if (lane == 0 && votes)
+      atomicAdd(&counts[r - 1], (unsigned long long)__popcll(votes));
+  }
 }
 
 // The overlap state of each grown sphere (radius >= 0) against the catalogue

@@ -14,17 +14,6 @@
 
 namespace sif_amd {
 
-__global__ void __launch_bounds__(256) cand_flag_kernel(const float* __restrict__ delta,
-  const uint64_t* __restrict__ mask, uint64_t total, float t,
-  uint32_t* __restrict__ flags) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += stride) {
-    const bool masked = mask && ((mask[i >> 6] >> (i & 63)) & 1ull);
-    flags[i] = (delta[i] <= t && !masked) ? 1u : 0u;
-  }
-}
-
 __device__ __forceinline__ uint32_t float_key(float f) {
   const uint32_t b = __float_as_uint(f);
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
@@ -34,17 +23,90 @@ __device__ __forceinline__ float key_float(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-__global__ void cand_scatter_kernel(const float* __restrict__ delta,
-  const uint32_t* __restrict__ flags, const uint64_t* __restrict__ pos,
-  uint64_t total, uint32_t* __restrict__ keys, uint32_t* __restrict__ flat) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += stride) {
-    if (flags[i]) {
-      const uint64_t p = pos[i];
-      keys[p] = float_key(delta[i]);
-      flat[p] = (uint32_t)i;
+// The scan is a two-pass compaction in tiles of 4096 consecutive cells: each
+// thread owns 16 consecutive cells (four float4 loads and one 16-bit slice of
+// the mask), a tile's count and then its in-tile offsets come from a block
+// scan, and candidates are written in flat order. delta is read twice and
+// nothing else of the grid's size is written.
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 16;
+constexpr uint64_t SC_TILE = (uint64_t)SC_THREADS * SC_ITEMS;
+
+// Bit j of the result: cell first + j is a candidate (delta <= t, mask clear).
+__device__ __forceinline__ uint32_t cand_bits(const float* __restrict__ delta,
+  const uint64_t* __restrict__ mask, uint64_t total, float t, uint64_t first) {
+  uint32_t bits = 0;
+  if (first + SC_ITEMS <= total) {
+    const float4* d4 = reinterpret_cast<const float4*>(delta + first);
+    for (int q = 0; q < SC_ITEMS / 4; q++) {
+      const float4 v = d4[q];
+      bits |= (uint32_t)(v.x <= t) << (4 * q);
+      bits |= (uint32_t)(v.y <= t) << (4 * q + 1);
+      bits |= (uint32_t)(v.z <= t) << (4 * q + 2);
+      bits |= (uint32_t)(v.w <= t) << (4 * q + 3);
     }
+    if (mask)
+      bits &= ~(uint32_t)(mask[first >> 6] >> (first & 63)) & 0xffffu;
+  } else {
+    for (int j = 0; j < SC_ITEMS; j++) {
+      const uint64_t i = first + j;
+      if (i < total && delta[i] <= t && !(mask && ((mask[i >> 6] >> (i & 63)) & 1ull)))
+        bits |= 1u << j;
+    }
+  }
+  return bits;
+}
+
+// Block-wide exclusive scan of one count per thread.
+__device__ __forceinline__ uint32_t sc_block_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wave_sums[SC_THREADS / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off)
+      x += y;
+  }
+  if (lane == 63)
+    wave_sums[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int w = 0; w < SC_THREADS / 64; w++) {
+    if (w < wave)
+      before += wave_sums[w];
+    all += wave_sums[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) cand_count_kernel(
+  const float* __restrict__ delta, const uint64_t* __restrict__ mask, uint64_t total,
+  float t, uint64_t* __restrict__ tile_counts) {
+  const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+  const uint32_t c = (uint32_t)__popc(cand_bits(delta, mask, total, t, first));
+  uint32_t all;
+  sc_block_scan(c, &all);
+  if (threadIdx.x == 0)
+    tile_counts[blockIdx.x] = all;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) cand_write_kernel(
+  const float* __restrict__ delta, const uint64_t* __restrict__ mask, uint64_t total,
+  float t, const uint64_t* __restrict__ tile_offsets, uint32_t* __restrict__ keys,
+  uint32_t* __restrict__ flat) {
+  const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+  uint32_t bits = cand_bits(delta, mask, total, t, first);
+  uint32_t all;
+  uint64_t pos = tile_offsets[blockIdx.x] + sc_block_scan((uint32_t)__popc(bits), &all);
+  while (bits) {
+    const int j = __ffs(bits) - 1;
+    bits &= bits - 1;
+    const uint64_t i = first + j;
+    keys[pos] = float_key(delta[i]);
+    flat[pos] = (uint32_t)i;
+    pos++;
   }
 }
 
@@ -56,7 +118,7 @@ __global__ void cand_unkey_kernel(const uint32_t* __restrict__ keys, uint64_t n,
 }
 
 namespace {
-DeviceBuffer g_flags, g_pos, g_count;
+DeviceBuffer g_tiles, g_tile_pos, g_count;
 }
 
 // Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)
@@ -70,20 +132,26 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
       (unsigned long long)total);
     return SIF_ERR_RANGE;
   }
-  int st = g_flags.reserve(total * 4);
+  if (reinterpret_cast<uintptr_t>(delta) % 16 != 0) {
+    error_set(SIF_ERR_INVALID, "scan: the grid must be 16-byte aligned");
+    return SIF_ERR_INVALID;
+  }
+  const uint64_t tiles = (total + SC_TILE - 1) / SC_TILE;
+  int st = g_tiles.reserve((tiles ? tiles : 1) * 8);
   if (st == SIF_OK)
-    st = g_pos.reserve(total * 8);
+    st = g_tile_pos.reserve((tiles ? tiles : 1) * 8);
   if (st == SIF_OK)
     st = g_count.reserve(8);
   if (st != SIF_OK)
     return st;
+  *count = 0;
+  if (total == 0)
+    return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SCAN);
-  unsigned g = grid_for(total, 256);
-  if (g > 8192)
-    g = 8192;
-  cand_flag_kernel<<<g, 256, 0, stream()>>>(delta, mask, total, t, g_flags.as<uint32_t>());
+  cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
+    g_tiles.as<uint64_t>());
   SIF_HIP(hipGetLastError());
-  st = exclusive_scan_u32(g_flags.as<uint32_t>(), total, g_pos.as<uint64_t>(),
+  st = exclusive_scan_u64(g_tiles.as<uint64_t>(), tiles, g_tile_pos.as<uint64_t>(),
     g_count.as<uint64_t>());
   if (st != SIF_OK)
     return st;
@@ -91,9 +159,9 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
   SIF_HIP(hipMemcpyAsync(&n, g_count.ptr, 8, hipMemcpyDeviceToHost, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
   *count = n;
-  if (n <= cap) {
-    cand_scatter_kernel<<<g, 256, 0, stream()>>>(delta, g_flags.as<uint32_t>(),
-      g_pos.as<uint64_t>(), total, keys, flat);
+  if (n <= cap && n > 0) {
+    cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
+      g_tile_pos.as<uint64_t>(), keys, flat);
     SIF_HIP(hipGetLastError());
   }
   timing_end(SIF_AMD_STAGE_SCAN);
