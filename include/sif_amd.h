@@ -30,7 +30,8 @@ SIF_API int sif_amd_memcpy_d2h(void* dst, const void* d_src, uint64_t bytes);
 SIF_API int sif_amd_memset(void* d_dst, int value, uint64_t bytes);
 SIF_API int sif_amd_synchronize(void);
 
-/* Per-stage device time (HIP events on the library stream). Stage order:
+/* Per-stage device time (HIP events on the library stream, recorded without
+ * synchronising and resolved when read). Stage order:
  * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit,
  * retest, sort. commit and retest are host time: the ordered accept loop,
  * and the re-tests that assemble each growth batch. */
@@ -38,6 +39,9 @@ SIF_API int sif_amd_synchronize(void);
 SIF_API int sif_amd_timing_enable(int on);
 SIF_API void sif_amd_timing_reset(void);
 SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);
+/* Algorithmic bytes of the timed launches per stage, where the stage knows
+ * them (the minimal HBM traffic of the work done; 0 elsewhere). */
+SIF_API int sif_amd_timing_bytes(double* bytes, int n_stages);
 
 /* --- field --- */
 /* sif__field_check_in_box (src/structures/field.c:889-935). */

@@ -74,6 +74,7 @@ _sig("sif_amd_synchronize", i32)
 _sig("sif_amd_timing_enable", i32, i32)
 _sig("sif_amd_timing_reset", None)
 _sig("sif_amd_timing_read", i32, C.POINTER(f64), C.POINTER(u64), i32)
+_sig("sif_amd_timing_bytes", i32, C.POINTER(f64), i32)
 _sig("sif_amd_check_in_box", i32, P, P, P, u64, f32,
      C.POINTER(u64), C.POINTER(u64), C.POINTER(u64))
 _sig("sif_amd_cic_work_bytes", u64, u32)
@@ -187,7 +188,10 @@ def catalogue_to_numpy(cat_ptr):
 
 
 def timing_read():
+    """{stage: (ms, launches, algorithmic bytes)} since the last reset."""
     ms = (f64 * N_STAGES)()
     cnt = (u64 * N_STAGES)()
+    by = (f64 * N_STAGES)()
     lib.sif_amd_timing_read(ms, cnt, N_STAGES)
-    return {STAGE_NAMES[i]: (ms[i], int(cnt[i])) for i in range(N_STAGES)}
+    lib.sif_amd_timing_bytes(by, N_STAGES)
+    return {STAGE_NAMES[i]: (ms[i], int(cnt[i]), by[i]) for i in range(N_STAGES)}

@@ -235,6 +235,8 @@ int cic_deposit(const float* x, const float* y, const float* z, const float* w,
     acc, total, inv_vol, values);
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_CIC);
+  if (timing_active())  // positions (and weights) once, the grid once
+    timing_bytes(SIF_AMD_STAGE_CIC, (double)n * (w ? 16.0 : 12.0) + 4.0 * (double)total);
   return SIF_OK;
 }
 

@@ -26,6 +26,7 @@ void timing_begin(int stage);
 void timing_end(int stage);
 bool timing_active();
 void timing_add(int stage, double ms);  // host-side stages (the commit)
+void timing_bytes(int stage, double bytes);  // algorithmic bytes of a stage
 
 // Checks a HIP call; on failure records SIF_ERR_ALLOC/INVALID and returns it.
 int hip_check(hipError_t e, const char* what);

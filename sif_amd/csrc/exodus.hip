@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -188,10 +189,59 @@ struct RungVoids {
 
 struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys;
-  DeviceBuffer bflat, bcx, bcy, bcz, brad, breason, bnin, bstate;
   DeviceBuffer vcx, vcy, vcz, vr, vstart, vitems;
 };
 DeviceState g_dev;
+
+// One growth batch in flight: device buffers, pinned host copies of its
+// inputs and results, and the event that marks the results in host memory.
+struct BatchSlot {
+  DeviceBuffer d_flat, d_cx, d_cy, d_cz, d_rad, d_reason, d_nin, d_state, d_algo;
+  uint32_t* h_flat = nullptr;
+  float* h_rad = nullptr;
+  uint32_t* h_reason = nullptr;
+  uint64_t* h_nin = nullptr;
+  uint32_t* h_state = nullptr;
+  uint64_t* h_algo = nullptr;
+  std::vector<uint8_t> edge;
+  hipEvent_t done = nullptr;
+  uint32_t count = 0;
+  bool algo = false;
+};
+BatchSlot g_slots[2];
+
+int host_alloc(void** p, size_t bytes) {
+  SIF_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
+  return SIF_OK;
+}
+
+// Both slots at BATCH_MAX, once per process.
+int reserve_slots() {
+  for (BatchSlot& B : g_slots) {
+    if (B.done)
+      continue;
+    int e = B.d_flat.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_cx.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_cy.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_cz.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_rad.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_reason.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_nin.reserve(BATCH_MAX * 8);
+    if (e == SIF_OK) e = B.d_state.reserve(BATCH_MAX * 4);
+    if (e == SIF_OK) e = B.d_algo.reserve(BATCH_MAX * 8);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_flat, BATCH_MAX * 4);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_rad, BATCH_MAX * 4);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_reason, BATCH_MAX * 4);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_nin, BATCH_MAX * 8);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_state, BATCH_MAX * 4);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_algo, BATCH_MAX * 8);
+    if (e != SIF_OK)
+      return e;
+    B.edge.assign(BATCH_MAX, 0);
+    SIF_HIP(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
+  }
+  return SIF_OK;
+}
 
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
@@ -315,16 +365,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   }
 
   st = g_dev.mask.reserve(mask.words.size() * sizeof(uint64_t));
+  if (st == SIF_OK)
+    st = reserve_slots();
   if (st != SIF_OK)
     return st;
 
   std::vector<uint32_t> surv;
-  std::vector<uint32_t> bflat;
-  std::vector<float> brad;
-  std::vector<uint32_t> breason;
-  std::vector<uint64_t> bnin;
-  std::vector<uint32_t> bstate;
-  std::vector<uint8_t> bedge;
   uint64_t cand_cap = 0;
 
   for (size_t i = 0; i < R.radii.size() && st == SIF_OK; i++) {
@@ -416,16 +462,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         stream()));
     SIF_HIP(hipStreamSynchronize(stream()));
 
-    // speculative batches, sequential commit
+    // Speculative batches, sequential commit. While the host commits one
+    // batch, the GPU grows the next, assembled from the catalogue as it stood
+    // before that commit; every candidate is tested again as it is
+    // committed, so the catalogue is that of a sequential pass whatever the
+    // batching.
     uint32_t cap = BATCH_FLOOR;
-    if ((st = g_dev.bflat.reserve(BATCH_MAX * 4)) != SIF_OK) break;
-    if ((st = g_dev.bcx.reserve(BATCH_MAX * 4)) != SIF_OK) break;
-    if ((st = g_dev.bcy.reserve(BATCH_MAX * 4)) != SIF_OK) break;
-    if ((st = g_dev.bcz.reserve(BATCH_MAX * 4)) != SIF_OK) break;
-    if ((st = g_dev.brad.reserve(BATCH_MAX * 4)) != SIF_OK) break;
-    if ((st = g_dev.breason.reserve(BATCH_MAX * 4)) != SIF_OK) break;
-    if ((st = g_dev.bnin.reserve(BATCH_MAX * 8)) != SIF_OK) break;
-    if ((st = g_dev.bstate.reserve(BATCH_MAX * 4)) != SIF_OK) break;
     uint64_t k = 0;
     const uint64_t nn = n;
     auto node_of = [&](uint32_t flat, uint32_t* ix, uint32_t* iy, uint32_t* iz) {
@@ -462,83 +504,85 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         return 3;
       return 0;
     };
-    while (k < ns && st == SIF_OK) {
+    auto count_refusal = [&](int why) {
+      if (why == 1)
+        S.masked++;
+      else if (why == 2)
+        S.proxy++;
+      else
+        S.overlap++;
+    };
+    // the next `cap` survivors that pass the rung tests now
+    auto assemble = [&](BatchSlot& B) {
       const double t0 = now_ms();
-      bflat.clear();
-      bedge.clear();
-      while (k < ns && bflat.size() < cap) {
+      B.count = 0;
+      while (k < ns && B.count < cap) {
         const uint32_t flat = surv[k] & 0x7fffffffu;
         const bool edge = (surv[k] >> 31) != 0;
         k++;
         const int why = rung_tests(flat, edge);
-        if (why == 1)
-          S.masked++;
-        else if (why == 2)
-          S.proxy++;
-        else if (why == 3)
-          S.overlap++;
-        else {
-          bflat.push_back(flat);
-          bedge.push_back(edge ? 1 : 0);
+        if (why) {
+          count_refusal(why);
+        } else {
+          B.h_flat[B.count] = flat;
+          B.edge[B.count] = edge ? 1 : 0;
+          B.count++;
         }
       }
-      const uint32_t bc = (uint32_t)bflat.size();
-      if (bc == 0)
-        break;
-      const double t1 = now_ms();
       if (timing_active())
-        timing_add(SIF_AMD_STAGE_RETEST, t1 - t0);
-      SIF_HIP(hipMemcpyAsync(g_dev.bflat.ptr, bflat.data(), bc * 4, hipMemcpyHostToDevice,
+        timing_add(SIF_AMD_STAGE_RETEST, now_ms() - t0);
+    };
+    // growth and the grown-radius overlap state, results to pinned memory
+    auto launch = [&](BatchSlot& B) -> int {
+      const uint32_t bc = B.count;
+      const bool algo = timing_active();
+      SIF_HIP(hipMemcpyAsync(B.d_flat.ptr, B.h_flat, bc * 4, hipMemcpyHostToDevice, stream()));
+      int e = node_centres(B.d_flat.as<uint32_t>(), bc, n, l, B.d_cx.as<float>(),
+        B.d_cy.as<float>(), B.d_cz.as<float>());
+      if (e == SIF_OK)
+        e = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
+          B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, rmin, r_search,
+          R.threshold, vol, B.d_rad.as<float>(), B.d_reason.as<uint32_t>(),
+          B.d_nin.as<uint64_t>(), algo ? B.d_algo.as<uint64_t>() : nullptr);
+      if (e == SIF_OK)
+        e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
+          B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale, B.d_state.as<uint32_t>());
+      if (e != SIF_OK)
+        return e;
+      SIF_HIP(hipMemcpyAsync(B.h_rad, B.d_rad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
+      SIF_HIP(hipMemcpyAsync(B.h_reason, B.d_reason.ptr, bc * 4, hipMemcpyDeviceToHost,
         stream()));
-      st = node_centres(g_dev.bflat.as<uint32_t>(), bc, n, l, g_dev.bcx.as<float>(),
-        g_dev.bcy.as<float>(), g_dev.bcz.as<float>());
-      if (st == SIF_OK)
-        st = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
-          g_dev.bcx.as<float>(), g_dev.bcy.as<float>(), g_dev.bcz.as<float>(), bc, rmin,
-          r_search, R.threshold, vol, g_dev.brad.as<float>(), g_dev.breason.as<uint32_t>(),
-          g_dev.bnin.as<uint64_t>());
-      if (st == SIF_OK)
-        st = grown_overlap(g_dev.bcx.as<float>(), g_dev.bcy.as<float>(), g_dev.bcz.as<float>(),
-          g_dev.brad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
-          g_dev.bstate.as<uint32_t>());
-      if (st != SIF_OK)
-        break;
+      SIF_HIP(hipMemcpyAsync(B.h_nin, B.d_nin.ptr, bc * 8, hipMemcpyDeviceToHost, stream()));
+      SIF_HIP(hipMemcpyAsync(B.h_state, B.d_state.ptr, bc * 4, hipMemcpyDeviceToHost,
+        stream()));
+      if (algo)
+        SIF_HIP(hipMemcpyAsync(B.h_algo, B.d_algo.ptr, bc * 8, hipMemcpyDeviceToHost,
+          stream()));
+      B.algo = algo;
+      SIF_HIP(hipEventRecord(B.done, stream()));
       S.evaluated += bc;
-      brad.resize(bc);
-      breason.resize(bc);
-      bnin.resize(bc);
-      bstate.resize(bc);
-      SIF_HIP(hipMemcpyAsync(brad.data(), g_dev.brad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
-      SIF_HIP(hipMemcpyAsync(breason.data(), g_dev.breason.ptr, bc * 4, hipMemcpyDeviceToHost,
-        stream()));
-      SIF_HIP(hipMemcpyAsync(bnin.data(), g_dev.bnin.ptr, bc * 8, hipMemcpyDeviceToHost, stream()));
-      SIF_HIP(hipMemcpyAsync(bstate.data(), g_dev.bstate.ptr, bc * 4, hipMemcpyDeviceToHost,
-        stream()));
-      SIF_HIP(hipStreamSynchronize(stream()));
+      return SIF_OK;
+    };
+    // exodus_finder.c:2174-2329, in candidate order
+    auto commit = [&](BatchSlot& B, uint64_t* accepted_out) -> int {
+      SIF_HIP(hipEventSynchronize(B.done));
       const double t2 = now_ms();
-
       uint64_t accepted = 0;
-      for (uint32_t b = 0; b < bc; b++) {
-        const uint32_t flat = bflat[b];
-        if (breason[b] == 5) {
+      for (uint32_t b = 0; b < B.count; b++) {
+        const uint32_t flat = B.h_flat[b];
+        if (B.h_reason[b] == 5) {
           error_set(SIF_ERR_RANGE,
             "exodus: a search shell at rung %g holds more tracers in one histogram "
             "bin than the refinement buffer takes",
             (double)radius);
-          st = SIF_ERR_RANGE;
-          break;
+          return SIF_ERR_RANGE;
         }
-        const int why = rung_tests(flat, bedge[b] != 0);
+        const int why = rung_tests(flat, B.edge[b] != 0);
         if (why) {
-          if (why == 1)
-            S.masked++;
-          else if (why == 2)
-            S.proxy++;
-          else
-            S.overlap++;
+          count_refusal(why);
           continue;
         }
-        const float r = brad[b];
+        const float r = B.h_rad[b];
         if (r < 0.0f) {
           S.rescale_fail++;
           continue;
@@ -551,18 +595,46 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           S.proxy++;
           continue;
         }
-        if (exact_overlap(cx, cy, cz, r, bstate[b])) {
+        if (exact_overlap(cx, cy, cz, r, B.h_state[b])) {
           S.overlap++;
           continue;
         }
-        V.add(cx, cy, cz, r, bnin[b]);
+        V.add(cx, cy, cz, r, B.h_nin[b]);
         RV.add(cx, cy, cz, r);
         mark_sphere(mask, cx, cy, cz, scale * r, l);
         accepted++;
       }
       S.accepted += accepted;
-      if (timing_active())
+      *accepted_out = accepted;
+      if (timing_active()) {
         timing_add(SIF_AMD_STAGE_COMMIT, now_ms() - t2);
+        if (B.algo) {
+          double bytes = 0.0;
+          for (uint32_t b = 0; b < B.count; b++)
+            bytes += (double)B.h_algo[b];
+          timing_bytes(SIF_AMD_STAGE_GROW, bytes);
+        }
+      }
+      return SIF_OK;
+    };
+
+    BatchSlot* cur = &g_slots[0];
+    BatchSlot* nxt = &g_slots[1];
+    assemble(*cur);
+    if (cur->count)
+      st = launch(*cur);
+    while (st == SIF_OK && cur->count) {
+      nxt->count = 0;
+      if (k < ns) {
+        assemble(*nxt);
+        if (nxt->count)
+          st = launch(*nxt);
+      }
+      uint64_t accepted = 0;
+      if (st == SIF_OK)
+        st = commit(*cur, &accepted);
+      if (st != SIF_OK)
+        break;
       // exodus_finder.c:2331-2349: widen after a quiet batch, back off after
       // accepts
       if (accepted == 0) {
@@ -574,6 +646,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         if (cap < BATCH_FLOOR)
           cap = BATCH_FLOOR;
       }
+      std::swap(cur, nxt);
     }
     if (stats_out)
       stats_out[i] = S;
@@ -582,10 +655,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   if (R.restore && st == SIF_OK)
     st = fft_filter_c2r(g_dev.spec.as<float2>(), n, R.box, SIF_AMD_FILTER_NONE, 0.0f, 0.0f,
       0, 0, R.d_delta, g_dev.work.as<float2>());
-  if (st == SIF_OK)
-    SIF_HIP(hipStreamSynchronize(stream()));
+  // a failed rung may leave a batch in flight: drain the stream either way
+  const hipError_t drained = hipStreamSynchronize(stream());
   if (st != SIF_OK)
     return st;
+  SIF_HIP(drained);
 
   const uint64_t nv = V.cx.size();
   sif_catalogue_t* cat = sif_catalogue_alloc(nv ? nv : 1);

@@ -305,6 +305,8 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
     (uint64_t)nz, (uint64_t)n * nz, g_tw.fwd.as<float2>(), nullptr, nullptr);
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_FFT_FORWARD);
+  if (timing_active())  // the grid once, the spectrum once
+    timing_bytes(SIF_AMD_STAGE_FFT_FORWARD, 4.0 * (double)rows * n + 8.0 * (double)rows * nz);
   return SIF_OK;
 }
 
@@ -358,6 +360,8 @@ int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
     work, out, log2n, rows, norm, g_tw.inv.as<float2>());
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_SMOOTH);
+  if (timing_active())  // the spectrum once, the field once
+    timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + 4.0 * (double)rows * n);
   return SIF_OK;
 }
 

@@ -260,7 +260,8 @@ static int template_device(uint32_t mesh_n, float l, float rmin, float r_search)
 int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
   const float* cz, uint32_t n, float rmin, float r_search, float threshold,
-  float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in) {
+  float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
+  uint64_t* algo_bytes) {
   const float l = box / (float)mesh_n;
   int st = template_device(mesh_n, l, rmin, r_search);
   if (st != SIF_OK)
@@ -289,6 +290,7 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   P.radius = radius;
   P.reason = reason;
   P.n_in = n_in;
+  P.algo_bytes = algo_bytes;
   timing_begin(SIF_AMD_STAGE_GROW);
   const uint64_t blocks = ((uint64_t)n + GROW_WAVES - 1) / GROW_WAVES;
   grow_kernel<<<(unsigned)blocks, 64 * GROW_WAVES, 0, stream()>>>(P);

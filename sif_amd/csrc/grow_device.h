@@ -41,6 +41,10 @@ struct GrowParams {
   float* radius;
   uint32_t* reason;
   uint64_t* n_in;
+  // optional: the candidate's algorithmic bytes -- 8 per template cell whose
+  // offsets it reads, 12 per particle of the cells whose particles it must
+  // read once (the surface for the early refusal, every non-CORE cell past it)
+  uint64_t* algo_bytes;
 };
 
 struct WaveLDS {
@@ -271,6 +275,7 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
   float radius = -1.0f;
   uint32_t reason = R_OK;
   uint64_t n_in_out = 0;
+  uint64_t algo = 0;
   {
     Query q;
     q.cx = P.cx[c];
@@ -290,23 +295,30 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
       reason = R_DEGENERATE;
     } else {
       // 1. early refusal, O(surface): exodus_rescale.c:711-782
-      uint64_t within = 0;
+      uint64_t within = 0, surface = 0, shell = 0;
       for_particles(P, q, S, lane,
         [](uint32_t, float2) { return true; },
         [&](uint32_t type, uint32_t cnt) {
           if (type == CELL_CORE || type == CELL_SHELL) {
             within += cnt;
+            shell += type == CELL_SHELL ? cnt : 0u;
             return true;
           }
           return false;
         },
-        [&](float d2, uint32_t) { within += d2 <= rs2 ? 1u : 0u; });
+        [&](float d2, uint32_t) {
+          within += d2 <= rs2 ? 1u : 0u;
+          surface++;
+        });
       within = uni(wave_sum_u64(within));
+      surface = uni(wave_sum_u64(surface));
+      shell = uni(wave_sum_u64(shell));
       if (within == 0) {
         reason = R_EMPTY;
       } else if (__fsub_rn(__fdiv_rn((float)within, expected), 1.0f) <= P.threshold) {
         reason = R_BEYOND;
       }
+      algo = 8ull * P.tcount + 12ull * (surface + (reason == R_OK ? shell : 0ull));
     }
 
     if (reason == R_OK) {
@@ -514,6 +526,8 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
     P.reason[c] = reason;
     if (P.n_in)
       P.n_in[c] = n_in_out;
+    if (P.algo_bytes)
+      P.algo_bytes[c] = algo;
   }
 }
 

@@ -59,7 +59,8 @@ struct VoidIndex {
 int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
   const float* cz, uint32_t n, float rmin, float r_search, float threshold,
-  float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in);
+  float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
+  uint64_t* algo_bytes = nullptr);
 int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
   float* cx, float* cy, float* cz);
 int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,

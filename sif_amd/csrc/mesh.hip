@@ -172,6 +172,11 @@ int mesh_build(const float* x, const float* y, const float* z, const float* w,
     *total_weight = (double)n;
   }
   timing_end(SIF_AMD_STAGE_MESH);
+  if (timing_active()) {  // positions in and out once, offsets, indices
+    const double per = w ? 16.0 : 12.0;
+    timing_bytes(SIF_AMD_STAGE_MESH, 2.0 * per * (double)n + 8.0 * (double)(cells + 1) +
+      (orig ? 8.0 * (double)n : 0.0) + (cw ? 4.0 * (double)cells : 0.0));
+  }
   return SIF_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "sif/core/system.h"
@@ -24,13 +25,22 @@ bool g_init = false;
 int g_device = 0;
 hipStream_t g_stream = nullptr;
 
+// Stage timing: a begin and an end event per bracketed launch, recorded on
+// the library stream and resolved only when read (or when the pending list
+// fills), so timing adds no synchronisation to the run it measures.
 bool g_timing = false;
-hipEvent_t g_ev_start[SIF_AMD_N_STAGES];
-hipEvent_t g_ev_stop[SIF_AMD_N_STAGES];
-bool g_ev_made = false;
+struct Pending {
+  int stage;
+  hipEvent_t start, stop;
+};
+std::vector<hipEvent_t> g_free_events;
+std::vector<Pending> g_pending;
+hipEvent_t g_open_start[SIF_AMD_N_STAGES];
 bool g_open[SIF_AMD_N_STAGES];
 double g_ms[SIF_AMD_N_STAGES];
+double g_bytes[SIF_AMD_N_STAGES];
 uint64_t g_count[SIF_AMD_N_STAGES];
+constexpr size_t PENDING_MAX = 8192;
 }  // namespace
 
 void error_set(int status, const char* fmt, ...) {
@@ -98,23 +108,59 @@ void DeviceBuffer::release() {
   bytes = 0;
 }
 
-void timing_begin(int s) {
-  if (!g_timing || !g_ev_made)
+static hipEvent_t take_event() {
+  if (!g_free_events.empty()) {
+    hipEvent_t e = g_free_events.back();
+    g_free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess)
+    return nullptr;
+  return e;
+}
+
+// Waits for the recorded events and adds their intervals to the stages.
+static void timing_resolve() {
+  if (g_pending.empty())
     return;
-  (void)hipEventRecord(g_ev_start[s], g_stream);
+  (void)hipEventSynchronize(g_pending.back().stop);
+  for (const Pending& p : g_pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      g_ms[p.stage] += ms;
+      g_count[p.stage]++;
+    }
+    g_free_events.push_back(p.start);
+    g_free_events.push_back(p.stop);
+  }
+  g_pending.clear();
+}
+
+void timing_begin(int s) {
+  if (!g_timing)
+    return;
+  hipEvent_t e = take_event();
+  if (!e)
+    return;
+  (void)hipEventRecord(e, g_stream);
+  if (g_open[s])
+    g_free_events.push_back(g_open_start[s]);
+  g_open_start[s] = e;
   g_open[s] = true;
 }
 
 void timing_end(int s) {
-  if (!g_timing || !g_ev_made || !g_open[s])
+  if (!g_timing || !g_open[s])
     return;
-  (void)hipEventRecord(g_ev_stop[s], g_stream);
-  (void)hipEventSynchronize(g_ev_stop[s]);
-  float ms = 0.0f;
-  (void)hipEventElapsedTime(&ms, g_ev_start[s], g_ev_stop[s]);
-  g_ms[s] += ms;
-  g_count[s]++;
+  hipEvent_t e = take_event();
+  if (!e)
+    return;
+  (void)hipEventRecord(e, g_stream);
+  g_pending.push_back(Pending{s, g_open_start[s], e});
   g_open[s] = false;
+  if (g_pending.size() >= PENDING_MAX)
+    timing_resolve();
 }
 
 bool timing_active() { return g_timing; }
@@ -123,6 +169,8 @@ void timing_add(int s, double ms) {
   g_ms[s] += ms;
   g_count[s]++;
 }
+
+void timing_bytes(int s, double bytes) { g_bytes[s] += bytes; }
 
 }  // namespace sif_amd
 
@@ -148,14 +196,16 @@ void sif_system_finalise(void) {
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!g_init)
     return;
-  if (g_ev_made) {
-    for (int s = 0; s < SIF_AMD_N_STAGES; s++) {
-      (void)hipEventDestroy(g_ev_start[s]);
-      (void)hipEventDestroy(g_ev_stop[s]);
-    }
-    g_ev_made = false;
-  }
   (void)hipStreamSynchronize(g_stream);
+  timing_resolve();
+  for (int s = 0; s < SIF_AMD_N_STAGES; s++)
+    if (g_open[s]) {
+      g_free_events.push_back(g_open_start[s]);
+      g_open[s] = false;
+    }
+  for (hipEvent_t e : g_free_events)
+    (void)hipEventDestroy(e);
+  g_free_events.clear();
   (void)hipStreamDestroy(g_stream);
   g_stream = nullptr;
   g_init = false;
@@ -223,32 +273,36 @@ int sif_amd_timing_enable(int on) {
   int st = ensure_init();
   if (st != SIF_OK)
     return st;
-  if (on && !g_ev_made) {
-    for (int s = 0; s < SIF_AMD_N_STAGES; s++) {
-      SIF_HIP(hipEventCreate(&g_ev_start[s]));
-      SIF_HIP(hipEventCreate(&g_ev_stop[s]));
-      g_open[s] = false;
-    }
-    g_ev_made = true;
-  }
+  if (!on)
+    timing_resolve();
   g_timing = on != 0;
   return SIF_OK;
 }
 
 void sif_amd_timing_reset(void) {
+  timing_resolve();
   for (int s = 0; s < SIF_AMD_N_STAGES; s++) {
     g_ms[s] = 0.0;
+    g_bytes[s] = 0.0;
     g_count[s] = 0;
   }
 }
 
 int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages) {
+  timing_resolve();
   for (int s = 0; s < n_stages && s < SIF_AMD_N_STAGES; s++) {
     if (ms)
       ms[s] = g_ms[s];
     if (launches)
       launches[s] = g_count[s];
   }
+  return SIF_AMD_N_STAGES;
+}
+
+int sif_amd_timing_bytes(double* bytes, int n_stages) {
+  timing_resolve();
+  for (int s = 0; s < n_stages && s < SIF_AMD_N_STAGES; s++)
+    bytes[s] = g_bytes[s];
   return SIF_AMD_N_STAGES;
 }
 

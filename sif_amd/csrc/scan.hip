@@ -165,6 +165,9 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
     SIF_HIP(hipGetLastError());
   }
   timing_end(SIF_AMD_STAGE_SCAN);
+  if (timing_active())  // the field and the mask once, the candidates once
+    timing_bytes(SIF_AMD_STAGE_SCAN, 4.0 * (double)total + (double)total / 8.0 +
+      (n <= cap ? 8.0 * (double)n : 0.0));
   return SIF_OK;
 }
 

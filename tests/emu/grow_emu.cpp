@@ -49,6 +49,7 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
     bounds[i] = make_float2(tmin[i], tmax[i]);
   }
   sif_amd::GrowParams P;
+  P.algo_bytes = nullptr;
   P.offsets = offsets;
   P.mx = mx;
   P.my = my;
