@@ -47,9 +47,16 @@ struct GrowParams {
   uint64_t* algo_bytes;
 };
 
+// Annulus distances a wave keeps in LDS: a candidate whose annulus fits is
+// resolved from LDS after one pass over its cells.
+constexpr uint32_t CACHE_CAP = 2048;
+
 struct WaveLDS {
   uint32_t bins[MAX_BINS];
-  float buf[REFINE_CAP];
+  union {
+    float buf[REFINE_CAP];   // a window's distances (the pass-per-window walk)
+    float cache[CACHE_CAP];  // every annulus distance (the cached walk)
+  };
   uint32_t t_start[64];
   uint32_t t_scan[64];
   float t_ex[64], t_ey[64], t_ez[64];
@@ -216,7 +223,8 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
   }
 }
 
-// Ascending bitonic sort of buf[0..n) (n <= REFINE_CAP) by one wave.
+// Ascending bitonic sort of buf[0..n) by one wave; the next power of two
+// above n must fit the buffer.
 __device__ void wave_bitonic(float* buf, uint32_t n, int lane) {
   uint32_t P2 = 1;
   while (P2 < n)
@@ -276,6 +284,8 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
   uint32_t reason = R_OK;
   uint64_t n_in_out = 0;
   uint64_t algo = 0;
+  bool cached = false;
+  uint64_t n_annulus = 0, n_core_cached = 0;
   {
     Query q;
     q.cx = P.cx[c];
@@ -294,31 +304,54 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
     } else if (!(span > 0.0f)) {
       reason = R_DEGENERATE;
     } else {
-      // 1. early refusal, O(surface): exodus_rescale.c:711-782
-      uint64_t within = 0, surface = 0, shell = 0;
+      // 1. early refusal, O(surface): exodus_rescale.c:711-782. The pass also
+      // reads the SHELL cells (counted whole by the reference) and keeps every
+      // annulus distance in LDS while they fit.
+      uint64_t within = 0, surface = 0, shell = 0, core = 0;
+      S.counter = 0;  // every lane stores the same zero
+      wave_sync();
       for_particles(P, q, S, lane,
         [](uint32_t, float2) { return true; },
         [&](uint32_t type, uint32_t cnt) {
-          if (type == CELL_CORE || type == CELL_SHELL) {
+          if (type == CELL_CORE) {
             within += cnt;
-            shell += type == CELL_SHELL ? cnt : 0u;
+            core += cnt;
             return true;
           }
           return false;
         },
-        [&](float d2, uint32_t) {
-          within += d2 <= rs2 ? 1u : 0u;
-          surface++;
+        [&](float d2, uint32_t type) {
+          bool annulus;
+          if (type == CELL_SHELL) {
+            within++;
+            shell++;
+            annulus = true;
+          } else {
+            within += d2 <= rs2 ? 1u : 0u;
+            surface++;
+            core += d2 <= rc2 ? 1u : 0u;
+            annulus = d2 > rc2 && d2 <= rs2;
+          }
+          if (annulus) {
+            const uint32_t slot = atomicAdd(&S.counter, 1u);
+            if (slot < CACHE_CAP)
+              S.cache[slot] = d2;
+          }
         });
+      wave_sync();
       within = uni(wave_sum_u64(within));
       surface = uni(wave_sum_u64(surface));
       shell = uni(wave_sum_u64(shell));
+      core = uni(wave_sum_u64(core));
+      n_annulus = uni(S.counter);
       if (within == 0) {
         reason = R_EMPTY;
       } else if (__fsub_rn(__fdiv_rn((float)within, expected), 1.0f) <= P.threshold) {
         reason = R_BEYOND;
       }
       algo = 8ull * P.tcount + 12ull * (surface + (reason == R_OK ? shell : 0ull));
+      cached = n_annulus <= CACHE_CAP;
+      n_core_cached = core;
     }
 
     if (reason == R_OK) {
@@ -327,7 +360,14 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
         S.bins[b] = 0;
       wave_sync();
       uint64_t n_core = 0, n_shell = 0;
-      for_particles(P, q, S, lane,
+      if (cached) {
+        for (uint32_t k = lane; k < n_annulus; k += 64)
+          atomicAdd(&S.bins[bin_of(S.cache[k], rc2, inv_w, nb)], 1u);
+        if (lane == 0) {
+          n_core = n_core_cached;
+          n_shell = n_annulus;
+        }
+      } else for_particles(P, q, S, lane,
         [](uint32_t type, float2) { return true; },
         [&](uint32_t type, uint32_t cnt) {
           if (type == CELL_CORE) {
@@ -385,6 +425,8 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
           above += cnt;
         }
         wave_sync();
+        if (cached)  // ascending, so a window's bins are one run of the cache
+          wave_bitonic(S.cache, (uint32_t)n_annulus, lane);
 
         // 4. walk: next flagged bin below `limit`, resolve a window, repeat.
         int32_t limit = (int32_t)nb;
@@ -422,6 +464,33 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
             hi = __fmul_rn(rs2, __fadd_rn(1.0f, 1e-6f));
           const float lo = __fadd_rn(rc2, __fmul_rn((float)blo, bin_w));
           const uint32_t win_total = uni(__shfl(wscan, W - 1, 64));
+
+          if (cached) {
+            // the window is the run of the sorted cache that ends where the
+            // bins above `best` begin; walked outside in as below
+            const uint32_t seg0 = (uint32_t)(n_annulus - abv - win_total);
+            int found = -1;
+            for (uint32_t k = lane; k < win_total; k += 64) {
+              const float d2 = S.cache[seg0 + k];
+              if (d2 == 0.0f)
+                continue;
+              const uint64_t n_in = cur_n - win_total + k;
+              const float rn = (float)n_in;
+              const float lhs = __fmul_rn(rn, rn);
+              const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
+              if (lhs <= rhs)
+                found = (int)k;
+            }
+            found = uni(wave_max_i32(found));
+            if (found >= 0) {
+              radius = __builtin_sqrtf(unif(S.cache[seg0 + found]));
+              n_in_out = cur_n - win_total + (uint64_t)found;
+            }
+            if (radius < 0.0f)
+              limit = blo;
+            wave_sync();
+            continue;
+          }
 
           // sub-ranges of squared distance: 1 normally, SUB_BINS when split
           const int n_sub = split ? SUB_BINS : 1;
