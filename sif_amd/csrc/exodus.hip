@@ -24,6 +24,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -188,7 +190,7 @@ struct RungVoids {
 };
 
 struct DeviceState {
-  DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys;
+  DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
   DeviceBuffer vcx, vcy, vcz, vr, vstart, vitems;
 };
 DeviceState g_dev;
@@ -328,6 +330,44 @@ int exodus_prepare_spectrum(const float* d_delta, uint32_t n) {
   return fft_r2c(d_delta, n, g_dev.spec.as<float2>(), g_dev.work.as<float2>());
 }
 
+// The top-hat LUT of every rung (fft.c:535-566, float libm on the host, as
+// the reference builds it), built by host threads while the device runs the
+// forward transform, and uploaded once; plus the window axis.
+static int exodus_prepare_luts(uint32_t n, float box, const std::vector<float>& radii,
+  sif_option opt) {
+  const uint64_t lsz = filter_lut_size(n);
+  std::vector<float> all(radii.size() * lsz);
+  const size_t workers = std::min<size_t>(
+    radii.size(), std::max<unsigned>(1u, std::min(std::thread::hardware_concurrency(), 16u)));
+  std::vector<std::thread> pool;
+  for (size_t w = 0; w < workers; w++)
+    pool.emplace_back([&, w]() {
+      std::vector<float> lut;
+      for (size_t i = w; i < radii.size(); i += workers) {
+        build_filter_lut(SIF_AMD_FILTER_TOP_HAT, radii[i], 0.0f, box, n, lut);
+        std::copy(lut.begin(), lut.end(), all.begin() + i * lsz);
+      }
+    });
+  for (std::thread& t : pool)
+    t.join();
+  int st = g_dev.luts.reserve(all.size() * sizeof(float));
+  if (st == SIF_OK)
+    st = g_dev.axis.reserve(n * sizeof(double));
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemcpyAsync(g_dev.luts.ptr, all.data(), all.size() * sizeof(float),
+    hipMemcpyHostToDevice, stream()));
+  const int order = window_order_of(opt);
+  if (order > 0) {
+    std::vector<double> axis;
+    build_window_axis(n, order, (opt & SIF_FINDER_ALIASING_WINDOW) ? 1 : 0, axis);
+    SIF_HIP(hipMemcpyAsync(g_dev.axis.ptr, axis.data(), n * sizeof(double),
+      hipMemcpyHostToDevice, stream()));
+  }
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
 int exodus_smooth(uint32_t n, float box, float radius, sif_option opt, float* d_out) {
   return fft_filter_c2r(g_dev.spec.as<float2>(), n, box, SIF_AMD_FILTER_TOP_HAT, radius,
     0.0f, window_order_of(opt), (opt & SIF_FINDER_ALIASING_WINDOW) ? 1 : 0, d_out,
@@ -347,8 +387,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   const float sf = sif__finder_search_factor(R.opt);
 
   int st = exodus_prepare_spectrum(R.d_delta, n);
+  if (st == SIF_OK)
+    st = exodus_prepare_luts(n, R.box, R.radii, R.opt);
   if (st != SIF_OK)
     return st;
+  const uint64_t lut_size = filter_lut_size(n);
+  const bool windowed = window_order_of(R.opt) > 0;
 
   Mask mask;
   mask.init(n);
@@ -379,7 +423,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     memset(&S, 0, sizeof(S));
     S.radius = radius;
 
-    st = exodus_smooth(n, R.box, radius, R.opt, R.d_delta);
+    st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>() + i * lut_size,
+      windowed ? g_dev.axis.as<double>() : nullptr, R.d_delta, g_dev.work.as<float2>());
     if (st != SIF_OK)
       break;
     if (mask.dirty) {

@@ -5,10 +5,18 @@
 // and the filter/window multiply of fft.c:698-788 of sif v0.5.0.
 //
 // Layout is FFTW's: real field [n][n][n] (z fastest), spectrum
-// [n][n][n/2+1] complex. Each axis is a batch of 1D Stockham radix-2
-// transforms staged in LDS (ping-pong buffers, natural-order output, no bit
-// reversal). The x and y passes load tiles of consecutive kz columns so that
-// every global access is a contiguous run; the z pass works on whole rows.
+// [n][n][n/2+1] complex. Each axis is one pass over memory: a workgroup loads
+// a tile of 8192 complex values -- TK = 8192/n pencils of length n -- into LDS
+// with coalesced reads (consecutive kz columns for the x and y passes, whole
+// rows for the z pass), transforms every pencil, and writes the tile back the
+// same way. The transform is a Stockham autosort FFT in registers (the
+// formulation of Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC 2008): each of the n/8 threads of a
+// pencil holds eight elements, does one radix-8 butterfly per stage (a
+// radix-2 or radix-4 first stage when log2 n is not a multiple of 3), and
+// exchanges through LDS between stages -- three stages for n = 512. Twiddles
+// come from a table of exp(-2 pi i m / n) computed in double on the host.
+//
 // The per-mode smoothing factor (float)(LUT[k^2] * w(kx) w(ky) w(kz)) of
 // fft.c:753-780 is applied as the spectrum is first read by the inverse x
 // pass, so filtering costs no extra pass over memory. Power-of-two n from 16
@@ -24,125 +32,190 @@
 namespace sif_amd {
 
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_TILE = 4096;  // complex elements per block (per buffer)
+constexpr int FFT_TILE = 8192;  // complex values per workgroup tile
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+// a * (-i) forward, a * (+i) inverse
+template <bool INV>
+__device__ __forceinline__ float2 rot(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
 
-// P pencils of length n = 2^log2n in buffer a (pencil-major). Returns the
-// buffer holding the natural-order result. tw[m] = exp(sign 2 pi i m / n).
-__device__ float2* stockham(float2* a, float2* b, int log2n, int P,
-  const float2* __restrict__ tw) {
-  const int n = 1 << log2n;
-  const int half = n >> 1;
-  const int total = P * half;
-  for (int ls = 0; ls < log2n; ls++) {
-    const int Ns = 1 << ls;
-    for (int t = threadIdx.x; t < total; t += FFT_THREADS) {
-      const int p = t >> (log2n - 1);
-      const int j = t & (half - 1);
-      const int k = j & (Ns - 1);
-      const float2 w = tw[k << (log2n - 1 - ls)];
-      const float2 x0 = a[p * n + j];
-      const float2 x1 = cmul(a[p * n + j + half], w);
-      const int dst = p * n + ((j >> ls) << (ls + 1)) + k;
-      b[dst] = make_float2(x0.x + x1.x, x0.y + x1.y);
-      b[dst + Ns] = make_float2(x0.x - x1.x, x0.y - x1.y);
+// In-register DFTs of size 2, 4 and 8, exp(-+2 pi i / R) for INV false/true,
+// decimation in frequency with natural-order outputs.
+template <bool INV>
+__device__ __forceinline__ void dft2(float2* v) {
+  const float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <bool INV>
+__device__ __forceinline__ void dft4(float2* v) {
+  const float2 a0 = cadd(v[0], v[2]), a2 = csub(v[0], v[2]);
+  const float2 a1 = cadd(v[1], v[3]), a3 = rot<INV>(csub(v[1], v[3]));
+  v[0] = cadd(a0, a1);
+  v[2] = csub(a0, a1);
+  v[1] = cadd(a2, a3);
+  v[3] = csub(a2, a3);
+}
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* v) {
+  const float c = 0.70710678118654752440f;
+  const float2 a0 = cadd(v[0], v[4]), a4 = csub(v[0], v[4]);
+  const float2 a1 = cadd(v[1], v[5]), d5 = csub(v[1], v[5]);
+  const float2 a2 = cadd(v[2], v[6]), d6 = csub(v[2], v[6]);
+  const float2 a3 = cadd(v[3], v[7]), d7 = csub(v[3], v[7]);
+  // the odd half times w8^1, w8^2, w8^3
+  const float2 a5 = INV ? make_float2(c * (d5.x - d5.y), c * (d5.x + d5.y))
+                        : make_float2(c * (d5.x + d5.y), c * (d5.y - d5.x));
+  const float2 a6 = rot<INV>(d6);
+  const float2 a7 = INV ? make_float2(-c * (d7.x + d7.y), c * (d7.x - d7.y))
+                        : make_float2(c * (d7.y - d7.x), -c * (d7.x + d7.y));
+  const float2 b0 = cadd(a0, a2), b2 = csub(a0, a2);
+  const float2 b1 = cadd(a1, a3), b3 = rot<INV>(csub(a1, a3));
+  const float2 b4 = cadd(a4, a6), b6 = csub(a4, a6);
+  const float2 b5 = cadd(a5, a7), b7 = rot<INV>(csub(a5, a7));
+  v[0] = cadd(b0, b1);
+  v[4] = csub(b0, b1);
+  v[2] = cadd(b2, b3);
+  v[6] = csub(b2, b3);
+  v[1] = cadd(b4, b5);
+  v[5] = csub(b4, b5);
+  v[3] = cadd(b6, b7);
+  v[7] = csub(b6, b7);
+}
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v) {
+  if (R == 2)
+    dft2<INV>(v);
+  else if (R == 4)
+    dft4<INV>(v);
+  else
+    dft8<INV>(v);
+}
+
+// One Stockham stage of radix R over a pencil X (in LDS) of length N, after
+// stages whose radices multiply to NS. Thread t of the pencil's N/8 threads
+// does 8/R butterflies j = t + b N/8: inputs X[j + r N/R], twiddles
+// w^(r (j mod NS) N / (NS R)), outputs Y[(j / NS) NS R + j mod NS + r NS].
+template <int N, int R, int NS, bool INV>
+__device__ __forceinline__ void stage_read(const float2* X, int t, float2* v) {
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int b = 0; b < 8 / R; b++)
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      v[b * R + r] = X[t + b * T + r * (N / R)];
+}
+template <int N, int R, int NS, bool INV>
+__device__ __forceinline__ void stage_write(float2* Y, int t, float2* v, const float2* tw) {
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int b = 0; b < 8 / R; b++) {
+    const int j = t + b * T;
+    const int jm = j & (NS - 1);
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        float2 w = tw[jm * r * (N / (NS * R))];
+        if (INV)
+          w.y = -w.y;
+        v[b * R + r] = cmul(v[b * R + r], w);
+      }
     }
-    __syncthreads();
-    float2* t = a;
-    a = b;
-    b = t;
-  }
-  return a;
-}
-
-// --- z pass: whole rows ---
-
-__global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __restrict__ in,
-  float2* __restrict__ spec, int log2n, uint64_t rows, const float2* __restrict__ tw_g) {
-  __shared__ float2 A[FFT_TILE], B[FFT_TILE];
-  __shared__ float2 tw[512];
-  const int n = 1 << log2n, nz = n / 2 + 1;
-  const int P = FFT_TILE / n;
-  for (int i = threadIdx.x; i < n / 2; i += FFT_THREADS)
-    tw[i] = tw_g[i];
-  const uint64_t row0 = (uint64_t)blockIdx.x * P;
-  for (int t = threadIdx.x; t < P * n; t += FFT_THREADS) {
-    const uint64_t row = row0 + t / n;
-    A[t] = make_float2(row < rows ? in[row * n + (t % n)] : 0.0f, 0.0f);
-  }
-  __syncthreads();
-  const float2* r = stockham(A, B, log2n, P, tw);
-  for (int t = threadIdx.x; t < P * nz; t += FFT_THREADS) {
-    const int p = t / nz, k = t % nz;
-    const uint64_t row = row0 + p;
-    if (row < rows)
-      spec[row * nz + k] = r[p * n + k];
+    dft<R, INV>(&v[b * R]);
+    const int d = (j / NS) * NS * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      Y[d + r * NS] = v[b * R + r];
   }
 }
 
-__global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __restrict__ spec,
-  float* __restrict__ out, int log2n, uint64_t rows, float norm,
-  const float2* __restrict__ tw_g) {
-  __shared__ float2 A[FFT_TILE], B[FFT_TILE];
-  __shared__ float2 tw[512];
-  const int n = 1 << log2n, nz = n / 2 + 1;
-  const int P = FFT_TILE / n;
-  for (int i = threadIdx.x; i < n / 2; i += FFT_THREADS)
-    tw[i] = tw_g[i];
-  const uint64_t row0 = (uint64_t)blockIdx.x * P;
-  for (int t = threadIdx.x; t < P * nz; t += FFT_THREADS) {
-    const int p = t / nz, k = t % nz;
-    const uint64_t row = row0 + p;
-    const float2 v = row < rows ? spec[row * nz + k] : make_float2(0.0f, 0.0f);
-    A[p * n + k] = v;
-    if (k > 0 && k < n - k)
-      A[p * n + n - k] = make_float2(v.x, -v.y);
-  }
+// The radices of a length-N transform: a first stage of 2, 4 or 8 so that
+// the rest are radix 8.
+template <int N>
+struct Plan {
+  static constexpr int L = N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6 : N == 128 ? 7 :
+                           N == 256 ? 8 : N == 512 ? 9 : 10;
+  static constexpr int R0 = (L % 3 == 0) ? 8 : (1 << (L % 3));
+};
+
+// Every pencil of the tile (TK pencils, pitch N + 1) through all stages. Each
+// of the 256 threads serves four pencils; the tile is transformed in place,
+// a workgroup barrier between the reads and the writes of a stage.
+template <int N, int R, int NS, bool INV>
+__device__ __forceinline__ void tile_stage(float2* L, const float2* tw) {
+  constexpr int T = N / 8, PG = FFT_THREADS / T, PP = FFT_TILE / N / PG;
+  const int t = threadIdx.x % T, g = threadIdx.x / T;
+  float2 v[PP][8];
+#pragma unroll
+  for (int p = 0; p < PP; p++)
+    stage_read<N, R, NS, INV>(L + (p * PG + g) * (N + 1), t, v[p]);
   __syncthreads();
-  const float2* r = stockham(A, B, log2n, P, tw);
-  for (int t = threadIdx.x; t < P * n; t += FFT_THREADS) {
-    const uint64_t row = row0 + t / n;
-    if (row < rows)
-      out[row * n + (t % n)] = r[t].x * norm;
+#pragma unroll
+  for (int p = 0; p < PP; p++)
+    stage_write<N, R, NS, INV>(L + (p * PG + g) * (N + 1), t, v[p], tw);
+  __syncthreads();
+}
+
+template <int N, bool INV, int NS>
+__device__ __forceinline__ void tile_rest(float2* L, const float2* tw) {
+  if constexpr (NS < N) {
+    tile_stage<N, 8, NS, INV>(L, tw);
+    tile_rest<N, INV, NS * 8>(L, tw);
   }
+}
+
+template <int N, bool INV>
+__device__ __forceinline__ void tile_fft(float2* L, const float2* tw) {
+  constexpr int R0 = Plan<N>::R0;
+  tile_stage<N, R0, 1, INV>(L, tw);
+  tile_rest<N, INV, R0>(L, tw);
+}
+
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* tw, const float2* __restrict__ g) {
+  for (int i = threadIdx.x; i < N; i += FFT_THREADS)
+    tw[i] = g[i];
 }
 
 // --- x and y passes: tiles of TK consecutive kz columns ---
 //
-// Element i of pencil (a, kz) sits at base(a) + i * stride_i + kz with
-// base(a) = a * stride_a. With `filter`, the element read from `in` is
-// multiplied by the smoothing factor of its mode; the pass index i is then the
-// x index, a the y index.
+// Element i of pencil (a, kz) sits at a * stride_a + i * stride_i + kz. With
+// `lut`, the element read is multiplied by the smoothing factor of its mode;
+// the pass index i is then the x index and a the y index.
+template <int N, bool INV>
 __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restrict__ in,
-  float2* __restrict__ out, int log2n, int nz, uint64_t stride_a,
-  uint64_t stride_i, const float2* __restrict__ tw_g,
-  const float* __restrict__ lut, const double* __restrict__ axis) {
-  __shared__ float2 A[FFT_TILE], B[FFT_TILE];
-  __shared__ float2 tw[512];
-  const int n = 1 << log2n;
-  int TK = FFT_TILE / n;
-  if (TK > nz)
-    TK = nz;
+  float2* __restrict__ out, int nz, uint64_t stride_a, uint64_t stride_i,
+  const float2* __restrict__ tw_g, const float* __restrict__ lut,
+  const double* __restrict__ axis) {
+  constexpr int TK = FFT_TILE / N;
+  __shared__ float2 L[TK * (N + 1)];
+  __shared__ float2 tw[N];
   const int tiles = (nz + TK - 1) / TK;
   const int a = blockIdx.x / tiles;
   const int kz0 = (blockIdx.x % tiles) * TK;
   const int cols = nz - kz0 < TK ? nz - kz0 : TK;
-  for (int i = threadIdx.x; i < n / 2; i += FFT_THREADS)
-    tw[i] = tw_g[i];
+  load_twiddles<N>(tw, tw_g);
   const uint64_t base = (uint64_t)a * stride_a + kz0;
-  const int half = n >> 1;
-  for (int t = threadIdx.x; t < n * TK; t += FFT_THREADS) {
-    const int i = t / TK, c = t % TK;
+  constexpr int half = N / 2;
+  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+    const int i = e / TK, c = e % TK;
     float2 v = make_float2(0.0f, 0.0f);
     if (c < cols) {
       v = in[base + (uint64_t)i * stride_i + c];
       if (lut) {
         // i = x index, a = y index, kz0 + c = z index (fft.c:753-780)
-        const int kx = i > half ? i - n : i;
-        const int ky = a > half ? a - n : a;
+        const int kx = i > half ? i - N : i;
+        const int ky = a > half ? a - N : a;
         const int kz = kz0 + c;
         const uint32_t k2 = (uint32_t)(kx * kx) + (uint32_t)(ky * ky) + (uint32_t)(kz * kz);
         float s = lut[k2];
@@ -153,14 +226,67 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
         v = make_float2(v.x * s, v.y * s);
       }
     }
-    A[c * n + i] = v;
+    L[c * (N + 1) + i] = v;
   }
   __syncthreads();
-  const float2* r = stockham(A, B, log2n, TK, tw);
-  for (int t = threadIdx.x; t < n * TK; t += FFT_THREADS) {
-    const int i = t / TK, c = t % TK;
+  tile_fft<N, INV>(L, tw);
+  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+    const int i = e / TK, c = e % TK;
     if (c < cols)
-      out[base + (uint64_t)i * stride_i + c] = r[c * n + i];
+      out[base + (uint64_t)i * stride_i + c] = L[c * (N + 1) + i];
+  }
+}
+
+// --- z pass: TK whole rows ---
+
+template <int N>
+__global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __restrict__ in,
+  float2* __restrict__ spec, uint64_t rows, const float2* __restrict__ tw_g) {
+  constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
+  __shared__ float2 L[TK * (N + 1)];
+  __shared__ float2 tw[N];
+  load_twiddles<N>(tw, tw_g);
+  const uint64_t row0 = (uint64_t)blockIdx.x * TK;
+  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+    const int p = e / N, i = e % N;
+    const uint64_t row = row0 + p;
+    L[p * (N + 1) + i] = make_float2(row < rows ? in[row * N + i] : 0.0f, 0.0f);
+  }
+  __syncthreads();
+  tile_fft<N, false>(L, tw);
+  for (int e = threadIdx.x; e < nz * TK; e += FFT_THREADS) {
+    const int p = e / nz, k = e % nz;
+    const uint64_t row = row0 + p;
+    if (row < rows)
+      spec[row * nz + k] = L[p * (N + 1) + k];
+  }
+}
+
+// The Hermitian half row is extended to the full row (v(N-k) = conj v(k)),
+// transformed, and the real part scaled by `norm` (fft.c:832).
+template <int N>
+__global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __restrict__ spec,
+  float* __restrict__ out, uint64_t rows, float norm, const float2* __restrict__ tw_g) {
+  constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
+  __shared__ float2 L[TK * (N + 1)];
+  __shared__ float2 tw[N];
+  load_twiddles<N>(tw, tw_g);
+  const uint64_t row0 = (uint64_t)blockIdx.x * TK;
+  for (int e = threadIdx.x; e < nz * TK; e += FFT_THREADS) {
+    const int p = e / nz, k = e % nz;
+    const uint64_t row = row0 + p;
+    const float2 v = row < rows ? spec[row * nz + k] : make_float2(0.0f, 0.0f);
+    L[p * (N + 1) + k] = v;
+    if (k > 0 && k < N - k)
+      L[p * (N + 1) + N - k] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  tile_fft<N, true>(L, tw);
+  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+    const int p = e / N, i = e % N;
+    const uint64_t row = row0 + p;
+    if (row < rows)
+      out[row * N + i] = L[p * (N + 1) + i].x * norm;
   }
 }
 
@@ -170,7 +296,7 @@ namespace {
 
 struct Twiddles {
   int n = 0;
-  DeviceBuffer fwd, inv;
+  DeviceBuffer fwd;  // exp(-2 pi i m / n), m < n
 };
 Twiddles g_tw;
 DeviceBuffer g_lut, g_axis;
@@ -187,21 +313,75 @@ int log2_exact(uint32_t n) {
 int twiddles(uint32_t n) {
   if (g_tw.n == (int)n)
     return SIF_OK;
-  std::vector<float2> f(n / 2), b(n / 2);
-  for (uint32_t m = 0; m < n / 2; m++) {
+  std::vector<float2> f(n);
+  for (uint32_t m = 0; m < n; m++) {
     const double ang = 2.0 * 3.14159265358979323846 * (double)m / (double)n;
     f[m] = make_float2((float)cos(ang), (float)-sin(ang));
-    b[m] = make_float2((float)cos(ang), (float)sin(ang));
   }
-  int st = g_tw.fwd.reserve(n / 2 * sizeof(float2));
-  if (st == SIF_OK)
-    st = g_tw.inv.reserve(n / 2 * sizeof(float2));
+  int st = g_tw.fwd.reserve(n * sizeof(float2));
   if (st != SIF_OK)
     return st;
-  SIF_HIP(hipMemcpy(g_tw.fwd.ptr, f.data(), n / 2 * sizeof(float2), hipMemcpyHostToDevice));
-  SIF_HIP(hipMemcpy(g_tw.inv.ptr, b.data(), n / 2 * sizeof(float2), hipMemcpyHostToDevice));
+  SIF_HIP(hipMemcpy(g_tw.fwd.ptr, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
   g_tw.n = (int)n;
   return SIF_OK;
+}
+
+// The three passes for one n, dispatched to the kernels built for it.
+template <int N>
+int pass_cols(bool inv, const float2* in, float2* out, uint64_t stride_a, uint64_t stride_i,
+  const float* lut, const double* axis) {
+  constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
+  const int tiles = (nz + TK - 1) / TK;
+  if (inv)
+    cols_kernel<N, true><<<N * tiles, FFT_THREADS, 0, stream()>>>(in, out, nz, stride_a,
+      stride_i, g_tw.fwd.as<float2>(), lut, axis);
+  else
+    cols_kernel<N, false><<<N * tiles, FFT_THREADS, 0, stream()>>>(in, out, nz, stride_a,
+      stride_i, g_tw.fwd.as<float2>(), lut, axis);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+template <int N>
+int pass_r2c_rows(const float* in, float2* spec) {
+  constexpr int TK = FFT_TILE / N;
+  const uint64_t rows = (uint64_t)N * N;
+  r2c_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, stream()>>>(in, spec,
+    rows, g_tw.fwd.as<float2>());
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+template <int N>
+int pass_c2r_rows(const float2* spec, float* out, float norm) {
+  constexpr int TK = FFT_TILE / N;
+  const uint64_t rows = (uint64_t)N * N;
+  c2r_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, stream()>>>(spec, out,
+    rows, norm, g_tw.fwd.as<float2>());
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+#define SIF_FFT_DISPATCH(n, call)  \
+  switch (n) {                     \
+  case 16: { constexpr int N = 16; return call; }     \
+  case 32: { constexpr int N = 32; return call; }     \
+  case 64: { constexpr int N = 64; return call; }     \
+  case 128: { constexpr int N = 128; return call; }   \
+  case 256: { constexpr int N = 256; return call; }   \
+  case 512: { constexpr int N = 512; return call; }   \
+  default: { constexpr int N = 1024; return call; }   \
+  }
+
+int run_cols(uint32_t n, bool inv, const float2* in, float2* out, uint64_t stride_a,
+  uint64_t stride_i, const float* lut, const double* axis) {
+  SIF_FFT_DISPATCH(n, pass_cols<N>(inv, in, out, stride_a, stride_i, lut, axis))
+}
+int run_r2c_rows(uint32_t n, const float* in, float2* spec) {
+  SIF_FFT_DISPATCH(n, pass_r2c_rows<N>(in, spec))
+}
+int run_c2r_rows(uint32_t n, const float2* spec, float* out, float norm) {
+  SIF_FFT_DISPATCH(n, pass_c2r_rows<N>(spec, out, norm))
 }
 
 int check_n(uint32_t n, int* log2n) {
@@ -285,29 +465,56 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
     st = twiddles(n);
   if (st != SIF_OK)
     return st;
-  const int nz = n / 2 + 1;
+  const uint64_t nz = n / 2 + 1;
   const uint64_t rows = (uint64_t)n * n;
-  const int P = FFT_TILE / n;
   timing_begin(SIF_AMD_STAGE_FFT_FORWARD);
-  r2c_rows_kernel<<<grid_for(rows, P), FFT_THREADS, 0, stream()>>>(
-    values, work, log2n, rows, g_tw.fwd.as<float2>());
-  SIF_HIP(hipGetLastError());
-  int TK = FFT_TILE / n;
-  if (TK > nz)
-    TK = nz;
-  const int tiles = (nz + TK - 1) / TK;
+  st = run_r2c_rows(n, values, work);
   // y pass: pencil a = x, i = y
-  cols_kernel<<<n * tiles, FFT_THREADS, 0, stream()>>>(work, work, log2n, nz,
-    (uint64_t)n * nz, (uint64_t)nz, g_tw.fwd.as<float2>(), nullptr, nullptr);
-  SIF_HIP(hipGetLastError());
+  if (st == SIF_OK)
+    st = run_cols(n, false, work, work, (uint64_t)n * nz, nz, nullptr, nullptr);
   // x pass: pencil a = y, i = x
-  cols_kernel<<<n * tiles, FFT_THREADS, 0, stream()>>>(work, spec, log2n, nz,
-    (uint64_t)nz, (uint64_t)n * nz, g_tw.fwd.as<float2>(), nullptr, nullptr);
-  SIF_HIP(hipGetLastError());
+  if (st == SIF_OK)
+    st = run_cols(n, false, work, spec, nz, (uint64_t)n * nz, nullptr, nullptr);
+  if (st != SIF_OK)
+    return st;
   timing_end(SIF_AMD_STAGE_FFT_FORWARD);
   if (timing_active())  // the grid once, the spectrum once
     timing_bytes(SIF_AMD_STAGE_FFT_FORWARD, 4.0 * (double)rows * n + 8.0 * (double)rows * nz);
   return SIF_OK;
+}
+
+// The three inverse passes with a device-resident filter LUT (indexed by
+// |k|^2; null for none) and window axis (null for none).
+int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
+  float* out, float2* work) {
+  int log2n;
+  int st = check_n(n, &log2n);
+  if (st == SIF_OK)
+    st = twiddles(n);
+  if (st != SIF_OK)
+    return st;
+  const uint64_t nz = n / 2 + 1;
+  const uint64_t rows = (uint64_t)n * n;
+  const float norm = 1.0f / (float)((uint64_t)n * n * n);
+  timing_begin(SIF_AMD_STAGE_SMOOTH);
+  // x pass with the filter fused: pencil a = y, i = x
+  st = run_cols(n, true, spec, work, nz, (uint64_t)n * nz, d_lut, d_axis);
+  // y pass: pencil a = x, i = y
+  if (st == SIF_OK)
+    st = run_cols(n, true, work, work, (uint64_t)n * nz, nz, nullptr, nullptr);
+  if (st == SIF_OK)
+    st = run_c2r_rows(n, work, out, norm);
+  if (st != SIF_OK)
+    return st;
+  timing_end(SIF_AMD_STAGE_SMOOTH);
+  if (timing_active())  // the spectrum once, the field once
+    timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + 4.0 * (double)rows * n);
+  return SIF_OK;
+}
+
+uint64_t filter_lut_size(uint32_t n) {
+  const uint64_t h = n >> 1;
+  return 3 * h * h + 1;
 }
 
 // The backward stage of fft.c:703-888 for one filter: spectrum (unchanged)
@@ -317,52 +524,28 @@ int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
   float2* work) {
   int log2n;
   int st = check_n(n, &log2n);
-  if (st == SIF_OK)
-    st = twiddles(n);
   if (st != SIF_OK)
     return st;
-  build_filter_lut(filter, r, sigma, box, n, g_lut_host);
-  st = g_lut.reserve(g_lut_host.size() * sizeof(float));
-  if (st != SIF_OK)
-    return st;
-  const bool windowed = window_order > 0 && filter != SIF_AMD_FILTER_NONE;
+  const bool filtered = filter != SIF_AMD_FILTER_NONE;
+  const bool windowed = window_order > 0 && filtered;
+  if (filtered) {
+    build_filter_lut(filter, r, sigma, box, n, g_lut_host);
+    st = g_lut.reserve(g_lut_host.size() * sizeof(float));
+    if (st != SIF_OK)
+      return st;
+    SIF_HIP(hipMemcpyAsync(g_lut.ptr, g_lut_host.data(), g_lut_host.size() * sizeof(float),
+      hipMemcpyHostToDevice, stream()));
+  }
   if (windowed) {
     build_window_axis(n, window_order, window_mode, g_axis_host);
     st = g_axis.reserve(n * sizeof(double));
     if (st != SIF_OK)
       return st;
-  }
-  timing_begin(SIF_AMD_STAGE_SMOOTH);
-  SIF_HIP(hipMemcpyAsync(g_lut.ptr, g_lut_host.data(), g_lut_host.size() * sizeof(float),
-    hipMemcpyHostToDevice, stream()));
-  if (windowed)
     SIF_HIP(hipMemcpyAsync(g_axis.ptr, g_axis_host.data(), n * sizeof(double),
       hipMemcpyHostToDevice, stream()));
-  const int nz = n / 2 + 1;
-  int TK = FFT_TILE / n;
-  if (TK > nz)
-    TK = nz;
-  const int tiles = (nz + TK - 1) / TK;
-  const bool filtered = filter != SIF_AMD_FILTER_NONE;
-  // x pass with the filter fused: pencil a = y, i = x
-  cols_kernel<<<n * tiles, FFT_THREADS, 0, stream()>>>(spec, work, log2n, nz,
-    (uint64_t)nz, (uint64_t)n * nz, g_tw.inv.as<float2>(),
-    filtered ? g_lut.as<float>() : nullptr, windowed ? g_axis.as<double>() : nullptr);
-  SIF_HIP(hipGetLastError());
-  // y pass: pencil a = x, i = y
-  cols_kernel<<<n * tiles, FFT_THREADS, 0, stream()>>>(work, work, log2n, nz,
-    (uint64_t)n * nz, (uint64_t)nz, g_tw.inv.as<float2>(), nullptr, nullptr);
-  SIF_HIP(hipGetLastError());
-  const uint64_t rows = (uint64_t)n * n;
-  const int P = FFT_TILE / n;
-  const float norm = 1.0f / (float)((uint64_t)n * n * n);
-  c2r_rows_kernel<<<grid_for(rows, P), FFT_THREADS, 0, stream()>>>(
-    work, out, log2n, rows, norm, g_tw.inv.as<float2>());
-  SIF_HIP(hipGetLastError());
-  timing_end(SIF_AMD_STAGE_SMOOTH);
-  if (timing_active())  // the spectrum once, the field once
-    timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + 4.0 * (double)rows * n);
-  return SIF_OK;
+  }
+  return fft_c2r_lut(spec, n, filtered ? g_lut.as<float>() : nullptr,
+    windowed ? g_axis.as<double>() : nullptr, out, work);
 }
 
 }  // namespace sif_amd

@@ -144,22 +144,53 @@ __global__ void __launch_bounds__(256) prefilter_kernel(uint32_t* __restrict__ f
 
 // The overlap state of each grown sphere (radius >= 0) against the catalogue
 // as the rung began, for a reference max_r of the catalogue as the batch
-// began (exodus_finder.c:2302-2311, the test at the rescaled radius).
-__global__ void grown_overlap_kernel(const float* __restrict__ cx,
-  const float* __restrict__ cy, const float* __restrict__ cz,
+// began (exodus_finder.c:2302-2311, the test at the rescaled radius). One
+// wavefront per candidate: a grown sphere reaches many void cells, and the
+// lanes split them; no cross-lane operation until every lane is done.
+constexpr int GO_WAVES = 4;
+
+__global__ void __launch_bounds__(64 * GO_WAVES) grown_overlap_kernel(
+  const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
   const float* __restrict__ radius, uint32_t n, VoidIndex V, float max_r, float box,
   float f, float scale, uint32_t* __restrict__ state) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t k = blockIdx.x * GO_WAVES + (threadIdx.x >> 6);
   if (k >= n)
     return;
   const float r = radius[k];
-  int s = 0;
+  int st = 0;
   if (r >= 0.0f && V.n) {
     OverlapQuery q;
     overlap_query(q, cx[k], cy[k], cz[k], r, max_r, box, f, scale, V.inv_l);
-    s = overlap_state(V, q);
+    for (int cls = VOID_CLASSES - 1; cls >= 0 && st != 1; cls--) {
+      if (V.class_max_r[cls] < 0.0f)
+        continue;
+      int32_t lo[3], hi[3];
+      superset_scan(q, V.class_max_r[cls], V.inv_l, lo, hi);
+      const uint32_t na = (uint32_t)(hi[0] - lo[0] + 1), nb = (uint32_t)(hi[1] - lo[1] + 1),
+                     nc = (uint32_t)(hi[2] - lo[2] + 1);
+      const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+      for (uint32_t idx = lane; idx < na * nb * nc && st != 1; idx += 64) {
+        const uint32_t wa = void_wrap(lo[0] + (int32_t)(idx / (nb * nc)));
+        const uint32_t wb = void_wrap(lo[1] + (int32_t)((idx / nc) % nb));
+        const uint32_t wc = void_wrap(lo[2] + (int32_t)(idx % nc));
+        const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wc;
+        const uint32_t end = start[cell + 1];
+        for (uint32_t j = start[cell]; j < end; j++) {
+          const uint32_t i = V.items[j];
+          const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
+          if (s == 1) {
+            st = 1;
+            break;
+          }
+          st |= s;
+        }
+      }
+    }
   }
-  state[k] = (uint32_t)s;
+  const uint64_t ones = __ballot(st == 1), twos = __ballot(st == 2);
+  if (lane == 0)
+    state[k] = ones ? 1u : (twos ? 2u : 0u);
 }
 
 // Candidate nodes -> centres (exodus_finder.c:386-405 for a centre on its
@@ -332,8 +363,8 @@ int grown_overlap(const float* cx, const float* cy, const float* cz, const float
   uint32_t* state) {
   if (n == 0)
     return SIF_OK;
-  grown_overlap_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(cx, cy, cz, radius, n, V,
-    max_r, box, f, scale, state);
+  grown_overlap_kernel<<<(n + GO_WAVES - 1) / GO_WAVES, 64 * GO_WAVES, 0, stream()>>>(cx, cy,
+    cz, radius, n, V, max_r, box, f, scale, state);
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }

@@ -28,6 +28,14 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work);
 int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
   float r, float sigma, int window_order, int window_mode, float* out,
   float2* work);
+// The inverse passes with a device LUT (indexed by |k|^2, filter_lut_size(n)
+// entries) and window axis (n doubles); either may be null.
+int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
+  float* out, float2* work);
+uint64_t filter_lut_size(uint32_t n);
+void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
+  std::vector<float>& lut);
+void build_window_axis(uint32_t n, int order, int mode, std::vector<double>& axis);
 
 // scan.hip
 int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,

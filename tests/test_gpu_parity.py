@@ -140,7 +140,7 @@ def test_chain_mesh_edge_particles(native):
     np.testing.assert_array_equal(m.x.get(), want["x"])
 
 
-@pytest.mark.parametrize("n", [16, 32, 64])
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512])
 def test_fft_round_trip(native, n):
     """test_fft.c:49-125: r2c then c2r returns the field to 1e-4."""
     from sif_amd import device as D
@@ -149,11 +149,25 @@ def test_fft_round_trip(native, n):
     dv = D.to_device(v)
     spec = D.spectrum(dv, n)
     s = spec.get()
-    want = np.fft.rfftn(v.reshape(n, n, n).astype(np.float64))
+    import scipy.fft
+    want = scipy.fft.rfftn(v.reshape(n, n, n).astype(np.float64), workers=-1)
     got = s[..., 0] + 1j * s[..., 1]
     assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    del want, got, s
     back = D.smooth(spec, n, BOX, 0, 0.0, window_order=0).get()
     assert np.abs(back - v).max() < 1e-4
+    # a single mode: the spectrum has one nonzero entry at the right index,
+    # of magnitude n^3 / 2 (pins the twiddle sign and the axis order)
+    k = (1, 2, 3)
+    i = np.arange(n, dtype=np.float64)
+    phase = (k[0] * i)[:, None, None] + (k[1] * i)[None, :, None] + (k[2] * i)[None, None, :]
+    wave = np.cos(2 * np.pi * phase / n).astype(np.float32).ravel()
+    del phase
+    s1 = D.spectrum(D.to_device(wave), n).get()
+    mag = np.hypot(s1[..., 0], s1[..., 1])
+    assert abs(mag[k] - n ** 3 / 2) <= 1e-4 * n ** 3
+    mag[k] = 0.0
+    assert mag.max() <= 1e-4 * n ** 3
 
 
 def test_top_hat_preserves_constant(native):
