@@ -207,33 +207,51 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
   load_twiddles<N>(tw, tw_g);
   const uint64_t base = (uint64_t)a * stride_a + kz0;
   constexpr int half = N / 2;
-  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+  // every load of the thread issued before the first is used
+  constexpr int PER = N * TK / FFT_THREADS;
+  float2 v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
     const int i = e / TK, c = e % TK;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (c < cols) {
-      v = in[base + (uint64_t)i * stride_i + c];
-      if (lut) {
-        // i = x index, a = y index, kz0 + c = z index (fft.c:753-780)
-        const int kx = i > half ? i - N : i;
-        const int ky = a > half ? a - N : a;
-        const int kz = kz0 + c;
-        const uint32_t k2 = (uint32_t)(kx * kx) + (uint32_t)(ky * ky) + (uint32_t)(kz * kz);
-        float s = lut[k2];
-        if (axis) {
-          const double cxy = axis[i] * axis[a];
-          s = (float)((double)s * cxy * axis[kz]);
-        }
-        v = make_float2(v.x * s, v.y * s);
+    v[q] = c < cols ? in[base + (uint64_t)i * stride_i + c] : make_float2(0.0f, 0.0f);
+  }
+  if (lut) {
+    // i = x index, a = y index, kz0 + c = z index (fft.c:753-780)
+    const int ky = a > half ? a - N : a;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const int i = e / TK, c = e % TK;
+      const int kx = i > half ? i - N : i;
+      const int kz = kz0 + c < nz ? kz0 + c : nz - 1;
+      const uint32_t k2 = (uint32_t)(kx * kx) + (uint32_t)(ky * ky) + (uint32_t)(kz * kz);
+      float s = lut[k2];
+      if (axis) {
+        const double cxy = axis[i] * axis[a];
+        s = (float)((double)s * cxy * axis[kz]);
       }
+      v[q] = make_float2(v[q].x * s, v[q].y * s);
     }
-    L[c * (N + 1) + i] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    L[(e % TK) * (N + 1) + e / TK] = v[q];
   }
   __syncthreads();
   tile_fft<N, INV>(L, tw);
-  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    v[q] = L[(e % TK) * (N + 1) + e / TK];
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
     const int i = e / TK, c = e % TK;
     if (c < cols)
-      out[base + (uint64_t)i * stride_i + c] = L[c * (N + 1) + i];
+      out[base + (uint64_t)i * stride_i + c] = v[q];
   }
 }
 
@@ -247,18 +265,36 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __re
   __shared__ float2 tw[N];
   load_twiddles<N>(tw, tw_g);
   const uint64_t row0 = (uint64_t)blockIdx.x * TK;
-  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
-    const int p = e / N, i = e % N;
-    const uint64_t row = row0 + p;
-    L[p * (N + 1) + i] = make_float2(row < rows ? in[row * N + i] : 0.0f, 0.0f);
+  constexpr int PER = N * TK / FFT_THREADS;
+  constexpr int PER_H = (nz * TK + FFT_THREADS - 1) / FFT_THREADS;
+  {
+    float r[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const uint64_t row = row0 + e / N;
+      r[q] = row < rows ? in[row * N + e % N] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      L[(e / N) * (N + 1) + e % N] = make_float2(r[q], 0.0f);
+    }
   }
   __syncthreads();
   tile_fft<N, false>(L, tw);
-  for (int e = threadIdx.x; e < nz * TK; e += FFT_THREADS) {
-    const int p = e / nz, k = e % nz;
-    const uint64_t row = row0 + p;
-    if (row < rows)
-      spec[row * nz + k] = L[p * (N + 1) + k];
+  float2 v[PER_H];
+#pragma unroll
+  for (int q = 0; q < PER_H; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    v[q] = e < nz * TK ? L[(e / nz) * (N + 1) + e % nz] : make_float2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int q = 0; q < PER_H; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    const uint64_t row = row0 + e / nz;
+    if (e < nz * TK && row < rows)
+      spec[row * nz + e % nz] = v[q];
   }
 }
 
@@ -272,21 +308,41 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
   __shared__ float2 tw[N];
   load_twiddles<N>(tw, tw_g);
   const uint64_t row0 = (uint64_t)blockIdx.x * TK;
-  for (int e = threadIdx.x; e < nz * TK; e += FFT_THREADS) {
-    const int p = e / nz, k = e % nz;
-    const uint64_t row = row0 + p;
-    const float2 v = row < rows ? spec[row * nz + k] : make_float2(0.0f, 0.0f);
-    L[p * (N + 1) + k] = v;
-    if (k > 0 && k < N - k)
-      L[p * (N + 1) + N - k] = make_float2(v.x, -v.y);
+  constexpr int PER = N * TK / FFT_THREADS;
+  constexpr int PER_H = (nz * TK + FFT_THREADS - 1) / FFT_THREADS;
+  {
+    float2 v[PER_H];
+#pragma unroll
+    for (int q = 0; q < PER_H; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const uint64_t row = row0 + e / nz;
+      v[q] = e < nz * TK && row < rows ? spec[row * nz + e % nz] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int q = 0; q < PER_H; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      if (e < nz * TK) {
+        const int p = e / nz, k = e % nz;
+        L[p * (N + 1) + k] = v[q];
+        if (k > 0 && k < N - k)
+          L[p * (N + 1) + N - k] = make_float2(v[q].x, -v[q].y);
+      }
+    }
   }
   __syncthreads();
   tile_fft<N, true>(L, tw);
-  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
-    const int p = e / N, i = e % N;
-    const uint64_t row = row0 + p;
+  float r[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    r[q] = L[(e / N) * (N + 1) + e % N].x * norm;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    const uint64_t row = row0 + e / N;
     if (row < rows)
-      out[row * N + i] = L[p * (N + 1) + i].x * norm;
+      out[row * N + e % N] = r[q];
   }
 }
 

@@ -173,6 +173,8 @@ __device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
   *ez = cz;
 }
 
+constexpr int PARTICLE_UNROLL = 4;
+
 // Walks the template 64 entries at a time. `want(entry, type, bound)` says
 // whether the entry is visited; `whole(type, count)` consumes a whole cell
 // without reading its particles (returns true when it did); `visit(d2, type)`
@@ -204,20 +206,41 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
     S.t_ez[lane] = ez;
     S.t_type[lane] = type;
     wave_sync();
-    for (uint32_t j = lane; j < total; j += 64) {
-      // first entry whose inclusive prefix exceeds j
-      int lo = 0, hi = 63;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (S.t_scan[mid] > j)
-          hi = mid;
-        else
-          lo = mid + 1;
+    // PARTICLE_UNROLL particles per lane per round, their loads issued
+    // together (the visit order does not matter to any caller)
+    for (uint32_t j0 = 0; j0 < total; j0 += 64 * PARTICLE_UNROLL) {
+      uint32_t pi[PARTICLE_UNROLL];
+      int ent[PARTICLE_UNROLL];
+#pragma unroll
+      for (int u = 0; u < PARTICLE_UNROLL; u++) {
+        const uint32_t j = j0 + (uint32_t)u * 64 + lane;
+        // first entry whose inclusive prefix exceeds j
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (S.t_scan[mid] > j)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        const uint32_t before = lo ? S.t_scan[lo - 1] : 0u;
+        ent[u] = lo;
+        pi[u] = j < total ? S.t_start[lo] + (j - before) : 0xffffffffu;
       }
-      const uint32_t before = lo ? S.t_scan[lo - 1] : 0u;
-      const uint32_t p = S.t_start[lo] + (j - before);
-      const float d2 = dist2(P.mx[p], P.my[p], P.mz[p], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]);
-      visit(d2, S.t_type[lo]);
+      float px[PARTICLE_UNROLL], py[PARTICLE_UNROLL], pz[PARTICLE_UNROLL];
+#pragma unroll
+      for (int u = 0; u < PARTICLE_UNROLL; u++) {
+        const uint32_t p = pi[u] != 0xffffffffu ? pi[u] : 0u;
+        px[u] = P.mx[p];
+        py[u] = P.my[p];
+        pz[u] = P.mz[p];
+      }
+#pragma unroll
+      for (int u = 0; u < PARTICLE_UNROLL; u++)
+        if (pi[u] != 0xffffffffu) {
+          const int lo = ent[u];
+          visit(dist2(px[u], py[u], pz[u], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]), S.t_type[lo]);
+        }
     }
     wave_sync();
   }
