@@ -133,6 +133,13 @@ typedef struct {
  * capacity of the result) receives each void's enclosed count. stats
  * (optional, host, n_radii entries) receives the per-rung counters in ladder
  * order (largest first). */
+/* sif__mark_sphere (src/finder/utils.c:965-1033): paints n_voids spheres
+ * (radius scale * d_r[i]) into the n_cells^3-bit mask d_mask_words (device,
+ * (n_cells^3 + 63) / 64 words), OR-ing into what it holds. */
+SIF_API int sif_amd_mark_spheres(uint64_t* d_mask_words, uint32_t n_cells, float box,
+  const float* d_cx, const float* d_cy, const float* d_cz, const float* d_r,
+  uint32_t n_voids, float scale);
+
 SIF_API sif_catalogue_t* sif_amd_exodus_device(float* d_delta,
   uint32_t n_cells, float box, const uint64_t* d_offsets, const float* d_mx,
   const float* d_my, const float* d_mz, uint32_t mesh_cells,

@@ -87,6 +87,7 @@ _sig("sif_amd_spectrum_bytes", u64, u32)
 _sig("sif_amd_fft_r2c", i32, P, u32, P, P)
 _sig("sif_amd_fft_filter_c2r", i32, P, u32, f32, i32, f32, f32, i32, i32, P, P)
 _sig("sif_amd_scan_candidates", i32, P, P, u64, f32, P, P, u64, C.POINTER(u64))
+_sig("sif_amd_mark_spheres", i32, P, u32, f32, P, P, P, P, u32, f32)
 _sig("sif_amd_find_radius", i32, P, P, P, P, u32, f32, P, P, P, u32, f32, f32, f32, f32,
      P, P, P)
 _sig("sif_amd_exodus_device", C.POINTER(Catalogue), P, u32, f32, P, P, P, P, u32, u64,

@@ -31,8 +31,9 @@
 
 #include "common.h"
 #include "internal.h"
-#include "mask.h"
 #include "overlap.h"
+#include "paint.h"
+#include "rung_voids.h"
 #include "sif/structures/catalogue.h"
 #include "sif_amd.h"
 
@@ -125,72 +126,9 @@ struct Voids {
   }
 };
 
-// The voids accepted since the rung began, in a cell list of their own (the
-// device index holds the catalogue as the rung began). state() is the
-// overlap state of overlap.h over these voids alone.
-struct RungVoids {
-  std::vector<float> cx, cy, cz, r;
-  std::vector<int32_t> head, next;
-  std::vector<uint32_t> touched;
-  float inv_l = 0.0f, max_r = 0.0f;
-  void init(float box) {
-    inv_l = (float)VOID_CELLS / box;
-    head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
-  }
-  void clear() {
-    for (uint32_t c : touched)
-      head[c] = -1;
-    touched.clear();
-    cx.clear();
-    cy.clear();
-    cz.clear();
-    r.clear();
-    next.clear();
-    max_r = 0.0f;
-  }
-  void add(float x, float y, float z, float rr) {
-    const int32_t k = (int32_t)cx.size();
-    cx.push_back(x);
-    cy.push_back(y);
-    cz.push_back(z);
-    r.push_back(rr);
-    const uint32_t c =
-      (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
-      void_axis_cell(z, inv_l);
-    if (head[c] == -1)
-      touched.push_back(c);
-    next.push_back(head[c]);
-    head[c] = k;
-    if (rr > max_r)
-      max_r = rr;
-  }
-  int state(const OverlapQuery& q) const {
-    if (cx.empty())
-      return 0;
-    int32_t lo[3], hi[3];
-    superset_scan(q, max_r, inv_l, lo, hi);
-    int st = 0;
-    for (int32_t a = lo[0]; a <= hi[0]; a++) {
-      const uint32_t wa = void_wrap(a);
-      for (int32_t b = lo[1]; b <= hi[1]; b++) {
-        const uint32_t wb = void_wrap(b);
-        for (int32_t c = lo[2]; c <= hi[2]; c++) {
-          const uint32_t wc = void_wrap(c);
-          for (int32_t i = head[(wa * VOID_CELLS + wb) * VOID_CELLS + wc]; i != -1; i = next[i]) {
-            const int s = pair_state(q, wa, wb, wc, cx[i], cy[i], cz[i], r[i]);
-            if (s == 1)
-              return 1;
-            st |= s;
-          }
-        }
-      }
-    }
-    return st;
-  }
-};
-
 struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
+  DeviceBuffer pvx, pvy, pvz, pvr;  // voids to paint into the mask
   DeviceBuffer vcx, vcy, vcz, vr, vstart, vitems;
 };
 DeviceState g_dev;
@@ -309,6 +247,31 @@ int upload_voids(const Voids& V, VoidIndex* out) {
   return SIF_OK;
 }
 
+// Paints voids (host arrays) into the device mask.
+int paint_on_device(const float* x, const float* y, const float* z, const float* r,
+  uint32_t nv, float scale, float l, uint32_t n) {
+  if (nv == 0)
+    return SIF_OK;
+  const size_t fb = (size_t)nv * sizeof(float);
+  int st = g_dev.pvx.reserve(fb);
+  if (st == SIF_OK) st = g_dev.pvy.reserve(fb);
+  if (st == SIF_OK) st = g_dev.pvz.reserve(fb);
+  if (st == SIF_OK) st = g_dev.pvr.reserve(fb);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemcpyAsync(g_dev.pvx.ptr, x, fb, hipMemcpyHostToDevice, stream()));
+  SIF_HIP(hipMemcpyAsync(g_dev.pvy.ptr, y, fb, hipMemcpyHostToDevice, stream()));
+  SIF_HIP(hipMemcpyAsync(g_dev.pvz.ptr, z, fb, hipMemcpyHostToDevice, stream()));
+  SIF_HIP(hipMemcpyAsync(g_dev.pvr.ptr, r, fb, hipMemcpyHostToDevice, stream()));
+  st = paint_voids(g_dev.pvx.as<float>(), g_dev.pvy.as<float>(), g_dev.pvz.as<float>(),
+    g_dev.pvr.as<float>(), nv, scale, l, n, g_dev.mask.as<uint64_t>());
+  if (st != SIF_OK)
+    return st;
+  // the host arrays may change once this returns
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
 int cmp_desc(const void* a, const void* b) {
   const float x = *(const float*)a, y = *(const float*)b;
   return (x < y) - (x > y);
@@ -394,25 +357,27 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   const uint64_t lut_size = filter_lut_size(n);
   const bool windowed = window_order_of(R.opt) > 0;
 
-  Mask mask;
-  mask.init(n);
+  // The mask of accepted-void nodes lives on the device: painted there, at
+  // the end of each rung, with that rung's voids. Within a rung the host
+  // asks only about the voids accepted since it began (RungVoids).
   Voids V;
   V.init(R.box);
   RungVoids RV;
   RV.init(R.box);
-  if (R.prior) {
-    for (uint64_t v = 0; v < R.prior->n_voids; v++) {
-      V.add(R.prior->cx[v], R.prior->cy[v], R.prior->cz[v], R.prior->radii[v], 0);
-      mark_sphere(mask, R.prior->cx[v], R.prior->cy[v], R.prior->cz[v],
-        scale * R.prior->radii[v], l);
-    }
-  }
-
-  st = g_dev.mask.reserve(mask.words.size() * sizeof(uint64_t));
+  st = g_dev.mask.reserve((total + 63) / 64 * sizeof(uint64_t));
   if (st == SIF_OK)
     st = reserve_slots();
   if (st != SIF_OK)
     return st;
+  SIF_HIP(hipMemsetAsync(g_dev.mask.ptr, 0, (total + 63) / 64 * sizeof(uint64_t), stream()));
+  if (R.prior && R.prior->n_voids) {
+    for (uint64_t v = 0; v < R.prior->n_voids; v++)
+      V.add(R.prior->cx[v], R.prior->cy[v], R.prior->cz[v], R.prior->radii[v], 0);
+    st = paint_on_device(R.prior->cx, R.prior->cy, R.prior->cz, R.prior->radii,
+      (uint32_t)R.prior->n_voids, scale, l, n);
+    if (st != SIF_OK)
+      return st;
+  }
 
   std::vector<uint32_t> surv;
   uint64_t cand_cap = 0;
@@ -427,11 +392,6 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       windowed ? g_dev.axis.as<double>() : nullptr, R.d_delta, g_dev.work.as<float2>());
     if (st != SIF_OK)
       break;
-    if (mask.dirty) {
-      SIF_HIP(hipMemcpyAsync(g_dev.mask.ptr, mask.words.data(),
-        mask.words.size() * sizeof(uint64_t), hipMemcpyHostToDevice, stream()));
-      mask.dirty = false;
-    }
 
     // the candidates in flat order, growing the buffers when they are short
     uint64_t nc = 0;
@@ -521,31 +481,34 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       *iz = (uint32_t)(flat % nn);
     };
     // An exact overlap with the catalogue as it stands (utils.c:850-948),
-    // given the overlap state of the voids the rung began with: only the
-    // voids accepted since are scanned here, unless a state 2 sends the
-    // candidate to the reference's own scan of the whole catalogue.
-    auto exact_overlap = [&](float cx, float cy, float cz, float rr, uint32_t old_state) {
-      if (old_state == 1)
+    // from the overlap state of the voids the rung began with (old_state,
+    // from the device) and of those accepted since (new_state, from the
+    // host's probe); a state 2 on either side sends the candidate to the
+    // reference's own scan of the whole catalogue.
+    auto exact_overlap = [&](float cx, float cy, float cz, float rr, uint32_t old_state,
+                           int new_state) {
+      if (old_state == 1 || (old_state == 0 && new_state == 1))
         return true;
-      if (old_state == 0) {
-        OverlapQuery q;
-        overlap_query(q, cx, cy, cz, rr, V.max_r, R.box, R.f, scale, V.inv_l);
-        const int s = RV.state(q);
-        if (s != 2)
-          return s == 1;
-      }
+      if (old_state == 0 && new_state == 0)
+        return false;
       return V.overlap_exact(cx, cy, cz, rr, R.f, scale);
     };
-    // the rung-radius tests against the catalogue as it stands: 0 pass,
-    // 1 masked, 2 proxy, 3 overlap
+    // The rung-radius tests against the catalogue as it stands: 0 pass,
+    // 1 masked, 2 proxy, 3 overlap. The mask and pole probe against the
+    // rung's older voids were passed in the prefilter (the device mask does
+    // not change within a rung), so only the new voids are probed here.
     auto rung_tests = [&](uint32_t flat, bool edge) -> int {
-      if (mask.get(flat))
-        return 1;
       uint32_t ix, iy, iz;
       node_of(flat, &ix, &iy, &iz);
-      if (proxy > 0 && overlap_quick(mask, ix, iy, iz, (uint32_t)proxy))
+      const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+      OverlapQuery q;
+      overlap_query(q, cx, cy, cz, radius, V.max_r, R.box, R.f, scale, V.inv_l);
+      const RungProbe p = RV.probe(&q, ix, iy, iz, cx, cy, cz, true, proxy, scale, l, n);
+      if (p.node)
+        return 1;
+      if (proxy > 0 && p.pole)
         return 2;
-      if (exact_overlap((float)ix * l, (float)iy * l, (float)iz * l, radius, edge ? 2u : 0u))
+      if (exact_overlap(cx, cy, cz, radius, edge ? 2u : 0u, p.state))
         return 3;
       return 0;
     };
@@ -591,7 +554,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           B.d_nin.as<uint64_t>(), algo ? B.d_algo.as<uint64_t>() : nullptr);
       if (e == SIF_OK)
         e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
-          B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale, B.d_state.as<uint32_t>());
+          B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
+          g_dev.mask.as<uint64_t>(), n, l, B.d_state.as<uint32_t>());
       if (e != SIF_OK)
         return e;
       SIF_HIP(hipMemcpyAsync(B.h_rad, B.d_rad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
@@ -635,18 +599,24 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         uint32_t ix, iy, iz;
         node_of(flat, &ix, &iy, &iz);
         const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+        // the tests at the rescaled radius (exodus_finder.c:2290-2311): the
+        // device answered for the rung's older voids, the probe answers for
+        // the new ones
         const int32_t pole = (int32_t)(scale * r * (1.0f - R.f) / l) - 1;
-        if (pole > 0 && overlap_quick(mask, ix, iy, iz, (uint32_t)pole)) {
+        const uint32_t gs = B.h_state[b];
+        OverlapQuery q;
+        overlap_query(q, cx, cy, cz, r, V.max_r, R.box, R.f, scale, V.inv_l);
+        const RungProbe p = RV.probe(&q, ix, iy, iz, cx, cy, cz, false, pole, scale, l, n);
+        if (pole > 0 && ((gs & GROWN_POLE_BIT) || p.pole)) {
           S.proxy++;
           continue;
         }
-        if (exact_overlap(cx, cy, cz, r, B.h_state[b])) {
+        if (exact_overlap(cx, cy, cz, r, gs & 3u, p.state)) {
           S.overlap++;
           continue;
         }
         V.add(cx, cy, cz, r, B.h_nin[b]);
         RV.add(cx, cy, cz, r);
-        mark_sphere(mask, cx, cy, cz, scale * r, l);
         accepted++;
       }
       S.accepted += accepted;
@@ -693,6 +663,10 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       }
       std::swap(cur, nxt);
     }
+    // the rung's voids into the device mask, for the rungs below
+    if (st == SIF_OK)
+      st = paint_on_device(RV.cx.data(), RV.cy.data(), RV.cz.data(), RV.r.data(),
+        (uint32_t)RV.cx.size(), scale, l, n);
     if (stats_out)
       stats_out[i] = S;
   }

@@ -28,6 +28,7 @@
 #include "sif_amd.h"
 #include "grow_device.h"
 #include "overlap.h"
+#include "paint.h"
 
 namespace sif_amd {
 
@@ -151,14 +152,25 @@ constexpr int GO_WAVES = 4;
 
 __global__ void __launch_bounds__(64 * GO_WAVES) grown_overlap_kernel(
   const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
-  const float* __restrict__ radius, uint32_t n, VoidIndex V, float max_r, float box,
-  float f, float scale, uint32_t* __restrict__ state) {
+  const uint32_t* __restrict__ flat, const float* __restrict__ radius, uint32_t n,
+  VoidIndex V, float max_r, float box, float f, float scale, const uint64_t* __restrict__ mask,
+  uint32_t grid_n, float l, uint32_t* __restrict__ state) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t k = blockIdx.x * GO_WAVES + (threadIdx.x >> 6);
   if (k >= n)
     return;
   const float r = radius[k];
   int st = 0;
+  // the pole probe at the rescaled radius against the mask as the rung
+  // began (exodus_finder.c:2290-2300)
+  bool pole_hit = false;
+  if (r >= 0.0f && lane == 0) {
+    const int32_t pole = (int32_t)(scale * r * (1.0f - f) / l) - 1;
+    const uint32_t node = flat[k];
+    if (pole > 0)
+      pole_hit = overlap_quick(mask, grid_n, node / (grid_n * grid_n), (node / grid_n) % grid_n,
+        node % grid_n, (uint32_t)pole);
+  }
   if (r >= 0.0f && V.n) {
     OverlapQuery q;
     overlap_query(q, cx[k], cy[k], cz[k], r, max_r, box, f, scale, V.inv_l);
@@ -190,7 +202,51 @@ __global__ void __launch_bounds__(64 * GO_WAVES) grown_overlap_kernel(
   }
   const uint64_t ones = __ballot(st == 1), twos = __ballot(st == 2);
   if (lane == 0)
-    state[k] = ones ? 1u : (twos ? 2u : 0u);
+    state[k] = (ones ? 1u : (twos ? 2u : 0u)) | (pole_hit ? GROWN_POLE_BIT : 0u);
+}
+
+// sif__mark_sphere (utils.c:965-1033) for a list of voids into the device
+// mask: one workgroup per void walks the reference's cube of nodes, a row
+// (dx, dy) per thread, and ORs each 64-bit word's bits in with one atomic.
+// Setting bits is idempotent, so the mask does not depend on the order.
+__global__ void __launch_bounds__(256) paint_voids_kernel(const float* __restrict__ vx,
+  const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ vr,
+  uint32_t nv, float scale, float l, uint32_t grid_n, unsigned long long* __restrict__ mask) {
+  const uint32_t v = blockIdx.x;
+  if (v >= nv)
+    return;
+  PaintSphere P;
+  paint_sphere(P, vx[v], vy[v], vz[v], scale * vr[v], l, grid_n);
+  const int32_t span = 2 * P.c + 1;
+  const uint64_t N = grid_n;
+  for (int32_t row = (int32_t)threadIdx.x; row < span * span; row += (int32_t)blockDim.x) {
+    const int32_t gx = P.ox + row / span - P.c, gy = P.oy + row % span - P.c;
+    const float ddx = paint_dist(P, gx, P.cx);
+    const float dx2 = ddx * ddx;
+    if (dx2 > P.r2)
+      continue;
+    const float ddy = paint_dist(P, gy, P.cy);
+    const float dxy2 = dx2 + ddy * ddy;
+    if (dxy2 > P.r2)
+      continue;
+    const uint64_t base = ((uint64_t)paint_wrap(gx, P.n) * N + paint_wrap(gy, P.n)) * N;
+    uint64_t word = ~0ull, bits = 0;
+    for (int32_t gz = P.oz - P.c; gz <= P.oz + P.c; gz++) {
+      const float ddz = paint_dist(P, gz, P.cz);
+      if (!(dxy2 + ddz * ddz <= P.r2))
+        continue;
+      const uint64_t idx = base + paint_wrap(gz, P.n);
+      if ((idx >> 6) != word) {
+        if (bits)
+          atomicOr(&mask[word], (unsigned long long)bits);
+        word = idx >> 6;
+        bits = 0;
+      }
+      bits |= 1ull << (idx & 63);
+    }
+    if (bits)
+      atomicOr(&mask[word], (unsigned long long)bits);
+  }
 }
 
 // Candidate nodes -> centres (exodus_finder.c:386-405 for a centre on its
@@ -358,13 +414,23 @@ int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   return SIF_OK;
 }
 
-int grown_overlap(const float* cx, const float* cy, const float* cz, const float* radius,
-  uint32_t n, const VoidIndex& V, float max_r, float box, float f, float scale,
-  uint32_t* state) {
+int grown_overlap(const float* cx, const float* cy, const float* cz, const uint32_t* flat,
+  const float* radius, uint32_t n, const VoidIndex& V, float max_r, float box, float f,
+  float scale, const uint64_t* mask, uint32_t grid_n, float l, uint32_t* state) {
   if (n == 0)
     return SIF_OK;
   grown_overlap_kernel<<<(n + GO_WAVES - 1) / GO_WAVES, 64 * GO_WAVES, 0, stream()>>>(cx, cy,
-    cz, radius, n, V, max_r, box, f, scale, state);
+    cz, flat, radius, n, V, max_r, box, f, scale, mask, grid_n, l, state);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+int paint_voids(const float* vx, const float* vy, const float* vz, const float* vr,
+  uint32_t nv, float scale, float l, uint32_t grid_n, uint64_t* mask) {
+  if (nv == 0)
+    return SIF_OK;
+  paint_voids_kernel<<<nv, 256, 0, stream()>>>(vx, vy, vz, vr, nv, scale, l, grid_n,
+    reinterpret_cast<unsigned long long*>(mask));
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
@@ -374,6 +440,25 @@ int grown_overlap(const float* cx, const float* cy, const float* cz, const float
 using namespace sif_amd;
 
 extern "C" {
+
+int sif_amd_mark_spheres(uint64_t* d_mask_words, uint32_t n_cells, float box,
+  const float* d_cx, const float* d_cy, const float* d_cz, const float* d_r,
+  uint32_t n_voids, float scale) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!d_mask_words || n_cells == 0 || !(box > 0.0f) ||
+      (n_voids && (!d_cx || !d_cy || !d_cz || !d_r))) {
+    error_set(SIF_ERR_INVALID, "sif_amd_mark_spheres: invalid arguments");
+    return SIF_ERR_INVALID;
+  }
+  st = paint_voids(d_cx, d_cy, d_cz, d_r, n_voids, scale, box / (float)n_cells, n_cells,
+    d_mask_words);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
 
 int sif_amd_template(uint32_t mesh_cells, float box, float rmin, float r_search,
   int32_t* cells4, float* bounds2, uint32_t capacity, uint32_t* count) {

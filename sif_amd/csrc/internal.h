@@ -74,9 +74,15 @@ int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
 int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
   const VoidIndex& V, uint32_t* keep, uint64_t counts_out[3]);
-int grown_overlap(const float* cx, const float* cy, const float* cz, const float* radius,
-  uint32_t n, const VoidIndex& V, float max_r, float box, float f, float scale,
-  uint32_t* state);
+// Per grown sphere: its overlap state against the index (bits 0-1) and, in
+// GROWN_POLE_BIT, whether the pole probe at the grown radius hits the mask.
+constexpr uint32_t GROWN_POLE_BIT = 16u;
+int grown_overlap(const float* cx, const float* cy, const float* cz, const uint32_t* flat,
+  const float* radius, uint32_t n, const VoidIndex& V, float max_r, float box, float f,
+  float scale, const uint64_t* mask, uint32_t grid_n, float l, uint32_t* state);
+// sif__mark_sphere of nv voids (radius times scale) into the device mask.
+int paint_voids(const float* vx, const float* vy, const float* vz, const float* vr,
+  uint32_t nv, float scale, float l, uint32_t grid_n, uint64_t* mask);
 
 // primitives.hip compaction helper used by the driver
 int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,

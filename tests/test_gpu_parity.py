@@ -317,3 +317,34 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
             rs = max(rs, float(radii[k - 1]))
         want = oracle.brute_force_radius(x, y, z, BOX, (cx, cy, cz), rung, rs, THRESH)
         assert abs(r - want) <= 1e-5 * want
+
+
+@pytest.mark.parametrize("n,box,nodes", [(64, 100.0, True), (100, 333.3, False)])
+def test_device_painting_matches_reference_bits(native, n, box, nodes):
+    """sif__mark_sphere (utils.c:965-1033) on the device, bit for bit
+    against the oracle's node-by-node restatement."""
+    import ctypes as C
+    import oracle
+    from sif_amd import device as D
+    from sif_amd._native import DeviceArray
+    rng = np.random.default_rng(n)
+    l = np.float32(box) / np.float32(n)
+    k = 200
+    cen = (rng.integers(0, n, (k, 3)).astype(np.float32) * l if nodes
+           else rng.uniform(0, box, (k, 3)).astype(np.float32))
+    r = np.concatenate([rng.uniform(0.3, 3, k // 2) * l,
+                        rng.uniform(3, 20, k - k // 2) * l]).astype(np.float32)
+    scale = np.float32(np.cbrt(np.float32(0.2)))
+    words = (n ** 3 + 63) // 64
+    dm = DeviceArray((words,), np.uint64)
+    native.check(native.lib.sif_amd_memset(dm.ptr, 0, words * 8))
+    dx, dy, dz, dr = (D.to_device(np.ascontiguousarray(a)) for a in
+                      (cen[:, 0], cen[:, 1], cen[:, 2], r))
+    native.check(native.lib.sif_amd_mark_spheres(dm.ptr, n, box, dx.ptr, dy.ptr, dz.ptr, dr.ptr,
+                                                 k, scale))
+    want = np.zeros(words, np.uint64)
+    for (cx, cy, cz), rr in zip(cen, r):
+        oracle.lib.orc_mark_sphere(C.c_void_p(want.ctypes.data), C.c_float(cx), C.c_float(cy),
+                                   C.c_float(cz), C.c_float(np.float32(scale * rr)),
+                                   C.c_uint32(n), C.c_float(l))
+    np.testing.assert_array_equal(dm.get(), want)
