@@ -33,9 +33,11 @@ SIF_API int sif_amd_synchronize(void);
 /* Per-stage device time (HIP events on the library stream, recorded without
  * synchronising and resolved when read). Stage order:
  * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit,
- * retest, sort. commit and retest are host time: the ordered accept loop,
- * and the re-tests that assemble each growth batch. */
-#define SIF_AMD_TIMING_STAGES 11
+ * retest, sort, host. commit, retest and host are host time: the ordered
+ * accept loop, the re-tests that assemble each growth batch, and the rest of
+ * the driver's own work per rung (the void index, the survivors' copy, the
+ * filter tables). */
+#define SIF_AMD_TIMING_STAGES 12
 SIF_API int sif_amd_timing_enable(int on);
 SIF_API void sif_amd_timing_reset(void);
 SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);

@@ -22,9 +22,9 @@ SIF_ERR_UNSUPPORTED = -5
 SIF_ERR_UNINITIALISED = -6
 
 FILTER_NONE, FILTER_TOP_HAT, FILTER_GAUSSIAN = 0, 1, 2
-N_STAGES = 11
+N_STAGES = 12
 STAGE_NAMES = ("cic", "contrast", "mesh", "fft_forward", "smooth", "scan",
-               "prefilter", "grow", "commit", "retest", "sort")
+               "prefilter", "grow", "commit", "retest", "sort", "host")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

@@ -68,5 +68,6 @@ enum {
   SIF_AMD_STAGE_COMMIT,
   SIF_AMD_STAGE_RETEST,
   SIF_AMD_STAGE_SORT,
+  SIF_AMD_STAGE_HOST,
   SIF_AMD_N_STAGES
 };

@@ -350,8 +350,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   const float sf = sif__finder_search_factor(R.opt);
 
   int st = exodus_prepare_spectrum(R.d_delta, n);
+  const double t_lut = now_ms();
   if (st == SIF_OK)
     st = exodus_prepare_luts(n, R.box, R.radii, R.opt);
+  if (timing_active())
+    timing_add(SIF_AMD_STAGE_HOST, now_ms() - t_lut);
   if (st != SIF_OK)
     return st;
   const uint64_t lut_size = filter_lut_size(n);
@@ -436,9 +439,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // verdicts do not depend on the order; neighbouring candidates share
     // void cells), then sort the survivors deepest first
     VoidIndex VI;
+    double th = now_ms();
     st = upload_voids(V, &VI);
     if (st != SIF_OK)
       break;
+    if (timing_active())
+      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
     RV.clear();
     uint64_t pf[3] = {0, 0, 0};
     timing_begin(SIF_AMD_STAGE_PREFILTER);
@@ -461,11 +467,14 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     S.masked += pf[0];
     S.proxy += pf[1];
     S.overlap += pf[2];
+    th = now_ms();
     surv.resize(ns);
     if (ns)
       SIF_HIP(hipMemcpyAsync(surv.data(), g_dev.surv.ptr, ns * 4, hipMemcpyDeviceToHost,
         stream()));
     SIF_HIP(hipStreamSynchronize(stream()));
+    if (timing_active())
+      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
 
     // Speculative batches, sequential commit. While the host commits one
     // batch, the GPU grows the next, assembled from the catalogue as it stood
@@ -664,9 +673,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       std::swap(cur, nxt);
     }
     // the rung's voids into the device mask, for the rungs below
+    th = now_ms();
     if (st == SIF_OK)
       st = paint_on_device(RV.cx.data(), RV.cy.data(), RV.cz.data(), RV.r.data(),
         (uint32_t)RV.cx.size(), scale, l, n);
+    if (timing_active())
+      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
     if (stats_out)
       stats_out[i] = S;
   }
