@@ -43,7 +43,7 @@ namespace {
 
 // Growth batches: the reference's adaptive rule (exodus_finder.c:2331-2349)
 // with a floor that fills the GPU (about 2048 wavefronts resident).
-constexpr uint32_t BATCH_FLOOR = 2048;
+constexpr uint32_t BATCH_FLOOR = 4096;
 constexpr uint32_t BATCH_MAX = 32768;
 
 // The accepted voids with their 32^3 cell list (cell_linked_list.c:41-59,

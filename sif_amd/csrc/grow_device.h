@@ -173,12 +173,14 @@ __device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
   *ez = cz;
 }
 
-constexpr int PARTICLE_UNROLL = 4;
+constexpr int PARTICLE_UNROLL = 8;
 
-// Walks the template 64 entries at a time. `want(entry, type, bound)` says
-// whether the entry is visited; `whole(type, count)` consumes a whole cell
-// without reading its particles (returns true when it did); `visit(d2, type)`
-// sees every particle of the remaining cells.
+// Walks the template 64 entries at a time. `want(type, bound)` says whether
+// an entry is visited; `whole(type, count)` consumes a whole cell without
+// reading its particles (returns true when it did); `visit(d2, type, valid)`
+// sees every particle of the remaining cells. visit is called by every lane
+// the same number of times -- `valid` false on a lane with no particle that
+// round -- so a visitor may use wave-wide operations.
 template <class Want, class Whole, class Visit>
 __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& q,
   WaveLDS& S, int lane, Want want, Whole whole, Visit visit) {
@@ -206,26 +208,22 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
     S.t_ez[lane] = ez;
     S.t_type[lane] = type;
     wave_sync();
-    // PARTICLE_UNROLL particles per lane per round, their loads issued
-    // together (the visit order does not matter to any caller)
+    // particle j of the chunk goes to lane j % 64; a lane's j only grows, so
+    // its entry (the first whose inclusive prefix exceeds j) is found by
+    // stepping forward from the last one. PARTICLE_UNROLL particles per lane
+    // per round, their loads issued together.
+    int ent = 0;
     for (uint32_t j0 = 0; j0 < total; j0 += 64 * PARTICLE_UNROLL) {
       uint32_t pi[PARTICLE_UNROLL];
-      int ent[PARTICLE_UNROLL];
+      int en[PARTICLE_UNROLL];
 #pragma unroll
       for (int u = 0; u < PARTICLE_UNROLL; u++) {
         const uint32_t j = j0 + (uint32_t)u * 64 + lane;
-        // first entry whose inclusive prefix exceeds j
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (S.t_scan[mid] > j)
-            hi = mid;
-          else
-            lo = mid + 1;
-        }
-        const uint32_t before = lo ? S.t_scan[lo - 1] : 0u;
-        ent[u] = lo;
-        pi[u] = j < total ? S.t_start[lo] + (j - before) : 0xffffffffu;
+        while (ent < 63 && S.t_scan[ent] <= j)
+          ent++;
+        const uint32_t before = ent ? S.t_scan[ent - 1] : 0u;
+        en[u] = ent;
+        pi[u] = j < total ? S.t_start[ent] + (j - before) : 0xffffffffu;
       }
       float px[PARTICLE_UNROLL], py[PARTICLE_UNROLL], pz[PARTICLE_UNROLL];
 #pragma unroll
@@ -236,40 +234,96 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
         pz[u] = P.mz[p];
       }
 #pragma unroll
-      for (int u = 0; u < PARTICLE_UNROLL; u++)
-        if (pi[u] != 0xffffffffu) {
-          const int lo = ent[u];
-          visit(dist2(px[u], py[u], pz[u], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]), S.t_type[lo]);
-        }
+      for (int u = 0; u < PARTICLE_UNROLL; u++) {
+        const bool valid = pi[u] != 0xffffffffu;
+        const int lo = en[u];
+        const float d2 =
+          valid ? dist2(px[u], py[u], pz[u], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]) : 0.0f;
+        visit(d2, valid ? S.t_type[lo] : 0u, valid);
+      }
     }
     wave_sync();
   }
 }
 
-// Ascending bitonic sort of buf[0..n) by one wave; the next power of two
-// above n must fit the buffer.
-__device__ void wave_bitonic(float* buf, uint32_t n, int lane) {
-  uint32_t P2 = 1;
-  while (P2 < n)
-    P2 <<= 1;
-  for (uint32_t i = n + lane; i < P2; i += 64)
-    buf[i] = __int_as_float(0x7f800000);  // +inf pads sort last
-  wave_sync();
-  for (uint32_t k = 2; k <= P2; k <<= 1) {
+// Appends `d2` from every lane where `take` holds to buf[count ...] in lane
+// order (slots from a ballot, no shared counter); `count` is uniform. Slots
+// at or past `cap` are counted but not stored.
+__device__ __forceinline__ void wave_append(float* buf, uint32_t cap, uint32_t& count,
+  bool take, float d2, int lane) {
+  const uint64_t m = __ballot(take);
+  if (take) {
+    const uint32_t slot = count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < cap)
+      buf[slot] = d2;
+  }
+  count += (uint32_t)__popcll(m);
+}
+
+// Ascending bitonic sort of buf[0..n) in registers: element e = r * 64 + lane
+// is register r of the lane; partners less than 64 apart are exchanged by
+// shuffles, farther ones within the lane. The next power of two above n (at
+// least 64) is R * 64.
+template <int R>
+__device__ void wave_sort_regs(float* buf, uint32_t n, int lane) {
+  float v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t e = (uint32_t)r * 64 + lane;
+    v[r] = e < n ? buf[e] : __int_as_float(0x7f800000);  // +inf sorts last
+  }
+#pragma unroll
+  for (uint32_t k = 2; k <= (uint32_t)R * 64; k <<= 1) {
+#pragma unroll
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t t = lane; t < P2 / 2; t += 64) {
-        const uint32_t lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-        const uint32_t hi = lo + j;
-        const bool asc = (lo & k) == 0;
-        const float a = buf[lo], b = buf[hi];
-        if ((a > b) == asc) {
-          buf[lo] = b;
-          buf[hi] = a;
+      if (j >= 64) {
+        const int jr = (int)(j / 64);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if ((r & jr) == 0) {
+            const bool asc = ((((uint32_t)r * 64 + lane) & k) == 0);
+            const float a = v[r], b = v[r | jr];
+            const bool swap = (a > b) == asc;
+            v[r] = swap ? b : a;
+            v[r | jr] = swap ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint32_t e = (uint32_t)r * 64 + lane;
+          const float o = __shfl_xor(v[r], (int)j, 64);
+          const bool asc = (e & k) == 0, lower = (e & j) == 0;
+          const float mn = fminf(v[r], o), mx = fmaxf(v[r], o);
+          v[r] = (lower == asc) ? mn : mx;
         }
       }
-      wave_sync();
     }
   }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t e = (uint32_t)r * 64 + lane;
+    if (e < n)
+      buf[e] = v[r];
+  }
+}
+
+// Ascending sort of buf[0..n), n <= 2048, by one wave.
+__device__ __forceinline__ void wave_sort(float* buf, uint32_t n, int lane) {
+  wave_sync();
+  if (n <= 64)
+    wave_sort_regs<1>(buf, n, lane);
+  else if (n <= 128)
+    wave_sort_regs<2>(buf, n, lane);
+  else if (n <= 256)
+    wave_sort_regs<4>(buf, n, lane);
+  else if (n <= 512)
+    wave_sort_regs<8>(buf, n, lane);
+  else if (n <= 1024)
+    wave_sort_regs<16>(buf, n, lane);
+  else
+    wave_sort_regs<32>(buf, n, lane);
+  wave_sync();
 }
 
 __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
@@ -331,8 +385,7 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
       // reads the SHELL cells (counted whole by the reference) and keeps every
       // annulus distance in LDS while they fit.
       uint64_t within = 0, surface = 0, shell = 0, core = 0;
-      S.counter = 0;  // every lane stores the same zero
-      wave_sync();
+      uint32_t appended = 0;  // uniform
       for_particles(P, q, S, lane,
         [](uint32_t, float2) { return true; },
         [&](uint32_t type, uint32_t cnt) {
@@ -343,30 +396,28 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
           }
           return false;
         },
-        [&](float d2, uint32_t type) {
-          bool annulus;
-          if (type == CELL_SHELL) {
-            within++;
-            shell++;
-            annulus = true;
-          } else {
-            within += d2 <= rs2 ? 1u : 0u;
-            surface++;
-            core += d2 <= rc2 ? 1u : 0u;
-            annulus = d2 > rc2 && d2 <= rs2;
+        [&](float d2, uint32_t type, bool valid) {
+          bool annulus = false;
+          if (valid) {
+            if (type == CELL_SHELL) {
+              within++;
+              shell++;
+              annulus = true;
+            } else {
+              within += d2 <= rs2 ? 1u : 0u;
+              surface++;
+              core += d2 <= rc2 ? 1u : 0u;
+              annulus = d2 > rc2 && d2 <= rs2;
+            }
           }
-          if (annulus) {
-            const uint32_t slot = atomicAdd(&S.counter, 1u);
-            if (slot < CACHE_CAP)
-              S.cache[slot] = d2;
-          }
+          wave_append(S.cache, CACHE_CAP, appended, annulus, d2, lane);
         });
       wave_sync();
       within = uni(wave_sum_u64(within));
       surface = uni(wave_sum_u64(surface));
       shell = uni(wave_sum_u64(shell));
       core = uni(wave_sum_u64(core));
-      n_annulus = uni(S.counter);
+      n_annulus = uni(appended);
       if (within == 0) {
         reason = R_EMPTY;
       } else if (__fsub_rn(__fdiv_rn((float)within, expected), 1.0f) <= P.threshold) {
@@ -384,8 +435,20 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
       wave_sync();
       uint64_t n_core = 0, n_shell = 0;
       if (cached) {
-        for (uint32_t k = lane; k < n_annulus; k += 64)
-          atomicAdd(&S.bins[bin_of(S.cache[k], rc2, inv_w, nb)], 1u);
+        // sorted ascending, a bin is one run of the cache: its count is the
+        // distance from its first element to one past its last
+        wave_sort(S.cache, (uint32_t)n_annulus, lane);
+        for (uint32_t k = lane; k < n_annulus; k += 64) {
+          const uint32_t b = bin_of(S.cache[k], rc2, inv_w, nb);
+          if (k + 1 == n_annulus || bin_of(S.cache[k + 1], rc2, inv_w, nb) != b)
+            S.bins[b] = k + 1;
+        }
+        wave_sync();
+        for (uint32_t k = lane; k < n_annulus; k += 64) {
+          const uint32_t b = bin_of(S.cache[k], rc2, inv_w, nb);
+          if (k == 0 || bin_of(S.cache[k - 1], rc2, inv_w, nb) != b)
+            S.bins[b] -= k;
+        }
         if (lane == 0) {
           n_core = n_core_cached;
           n_shell = n_annulus;
@@ -399,7 +462,9 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
           }
           return false;
         },
-        [&](float d2, uint32_t type) {
+        [&](float d2, uint32_t type, bool valid) {
+          if (!valid)
+            return;
           if (type == CELL_SHELL) {
             atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
             n_shell++;
@@ -448,8 +513,6 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
           above += cnt;
         }
         wave_sync();
-        if (cached)  // ascending, so a window's bins are one run of the cache
-          wave_bitonic(S.cache, (uint32_t)n_annulus, lane);
 
         // 4. walk: next flagged bin below `limit`, resolve a window, repeat.
         int32_t limit = (int32_t)nb;
@@ -527,8 +590,8 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
                 return type != CELL_CORE && !(bd.y < lo) && !(bd.x > hi);
               },
               [](uint32_t, uint32_t) { return false; },
-              [&](float d2, uint32_t type) {
-                if (type != CELL_SHELL && !(d2 > rc2 && d2 <= rs2))
+              [&](float d2, uint32_t type, bool valid) {
+                if (!valid || (type != CELL_SHELL && !(d2 > rc2 && d2 <= rs2)))
                   return;
                 const uint32_t b = bin_of(d2, rc2, inv_w, nb);
                 if ((int32_t)b < blo || (int32_t)b > best)
@@ -551,36 +614,32 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
               reason = R_OVERFLOW;
               break;
             }
-            S.counter = 0;  // every lane stores the same zero
-            wave_sync();
+            uint32_t collected = 0;  // uniform
             for_particles(P, q, S, lane,
               [&](uint32_t type, float2 bd) {
                 return type != CELL_CORE && !(bd.y < lo) && !(bd.x > hi);
               },
               [](uint32_t, uint32_t) { return false; },
-              [&](float d2, uint32_t type) {
-                if (type != CELL_SHELL && !(d2 > rc2 && d2 <= rs2))
-                  return;
-                const uint32_t b = bin_of(d2, rc2, inv_w, nb);
-                if ((int32_t)b < blo || (int32_t)b > best)
-                  return;
-                if (split) {
+              [&](float d2, uint32_t type, bool valid) {
+                bool take = valid && (type == CELL_SHELL || (d2 > rc2 && d2 <= rs2));
+                if (take) {
+                  const uint32_t b = bin_of(d2, rc2, inv_w, nb);
+                  take = (int32_t)b >= blo && (int32_t)b <= best;
+                }
+                if (take && split) {
                   int32_t sb = (int32_t)__fmul_rn(__fsub_rn(d2, lo), inv_sub);
                   sb = sb < 0 ? 0 : (sb >= SUB_BINS ? SUB_BINS - 1 : sb);
-                  if (sb != s)
-                    return;
+                  take = sb == s;
                 }
-                const uint32_t slot = atomicAdd(&S.counter, 1u);
-                if (slot < (uint32_t)REFINE_CAP)
-                  S.buf[slot] = d2;
+                wave_append(S.buf, (uint32_t)REFINE_CAP, collected, take, d2, lane);
               });
             wave_sync();
-            const uint32_t got = uni(S.counter);
+            const uint32_t got = uni(collected);
             if (got != expect_cnt) {
               reason = R_OVERFLOW;  // counts disagree: never expected
               break;
             }
-            wave_bitonic(S.buf, got, lane);
+            wave_sort(S.buf, got, lane);
             // outside in: element k encloses top_n - got + k particles
             int found = -1;
             for (uint32_t k = lane; k < got; k += 64) {
