@@ -25,6 +25,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -128,10 +132,93 @@ struct Voids {
 
 struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
-  DeviceBuffer pvx, pvy, pvz, pvr;  // voids to paint into the mask
-  DeviceBuffer vcx, vcy, vcz, vr, vstart, vitems;
+  DeviceBuffer pf_counts;  // per rung: masked, proxy, overlap
 };
 DeviceState g_dev;
+
+// A fork-join pool for the host's independent per-candidate tests: run(n,
+// fn) calls fn(begin, end) over [0, n) in chunks from every worker and the
+// caller, and returns when all are done.
+class WorkPool {
+ public:
+  explicit WorkPool(unsigned workers) {
+    for (unsigned i = 0; i < workers; i++)
+      threads_.emplace_back([this]() { loop(); });
+  }
+  ~WorkPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread& t : threads_)
+      t.join();
+  }
+  void run(uint64_t n, const std::function<void(uint64_t, uint64_t)>& fn) {
+    if (threads_.empty() || n <= GRAIN) {
+      fn(0, n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &fn;
+      n_ = n;
+      next_.store(0);
+      active_ = (unsigned)threads_.size();
+      gen_++;
+    }
+    cv_.notify_all();
+    work(fn, n);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this]() { return active_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  static constexpr uint64_t GRAIN = 64;
+  void work(const std::function<void(uint64_t, uint64_t)>& fn, uint64_t n) {
+    for (;;) {
+      const uint64_t b = next_.fetch_add(GRAIN);
+      if (b >= n)
+        return;
+      fn(b, b + GRAIN < n ? b + GRAIN : n);
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(uint64_t, uint64_t)>* job;
+      uint64_t n;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
+        if (stop_)
+          return;
+        seen = gen_;
+        job = job_;
+        n = n_;
+      }
+      work(*job, n);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--active_ == 0)
+        done_.notify_one();
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(uint64_t, uint64_t)>* job_ = nullptr;
+  uint64_t n_ = 0;
+  std::atomic<uint64_t> next_{0};
+  unsigned active_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+WorkPool& host_pool() {
+  static WorkPool pool(std::min(std::max(std::thread::hardware_concurrency(), 2u) - 1u, 15u));
+  return pool;
+}
 
 // One growth batch in flight: device buffers, pinned host copies of its
 // inputs and results, and the event that marks the results in host memory.
@@ -189,86 +276,134 @@ double now_ms() {
     .count();
 }
 
-// Radius classes of the device index, by void-cell widths: a quarter, a
-// half, one, and more.
-uint32_t void_class(float r, float cell) {
-  return r < 0.25f * cell ? 0u : r < 0.5f * cell ? 1u : r < cell ? 2u : 3u;
-}
+// The catalogue on the device, appended rung by rung: the overlap index of the
+// next rung is built from it and the mask is painted from it.
+struct DeviceVoids {
+  DeviceBuffer x, y, z, r;
+  DeviceBuffer start, items;
+  uint32_t n = 0, cap = 0;
+  // pinned staging for an append
+  float* hx = nullptr;
+  float* hy = nullptr;
+  float* hz = nullptr;
+  float* hr = nullptr;
+  uint32_t hcap = 0;
+  float class_max[VOID_CLASSES] = {-1.0f, -1.0f, -1.0f, -1.0f};
+};
+DeviceVoids g_voids;
 
-int upload_voids(const Voids& V, VoidIndex* out) {
-  const uint32_t nv = (uint32_t)V.cx.size();
-  const uint32_t cells = VOID_CELLS * VOID_CELLS * VOID_CELLS;
-  const uint32_t slots = VOID_CLASSES * cells;
-  const float cell_w = V.box / (float)VOID_CELLS;
-  std::vector<uint32_t> start(slots + 1, 0), items(nv ? nv : 1), slot(nv ? nv : 1);
-  float cmax[VOID_CLASSES] = {-1.0f, -1.0f, -1.0f, -1.0f};
-  for (uint32_t i = 0; i < nv; i++) {
-    const uint32_t cls = void_class(V.r[i], cell_w);
-    slot[i] = cls * cells + V.cell_of(V.cx[i], V.cy[i], V.cz[i]);
-    start[slot[i] + 1]++;
-    if (V.r[i] > cmax[cls])
-      cmax[cls] = V.r[i];
-  }
-  for (uint32_t c = 0; c < slots; c++)
-    start[c + 1] += start[c];
-  std::vector<uint32_t> cur(start.begin(), start.end() - 1);
-  for (uint32_t i = 0; i < nv; i++)
-    items[cur[slot[i]]++] = i;
-  const size_t fb = (size_t)(nv ? nv : 1) * sizeof(float);
-  int st = g_dev.vcx.reserve(fb);
-  if (st == SIF_OK) st = g_dev.vcy.reserve(fb);
-  if (st == SIF_OK) st = g_dev.vcz.reserve(fb);
-  if (st == SIF_OK) st = g_dev.vr.reserve(fb);
-  if (st == SIF_OK) st = g_dev.vstart.reserve((slots + 1) * sizeof(uint32_t));
-  if (st == SIF_OK) st = g_dev.vitems.reserve(items.size() * sizeof(uint32_t));
+int grow_keep(DeviceBuffer& b, size_t used, size_t want) {
+  if (want <= b.bytes && b.ptr)
+    return SIF_OK;
+  DeviceBuffer nb;
+  int st = nb.reserve(want);
   if (st != SIF_OK)
     return st;
-  if (nv) {
-    SIF_HIP(hipMemcpyAsync(g_dev.vcx.ptr, V.cx.data(), nv * 4, hipMemcpyHostToDevice, stream()));
-    SIF_HIP(hipMemcpyAsync(g_dev.vcy.ptr, V.cy.data(), nv * 4, hipMemcpyHostToDevice, stream()));
-    SIF_HIP(hipMemcpyAsync(g_dev.vcz.ptr, V.cz.data(), nv * 4, hipMemcpyHostToDevice, stream()));
-    SIF_HIP(hipMemcpyAsync(g_dev.vr.ptr, V.r.data(), nv * 4, hipMemcpyHostToDevice, stream()));
-    SIF_HIP(hipMemcpyAsync(g_dev.vitems.ptr, items.data(), nv * 4, hipMemcpyHostToDevice, stream()));
-  }
-  SIF_HIP(hipMemcpyAsync(g_dev.vstart.ptr, start.data(), (slots + 1) * 4,
-    hipMemcpyHostToDevice, stream()));
+  if (used)
+    SIF_HIP(hipMemcpyAsync(nb.ptr, b.ptr, used, hipMemcpyDeviceToDevice, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
-  out->cx = g_dev.vcx.as<float>();
-  out->cy = g_dev.vcy.as<float>();
-  out->cz = g_dev.vcz.as<float>();
-  out->r = g_dev.vr.as<float>();
-  out->cell_start = g_dev.vstart.as<uint32_t>();
-  out->items = g_dev.vitems.as<uint32_t>();
-  out->n = nv;
-  out->inv_l = V.inv_l;
-  out->max_r = V.max_r;
-  for (int c = 0; c < VOID_CLASSES; c++)
-    out->class_max_r[c] = cmax[c];
+  b.release();
+  b.ptr = nb.ptr;
+  b.bytes = nb.bytes;
+  nb.ptr = nullptr;
+  nb.bytes = 0;
   return SIF_OK;
 }
 
-// Paints voids (host arrays) into the device mask.
-int paint_on_device(const float* x, const float* y, const float* z, const float* r,
-  uint32_t nv, float scale, float l, uint32_t n) {
-  if (nv == 0)
+// Appends voids (host arrays) to the device catalogue and paints them into
+// the device mask with radius scale * r. The host arrays may change as soon
+// as this returns (they are staged in pinned memory; the stream keeps order).
+int append_voids(const float* x, const float* y, const float* z, const float* r, uint32_t k,
+  float box, float scale, float l, uint32_t grid_n) {
+  if (k == 0)
     return SIF_OK;
-  const size_t fb = (size_t)nv * sizeof(float);
-  int st = g_dev.pvx.reserve(fb);
-  if (st == SIF_OK) st = g_dev.pvy.reserve(fb);
-  if (st == SIF_OK) st = g_dev.pvz.reserve(fb);
-  if (st == SIF_OK) st = g_dev.pvr.reserve(fb);
-  if (st != SIF_OK)
-    return st;
-  SIF_HIP(hipMemcpyAsync(g_dev.pvx.ptr, x, fb, hipMemcpyHostToDevice, stream()));
-  SIF_HIP(hipMemcpyAsync(g_dev.pvy.ptr, y, fb, hipMemcpyHostToDevice, stream()));
-  SIF_HIP(hipMemcpyAsync(g_dev.pvz.ptr, z, fb, hipMemcpyHostToDevice, stream()));
-  SIF_HIP(hipMemcpyAsync(g_dev.pvr.ptr, r, fb, hipMemcpyHostToDevice, stream()));
-  st = paint_voids(g_dev.pvx.as<float>(), g_dev.pvy.as<float>(), g_dev.pvz.as<float>(),
-    g_dev.pvr.as<float>(), nv, scale, l, n, g_dev.mask.as<uint64_t>());
-  if (st != SIF_OK)
-    return st;
-  // the host arrays may change once this returns
+  DeviceVoids& D = g_voids;
+  if (D.n + k > D.cap) {
+    uint32_t want = D.cap ? D.cap : 65536;
+    while (want < D.n + k)
+      want *= 2;
+    int st = grow_keep(D.x, (size_t)D.n * 4, (size_t)want * 4);
+    if (st == SIF_OK) st = grow_keep(D.y, (size_t)D.n * 4, (size_t)want * 4);
+    if (st == SIF_OK) st = grow_keep(D.z, (size_t)D.n * 4, (size_t)want * 4);
+    if (st == SIF_OK) st = grow_keep(D.r, (size_t)D.n * 4, (size_t)want * 4);
+    if (st == SIF_OK) st = D.items.reserve((size_t)want * 4);
+    if (st != SIF_OK)
+      return st;
+    D.cap = want;
+  }
+  // the previous append's copies read the staging: let them finish
   SIF_HIP(hipStreamSynchronize(stream()));
+  if (k > D.hcap) {
+    for (float** p : {&D.hx, &D.hy, &D.hz, &D.hr})
+      if (*p) {
+        (void)hipHostFree(*p);
+        *p = nullptr;
+      }
+    uint32_t want = D.hcap ? D.hcap : 16384;
+    while (want < k)
+      want *= 2;
+    for (float** p : {&D.hx, &D.hy, &D.hz, &D.hr})
+      SIF_HIP(hipHostMalloc((void**)p, (size_t)want * 4, hipHostMallocDefault));
+    D.hcap = want;
+  }
+  memcpy(D.hx, x, (size_t)k * 4);
+  memcpy(D.hy, y, (size_t)k * 4);
+  memcpy(D.hz, z, (size_t)k * 4);
+  memcpy(D.hr, r, (size_t)k * 4);
+  SIF_HIP(hipMemcpyAsync(D.x.as<float>() + D.n, D.hx, (size_t)k * 4, hipMemcpyHostToDevice,
+    stream()));
+  SIF_HIP(hipMemcpyAsync(D.y.as<float>() + D.n, D.hy, (size_t)k * 4, hipMemcpyHostToDevice,
+    stream()));
+  SIF_HIP(hipMemcpyAsync(D.z.as<float>() + D.n, D.hz, (size_t)k * 4, hipMemcpyHostToDevice,
+    stream()));
+  SIF_HIP(hipMemcpyAsync(D.r.as<float>() + D.n, D.hr, (size_t)k * 4, hipMemcpyHostToDevice,
+    stream()));
+  const float cell_w = box / (float)VOID_CELLS;
+  for (uint32_t i = 0; i < k; i++) {
+    const uint32_t cls = r[i] < 0.25f * cell_w ? 0u
+                         : r[i] < 0.5f * cell_w ? 1u
+                         : r[i] < cell_w        ? 2u
+                                                : 3u;
+    if (r[i] > D.class_max[cls])
+      D.class_max[cls] = r[i];
+  }
+  int st = paint_voids(D.x.as<float>() + D.n, D.y.as<float>() + D.n, D.z.as<float>() + D.n,
+    D.r.as<float>() + D.n, k, scale, l, grid_n, g_dev.mask.as<uint64_t>());
+  if (st != SIF_OK)
+    return st;
+  D.n += k;
+  return SIF_OK;
+}
+
+// The overlap index of the catalogue on the device, for a rung to begin.
+int device_void_index(const Voids& V, VoidIndex* out) {
+  DeviceVoids& D = g_voids;
+  const uint32_t slots = VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+  int st = D.start.reserve((size_t)(slots + 1) * 4);
+  if (st == SIF_OK)
+    st = D.items.reserve((size_t)(D.cap ? D.cap : 1) * 4);
+  if (st == SIF_OK && D.n == 0) {  // keep the pointers valid
+    st = D.x.reserve(4);
+    if (st == SIF_OK) st = D.y.reserve(4);
+    if (st == SIF_OK) st = D.z.reserve(4);
+    if (st == SIF_OK) st = D.r.reserve(4);
+  }
+  if (st == SIF_OK)
+    st = build_void_index(D.x.as<float>(), D.y.as<float>(), D.z.as<float>(), D.r.as<float>(),
+      D.n, V.box, D.start.as<uint32_t>(), D.items.as<uint32_t>());
+  if (st != SIF_OK)
+    return st;
+  out->cx = D.x.as<float>();
+  out->cy = D.y.as<float>();
+  out->cz = D.z.as<float>();
+  out->r = D.r.as<float>();
+  out->cell_start = D.start.as<uint32_t>();
+  out->items = D.items.as<uint32_t>();
+  out->n = D.n;
+  out->inv_l = V.inv_l;
+  out->max_r = V.max_r;
+  for (int c = 0; c < VOID_CLASSES; c++)
+    out->class_max_r[c] = D.class_max[c];
   return SIF_OK;
 }
 
@@ -373,11 +508,19 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipMemsetAsync(g_dev.mask.ptr, 0, (total + 63) / 64 * sizeof(uint64_t), stream()));
+  g_voids.n = 0;
+  for (int c = 0; c < VOID_CLASSES; c++)
+    g_voids.class_max[c] = -1.0f;
+  const size_t n_rungs = R.radii.size();
+  st = g_dev.pf_counts.reserve(n_rungs * 3 * sizeof(uint64_t));
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemsetAsync(g_dev.pf_counts.ptr, 0, n_rungs * 3 * sizeof(uint64_t), stream()));
   if (R.prior && R.prior->n_voids) {
     for (uint64_t v = 0; v < R.prior->n_voids; v++)
       V.add(R.prior->cx[v], R.prior->cy[v], R.prior->cz[v], R.prior->radii[v], 0);
-    st = paint_on_device(R.prior->cx, R.prior->cy, R.prior->cz, R.prior->radii,
-      (uint32_t)R.prior->n_voids, scale, l, n);
+    st = append_voids(R.prior->cx, R.prior->cy, R.prior->cz, R.prior->radii,
+      (uint32_t)R.prior->n_voids, R.box, scale, l, n);
     if (st != SIF_OK)
       return st;
   }
@@ -440,33 +583,27 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // void cells), then sort the survivors deepest first
     VoidIndex VI;
     double th = now_ms();
-    st = upload_voids(V, &VI);
+    st = device_void_index(V, &VI);
     if (st != SIF_OK)
       break;
     if (timing_active())
       timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
     RV.clear();
-    uint64_t pf[3] = {0, 0, 0};
     timing_begin(SIF_AMD_STAGE_PREFILTER);
     st = prefilter(g_dev.flat.as<uint32_t>(), (uint32_t)nc, n, l, R.box,
-      g_dev.mask.as<uint64_t>(), proxy, radius, R.f, scale, VI, g_dev.keep.as<uint32_t>(), pf);
+      g_dev.mask.as<uint64_t>(), proxy, radius, R.f, scale, VI, g_dev.keep.as<uint32_t>(),
+      g_dev.pf_counts.as<uint64_t>() + 3 * i);
     if (st != SIF_OK)
       break;
-    uint64_t ns = 0, nk = 0;
-    st = compact_u32(g_dev.flat.as<uint32_t>(), g_dev.keep.as<uint32_t>(), nc,
-      g_dev.surv.as<uint32_t>(), &ns);
-    if (st == SIF_OK)
-      st = compact_u32(g_dev.keys.as<uint32_t>(), g_dev.keep.as<uint32_t>(), nc,
-        g_dev.skeys.as<uint32_t>(), &nk);
+    uint64_t ns = 0;
+    st = compact2_u32(g_dev.flat.as<uint32_t>(), g_dev.keys.as<uint32_t>(),
+      g_dev.keep.as<uint32_t>(), nc, g_dev.surv.as<uint32_t>(), g_dev.skeys.as<uint32_t>(), &ns);
     if (st != SIF_OK)
       break;
     timing_end(SIF_AMD_STAGE_PREFILTER);
     st = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), ns);
     if (st != SIF_OK)
       break;
-    S.masked += pf[0];
-    S.proxy += pf[1];
-    S.overlap += pf[2];
     th = now_ms();
     surv.resize(ns);
     if (ns)
@@ -529,22 +666,34 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       else
         S.overlap++;
     };
-    // the next `cap` survivors that pass the rung tests now
+    // The next `cap` survivors that pass the rung tests now. The tests of a
+    // block of survivors are independent (nothing is accepted meanwhile), so
+    // they run on the host pool; the results are taken in order, and those
+    // past the last candidate taken are tested again next time.
+    std::vector<uint8_t> why_block;
     auto assemble = [&](BatchSlot& B) {
       const double t0 = now_ms();
       B.count = 0;
       while (k < ns && B.count < cap) {
-        const uint32_t flat = surv[k] & 0x7fffffffu;
-        const bool edge = (surv[k] >> 31) != 0;
-        k++;
-        const int why = rung_tests(flat, edge);
-        if (why) {
-          count_refusal(why);
-        } else {
-          B.h_flat[B.count] = flat;
-          B.edge[B.count] = edge ? 1 : 0;
-          B.count++;
+        const uint64_t m = std::min<uint64_t>(ns - k, std::max<uint64_t>(2048, 2 * (cap - B.count)));
+        why_block.resize(m);
+        const uint64_t k0 = k;
+        host_pool().run(m, [&](uint64_t a, uint64_t b) {
+          for (uint64_t t = a; t < b; t++)
+            why_block[t] =
+              (uint8_t)rung_tests(surv[k0 + t] & 0x7fffffffu, (surv[k0 + t] >> 31) != 0);
+        });
+        uint64_t t = 0;
+        for (; t < m && B.count < cap; t++) {
+          if (why_block[t]) {
+            count_refusal(why_block[t]);
+          } else {
+            B.h_flat[B.count] = surv[k0 + t] & 0x7fffffffu;
+            B.edge[B.count] = (surv[k0 + t] >> 31) != 0 ? 1 : 0;
+            B.count++;
+          }
         }
+        k = k0 + t;
       }
       if (timing_active())
         timing_add(SIF_AMD_STAGE_RETEST, now_ms() - t0);
@@ -675,8 +824,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // the rung's voids into the device mask, for the rungs below
     th = now_ms();
     if (st == SIF_OK)
-      st = paint_on_device(RV.cx.data(), RV.cy.data(), RV.cz.data(), RV.r.data(),
-        (uint32_t)RV.cx.size(), scale, l, n);
+      st = append_voids(RV.cx.data(), RV.cy.data(), RV.cz.data(), RV.r.data(),
+        (uint32_t)RV.cx.size(), R.box, scale, l, n);
     if (timing_active())
       timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
     if (stats_out)
@@ -691,6 +840,16 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   if (st != SIF_OK)
     return st;
   SIF_HIP(drained);
+  if (stats_out) {  // the prefilter's refusals, counted on the device
+    std::vector<uint64_t> pf(n_rungs * 3);
+    SIF_HIP(hipMemcpy(pf.data(), g_dev.pf_counts.ptr, pf.size() * sizeof(uint64_t),
+      hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < n_rungs; r++) {
+      stats_out[r].masked += pf[3 * r];
+      stats_out[r].proxy += pf[3 * r + 1];
+      stats_out[r].overlap += pf[3 * r + 2];
+    }
+  }
 
   const uint64_t nv = V.cx.size();
   sif_catalogue_t* cat = sif_catalogue_alloc(nv ? nv : 1);

@@ -25,6 +25,7 @@
 
 #include "common.h"
 #include "internal.h"
+#include "primitives.h"
 #include "sif_amd.h"
 #include "grow_device.h"
 #include "overlap.h"
@@ -397,20 +398,84 @@ int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
 
 int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
-  const VoidIndex& V, uint32_t* keep, uint64_t counts_out[3]) {
-  int st = g_pf_counts.reserve(3 * sizeof(unsigned long long));
-  if (st != SIF_OK)
-    return st;
-  SIF_HIP(hipMemsetAsync(g_pf_counts.ptr, 0, 3 * sizeof(unsigned long long), stream()));
+  const VoidIndex& V, uint32_t* keep, uint64_t* d_counts) {
   if (n)
     prefilter_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(flat, n, grid_n, l, box,
-      mask, proxy, radius, f, scale, V, keep, g_pf_counts.as<unsigned long long>());
+      mask, proxy, radius, f, scale, V, keep, reinterpret_cast<unsigned long long*>(d_counts));
   SIF_HIP(hipGetLastError());
-  unsigned long long c[3];
-  SIF_HIP(hipMemcpyAsync(c, g_pf_counts.ptr, sizeof(c), hipMemcpyDeviceToHost, stream()));
-  SIF_HIP(hipStreamSynchronize(stream()));
-  for (int i = 0; i < 3; i++)
-    counts_out[i] = c[i];
+  return SIF_OK;
+}
+
+// --- the device void index: class-split CSR over the reference's 32^3 cells ---
+
+__global__ void void_slot_kernel(const float* __restrict__ vx, const float* __restrict__ vy,
+  const float* __restrict__ vz, const float* __restrict__ vr, uint32_t nv, float inv_l,
+  float cell_w, uint32_t* __restrict__ slot, uint32_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv)
+    return;
+  const float r = vr[i];
+  const uint32_t cls = r < 0.25f * cell_w ? 0u : r < 0.5f * cell_w ? 1u : r < cell_w ? 2u : 3u;
+  const uint32_t cell =
+    (void_axis_cell(vx[i], inv_l) * VOID_CELLS + void_axis_cell(vy[i], inv_l)) * VOID_CELLS +
+    void_axis_cell(vz[i], inv_l);
+  const uint32_t sl = cls * VOID_CELLS * VOID_CELLS * VOID_CELLS + cell;
+  slot[i] = sl;
+  atomicAdd(&counts[sl], 1u);
+}
+
+__global__ void void_start_kernel(const uint64_t* __restrict__ pos, uint32_t slots,
+  uint32_t* __restrict__ start, uint32_t* __restrict__ cursor) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= slots)
+    start[s] = (uint32_t)pos[s];
+  if (s < slots)
+    cursor[s] = 0;
+}
+
+// Within a cell the order of the voids is not fixed; nothing that reads the
+// index depends on it (an overlap state is a yes/no over a set).
+__global__ void void_scatter_kernel(const uint32_t* __restrict__ slot, uint32_t nv,
+  const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+  uint32_t* __restrict__ items) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv)
+    return;
+  const uint32_t sl = slot[i];
+  items[start[sl] + atomicAdd(&cursor[sl], 1u)] = i;
+}
+
+namespace {
+DeviceBuffer g_vi_slot, g_vi_counts, g_vi_pos, g_vi_cursor;
+}
+
+int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
+  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items) {
+  const uint32_t slots = VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+  int st = g_vi_slot.reserve((size_t)(nv ? nv : 1) * 4);
+  if (st == SIF_OK) st = g_vi_counts.reserve((size_t)(slots + 1) * 4);
+  if (st == SIF_OK) st = g_vi_pos.reserve((size_t)(slots + 1) * 8);
+  if (st == SIF_OK) st = g_vi_cursor.reserve((size_t)slots * 4);
+  if (st != SIF_OK)
+    return st;
+  // counts[slots] stays 0 so that the exclusive scan's last entry is nv
+  SIF_HIP(hipMemsetAsync(g_vi_counts.ptr, 0, (size_t)(slots + 1) * 4, stream()));
+  if (nv)
+    void_slot_kernel<<<grid_for(nv, 256), 256, 0, stream()>>>(vx, vy, vz, vr, nv,
+      (float)VOID_CELLS / box, box / (float)VOID_CELLS, g_vi_slot.as<uint32_t>(),
+      g_vi_counts.as<uint32_t>());
+  SIF_HIP(hipGetLastError());
+  st = exclusive_scan_u32(g_vi_counts.as<uint32_t>(), slots + 1, g_vi_pos.as<uint64_t>(),
+    nullptr);
+  if (st != SIF_OK)
+    return st;
+  void_start_kernel<<<grid_for(slots + 1, 256), 256, 0, stream()>>>(g_vi_pos.as<uint64_t>(),
+    slots, d_start, g_vi_cursor.as<uint32_t>());
+  SIF_HIP(hipGetLastError());
+  if (nv)
+    void_scatter_kernel<<<grid_for(nv, 256), 256, 0, stream()>>>(g_vi_slot.as<uint32_t>(), nv,
+      d_start, g_vi_cursor.as<uint32_t>(), d_items);
+  SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
 

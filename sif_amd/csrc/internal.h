@@ -71,9 +71,14 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   uint64_t* algo_bytes = nullptr);
 int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
   float* cx, float* cy, float* cz);
+// keep[k] and the refusal counts (masked, proxy, overlap) added to d_counts.
 int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
-  const VoidIndex& V, uint32_t* keep, uint64_t counts_out[3]);
+  const VoidIndex& V, uint32_t* keep, uint64_t* d_counts);
+// The class-split CSR of nv device-resident voids: d_start has
+// VOID_CLASSES * 32^3 + 1 entries, d_items nv.
+int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
+  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items);
 // Per grown sphere: its overlap state against the index (bits 0-1) and, in
 // GROWN_POLE_BIT, whether the pole probe at the grown radius hits the mask.
 constexpr uint32_t GROWN_POLE_BIT = 16u;
@@ -84,9 +89,12 @@ int grown_overlap(const float* cx, const float* cy, const float* cz, const uint3
 int paint_voids(const float* vx, const float* vy, const float* vz, const float* vr,
   uint32_t nv, float scale, float l, uint32_t grid_n, uint64_t* mask);
 
-// primitives.hip compaction helper used by the driver
+// primitives.hip compaction helpers used by the driver
 int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,
   uint32_t* out, uint64_t* count);
+// Two arrays compacted by the same keep flags with one scan.
+int compact2_u32(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uint64_t n,
+  uint32_t* out_a, uint32_t* out_b, uint64_t* count);
 
 // One exodus run on device-resident data (exodus.hip).
 struct RunSpec {

@@ -284,6 +284,35 @@ namespace {
 DeviceBuffer g_cpos, g_ccount;
 }
 
+__global__ void compact2_scatter(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+  const uint32_t* __restrict__ keep, const uint64_t* __restrict__ pos, uint64_t n,
+  uint32_t* __restrict__ out_a, uint32_t* __restrict__ out_b) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && keep[i]) {
+    out_a[pos[i]] = a[i];
+    out_b[pos[i]] = b[i];
+  }
+}
+
+int compact2_u32(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uint64_t n,
+  uint32_t* out_a, uint32_t* out_b, uint64_t* count) {
+  int st = g_cpos.reserve((n ? n : 1) * 8);
+  if (st == SIF_OK)
+    st = g_ccount.reserve(8);
+  if (st != SIF_OK)
+    return st;
+  st = exclusive_scan_u32(keep, n, g_cpos.as<uint64_t>(), g_ccount.as<uint64_t>());
+  if (st != SIF_OK)
+    return st;
+  if (n)
+    compact2_scatter<<<grid_for(n, 256), 256, 0, stream()>>>(a, b, keep,
+      g_cpos.as<uint64_t>(), n, out_a, out_b);
+  SIF_HIP(hipGetLastError());
+  SIF_HIP(hipMemcpyAsync(count, g_ccount.ptr, 8, hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
 int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,
   uint32_t* out, uint64_t* count) {
   int st = g_cpos.reserve((n ? n : 1) * 8);
