@@ -14,6 +14,7 @@
 // the two differ by float rounding, within 1e-5 of the peak.)
 #include "common.h"
 #include "internal.h"
+#include "primitives.h"
 #include "sif_amd.h"
 
 namespace sif_amd {
@@ -35,44 +36,154 @@ __device__ __forceinline__ int64_t to_fixed(float v) {
   return (int64_t)__double2ll_rn((double)v * 4294967296.0);
 }
 
-// One thread per particle. Positions are read once (12 B, +4 with weights);
-// the eight fixed-point values go straight to the accumulators.
-__global__ void __launch_bounds__(256) cic_deposit_kernel(const float* __restrict__ xs,
-  const float* __restrict__ ys, const float* __restrict__ zs,
-  const float* __restrict__ ws, uint64_t n, uint32_t N, float idx, float box,
-  int64_t* __restrict__ acc) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += stride) {
-    const float cx = box_fold(xs[p], box) * idx;
-    const float cy = box_fold(ys[p], box) * idx;
-    const float cz = box_fold(zs[p], box) * idx;
-    const uint32_t rx = (uint32_t)cx, ry = (uint32_t)cy, rz = (uint32_t)cz;
-    const uint32_t ix = rx >= N ? 0u : rx;
-    const uint32_t iy = ry >= N ? 0u : ry;
-    const uint32_t iz = rz >= N ? 0u : rz;
-    const uint32_t ix1 = ix + 1 == N ? 0u : ix + 1;
-    const uint32_t iy1 = iy + 1 == N ? 0u : iy + 1;
-    const uint32_t iz1 = iz + 1 == N ? 0u : iz + 1;
-    const float tx = cx - (float)rx, ty = cy - (float)ry, tz = cz - (float)rz;
-    const float wx0 = 1.0f - tx, wx1 = tx;
-    const float wy0 = 1.0f - ty, wy1 = ty;
-    const float wz0 = 1.0f - tz, wz1 = tz;
-    const float w00 = wx0 * wy0, w01 = wx0 * wy1;
-    const float w10 = wx1 * wy0, w11 = wx1 * wy1;
-    const float pw = ws ? ws[p] : 1.0f;
-    const float mz0 = pw * wz0, mz1 = pw * wz1;
-    const uint64_t NN = (uint64_t)N * N;
-    const uint64_t a0 = (uint64_t)ix * NN, a1 = (uint64_t)ix1 * NN;
-    const uint64_t b0 = (uint64_t)iy * N, b1 = (uint64_t)iy1 * N;
-    atomic_add_i64(acc + a0 + b0 + iz, to_fixed(w00 * mz0));
-    atomic_add_i64(acc + a0 + b0 + iz1, to_fixed(w00 * mz1));
-    atomic_add_i64(acc + a0 + b1 + iz, to_fixed(w01 * mz0));
-    atomic_add_i64(acc + a0 + b1 + iz1, to_fixed(w01 * mz1));
-    atomic_add_i64(acc + a1 + b0 + iz, to_fixed(w10 * mz0));
-    atomic_add_i64(acc + a1 + b0 + iz1, to_fixed(w10 * mz1));
-    atomic_add_i64(acc + a1 + b1 + iz, to_fixed(w11 * mz0));
-    atomic_add_i64(acc + a1 + b1 + iz1, to_fixed(w11 * mz1));
+// The CIC footprint of one particle in the reference's float operations
+// (grid.c:585-650): base node (ix, iy, iz) of the scaled, folded position
+// c = fold(x) * n / L, and the eight corner values as 2^-32 fixed point, in
+// corner order (dx, dy, dz) = 000, 001, 010, 011, 100, 101, 110, 111.
+__device__ __forceinline__ void cic_corners(float cx, float cy, float cz, float pw, uint32_t N,
+  uint32_t* ix, uint32_t* iy, uint32_t* iz, int64_t v[8]) {
+  const uint32_t rx = (uint32_t)cx, ry = (uint32_t)cy, rz = (uint32_t)cz;
+  *ix = rx >= N ? 0u : rx;
+  *iy = ry >= N ? 0u : ry;
+  *iz = rz >= N ? 0u : rz;
+  const float tx = cx - (float)rx, ty = cy - (float)ry, tz = cz - (float)rz;
+  const float wx0 = 1.0f - tx, wx1 = tx;
+  const float wy0 = 1.0f - ty, wy1 = ty;
+  const float wz0 = 1.0f - tz, wz1 = tz;
+  const float w00 = wx0 * wy0, w01 = wx0 * wy1;
+  const float w10 = wx1 * wy0, w11 = wx1 * wy1;
+  const float mz0 = pw * wz0, mz1 = pw * wz1;
+  v[0] = to_fixed(w00 * mz0);
+  v[1] = to_fixed(w00 * mz1);
+  v[2] = to_fixed(w01 * mz0);
+  v[3] = to_fixed(w01 * mz1);
+  v[4] = to_fixed(w10 * mz0);
+  v[5] = to_fixed(w10 * mz1);
+  v[6] = to_fixed(w11 * mz0);
+  v[7] = to_fixed(w11 * mz1);
+}
+
+// The deposit is binned by tiles of CIC_T^3 nodes. A particle belongs to the
+// tile of its base node; its eight corners land in that tile or in the next
+// node layer up, so a tile's particles are accumulated in LDS over
+// (CIC_T + 1)^3 nodes, and the tile is flushed with one 64-bit integer atomic
+// per node it touched. Integer sums do not depend on order: the grid is the
+// same bits for any launch shape or particle order.
+constexpr uint32_t CIC_T = 8;
+constexpr uint32_t CIC_L = CIC_T + 1;
+
+__device__ __forceinline__ uint32_t cic_tile(float cx, float cy, float cz, uint32_t N,
+  uint32_t nt) {
+  const uint32_t rx = (uint32_t)cx, ry = (uint32_t)cy, rz = (uint32_t)cz;
+  const uint32_t ix = rx >= N ? 0u : rx, iy = ry >= N ? 0u : ry, iz = rz >= N ? 0u : rz;
+  return ((ix / CIC_T) * nt + iy / CIC_T) * nt + iz / CIC_T;
+}
+
+// Lanes holding the same key in a row (common: neighbouring particles share
+// a tile) form a run; returns the run's first lane and sets *len on it.
+__device__ __forceinline__ int key_run(uint32_t key, bool valid, int lane, uint32_t* len) {
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const uint64_t heads = __ballot(head);
+  const uint64_t live = __ballot(valid);
+  const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int first = 63 - __clzll((long long)(heads & upto));
+  if (head) {
+    const uint64_t later = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    const int end = later ? __ffsll((long long)later) - 1 : 64 - __clzll((long long)live);
+    *len = (uint32_t)(end - lane);
+  }
+  return first;
+}
+
+__global__ void __launch_bounds__(256) cic_count_kernel(const float* __restrict__ xs,
+  const float* __restrict__ ys, const float* __restrict__ zs, uint64_t n, uint32_t N,
+  float idx, float box, uint32_t nt, uint32_t* __restrict__ counts) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = p < n;
+  uint32_t key = 0xffffffffu;
+  if (valid)
+    key = cic_tile(box_fold(xs[p], box) * idx, box_fold(ys[p], box) * idx,
+      box_fold(zs[p], box) * idx, N, nt);
+  uint32_t len = 0;
+  const int first = key_run(key, valid, lane, &len);
+  if (valid && first == lane)
+    atomicAdd(&counts[key], len);
+}
+
+// Particles into tile order: the scaled, folded coordinates (and weights).
+__global__ void __launch_bounds__(256) cic_bin_kernel(const float* __restrict__ xs,
+  const float* __restrict__ ys, const float* __restrict__ zs, const float* __restrict__ ws,
+  uint64_t n, uint32_t N, float idx, float box, uint32_t nt,
+  const uint64_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+  float* __restrict__ bx, float* __restrict__ by, float* __restrict__ bz,
+  float* __restrict__ bw) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = p < n;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  uint32_t key = 0xffffffffu;
+  if (valid) {
+    cx = box_fold(xs[p], box) * idx;
+    cy = box_fold(ys[p], box) * idx;
+    cz = box_fold(zs[p], box) * idx;
+    key = cic_tile(cx, cy, cz, N, nt);
+  }
+  uint32_t len = 0;
+  const int first = key_run(key, valid, lane, &len);
+  uint32_t base = 0;
+  if (valid && first == lane)
+    base = atomicAdd(&cursor[key], len);
+  base = __shfl(base, first, 64);
+  if (valid) {
+    const uint64_t slot = offsets[key] + base + (uint32_t)(lane - first);
+    bx[slot] = cx;
+    by[slot] = cy;
+    bz[slot] = cz;
+    if (bw)
+      bw[slot] = ws[p];
+  }
+}
+
+// One workgroup per tile: LDS accumulation over (CIC_T + 1)^3 nodes, then the
+// flush.
+__global__ void __launch_bounds__(256) cic_tile_kernel(const float* __restrict__ bx,
+  const float* __restrict__ by, const float* __restrict__ bz, const float* __restrict__ bw,
+  const uint64_t* __restrict__ offsets, uint32_t N, uint32_t nt, int64_t* __restrict__ acc) {
+  __shared__ unsigned long long L[CIC_L * CIC_L * CIC_L];
+  const uint32_t t = blockIdx.x;
+  const uint64_t p0 = offsets[t], p1 = offsets[t + 1];
+  if (p0 == p1)
+    return;
+  for (uint32_t i = threadIdx.x; i < CIC_L * CIC_L * CIC_L; i += blockDim.x)
+    L[i] = 0ull;
+  __syncthreads();
+  const uint32_t tx0 = (t / (nt * nt)) * CIC_T, ty0 = ((t / nt) % nt) * CIC_T,
+                 tz0 = (t % nt) * CIC_T;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    uint32_t ix, iy, iz;
+    int64_t v[8];
+    cic_corners(bx[p], by[p], bz[p], bw ? bw[p] : 1.0f, N, &ix, &iy, &iz, v);
+    const uint32_t lx = ix - tx0, ly = iy - ty0, lz = iz - tz0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const uint32_t a = lx + (c >> 2), b = ly + ((c >> 1) & 1), d = lz + (c & 1);
+      atomicAdd(&L[(a * CIC_L + b) * CIC_L + d], (unsigned long long)v[c]);
+    }
+  }
+  __syncthreads();
+  const uint64_t NN = (uint64_t)N * N;
+  for (uint32_t i = threadIdx.x; i < CIC_L * CIC_L * CIC_L; i += blockDim.x) {
+    const unsigned long long s = L[i];
+    if (!s)
+      continue;
+    uint32_t gx = tx0 + i / (CIC_L * CIC_L), gy = ty0 + (i / CIC_L) % CIC_L,
+             gz = tz0 + i % CIC_L;
+    gx = gx >= N ? gx - N : gx;  // the layer past the last node wraps to 0
+    gy = gy >= N ? gy - N : gy;
+    gz = gz >= N ? gz - N : gz;
+    atomicAdd(reinterpret_cast<unsigned long long*>(acc + gx * NN + (uint64_t)gy * N + gz), s);
   }
 }
 
@@ -216,19 +327,51 @@ int check_in_box(const float* x, const float* y, const float* z, uint64_t n,
   return SIF_OK;
 }
 
+namespace {
+DeviceBuffer g_tile_counts, g_tile_offsets, g_tile_cursor, g_bx, g_by, g_bz, g_bw;
+}
+
 int cic_deposit(const float* x, const float* y, const float* z, const float* w,
   uint64_t n, uint32_t N, float box, float* values, int64_t* acc) {
   const uint64_t total = (uint64_t)N * N * N;
   const float idx = (float)N / box;
   const float cell = box / (float)N;
   const float inv_vol = 1.0f / (float)(cell * cell * cell);
+  const uint32_t nt = (N + CIC_T - 1) / CIC_T;
+  const uint64_t tiles = (uint64_t)nt * nt * nt;
+  if (n >= 0xffffffffull || tiles >= 0xffffffffull) {
+    error_set(SIF_ERR_RANGE, "cic: %llu particles (at most 2^32 - 2)", (unsigned long long)n);
+    return SIF_ERR_RANGE;
+  }
+  int st = g_tile_counts.reserve((tiles + 1) * 4);
+  if (st == SIF_OK) st = g_tile_offsets.reserve((tiles + 1) * 8);
+  if (st == SIF_OK) st = g_tile_cursor.reserve(tiles * 4);
+  if (st == SIF_OK) st = g_bx.reserve((n ? n : 1) * 4);
+  if (st == SIF_OK) st = g_by.reserve((n ? n : 1) * 4);
+  if (st == SIF_OK) st = g_bz.reserve((n ? n : 1) * 4);
+  if (st == SIF_OK && w) st = g_bw.reserve((n ? n : 1) * 4);
+  if (st != SIF_OK)
+    return st;
   timing_begin(SIF_AMD_STAGE_CIC);
   SIF_HIP(hipMemsetAsync(acc, 0, total * sizeof(int64_t), stream()));
   if (n) {
-    unsigned g = grid_for(n, 256);
-    if (g > 65536u * 4)
-      g = 65536u * 4;
-    cic_deposit_kernel<<<g, 256, 0, stream()>>>(x, y, z, w, n, N, idx, box, acc);
+    SIF_HIP(hipMemsetAsync(g_tile_counts.ptr, 0, (tiles + 1) * 4, stream()));
+    SIF_HIP(hipMemsetAsync(g_tile_cursor.ptr, 0, tiles * 4, stream()));
+    const unsigned g = grid_for(n, 256);
+    cic_count_kernel<<<g, 256, 0, stream()>>>(x, y, z, n, N, idx, box, nt,
+      g_tile_counts.as<uint32_t>());
+    SIF_HIP(hipGetLastError());
+    st = exclusive_scan_u32(g_tile_counts.as<uint32_t>(), tiles + 1,
+      g_tile_offsets.as<uint64_t>(), nullptr);
+    if (st != SIF_OK)
+      return st;
+    cic_bin_kernel<<<g, 256, 0, stream()>>>(x, y, z, w, n, N, idx, box, nt,
+      g_tile_offsets.as<uint64_t>(), g_tile_cursor.as<uint32_t>(), g_bx.as<float>(),
+      g_by.as<float>(), g_bz.as<float>(), w ? g_bw.as<float>() : nullptr);
+    SIF_HIP(hipGetLastError());
+    cic_tile_kernel<<<(unsigned)tiles, 256, 0, stream()>>>(g_bx.as<float>(), g_by.as<float>(),
+      g_bz.as<float>(), w ? g_bw.as<float>() : nullptr, g_tile_offsets.as<uint64_t>(), N, nt,
+      acc);
     SIF_HIP(hipGetLastError());
   }
   cic_finish_kernel<<<grid_for(total, 256), 256, 0, stream()>>>(
