@@ -81,7 +81,7 @@ class Step:
 STAGE_KERNELS = {
     "grow": "grow_kernel",
     "smooth": "cols_kernel x2 + c2r_rows_kernel (one filtered c2r)",
-    "cic": "cic_deposit_kernel + cic_finish_kernel (with the accumulator clear)",
+    "cic": "cic_count/bin/tile_kernel + cic_finish_kernel (tile binning, LDS deposit, flush)",
     "mesh": "mesh_key_kernel + radix sort + mesh_gather_kernel",
     "scan": "cand_count_kernel + tile scan + cand_write_kernel",
     "fft_forward": "r2c_rows_kernel + cols_kernel x2",
