@@ -18,12 +18,17 @@ void error_set(int status, const char* fmt, ...)
 // Library stream, created by sif_system_init(); blocking with respect to the
 // null stream so work queued by other users of the device orders with it.
 hipStream_t stream();
+// A second stream (same default priority) for work that may overlap the
+// library stream's: the next rung's smoothing while a rung is committed.
+hipStream_t aux_stream();
 bool initialised();
 int ensure_init();  // lazy device init for the sif_amd_* device entry points
 
 // Per-stage device timing (HIP events on the library stream).
 void timing_begin(int stage);
 void timing_end(int stage);
+void timing_begin(int stage, hipStream_t s);
+void timing_end(int stage, hipStream_t s);
 bool timing_active();
 void timing_add(int stage, double ms);  // host-side stages (the commit)
 void timing_bytes(int stage, double bytes);  // algorithmic bytes of a stage

@@ -133,6 +133,7 @@ struct Voids {
 struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
   DeviceBuffer pf_counts;  // per rung: masked, proxy, overlap
+  DeviceBuffer field2;     // the second smoothed field (rungs alternate)
 };
 DeviceState g_dev;
 
@@ -528,14 +529,34 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   std::vector<uint32_t> surv;
   uint64_t cand_cap = 0;
 
+  // Two smoothed fields in turn: the caller's grid and a second buffer.
+  st = g_dev.field2.reserve(total * sizeof(float));
+  if (st != SIF_OK)
+    return st;
+  float* const fields[2] = {R.d_delta, g_dev.field2.as<float>()};
+  static hipEvent_t ev_smoothed[2] = {nullptr, nullptr}, ev_scanned[2] = {nullptr, nullptr};
+  for (int b = 0; b < 2; b++) {
+    if (!ev_smoothed[b])
+      SIF_HIP(hipEventCreateWithFlags(&ev_smoothed[b], hipEventDisableTiming));
+    if (!ev_scanned[b])
+      SIF_HIP(hipEventCreateWithFlags(&ev_scanned[b], hipEventDisableTiming));
+  }
+
+
   for (size_t i = 0; i < R.radii.size() && st == SIF_OK; i++) {
     const float radius = R.radii[i];
     sif_amd_rung_stats_t S;
     memset(&S, 0, sizeof(S));
     S.radius = radius;
 
-    st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>() + i * lut_size,
-      windowed ? g_dev.axis.as<double>() : nullptr, R.d_delta, g_dev.work.as<float2>());
+    // Rung i's field: smoothed on the auxiliary stream while rung i - 1 was
+    // committed (rung 0 on the library stream).
+    float* const field = fields[i & 1];
+    if (i == 0)
+      st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>(),
+        windowed ? g_dev.axis.as<double>() : nullptr, field, g_dev.work.as<float2>(), stream());
+    else
+      SIF_HIP(hipStreamWaitEvent(stream(), ev_smoothed[i & 1], 0));
     if (st != SIF_OK)
       break;
 
@@ -555,7 +576,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         if ((st = reserve_candidates()) != SIF_OK)
           break;
       }
-      st = scan_candidates_flat(R.d_delta, g_dev.mask.as<uint64_t>(), total, R.threshold,
+      st = scan_candidates_flat(field, g_dev.mask.as<uint64_t>(), total, R.threshold,
         g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap, &nc);
       if (st != SIF_OK || nc <= cand_cap)
         break;
@@ -571,6 +592,21 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if (st != SIF_OK)
       break;
     S.candidates = nc;
+    // The scan was the field's last reader: the next rung's field may be
+    // smoothed into the other buffer now, beside this rung's commit.
+    SIF_HIP(hipEventRecord(ev_scanned[i & 1], stream()));
+    if (i + 1 < R.radii.size()) {
+      // after this scan: the other buffer's last reader (rung i - 1's scan)
+      // and every earlier use of the work buffer are behind it
+      const size_t j = i + 1;
+      SIF_HIP(hipStreamWaitEvent(aux_stream(), ev_scanned[i & 1], 0));
+      st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>() + j * lut_size,
+        windowed ? g_dev.axis.as<double>() : nullptr, fields[j & 1], g_dev.work.as<float2>(),
+        aux_stream());
+      if (st != SIF_OK)
+        break;
+      SIF_HIP(hipEventRecord(ev_smoothed[j & 1], aux_stream()));
+    }
 
     const float rmin = 1.00f * radius;
     float r_search = radius + fmaxf((sf - 1.0f) * radius, 3.0f * l);
@@ -832,6 +868,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       stats_out[i] = S;
   }
 
+  // the auxiliary stream may still smooth a rung the loop left (or read work)
+  (void)hipStreamSynchronize(aux_stream());
   if (R.restore && st == SIF_OK)
     st = fft_filter_c2r(g_dev.spec.as<float2>(), n, R.box, SIF_AMD_FILTER_NONE, 0.0f, 0.0f,
       0, 0, R.d_delta, g_dev.work.as<float2>());

@@ -385,14 +385,14 @@ int twiddles(uint32_t n) {
 // The three passes for one n, dispatched to the kernels built for it.
 template <int N>
 int pass_cols(bool inv, const float2* in, float2* out, uint64_t stride_a, uint64_t stride_i,
-  const float* lut, const double* axis) {
+  const float* lut, const double* axis, hipStream_t s) {
   constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
   const int tiles = (nz + TK - 1) / TK;
   if (inv)
-    cols_kernel<N, true><<<N * tiles, FFT_THREADS, 0, stream()>>>(in, out, nz, stride_a,
+    cols_kernel<N, true><<<N * tiles, FFT_THREADS, 0, s>>>(in, out, nz, stride_a,
       stride_i, g_tw.fwd.as<float2>(), lut, axis);
   else
-    cols_kernel<N, false><<<N * tiles, FFT_THREADS, 0, stream()>>>(in, out, nz, stride_a,
+    cols_kernel<N, false><<<N * tiles, FFT_THREADS, 0, s>>>(in, out, nz, stride_a,
       stride_i, g_tw.fwd.as<float2>(), lut, axis);
   SIF_HIP(hipGetLastError());
   return SIF_OK;
@@ -409,10 +409,10 @@ int pass_r2c_rows(const float* in, float2* spec) {
 }
 
 template <int N>
-int pass_c2r_rows(const float2* spec, float* out, float norm) {
+int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s) {
   constexpr int TK = FFT_TILE / N;
   const uint64_t rows = (uint64_t)N * N;
-  c2r_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, stream()>>>(spec, out,
+  c2r_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, s>>>(spec, out,
     rows, norm, g_tw.fwd.as<float2>());
   SIF_HIP(hipGetLastError());
   return SIF_OK;
@@ -430,14 +430,14 @@ int pass_c2r_rows(const float2* spec, float* out, float norm) {
   }
 
 int run_cols(uint32_t n, bool inv, const float2* in, float2* out, uint64_t stride_a,
-  uint64_t stride_i, const float* lut, const double* axis) {
-  SIF_FFT_DISPATCH(n, pass_cols<N>(inv, in, out, stride_a, stride_i, lut, axis))
+  uint64_t stride_i, const float* lut, const double* axis, hipStream_t s) {
+  SIF_FFT_DISPATCH(n, pass_cols<N>(inv, in, out, stride_a, stride_i, lut, axis, s))
 }
 int run_r2c_rows(uint32_t n, const float* in, float2* spec) {
   SIF_FFT_DISPATCH(n, pass_r2c_rows<N>(in, spec))
 }
-int run_c2r_rows(uint32_t n, const float2* spec, float* out, float norm) {
-  SIF_FFT_DISPATCH(n, pass_c2r_rows<N>(spec, out, norm))
+int run_c2r_rows(uint32_t n, const float2* spec, float* out, float norm, hipStream_t s) {
+  SIF_FFT_DISPATCH(n, pass_c2r_rows<N>(spec, out, norm, s))
 }
 
 int check_n(uint32_t n, int* log2n) {
@@ -527,10 +527,10 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
   st = run_r2c_rows(n, values, work);
   // y pass: pencil a = x, i = y
   if (st == SIF_OK)
-    st = run_cols(n, false, work, work, (uint64_t)n * nz, nz, nullptr, nullptr);
+    st = run_cols(n, false, work, work, (uint64_t)n * nz, nz, nullptr, nullptr, stream());
   // x pass: pencil a = y, i = x
   if (st == SIF_OK)
-    st = run_cols(n, false, work, spec, nz, (uint64_t)n * nz, nullptr, nullptr);
+    st = run_cols(n, false, work, spec, nz, (uint64_t)n * nz, nullptr, nullptr, stream());
   if (st != SIF_OK)
     return st;
   timing_end(SIF_AMD_STAGE_FFT_FORWARD);
@@ -542,7 +542,7 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
 // The three inverse passes with a device-resident filter LUT (indexed by
 // |k|^2; null for none) and window axis (null for none).
 int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
-  float* out, float2* work) {
+  float* out, float2* work, hipStream_t s) {
   int log2n;
   int st = check_n(n, &log2n);
   if (st == SIF_OK)
@@ -552,17 +552,17 @@ int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double
   const uint64_t nz = n / 2 + 1;
   const uint64_t rows = (uint64_t)n * n;
   const float norm = 1.0f / (float)((uint64_t)n * n * n);
-  timing_begin(SIF_AMD_STAGE_SMOOTH);
+  timing_begin(SIF_AMD_STAGE_SMOOTH, s);
   // x pass with the filter fused: pencil a = y, i = x
-  st = run_cols(n, true, spec, work, nz, (uint64_t)n * nz, d_lut, d_axis);
+  st = run_cols(n, true, spec, work, nz, (uint64_t)n * nz, d_lut, d_axis, s);
   // y pass: pencil a = x, i = y
   if (st == SIF_OK)
-    st = run_cols(n, true, work, work, (uint64_t)n * nz, nz, nullptr, nullptr);
+    st = run_cols(n, true, work, work, (uint64_t)n * nz, nz, nullptr, nullptr, s);
   if (st == SIF_OK)
-    st = run_c2r_rows(n, work, out, norm);
+    st = run_c2r_rows(n, work, out, norm, s);
   if (st != SIF_OK)
     return st;
-  timing_end(SIF_AMD_STAGE_SMOOTH);
+  timing_end(SIF_AMD_STAGE_SMOOTH, s);
   if (timing_active())  // the spectrum once, the field once
     timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + 4.0 * (double)rows * n);
   return SIF_OK;
@@ -601,7 +601,7 @@ int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
       hipMemcpyHostToDevice, stream()));
   }
   return fft_c2r_lut(spec, n, filtered ? g_lut.as<float>() : nullptr,
-    windowed ? g_axis.as<double>() : nullptr, out, work);
+    windowed ? g_axis.as<double>() : nullptr, out, work, stream());
 }
 
 }  // namespace sif_amd

@@ -31,7 +31,7 @@ int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
 // The inverse passes with a device LUT (indexed by |k|^2, filter_lut_size(n)
 // entries) and window axis (n doubles); either may be null.
 int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
-  float* out, float2* work);
+  float* out, float2* work, hipStream_t s);
 uint64_t filter_lut_size(uint32_t n);
 void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
   std::vector<float>& lut);

@@ -24,6 +24,7 @@ std::mutex g_mutex;
 bool g_init = false;
 int g_device = 0;
 hipStream_t g_stream = nullptr;
+hipStream_t g_aux = nullptr;
 
 // Stage timing: a begin and an end event per bracketed launch, recorded on
 // the library stream and resolved only when read (or when the pending list
@@ -71,6 +72,7 @@ static int init_locked() {
   }
   SIF_HIP(hipGetDevice(&g_device));
   SIF_HIP(hipStreamCreate(&g_stream));
+  SIF_HIP(hipStreamCreate(&g_aux));
   g_init = true;
   return SIF_OK;
 }
@@ -82,6 +84,7 @@ int ensure_init() {
 
 bool initialised() { return g_init; }
 hipStream_t stream() { return g_stream; }
+hipStream_t aux_stream() { return g_aux; }
 
 int DeviceBuffer::reserve(size_t want) {
   if (want <= bytes && ptr)
@@ -124,8 +127,8 @@ static hipEvent_t take_event() {
 static void timing_resolve() {
   if (g_pending.empty())
     return;
-  (void)hipEventSynchronize(g_pending.back().stop);
   for (const Pending& p : g_pending) {
+    (void)hipEventSynchronize(p.stop);  // stages on two streams end out of order
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
       g_ms[p.stage] += ms;
@@ -137,26 +140,29 @@ static void timing_resolve() {
   g_pending.clear();
 }
 
-void timing_begin(int s) {
+void timing_begin(int s) { timing_begin(s, g_stream); }
+void timing_end(int s) { timing_end(s, g_stream); }
+
+void timing_begin(int s, hipStream_t st) {
   if (!g_timing)
     return;
   hipEvent_t e = take_event();
   if (!e)
     return;
-  (void)hipEventRecord(e, g_stream);
+  (void)hipEventRecord(e, st);
   if (g_open[s])
     g_free_events.push_back(g_open_start[s]);
   g_open_start[s] = e;
   g_open[s] = true;
 }
 
-void timing_end(int s) {
+void timing_end(int s, hipStream_t st) {
   if (!g_timing || !g_open[s])
     return;
   hipEvent_t e = take_event();
   if (!e)
     return;
-  (void)hipEventRecord(e, g_stream);
+  (void)hipEventRecord(e, st);
   g_pending.push_back(Pending{s, g_open_start[s], e});
   g_open[s] = false;
   if (g_pending.size() >= PENDING_MAX)
@@ -197,6 +203,7 @@ void sif_system_finalise(void) {
   if (!g_init)
     return;
   (void)hipStreamSynchronize(g_stream);
+  (void)hipStreamSynchronize(g_aux);
   timing_resolve();
   for (int s = 0; s < SIF_AMD_N_STAGES; s++)
     if (g_open[s]) {
@@ -207,7 +214,9 @@ void sif_system_finalise(void) {
     (void)hipEventDestroy(e);
   g_free_events.clear();
   (void)hipStreamDestroy(g_stream);
+  (void)hipStreamDestroy(g_aux);
   g_stream = nullptr;
+  g_aux = nullptr;
   g_init = false;
 }
 
@@ -266,6 +275,7 @@ int sif_amd_synchronize(void) {
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipStreamSynchronize(g_stream));
+  SIF_HIP(hipStreamSynchronize(g_aux));
   return SIF_OK;
 }
 
