@@ -503,6 +503,13 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   V.init(R.box);
   RungVoids RV;
   RV.init(R.box);
+  RungVoids CL;  // the voids accepted by the commit under way
+  CL.init(R.box);
+  struct PreProbe {
+    bool node, pole, pole_r;
+    int8_t state, state_r;
+  };
+  std::vector<PreProbe> pre;
   st = g_dev.mask.reserve((total + 63) / 64 * sizeof(uint64_t));
   if (st == SIF_OK)
     st = reserve_slots();
@@ -766,13 +773,17 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       S.evaluated += bc;
       return SIF_OK;
     };
-    // exodus_finder.c:2174-2329, in candidate order
+    // exodus_finder.c:2174-2329, in candidate order. Every test is an OR over
+    // voids, so it splits by the voids' age: the rung's older voids (the
+    // device answered: prefilter, grown-radius state and pole bit), the rung's
+    // new voids as the commit begins (probed for every candidate at once, on
+    // the host pool), and the voids this commit accepts (probed in order, in
+    // a list of their own).
+    auto combine = [](int a, int b) { return (a == 1 || b == 1) ? 1 : (a | b); };
     auto commit = [&](BatchSlot& B, uint64_t* accepted_out) -> int {
       SIF_HIP(hipEventSynchronize(B.done));
       const double t2 = now_ms();
-      uint64_t accepted = 0;
-      for (uint32_t b = 0; b < B.count; b++) {
-        const uint32_t flat = B.h_flat[b];
+      for (uint32_t b = 0; b < B.count; b++)
         if (B.h_reason[b] == 5) {
           error_set(SIF_ERR_RANGE,
             "exodus: a search shell at rung %g holds more tracers in one histogram "
@@ -780,9 +791,53 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
             (double)radius);
           return SIF_ERR_RANGE;
         }
-        const int why = rung_tests(flat, B.edge[b] != 0);
-        if (why) {
-          count_refusal(why);
+      pre.resize(B.count);
+      host_pool().run(B.count, [&](uint64_t b0, uint64_t b1) {
+        for (uint64_t b = b0; b < b1; b++) {
+          uint32_t ix, iy, iz;
+          node_of(B.h_flat[b], &ix, &iy, &iz);
+          const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+          OverlapQuery q;
+          overlap_query(q, cx, cy, cz, radius, V.max_r, R.box, R.f, scale, V.inv_l);
+          const RungProbe p1 = RV.probe(&q, ix, iy, iz, cx, cy, cz, true, proxy, scale, l, n);
+          PreProbe& P = pre[b];
+          P.node = p1.node;
+          P.pole = p1.pole;
+          P.state = (int8_t)p1.state;
+          P.pole_r = false;
+          P.state_r = 0;
+          const float r = B.h_rad[b];
+          if (r >= 0.0f) {
+            const int32_t pole = (int32_t)(scale * r * (1.0f - R.f) / l) - 1;
+            OverlapQuery q2;
+            overlap_query(q2, cx, cy, cz, r, V.max_r, R.box, R.f, scale, V.inv_l);
+            const RungProbe p2 = RV.probe(&q2, ix, iy, iz, cx, cy, cz, false, pole, scale, l, n);
+            P.pole_r = p2.pole;
+            P.state_r = (int8_t)p2.state;
+          }
+        }
+      });
+      CL.clear();
+      uint64_t accepted = 0;
+      for (uint32_t b = 0; b < B.count; b++) {
+        const PreProbe& P = pre[b];
+        uint32_t ix, iy, iz;
+        node_of(B.h_flat[b], &ix, &iy, &iz);
+        const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+        // the rung-radius tests (mask, pole probe, overlap)
+        OverlapQuery q;
+        overlap_query(q, cx, cy, cz, radius, V.max_r, R.box, R.f, scale, V.inv_l);
+        const RungProbe c1 = CL.probe(&q, ix, iy, iz, cx, cy, cz, true, proxy, scale, l, n);
+        if (P.node || c1.node) {
+          count_refusal(1);
+          continue;
+        }
+        if (proxy > 0 && (P.pole || c1.pole)) {
+          count_refusal(2);
+          continue;
+        }
+        if (exact_overlap(cx, cy, cz, radius, B.edge[b] ? 2u : 0u, combine(P.state, c1.state))) {
+          count_refusal(3);
           continue;
         }
         const float r = B.h_rad[b];
@@ -790,27 +845,23 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           S.rescale_fail++;
           continue;
         }
-        uint32_t ix, iy, iz;
-        node_of(flat, &ix, &iy, &iz);
-        const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
-        // the tests at the rescaled radius (exodus_finder.c:2290-2311): the
-        // device answered for the rung's older voids, the probe answers for
-        // the new ones
+        // the tests at the rescaled radius (exodus_finder.c:2290-2311)
         const int32_t pole = (int32_t)(scale * r * (1.0f - R.f) / l) - 1;
         const uint32_t gs = B.h_state[b];
-        OverlapQuery q;
-        overlap_query(q, cx, cy, cz, r, V.max_r, R.box, R.f, scale, V.inv_l);
-        const RungProbe p = RV.probe(&q, ix, iy, iz, cx, cy, cz, false, pole, scale, l, n);
-        if (pole > 0 && ((gs & GROWN_POLE_BIT) || p.pole)) {
+        OverlapQuery q2;
+        overlap_query(q2, cx, cy, cz, r, V.max_r, R.box, R.f, scale, V.inv_l);
+        const RungProbe c2 = CL.probe(&q2, ix, iy, iz, cx, cy, cz, false, pole, scale, l, n);
+        if (pole > 0 && ((gs & GROWN_POLE_BIT) || P.pole_r || c2.pole)) {
           S.proxy++;
           continue;
         }
-        if (exact_overlap(cx, cy, cz, r, gs & 3u, p.state)) {
+        if (exact_overlap(cx, cy, cz, r, gs & 3u, combine(P.state_r, c2.state))) {
           S.overlap++;
           continue;
         }
         V.add(cx, cy, cz, r, B.h_nin[b]);
         RV.add(cx, cy, cz, r);
+        CL.add(cx, cy, cz, r);
         accepted++;
       }
       S.accepted += accepted;

@@ -26,15 +26,19 @@ struct RungVoids {
   std::vector<float> cx, cy, cz, r;
   std::vector<int32_t> head, next;
   std::vector<uint32_t> touched;
+  std::vector<uint64_t> occupied;  // one bit per cell: a probe skips empty cells
   float box = 0.0f, inv_l = 0.0f, max_r = 0.0f;
   void init(float box_) {
     box = box_;
     inv_l = (float)VOID_CELLS / box;
     head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
+    occupied.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS / 64, 0ull);
   }
   void clear() {
-    for (uint32_t c : touched)
+    for (uint32_t c : touched) {
       head[c] = -1;
+      occupied[c >> 6] = 0ull;
+    }
     touched.clear();
     cx.clear();
     cy.clear();
@@ -52,8 +56,10 @@ struct RungVoids {
     const uint32_t c =
       (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
       void_axis_cell(z, inv_l);
-    if (head[c] == -1)
+    if (head[c] == -1) {
       touched.push_back(c);
+      occupied[c >> 6] |= 1ull << (c & 63);
+    }
     next.push_back(head[c]);
     head[c] = k;
     if (rr > max_r)
@@ -108,7 +114,10 @@ struct RungVoids {
         const uint32_t wb = void_wrap(b);
         for (int32_t c = lo[2]; c <= hi[2]; c++) {
           const uint32_t wc = void_wrap(c);
-          for (int32_t i = head[(wa * VOID_CELLS + wb) * VOID_CELLS + wc]; i != -1; i = next[i]) {
+          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wc;
+          if (!((occupied[cell >> 6] >> (cell & 63)) & 1ull))
+            continue;
+          for (int32_t i = head[cell]; i != -1; i = next[i]) {
             if (q && out.state != 1) {
               const int s = pair_state(*q, wa, wb, wc, cx[i], cy[i], cz[i], r[i]);
               out.state = s == 1 ? 1 : (out.state | s);
