@@ -22,10 +22,13 @@ ARCH = os.environ.get("SIF_AMD_ARCH", "gfx950")
 
 # -ffp-contract=off: every float expression of the path is evaluated with one
 # rounding per operation, as the reference's decisions assume; -packed-fp32-ops
-# keeps the compiler from emitting packed FP32 instructions.
+# keeps the compiler from emitting packed FP32 instructions. The host side
+# targets x86-64-v2 (SSE4.2, POPCNT): floorf and the bit counts of the host's
+# cell walks become single instructions rather than library calls.
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
     "-ffp-contract=off", "-fno-fast-math",
+    "-Xarch_host", "-march=x86-64-v2",
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,

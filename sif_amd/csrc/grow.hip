@@ -379,9 +379,12 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   P.reason = reason;
   P.n_in = n_in;
   P.algo_bytes = algo_bytes;
+  grow_lds_words(grow_bin_count(r_search, rmin, vol_factor), 0, &P.lds_bins, &P.lds_cache);
+  const size_t lds_bytes =
+    (size_t)GROW_WAVES * grow_wave_words(P.lds_bins, P.lds_cache) * sizeof(uint32_t);
   timing_begin(SIF_AMD_STAGE_GROW);
   const uint64_t blocks = ((uint64_t)n + GROW_WAVES - 1) / GROW_WAVES;
-  grow_kernel<<<(unsigned)blocks, 64 * GROW_WAVES, 0, stream()>>>(P);
+  grow_kernel<<<(unsigned)blocks, 64 * GROW_WAVES, lds_bytes, stream()>>>(P);
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_GROW);
   return SIF_OK;

@@ -4,12 +4,41 @@
 // lane without a GPU.
 #pragma once
 
+#if defined(__HIPCC__)
+#define SIF_GROW_HD __host__ __device__ __forceinline__
+#else
+#define SIF_GROW_HD inline
+#endif
+
+// The kernel's LDS is dynamic, sized per launch; the CPU emulator of the
+// kernel supplies its own block buffer through this hook.
+#ifndef GROW_DYNAMIC_LDS
+#define GROW_DYNAMIC_LDS(name) extern __shared__ __align__(16) uint32_t name[]
+#endif
+
 namespace sif_amd {
 
-constexpr int GROW_WAVES = 4;
+// One wave per candidate, one wave per block: a block's LDS is its wave's,
+// so occupancy follows each launch's LDS size wave by wave.
+constexpr int GROW_WAVES = 1;
 constexpr int MAX_BINS = 2048;
 constexpr int REFINE_CAP = 1024;
 constexpr int SUB_BINS = 64;
+
+// exodus_rescale.c:793-800: the histogram's bin count, about 16 expected
+// particles a bin over the annulus, between 64 and MAX_BINS. Host and device
+// evaluate the same float operations (no contraction), so the host sizes the
+// launch's LDS for exactly the count every wave computes.
+SIF_GROW_HD uint32_t grow_bin_count(float r_search, float rmin, float vol_factor) {
+  const float rs3 = r_search * r_search * r_search;
+  const float rm3 = rmin * rmin * rmin;
+  const float want = vol_factor * (rs3 - rm3) / 16.0f;
+  if (!(want >= 64.0f))
+    return 64;
+  if (want < (float)MAX_BINS)
+    return (uint32_t)want;
+  return (uint32_t)MAX_BINS;
+}
 
 enum { CELL_CORE = 1, CELL_STRADDLES = 2, CELL_EDGE = 3, CELL_SHELL = 4 };
 
@@ -41,28 +70,55 @@ struct GrowParams {
   float* radius;
   uint32_t* reason;
   uint64_t* n_in;
+  // LDS words a wave holds for its histogram (at least the bin count) and
+  // for its annulus cache (at least REFINE_CAP: the window buffer shares it)
+  uint32_t lds_bins, lds_cache;
   // optional: the candidate's algorithmic bytes -- 8 per template cell whose
   // offsets it reads, 12 per particle of the cells whose particles it must
   // read once (the surface for the early refusal, every non-CORE cell past it)
   uint64_t* algo_bytes;
 };
 
-// Annulus distances a wave keeps in LDS: a candidate whose annulus fits is
-// resolved from LDS after one pass over its cells.
-constexpr uint32_t CACHE_CAP = 2048;
+// A wave's LDS, carved from the block's: the histogram (lds_bins words),
+// the annulus cache (lds_cache words) -- every annulus distance while they
+// fit, so a candidate is resolved from LDS after one pass over its cells; the
+// pass-per-window walk of a candidate whose annulus does not fit collects its
+// window in the same words -- and the tables of the template chunk in flight.
+constexpr uint32_t CACHE_MAX = 2048;
+constexpr uint32_t LDS_TAIL_WORDS = 6 * 64 + SUB_BINS;
+
+SIF_GROW_HD uint32_t grow_wave_words(uint32_t lds_bins, uint32_t lds_cache) {
+  return lds_bins + lds_cache + LDS_TAIL_WORDS;
+}
+
+// A launch's LDS words a wave: the rung's histogram, then as much annulus
+// cache as keeps 16 waves on a CU (four a SIMD, what the kernel's registers
+// allow) within its 160 KiB, and never less than the window buffer.
+// cache_words > 0 asks for that cache instead.
+SIF_GROW_HD void grow_lds_words(uint32_t nb, uint32_t cache_words, uint32_t* lds_bins,
+  uint32_t* lds_cache) {
+  *lds_bins = (nb + 3u) & ~3u;
+  uint32_t cache = cache_words;
+  if (cache == 0) {
+    const uint32_t budget = (160u * 1024u / 16u - 256u) / 4u;
+    const uint32_t used = *lds_bins + LDS_TAIL_WORDS;
+    cache = (budget > used ? budget - used : 0u) & ~63u;
+    cache = cache > CACHE_MAX ? CACHE_MAX : cache;
+  }
+  *lds_cache = cache < (uint32_t)REFINE_CAP ? (uint32_t)REFINE_CAP : cache;
+}
 
 struct WaveLDS {
-  uint32_t bins[MAX_BINS];
-  union {
-    float buf[REFINE_CAP];   // a window's distances (the pass-per-window walk)
-    float cache[CACHE_CAP];  // every annulus distance (the cached walk)
-  };
-  uint32_t t_start[64];
-  uint32_t t_scan[64];
-  float t_ex[64], t_ey[64], t_ez[64];
-  uint32_t t_type[64];
-  uint32_t sub[SUB_BINS];
-  uint32_t counter;
+  uint32_t* bins;
+  float* cache;
+  float* buf;
+  uint32_t* t_start;
+  uint32_t* t_scan;
+  float* t_ex;
+  float* t_ey;
+  float* t_ez;
+  uint32_t* t_type;
+  uint32_t* sub;
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -326,26 +382,46 @@ __device__ __forceinline__ void wave_sort(float* buf, uint32_t n, int lane) {
   wave_sync();
 }
 
-__global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
-  __shared__ WaveLDS lds[GROW_WAVES];
+// The same for n <= CACHE_MAX / 2, the windows of a cache at most half full.
+__device__ __forceinline__ void wave_sort_half(float* buf, uint32_t n, int lane) {
+  wave_sync();
+  if (n <= 64)
+    wave_sort_regs<1>(buf, n, lane);
+  else if (n <= 128)
+    wave_sort_regs<2>(buf, n, lane);
+  else if (n <= 256)
+    wave_sort_regs<4>(buf, n, lane);
+  else if (n <= 512)
+    wave_sort_regs<8>(buf, n, lane);
+  else
+    wave_sort_regs<16>(buf, n, lane);
+  wave_sync();
+}
+
+__global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) {
+  GROW_DYNAMIC_LDS(lds_words);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  WaveLDS& S = lds[w];
+  WaveLDS S;
+  {
+    uint32_t* base = lds_words + uni((uint32_t)w * grow_wave_words(P.lds_bins, P.lds_cache));
+    S.bins = base;
+    S.cache = reinterpret_cast<float*>(base + P.lds_bins);
+    S.buf = S.cache;
+    uint32_t* tail = base + P.lds_bins + P.lds_cache;
+    S.t_start = tail;
+    S.t_scan = tail + 64;
+    S.t_ex = reinterpret_cast<float*>(tail + 128);
+    S.t_ey = reinterpret_cast<float*>(tail + 192);
+    S.t_ez = reinterpret_cast<float*>(tail + 256);
+    S.t_type = tail + 320;
+    S.sub = tail + 384;
+  }
 
   const float rs2 = __fmul_rn(P.r_search, P.r_search);
   const float rc2 = __fmul_rn(P.rmin, P.rmin);
   const float span = __fsub_rn(rs2, rc2);
-  const float rs3 = __fmul_rn(__fmul_rn(P.r_search, P.r_search), P.r_search);
-  const float rm3 = __fmul_rn(__fmul_rn(P.rmin, P.rmin), P.rmin);
-  uint32_t nb = MAX_BINS;
-  {
-    const float exp_shell = __fmul_rn(P.vol_factor, __fsub_rn(rs3, rm3));
-    const float want = __fdiv_rn(exp_shell, 16.0f);
-    if (!(want >= 64.0f))
-      nb = 64;
-    else if (want < (float)MAX_BINS)
-      nb = (uint32_t)want;
-  }
+  const uint32_t nb = grow_bin_count(P.r_search, P.rmin, P.vol_factor);
   const float expected = __fmul_rn(__fmul_rn(__fmul_rn(P.vol_factor, P.r_search), P.r_search), P.r_search);
   const float K = __fmul_rn(__fadd_rn(P.threshold, 1.0f), P.vol_factor);
   const float K2 = __fmul_rn(K, K);
@@ -362,7 +438,7 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
   uint64_t n_in_out = 0;
   uint64_t algo = 0;
   bool cached = false;
-  uint64_t n_annulus = 0, n_core_cached = 0;
+  uint64_t n_annulus = 0, n_core = 0;
   {
     Query q;
     q.cx = P.cx[c];
@@ -380,10 +456,17 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
       reason = R_DEGENERATE;
     } else if (!(span > 0.0f)) {
       reason = R_DEGENERATE;
+    } else if (nb > P.lds_bins || P.lds_cache < (uint32_t)REFINE_CAP) {
+      reason = R_OVERFLOW;  // a launch sized for fewer bins: never expected
     } else {
       // 1. early refusal, O(surface): exodus_rescale.c:711-782. The pass also
-      // reads the SHELL cells (counted whole by the reference) and keeps every
-      // annulus distance in LDS while they fit.
+      // reads the SHELL cells (counted whole by the reference), keeps every
+      // annulus distance in LDS while they fit, and builds the histogram of
+      // the annulus (exodus_rescale.c:793-865, which reads the same
+      // particles again) as it goes.
+      for (uint32_t b = lane; b < nb; b += 64)
+        S.bins[b] = 0;
+      wave_sync();
       uint64_t within = 0, surface = 0, shell = 0, core = 0;
       uint32_t appended = 0;  // uniform
       for_particles(P, q, S, lane,
@@ -409,8 +492,10 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
               core += d2 <= rc2 ? 1u : 0u;
               annulus = d2 > rc2 && d2 <= rs2;
             }
+            if (annulus)
+              atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
           }
-          wave_append(S.cache, CACHE_CAP, appended, annulus, d2, lane);
+          wave_append(S.cache, P.lds_cache, appended, annulus, d2, lane);
         });
       wave_sync();
       within = uni(wave_sum_u64(within));
@@ -424,60 +509,18 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
         reason = R_BEYOND;
       }
       algo = 8ull * P.tcount + 12ull * (surface + (reason == R_OK ? shell : 0ull));
-      cached = n_annulus <= CACHE_CAP;
-      n_core_cached = core;
+      cached = n_annulus <= P.lds_cache;
+      n_core = core;
     }
 
     if (reason == R_OK) {
-      // 2. histogram of the annulus, uniform in d^2: exodus_rescale.c:793-865
-      for (uint32_t b = lane; b < nb; b += 64)
-        S.bins[b] = 0;
-      wave_sync();
-      uint64_t n_core = 0, n_shell = 0;
-      if (cached) {
-        // sorted ascending, a bin is one run of the cache: its count is the
-        // distance from its first element to one past its last
+      // 2. the histogram is the first pass's; every particle it holds is in
+      // the annulus. A cached annulus of more than half the cache is sorted
+      // whole now; a smaller one leaves room after it for any window.
+      const bool cache_sorted = cached && n_annulus > P.lds_cache / 2;
+      if (cache_sorted)
         wave_sort(S.cache, (uint32_t)n_annulus, lane);
-        for (uint32_t k = lane; k < n_annulus; k += 64) {
-          const uint32_t b = bin_of(S.cache[k], rc2, inv_w, nb);
-          if (k + 1 == n_annulus || bin_of(S.cache[k + 1], rc2, inv_w, nb) != b)
-            S.bins[b] = k + 1;
-        }
-        wave_sync();
-        for (uint32_t k = lane; k < n_annulus; k += 64) {
-          const uint32_t b = bin_of(S.cache[k], rc2, inv_w, nb);
-          if (k == 0 || bin_of(S.cache[k - 1], rc2, inv_w, nb) != b)
-            S.bins[b] -= k;
-        }
-        if (lane == 0) {
-          n_core = n_core_cached;
-          n_shell = n_annulus;
-        }
-      } else for_particles(P, q, S, lane,
-        [](uint32_t type, float2) { return true; },
-        [&](uint32_t type, uint32_t cnt) {
-          if (type == CELL_CORE) {
-            n_core += cnt;
-            return true;
-          }
-          return false;
-        },
-        [&](float d2, uint32_t type, bool valid) {
-          if (!valid)
-            return;
-          if (type == CELL_SHELL) {
-            atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
-            n_shell++;
-          } else {
-            n_core += d2 <= rc2 ? 1u : 0u;
-            if (d2 > rc2 && d2 <= rs2) {
-              atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
-              n_shell++;
-            }
-          }
-        });
-      wave_sync();
-      const uint64_t total = uni(wave_sum_u64(n_core) + wave_sum_u64(n_shell));
+      const uint64_t total = n_core + n_annulus;
       if (total == 0) {
         reason = R_EMPTY;
       } else if (__fsub_rn(__fdiv_rn((float)total, expected), 1.0f) <= P.threshold) {
@@ -552,9 +595,34 @@ __global__ void __launch_bounds__(64 * GROW_WAVES) grow_kernel(GrowParams P) {
           const uint32_t win_total = uni(__shfl(wscan, W - 1, 64));
 
           if (cached) {
-            // the window is the run of the sorted cache that ends where the
-            // bins above `best` begin; walked outside in as below
-            const uint32_t seg0 = (uint32_t)(n_annulus - abv - win_total);
+            // the window's distances in ascending order: the run of the
+            // cache sorted whole that ends where the bins above `best` begin,
+            // or copied out of the cache into its free tail and sorted there.
+            // Walked outside in as below.
+            uint32_t seg0;
+            if (cache_sorted) {
+              seg0 = (uint32_t)(n_annulus - abv - win_total);
+            } else {
+              float* win = S.cache + n_annulus;
+              uint32_t got = 0;  // uniform
+              for (uint32_t k0 = 0; k0 < (uint32_t)n_annulus; k0 += 64) {
+                const uint32_t k = k0 + (uint32_t)lane;
+                float d2 = 0.0f;
+                bool take = false;
+                if (k < (uint32_t)n_annulus) {
+                  d2 = S.cache[k];
+                  const int32_t b = (int32_t)bin_of(d2, rc2, inv_w, nb);
+                  take = b >= blo && b <= best;
+                }
+                wave_append(win, win_total, got, take, d2, lane);
+              }
+              if (uni(got) != win_total) {
+                reason = R_OVERFLOW;  // counts disagree: never expected
+                break;
+              }
+              wave_sort_half(win, win_total, lane);
+              seg0 = (uint32_t)n_annulus;
+            }
             int found = -1;
             for (uint32_t k = lane; k < win_total; k += 64) {
               const float d2 = S.cache[seg0 + k];

@@ -59,6 +59,8 @@ SIF_HD uint32_t paint_wrap(int32_t g, int32_t n) {
 // the first image i + k n at or above lo
 SIF_HD int32_t paint_first_image(uint32_t i, int32_t lo, int32_t n) {
   const int32_t d = lo - (int32_t)i;
+  if (d > -n && d <= n)  // within a box of i: no division
+    return (int32_t)i + (d > 0 ? n : 0);
   const int32_t k = d >= 0 ? (d + n - 1) / n : -((-d) / n);
   return (int32_t)i + k * n;
 }

@@ -23,8 +23,15 @@ struct RungProbe {
 };
 
 struct RungVoids {
-  std::vector<float> cx, cy, cz, r;
-  std::vector<int32_t> head, next;
+  // one cache line's half per void: what a probe reads of it
+  struct alignas(32) Rec {
+    float x, y, z, r;
+    int32_t next;   // the next void of its cell, or -1
+    uint32_t cell;  // its cell, (wx 32 + wy) 32 + wz
+  };
+  std::vector<float> cx, cy, cz, r;  // the same voids as columns, for upload
+  std::vector<Rec> rec;
+  std::vector<int32_t> head;
   std::vector<uint32_t> touched;
   std::vector<uint64_t> occupied;  // one bit per cell: a probe skips empty cells
   float box = 0.0f, inv_l = 0.0f, max_r = 0.0f;
@@ -44,7 +51,7 @@ struct RungVoids {
     cy.clear();
     cz.clear();
     r.clear();
-    next.clear();
+    rec.clear();
     max_r = 0.0f;
   }
   void add(float x, float y, float z, float rr) {
@@ -60,7 +67,7 @@ struct RungVoids {
       touched.push_back(c);
       occupied[c >> 6] |= 1ull << (c & 63);
     }
-    next.push_back(head[c]);
+    rec.push_back(Rec{x, y, z, rr, head[c], c});
     head[c] = k;
     if (rr > max_r)
       max_r = rr;
@@ -70,12 +77,22 @@ struct RungVoids {
   // (x, y, z). q: the overlap query, or null for none. want_node: test the
   // node's own mask bit. pole > 0: test the six nodes pole away along the
   // axes. `scale` is the painting scale of a void's radius.
+  //
+  // The caller refuses on the first of node, pole, overlap that holds, so
+  // the probe stops once its answer can no longer change that: at a node hit,
+  // or at a pole hit when the node is not asked; after a pole hit it no
+  // longer needs the state. Every test is an OR over voids, so the voids may
+  // be visited in any order; a short list is walked directly, testing each
+  // void's cell against the reach, which visits the same voids as the scan
+  // of the cells.
   RungProbe probe(const OverlapQuery* q, uint32_t ix, uint32_t iy, uint32_t iz, float x,
     float y, float z, bool want_node, int32_t pole, float scale, float l, uint32_t n) const {
     RungProbe out;
-    if (cx.empty())
+    if (rec.empty())
       return out;
     const bool want_pole = pole > 0;
+    if (!q && !want_node && !want_pole)
+      return out;
     // reach: the overlap superset, and painting (a void paints only nodes
     // within scale r + 2 l of its centre on each axis) around the node and
     // its poles
@@ -102,37 +119,71 @@ struct RungVoids {
         }
       }
     }
-    const uint32_t px[2] = {paint_wrap((int32_t)ix + pole, (int32_t)n),
-      paint_wrap((int32_t)ix - pole, (int32_t)n)};
-    const uint32_t py[2] = {paint_wrap((int32_t)iy + pole, (int32_t)n),
-      paint_wrap((int32_t)iy - pole, (int32_t)n)};
-    const uint32_t pz[2] = {paint_wrap((int32_t)iz + pole, (int32_t)n),
-      paint_wrap((int32_t)iz - pole, (int32_t)n)};
+    // the six pole nodes, wrapped when a first void needs them
+    uint32_t px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
+    bool poles_ready = false;
+    // one void: false once the answer is decided
+    auto visit = [&](const Rec& v, uint32_t wa, uint32_t wb, uint32_t wc) -> bool {
+      if (q && out.state != 1 && !(want_pole && out.pole)) {
+        const int s = pair_state(*q, wa, wb, wc, v.x, v.y, v.z, v.r);
+        out.state = s == 1 ? 1 : (out.state | s);
+      }
+      if ((want_node && !out.node) || (want_pole && !out.pole)) {
+        PaintSphere P;
+        paint_sphere(P, v.x, v.y, v.z, scale * v.r, l, n);
+        if (want_node && !out.node)
+          out.node = paint_hits(P, ix, iy, iz);
+        if (want_pole && !out.pole && !poles_ready) {
+          px[0] = paint_wrap((int32_t)ix + pole, (int32_t)n);
+          px[1] = paint_wrap((int32_t)ix - pole, (int32_t)n);
+          py[0] = paint_wrap((int32_t)iy + pole, (int32_t)n);
+          py[1] = paint_wrap((int32_t)iy - pole, (int32_t)n);
+          pz[0] = paint_wrap((int32_t)iz + pole, (int32_t)n);
+          pz[1] = paint_wrap((int32_t)iz - pole, (int32_t)n);
+          poles_ready = true;
+        }
+        if (want_pole && !out.pole)
+          out.pole = paint_hits(P, px[0], iy, iz) || paint_hits(P, px[1], iy, iz) ||
+                     paint_hits(P, ix, py[0], iz) || paint_hits(P, ix, py[1], iz) ||
+                     paint_hits(P, ix, iy, pz[0]) || paint_hits(P, ix, iy, pz[1]);
+      }
+      return !(out.node || (out.pole && !want_node));
+    };
+    const uint32_t span[3] = {(uint32_t)(hi[0] - lo[0]), (uint32_t)(hi[1] - lo[1]),
+      (uint32_t)(hi[2] - lo[2])};
+    const uint32_t rows = (span[0] + 1) * (span[1] + 1);
+    if (rec.size() <= rows) {
+      const uint32_t w0[3] = {void_wrap(lo[0]), void_wrap(lo[1]), void_wrap(lo[2])};
+      for (const Rec& v : rec) {
+        const uint32_t wa = v.cell >> 10, wb = (v.cell >> 5) & 31u, wc = v.cell & 31u;
+        if (((wa - w0[0]) & 31u) > span[0] || ((wb - w0[1]) & 31u) > span[1] ||
+            ((wc - w0[2]) & 31u) > span[2])
+          continue;
+        if (!visit(v, wa, wb, wc))
+          return out;
+      }
+      return out;
+    }
+    // the z range as a mask over a row's 32 cells (wrapping)
+    uint32_t zmask = 0xffffffffu;
+    if (span[2] + 1 < (uint32_t)VOID_CELLS) {
+      const uint64_t m = ((1ull << (span[2] + 1)) - 1ull) << void_wrap(lo[2]);
+      zmask = (uint32_t)(m | (m >> VOID_CELLS));
+    }
     for (int32_t a = lo[0]; a <= hi[0]; a++) {
       const uint32_t wa = void_wrap(a);
       for (int32_t b = lo[1]; b <= hi[1]; b++) {
         const uint32_t wb = void_wrap(b);
-        for (int32_t c = lo[2]; c <= hi[2]; c++) {
-          const uint32_t wc = void_wrap(c);
-          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wc;
-          if (!((occupied[cell >> 6] >> (cell & 63)) & 1ull))
-            continue;
-          for (int32_t i = head[cell]; i != -1; i = next[i]) {
-            if (q && out.state != 1) {
-              const int s = pair_state(*q, wa, wb, wc, cx[i], cy[i], cz[i], r[i]);
-              out.state = s == 1 ? 1 : (out.state | s);
-            }
-            if ((want_node && !out.node) || (want_pole && !out.pole)) {
-              PaintSphere P;
-              paint_sphere(P, cx[i], cy[i], cz[i], scale * r[i], l, n);
-              if (want_node && !out.node)
-                out.node = paint_hits(P, ix, iy, iz);
-              if (want_pole && !out.pole)
-                out.pole = paint_hits(P, px[0], iy, iz) || paint_hits(P, px[1], iy, iz) ||
-                           paint_hits(P, ix, py[0], iz) || paint_hits(P, ix, py[1], iz) ||
-                           paint_hits(P, ix, iy, pz[0]) || paint_hits(P, ix, iy, pz[1]);
-            }
-          }
+        // a row of 32 cells is half an occupancy word: only its set cells
+        const uint32_t row = wa * VOID_CELLS + wb;
+        uint32_t cells = (uint32_t)(occupied[row >> 1] >> ((row & 1u) * 32u)) & zmask;
+        while (cells) {
+          const uint32_t wc = (uint32_t)__builtin_ctz(cells);
+          cells &= cells - 1u;
+          const uint32_t cell = row * VOID_CELLS + wc;
+          for (int32_t i = head[cell]; i != -1; i = rec[i].next)
+            if (!visit(rec[i], wa, wb, wc))
+              return out;
         }
       }
     }

@@ -10,6 +10,10 @@ thread_local emu_dim3 threadIdx;
 thread_local emu_dim3 blockIdx;
 emu_dim3 blockDim, gridDim;
 
+// the block's dynamic LDS (blocks run one at a time)
+static std::vector<uint32_t> g_lds;
+#define GROW_DYNAMIC_LDS(name) uint32_t* name = g_lds.data()
+
 #include "grow_device.h"
 
 static EmuWave g_waves[sif_amd::GROW_WAVES];
@@ -40,8 +44,10 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
   const int32_t* tdz, const uint8_t* ttype, const float* tmin, const float* tmax,
   uint32_t tcount, float rmin, float r_search, float threshold, float vol_factor,
   const float* cx, const float* cy, const float* cz, uint32_t n, float* radius,
-  uint32_t* reason, uint64_t* n_in, uint32_t blocks) {
-  blocks = (n + sif_amd::GROW_WAVES - 1) / sif_amd::GROW_WAVES;
+  uint32_t* reason, uint64_t* n_in, uint32_t cache_words) {
+  // cache_words: the annulus cache a wave holds; 0 sizes it as the library
+  // does for the launch
+  const uint32_t blocks = (n + sif_amd::GROW_WAVES - 1) / sif_amd::GROW_WAVES;
   std::vector<int4> cells(tcount);
   std::vector<float2> bounds(tcount);
   for (uint32_t i = 0; i < tcount; i++) {
@@ -71,6 +77,10 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
   P.radius = radius;
   P.reason = reason;
   P.n_in = n_in;
+  sif_amd::grow_lds_words(sif_amd::grow_bin_count(r_search, rmin, vol_factor), cache_words,
+    &P.lds_bins, &P.lds_cache);
+  g_lds.assign((size_t)sif_amd::GROW_WAVES * sif_amd::grow_wave_words(P.lds_bins, P.lds_cache),
+    0xdeadbeefu);
   blockDim.x = 64 * sif_amd::GROW_WAVES;
   gridDim.x = blocks;
   for (uint32_t b = 0; b < blocks; b++) {

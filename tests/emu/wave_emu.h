@@ -27,7 +27,7 @@ extern emu_dim3 blockDim, gridDim;
 #define __device__
 #define __host__
 #define __forceinline__ inline
-#define __launch_bounds__(x)
+#define __launch_bounds__(...)
 #define __shared__ static
 
 struct int4 {

@@ -35,7 +35,7 @@ def field():
     return x, y, z, m
 
 
-def run_emu(lib, m, tpl, centres, rung, rs, t, vol):
+def run_emu(lib, m, tpl, centres, rung, rs, t, vol, cache_words=0):
     k = len(centres)
     r = np.zeros(k, np.float32)
     why = np.zeros(k, np.uint32)
@@ -50,21 +50,45 @@ def run_emu(lib, m, tpl, centres, rung, rs, t, vol):
         C.c_uint32(len(tpl["dx"])), C.c_float(rung), C.c_float(rs), C.c_float(t),
         C.c_float(vol), V(cx.ctypes.data), V(cy.ctypes.data), V(cz.ctypes.data),
         C.c_uint32(k), V(r.ctypes.data), V(why.ctypes.data), V(nin.ctypes.data),
-        C.c_uint32(0))
+        C.c_uint32(cache_words))
     assert st == 0
     return r, why, nin
 
 
-@pytest.mark.parametrize("rung,factor,t", [(6.0, 1.5, -0.8), (9.0, 2.0, -0.8),
-                                           (4.0, 1.25, -0.6)])
-def test_emulated_kernel_matches_oracle(emu, field, rung, factor, t):
+def annulus_sizes(x, y, z, centres, rung, rs):
+    """Particles with rung^2 < d^2 <= rs^2 (periodic), about what a wave's
+    first pass caches."""
+    p = np.c_[x, y, z].astype(np.float64)
+    out = []
+    for c in centres.astype(np.float64):
+        d = np.abs(p - c)
+        d = np.minimum(d, BOX - d)
+        d2 = (d * d).sum(1)
+        out.append(int(((d2 > rung * rung) & (d2 <= rs * rs)).sum()))
+    return np.array(out)
+
+
+# cache_words 0: the LDS the library gives the launch (a cache of 1920 here);
+# 1024, the least: the annulus is then sorted in windows after it (at most
+# 512), sorted whole (512..1024), or too large to cache (walked in passes over
+# the mesh), and the case covers all three.
+@pytest.mark.parametrize("rung,factor,t,cache_words", [(6.0, 1.5, -0.8, 0),
+                                                       (9.0, 2.0, -0.8, 1024),
+                                                       (4.0, 1.25, -0.6, 0)])
+def test_emulated_kernel_matches_oracle(emu, field, rung, factor, t, cache_words):
     x, y, z, m = field
     rs = rung + max((factor - 1.0) * rung, 3.0 * BOX / 32)
     tpl = oracle.OracleTemplate(MESH, np.float32(BOX) / np.float32(MESH), rung, rs).arrays()
     vol = (np.float32(4) / np.float32(3)) * np.float32(3.14159265358979323846) * \
         (np.float32(len(x)) / (np.float32(BOX) ** 3))
     centres = np.random.default_rng(int(rung * 10)).uniform(0, BOX, (64, 3)).astype(np.float32)
-    r, why, nin = run_emu(emu, m, tpl, centres, rung, rs, np.float32(t), vol)
+    if cache_words:
+        ann = annulus_sizes(x, y, z, centres, rung, rs)
+        half = cache_words // 2
+        assert (ann <= half - 16).sum() >= 5
+        assert ((ann > half + 16) & (ann <= cache_words - 16)).sum() >= 5
+        assert (ann > cache_words + 16).sum() >= 5
+    r, why, nin = run_emu(emu, m, tpl, centres, rung, rs, np.float32(t), vol, cache_words)
     om = oracle.OracleMesh(m, MESH, BOX)
     orr, owhy, onin = oracle.find_radius(om, centres, rung, rs, np.float32(t), vol)
     np.testing.assert_array_equal(why, owhy)
