@@ -38,6 +38,7 @@
 #include "overlap.h"
 #include "paint.h"
 #include "rung_voids.h"
+#include "batch_pairs.h"
 #include "sif/structures/catalogue.h"
 #include "sif_amd.h"
 
@@ -56,6 +57,7 @@ struct Voids {
   std::vector<float> cx, cy, cz, r;
   std::vector<uint64_t> n_in;
   std::vector<int32_t> head, next;
+  size_t linked = 0;  // voids [0, linked) are in the cell list
   float inv_l = 0.0f, box = 0.0f, max_r = 0.0f;
   bool dirty = true;
   void init(float box_) {
@@ -67,22 +69,29 @@ struct Voids {
     return (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
            void_axis_cell(z, inv_l);
   }
+  // The cell list is only read by the reference's scan, which the commit
+  // needs rarely: voids are linked into it when it is next read.
   void add(float x, float y, float z, float rr, uint64_t nin) {
-    const int32_t k = (int32_t)cx.size();
     cx.push_back(x);
     cy.push_back(y);
     cz.push_back(z);
     r.push_back(rr);
     n_in.push_back(nin);
-    const uint32_t c = cell_of(x, y, z);
-    next.push_back(head[c]);
-    head[c] = k;
     if (rr > max_r)
       max_r = rr;
     dirty = true;
   }
+  void link() {
+    next.resize(cx.size());
+    for (; linked < cx.size(); linked++) {
+      const uint32_t c = cell_of(cx[linked], cy[linked], cz[linked]);
+      next[linked] = head[c];
+      head[c] = (int32_t)linked;
+    }
+  }
   // utils.c:850-960
-  bool overlap_exact(float x, float y, float z, float rr, float f, float scale) const {
+  bool overlap_exact(float x, float y, float z, float rr, float f, float scale) {
+    link();
     const float half = box * 0.5f;
     const float sr = scale * (rr + max_r - f * (rr < max_r ? rr : max_r));
     const int32_t lo[3] = {(int32_t)floorf((x - sr) * inv_l), (int32_t)floorf((y - sr) * inv_l),
@@ -505,11 +514,29 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   RV.init(R.box);
   RungVoids CL;  // the voids accepted by the commit under way
   CL.init(R.box);
+  CL.columns = false;
+  RungVoids BI;  // the batch's grown candidates, as voids (batch_pairs.h)
+  BI.init(R.box);
+  BI.columns = false;
+  std::vector<uint32_t> bi_place;
+  struct Node {
+    uint32_t ix, iy, iz;
+    float x, y, z;
+  };
+  std::vector<Node> nodes;      // a batch's candidates: node and centre
+  std::vector<uint32_t> taken;  // a commit's accepted candidates, in order
+  // A candidate's tests against the rung's voids as its commit begins, and
+  // the effects of the batch's earlier candidates on it (n_eff < 0: more
+  // than fit, probed in order instead)
+  constexpr int EFF_CAP = 12;
   struct PreProbe {
     bool node, pole, pole_r;
     int8_t state, state_r;
+    int8_t n_eff;
+    PairEffect eff[EFF_CAP];
   };
   std::vector<PreProbe> pre;
+  std::vector<uint8_t> is_taken;  // accepted, by batch place
   st = g_dev.mask.reserve((total + 63) / 64 * sizeof(uint64_t));
   if (st == SIF_OK)
     st = reserve_slots();
@@ -663,11 +690,13 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // batching.
     uint32_t cap = BATCH_FLOOR;
     uint64_t k = 0;
-    const uint64_t nn = n;
+    // flat < n^3 <= 2^32: 32-bit arithmetic
+    const uint32_t nn = n, nn2 = n * n;
     auto node_of = [&](uint32_t flat, uint32_t* ix, uint32_t* iy, uint32_t* iz) {
-      *ix = (uint32_t)(flat / (nn * nn));
-      *iy = (uint32_t)((flat / nn) % nn);
-      *iz = (uint32_t)(flat % nn);
+      *ix = flat / nn2;
+      const uint32_t rest = flat - *ix * nn2;
+      *iy = rest / nn;
+      *iz = rest - *iy * nn;
     };
     // An exact overlap with the catalogue as it stands (utils.c:850-948),
     // from the overlap state of the voids the rung began with (old_state,
@@ -776,9 +805,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // exodus_finder.c:2174-2329, in candidate order. Every test is an OR over
     // voids, so it splits by the voids' age: the rung's older voids (the
     // device answered: prefilter, grown-radius state and pole bit), the rung's
-    // new voids as the commit begins (probed for every candidate at once, on
-    // the host pool), and the voids this commit accepts (probed in order, in
-    // a list of their own).
+    // new voids as the commit begins, and the voids this commit accepts --
+    // the batch's earlier candidates, whose effects on each candidate are
+    // listed before the commit (batch_pairs.h). The first two run for every
+    // candidate at once on the host pool; the commit in order then ORs the
+    // effects of the candidates it has accepted.
     auto combine = [](int a, int b) { return (a == 1 || b == 1) ? 1 : (a | b); };
     auto commit = [&](BatchSlot& B, uint64_t* accepted_out) -> int {
       SIF_HIP(hipEventSynchronize(B.done));
@@ -791,44 +822,143 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
             (double)radius);
           return SIF_ERR_RANGE;
         }
+      nodes.resize(B.count);
+      host_pool().run(B.count, [&](uint64_t b0, uint64_t b1) {
+        for (uint64_t b = b0; b < b1; b++) {
+          Node& N = nodes[b];
+          node_of(B.h_flat[b], &N.ix, &N.iy, &N.iz);
+          N.x = (float)N.ix * l;
+          N.y = (float)N.iy * l;
+          N.z = (float)N.iz * l;
+        }
+      });
+      BI.clear();
+      bi_place.clear();
+      for (uint32_t b = 0; b < B.count; b++)
+        if (B.h_rad[b] >= 0.0f) {
+          BI.add(nodes[b].x, nodes[b].y, nodes[b].z, B.h_rad[b]);
+          bi_place.push_back(b);
+        }
+      const float cat_max_r = V.max_r;
       pre.resize(B.count);
       host_pool().run(B.count, [&](uint64_t b0, uint64_t b1) {
         for (uint64_t b = b0; b < b1; b++) {
-          uint32_t ix, iy, iz;
-          node_of(B.h_flat[b], &ix, &iy, &iz);
-          const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+          const Node& N = nodes[b];
+          const bool edge = B.edge[b] != 0;
           OverlapQuery q;
-          overlap_query(q, cx, cy, cz, radius, V.max_r, R.box, R.f, scale, V.inv_l);
-          const RungProbe p1 = RV.probe(&q, ix, iy, iz, cx, cy, cz, true, proxy, scale, l, n);
+          overlap_query(q, N.x, N.y, N.z, radius, cat_max_r, R.box, R.f, scale, V.inv_l);
+          const RungProbe p1 =
+            RV.probe(&q, N.ix, N.iy, N.iz, N.x, N.y, N.z, true, proxy, scale, l, n);
           PreProbe& P = pre[b];
           P.node = p1.node;
           P.pole = p1.pole;
           P.state = (int8_t)p1.state;
           P.pole_r = false;
           P.state_r = 0;
+          P.n_eff = 0;
           const float r = B.h_rad[b];
-          if (r >= 0.0f) {
-            const int32_t pole = (int32_t)(scale * r * (1.0f - R.f) / l) - 1;
-            OverlapQuery q2;
-            overlap_query(q2, cx, cy, cz, r, V.max_r, R.box, R.f, scale, V.inv_l);
-            const RungProbe p2 = RV.probe(&q2, ix, iy, iz, cx, cy, cz, false, pole, scale, l, n);
-            P.pole_r = p2.pole;
-            P.state_r = (int8_t)p2.state;
+          const int32_t pole_r = r >= 0.0f ? (int32_t)(scale * r * (1.0f - R.f) / l) - 1 : 0;
+          const uint32_t gs = B.h_state[b];
+          // refused at the rung radius by the older voids alone
+          const bool rung_refused = p1.node || (proxy > 0 && p1.pole) || p1.state == 1;
+          if (r >= 0.0f && !rung_refused) {
+            if (pole_r > 0 && (gs & GROWN_POLE_BIT)) {
+              P.pole_r = true;
+            } else {
+              OverlapQuery q2;
+              overlap_query(q2, N.x, N.y, N.z, r, cat_max_r, R.box, R.f, scale, V.inv_l);
+              const RungProbe p2 = RV.probe((gs & 3u) == 1u ? nullptr : &q2, N.ix, N.iy, N.iz,
+                N.x, N.y, N.z, false, pole_r, scale, l, n);
+              P.pole_r = p2.pole;
+              P.state_r = (int8_t)p2.state;
+            }
           }
+          // what the batch's earlier candidates can still change, test by
+          // test in the commit's order: node, pole, overlap, then at the
+          // grown radius pole and overlap
+          uint8_t need = 0;
+          if (!P.node) {
+            need |= NEED_NODE;
+            if (!(proxy > 0 && P.pole)) {
+              if (proxy > 0)
+                need |= NEED_POLE;
+              if (!edge && P.state != 1)
+                need |= NEED_STATE;
+              if (r >= 0.0f && P.state != 1) {
+                const bool pole_refused = pole_r > 0 && ((gs & GROWN_POLE_BIT) || P.pole_r);
+                if (pole_r > 0 && !pole_refused)
+                  need |= NEED_POLE_R;
+                if (!pole_refused && (gs & 3u) == 0u && P.state_r != 1)
+                  need |= NEED_STATE_R;
+              }
+            }
+          }
+          PairQuery pq;
+          pq.b = (uint32_t)b;
+          pq.ix = N.ix;
+          pq.iy = N.iy;
+          pq.iz = N.iz;
+          pq.x = N.x;
+          pq.y = N.y;
+          pq.z = N.z;
+          pq.r = r;
+          pq.pole = proxy;
+          pq.pole_r = pole_r;
+          pq.need = need;
+          P.n_eff = (int8_t)pair_effects(BI, bi_place.data(), pq, radius, cat_max_r, R.f, scale,
+            l, n, P.eff, EFF_CAP);
         }
       });
+      // In order. The voids accepted join the catalogue at once; the list of
+      // this commit's voids (CL) is filled only when a candidate with more
+      // effects than were listed needs it.
       CL.clear();
+      size_t cl_filled = 0;
+      taken.clear();
+      is_taken.assign(B.count, 0);
       uint64_t accepted = 0;
       for (uint32_t b = 0; b < B.count; b++) {
         const PreProbe& P = pre[b];
-        uint32_t ix, iy, iz;
-        node_of(B.h_flat[b], &ix, &iy, &iz);
-        const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+        if (P.node) {
+          count_refusal(1);
+          continue;
+        }
+        const Node& N = nodes[b];
+        const float r = B.h_rad[b];
+        const int32_t pole = r >= 0.0f ? (int32_t)(scale * r * (1.0f - R.f) / l) - 1 : 0;
+        // the effects of this commit's voids: listed, or probed when too many
+        RungProbe c1, c2;
+        if (P.n_eff >= 0) {
+          for (int e = 0; e < P.n_eff; e++) {
+            const PairEffect& E = P.eff[e];
+            if (!is_taken[E.a])
+              continue;
+            c1.node |= E.node;
+            c1.pole |= E.pole;
+            c1.state = combine(c1.state, E.state);
+            c2.pole |= E.pole_r;
+            c2.state = combine(c2.state, E.state_r);
+          }
+        } else {
+          for (; cl_filled < taken.size(); cl_filled++) {
+            const uint32_t a = taken[cl_filled];
+            CL.add(nodes[a].x, nodes[a].y, nodes[a].z, B.h_rad[a]);
+          }
+          const bool pole_refuses = proxy > 0 && P.pole;
+          OverlapQuery q;
+          overlap_query(q, N.x, N.y, N.z, radius, V.max_r, R.box, R.f, scale, V.inv_l);
+          c1 = CL.probe(pole_refuses || P.state == 1 ? nullptr : &q, N.ix, N.iy, N.iz, N.x, N.y,
+            N.z, true, pole_refuses ? 0 : proxy, scale, l, n);
+          if (!c1.node && !(proxy > 0 && (P.pole || c1.pole)) && r >= 0.0f) {
+            const uint32_t gs = B.h_state[b];
+            OverlapQuery q2;
+            overlap_query(q2, N.x, N.y, N.z, r, V.max_r, R.box, R.f, scale, V.inv_l);
+            c2 = CL.probe((gs & 3u) == 1u || P.state_r == 1 ? nullptr : &q2, N.ix, N.iy, N.iz,
+              N.x, N.y, N.z, false, pole, scale, l, n);
+          }
+        }
         // the rung-radius tests (mask, pole probe, overlap)
-        OverlapQuery q;
-        overlap_query(q, cx, cy, cz, radius, V.max_r, R.box, R.f, scale, V.inv_l);
-        const RungProbe c1 = CL.probe(&q, ix, iy, iz, cx, cy, cz, true, proxy, scale, l, n);
-        if (P.node || c1.node) {
+        if (c1.node) {
           count_refusal(1);
           continue;
         }
@@ -836,34 +966,33 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           count_refusal(2);
           continue;
         }
-        if (exact_overlap(cx, cy, cz, radius, B.edge[b] ? 2u : 0u, combine(P.state, c1.state))) {
+        if (exact_overlap(N.x, N.y, N.z, radius, B.edge[b] ? 2u : 0u,
+              combine(P.state, c1.state))) {
           count_refusal(3);
           continue;
         }
-        const float r = B.h_rad[b];
         if (r < 0.0f) {
           S.rescale_fail++;
           continue;
         }
         // the tests at the rescaled radius (exodus_finder.c:2290-2311)
-        const int32_t pole = (int32_t)(scale * r * (1.0f - R.f) / l) - 1;
         const uint32_t gs = B.h_state[b];
-        OverlapQuery q2;
-        overlap_query(q2, cx, cy, cz, r, V.max_r, R.box, R.f, scale, V.inv_l);
-        const RungProbe c2 = CL.probe(&q2, ix, iy, iz, cx, cy, cz, false, pole, scale, l, n);
         if (pole > 0 && ((gs & GROWN_POLE_BIT) || P.pole_r || c2.pole)) {
           S.proxy++;
           continue;
         }
-        if (exact_overlap(cx, cy, cz, r, gs & 3u, combine(P.state_r, c2.state))) {
+        if (exact_overlap(N.x, N.y, N.z, r, gs & 3u, combine(P.state_r, c2.state))) {
           S.overlap++;
           continue;
         }
-        V.add(cx, cy, cz, r, B.h_nin[b]);
-        RV.add(cx, cy, cz, r);
-        CL.add(cx, cy, cz, r);
+        V.add(N.x, N.y, N.z, r, B.h_nin[b]);
+        is_taken[b] = 1;
+        taken.push_back(b);
         accepted++;
       }
+      // the rung's list, for the next commits' probes and the mask
+      for (uint32_t a : taken)
+        RV.add(nodes[a].x, nodes[a].y, nodes[a].z, B.h_rad[a]);
       S.accepted += accepted;
       *accepted_out = accepted;
       if (timing_active()) {

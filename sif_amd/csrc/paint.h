@@ -52,6 +52,8 @@ SIF_HD float paint_dist(const PaintSphere& P, int32_t g, float centre) {
 }
 
 SIF_HD uint32_t paint_wrap(int32_t g, int32_t n) {
+  if (g >= -n && g < 2 * n)  // within a box of the grid: no division
+    return (uint32_t)(g < 0 ? g + n : (g >= n ? g - n : g));
   const int32_t w = g % n;
   return (uint32_t)(w < 0 ? w + n : w);
 }

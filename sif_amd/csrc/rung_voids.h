@@ -5,7 +5,8 @@
 // (the overlap state of overlap.h), do they paint a candidate's node or one
 // of its six pole nodes (the mask and the pole probe of
 // sif__overlap_quick, src/finder/utils.c:819-848) -- probe() answers in one
-// pass over the cells around the candidate.
+// pass over the cells around the candidate. walk() visits the voids of any
+// such reach one by one, for callers that need each void's answer.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -22,6 +23,64 @@ struct RungProbe {
   bool pole = false;  // a new void paints one of its six pole nodes
 };
 
+// The six nodes `pole` away from (ix, iy, iz) along the axes, wrapped.
+struct PoleNodes {
+  uint32_t px[2], py[2], pz[2];
+  void set(uint32_t ix, uint32_t iy, uint32_t iz, int32_t pole, uint32_t n) {
+    px[0] = paint_wrap((int32_t)ix + pole, (int32_t)n);
+    px[1] = paint_wrap((int32_t)ix - pole, (int32_t)n);
+    py[0] = paint_wrap((int32_t)iy + pole, (int32_t)n);
+    py[1] = paint_wrap((int32_t)iy - pole, (int32_t)n);
+    pz[0] = paint_wrap((int32_t)iz + pole, (int32_t)n);
+    pz[1] = paint_wrap((int32_t)iz - pole, (int32_t)n);
+  }
+  bool painted(const PaintSphere& P, uint32_t ix, uint32_t iy, uint32_t iz) const {
+    return paint_hits(P, px[0], iy, iz) || paint_hits(P, px[1], iy, iz) ||
+           paint_hits(P, ix, py[0], iz) || paint_hits(P, ix, py[1], iz) ||
+           paint_hits(P, ix, iy, pz[0]) || paint_hits(P, ix, iy, pz[1]);
+  }
+};
+
+// Cells of the 32^3 list, per axis before wrapping: a union of reaches.
+struct CellRegion {
+  int32_t lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+  bool empty() const { return hi[0] < lo[0]; }
+  void merge(const int32_t l3[3], const int32_t h3[3]) {
+    for (int a = 0; a < 3; a++) {
+      if (hi[a] < lo[a]) {
+        lo[a] = l3[a];
+        hi[a] = h3[a];
+      } else {
+        lo[a] = l3[a] < lo[a] ? l3[a] : lo[a];
+        hi[a] = h3[a] > hi[a] ? h3[a] : hi[a];
+      }
+      if (hi[a] - lo[a] >= VOID_CELLS) {  // the whole axis, each cell once
+        lo[a] = 0;
+        hi[a] = VOID_CELLS - 1;
+      }
+    }
+  }
+  // every void whose overlap with q may hold, for a set of largest radius
+  // set_max_r (overlap.h)
+  void add_overlap(const OverlapQuery& q, float set_max_r, float inv_l) {
+    int32_t l3[3], h3[3];
+    superset_scan(q, set_max_r, inv_l, l3, h3);
+    merge(l3, h3);
+  }
+  // every void that may paint a node within `reach` of (x, y, z) on each
+  // axis, widened by a margin far above float rounding
+  void add_paint(float x, float y, float z, float reach, float box, float inv_l) {
+    const float s = reach + reach * 1e-5f + box * 1e-4f;
+    const float c3[3] = {x, y, z};
+    int32_t l3[3], h3[3];
+    for (int a = 0; a < 3; a++) {
+      l3[a] = (int32_t)floorf((c3[a] - s) * inv_l);
+      h3[a] = (int32_t)floorf((c3[a] + s) * inv_l);
+    }
+    merge(l3, h3);
+  }
+};
+
 struct RungVoids {
   // one cache line's half per void: what a probe reads of it
   struct alignas(32) Rec {
@@ -30,6 +89,7 @@ struct RungVoids {
     uint32_t cell;  // its cell, (wx 32 + wy) 32 + wz
   };
   std::vector<float> cx, cy, cz, r;  // the same voids as columns, for upload
+  bool columns = true;                // kept only when asked for
   std::vector<Rec> rec;
   std::vector<int32_t> head;
   std::vector<uint32_t> touched;
@@ -41,6 +101,7 @@ struct RungVoids {
     head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
     occupied.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS / 64, 0ull);
   }
+  size_t size() const { return rec.size(); }
   void clear() {
     for (uint32_t c : touched) {
       head[c] = -1;
@@ -55,11 +116,13 @@ struct RungVoids {
     max_r = 0.0f;
   }
   void add(float x, float y, float z, float rr) {
-    const int32_t k = (int32_t)cx.size();
-    cx.push_back(x);
-    cy.push_back(y);
-    cz.push_back(z);
-    r.push_back(rr);
+    const int32_t k = (int32_t)rec.size();
+    if (columns) {
+      cx.push_back(x);
+      cy.push_back(y);
+      cz.push_back(z);
+      r.push_back(rr);
+    }
     const uint32_t c =
       (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
       void_axis_cell(z, inv_l);
@@ -73,6 +136,56 @@ struct RungVoids {
       max_r = rr;
   }
 
+  // Calls visit(void index, record, wa, wb, wc) for every void stored in the
+  // region's cells (wrapped cell coordinates wa, wb, wc), in no particular
+  // order, until visit returns false. A list shorter than the region's rows
+  // is walked directly, testing each void's cell against the region, which
+  // visits the same voids as the scan of the cells.
+  template <class Visit>
+  void walk(const CellRegion& R, Visit visit) const {
+    if (rec.empty() || R.empty())
+      return;
+    const uint32_t span[3] = {(uint32_t)(R.hi[0] - R.lo[0]), (uint32_t)(R.hi[1] - R.lo[1]),
+      (uint32_t)(R.hi[2] - R.lo[2])};
+    const uint32_t rows = (span[0] + 1) * (span[1] + 1);
+    if (rec.size() <= rows) {
+      const uint32_t w0[3] = {void_wrap(R.lo[0]), void_wrap(R.lo[1]), void_wrap(R.lo[2])};
+      for (size_t i = 0; i < rec.size(); i++) {
+        const Rec& v = rec[i];
+        const uint32_t wa = v.cell >> 10, wb = (v.cell >> 5) & 31u, wc = v.cell & 31u;
+        if (((wa - w0[0]) & 31u) > span[0] || ((wb - w0[1]) & 31u) > span[1] ||
+            ((wc - w0[2]) & 31u) > span[2])
+          continue;
+        if (!visit((uint32_t)i, v, wa, wb, wc))
+          return;
+      }
+      return;
+    }
+    // the z range as a mask over a row's 32 cells (wrapping)
+    uint32_t zmask = 0xffffffffu;
+    if (span[2] + 1 < (uint32_t)VOID_CELLS) {
+      const uint64_t m = ((1ull << (span[2] + 1)) - 1ull) << void_wrap(R.lo[2]);
+      zmask = (uint32_t)(m | (m >> VOID_CELLS));
+    }
+    for (int32_t a = R.lo[0]; a <= R.hi[0]; a++) {
+      const uint32_t wa = void_wrap(a);
+      for (int32_t b = R.lo[1]; b <= R.hi[1]; b++) {
+        const uint32_t wb = void_wrap(b);
+        // a row of 32 cells is half an occupancy word: only its set cells
+        const uint32_t row = wa * VOID_CELLS + wb;
+        uint32_t cells = (uint32_t)(occupied[row >> 1] >> ((row & 1u) * 32u)) & zmask;
+        while (cells) {
+          const uint32_t wc = (uint32_t)__builtin_ctz(cells);
+          cells &= cells - 1u;
+          const uint32_t cell = row * VOID_CELLS + wc;
+          for (int32_t i = head[cell]; i != -1; i = rec[i].next)
+            if (!visit((uint32_t)i, rec[i], wa, wb, wc))
+              return;
+        }
+      }
+    }
+  }
+
   // The candidate sits on node (ix, iy, iz) of an n^3 grid of spacing l, at
   // (x, y, z). q: the overlap query, or null for none. want_node: test the
   // node's own mask bit. pole > 0: test the six nodes pole away along the
@@ -82,9 +195,7 @@ struct RungVoids {
   // the probe stops once its answer can no longer change that: at a node hit,
   // or at a pole hit when the node is not asked; after a pole hit it no
   // longer needs the state. Every test is an OR over voids, so the voids may
-  // be visited in any order; a short list is walked directly, testing each
-  // void's cell against the reach, which visits the same voids as the scan
-  // of the cells.
+  // be visited in any order.
   RungProbe probe(const OverlapQuery* q, uint32_t ix, uint32_t iy, uint32_t iz, float x,
     float y, float z, bool want_node, int32_t pole, float scale, float l, uint32_t n) const {
     RungProbe out;
@@ -96,34 +207,15 @@ struct RungVoids {
     // reach: the overlap superset, and painting (a void paints only nodes
     // within scale r + 2 l of its centre on each axis) around the node and
     // its poles
-    int32_t lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+    CellRegion R;
     if (q)
-      superset_scan(*q, max_r, inv_l, lo, hi);
-    if (want_node || want_pole) {
-      const float paint_reach = scale * max_r + 2.0f * l + (want_pole ? (float)pole * l : 0.0f);
-      const float s = paint_reach + paint_reach * 1e-5f + box * 1e-4f;
-      const float c3[3] = {x, y, z};
-      for (int a = 0; a < 3; a++) {
-        int32_t plo = (int32_t)floorf((c3[a] - s) * inv_l);
-        int32_t phi = (int32_t)floorf((c3[a] + s) * inv_l);
-        if (hi[a] < lo[a]) {
-          lo[a] = plo;
-          hi[a] = phi;
-        } else {
-          lo[a] = plo < lo[a] ? plo : lo[a];
-          hi[a] = phi > hi[a] ? phi : hi[a];
-        }
-        if (hi[a] - lo[a] >= VOID_CELLS) {
-          lo[a] = 0;
-          hi[a] = VOID_CELLS - 1;
-        }
-      }
-    }
-    // the six pole nodes, wrapped when a first void needs them
-    uint32_t px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
-    bool poles_ready = false;
-    // one void: false once the answer is decided
-    auto visit = [&](const Rec& v, uint32_t wa, uint32_t wb, uint32_t wc) -> bool {
+      R.add_overlap(*q, max_r, inv_l);
+    if (want_node || want_pole)
+      R.add_paint(x, y, z, scale * max_r + 2.0f * l + (want_pole ? (float)pole * l : 0.0f),
+        box, inv_l);
+    PoleNodes poles;
+    bool poles_ready = false;  // wrapped when a first void needs them
+    walk(R, [&](uint32_t, const Rec& v, uint32_t wa, uint32_t wb, uint32_t wc) {
       if (q && out.state != 1 && !(want_pole && out.pole)) {
         const int s = pair_state(*q, wa, wb, wc, v.x, v.y, v.z, v.r);
         out.state = s == 1 ? 1 : (out.state | s);
@@ -133,60 +225,16 @@ struct RungVoids {
         paint_sphere(P, v.x, v.y, v.z, scale * v.r, l, n);
         if (want_node && !out.node)
           out.node = paint_hits(P, ix, iy, iz);
-        if (want_pole && !out.pole && !poles_ready) {
-          px[0] = paint_wrap((int32_t)ix + pole, (int32_t)n);
-          px[1] = paint_wrap((int32_t)ix - pole, (int32_t)n);
-          py[0] = paint_wrap((int32_t)iy + pole, (int32_t)n);
-          py[1] = paint_wrap((int32_t)iy - pole, (int32_t)n);
-          pz[0] = paint_wrap((int32_t)iz + pole, (int32_t)n);
-          pz[1] = paint_wrap((int32_t)iz - pole, (int32_t)n);
-          poles_ready = true;
+        if (want_pole && !out.pole) {
+          if (!poles_ready) {
+            poles.set(ix, iy, iz, pole, n);
+            poles_ready = true;
+          }
+          out.pole = poles.painted(P, ix, iy, iz);
         }
-        if (want_pole && !out.pole)
-          out.pole = paint_hits(P, px[0], iy, iz) || paint_hits(P, px[1], iy, iz) ||
-                     paint_hits(P, ix, py[0], iz) || paint_hits(P, ix, py[1], iz) ||
-                     paint_hits(P, ix, iy, pz[0]) || paint_hits(P, ix, iy, pz[1]);
       }
       return !(out.node || (out.pole && !want_node));
-    };
-    const uint32_t span[3] = {(uint32_t)(hi[0] - lo[0]), (uint32_t)(hi[1] - lo[1]),
-      (uint32_t)(hi[2] - lo[2])};
-    const uint32_t rows = (span[0] + 1) * (span[1] + 1);
-    if (rec.size() <= rows) {
-      const uint32_t w0[3] = {void_wrap(lo[0]), void_wrap(lo[1]), void_wrap(lo[2])};
-      for (const Rec& v : rec) {
-        const uint32_t wa = v.cell >> 10, wb = (v.cell >> 5) & 31u, wc = v.cell & 31u;
-        if (((wa - w0[0]) & 31u) > span[0] || ((wb - w0[1]) & 31u) > span[1] ||
-            ((wc - w0[2]) & 31u) > span[2])
-          continue;
-        if (!visit(v, wa, wb, wc))
-          return out;
-      }
-      return out;
-    }
-    // the z range as a mask over a row's 32 cells (wrapping)
-    uint32_t zmask = 0xffffffffu;
-    if (span[2] + 1 < (uint32_t)VOID_CELLS) {
-      const uint64_t m = ((1ull << (span[2] + 1)) - 1ull) << void_wrap(lo[2]);
-      zmask = (uint32_t)(m | (m >> VOID_CELLS));
-    }
-    for (int32_t a = lo[0]; a <= hi[0]; a++) {
-      const uint32_t wa = void_wrap(a);
-      for (int32_t b = lo[1]; b <= hi[1]; b++) {
-        const uint32_t wb = void_wrap(b);
-        // a row of 32 cells is half an occupancy word: only its set cells
-        const uint32_t row = wa * VOID_CELLS + wb;
-        uint32_t cells = (uint32_t)(occupied[row >> 1] >> ((row & 1u) * 32u)) & zmask;
-        while (cells) {
-          const uint32_t wc = (uint32_t)__builtin_ctz(cells);
-          cells &= cells - 1u;
-          const uint32_t cell = row * VOID_CELLS + wc;
-          for (int32_t i = head[cell]; i != -1; i = rec[i].next)
-            if (!visit(rec[i], wa, wb, wc))
-              return out;
-        }
-      }
-    }
+    });
     return out;
   }
 };
