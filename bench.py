@@ -77,34 +77,53 @@ class Step:
         return cat, stats
 
 
-# stage -> the kernel(s) its HIP events bracket
+# stage -> (the kernels one timed launch of it runs, how many)
 STAGE_KERNELS = {
-    "grow": "grow_kernel",
-    "smooth": "cols_kernel x2 + c2r_rows_kernel (one filtered c2r)",
-    "cic": "cic_count/bin/tile_kernel + cic_finish_kernel (tile binning, LDS deposit, flush)",
-    "mesh": "mesh_key_kernel + radix sort + mesh_gather_kernel",
-    "scan": "cand_count_kernel + tile scan + cand_write_kernel",
-    "fft_forward": "r2c_rows_kernel + cols_kernel x2",
+    "grow": ("grow_kernel", 1),
+    "smooth": ("cols_kernel x2 + c2r_rows_kernel (one filtered c2r)", 3),
+    "cic": ("cic_count/bin/tile_kernel + cic_finish_kernel (tile binning, LDS deposit, flush)", 4),
+    "mesh": ("mesh_key_kernel + radix sort + mesh_gather_kernel", 17),
+    "scan": ("cand_count_kernel + tile scan + cand_write_kernel", 3),
+    "fft_forward": ("r2c_rows_kernel + cols_kernel x2", 3),
 }
 
 
-def roofline(timing):
+# HBM traffic per launch of a kernel, measured with rocprof's FETCH_SIZE and
+# WRITE_SIZE counters scaled against a copy kernel of known size; the summary
+# that holds it names the commands
+TRAFFIC = {("plain512", "grow"): "profiles/grow_kernel_hbm_traffic_plain512.md"}
+
+
+def measured_traffic(config, stage):
+    path = TRAFFIC.get((config, stage))
+    if not path or not os.path.exists(os.path.join(ROOT, path)):
+        return None
+    for line in open(os.path.join(ROOT, path)):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def roofline(timing, config):
     """Achieved HBM rate of each timed stage whose algorithmic bytes are
-    known (bytes / event-timed duration over the timed region), and the one
-    that takes the most device time."""
+    known (bytes / event-timed duration over the timed region), and that of
+    the dominant kernel: the stage with the most device time per kernel it
+    runs (a stage of several kernels is timed as one region; the smoothing's
+    runs on a second stream beside the commit)."""
     stages = {}
     for name, (ms, launches, by) in timing.items():
         if name not in STAGE_KERNELS or launches == 0 or by <= 0 or ms <= 0:
             continue
         gbs = by / (ms * 1e-3) / 1e9
-        stages[name] = {"kernel": STAGE_KERNELS[name], "launches": launches,
+        stages[name] = {"kernel": STAGE_KERNELS[name][0], "launches": launches,
                         "avg_launch_ms": round(ms / launches, 4),
                         "bytes_per_launch": round(by / launches),
                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    top = max(stages, key=lambda k: timing[k][0])
+    top = max(stages, key=lambda k: timing[k][0] / STAGE_KERNELS[k][1])
     d = stages[top]
     return {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved"],
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+            "traffic": measured_traffic(config, top),
             "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
             "stages": stages}
 
@@ -140,6 +159,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="plain512", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device, 4 times: the known traffic that "
+                         "scales rocprof's FETCH_SIZE and WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU figures are not part of "
@@ -150,6 +172,15 @@ def main():
     if not N.has_gpu():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        from sif_amd._native import DeviceArray
+        src = DeviceArray((1 << 28,), np.float32)
+        dst = DeviceArray((1 << 28,), np.float32)
+        N.check(N.lib.sif_amd_memset(src.ptr, 1, 1 << 30))
+        for _ in range(4):
+            N.check(N.lib.sif_amd_copy(dst.ptr, src.ptr, 1 << 30))
+        log("[bench] copy probe: 4 launches of copy_kernel, 1073741824 B read and written each")
+        return
 
     c, x, y, z, radii, mesh_cells = workload(args.config)
     step = Step(c, x, y, z, radii, mesh_cells)
@@ -176,7 +207,7 @@ def main():
     log("[bench] candidates over all rungs: " + ", ".join(f"{k} {v}" for k, v in tot.items()))
     log("[bench] stage times over the timed steps (ms, launches): " +
         ", ".join(f"{k} {v[0]:.1f}/{v[1]}" for k, v in timing.items()))
-    roof = roofline(timing)
+    roof = roofline(timing, args.config)
 
     cpu = None
     if not args.no_cpu:

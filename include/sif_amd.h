@@ -29,6 +29,10 @@ SIF_API int sif_amd_memcpy_h2d(void* d_dst, const void* src, uint64_t bytes);
 SIF_API int sif_amd_memcpy_d2h(void* dst, const void* d_src, uint64_t bytes);
 SIF_API int sif_amd_memset(void* d_dst, int value, uint64_t bytes);
 SIF_API int sif_amd_synchronize(void);
+/* A plain device-to-device copy kernel (16-byte loads and stores, bytes a
+ * multiple of 16, both pointers 16-byte aligned): traffic of known size, the
+ * yardstick rocprof's FETCH_SIZE and WRITE_SIZE counters are scaled by. */
+SIF_API int sif_amd_copy(void* d_dst, const void* d_src, uint64_t bytes);
 
 /* Per-stage device time (HIP events on the library stream, recorded without
  * synchronising and resolved when read). Stage order:

@@ -178,6 +178,14 @@ void timing_add(int s, double ms) {
 
 void timing_bytes(int s, double bytes) { g_bytes[s] += bytes; }
 
+// grid-stride copy, one 16-byte element per thread per step
+__global__ void __launch_bounds__(256) copy_kernel(uint4* __restrict__ dst,
+  const uint4* __restrict__ src, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
 }  // namespace sif_amd
 
 using namespace sif_amd;
@@ -266,6 +274,24 @@ int sif_amd_memset(void* dst, int value, uint64_t bytes) {
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipMemsetAsync(dst, value, bytes, g_stream));
+  SIF_HIP(hipStreamSynchronize(g_stream));
+  return SIF_OK;
+}
+
+int sif_amd_copy(void* dst, const void* src, uint64_t bytes) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if ((bytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u)) {
+    error_set(SIF_ERR_INVALID, "sif_amd_copy: bytes and both pointers must be 16-byte aligned");
+    return SIF_ERR_INVALID;
+  }
+  if (bytes == 0)
+    return SIF_OK;
+  const uint64_t n = bytes / 16u;
+  copy_kernel<<<grid_for(n, 256) < 8192 ? grid_for(n, 256) : 8192, 256, 0, g_stream>>>(
+    static_cast<uint4*>(dst), static_cast<const uint4*>(src), n);
+  SIF_HIP(hipGetLastError());
   SIF_HIP(hipStreamSynchronize(g_stream));
   return SIF_OK;
 }
