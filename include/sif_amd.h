@@ -153,6 +153,36 @@ SIF_API sif_catalogue_t* sif_amd_exodus_device(float* d_delta,
   sif_real threshold, sif_real overlap_fraction, sif_option options,
   int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats);
 
+/* --- profiles of a grid, and the adjoint of the profile chain --- */
+/* sif_profiles (src/measure/profiles.c:354-673), density part, cumulative
+ * (profiles.c:332-350, 619-635), of a density-contrast grid d_delta
+ * (n_cells^3, node g at g * box / n_cells) whose every node is a mass element
+ * (1 + delta) rho_bar l^3, binned as profiles.c:202-223 bins a tracer:
+ * d_profiles[c * n_bins + j] = l^3 sum over nodes of bins <= j of
+ * (1 + delta) / ((4/3) pi ((j+1) ext / n_bins)^3 rv^3) - 1, for the centres
+ * (d_cx, d_cy, d_cz) of radius d_rv out to ext * rv; a centre of radius <= 0
+ * gets a zero row. 1 <= n_bins <= 32. */
+SIF_API int sif_amd_grid_profiles(const float* d_delta, uint32_t n_cells,
+  float box, const float* d_cx, const float* d_cy, const float* d_cz,
+  const float* d_rv, uint32_t n_centres, float ext, uint32_t n_bins,
+  float* d_profiles);
+/* Its transpose (no reference counterpart: the adjoint of the profiles):
+ * d_grad_delta[g] = sum over c, j of d_grad_profiles[c * n_bins + j]
+ * * d profiles[c, j] / d delta[g], every node written. n_cells a multiple of
+ * 16. Deterministic: no float atomics. */
+SIF_API int sif_amd_grid_profiles_adjoint(const float* d_grad_profiles,
+  uint32_t n_cells, float box, const float* d_cx, const float* d_cy,
+  const float* d_cz, const float* d_rv, uint32_t n_centres, float ext,
+  uint32_t n_bins, float* d_grad_delta);
+/* The transpose of the CIC deposit (no reference counterpart; the deposit is
+ * grid_compute_cic, src/structures/grid.c:585-650): for the density per unit
+ * volume rho = sif_amd_grid_assign_cic(x, y, z) and a grid d_grad_rho, the
+ * gradient of sum_g d_grad_rho[g] rho[g] with respect to each particle's
+ * position, into d_gx, d_gy, d_gz. */
+SIF_API int sif_amd_cic_adjoint(const float* d_x, const float* d_y,
+  const float* d_z, uint64_t n, uint32_t n_cells, float box,
+  const float* d_grad_rho, float* d_gx, float* d_gy, float* d_gz);
+
 /* Per-rung smoothed fields as the finder scans them, for parity checks:
  * rung i (descending ladder) into d_fields + i * n_cells^3. */
 SIF_API int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells,

@@ -7,6 +7,7 @@ a different implementation.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -30,6 +31,12 @@ if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} is missing: build it with `python -m sif_amd.build` "
         "(the HIP library is the only implementation of this package)")
+
+# torch ships its own HIP runtime under the same soname as the system's. A
+# library loaded after torch binds to torch's runtime, and the two share the
+# device; torch imported after the library would load a second runtime, which
+# then sees no device. sif_amd.autograd checks this.
+TORCH_FIRST = "torch" in sys.modules
 
 lib = C.CDLL(LIB_PATH)
 
@@ -89,6 +96,9 @@ _sig("sif_amd_fft_r2c", i32, P, u32, P, P)
 _sig("sif_amd_fft_filter_c2r", i32, P, u32, f32, i32, f32, f32, i32, i32, P, P)
 _sig("sif_amd_scan_candidates", i32, P, P, u64, f32, P, P, u64, C.POINTER(u64))
 _sig("sif_amd_mark_spheres", i32, P, u32, f32, P, P, P, P, u32, f32)
+_sig("sif_amd_grid_profiles", i32, P, u32, f32, P, P, P, P, u32, f32, u32, P)
+_sig("sif_amd_grid_profiles_adjoint", i32, P, u32, f32, P, P, P, P, u32, f32, u32, P)
+_sig("sif_amd_cic_adjoint", i32, P, P, P, u64, u32, f32, P, P, P, P)
 _sig("sif_amd_find_radius", i32, P, P, P, P, u32, f32, P, P, P, u32, f32, f32, f32, f32,
      P, P, P)
 _sig("sif_amd_exodus_device", C.POINTER(Catalogue), P, u32, f32, P, P, P, P, u32, u64,

@@ -14,6 +14,13 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def native():
+    # torch first, when it is installed: the library then shares torch's HIP
+    # runtime, and the tests that hand it torch tensors can run in the same
+    # process as the others
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     from sif_amd import _native
     if not _native.has_gpu():
         pytest.skip("no GPU visible")
