@@ -152,6 +152,213 @@ def cpu_baseline(name="sample256"):
                       f"{'/'.join(f'{s:.2f}' for s in stage)} s"}
 
 
+# the differentiable profile chain: 256^3 Zel'dovich particles in a
+# 500 Mpc/h box, 256^3 grid, Gaussian R = 3 Mpc/h, 1e5 centres uniform in the
+# box with radii uniform in 5-40 Mpc/h, 16 cumulative bins to 3R, loss the sum
+# of squared residuals against a seeded perturbation of the forward profile
+ADJOINT = dict(n_side=256, box=500.0, n_cells=256, r_smooth=3.0, n_centres=100_000,
+               r_lo=5.0, r_hi=40.0, ext=3.0, n_bins=16, seed=4, centre_seed=41,
+               radius_seed=42, target_seed=43)
+
+
+class AdjointChain:
+    """The chain on device-resident arrays through the C ABI: forward
+    (deposit, contrast, smoothing, profiles, loss) and adjoint (the loss's
+    gradient, the profiles' transpose, the same smoothing, the deposit's
+    transpose). The adjoint stores no field of the forward."""
+
+    def __init__(self):
+        import ctypes as C
+        from sif_amd import _native as N, zeldovich
+        from sif_amd._native import DeviceArray
+        self.C, self.N = C, N
+        a = ADJOINT
+        x, y, z = zeldovich.generate(a["n_side"], a["box"], a["seed"])
+        self.n = len(x)
+        rng = np.random.default_rng(a["centre_seed"])
+        cen = rng.uniform(0, a["box"], (a["n_centres"], 3)).astype(np.float32)
+        rv = np.random.default_rng(a["radius_seed"]).uniform(
+            a["r_lo"], a["r_hi"], a["n_centres"]).astype(np.float32)
+        up = lambda v: DeviceArray.from_host(np.ascontiguousarray(v, np.float32))
+        self.x, self.y, self.z = up(x), up(y), up(z)
+        self.cx, self.cy, self.cz, self.rv = up(cen[:, 0]), up(cen[:, 1]), up(cen[:, 2]), up(rv)
+        nc3 = a["n_cells"] ** 3
+        spec = int(N.lib.sif_amd_spectrum_bytes(a["n_cells"]))
+        work = max(spec, int(N.lib.sif_amd_cic_work_bytes(a["n_cells"])))
+        self.grid = DeviceArray((nc3,), np.float32)
+        self.field = DeviceArray((nc3,), np.float32)
+        self.spec = DeviceArray((spec,), np.uint8)
+        self.work = DeviceArray((work,), np.uint8)
+        npr = a["n_centres"] * a["n_bins"]
+        self.p = DeviceArray((npr,), np.float32)
+        self.gp = DeviceArray((npr,), np.float32)
+        self.gx, self.gy, self.gz = (DeviceArray((self.n,), np.float32) for _ in range(3))
+        self.target = self.p  # the first forward's profiles, perturbed below
+        self.forward()
+        p0 = self.p.get()
+        noise = np.random.default_rng(a["target_seed"]).normal(0.0, 0.1, p0.shape)
+        self.target = DeviceArray.from_host((p0 + noise).astype(np.float32))
+
+    def _smooth(self, src, dst):
+        a, N = ADJOINT, self.N
+        N.check(N.lib.sif_amd_fft_r2c(src.ptr, a["n_cells"], self.spec.ptr, self.work.ptr))
+        N.check(N.lib.sif_amd_fft_filter_c2r(self.spec.ptr, a["n_cells"], a["box"],
+                                             N.FILTER_GAUSSIAN, a["r_smooth"], 0.0, 0, 0,
+                                             dst.ptr, self.work.ptr))
+
+    def forward(self):
+        a, N, C = ADJOINT, self.N, self.C
+        N.check(N.lib.sif_amd_grid_assign_cic(self.x.ptr, self.y.ptr, self.z.ptr, None, self.n,
+                                              a["n_cells"], a["box"], self.grid.ptr,
+                                              self.work.ptr))
+        mean = C.c_double()
+        N.check(N.lib.sif_amd_grid_to_density_contrast(self.grid.ptr, a["n_cells"] ** 3,
+                                                       C.byref(mean)))
+        self._smooth(self.grid, self.field)
+        N.check(N.lib.sif_amd_grid_profiles(self.field.ptr, a["n_cells"], a["box"], self.cx.ptr,
+                                            self.cy.ptr, self.cz.ptr, self.rv.ptr,
+                                            a["n_centres"], a["ext"], a["n_bins"], self.p.ptr))
+        self.mean = mean.value
+        return self.loss(None)
+
+    def loss(self, grad):
+        N, C = self.N, self.C
+        loss = C.c_double()
+        N.check(N.lib.sif_amd_squared_residual(self.p.ptr, self.target.ptr, self.p.shape[0],
+                                               1.0 / self.mean, grad.ptr if grad else None,
+                                               C.byref(loss)))
+        return loss.value
+
+    def forward_adjoint(self):
+        a, N = ADJOINT, self.N
+        self.forward()
+        loss = self.loss(self.gp)  # dL/dp over the contrast's mean
+        N.check(N.lib.sif_amd_grid_profiles_adjoint(self.gp.ptr, a["n_cells"], a["box"],
+                                                    self.cx.ptr, self.cy.ptr, self.cz.ptr,
+                                                    self.rv.ptr, a["n_centres"], a["ext"],
+                                                    a["n_bins"], self.field.ptr))
+        self._smooth(self.field, self.grid)
+        N.check(N.lib.sif_amd_cic_adjoint(self.x.ptr, self.y.ptr, self.z.ptr, self.n,
+                                          a["n_cells"], a["box"], self.grid.ptr, self.gx.ptr,
+                                          self.gy.ptr, self.gz.ptr))
+        return loss
+
+
+def adjoint_form(steps, warmup):
+    """Forward and forward+adjoint times of the profile chain, positions
+    resident in HBM, each step synchronised."""
+    from sif_amd import _native as N
+    t0 = time.time()
+    ch = AdjointChain()
+    log(f"[bench] adjoint chain set up in {time.time() - t0:.1f}s")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        N.check(N.lib.sif_amd_synchronize())
+        t = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        N.check(N.lib.sif_amd_synchronize())
+        return (time.perf_counter() - t) / steps, out
+
+    t_fwd, loss = timed(ch.forward)
+    t_both, _ = timed(ch.forward_adjoint)
+    g = np.concatenate([v.get()[:4] for v in (ch.gx, ch.gy, ch.gz)])
+    a = ADJOINT
+    log(f"[bench] adjoint chain: forward {t_fwd * 1e3:.2f} ms, forward+adjoint "
+        f"{t_both * 1e3:.2f} ms, loss {loss:.6g}")
+    return {"metric": "profile-chain particles/s, forward + adjoint",
+            "value": round(ch.n / t_both, 1), "unit": "particles/s",
+            "ms_forward": round(t_fwd * 1e3, 3), "ms_forward_adjoint": round(t_both * 1e3, 3),
+            "ratio_forward_adjoint_to_forward": round(t_both / t_fwd, 3),
+            "loss": loss, "grad_sample": [float(v) for v in g],
+            "config": {"workload": f"{a['n_side']}^3 Zel'dovich particles, L={a['box']:g}, "
+                                   f"grid {a['n_cells']}^3, Gaussian R={a['r_smooth']:g}, "
+                                   f"{a['n_centres']} centres radii {a['r_lo']:g}-{a['r_hi']:g}, "
+                                   f"{a['n_bins']} cumulative bins to {a['ext']:g}R"}}
+
+
+# many boxes of uneven size: 96 seeded Zel'dovich sets, N^3 particles with N
+# log-uniform in 64-256 (a multiple of 8), boxes at the fixed mean separation
+# 100/64 Mpc/h (100-400 Mpc/h a side), grids at the power of two nearest N
+# (the FFT here takes powers of two), mesh cell 10 Mpc/h, radii from 3 Mpc/h
+# to min(60, L/4) Mpc/h (the growth template must stay inside the mesh),
+# threshold -0.8; box b uses seed 3000 + b
+BATCHED = dict(n_boxes=96, side_lo=64, side_hi=256, spacing=100.0 / 64.0, mesh_cell=10.0,
+               r_lo=3.0, r_hi=60.0, seed0=3000, size_seed=3)
+
+
+def batched_boxes():
+    a = BATCHED
+    rng = np.random.default_rng(a["size_seed"])
+    sides = np.exp(rng.uniform(np.log(a["side_lo"]), np.log(a["side_hi"]), a["n_boxes"]))
+    out = []
+    for b, side in enumerate(sides):
+        n_side = int(min(max(round(side / 8.0) * 8, a["side_lo"]), a["side_hi"]))
+        box = n_side * a["spacing"]
+        n_cells = 1 << int(round(np.log2(n_side)))
+        mesh_cells = int(round(box / a["mesh_cell"]))
+        r_hi = min(a["r_hi"], box / 4.0)
+        out.append(dict(seed=a["seed0"] + b, n_side=n_side, box=box, n_cells=n_cells,
+                        mesh_cells=mesh_cells, r_hi=r_hi))
+    return out
+
+
+def batched_form():
+    """Sum of particles over the 96 boxes / the time of all their catalogues
+    (deposit, contrast, mesh, every rung), positions resident in HBM, one box
+    after another on the library stream."""
+    from sif_amd import device as D, zeldovich
+    from sif_amd._native import DeviceArray
+    t0 = time.time()
+    boxes = batched_boxes()
+    for bx in boxes:
+        x, y, z = zeldovich.generate(bx["n_side"], bx["box"], bx["seed"])
+        bx["n"] = len(x)
+        bx["d"] = (D.to_device(x), D.to_device(y), D.to_device(z))
+        bx["radii"] = zeldovich.ladder(BATCHED["r_lo"], bx["r_hi"], 0.05)
+    n_max = max(bx["n_cells"] for bx in boxes) ** 3
+    grid = DeviceArray((n_max,), np.float32)
+    work = DeviceArray((n_max,), np.int64)
+    total = sum(bx["n"] for bx in boxes)
+    log(f"[bench] batched: {len(boxes)} boxes, {total} particles "
+        f"({min(bx['n_side'] for bx in boxes)}^3..{max(bx['n_side'] for bx in boxes)}^3), "
+        f"set up in {time.time() - t0:.1f}s")
+
+    def one(bx):
+        dx, dy, dz = bx["d"]
+        nc = bx["n_cells"]
+        D.cic(dx, dy, dz, bx["n"], nc, bx["box"], out=grid, work=work)
+        D.density_contrast(grid, nc ** 3)
+        mesh = D.Mesh(dx, dy, dz, bx["n"], bx["mesh_cells"], bx["box"], indices=False)
+        cat, _, _ = D.exodus(grid, nc, bx["box"], mesh, bx["radii"], THRESHOLD, restore=False)
+        return len(cat)
+
+    one(boxes[0])  # warm-up: the library's buffers at the smallest sizes
+    from sif_amd import _native as N
+    N.check(N.lib.sif_amd_synchronize())
+    t = time.perf_counter()
+    per, voids = [], 0
+    for bx in boxes:
+        tb = time.perf_counter()
+        voids += one(bx)
+        per.append(time.perf_counter() - tb)
+    N.check(N.lib.sif_amd_synchronize())
+    dt = time.perf_counter() - t
+    log(f"[bench] batched: {dt:.2f}s for {len(boxes)} catalogues, {voids} voids")
+    per = np.array(per)
+    return {"metric": "exodus particles/s over 96 boxes of uneven size",
+            "value": round(total / dt, 1), "unit": "particles/s", "seconds": round(dt, 3),
+            "voids": int(voids), "particles": int(total),
+            "ms_per_box": {"min": round(per.min() * 1e3, 2), "median": round(float(np.median(per)) * 1e3, 2),
+                           "max": round(per.max() * 1e3, 2)},
+            "config": {"workload": "96 Zel'dovich boxes, N^3 log-uniform in 64^3-256^3, "
+                                   "L = N * 100/64 Mpc/h, grid the power of two nearest N, "
+                                   "mesh cell 10 Mpc/h, radii 3-min(60, L/4) Mpc/h, t=-0.8, "
+                                   "one box after another"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +366,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="plain512", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiable profile chain's leg")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 96-box leg")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="run only the adjoint and batched legs (for profiling them)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device, 4 times: the known traffic that "
                          "scales rocprof's FETCH_SIZE and WRITE_SIZE counters")
@@ -180,6 +393,15 @@ def main():
         for _ in range(4):
             N.check(N.lib.sif_amd_copy(dst.ptr, src.ptr, 1 << 30))
         log("[bench] copy probe: 4 launches of copy_kernel, 1073741824 B read and written each")
+        return
+
+    if args.forms_only:
+        forms = {}
+        if not args.no_adjoint:
+            forms["adjoint"] = adjoint_form(args.steps, args.warmup)
+        if not args.no_batched:
+            forms["batched"] = batched_form()
+        print(json.dumps({"forms": forms}), flush=True)
         return
 
     c, x, y, z, radii, mesh_cells = workload(args.config)
@@ -209,6 +431,12 @@ def main():
         ", ".join(f"{k} {v[0]:.1f}/{v[1]}" for k, v in timing.items()))
     roof = roofline(timing, args.config)
 
+    forms = {}
+    if not args.no_adjoint:
+        forms["adjoint"] = adjoint_form(args.steps, args.warmup)
+    if not args.no_batched:
+        forms["batched"] = batched_form()
+
     cpu = None
     if not args.no_cpu:
         t1 = time.time()
@@ -227,6 +455,7 @@ def main():
                    "voids": int(len(cat)), "parallelism": "single GPU"},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timing.items()},
         "roofline": roof,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -183,6 +183,12 @@ SIF_API int sif_amd_cic_adjoint(const float* d_x, const float* d_y,
   const float* d_z, uint64_t n, uint32_t n_cells, float box,
   const float* d_grad_rho, float* d_gx, float* d_gy, float* d_gz);
 
+/* The chain's loss and its gradient (no reference counterpart): *loss =
+ * sum over i < n of (d_p[i] - d_target[i])^2 (double, deterministic) and, when
+ * d_grad is not NULL, d_grad[i] = scale * 2 (d_p[i] - d_target[i]). */
+SIF_API int sif_amd_squared_residual(const float* d_p, const float* d_target,
+  uint64_t n, float scale, float* d_grad, double* loss);
+
 /* Per-rung smoothed fields as the finder scans them, for parity checks:
  * rung i (descending ladder) into d_fields + i * n_cells^3. */
 SIF_API int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells,

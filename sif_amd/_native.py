@@ -99,6 +99,7 @@ _sig("sif_amd_mark_spheres", i32, P, u32, f32, P, P, P, P, u32, f32)
 _sig("sif_amd_grid_profiles", i32, P, u32, f32, P, P, P, P, u32, f32, u32, P)
 _sig("sif_amd_grid_profiles_adjoint", i32, P, u32, f32, P, P, P, P, u32, f32, u32, P)
 _sig("sif_amd_cic_adjoint", i32, P, P, P, u64, u32, f32, P, P, P, P)
+_sig("sif_amd_squared_residual", i32, P, P, u64, f32, P, C.POINTER(f64))
 _sig("sif_amd_find_radius", i32, P, P, P, P, u32, f32, P, P, P, u32, f32, f32, f32, f32,
      P, P, P)
 _sig("sif_amd_exodus_device", C.POINTER(Catalogue), P, u32, f32, P, P, P, P, u32, u64,

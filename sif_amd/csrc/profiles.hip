@@ -24,6 +24,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "common.h"
 #include "internal.h"
@@ -37,6 +38,7 @@ namespace {
 constexpr uint32_t PROFILE_THREADS = 256;
 constexpr uint32_t PROFILE_TILE = 16;  // adjoint tile side, in nodes
 constexpr uint32_t PROFILE_CHUNK = 32;  // centres staged in LDS at a time
+constexpr uint32_t PROFILE_ROWS = 8;    // rows a wave of the gather has in flight
 
 // One centre's walk: the node cube around it, as the reference bounds a
 // void's cells, at most one full row per axis.
@@ -82,16 +84,21 @@ __device__ __forceinline__ float bin_volume(uint32_t j, float bw) {
   return (float)(4.0 / 3.0) * (float)SIF_PI * (r_outer * r_outer * r_outer);
 }
 
-// One workgroup per centre: count and delta sum per bin over the nodes of
-// its sphere, then the cumulative profile.
+// One workgroup per centre: the mass (1 + delta) per bin over the nodes of
+// its sphere, then the cumulative profile. Each lane keeps its own histogram
+// in LDS (bin-major, so the lanes of a wave hit distinct words) and updates it
+// with plain loads and stores: no lane writes another's words, so no atomics.
+// A lane's float sum covers the few thousand nodes it visits (its rounding
+// stays near 1e-7 of the sum); the lanes' sums are added in double, in lane
+// order.
 template <int NB>
 __global__ void __launch_bounds__(PROFILE_THREADS) profile_kernel(
   const float* __restrict__ delta, uint32_t N, float l, float box,
   const float* __restrict__ cxs, const float* __restrict__ cys, const float* __restrict__ czs,
   const float* __restrict__ rvs, float ext, uint32_t n_bins, float bw,
   float* __restrict__ out) {
-  __shared__ double s_sum[PROFILE_THREADS / 64][NB];
-  __shared__ uint32_t s_cnt[PROFILE_THREADS / 64][NB];
+  __shared__ float s_mass[NB][PROFILE_THREADS];
+  __shared__ double s_tot[NB];
   const uint32_t c = blockIdx.x;
   const float rv = rvs[c];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -100,75 +107,96 @@ __global__ void __launch_bounds__(PROFILE_THREADS) profile_kernel(
       out[(uint64_t)c * n_bins + j] = 0.0f;
     return;
   }
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+    s_mass[j][threadIdx.x] = 0.0f;
   const float cx = cxs[c], cy = cys[c], cz = czs[c];
   const float r_max = rv * ext;
   const float r_max_sq = r_max * r_max;
   const float inv_bin = 1.0f / (rv * bw);
   const float half = box * 0.5f;
   const Cube k = centre_cube(cx, cy, cz, r_max, l, N);
-  double sum[NB];
-  uint32_t cnt[NB];
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    sum[j] = 0.0;
-    cnt[j] = 0;
-  }
+  // the lane's z offsets in the row, wrapped once (span <= N: one step)
+  const uint32_t z_base = wrap_node(k.lo[2], (int32_t)N);
   const uint32_t rows = (uint32_t)k.span[0] * (uint32_t)k.span[1];
-  for (uint32_t row = wave; row < rows; row += PROFILE_THREADS / 64) {
-    const uint32_t gx = wrap_node(k.lo[0] + (int32_t)(row / (uint32_t)k.span[1]), (int32_t)N);
-    const uint32_t gy = wrap_node(k.lo[1] + (int32_t)(row % (uint32_t)k.span[1]), (int32_t)N);
-    const float dx = nearest((float)gx * l - cx, box, half);
-    const float dy = nearest((float)gy * l - cy, box, half);
-    const float dxy = dx * dx + dy * dy;
-    if (dxy > r_max_sq)
-      continue;
-    const float* rowp = delta + ((uint64_t)gx * N + gy) * N;
-    for (int32_t z = lane; z < k.span[2]; z += 64) {
-      const uint32_t gz = wrap_node(k.lo[2] + z, (int32_t)N);
-      const float dz = nearest((float)gz * l - cz, box, half);
-      const float r2 = dxy + dz * dz;
-      if (r2 <= r_max_sq) {
-        const uint32_t bin = (uint32_t)(__builtin_sqrtf(r2) * inv_bin);
-        const double v = (double)rowp[gz];
+  const uint32_t gx0 = wrap_node(k.lo[0], (int32_t)N), gy0 = wrap_node(k.lo[1], (int32_t)N);
+  float* my_mass = &s_mass[0][threadIdx.x];
+  // PROFILE_ROWS rows of the cube per wave at a time, their loads issued
+  // together; the waves take interleaved groups of rows
+  const uint32_t span_y = (uint32_t)k.span[1];
+  const uint32_t wave_u = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+  for (uint32_t row0 = wave_u * PROFILE_ROWS; row0 < rows;
+       row0 += (PROFILE_THREADS / 64) * PROFILE_ROWS) {
+    uint64_t rowo[PROFILE_ROWS];  // row offsets into delta (kept global loads)
+    float dxy[PROFILE_ROWS];
+    bool live = false;
+    // the group's first (x, y) node once, then step along y (uniform values:
+    // scalar arithmetic)
+    uint32_t ax = row0 / span_y, ay = row0 - ax * span_y;
 #pragma unroll
-        for (int j = 0; j < NB; j++) {
-          const bool hit = bin == (uint32_t)j;
-          sum[j] += hit ? v : 0.0;
-          cnt[j] += hit ? 1u : 0u;
+    for (uint32_t r = 0; r < PROFILE_ROWS; r++) {
+      const uint32_t row = row0 + r;
+      dxy[r] = r_max_sq + 1.0f;  // past the sphere: skipped
+      rowo[r] = 0;
+      if (row < rows) {
+        uint32_t gx = gx0 + ax, gy = gy0 + ay;
+        gx = gx >= N ? gx - N : gx;
+        gy = gy >= N ? gy - N : gy;
+        const float dx = nearest((float)gx * l - cx, box, half);
+        const float dy = nearest((float)gy * l - cy, box, half);
+        dxy[r] = dx * dx + dy * dy;
+        rowo[r] = ((uint64_t)gx * N + gy) * N;
+        live |= !(dxy[r] > r_max_sq);
+      }
+      if (++ay == span_y) {
+        ay = 0;
+        ax++;
+      }
+    }
+    if (!live)
+      continue;
+    for (int32_t z0 = 0; z0 < k.span[2]; z0 += 64) {
+      const int32_t z = z0 + lane;
+      uint32_t gz = z_base + (uint32_t)z;
+      gz = gz >= N ? gz - N : gz;
+      const bool in_row = z < k.span[2];
+      const float dz = nearest((float)gz * l - cz, box, half);
+      const float dz2 = dz * dz;
+      // every lane loads (an in-bounds node where it has none), so the loads
+      // issue together without a branch between them
+      const uint32_t gzl = in_row ? gz : 0u;
+      float v[PROFILE_ROWS];
+#pragma unroll
+      for (uint32_t r = 0; r < PROFILE_ROWS; r++)
+        v[r] = delta[rowo[r] + gzl];
+#pragma unroll
+      for (uint32_t r = 0; r < PROFILE_ROWS; r++) {
+        const float r2 = dxy[r] + dz2;
+        if (in_row && r2 <= r_max_sq) {
+          const uint32_t bin = (uint32_t)(__builtin_sqrtf(r2) * inv_bin);
+          if (bin < n_bins)
+            my_mass[bin * PROFILE_THREADS] += 1.0f + v[r];
         }
       }
     }
   }
-  // wave sums, then the workgroup's in wave order
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    double s = sum[j];
-    uint32_t n = cnt[j];
-    for (int off = 32; off > 0; off >>= 1) {
-      s += __shfl_xor(s, off, 64);
-      n += __shfl_xor(n, off, 64);
-    }
-    if (lane == 0) {
-      s_sum[wave][j] = s;
-      s_cnt[wave][j] = n;
-    }
+  __syncthreads();
+  // bin j: the 256 lanes' sums in double, in lane order
+  if (threadIdx.x < NB) {
+    const uint32_t j = threadIdx.x;
+    double m = 0.0;
+    for (uint32_t t = 0; t < PROFILE_THREADS; t++)
+      m += (double)s_mass[j][t];
+    s_tot[j] = m;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     const double l3 = (double)l * (double)l * (double)l;
     const double rv3 = (double)rv * (double)rv * (double)rv;
-    double cum_sum = 0.0;
-    uint64_t cum_cnt = 0;
+    double cum = 0.0;
     for (uint32_t j = 0; j < n_bins; j++) {
-      double s = 0.0;
-      uint64_t n = 0;
-      for (int w = 0; w < (int)(PROFILE_THREADS / 64); w++) {
-        s += s_sum[w][j];
-        n += s_cnt[w][j];
-      }
-      cum_sum += s;
-      cum_cnt += n;
-      const double mass = l3 * ((double)cum_cnt + cum_sum);
+      cum += s_tot[j];
+      const double mass = l3 * cum;
       out[(uint64_t)c * n_bins + j] =
         (float)(mass / ((double)bin_volume(j, bw) * rv3) - 1.0);
     }
@@ -380,7 +408,38 @@ __global__ void __launch_bounds__(256) cic_adjoint_kernel(const float* __restric
   gzs[p] = sz * scale;
 }
 
+// loss = sum (p - t)^2 and its gradient scale * 2 (p - t): per-block sums in
+// double in a fixed tree, added in block order on the host
+constexpr uint32_t RESID_THREADS = 256, RESID_PER_BLOCK = 8192;
+
+__global__ void __launch_bounds__(RESID_THREADS) residual_kernel(const float* __restrict__ p,
+  const float* __restrict__ t, uint64_t n, float scale, float* __restrict__ grad,
+  double* __restrict__ block_sums) {
+  __shared__ double s[RESID_THREADS];
+  const uint64_t b0 = (uint64_t)blockIdx.x * RESID_PER_BLOCK;
+  double acc = 0.0;
+  for (uint32_t k = threadIdx.x; k < RESID_PER_BLOCK; k += RESID_THREADS) {
+    const uint64_t i = b0 + k;
+    if (i < n) {
+      const float d = p[i] - t[i];
+      acc += (double)d * (double)d;
+      if (grad)
+        grad[i] = scale * (2.0f * d);
+    }
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t off = RESID_THREADS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    block_sums[blockIdx.x] = s[0];
+}
+
 DeviceBuffer g_pair_count, g_pair_offset, g_pair_keys, g_pair_vals, g_tile_start, g_tile_end;
+DeviceBuffer g_resid;
 
 bool bins_supported(uint32_t n_bins) { return n_bins >= 1 && n_bins <= 32; }
 
@@ -496,6 +555,36 @@ int sif_amd_grid_profiles_adjoint(const float* d_grad_profiles, uint32_t n_cells
       n_cells, l, box, d_cx, d_cy, d_cz, d_rv, ext, n_bins, bw, nt, g_tile_start.as<uint32_t>(),
       g_tile_end.as<uint32_t>(), g_pair_vals.as<uint32_t>(), d_grad_delta);
   SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+int sif_amd_squared_residual(const float* d_p, const float* d_target, uint64_t n, float scale,
+  float* d_grad, double* loss) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!d_p || !d_target || !loss) {
+    error_set(SIF_ERR_INVALID, "sif_amd_squared_residual: invalid arguments");
+    return SIF_ERR_INVALID;
+  }
+  *loss = 0.0;
+  if (n == 0)
+    return SIF_OK;
+  const uint64_t blocks = (n + RESID_PER_BLOCK - 1) / RESID_PER_BLOCK;
+  st = g_resid.reserve(blocks * sizeof(double));
+  if (st != SIF_OK)
+    return st;
+  residual_kernel<<<(unsigned)blocks, RESID_THREADS, 0, stream()>>>(d_p, d_target, n, scale,
+    d_grad, g_resid.as<double>());
+  SIF_HIP(hipGetLastError());
+  std::vector<double> sums(blocks);
+  SIF_HIP(hipMemcpyAsync(sums.data(), g_resid.ptr, blocks * sizeof(double),
+    hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  double total = 0.0;
+  for (double v : sums)
+    total += v;
+  *loss = total;
   return SIF_OK;
 }
 
