@@ -64,6 +64,7 @@ static inline float sif__finder_search_factor(uint32_t opt) {
 /* Candidate seeds: every cell under the threshold, or the local minima. */
 #define SIF_FINDER_SEEDS_CELLS  (0u << 9)
 #define SIF_FINDER_SEEDS_MINIMA (1u << 9)
+#define SIF__FINDER_SEEDS_MASK  (1u << 9)
 /* Write no checkpoint. */
 #define SIF_FINDER_NO_CHECKPOINT (1u << 11)
 /* Leave the assignment window in the spectrum. */

@@ -56,6 +56,48 @@ SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid,
   sif_real threshold, sif_real overlap_fraction, sif_option opt,
   const sif_finder_exodus_extras_t* extras);
 
+/* exodus_finder.h:409-531: the checkpoint a run writes after every rung. A
+ * 192-byte header, then in host byte order the ladder largest first (n_radii
+ * floats) and the voids as four columns of n_voids floats (every x, then every
+ * y, z and radius). Float run parameters are stored widened to double. */
+#define SIF_FINDER_CHECKPOINT_MAGIC "XCKP"
+#define SIF_FINDER_CHECKPOINT_VERSION 3
+
+typedef struct {
+  char magic[4];
+  uint32_t version;
+  uint32_t byte_order;   /* 0x01020304 as written */
+  uint32_t is_double;    /* 1 for a 64-bit sif_real */
+  uint32_t is_survey;
+  uint32_t is_weighted;
+  uint32_t options;      /* the options that change the catalogue */
+  uint32_t n_radii;
+  uint32_t next_rung;    /* rungs done */
+  uint32_t grid_n_cells;
+  double box_length;
+  double threshold;
+  double overlap_fraction;
+  uint64_t n_particles;
+  double total_weight;
+  uint64_t random_n_particles;
+  double random_total_weight;
+  double offset[3];
+  uint64_t n_prior;      /* the first n_prior voids came from extras->prior */
+  uint32_t prior_crc32;  /* CRC32 of the prior's four columns, 0 for none */
+  uint32_t crc32;        /* CRC32 of the header (this field 0) and the rest */
+  uint64_t n_voids;
+  uint64_t n_mismatched; /* added voids more than twice their rung */
+  uint32_t assignment;   /* sif_assignment_t, | 1 << 8 if interlaced */
+  char reserved[36];
+} sif_finder_exodus_checkpoint_t;
+
+/* exodus_finder.h:533-549: the header and the voids (caller's frame) of a
+ * checkpoint; NULL, with the error record set, if the file cannot be read, is
+ * not a checkpoint, is of another byte order or precision, or fails its
+ * checksum. Older versions are brought to this one's option meanings. */
+SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus_checkpoint_read(
+  const char* filepath, sif_finder_exodus_checkpoint_t* header);
+
 #define SIF_FINDER_MESH_PARTICLES_PER_CELL 30.0
 #define SIF_FINDER_MESH_MAX_CELLS          1024u
 

@@ -11,6 +11,9 @@
 #include <cstring>
 #include <vector>
 
+#include <unistd.h>
+
+#include "checkpoint.h"
 #include "common.h"
 #include "internal.h"
 #include "sif/core/system.h"
@@ -808,6 +811,16 @@ sif_catalogue_t* sif_finder_exodus(sif_grid_t* grid, const sif_chain_mesh_t* mes
   return sif_finder_exodus_raw(grid, mesh, radii, n_radii, threshold, f, opt, extras);
 }
 
+// A run's checkpoint and where it stands (exodus_finder.c:1384-1393).
+struct Checkpointing {
+  bool active = false;    // written after every rung
+  bool on_disk = false;   // this run wrote one, or resumed one
+  bool progress = false;  // the one on disk holds a finished rung
+  bool keep = false;      // left in place on success, for the caller
+  char path[4096] = {0};
+  sif_finder_exodus_checkpoint_t header;
+};
+
 sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t* mesh,
   const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real overlap_fraction,
   sif_option options, const sif_finder_exodus_extras_t* extras) {
@@ -834,13 +847,97 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
   }
   if ((st = check_options(options)) != SIF_OK)
     return nullptr;
-  if (extras && extras->checkpoint) {
-    error_set(SIF_ERR_UNSUPPORTED, "checkpoint: %s: checkpoints are not in this build",
-      extras->checkpoint);
+  const char* named = extras ? extras->checkpoint : nullptr;
+  if (named && (options & SIF_FINDER_NO_CHECKPOINT)) {
+    error_set(SIF_ERR_INVALID, "checkpoint: %s, with SIF_FINDER_NO_CHECKPOINT set", named);
+    return nullptr;
+  }
+  if (named && strlen(named) + 32 >= 4096) {
+    error_set(SIF_ERR_INVALID, "checkpoint: path of %zu characters (expected under %d)",
+      strlen(named), 4096 - 32);
     return nullptr;
   }
   if ((st = need_init()) != SIF_OK)
     return nullptr;
+
+  // exodus_finder.c:1435-1575: where the run starts. A checkpoint the caller
+  // named and that exists is resumed once its header describes this run;
+  // otherwise the run starts from the prior voids and writes its first
+  // checkpoint now (only a named one is worth failing over).
+  std::vector<float> sorted;
+  sorted_ladder(radii, n_radii, sorted);
+  const sif_catalogue_t* prior = extras ? extras->prior : nullptr;
+  Checkpointing ck;
+  ck.keep = extras && extras->keep_checkpoint;
+  sif_catalogue_t* resumed = nullptr;
+  uint32_t first_rung = 0;
+  uint64_t mismatched0 = 0;
+  if (!(options & SIF_FINDER_NO_CHECKPOINT)) {
+    sif_finder_exodus_checkpoint_t& h = ck.header;
+    memset(&h, 0, sizeof(h));
+    h.is_weighted = mesh->weights != nullptr;
+    h.options = options & (SIF__FINDER_SEARCH_MASK | SIF_FINDER_KEEP_CIC_WINDOW |
+                            SIF__FINDER_SEEDS_MASK | SIF__FINDER_REFINE_MASK |
+                            SIF_FINDER_EXTEND_REACH | SIF_FINDER_DISPLACE |
+                            SIF_FINDER_ALIASING_WINDOW | SIF_FINDER_LAGRANGIAN_OVERLAP);
+    h.n_radii = n_radii;
+    h.grid_n_cells = grid->n_cells;
+    h.assignment = ((uint32_t)grid->assignment & 0xffu) | ((grid->interlaced ? 1u : 0u) << 8);
+    h.box_length = (double)grid->box_length;
+    h.threshold = (double)threshold;
+    h.overlap_fraction = (double)overlap_fraction;
+    h.n_particles = mesh->n_particles;
+    h.total_weight = mesh->total_weight;
+    if (prior && prior->n_voids > 0) {
+      h.n_prior = prior->n_voids;
+      h.prior_crc32 = catalogue_crc(prior->cx, prior->cy, prior->cz, prior->radii,
+        prior->n_voids);
+    }
+    if (named)
+      snprintf(ck.path, sizeof(ck.path), "%s", named);
+    else
+      checkpoint_name(ck.path, sizeof(ck.path));
+    if (named && access(named, F_OK) == 0) {
+      sif_finder_exodus_checkpoint_t got;
+      std::vector<float> got_radii, cols[4];
+      st = checkpoint_load(named, &got, &got_radii, cols);
+      if (st == SIF_OK)
+        st = checkpoint_match(named, h, sorted.data(), got, got_radii.data());
+      if (st == SIF_OK) {
+        resumed = sif_catalogue_alloc(got.n_voids ? got.n_voids : 1);
+        if (!resumed) {
+          error_set(SIF_ERR_ALLOC, "checkpoint: %s: cannot allocate its voids", named);
+          st = SIF_ERR_ALLOC;
+        }
+        for (uint64_t v = 0; st == SIF_OK && v < got.n_voids; v++)
+          st = sif_catalogue_append(resumed, cols[0][v], cols[1][v], cols[2][v], cols[3][v]);
+      }
+      if (st != SIF_OK) {
+        sif_catalogue_free(resumed);
+        return nullptr;
+      }
+      first_rung = got.next_rung;
+      mismatched0 = got.n_mismatched;
+      h.next_rung = got.next_rung;
+      h.n_mismatched = got.n_mismatched;
+      ck.active = ck.on_disk = ck.progress = true;
+    } else {
+      std::vector<float> px, py, pz, pr;
+      const uint64_t np = prior ? prior->n_voids : 0;
+      Columns v{{prior ? prior->cx : nullptr, prior ? prior->cy : nullptr,
+                  prior ? prior->cz : nullptr, prior ? prior->radii : nullptr}, np};
+      st = checkpoint_write(ck.path, &h, sorted.data(), v);
+      if (st == SIF_OK) {
+        ck.active = ck.on_disk = true;
+      } else if (named) {
+        return nullptr;
+      } else {
+        sif_error_clear();  // running without a checkpoint
+        st = SIF_OK;
+      }
+    }
+  }
+
   const uint64_t cells = grid->total_cells;
   const uint64_t mcells = mesh->total_cells;
   const uint64_t n = mesh->n_particles;
@@ -850,8 +947,12 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
   if (st == SIF_OK) st = dmx.upload(mesh->x, n * 4);
   if (st == SIF_OK) st = dmy.upload(mesh->y, n * 4);
   if (st == SIF_OK) st = dmz.upload(mesh->z, n * 4);
-  if (st != SIF_OK)
+  if (st != SIF_OK) {
+    if (ck.on_disk && !ck.progress)
+      unlink(ck.path);
+    sif_catalogue_free(resumed);
     return nullptr;
+  }
   RunSpec R;
   R.d_delta = dv.as<float>();
   R.n = grid->n_cells;
@@ -867,9 +968,29 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
   R.f = overlap_fraction;
   R.opt = options;
   R.restore = !(options & SIF_FINDER_CONSUME_GRID);
-  R.prior = extras ? extras->prior : nullptr;
+  R.prior = resumed ? resumed : prior;
+  R.first_rung = first_rung;
+  R.mismatched0 = mismatched0;
+  if (ck.active)
+    R.on_rung = [&](uint32_t done, const float* x, const float* y, const float* z,
+                  const float* r, uint64_t nv, uint64_t mism) {
+      // a failed write leaves the previous checkpoint standing; the next rung
+      // tries again
+      ck.header.next_rung = done;
+      ck.header.n_mismatched = mism;
+      if (checkpoint_write(ck.path, &ck.header, R.radii.data(), Columns{{x, y, z, r}, nv}) ==
+          SIF_OK)
+        ck.progress = true;
+      else
+        sif_error_clear();
+    };
   sif_catalogue_t* cat = nullptr;
   st = exodus_run(R, &cat, nullptr, nullptr);
+  sif_catalogue_free(resumed);
+  // exodus_finder.c:1560-1575: a finished run has no use for its checkpoint
+  // (unless the caller keeps it); a failed one keeps it once it holds a rung
+  if (ck.on_disk && !(st != SIF_OK && ck.progress) && !(st == SIF_OK && ck.keep))
+    unlink(ck.path);
   // the grid comes back restored (or holding the last smoothing under
   // SIF_FINDER_CONSUME_GRID), on failure too
   if (hipMemcpyAsync(grid->values, dv.p, cells * 4, hipMemcpyDeviceToHost, stream()) !=

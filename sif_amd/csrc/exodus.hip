@@ -577,17 +577,18 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   }
 
 
-  for (size_t i = 0; i < R.radii.size() && st == SIF_OK; i++) {
+  uint64_t mismatched = R.mismatched0;
+  for (size_t i = R.first_rung; i < R.radii.size() && st == SIF_OK; i++) {
     const float radius = R.radii[i];
     sif_amd_rung_stats_t S;
     memset(&S, 0, sizeof(S));
     S.radius = radius;
 
     // Rung i's field: smoothed on the auxiliary stream while rung i - 1 was
-    // committed (rung 0 on the library stream).
+    // committed (the first rung run on the library stream).
     float* const field = fields[i & 1];
-    if (i == 0)
-      st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>(),
+    if (i == R.first_rung)
+      st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>() + i * lut_size,
         windowed ? g_dev.axis.as<double>() : nullptr, field, g_dev.work.as<float2>(), stream());
     else
       SIF_HIP(hipStreamWaitEvent(stream(), ev_smoothed[i & 1], 0));
@@ -986,6 +987,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           continue;
         }
         V.add(N.x, N.y, N.z, r, B.h_nin[b]);
+        if (r / radius > 2.0f)  // exodus_finder.c:2321-2328
+          mismatched++;
         is_taken[b] = 1;
         taken.push_back(b);
         accepted++;
@@ -1046,6 +1049,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
     if (stats_out)
       stats_out[i] = S;
+    if (st == SIF_OK && R.on_rung)
+      R.on_rung((uint32_t)(i + 1), V.cx.data(), V.cy.data(), V.cz.data(), V.r.data(),
+        (uint64_t)V.cx.size(), mismatched);
   }
 
   // the auxiliary stream may still smooth a rung the loop left (or read work)

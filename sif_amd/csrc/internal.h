@@ -2,6 +2,7 @@
 // Every one enqueues on the library stream; the caller synchronises.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -110,6 +111,13 @@ struct RunSpec {
   sif_option opt;
   bool restore;
   const sif_catalogue_t* prior;
+  // a resumed run: the first rung to run, and the mismatch count so far
+  uint32_t first_rung = 0;
+  uint64_t mismatched0 = 0;
+  // after each rung: rungs done, the catalogue's columns, the mismatch count
+  // (added voids more than twice the rung that found them)
+  std::function<void(uint32_t, const float*, const float*, const float*, const float*,
+    uint64_t, uint64_t)> on_rung;
 };
 int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   sif_amd_rung_stats_t* stats_out);
