@@ -114,6 +114,18 @@ SIF_API int sif_amd_find_radius(const uint64_t* d_offsets, const float* d_mx,
   float rmin, float r_search, float threshold, float vol_factor,
   float* d_radius, uint32_t* d_reason, uint64_t* d_n_in);
 
+/* sif__exodus_find_radius_weighted (src/finder/exodus_rescale.c:1029-1037):
+ * the same on weighted tracers -- d_mw and d_cell_weights (device) as
+ * sif_amd_chain_mesh_build gives them, the density tests against
+ * mass_factor, (4/3) pi total_weight / box^3 (exodus_finder.c:1639-1642);
+ * vol_factor still sizes the histogram. Weights are not checked here (see
+ * sif_amd_exodus_device_weighted). n_in receives the tracers enclosed. */
+SIF_API int sif_amd_find_radius_weighted(const uint64_t* d_offsets, const float* d_mx,
+  const float* d_my, const float* d_mz, const float* d_mw, const float* d_cell_weights,
+  uint32_t mesh_cells, float box, const float* d_cx, const float* d_cy, const float* d_cz,
+  uint32_t n, float rmin, float r_search, float threshold, float vol_factor,
+  float mass_factor, float* d_radius, uint32_t* d_reason, uint64_t* d_n_in);
+
 /* sif__exodus_template_build (exodus_rescale.c:175-283), host only: the
  * per-rung stencil the growth kernel walks, as (dx, dy, dz, class) and
  * (min d^2, max d^2) per entry; *count is the total, entries past capacity
@@ -132,13 +144,6 @@ typedef struct {
   uint64_t evaluated; /* growths the GPU ran for this rung */
 } sif_amd_rung_stats_t;
 
-/* sif__exodus_run (src/finder/exodus_finder.c:1585-2492) for a periodic box,
- * SIF_FINDER_SEEDS_CELLS on their nodes, unweighted: the grid d_delta (a
- * density contrast, n_cells^3, left as the last rung's smoothing unless
- * restore) and the mesh arrays are device memory. n_in (optional, host,
- * capacity of the result) receives each void's enclosed count. stats
- * (optional, host, n_radii entries) receives the per-rung counters in ladder
- * order (largest first). */
 /* sif__mark_sphere (src/finder/utils.c:965-1033): paints n_voids spheres
  * (radius scale * d_r[i]) into the n_cells^3-bit mask d_mask_words (device,
  * (n_cells^3 + 63) / 64 words), OR-ing into what it holds. */
@@ -146,9 +151,31 @@ SIF_API int sif_amd_mark_spheres(uint64_t* d_mask_words, uint32_t n_cells, float
   const float* d_cx, const float* d_cy, const float* d_cz, const float* d_r,
   uint32_t n_voids, float scale);
 
+/* sif__exodus_run (src/finder/exodus_finder.c:1585-2492) for a periodic box,
+ * SIF_FINDER_SEEDS_CELLS on their nodes, unweighted: the grid d_delta (a
+ * density contrast, n_cells^3, left as the last rung's smoothing unless
+ * restore) and the mesh arrays are device memory. n_in (optional, host,
+ * capacity of the result) receives each void's enclosed count. stats
+ * (optional, host, n_radii entries) receives the per-rung counters in ladder
+ * order (largest first). */
 SIF_API sif_catalogue_t* sif_amd_exodus_device(float* d_delta,
   uint32_t n_cells, float box, const uint64_t* d_offsets, const float* d_mx,
   const float* d_my, const float* d_mz, uint32_t mesh_cells,
+  uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_real overlap_fraction, sif_option options,
+  int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats);
+
+/* The same on weighted tracers (sif__exodus_find_radius_weighted,
+ * exodus_rescale.c:1029-1037; the density tests against the mean weight
+ * density, exodus_finder.c:1634-1642): d_mw and d_cell_weights as
+ * sif_amd_chain_mesh_build gives them (device), total_weight the mesh's
+ * summed weight. The weights are checked as exodus_finder.c:1051-1106 checks
+ * them (finite and non-negative by their bits, a positive total):
+ * SIF_ERR_INVALID otherwise. n_in receives the tracers each void encloses. */
+SIF_API sif_catalogue_t* sif_amd_exodus_device_weighted(float* d_delta,
+  uint32_t n_cells, float box, const uint64_t* d_offsets, const float* d_mx,
+  const float* d_my, const float* d_mz, const float* d_mw,
+  const float* d_cell_weights, double total_weight, uint32_t mesh_cells,
   uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
   sif_real threshold, sif_real overlap_fraction, sif_option options,
   int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats);

@@ -42,7 +42,8 @@ class Template(C.Structure):
 
 class Mesh(C.Structure):
     _fields_ = [("n_cells", u32), ("box_length", f32), ("cell_length", f32),
-                ("offsets", P), ("x", P), ("y", P), ("z", P)]
+                ("offsets", P), ("x", P), ("y", P), ("z", P), ("w", P), ("cell_w", P),
+                ("total_weight", f64)]
 
 
 class Catalogue(C.Structure):
@@ -87,6 +88,8 @@ _sig("orc_template_build", i32, C.POINTER(Template), u32, f32, f32, f32)
 _sig("orc_template_free", None, C.POINTER(Template))
 _sig("orc_find_radius", f32, C.POINTER(Mesh), f32, f32, f32, f32, f32, f32, f32,
      C.POINTER(Template), u32, C.POINTER(C.c_uint8), C.POINTER(u64))
+_sig("orc_find_radius_weighted", f32, C.POINTER(Mesh), f32, f32, f32, f32, f32, f32, f32,
+     f32, C.POINTER(Template), u32, C.POINTER(C.c_uint8), C.POINTER(u64))
 _sig("orc_witness", i32, C.POINTER(Mesh), f32, f32, f32, f32, f32, f32, f32,
      C.POINTER(Template), i32)
 _sig("orc_brute_force_radius", f64, P, P, P, u64, f32, f64, f64, f64, f64, f64, f64)
@@ -266,9 +269,13 @@ def sort_candidates_reference_build(delta, flat):
 class OracleMesh:
     def __init__(self, m, n_cells, box):
         self.keep = m
+        w = m.get("w")
+        cw = m.get("cell_weights")
         self.s = Mesh(n_cells, box, np.float32(box) / np.float32(n_cells),
                       m["offsets"].ctypes.data, m["x"].ctypes.data, m["y"].ctypes.data,
-                      m["z"].ctypes.data)
+                      m["z"].ctypes.data, None if w is None else w.ctypes.data,
+                      None if cw is None else cw.ctypes.data,
+                      float(m.get("total_weight", len(m["x"]))))
 
 
 class OracleTemplate:
@@ -289,17 +296,29 @@ class OracleTemplate:
         lib.orc_template_free(C.byref(self.t))
 
 
-def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, window=8):
+def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, window=8,
+                mass_factor=None):
+    """Rescaled radii of `centres`; weighted (against mass_factor) when the
+    mesh carries weights."""
     tpl = OracleTemplate(mesh.s.n_cells, mesh.s.cell_length, rmin, r_search)
     out = np.empty(len(centres), np.float32)
     reason = np.empty(len(centres), np.uint8)
     nin = np.zeros(len(centres), np.uint64)
+    weighted = bool(mesh.s.w)
+    if weighted and mass_factor is None:
+        raise ValueError("a weighted mesh needs its mass_factor")
     for i, (x, y, z) in enumerate(np.asarray(centres, np.float32)):
         rs = C.c_uint8()
         ni = C.c_uint64()
-        out[i] = lib.orc_find_radius(C.byref(mesh.s), x, y, z, r_search, threshold,
-                                     vol_factor, rmin, C.byref(tpl.t), window, C.byref(rs),
-                                     C.byref(ni))
+        if weighted:
+            out[i] = lib.orc_find_radius_weighted(C.byref(mesh.s), x, y, z, r_search,
+                                                  threshold, vol_factor, mass_factor, rmin,
+                                                  C.byref(tpl.t), window, C.byref(rs),
+                                                  C.byref(ni))
+        else:
+            out[i] = lib.orc_find_radius(C.byref(mesh.s), x, y, z, r_search, threshold,
+                                         vol_factor, rmin, C.byref(tpl.t), window,
+                                         C.byref(rs), C.byref(ni))
         reason[i] = rs.value
         nin[i] = ni.value
     return out, reason, nin

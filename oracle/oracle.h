@@ -114,6 +114,11 @@ typedef struct {
   const float* x;
   const float* y;
   const float* z;
+  /* a weighted mesh (chain_mesh.h:80-95): tracer weights, per-cell sums (a
+   * float each, summed in double) and the total; NULL on an unweighted one */
+  const float* w;
+  const float* cell_w;
+  double total_weight;
 } orc_mesh_t;
 
 #define ORC_RESCALE_OK         0
@@ -126,6 +131,12 @@ typedef struct {
 float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
   float r_search, float threshold, float vol_factor, float rmin,
   const orc_template_t* tpl, uint32_t window, uint8_t* reason, uint64_t* n_in);
+/* The weighted one (exodus_rescale.c:1029-1037), on m->w and m->cell_w, the
+ * density tests against mass_factor. n_in: the tracers enclosed. */
+float orc_find_radius_weighted(const orc_mesh_t* m, float cx, float cy,
+  float cz, float r_search, float threshold, float vol_factor,
+  float mass_factor, float rmin, const orc_template_t* tpl, uint32_t window,
+  uint8_t* reason, uint64_t* n_in);
 int orc_witness(const orc_mesh_t* m, float cx, float cy, float cz, float rho2,
   float r_search, float threshold, float vol_factor, const orc_template_t* tpl,
   int closed);

@@ -207,6 +207,16 @@ static int query_cell(const query_t* q, uint32_t i, uint64_t* s, uint64_t* e,
   return *e > *s;
 }
 
+/* The mesh cell of template entry i, as query_cell() finds it. */
+static uint64_t orc_query_flat(const query_t* q, uint32_t i, int32_t N) {
+  int32_t ix = q->ix + q->t->dx[i], iy = q->iy + q->t->dy[i],
+          iz = q->iz + q->t->dz[i];
+  ix = ix < 0 ? ix + N : ix >= N ? ix - N : ix;
+  iy = iy < 0 ? iy + N : iy >= N ? iy - N : iy;
+  iz = iz < 0 ? iz + N : iz >= N ? iz - N : iz;
+  return (uint64_t)ix * N * N + (uint64_t)iy * N + iz;
+}
+
 /* exodus_rescale.c:349-355 */
 static inline float dist2(float px, float py, float pz, float cx, float cy,
   float cz) {
@@ -229,13 +239,32 @@ static int cmp_float(const void* pa, const void* pb) {
   return (a > b) - (a < b);
 }
 
-/* exodus_rescale.c:391-507, unweighted. Returns 1 found, 0 exhausted. */
+typedef struct {
+  float d2, w;
+} pair_t;
+
+/* exodus_rescale.c:29 sorts the pairs on d2 alone, so the order of equal
+ * distances is its quicksort's. Equal distances give one radius whichever
+ * passes first (the walk's enclosed weight only falls through them); here
+ * they are ordered by weight too, the order the device sort gives, which
+ * fixes the reported enclosed count. */
+static int cmp_pair(const void* pa, const void* pb) {
+  const pair_t* a = pa;
+  const pair_t* b = pb;
+  if (a->d2 != b->d2)
+    return (a->d2 > b->d2) - (a->d2 < b->d2);
+  const float wa = a->w + 0.0f, wb = b->w + 0.0f; /* -0 as +0 */
+  return (wa > wb) - (wa < wb);
+}
+
+/* exodus_rescale.c:391-507. Returns 1 found, 0 exhausted. cur_w is read
+ * when weighted, cur_n always (the enclosed count reported). */
 static int resolve(const query_t* q, uint32_t blo, uint32_t bhi, float lo,
   float hi, float rc2, float rs2, float inv_w, uint32_t nb, uint64_t cur_n,
-  float K2, float* out, uint64_t* n_in_out) {
+  double cur_w, float K2, float* out, uint64_t* n_in_out, int weighted) {
   const orc_template_t* t = q->t;
   uint32_t cap = 1024, cnt = 0;
-  float* buf = malloc(sizeof(float) * cap);
+  pair_t* buf = malloc(sizeof(pair_t) * cap);
   for (uint32_t i = 0; i < t->count; i++) {
     const uint8_t ty = t->type[i];
     if (ty == ORC_CELL_CORE || t->max_d2[i] < lo || t->min_d2[i] > hi)
@@ -253,23 +282,38 @@ static int resolve(const query_t* q, uint32_t blo, uint32_t bhi, float lo,
         continue;
       if (cnt == cap) {
         cap *= 2;
-        buf = realloc(buf, sizeof(float) * cap);
+        buf = realloc(buf, sizeof(pair_t) * cap);
       }
-      buf[cnt++] = d2;
+      buf[cnt].d2 = d2;
+      buf[cnt].w = weighted ? q->m->w[p] : 1.0f;
+      cnt++;
     }
   }
-  if (cnt > 1)
-    qsort(buf, cnt, sizeof(float), cmp_float);
+  if (cnt > 1) {
+    if (weighted) {
+      qsort(buf, cnt, sizeof(pair_t), cmp_pair);
+    } else {
+      float* d = malloc(sizeof(float) * cnt);
+      for (uint32_t k = 0; k < cnt; k++)
+        d[k] = buf[k].d2;
+      qsort(d, cnt, sizeof(float), cmp_float);
+      for (uint32_t k = 0; k < cnt; k++)
+        buf[k].d2 = d[k];
+      free(d);
+    }
+  }
   int found = 0;
   for (int64_t k = (int64_t)cnt - 1; k >= 0; k--) {
-    const float d2 = buf[k];
+    const float d2 = buf[k].d2;
+    const double w = (double)buf[k].w;
     if (d2 == 0.0f) {
       cur_n--;
+      cur_w -= w;
       continue;
     }
     const uint64_t n_in = cur_n - 1;
     const float d2c = d2 * d2 * d2;
-    const float rn = (float)n_in;
+    const float rn = weighted ? (float)(cur_w - w) : (float)n_in;
     if (rn * rn <= K2 * d2c) {
       *out = sqrtf(d2);
       if (n_in_out)
@@ -278,19 +322,23 @@ static int resolve(const query_t* q, uint32_t blo, uint32_t bhi, float lo,
       break;
     }
     cur_n--;
+    cur_w -= w;
   }
   free(buf);
   return found;
 }
 
-float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
-  float r_search, float threshold, float vol_factor, float rmin,
-  const orc_template_t* tpl, uint32_t window, uint8_t* reason,
-  uint64_t* n_in_out) {
+/* exodus_rescale.c:642-1012, find_exact_radius_impl, both forms. */
+static float find_radius_impl(const orc_mesh_t* m, float cx, float cy,
+  float cz, float r_search, float threshold, float vol_factor,
+  float mass_factor, float rmin, const orc_template_t* tpl, uint32_t window,
+  uint8_t* reason, uint64_t* n_in_out, int weighted) {
   *reason = ORC_RESCALE_OK;
   const float inv_l = 1.0f / m->cell_length;
   const query_t q = {m, tpl, cx, cy, cz, (int32_t)(cx * inv_l),
     (int32_t)(cy * inv_l), (int32_t)(cz * inv_l)};
+  const float* mw = m->w;
+  const float* cw = m->cell_w;
   const float rs2 = r_search * r_search;
   const float rc2 = rmin * rmin;
   const float span = rs2 - rc2;
@@ -308,10 +356,11 @@ float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
     *reason = ORC_RESCALE_DEGENERATE;
     return -1.0f;
   }
-  /* early refusal, :711-782 */
+  /* early refusal, :711-782; weighted, whole cells from the cell table */
   {
-    const float expected = vol_factor * r_search * r_search * r_search;
+    const float expected = mass_factor * r_search * r_search * r_search;
     uint64_t within = 0;
+    double w_within = 0.0;
     for (uint32_t i = 0; i < tpl->count; i++) {
       const uint8_t ty = tpl->type[i];
       uint64_t s, e;
@@ -319,17 +368,31 @@ float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
       if (!query_cell(&q, i, &s, &e, &ex, &ey, &ez))
         continue;
       if (ty == ORC_CELL_CORE || ty == ORC_CELL_SHELL) {
-        within += e - s;
+        if (weighted) {
+          const int32_t N = (int32_t)m->n_cells;
+          const uint64_t flat = orc_query_flat(&q, i, N);
+          w_within += (double)cw[flat];
+        } else {
+          within += e - s;
+        }
         continue;
       }
-      for (uint64_t p = s; p < e; p++)
-        within += dist2(m->x[p], m->y[p], m->z[p], ex, ey, ez) <= rs2;
+      for (uint64_t p = s; p < e; p++) {
+        const int in = dist2(m->x[p], m->y[p], m->z[p], ex, ey, ez) <= rs2;
+        if (weighted) {
+          if (in)
+            w_within += (double)mw[p];
+        } else {
+          within += in;
+        }
+      }
     }
-    if (within == 0) {
+    if (weighted ? !(w_within > 0.0) : within == 0) {
       *reason = ORC_RESCALE_EMPTY;
       return -1.0f;
     }
-    if (((float)within / expected) - 1.0f <= threshold) {
+    const float inside = weighted ? (float)w_within : (float)within;
+    if ((inside / expected) - 1.0f <= threshold) {
       *reason = ORC_RESCALE_BEYOND;
       return -1.0f;
     }
@@ -337,7 +400,9 @@ float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
   const float inv_w = (float)nb / span;
   const float bin_w = span / (float)nb;
   uint32_t* bins = calloc(nb, sizeof(uint32_t));
+  double* wbins = calloc(nb, sizeof(double));
   uint64_t n_core = 0, n_shell = 0;
+  double w_core = 0.0, w_shell = 0.0;
   for (uint32_t i = 0; i < tpl->count; i++) {
     const uint8_t ty = tpl->type[i];
     uint64_t s, e;
@@ -346,70 +411,115 @@ float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
       continue;
     if (ty == ORC_CELL_CORE) {
       n_core += e - s;
+      if (weighted)
+        w_core += (double)cw[orc_query_flat(&q, i, (int32_t)m->n_cells)];
     } else if (ty == ORC_CELL_SHELL) {
-      for (uint64_t p = s; p < e; p++)
-        bins[bin_of(dist2(m->x[p], m->y[p], m->z[p], ex, ey, ez), rc2, inv_w,
-          nb)]++;
+      for (uint64_t p = s; p < e; p++) {
+        const uint32_t b =
+          bin_of(dist2(m->x[p], m->y[p], m->z[p], ex, ey, ez), rc2, inv_w, nb);
+        bins[b]++;
+        if (weighted) {
+          wbins[b] += (double)mw[p];
+          w_shell += (double)mw[p];
+        }
+      }
       n_shell += e - s;
     } else {
       for (uint64_t p = s; p < e; p++) {
         const float d2 = dist2(m->x[p], m->y[p], m->z[p], ex, ey, ez);
-        n_core += d2 <= rc2;
-        if (d2 > rc2 && d2 <= rs2) {
-          bins[bin_of(d2, rc2, inv_w, nb)]++;
+        const double w = weighted ? (double)mw[p] : 0.0;
+        if (d2 <= rc2) {
+          n_core++;
+          w_core += w;
+        } else if (d2 <= rs2) {
+          const uint32_t b = bin_of(d2, rc2, inv_w, nb);
+          bins[b]++;
           n_shell++;
+          wbins[b] += w;
+          w_shell += w;
         }
       }
     }
   }
   const uint64_t total = n_core + n_shell;
-  if (total == 0) {
+  const double total_w = w_core + w_shell;
+  const float expected = mass_factor * r_search * r_search * r_search;
+  if (weighted ? !(total_w > 0.0) : total == 0) {
     free(bins);
+    free(wbins);
     *reason = ORC_RESCALE_EMPTY;
     return -1.0f;
   }
-  const float expected = vol_factor * r_search * r_search * r_search;
-  if (((float)total / expected) - 1.0f <= threshold) {
+  if (((weighted ? (float)total_w : (float)total) / expected) - 1.0f <= threshold) {
     free(bins);
+    free(wbins);
     *reason = ORC_RESCALE_BEYOND;
     return -1.0f;
   }
-  const float K = (threshold + 1.0f) * vol_factor;
+  const float K = (threshold + 1.0f) * mass_factor;
   const float K2 = K * K;
   if (window < 1)
     window = 1;
   uint64_t above = 0;
+  double above_w = 0.0;
   for (int32_t b = (int32_t)nb - 1; b >= 0; b--) {
     const uint32_t cnt = bins[b];
-    if (cnt == 0)
+    const double cnt_w = wbins[b];
+    if (weighted ? !(cnt_w > 0.0) : cnt == 0) {
+      above += cnt; /* weightless tracers: skipped, still counted above */
       continue;
+    }
     float hi = rc2 + (float)(b + 1) * bin_w;
     if (b == (int32_t)nb - 1)
       hi = rs2 * (1.0f + 1e-6f);
-    const float n_min = (float)(total - above - cnt);
+    const float n_min = weighted ? (float)(total_w - above_w - cnt_w)
+                                 : (float)(total - above - cnt);
     if (n_min * n_min > K2 * hi * hi * hi) {
       above += cnt;
+      above_w += cnt_w;
       continue;
     }
     int32_t blo = b - (int32_t)(window - 1);
     if (blo < 0)
       blo = 0;
     uint64_t wcount = 0;
-    for (int32_t bb = blo; bb <= b; bb++)
+    double wweight = 0.0;
+    for (int32_t bb = blo; bb <= b; bb++) {
       wcount += bins[bb];
+      wweight += wbins[bb];
+    }
     const float wlo = rc2 + (float)blo * bin_w;
     float r = -1.0f;
     if (resolve(&q, (uint32_t)blo, (uint32_t)b, wlo, hi, rc2, rs2, inv_w, nb,
-          total - above, K2, &r, n_in_out)) {
+          total - above, total_w - above_w, K2, &r, n_in_out, weighted)) {
       free(bins);
+      free(wbins);
       return r;
     }
     above += wcount;
+    above_w += wweight;
     b = blo;
   }
   free(bins);
+  free(wbins);
   *reason = ORC_RESCALE_BELOW_RUNG;
   return -1.0f;
+}
+
+float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
+  float r_search, float threshold, float vol_factor, float rmin,
+  const orc_template_t* tpl, uint32_t window, uint8_t* reason,
+  uint64_t* n_in_out) {
+  return find_radius_impl(m, cx, cy, cz, r_search, threshold, vol_factor,
+    vol_factor, rmin, tpl, window, reason, n_in_out, 0);
+}
+
+float orc_find_radius_weighted(const orc_mesh_t* m, float cx, float cy,
+  float cz, float r_search, float threshold, float vol_factor,
+  float mass_factor, float rmin, const orc_template_t* tpl, uint32_t window,
+  uint8_t* reason, uint64_t* n_in_out) {
+  return find_radius_impl(m, cx, cy, cz, r_search, threshold, vol_factor,
+    mass_factor, rmin, tpl, window, reason, n_in_out, 1);
 }
 
 /* exodus_rescale.c:1074-1167 */
@@ -812,6 +922,14 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   const float box_volume = box * box * box;
   const float mean_density = (float)n_particles / box_volume;
   const float vol = (4.0f / 3.0f) * PI_F * mean_density;
+  /* exodus_finder.c:1639-1642: a weighted mesh's density tests use the mean
+   * weight density; the witness is never run on one (:1868) */
+  const int weighted = mesh->w != NULL;
+  const float mass =
+    weighted ? (4.0f / 3.0f) * PI_F * ((float)mesh->total_weight / box_volume)
+             : vol;
+  if (weighted)
+    use_witness = 0;
   const float scale = lagrangian ? (float)cbrt(1.0 + (double)t) : 1.0f;
   const float factors[4] = {1.5f, 1.25f, 1.75f, 2.0f};
   const float sf = factors[(search_opt >> 12) & 3u];
@@ -892,8 +1010,11 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
         }
         uint8_t reason;
         uint64_t n_in = 0;
-        const float r = orc_find_radius(mesh, cx, cy, cz, r_search, t, vol,
-          rmin, &tpl, window, &reason, &n_in);
+        const float r = weighted
+          ? orc_find_radius_weighted(mesh, cx, cy, cz, r_search, t, vol, mass,
+              rmin, &tpl, window, &reason, &n_in)
+          : orc_find_radius(mesh, cx, cy, cz, r_search, t, vol, rmin, &tpl,
+              window, &reason, &n_in);
         e->status = r < 0.0f ? 2 : 3;
         e->r = r;
         e->n_in = n_in;

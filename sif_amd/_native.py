@@ -102,8 +102,13 @@ _sig("sif_amd_cic_adjoint", i32, P, P, P, u64, u32, f32, P, P, P, P)
 _sig("sif_amd_squared_residual", i32, P, P, u64, f32, P, C.POINTER(f64))
 _sig("sif_amd_find_radius", i32, P, P, P, P, u32, f32, P, P, P, u32, f32, f32, f32, f32,
      P, P, P)
+_sig("sif_amd_find_radius_weighted", i32, P, P, P, P, P, P, u32, f32, P, P, P, u32, f32, f32,
+     f32, f32, f32, P, P, P)
 _sig("sif_amd_exodus_device", C.POINTER(Catalogue), P, u32, f32, P, P, P, P, u32, u64,
      C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
+     C.POINTER(RungStats))
+_sig("sif_amd_exodus_device_weighted", C.POINTER(Catalogue), P, u32, f32, P, P, P, P, P, P,
+     f64, u32, u64, C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
      C.POINTER(RungStats))
 _sig("sif_amd_exodus_fields", i32, P, u32, f32, C.POINTER(f32), u32, u32, P)
 _sig("sif_catalogue_free", None, C.POINTER(Catalogue))

@@ -841,10 +841,10 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
       (double)mesh->box_length, (double)grid->box_length);
     return nullptr;
   }
-  if (mesh->weights) {
-    error_set(SIF_ERR_UNSUPPORTED, "mesh: weighted tracers are not in this build");
+  if (mesh->weights &&
+      check_weights_host(mesh->weights, mesh->n_particles, mesh->cell_weights,
+        mesh->total_weight) != SIF_OK)
     return nullptr;
-  }
   if ((st = check_options(options)) != SIF_OK)
     return nullptr;
   const char* named = extras ? extras->checkpoint : nullptr;
@@ -941,12 +941,14 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
   const uint64_t cells = grid->total_cells;
   const uint64_t mcells = mesh->total_cells;
   const uint64_t n = mesh->n_particles;
-  Dev dv, doff, dmx, dmy, dmz;
+  Dev dv, doff, dmx, dmy, dmz, dmw, dcw;
   st = dv.upload(grid->values, cells * 4);
   if (st == SIF_OK) st = doff.upload(mesh->cell_offsets, (mcells + 1) * 8);
   if (st == SIF_OK) st = dmx.upload(mesh->x, n * 4);
   if (st == SIF_OK) st = dmy.upload(mesh->y, n * 4);
   if (st == SIF_OK) st = dmz.upload(mesh->z, n * 4);
+  if (st == SIF_OK && mesh->weights) st = dmw.upload(mesh->weights, n * 4);
+  if (st == SIF_OK && mesh->weights) st = dcw.upload(mesh->cell_weights, mcells * 4);
   if (st != SIF_OK) {
     if (ck.on_disk && !ck.progress)
       unlink(ck.path);
@@ -961,6 +963,11 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
   R.d_mx = dmx.as<float>();
   R.d_my = dmy.as<float>();
   R.d_mz = dmz.as<float>();
+  if (mesh->weights) {
+    R.d_mw = dmw.as<float>();
+    R.d_cw = dcw.as<float>();
+  }
+  R.total_weight = mesh->weights ? mesh->total_weight : (double)n;
   R.mesh_n = mesh->n_cells;
   R.n_particles = n;
   sorted_ladder(radii, n_radii, R.radii);

@@ -490,6 +490,14 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   const float box_volume = R.box * R.box * R.box;
   const float mean_density = (float)R.n_particles / box_volume;
   const float vol = (4.0f / 3.0f) * SIF_PI * mean_density;
+  // exodus_finder.c:1634-1642: the density tests of a weighted mesh use the
+  // mean weight density, rounded and divided as the count's is
+  MeshWeights mw;
+  if (R.d_mw) {
+    mw.w = R.d_mw;
+    mw.cell = R.d_cw;
+    mw.mass_factor = (4.0f / 3.0f) * SIF_PI * ((float)R.total_weight / box_volume);
+  }
   const float scale =
     (R.opt & SIF_FINDER_LAGRANGIAN_OVERLAP) ? (float)cbrt(1.0 + (double)R.threshold) : 1.0f;
   const float sf = sif__finder_search_factor(R.opt);
@@ -782,7 +790,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         e = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
           B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, rmin, r_search,
           R.threshold, vol, B.d_rad.as<float>(), B.d_reason.as<uint32_t>(),
-          B.d_nin.as<uint64_t>(), algo ? B.d_algo.as<uint64_t>() : nullptr);
+          B.d_nin.as<uint64_t>(), algo ? B.d_algo.as<uint64_t>() : nullptr, &mw);
       if (e == SIF_OK)
         e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
           B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
@@ -1140,27 +1148,119 @@ int check_options(sif_option opt) {
   return SIF_OK;
 }
 
+namespace {
+
+enum { WEIGHT_NEGATIVE = 0, WEIGHT_NONFINITE = 1 };
+
+// exodus_finder.c:1039-1060: a weight by its bits -- exponent all ones is not
+// finite; the sign bit on anything but a zero is negative (-0 passes).
+SIF_HD inline int weight_kind(uint32_t b) {
+  if (((b >> 23) & 0xffu) == 0xffu)
+    return WEIGHT_NONFINITE;
+  return ((b >> 31) && (b << 1)) ? WEIGHT_NEGATIVE : -1;
+}
+
+__global__ void weight_check_kernel(const uint32_t* __restrict__ w, uint64_t n,
+  unsigned long long* __restrict__ counts) {
+  unsigned long long neg = 0, bad = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int k = weight_kind(w[i]);
+    neg += k == WEIGHT_NEGATIVE;
+    bad += k == WEIGHT_NONFINITE;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    neg += __shfl_xor(neg, off, 64);
+    bad += __shfl_xor(bad, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (neg || bad)) {
+    atomicAdd(&counts[0], neg);
+    atomicAdd(&counts[1], bad);
+  }
+}
+
+int weights_verdict(uint64_t negative, uint64_t nonfinite, uint64_t n, double total) {
+  if (negative || nonfinite) {
+    error_set(SIF_ERR_INVALID,
+      "mesh: weights must be finite and non-negative (%llu negative, %llu not finite, of "
+      "%llu)", (unsigned long long)negative, (unsigned long long)nonfinite,
+      (unsigned long long)n);
+    return SIF_ERR_INVALID;
+  }
+  if (!(total > 0.0)) {
+    error_set(SIF_ERR_INVALID, "mesh: weights sum to zero (no mean density)");
+    return SIF_ERR_INVALID;
+  }
+  return SIF_OK;
+}
+
+DeviceBuffer g_weight_counts;
+
+}  // namespace
+
+int check_weights_host(const float* w, uint64_t n, const float* cell_w, double total) {
+  if (!cell_w) {
+    error_set(SIF_ERR_INVALID, "mesh: weights but no per-cell weight table");
+    return SIF_ERR_INVALID;
+  }
+  uint64_t neg = 0, bad = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    uint32_t b;
+    memcpy(&b, w + i, 4);
+    const int k = weight_kind(b);
+    neg += k == WEIGHT_NEGATIVE;
+    bad += k == WEIGHT_NONFINITE;
+  }
+  return weights_verdict(neg, bad, n, total);
+}
+
+int check_weights_device(const float* d_w, uint64_t n, const float* d_cell_w, double total) {
+  if (!d_cell_w) {
+    error_set(SIF_ERR_INVALID, "mesh: weights but no per-cell weight table");
+    return SIF_ERR_INVALID;
+  }
+  int st = g_weight_counts.reserve(2 * sizeof(uint64_t));
+  if (st != SIF_OK)
+    return st;
+  uint64_t counts[2] = {0, 0};
+  SIF_HIP(hipMemcpyAsync(g_weight_counts.ptr, counts, sizeof(counts), hipMemcpyHostToDevice,
+    stream()));
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+  if (n)
+    weight_check_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(
+      reinterpret_cast<const uint32_t*>(d_w), n,
+      reinterpret_cast<unsigned long long*>(g_weight_counts.ptr));
+  SIF_HIP(hipGetLastError());
+  SIF_HIP(hipMemcpyAsync(counts, g_weight_counts.ptr, sizeof(counts), hipMemcpyDeviceToHost,
+    stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return weights_verdict(counts[0], counts[1], n, total);
+}
+
 }  // namespace sif_amd
 
 using namespace sif_amd;
 
-extern "C" {
+namespace {
 
-sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float box,
+sif_catalogue_t* exodus_device(float* d_delta, uint32_t n_cells, float box,
   const uint64_t* d_offsets, const float* d_mx, const float* d_my, const float* d_mz,
-  uint32_t mesh_cells, uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
-  sif_real threshold, sif_real overlap_fraction, sif_option options, int restore,
-  uint64_t** n_in, sif_amd_rung_stats_t* stats) {
+  const float* d_mw, const float* d_cw, double total_weight, uint32_t mesh_cells,
+  uint64_t n_particles, const sif_real* radii, uint32_t n_radii, sif_real threshold,
+  sif_real overlap_fraction, sif_option options, int restore, uint64_t** n_in,
+  sif_amd_rung_stats_t* stats) {
   int st = ensure_init();
   if (st == SIF_OK)
     st = check_args(radii, n_radii, threshold, overlap_fraction);
-  if (st == SIF_OK)
-    st = check_options(options);
   if (st == SIF_OK && (!d_delta || !d_offsets || !d_mx || !d_my || !d_mz ||
                         n_particles == 0 || mesh_cells == 0 || !(box > 0.0f))) {
     error_set(SIF_ERR_INVALID, "exodus: grid and mesh arrays are required");
     st = SIF_ERR_INVALID;
   }
+  if (st == SIF_OK && d_mw)
+    st = check_weights_device(d_mw, n_particles, d_cw, total_weight);
+  if (st == SIF_OK)
+    st = check_options(options);
   if (st != SIF_OK)
     return nullptr;
   RunSpec R;
@@ -1171,6 +1271,9 @@ sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float b
   R.d_mx = d_mx;
   R.d_my = d_my;
   R.d_mz = d_mz;
+  R.d_mw = d_mw;
+  R.d_cw = d_cw;
+  R.total_weight = d_mw ? total_weight : (double)n_particles;
   R.mesh_n = mesh_cells;
   R.n_particles = n_particles;
   sorted_ladder(radii, n_radii, R.radii);
@@ -1182,6 +1285,35 @@ sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float b
   sif_catalogue_t* cat = nullptr;
   st = exodus_run(R, &cat, n_in, stats);
   return st == SIF_OK ? cat : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+sif_catalogue_t* sif_amd_exodus_device_weighted(float* d_delta, uint32_t n_cells, float box,
+  const uint64_t* d_offsets, const float* d_mx, const float* d_my, const float* d_mz,
+  const float* d_mw, const float* d_cell_weights, double total_weight, uint32_t mesh_cells,
+  uint64_t n_particles, const sif_real* radii, uint32_t n_radii, sif_real threshold,
+  sif_real overlap_fraction, sif_option options, int restore, uint64_t** n_in,
+  sif_amd_rung_stats_t* stats) {
+  if (!d_mw) {
+    error_set(SIF_ERR_INVALID, "exodus: weighted: no weights");
+    return nullptr;
+  }
+  return exodus_device(d_delta, n_cells, box, d_offsets, d_mx, d_my, d_mz, d_mw,
+    d_cell_weights, total_weight, mesh_cells, n_particles, radii, n_radii, threshold,
+    overlap_fraction, options, restore, n_in, stats);
+}
+
+sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float box,
+  const uint64_t* d_offsets, const float* d_mx, const float* d_my, const float* d_mz,
+  uint32_t mesh_cells, uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_real overlap_fraction, sif_option options, int restore,
+  uint64_t** n_in, sif_amd_rung_stats_t* stats) {
+  return exodus_device(d_delta, n_cells, box, d_offsets, d_mx, d_my, d_mz, nullptr, nullptr,
+    0.0, mesh_cells, n_particles, radii, n_radii, threshold, overlap_fraction, options,
+    restore, n_in, stats);
 }
 
 int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells, float box,

@@ -349,8 +349,9 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
   const float* cz, uint32_t n, float rmin, float r_search, float threshold,
   float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
-  uint64_t* algo_bytes) {
+  uint64_t* algo_bytes, const MeshWeights* weights) {
   const float l = box / (float)mesh_n;
+  const bool weighted = weights && weights->w;
   int st = template_device(mesh_n, l, rmin, r_search);
   if (st != SIF_OK)
     return st;
@@ -375,16 +376,26 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   P.cy = cy;
   P.cz = cz;
   P.n_cand = n;
+  P.mw = weighted ? weights->w : nullptr;
+  P.cw = weighted ? weights->cell : nullptr;
+  P.mass_factor = weighted ? weights->mass_factor : vol_factor;
   P.radius = radius;
   P.reason = reason;
   P.n_in = n_in;
   P.algo_bytes = algo_bytes;
-  grow_lds_words(grow_bin_count(r_search, rmin, vol_factor), 0, &P.lds_bins, &P.lds_cache);
+  const uint32_t nb = grow_bin_count(r_search, rmin, vol_factor);
+  if (weighted)
+    grow_lds_words_weighted(nb, &P.lds_bins, &P.lds_cache);
+  else
+    grow_lds_words(nb, 0, &P.lds_bins, &P.lds_cache);
   const size_t lds_bytes =
-    (size_t)GROW_WAVES * grow_wave_words(P.lds_bins, P.lds_cache) * sizeof(uint32_t);
+    (size_t)GROW_WAVES * grow_wave_words(P.lds_bins, P.lds_cache, weighted) * sizeof(uint32_t);
   timing_begin(SIF_AMD_STAGE_GROW);
   const uint64_t blocks = ((uint64_t)n + GROW_WAVES - 1) / GROW_WAVES;
-  grow_kernel<<<(unsigned)blocks, 64 * GROW_WAVES, lds_bytes, stream()>>>(P);
+  if (weighted)
+    grow_kernel<true><<<(unsigned)blocks, 64 * GROW_WAVES, lds_bytes, stream()>>>(P);
+  else
+    grow_kernel<false><<<(unsigned)blocks, 64 * GROW_WAVES, lds_bytes, stream()>>>(P);
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_GROW);
   return SIF_OK;
@@ -562,6 +573,31 @@ int sif_amd_find_radius(const uint64_t* offsets, const float* mx, const float* m
   }
   st = grow_batch(offsets, mx, my, mz, mesh_cells, box, cx, cy, cz, n, rmin,
     r_search, threshold, vol_factor, radius, reason, n_in);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+int sif_amd_find_radius_weighted(const uint64_t* offsets, const float* mx, const float* my,
+  const float* mz, const float* mw, const float* cell_weights, uint32_t mesh_cells, float box,
+  const float* cx, const float* cy, const float* cz, uint32_t n, float rmin, float r_search,
+  float threshold, float vol_factor, float mass_factor, float* radius, uint32_t* reason,
+  uint64_t* n_in) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!offsets || !mx || !my || !mz || !mw || !cell_weights || !cx || !cy || !cz ||
+      !radius || !reason || mesh_cells == 0 || !(box > 0.0f)) {
+    error_set(SIF_ERR_INVALID, "sif_amd_find_radius_weighted: invalid arguments");
+    return SIF_ERR_INVALID;
+  }
+  MeshWeights w;
+  w.w = mw;
+  w.cell = cell_weights;
+  w.mass_factor = mass_factor;
+  st = grow_batch(offsets, mx, my, mz, mesh_cells, box, cx, cy, cz, n, rmin, r_search,
+    threshold, vol_factor, radius, reason, n_in, nullptr, &w);
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipStreamSynchronize(stream()));

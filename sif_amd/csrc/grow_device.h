@@ -67,11 +67,19 @@ struct GrowParams {
   const float* cy;
   const float* cz;
   uint32_t n_cand;
+  // weighted tracers (exodus_rescale.c:1029-1037): per-tracer and per-cell
+  // weights, and the mean weight density in place of vol_factor in every
+  // density test; NULL and unused on an unweighted mesh
+  const float* mw;
+  const float* cw;
+  float mass_factor;
   float* radius;
   uint32_t* reason;
   uint64_t* n_in;
-  // LDS words a wave holds for its histogram (at least the bin count) and
-  // for its annulus cache (at least REFINE_CAP: the window buffer shares it)
+  // LDS words a wave holds for its histogram (at least the bin count; three
+  // words a bin when weighted: the count and the weight in double) and the
+  // entries of its annulus cache (at least REFINE_CAP: the window buffer
+  // shares it; a word an entry, two when weighted: distance and weight)
   uint32_t lds_bins, lds_cache;
   // optional: the candidate's algorithmic bytes -- 8 per template cell whose
   // offsets it reads, 12 per particle of the cells whose particles it must
@@ -87,9 +95,13 @@ struct GrowParams {
 constexpr uint32_t CACHE_MAX = 2048;
 constexpr uint32_t LDS_TAIL_WORDS = 6 * 64 + SUB_BINS;
 
-SIF_GROW_HD uint32_t grow_wave_words(uint32_t lds_bins, uint32_t lds_cache) {
-  return lds_bins + lds_cache + LDS_TAIL_WORDS;
+SIF_GROW_HD uint32_t grow_wave_words(uint32_t lds_bins, uint32_t lds_cache,
+  bool weighted = false) {
+  return lds_bins + lds_cache * (weighted ? 2u : 1u) + LDS_TAIL_WORDS;
 }
+
+// Words of a weighted histogram of nb bins: nb counts, then nb doubles.
+SIF_GROW_HD uint32_t grow_weighted_bin_words(uint32_t nb) { return 3u * ((nb + 3u) & ~3u); }
 
 // A launch's LDS words a wave: the rung's histogram, then as much annulus
 // cache as keeps 16 waves on a CU (four a SIMD, what the kernel's registers
@@ -108,10 +120,21 @@ SIF_GROW_HD void grow_lds_words(uint32_t nb, uint32_t cache_words, uint32_t* lds
   *lds_cache = cache < (uint32_t)REFINE_CAP ? (uint32_t)REFINE_CAP : cache;
 }
 
+// The weighted kernel's: the histogram's counts and weights, and a cache of
+// REFINE_CAP (distance, weight) pairs -- what its pair sort holds in
+// registers.
+SIF_GROW_HD void grow_lds_words_weighted(uint32_t nb, uint32_t* lds_bins, uint32_t* lds_cache) {
+  *lds_bins = grow_weighted_bin_words(nb);
+  *lds_cache = (uint32_t)REFINE_CAP;
+}
+
 struct WaveLDS {
   uint32_t* bins;
+  double* wbins;    // weighted: the bins' weights
   float* cache;
+  float* cache_w;   // weighted: the weights of the cached distances
   float* buf;
+  float* buf_w;
   uint32_t* t_start;
   uint32_t* t_scan;
   float* t_ex;
@@ -128,6 +151,14 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Every lane ends with the same sum: each butterfly step adds the same two
+// values on both lanes of a pair, so the order is fixed by the lane layout.
+__device__ __forceinline__ double wave_sum_f64(double v) {
   for (int off = 32; off > 0; off >>= 1)
     v += __shfl_xor(v, off, 64);
   return v;
@@ -164,6 +195,14 @@ __device__ __forceinline__ uint64_t uni(uint64_t v) {
   return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | (uint64_t)uni((uint32_t)v);
 }
 __device__ __forceinline__ float unif(float v) { return __int_as_float(uni(__float_as_int(v))); }
+__device__ __forceinline__ double unid(double v) {
+  uint64_t b;
+  __builtin_memcpy(&b, &v, 8);
+  b = uni(b);
+  double r;
+  __builtin_memcpy(&r, &b, 8);
+  return r;
+}
 
 // exodus_rescale.c:349-355, no contraction
 __device__ __forceinline__ float dist2(float px, float py, float pz, float cx,
@@ -190,7 +229,8 @@ struct Query {
 // exodus_rescale.c:295-342: template entry -> particle range and the image
 // of the centre it is measured against.
 __device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
-  const int4 t, uint32_t* start, uint32_t* count, float* ex, float* ey, float* ez) {
+  const int4 t, uint32_t* start, uint32_t* count, float* ex, float* ey, float* ez,
+  uint64_t* cell) {
   const int32_t N = P.mesh_n;
   const float L = P.box;
   int32_t ix = q.ix + t.x;
@@ -224,6 +264,7 @@ __device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
   const uint64_t s = P.offsets[flat], e = P.offsets[flat + 1];
   *start = (uint32_t)s;
   *count = (uint32_t)(e - s);
+  *cell = flat;
   *ex = cx;
   *ey = cy;
   *ez = cz;
@@ -232,12 +273,13 @@ __device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
 constexpr int PARTICLE_UNROLL = 8;
 
 // Walks the template 64 entries at a time. `want(type, bound)` says whether
-// an entry is visited; `whole(type, count)` consumes a whole cell without
-// reading its particles (returns true when it did); `visit(d2, type, valid)`
-// sees every particle of the remaining cells. visit is called by every lane
-// the same number of times -- `valid` false on a lane with no particle that
-// round -- so a visitor may use wave-wide operations.
-template <class Want, class Whole, class Visit>
+// an entry is visited; `whole(type, count, cell)` consumes a whole cell
+// without reading its particles (returns true when it did); `visit(d2, w,
+// type, valid)` sees every particle of the remaining cells, with its weight
+// when WT (1 otherwise). visit is called by every lane the same number of
+// times -- `valid` false on a lane with no particle that round -- so a
+// visitor may use wave-wide operations.
+template <bool WT, class Want, class Whole, class Visit>
 __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& q,
   WaveLDS& S, int lane, Want want, Whole whole, Visit visit) {
   for (uint32_t base = 0; base < P.tcount; base += 64) {
@@ -248,8 +290,9 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
       const int4 t = P.tcell[e];
       type = (uint32_t)t.w;
       if (want(type, P.tbound[e])) {
-        query_cell(P, q, t, &start, &cnt, &ex, &ey, &ez);
-        if (cnt && whole(type, cnt))
+        uint64_t cell = 0;
+        query_cell(P, q, t, &start, &cnt, &ex, &ey, &ez, &cell);
+        if (cnt && whole(type, cnt, cell))
           cnt = 0;
       }
     }
@@ -282,12 +325,17 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
         pi[u] = j < total ? S.t_start[ent] + (j - before) : 0xffffffffu;
       }
       float px[PARTICLE_UNROLL], py[PARTICLE_UNROLL], pz[PARTICLE_UNROLL];
+      float pw[PARTICLE_UNROLL];
 #pragma unroll
       for (int u = 0; u < PARTICLE_UNROLL; u++) {
         const uint32_t p = pi[u] != 0xffffffffu ? pi[u] : 0u;
         px[u] = P.mx[p];
         py[u] = P.my[p];
         pz[u] = P.mz[p];
+        if constexpr (WT)
+          pw[u] = P.mw[p];
+        else
+          pw[u] = 1.0f;
       }
 #pragma unroll
       for (int u = 0; u < PARTICLE_UNROLL; u++) {
@@ -295,7 +343,7 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
         const int lo = en[u];
         const float d2 =
           valid ? dist2(px[u], py[u], pz[u], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]) : 0.0f;
-        visit(d2, valid ? S.t_type[lo] : 0u, valid);
+        visit(d2, valid ? pw[u] : 0.0f, valid ? S.t_type[lo] : 0u, valid);
       }
     }
     wave_sync();
@@ -398,17 +446,164 @@ __device__ __forceinline__ void wave_sort_half(float* buf, uint32_t n, int lane)
   wave_sync();
 }
 
+// wave_append for (distance, weight) pairs: both go to the same slot.
+__device__ __forceinline__ void wave_append2(float* buf, float* buf_w, uint32_t cap,
+  uint32_t& count, bool take, float d2, float w, int lane) {
+  const uint64_t m = __ballot(take);
+  if (take) {
+    const uint32_t slot = count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < cap) {
+      buf[slot] = d2;
+      buf_w[slot] = w;
+    }
+  }
+  count += (uint32_t)__popcll(m);
+}
+
+// Ascending bitonic sort of the pairs (key[e], val[e]), e < n, by key and
+// then by value: both are non-negative floats (a -0 weight is taken as +0),
+// so their bit patterns, key above value, order as one 64-bit integer.
+// Layout as wave_sort_regs.
+template <int R>
+__device__ void wave_sort_pairs_regs(float* key, float* val, uint32_t n, int lane) {
+  uint64_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t e = (uint32_t)r * 64 + lane;
+    v[r] = e < n ? ((uint64_t)__float_as_uint(key[e]) << 32) |
+                     (uint64_t)__float_as_uint(__fadd_rn(val[e], 0.0f))
+                 : ~0ull;  // sorts last
+  }
+#pragma unroll
+  for (uint32_t k = 2; k <= (uint32_t)R * 64; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const int jr = (int)(j / 64);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if ((r & jr) == 0) {
+            const bool asc = ((((uint32_t)r * 64 + lane) & k) == 0);
+            const uint64_t a = v[r], b = v[r | jr];
+            const bool swap = (a > b) == asc;
+            v[r] = swap ? b : a;
+            v[r | jr] = swap ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint32_t e = (uint32_t)r * 64 + lane;
+          const uint64_t o = __shfl_xor(v[r], (int)j, 64);
+          const bool asc = (e & k) == 0, lower = (e & j) == 0;
+          const uint64_t mn = v[r] < o ? v[r] : o, mx = v[r] < o ? o : v[r];
+          v[r] = (lower == asc) ? mn : mx;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t e = (uint32_t)r * 64 + lane;
+    if (e < n) {
+      key[e] = __uint_as_float((uint32_t)(v[r] >> 32));
+      val[e] = __uint_as_float((uint32_t)v[r]);
+    }
+  }
+}
+
+// Ascending sort of the pairs e < n, n <= REFINE_CAP, by one wave.
+__device__ __forceinline__ void wave_sort_pairs(float* key, float* val, uint32_t n, int lane) {
+  wave_sync();
+  if (n <= 64)
+    wave_sort_pairs_regs<1>(key, val, n, lane);
+  else if (n <= 128)
+    wave_sort_pairs_regs<2>(key, val, n, lane);
+  else if (n <= 256)
+    wave_sort_pairs_regs<4>(key, val, n, lane);
+  else if (n <= 512)
+    wave_sort_pairs_regs<8>(key, val, n, lane);
+  else
+    wave_sort_pairs_regs<16>(key, val, n, lane);
+  wave_sync();
+}
+
+// The same for n <= REFINE_CAP / 2, the windows sorted after the cache.
+__device__ __forceinline__ void wave_sort_pairs_half(float* key, float* val, uint32_t n,
+  int lane) {
+  wave_sync();
+  if (n <= 64)
+    wave_sort_pairs_regs<1>(key, val, n, lane);
+  else if (n <= 128)
+    wave_sort_pairs_regs<2>(key, val, n, lane);
+  else if (n <= 256)
+    wave_sort_pairs_regs<4>(key, val, n, lane);
+  else
+    wave_sort_pairs_regs<8>(key, val, n, lane);
+  wave_sync();
+}
+
+// exodus_rescale.c:460-484: the weighted walk, outside in, over the pairs
+// key/val[0..cnt) in ascending order, top_w the weight at or below the top
+// of the range. The tracer at index k encloses top_w less the weight of
+// every tracer from k up, itself included; it passes when that weight
+// squared is at most K2 d2^3. The weight above each index comes from a
+// suffix scan 64 pairs at a time, from the top; the first block (from the
+// top) holding a pass holds the answer, its highest passing index. Returns
+// that index (uniform), or -1 with *range_w the weight of the whole range.
+__device__ __forceinline__ int weighted_walk(const float* key, const float* val, uint32_t cnt,
+  double top_w, float K2, int lane, double* range_w) {
+  double carry = 0.0;  // uniform: weight of the blocks above
+  int found = -1;
+  for (int32_t top = (int32_t)cnt; top > 0; top -= 64) {
+    const int32_t k = top - 64 + lane;
+    float d2 = 0.0f, w = 0.0f;
+    if (k >= 0) {
+      d2 = key[k];
+      w = val[k];
+    }
+    double s = (double)w;  // weight from k to the top of the block
+    for (int off = 1; off < 64; off <<= 1) {
+      const double y = __shfl_down(s, off, 64);
+      if (lane + off < 64)
+        s += y;
+    }
+    int f = -1;
+    if (k >= 0 && d2 != 0.0f) {
+      const float w_in = (float)(top_w - (carry + s));
+      const float lhs = __fmul_rn(w_in, w_in);
+      const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
+      if (lhs <= rhs)
+        f = (int)k;
+    }
+    f = uni(wave_max_i32(f));
+    if (f >= 0) {
+      found = f;
+      break;
+    }
+    carry += unid(__shfl(s, 0, 64));
+  }
+  *range_w = carry;
+  return found;
+}
+
+template <bool WT>
 __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) {
   GROW_DYNAMIC_LDS(lds_words);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   WaveLDS S;
   {
-    uint32_t* base = lds_words + uni((uint32_t)w * grow_wave_words(P.lds_bins, P.lds_cache));
+    uint32_t* base =
+      lds_words + uni((uint32_t)w * grow_wave_words(P.lds_bins, P.lds_cache, WT));
     S.bins = base;
+    // weighted: the doubles follow the counts, 16-byte aligned
+    S.wbins = reinterpret_cast<double*>(base + ((P.lds_bins / 3u) & ~3u));
     S.cache = reinterpret_cast<float*>(base + P.lds_bins);
+    S.cache_w = S.cache + P.lds_cache;
     S.buf = S.cache;
-    uint32_t* tail = base + P.lds_bins + P.lds_cache;
+    S.buf_w = S.cache_w;
+    uint32_t* tail = base + P.lds_bins + P.lds_cache * (WT ? 2u : 1u);
     S.t_start = tail;
     S.t_scan = tail + 64;
     S.t_ex = reinterpret_cast<float*>(tail + 128);
@@ -421,9 +616,12 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
   const float rs2 = __fmul_rn(P.r_search, P.r_search);
   const float rc2 = __fmul_rn(P.rmin, P.rmin);
   const float span = __fsub_rn(rs2, rc2);
+  // the histogram is sized by the tracer count either way (exodus_rescale.c:
+  // 631-636); the density tests use the mean weight density when weighted
   const uint32_t nb = grow_bin_count(P.r_search, P.rmin, P.vol_factor);
-  const float expected = __fmul_rn(__fmul_rn(__fmul_rn(P.vol_factor, P.r_search), P.r_search), P.r_search);
-  const float K = __fmul_rn(__fadd_rn(P.threshold, 1.0f), P.vol_factor);
+  const float dens = WT ? P.mass_factor : P.vol_factor;
+  const float expected = __fmul_rn(__fmul_rn(__fmul_rn(dens, P.r_search), P.r_search), P.r_search);
+  const float K = __fmul_rn(__fadd_rn(P.threshold, 1.0f), dens);
   const float K2 = __fmul_rn(K, K);
   const float inv_w = __fdiv_rn((float)nb, span);
   const float bin_w = __fdiv_rn(span, (float)nb);
@@ -439,6 +637,7 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
   uint64_t algo = 0;
   bool cached = false;
   uint64_t n_annulus = 0, n_core = 0;
+  double total_w = 0.0;  // weighted: the search sphere's weight
   {
     Query q;
     q.cx = P.cx[c];
@@ -456,7 +655,9 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
       reason = R_DEGENERATE;
     } else if (!(span > 0.0f)) {
       reason = R_DEGENERATE;
-    } else if (nb > P.lds_bins || P.lds_cache < (uint32_t)REFINE_CAP) {
+    } else if ((WT ? grow_weighted_bin_words(nb) : nb) > P.lds_bins ||
+               P.lds_cache < (uint32_t)REFINE_CAP ||
+               (WT && P.lds_cache != (uint32_t)REFINE_CAP)) {
       reason = R_OVERFLOW;  // a launch sized for fewer bins: never expected
     } else {
       // 1. early refusal, O(surface): exodus_rescale.c:711-782. The pass also
@@ -464,14 +665,31 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
       // annulus distance in LDS while they fit, and builds the histogram of
       // the annulus (exodus_rescale.c:793-865, which reads the same
       // particles again) as it goes.
-      for (uint32_t b = lane; b < nb; b += 64)
+      for (uint32_t b = lane; b < nb; b += 64) {
         S.bins[b] = 0;
+        if constexpr (WT)
+          S.wbins[b] = 0.0;
+      }
       wave_sync();
-      uint64_t within = 0, surface = 0, shell = 0, core = 0;
+      uint64_t within = 0, surface = 0, shell = 0, core = 0, whole_cells = 0;
+      // weighted (exodus_rescale.c:745-865): the early refusal's weight takes
+      // whole CORE and SHELL cells from the cell table; the histogram's core
+      // and annulus weights are the tracers' own, but for whole CORE cells.
+      // Each lane sums its own share in double, in a fixed order.
+      double within_w = 0.0, core_w = 0.0, shell_w = 0.0;
       uint32_t appended = 0;  // uniform
-      for_particles(P, q, S, lane,
+      for_particles<WT>(P, q, S, lane,
         [](uint32_t, float2) { return true; },
-        [&](uint32_t type, uint32_t cnt) {
+        [&](uint32_t type, uint32_t cnt, uint64_t cell) {
+          if constexpr (WT) {
+            if (type == CELL_CORE || type == CELL_SHELL) {
+              const double c = (double)P.cw[cell];
+              within_w += c;
+              if (type == CELL_CORE)
+                core_w += c;
+              whole_cells++;
+            }
+          }
           if (type == CELL_CORE) {
             within += cnt;
             core += cnt;
@@ -479,23 +697,42 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
           }
           return false;
         },
-        [&](float d2, uint32_t type, bool valid) {
+        [&](float d2, float wt, uint32_t type, bool valid) {
           bool annulus = false;
           if (valid) {
             if (type == CELL_SHELL) {
               within++;
               shell++;
               annulus = true;
+              if constexpr (WT)
+                shell_w += (double)wt;
             } else {
               within += d2 <= rs2 ? 1u : 0u;
               surface++;
               core += d2 <= rc2 ? 1u : 0u;
               annulus = d2 > rc2 && d2 <= rs2;
+              if constexpr (WT) {
+                if (d2 <= rs2)
+                  within_w += (double)wt;
+                if (d2 <= rc2)
+                  core_w += (double)wt;
+                else if (annulus)
+                  shell_w += (double)wt;
+              }
             }
-            if (annulus)
-              atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
+            if (annulus) {
+              const uint32_t b = bin_of(d2, rc2, inv_w, nb);
+              atomicAdd(&S.bins[b], 1u);
+              // one wave owns these bins, so the order of the additions is
+              // the wave's own program and lane order
+              if constexpr (WT)
+                atomicAdd(&S.wbins[b], (double)wt);
+            }
           }
-          wave_append(S.cache, P.lds_cache, appended, annulus, d2, lane);
+          if constexpr (WT)
+            wave_append2(S.cache, S.cache_w, P.lds_cache, appended, annulus, d2, wt, lane);
+          else
+            wave_append(S.cache, P.lds_cache, appended, annulus, d2, lane);
         });
       wave_sync();
       within = uni(wave_sum_u64(within));
@@ -503,12 +740,27 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
       shell = uni(wave_sum_u64(shell));
       core = uni(wave_sum_u64(core));
       n_annulus = uni(appended);
-      if (within == 0) {
-        reason = R_EMPTY;
-      } else if (__fsub_rn(__fdiv_rn((float)within, expected), 1.0f) <= P.threshold) {
-        reason = R_BEYOND;
+      if constexpr (WT) {
+        within_w = unid(wave_sum_f64(within_w));
+        // exodus_rescale.c:864: w_core + w_shell
+        total_w = unid(wave_sum_f64(core_w)) + unid(wave_sum_f64(shell_w));
+        whole_cells = uni(wave_sum_u64(whole_cells));
+        // on a weighted mesh "empty" is "no weight inside"
+        if (!(within_w > 0.0)) {
+          reason = R_EMPTY;
+        } else if (__fsub_rn(__fdiv_rn((float)within_w, expected), 1.0f) <= P.threshold) {
+          reason = R_BEYOND;
+        }
+        algo = 8ull * P.tcount + 4ull * whole_cells +
+               16ull * (surface + (reason == R_OK ? shell : 0ull));
+      } else {
+        if (within == 0) {
+          reason = R_EMPTY;
+        } else if (__fsub_rn(__fdiv_rn((float)within, expected), 1.0f) <= P.threshold) {
+          reason = R_BEYOND;
+        }
+        algo = 8ull * P.tcount + 12ull * (surface + (reason == R_OK ? shell : 0ull));
       }
-      algo = 8ull * P.tcount + 12ull * (surface + (reason == R_OK ? shell : 0ull));
       cached = n_annulus <= P.lds_cache;
       n_core = core;
     }
@@ -518,12 +770,18 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
       // the annulus. A cached annulus of more than half the cache is sorted
       // whole now; a smaller one leaves room after it for any window.
       const bool cache_sorted = cached && n_annulus > P.lds_cache / 2;
-      if (cache_sorted)
-        wave_sort(S.cache, (uint32_t)n_annulus, lane);
+      if (cache_sorted) {
+        if constexpr (WT)
+          wave_sort_pairs(S.cache, S.cache_w, (uint32_t)n_annulus, lane);
+        else
+          wave_sort(S.cache, (uint32_t)n_annulus, lane);
+      }
       const uint64_t total = n_core + n_annulus;
-      if (total == 0) {
+      const bool empty = WT ? !(total_w > 0.0) : total == 0;
+      const float total_f = WT ? (float)total_w : (float)total;
+      if (empty) {
         reason = R_EMPTY;
-      } else if (__fsub_rn(__fdiv_rn((float)total, expected), 1.0f) <= P.threshold) {
+      } else if (__fsub_rn(__fdiv_rn(total_f, expected), 1.0f) <= P.threshold) {
         reason = R_BEYOND;
       } else {
         // 3. skip flags: lane L owns bins [L*per, (L+1)*per), top down.
@@ -541,19 +799,51 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
             suffix += y;
         }
         uint64_t above = suffix - mine;
-        for (int32_t b = (int32_t)b1 - 1; b >= (int32_t)b0; b--) {
-          const uint32_t cnt = S.bins[b];
-          if (cnt == 0)
-            continue;
-          float hi = __fadd_rn(rc2, __fmul_rn((float)(b + 1), bin_w));
-          if (b == (int32_t)nb - 1)
-            hi = __fmul_rn(rs2, __fadd_rn(1.0f, 1e-6f));
-          const float n_min = (float)(total - above - cnt);
-          const float lhs = __fmul_rn(n_min, n_min);
-          const float rhs = __fmul_rn(__fmul_rn(__fmul_rn(K2, hi), hi), hi);
-          if (!(lhs > rhs))
-            S.bins[b] = cnt | 0x80000000u;  // a bin the walk must open
-          above += cnt;
+        if constexpr (WT) {
+          // the same in weight (exodus_rescale.c:915-957): a bin is skipped
+          // on the weight below it; one with no weight is never opened. A
+          // bin the walk must open is flagged by negating its weight.
+          double mine_w = 0.0;
+          for (uint32_t b = b0; b < b1; b++)
+            mine_w += S.wbins[b];
+          double suffix_w = mine_w;
+          for (int off = 1; off < 64; off <<= 1) {
+            const double y = __shfl_down(suffix_w, off, 64);
+            if (lane + off < 64)
+              suffix_w += y;
+          }
+          double above_w = __shfl_down(suffix_w, 1, 64);
+          if (lane == 63)
+            above_w = 0.0;
+          for (int32_t b = (int32_t)b1 - 1; b >= (int32_t)b0; b--) {
+            const double cw_b = S.wbins[b];
+            if (!(cw_b > 0.0))
+              continue;
+            float hi = __fadd_rn(rc2, __fmul_rn((float)(b + 1), bin_w));
+            if (b == (int32_t)nb - 1)
+              hi = __fmul_rn(rs2, __fadd_rn(1.0f, 1e-6f));
+            const float n_min = (float)(total_w - above_w - cw_b);
+            const float lhs = __fmul_rn(n_min, n_min);
+            const float rhs = __fmul_rn(__fmul_rn(__fmul_rn(K2, hi), hi), hi);
+            if (!(lhs > rhs))
+              S.wbins[b] = -cw_b;
+            above_w += cw_b;
+          }
+        } else {
+          for (int32_t b = (int32_t)b1 - 1; b >= (int32_t)b0; b--) {
+            const uint32_t cnt = S.bins[b];
+            if (cnt == 0)
+              continue;
+            float hi = __fadd_rn(rc2, __fmul_rn((float)(b + 1), bin_w));
+            if (b == (int32_t)nb - 1)
+              hi = __fmul_rn(rs2, __fadd_rn(1.0f, 1e-6f));
+            const float n_min = (float)(total - above - cnt);
+            const float lhs = __fmul_rn(n_min, n_min);
+            const float rhs = __fmul_rn(__fmul_rn(__fmul_rn(K2, hi), hi), hi);
+            if (!(lhs > rhs))
+              S.bins[b] = cnt | 0x80000000u;  // a bin the walk must open
+            above += cnt;
+          }
         }
         wave_sync();
 
@@ -562,7 +852,7 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
         while (reason == R_OK && radius < 0.0f) {
           int best = -1;
           for (int32_t b = (int32_t)b1 - 1; b >= (int32_t)b0; b--)
-            if (b < limit && (S.bins[b] & 0x80000000u)) {
+            if (b < limit && (WT ? S.wbins[b] < 0.0 : (S.bins[b] & 0x80000000u) != 0u)) {
               best = b;
               break;
             }
@@ -577,6 +867,14 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
             abv += S.bins[b] & 0x7fffffffu;
           abv = uni(wave_sum_u64(abv));
           const uint64_t cur_n = total - abv;
+          // weighted: the weight at or below the top of bin `best`
+          double cur_w = 0.0;
+          if constexpr (WT) {
+            double abv_w = 0.0;
+            for (uint32_t b = (uint32_t)best + 1 + lane; b < nb; b += 64)
+              abv_w += fabs(S.wbins[b]);
+            cur_w = total_w - unid(wave_sum_f64(abv_w));
+          }
           // window [best - W + 1, best]: as many bins as fit the buffer
           uint32_t wcnt = 0;
           if (best - lane >= 0)
@@ -604,38 +902,53 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
               seg0 = (uint32_t)(n_annulus - abv - win_total);
             } else {
               float* win = S.cache + n_annulus;
+              float* win_w = S.cache_w + n_annulus;
               uint32_t got = 0;  // uniform
               for (uint32_t k0 = 0; k0 < (uint32_t)n_annulus; k0 += 64) {
                 const uint32_t k = k0 + (uint32_t)lane;
-                float d2 = 0.0f;
+                float d2 = 0.0f, wt = 0.0f;
                 bool take = false;
                 if (k < (uint32_t)n_annulus) {
                   d2 = S.cache[k];
+                  if constexpr (WT)
+                    wt = S.cache_w[k];
                   const int32_t b = (int32_t)bin_of(d2, rc2, inv_w, nb);
                   take = b >= blo && b <= best;
                 }
-                wave_append(win, win_total, got, take, d2, lane);
+                if constexpr (WT)
+                  wave_append2(win, win_w, win_total, got, take, d2, wt, lane);
+                else
+                  wave_append(win, win_total, got, take, d2, lane);
               }
               if (uni(got) != win_total) {
                 reason = R_OVERFLOW;  // counts disagree: never expected
                 break;
               }
-              wave_sort_half(win, win_total, lane);
+              if constexpr (WT)
+                wave_sort_pairs_half(win, win_w, win_total, lane);
+              else
+                wave_sort_half(win, win_total, lane);
               seg0 = (uint32_t)n_annulus;
             }
             int found = -1;
-            for (uint32_t k = lane; k < win_total; k += 64) {
-              const float d2 = S.cache[seg0 + k];
-              if (d2 == 0.0f)
-                continue;
-              const uint64_t n_in = cur_n - win_total + k;
-              const float rn = (float)n_in;
-              const float lhs = __fmul_rn(rn, rn);
-              const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
-              if (lhs <= rhs)
-                found = (int)k;
+            if constexpr (WT) {
+              double unused;
+              found = weighted_walk(S.cache + seg0, S.cache_w + seg0, win_total, cur_w, K2,
+                lane, &unused);
+            } else {
+              for (uint32_t k = lane; k < win_total; k += 64) {
+                const float d2 = S.cache[seg0 + k];
+                if (d2 == 0.0f)
+                  continue;
+                const uint64_t n_in = cur_n - win_total + k;
+                const float rn = (float)n_in;
+                const float lhs = __fmul_rn(rn, rn);
+                const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
+                if (lhs <= rhs)
+                  found = (int)k;
+              }
+              found = uni(wave_max_i32(found));
             }
-            found = uni(wave_max_i32(found));
             if (found >= 0) {
               radius = __builtin_sqrtf(unif(S.cache[seg0 + found]));
               n_in_out = cur_n - win_total + (uint64_t)found;
@@ -653,12 +966,12 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
           if (split) {
             S.sub[lane] = 0;
             wave_sync();
-            for_particles(P, q, S, lane,
+            for_particles<false>(P, q, S, lane,
               [&](uint32_t type, float2 bd) {
                 return type != CELL_CORE && !(bd.y < lo) && !(bd.x > hi);
               },
-              [](uint32_t, uint32_t) { return false; },
-              [&](float d2, uint32_t type, bool valid) {
+              [](uint32_t, uint32_t, uint64_t) { return false; },
+              [&](float d2, float, uint32_t type, bool valid) {
                 if (!valid || (type != CELL_SHELL && !(d2 > rc2 && d2 <= rs2)))
                   return;
                 const uint32_t b = bin_of(d2, rc2, inv_w, nb);
@@ -673,6 +986,7 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
           }
 
           uint64_t top_n = cur_n;  // particles at or below the top of the range
+          double top_w = cur_w;    // their weight, when weighted
           bool exhausted = false;
           for (int s = n_sub - 1; s >= 0 && radius < 0.0f && !exhausted; s--) {
             const uint32_t expect_cnt = uni(split ? __shfl(sub_count_mine, s, 64) : win_total);
@@ -683,12 +997,12 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
               break;
             }
             uint32_t collected = 0;  // uniform
-            for_particles(P, q, S, lane,
+            for_particles<WT>(P, q, S, lane,
               [&](uint32_t type, float2 bd) {
                 return type != CELL_CORE && !(bd.y < lo) && !(bd.x > hi);
               },
-              [](uint32_t, uint32_t) { return false; },
-              [&](float d2, uint32_t type, bool valid) {
+              [](uint32_t, uint32_t, uint64_t) { return false; },
+              [&](float d2, float wt, uint32_t type, bool valid) {
                 bool take = valid && (type == CELL_SHELL || (d2 > rc2 && d2 <= rs2));
                 if (take) {
                   const uint32_t b = bin_of(d2, rc2, inv_w, nb);
@@ -699,7 +1013,11 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
                   sb = sb < 0 ? 0 : (sb >= SUB_BINS ? SUB_BINS - 1 : sb);
                   take = sb == s;
                 }
-                wave_append(S.buf, (uint32_t)REFINE_CAP, collected, take, d2, lane);
+                if constexpr (WT)
+                  wave_append2(S.buf, S.buf_w, (uint32_t)REFINE_CAP, collected, take, d2, wt,
+                    lane);
+                else
+                  wave_append(S.buf, (uint32_t)REFINE_CAP, collected, take, d2, lane);
               });
             wave_sync();
             const uint32_t got = uni(collected);
@@ -707,21 +1025,29 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
               reason = R_OVERFLOW;  // counts disagree: never expected
               break;
             }
-            wave_sort(S.buf, got, lane);
-            // outside in: element k encloses top_n - got + k particles
             int found = -1;
-            for (uint32_t k = lane; k < got; k += 64) {
-              const float d2 = S.buf[k];
-              if (d2 == 0.0f)
-                continue;
-              const uint64_t n_in = top_n - got + k;
-              const float rn = (float)n_in;
-              const float lhs = __fmul_rn(rn, rn);
-              const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
-              if (lhs <= rhs)
-                found = (int)k;
+            if constexpr (WT) {
+              wave_sort_pairs(S.buf, S.buf_w, got, lane);
+              double range_w = 0.0;
+              found = weighted_walk(S.buf, S.buf_w, got, top_w, K2, lane, &range_w);
+              if (found < 0)
+                top_w -= range_w;
+            } else {
+              wave_sort(S.buf, got, lane);
+              // outside in: element k encloses top_n - got + k particles
+              for (uint32_t k = lane; k < got; k += 64) {
+                const float d2 = S.buf[k];
+                if (d2 == 0.0f)
+                  continue;
+                const uint64_t n_in = top_n - got + k;
+                const float rn = (float)n_in;
+                const float lhs = __fmul_rn(rn, rn);
+                const float rhs = __fmul_rn(K2, __fmul_rn(__fmul_rn(d2, d2), d2));
+                if (lhs <= rhs)
+                  found = (int)k;
+              }
+              found = uni(wave_max_i32(found));
             }
-            found = uni(wave_max_i32(found));
             if (found >= 0) {
               const float d2 = unif(S.buf[found]);
               // correctly rounded: HIP's __fsqrt_rn is the 1-ulp native

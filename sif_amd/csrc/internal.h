@@ -65,11 +65,18 @@ struct VoidIndex {
   float max_r;                 // largest radius in the index
   float class_max_r[VOID_CLASSES];  // < 0 for an empty class
 };
+// A weighted mesh's tracer and cell weights (device) and its mean weight
+// density, (4/3) pi total_weight / box^3 (exodus_finder.c:1639-1642).
+struct MeshWeights {
+  const float* w = nullptr;
+  const float* cell = nullptr;
+  float mass_factor = 0.0f;
+};
 int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
   const float* cz, uint32_t n, float rmin, float r_search, float threshold,
   float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
-  uint64_t* algo_bytes = nullptr);
+  uint64_t* algo_bytes = nullptr, const MeshWeights* weights = nullptr);
 int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
   float* cx, float* cy, float* cz);
 // keep[k] and the refusal counts (masked, proxy, overlap) added to d_counts.
@@ -104,6 +111,11 @@ struct RunSpec {
   float box;
   const uint64_t* d_offsets;
   const float *d_mx, *d_my, *d_mz;
+  // a weighted mesh: tracer and cell weights (device) and the summed weight;
+  // d_mw NULL on an unweighted one
+  const float* d_mw = nullptr;
+  const float* d_cw = nullptr;
+  double total_weight = 0.0;
   uint32_t mesh_n;
   uint64_t n_particles;
   std::vector<float> radii;  // descending
@@ -124,5 +136,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
 int sorted_ladder(const sif_real* radii, uint32_t n_radii, std::vector<float>& out);
 int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real f);
 int check_options(sif_option opt);
+// exodus_finder.c:1051-1106: a weighted mesh the rescaling can use -- a
+// cell table, every weight finite and non-negative (by its bits), a
+// positive total. The host form reads host weights, the device form device
+// ones.
+int check_weights_host(const float* w, uint64_t n, const float* cell_w, double total);
+int check_weights_device(const float* d_w, uint64_t n, const float* d_cell_w, double total);
 
 }  // namespace sif_amd

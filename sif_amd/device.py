@@ -100,32 +100,49 @@ def scan_candidates(ddelta, total, threshold, dmask=None, capacity=None):
     return d_delta.get()[:k], d_flat.get()[:k]
 
 
-def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor):
-    """Grows spheres at (k, 3) float32 centres; returns radius, reason, n_in."""
+def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, mass_factor=None):
+    """Grows spheres at (k, 3) float32 centres; returns radius, reason, n_in.
+    On a weighted mesh the density tests use mass_factor."""
     c = np.ascontiguousarray(centres, dtype=np.float32)
     k = c.shape[0]
     dcx, dcy, dcz = (to_device(c[:, a]) for a in range(3))
     dr = DeviceArray((max(k, 1),), np.float32)
     drs = DeviceArray((max(k, 1),), np.uint32)
     dn = DeviceArray((max(k, 1),), np.uint64)
-    check(lib.sif_amd_find_radius(mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
-                                  mesh.n_cells, mesh.box, dcx.ptr, dcy.ptr, dcz.ptr, k,
-                                  rmin, r_search, threshold, vol_factor, dr.ptr, drs.ptr,
-                                  dn.ptr))
+    if mesh.w is not None:
+        if mass_factor is None:
+            raise ValueError("a weighted mesh needs its mass_factor")
+        check(lib.sif_amd_find_radius_weighted(
+            mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr, mesh.w.ptr,
+            mesh.cell_weights.ptr, mesh.n_cells, mesh.box, dcx.ptr, dcy.ptr, dcz.ptr, k, rmin,
+            r_search, threshold, vol_factor, mass_factor, dr.ptr, drs.ptr, dn.ptr))
+    else:
+        check(lib.sif_amd_find_radius(mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+                                      mesh.n_cells, mesh.box, dcx.ptr, dcy.ptr, dcz.ptr, k,
+                                      rmin, r_search, threshold, vol_factor, dr.ptr, drs.ptr,
+                                      dn.ptr))
     return dr.get()[:k], drs.get()[:k], dn.get()[:k]
 
 
 def exodus(ddelta, n_cells, box, mesh, radii, threshold, overlap=0.0, options=0,
            restore=True):
-    """The exodus finder on device data: (catalogue (n, 4), n_in, stats)."""
+    """The exodus finder on device data: (catalogue (n, 4), n_in, stats); on
+    the mesh's weights when it carries them."""
     radii = np.ascontiguousarray(radii, dtype=np.float32)
     nr = len(radii)
     stats = (N.RungStats * nr)()
     nin = C.POINTER(C.c_uint64)()
-    cat = lib.sif_amd_exodus_device(
-        ddelta.ptr, n_cells, box, mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
-        mesh.n_cells, mesh.n, radii.ctypes.data_as(C.POINTER(C.c_float)), nr, threshold,
-        overlap, options, 1 if restore else 0, C.byref(nin), stats)
+    rp = radii.ctypes.data_as(C.POINTER(C.c_float))
+    if mesh.w is not None:
+        cat = lib.sif_amd_exodus_device_weighted(
+            ddelta.ptr, n_cells, box, mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+            mesh.w.ptr, mesh.cell_weights.ptr, mesh.total_weight, mesh.n_cells, mesh.n, rp,
+            nr, threshold, overlap, options, 1 if restore else 0, C.byref(nin), stats)
+    else:
+        cat = lib.sif_amd_exodus_device(
+            ddelta.ptr, n_cells, box, mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+            mesh.n_cells, mesh.n, rp, nr, threshold, overlap, options, 1 if restore else 0,
+            C.byref(nin), stats)
     if not cat:
         raise N.error_for(lib.sif_error_status())
     out = N.catalogue_to_numpy(cat)

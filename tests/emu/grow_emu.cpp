@@ -44,9 +44,11 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
   const int32_t* tdz, const uint8_t* ttype, const float* tmin, const float* tmax,
   uint32_t tcount, float rmin, float r_search, float threshold, float vol_factor,
   const float* cx, const float* cy, const float* cz, uint32_t n, float* radius,
-  uint32_t* reason, uint64_t* n_in, uint32_t cache_words) {
+  uint32_t* reason, uint64_t* n_in, uint32_t cache_words, const float* mw,
+  const float* cw, float mass_factor) {
   // cache_words: the annulus cache a wave holds; 0 sizes it as the library
-  // does for the launch
+  // does for the launch. mw non-NULL runs the weighted kernel (its cache is
+  // always REFINE_CAP pairs).
   const uint32_t blocks = (n + sif_amd::GROW_WAVES - 1) / sif_amd::GROW_WAVES;
   std::vector<int4> cells(tcount);
   std::vector<float2> bounds(tcount);
@@ -77,9 +79,18 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
   P.radius = radius;
   P.reason = reason;
   P.n_in = n_in;
-  sif_amd::grow_lds_words(sif_amd::grow_bin_count(r_search, rmin, vol_factor), cache_words,
-    &P.lds_bins, &P.lds_cache);
-  g_lds.assign((size_t)sif_amd::GROW_WAVES * sif_amd::grow_wave_words(P.lds_bins, P.lds_cache),
+  const bool weighted = mw != nullptr;
+  P.mw = mw;
+  P.cw = cw;
+  P.mass_factor = weighted ? mass_factor : vol_factor;
+  const uint32_t nb = sif_amd::grow_bin_count(r_search, rmin, vol_factor);
+  if (weighted)
+    sif_amd::grow_lds_words_weighted(nb, &P.lds_bins, &P.lds_cache);
+  else
+    sif_amd::grow_lds_words(nb, cache_words, &P.lds_bins, &P.lds_cache);
+  // whole words, 8-byte aligned for the weighted bins
+  g_lds.assign((size_t)sif_amd::GROW_WAVES *
+                   sif_amd::grow_wave_words(P.lds_bins, P.lds_cache, weighted) + 2,
     0xdeadbeefu);
   blockDim.x = 64 * sif_amd::GROW_WAVES;
   gridDim.x = blocks;
@@ -89,7 +100,10 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
       th.emplace_back([&, t, b] {
         threadIdx.x = t;
         blockIdx.x = b;
-        sif_amd::grow_kernel(P);
+        if (weighted)
+          sif_amd::grow_kernel<true>(P);
+        else
+          sif_amd::grow_kernel<false>(P);
       });
     for (auto& x : th)
       x.join();

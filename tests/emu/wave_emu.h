@@ -115,6 +115,17 @@ inline uint32_t atomicAdd(uint32_t* p, uint32_t v) {
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
   return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
 }
+// The emulated lanes are threads: the order of their additions to one bin is
+// the scheduler's, so a sum is order-independent only where it is exact.
+inline double atomicAdd(double* p, double v) {
+  double old, want;
+  __atomic_load(p, &old, __ATOMIC_SEQ_CST);
+  do {
+    want = old + v;
+  } while (!__atomic_compare_exchange(p, &old, &want, false, __ATOMIC_SEQ_CST,
+    __ATOMIC_SEQ_CST));
+  return old;
+}
 
 inline float __fadd_rn(float a, float b) { return a + b; }
 inline float __fsub_rn(float a, float b) { return a - b; }
@@ -123,3 +134,5 @@ inline float __fdiv_rn(float a, float b) { return a / b; }
 inline float __fsqrt_rn(float a) { return std::sqrt(a); }
 inline float __int_as_float(int b) { return emu_from<float>((uint32_t)b); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline unsigned __float_as_uint(float v) { return emu_from<unsigned>(emu_bits(v)); }
+inline float __uint_as_float(unsigned b) { return emu_from<float>(b); }
