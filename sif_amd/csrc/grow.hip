@@ -317,6 +317,15 @@ struct TemplateCache {
 };
 TemplateCache g_tpl;
 DeviceBuffer g_pf_counts;
+// the split first pass's partial histograms and counts
+DeviceBuffer g_part_bins, g_part_counts;
+
+// A launch of fewer candidates than the chip holds waves (256 CUs, 16 waves
+// each at this kernel's registers and LDS) gives each candidate's first pass
+// to several waves, up to this many, so that one large candidate does not
+// hold the launch for the time a single wave takes over its sphere.
+constexpr uint32_t WAVE_SLOTS = 4096;
+constexpr uint32_t MAX_SLICES = 16;
 }  // namespace
 
 static int template_device(uint32_t mesh_n, float l, float rmin, float r_search) {
@@ -376,6 +385,9 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   P.cy = cy;
   P.cz = cz;
   P.n_cand = n;
+  P.slices = 1;
+  P.part_bins = nullptr;
+  P.part_counts = nullptr;
   P.mw = weighted ? weights->w : nullptr;
   P.cw = weighted ? weights->cell : nullptr;
   P.mass_factor = weighted ? weights->mass_factor : vol_factor;
@@ -390,7 +402,23 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
     grow_lds_words(nb, 0, &P.lds_bins, &P.lds_cache);
   const size_t lds_bytes =
     (size_t)GROW_WAVES * grow_wave_words(P.lds_bins, P.lds_cache, weighted) * sizeof(uint32_t);
+  if (!weighted && n < WAVE_SLOTS / 2) {
+    const uint32_t slices = std::min<uint32_t>(MAX_SLICES, WAVE_SLOTS / n);
+    int st2 = g_part_bins.reserve((size_t)n * slices * nb * sizeof(uint32_t));
+    if (st2 == SIF_OK)
+      st2 = g_part_counts.reserve((size_t)n * slices * PART_COUNTS * sizeof(uint64_t));
+    if (st2 != SIF_OK)
+      return st2;
+    P.slices = slices;
+    P.part_bins = g_part_bins.as<uint32_t>();
+    P.part_counts = g_part_counts.as<uint64_t>();
+  }
   timing_begin(SIF_AMD_STAGE_GROW);
+  if (P.slices > 1) {
+    const size_t bin_lds = (size_t)(P.lds_bins + LDS_TAIL_WORDS) * sizeof(uint32_t);
+    grow_bin_kernel<<<n * P.slices, 64, bin_lds, stream()>>>(P);
+    SIF_HIP(hipGetLastError());
+  }
   const uint64_t blocks = ((uint64_t)n + GROW_WAVES - 1) / GROW_WAVES;
   if (weighted)
     grow_kernel<true><<<(unsigned)blocks, 64 * GROW_WAVES, lds_bytes, stream()>>>(P);

@@ -81,6 +81,13 @@ struct GrowParams {
   // entries of its annulus cache (at least REFINE_CAP: the window buffer
   // shares it; a word an entry, two when weighted: distance and weight)
   uint32_t lds_bins, lds_cache;
+  // The first pass split over `slices` waves a candidate (1: not split):
+  // grow_bin_kernel leaves each wave's histogram (nb words) in part_bins and
+  // its counts (PART_COUNTS words) in part_counts, slice-major per candidate;
+  // grow_kernel then starts from their sums.
+  uint32_t slices;
+  uint32_t* part_bins;
+  uint64_t* part_counts;
   // optional: the candidate's algorithmic bytes -- 8 per template cell whose
   // offsets it reads, 12 per particle of the cells whose particles it must
   // read once (the surface for the early refusal, every non-CORE cell past it)
@@ -279,10 +286,13 @@ constexpr int PARTICLE_UNROLL = 8;
 // when WT (1 otherwise). visit is called by every lane the same number of
 // times -- `valid` false on a lane with no particle that round -- so a
 // visitor may use wave-wide operations.
+// The template is read in chunks of 64 entries, from chunk `first`, every
+// `stride`-th chunk.
 template <bool WT, class Want, class Whole, class Visit>
 __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& q,
-  WaveLDS& S, int lane, Want want, Whole whole, Visit visit) {
-  for (uint32_t base = 0; base < P.tcount; base += 64) {
+  WaveLDS& S, int lane, Want want, Whole whole, Visit visit, uint32_t first = 0,
+  uint32_t stride = 1) {
+  for (uint32_t base = first * 64; base < P.tcount; base += stride * 64) {
     const uint32_t e = base + lane;
     uint32_t cnt = 0, start = 0, type = 0;
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
@@ -587,6 +597,106 @@ __device__ __forceinline__ int weighted_walk(const float* key, const float* val,
   return found;
 }
 
+enum { PC_WITHIN = 0, PC_SURFACE, PC_SHELL, PC_CORE, PC_ANNULUS, PART_COUNTS };
+
+// exodus_rescale.c:711-865 for one slice of a candidate's template: the
+// chunks s, s + slices, ... of candidate c = block / slices, s = block %
+// slices. The same counts and histogram as grow_kernel's first pass over
+// those chunks (unweighted, no annulus cache), written out for grow_kernel
+// to sum: integers, so the sums do not depend on how the chunks are shared.
+// LDS: lds_bins words of histogram, then the chunk tables.
+__global__ void __launch_bounds__(64, 4) grow_bin_kernel(GrowParams P) {
+  GROW_DYNAMIC_LDS(lds_words);
+  const int lane = threadIdx.x & 63;
+  WaveLDS S;
+  S.bins = lds_words;
+  S.wbins = nullptr;
+  S.cache = S.cache_w = S.buf = S.buf_w = nullptr;
+  uint32_t* tail = lds_words + P.lds_bins;
+  S.t_start = tail;
+  S.t_scan = tail + 64;
+  S.t_ex = reinterpret_cast<float*>(tail + 128);
+  S.t_ey = reinterpret_cast<float*>(tail + 192);
+  S.t_ez = reinterpret_cast<float*>(tail + 256);
+  S.t_type = tail + 320;
+  S.sub = tail + 384;
+
+  const uint32_t slices = P.slices;
+  const uint32_t c = uni((uint32_t)blockIdx.x / slices);
+  const uint32_t sl = uni((uint32_t)blockIdx.x - c * slices);
+  if (c >= P.n_cand)
+    return;
+  const float rs2 = __fmul_rn(P.r_search, P.r_search);
+  const float rc2 = __fmul_rn(P.rmin, P.rmin);
+  const float span = __fsub_rn(rs2, rc2);
+  const uint32_t nb = grow_bin_count(P.r_search, P.rmin, P.vol_factor);
+  const float inv_w = __fdiv_rn((float)nb, span);
+  Query q;
+  q.cx = P.cx[c];
+  q.cy = P.cy[c];
+  q.cz = P.cz[c];
+  q.ix = (int32_t)__fmul_rn(q.cx, P.inv_l);
+  q.iy = (int32_t)__fmul_rn(q.cy, P.inv_l);
+  q.iz = (int32_t)__fmul_rn(q.cz, P.inv_l);
+  uint32_t* out_bins = P.part_bins + ((uint64_t)c * slices + sl) * nb;
+  uint64_t* out_counts = P.part_counts + ((uint64_t)c * slices + sl) * PART_COUNTS;
+  // grow_kernel refuses these before it reads the partial results
+  const bool inside = q.cx >= 0.0f && q.cx <= P.box && q.cy >= 0.0f &&
+                      q.cy <= P.box && q.cz >= 0.0f && q.cz <= P.box &&
+                      q.ix <= P.mesh_n && q.iy <= P.mesh_n && q.iz <= P.mesh_n;
+  if (!inside || !(span > 0.0f) || nb > P.lds_bins)
+    return;
+  for (uint32_t b = lane; b < nb; b += 64)
+    S.bins[b] = 0;
+  wave_sync();
+  uint64_t within = 0, surface = 0, shell = 0, core = 0, annulus_n = 0;
+  for_particles<false>(P, q, S, lane,
+    [](uint32_t, float2) { return true; },
+    [&](uint32_t type, uint32_t cnt, uint64_t) {
+      if (type == CELL_CORE) {
+        within += cnt;
+        core += cnt;
+        return true;
+      }
+      return false;
+    },
+    [&](float d2, float, uint32_t type, bool valid) {
+      if (!valid)
+        return;
+      bool annulus;
+      if (type == CELL_SHELL) {
+        within++;
+        shell++;
+        annulus = true;
+      } else {
+        within += d2 <= rs2 ? 1u : 0u;
+        surface++;
+        core += d2 <= rc2 ? 1u : 0u;
+        annulus = d2 > rc2 && d2 <= rs2;
+      }
+      if (annulus) {
+        atomicAdd(&S.bins[bin_of(d2, rc2, inv_w, nb)], 1u);
+        annulus_n++;
+      }
+    },
+    sl, slices);
+  wave_sync();
+  within = wave_sum_u64(within);
+  surface = wave_sum_u64(surface);
+  shell = wave_sum_u64(shell);
+  core = wave_sum_u64(core);
+  annulus_n = wave_sum_u64(annulus_n);
+  for (uint32_t b = lane; b < nb; b += 64)
+    out_bins[b] = S.bins[b];
+  if (lane == 0) {
+    out_counts[PC_WITHIN] = within;
+    out_counts[PC_SURFACE] = surface;
+    out_counts[PC_SHELL] = shell;
+    out_counts[PC_CORE] = core;
+    out_counts[PC_ANNULUS] = annulus_n;
+  }
+}
+
 template <bool WT>
 __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) {
   GROW_DYNAMIC_LDS(lds_words);
@@ -665,81 +775,105 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
       // annulus distance in LDS while they fit, and builds the histogram of
       // the annulus (exodus_rescale.c:793-865, which reads the same
       // particles again) as it goes.
-      for (uint32_t b = lane; b < nb; b += 64) {
-        S.bins[b] = 0;
-        if constexpr (WT)
-          S.wbins[b] = 0.0;
-      }
-      wave_sync();
       uint64_t within = 0, surface = 0, shell = 0, core = 0, whole_cells = 0;
+      // the first pass done by grow_bin_kernel: its slices' sums
+      const bool binned = !WT && P.slices > 1;
+      if (binned) {
+        const uint64_t first = (uint64_t)c * P.slices;
+        for (uint32_t b = lane; b < nb; b += 64) {
+          uint32_t v = 0;
+          for (uint32_t s = 0; s < P.slices; s++)
+            v += P.part_bins[(first + s) * nb + b];
+          S.bins[b] = v;
+        }
+        uint64_t mine = 0;  // lane k < PART_COUNTS: count k over the slices
+        if (lane < PART_COUNTS)
+          for (uint32_t s = 0; s < P.slices; s++)
+            mine += P.part_counts[(first + s) * PART_COUNTS + lane];
+        within = uni(__shfl(mine, PC_WITHIN, 64));
+        surface = uni(__shfl(mine, PC_SURFACE, 64));
+        shell = uni(__shfl(mine, PC_SHELL, 64));
+        core = uni(__shfl(mine, PC_CORE, 64));
+        n_annulus = uni(__shfl(mine, PC_ANNULUS, 64));
+        wave_sync();
+      } else {
+        for (uint32_t b = lane; b < nb; b += 64) {
+          S.bins[b] = 0;
+          if constexpr (WT)
+            S.wbins[b] = 0.0;
+        }
+        wave_sync();
+      }
       // weighted (exodus_rescale.c:745-865): the early refusal's weight takes
       // whole CORE and SHELL cells from the cell table; the histogram's core
       // and annulus weights are the tracers' own, but for whole CORE cells.
       // Each lane sums its own share in double, in a fixed order.
       double within_w = 0.0, core_w = 0.0, shell_w = 0.0;
       uint32_t appended = 0;  // uniform
-      for_particles<WT>(P, q, S, lane,
-        [](uint32_t, float2) { return true; },
-        [&](uint32_t type, uint32_t cnt, uint64_t cell) {
-          if constexpr (WT) {
-            if (type == CELL_CORE || type == CELL_SHELL) {
-              const double c = (double)P.cw[cell];
-              within_w += c;
-              if (type == CELL_CORE)
-                core_w += c;
-              whole_cells++;
-            }
-          }
-          if (type == CELL_CORE) {
-            within += cnt;
-            core += cnt;
-            return true;
-          }
-          return false;
-        },
-        [&](float d2, float wt, uint32_t type, bool valid) {
-          bool annulus = false;
-          if (valid) {
-            if (type == CELL_SHELL) {
-              within++;
-              shell++;
-              annulus = true;
-              if constexpr (WT)
-                shell_w += (double)wt;
-            } else {
-              within += d2 <= rs2 ? 1u : 0u;
-              surface++;
-              core += d2 <= rc2 ? 1u : 0u;
-              annulus = d2 > rc2 && d2 <= rs2;
-              if constexpr (WT) {
-                if (d2 <= rs2)
-                  within_w += (double)wt;
-                if (d2 <= rc2)
-                  core_w += (double)wt;
-                else if (annulus)
-                  shell_w += (double)wt;
+      if (!binned) {
+        for_particles<WT>(P, q, S, lane,
+          [](uint32_t, float2) { return true; },
+          [&](uint32_t type, uint32_t cnt, uint64_t cell) {
+            if constexpr (WT) {
+              if (type == CELL_CORE || type == CELL_SHELL) {
+                const double c = (double)P.cw[cell];
+                within_w += c;
+                if (type == CELL_CORE)
+                  core_w += c;
+                whole_cells++;
               }
             }
-            if (annulus) {
-              const uint32_t b = bin_of(d2, rc2, inv_w, nb);
-              atomicAdd(&S.bins[b], 1u);
-              // one wave owns these bins, so the order of the additions is
-              // the wave's own program and lane order
-              if constexpr (WT)
-                atomicAdd(&S.wbins[b], (double)wt);
+            if (type == CELL_CORE) {
+              within += cnt;
+              core += cnt;
+              return true;
             }
-          }
-          if constexpr (WT)
-            wave_append2(S.cache, S.cache_w, P.lds_cache, appended, annulus, d2, wt, lane);
-          else
-            wave_append(S.cache, P.lds_cache, appended, annulus, d2, lane);
-        });
-      wave_sync();
-      within = uni(wave_sum_u64(within));
-      surface = uni(wave_sum_u64(surface));
-      shell = uni(wave_sum_u64(shell));
-      core = uni(wave_sum_u64(core));
-      n_annulus = uni(appended);
+            return false;
+          },
+          [&](float d2, float wt, uint32_t type, bool valid) {
+            bool annulus = false;
+            if (valid) {
+              if (type == CELL_SHELL) {
+                within++;
+                shell++;
+                annulus = true;
+                if constexpr (WT)
+                  shell_w += (double)wt;
+              } else {
+                within += d2 <= rs2 ? 1u : 0u;
+                surface++;
+                core += d2 <= rc2 ? 1u : 0u;
+                annulus = d2 > rc2 && d2 <= rs2;
+                if constexpr (WT) {
+                  if (d2 <= rs2)
+                    within_w += (double)wt;
+                  if (d2 <= rc2)
+                    core_w += (double)wt;
+                  else if (annulus)
+                    shell_w += (double)wt;
+                }
+              }
+              if (annulus) {
+                const uint32_t b = bin_of(d2, rc2, inv_w, nb);
+                atomicAdd(&S.bins[b], 1u);
+                // one wave owns these bins, so the order of the additions is
+                // the wave's own program and lane order
+                if constexpr (WT)
+                  atomicAdd(&S.wbins[b], (double)wt);
+              }
+            }
+            if constexpr (WT)
+              wave_append2(S.cache, S.cache_w, P.lds_cache, appended, annulus, d2, wt, lane);
+            else
+              wave_append(S.cache, P.lds_cache, appended, annulus, d2, lane);
+          });
+        wave_sync();
+        within = uni(wave_sum_u64(within));
+        surface = uni(wave_sum_u64(surface));
+        shell = uni(wave_sum_u64(shell));
+        core = uni(wave_sum_u64(core));
+        n_annulus = uni(appended);
+      }
       if constexpr (WT) {
         within_w = unid(wave_sum_f64(within_w));
         // exodus_rescale.c:864: w_core + w_shell
@@ -761,7 +895,8 @@ __global__ void __launch_bounds__(64 * GROW_WAVES, 4) grow_kernel(GrowParams P) 
         }
         algo = 8ull * P.tcount + 12ull * (surface + (reason == R_OK ? shell : 0ull));
       }
-      cached = n_annulus <= P.lds_cache;
+      // nothing cached when the first pass was split
+      cached = !binned && n_annulus <= P.lds_cache;
       n_core = core;
     }
 

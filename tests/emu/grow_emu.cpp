@@ -45,7 +45,7 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
   uint32_t tcount, float rmin, float r_search, float threshold, float vol_factor,
   const float* cx, const float* cy, const float* cz, uint32_t n, float* radius,
   uint32_t* reason, uint64_t* n_in, uint32_t cache_words, const float* mw,
-  const float* cw, float mass_factor) {
+  const float* cw, float mass_factor, uint32_t slices) {
   // cache_words: the annulus cache a wave holds; 0 sizes it as the library
   // does for the launch. mw non-NULL runs the weighted kernel (its cache is
   // always REFINE_CAP pairs).
@@ -88,10 +88,39 @@ extern "C" int emu_find_radius(const uint64_t* offsets, const float* mx, const f
     sif_amd::grow_lds_words_weighted(nb, &P.lds_bins, &P.lds_cache);
   else
     sif_amd::grow_lds_words(nb, cache_words, &P.lds_bins, &P.lds_cache);
+  // slices > 1 (unweighted): the first pass split over that many waves a
+  // candidate, by grow_bin_kernel, before grow_kernel
+  std::vector<uint32_t> part_bins;
+  std::vector<uint64_t> part_counts;
+  P.slices = 1;
+  P.part_bins = nullptr;
+  P.part_counts = nullptr;
+  if (slices > 1 && !weighted) {
+    part_bins.assign((size_t)n * slices * nb, 0xdeadbeefu);
+    part_counts.assign((size_t)n * slices * sif_amd::PART_COUNTS, 0xdeadbeefdeadbeefull);
+    P.slices = slices;
+    P.part_bins = part_bins.data();
+    P.part_counts = part_counts.data();
+  }
   // whole words, 8-byte aligned for the weighted bins
   g_lds.assign((size_t)sif_amd::GROW_WAVES *
                    sif_amd::grow_wave_words(P.lds_bins, P.lds_cache, weighted) + 2,
     0xdeadbeefu);
+  if (P.slices > 1) {
+    blockDim.x = 64;
+    gridDim.x = n * P.slices;
+    for (uint32_t b = 0; b < n * P.slices; b++) {
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < 64; t++)
+        th.emplace_back([&, t, b] {
+          threadIdx.x = t;
+          blockIdx.x = b;
+          sif_amd::grow_bin_kernel(P);
+        });
+      for (auto& x : th)
+        x.join();
+    }
+  }
   blockDim.x = 64 * sif_amd::GROW_WAVES;
   gridDim.x = blocks;
   for (uint32_t b = 0; b < blocks; b++) {

@@ -263,6 +263,27 @@ def test_find_radius_bit_exact_and_brute_force(dev, particles, rung, factor):
         assert want > 0 and abs(gr[i] - want) <= 1e-5 * want
 
 
+def test_find_radius_split_first_pass(dev, particles):
+    """A launch of fewer candidates than the chip holds waves shares each
+    candidate's first pass among several waves; a large launch does not.
+    The same centres through both give the same bits, and the oracle's."""
+    import oracle
+    D = dev[0]
+    x, y, z = particles
+    m, om = _mesh_pair(dev, particles, MESH_CELLS)
+    centres = np.random.default_rng(23).uniform(0, BOX, (2400, 3)).astype(np.float32)
+    vol = (np.float32(4.0) / np.float32(3.0)) * np.float32(3.14159265358979323846) * \
+        (np.float32(len(x)) / (np.float32(BOX) ** 3))
+    for rung, rs in ((6.0, 9.375), (9.0, 18.0)):
+        whole = D.find_radius(m, centres, rung, rs, THRESH, vol)        # one wave each
+        split = D.find_radius(m, centres[:150], rung, rs, THRESH, vol)  # 16 waves each
+        for a, b in zip(whole, split):
+            np.testing.assert_array_equal(a[:150], b)
+        orr, ore, _ = oracle.find_radius(om, centres[:150], rung, rs, THRESH, vol)
+        np.testing.assert_array_equal(split[0].view(np.uint32), orr.view(np.uint32))
+        np.testing.assert_array_equal(split[1], ore)
+
+
 def test_find_radius_window_invariance(dev, particles):
     """The oracle's radius does not depend on the resolve window (the
     reference's claim, exodus_rescale.c:384-390), so the GPU's choice of

@@ -43,7 +43,7 @@ def field():
     return x, y, z, m
 
 
-def run_emu(lib, m, tpl, centres, rung, rs, t, vol, cache_words=0, mass=None):
+def run_emu(lib, m, tpl, centres, rung, rs, t, vol, cache_words=0, mass=None, slices=1):
     k = len(centres)
     r = np.zeros(k, np.float32)
     why = np.zeros(k, np.uint32)
@@ -60,9 +60,32 @@ def run_emu(lib, m, tpl, centres, rung, rs, t, vol, cache_words=0, mass=None):
         C.c_uint32(k), V(r.ctypes.data), V(why.ctypes.data), V(nin.ctypes.data),
         C.c_uint32(cache_words), V(m["w"].ctypes.data if mass is not None else None),
         V(m["cell_weights"].ctypes.data if mass is not None else None),
-        C.c_float(mass if mass is not None else 0.0))
+        C.c_float(mass if mass is not None else 0.0), C.c_uint32(slices))
     assert st == 0
     return r, why, nin
+
+
+@pytest.mark.parametrize("slices", [2, 3])
+def test_emulated_split_first_pass_matches_oracle(emu, field, slices):
+    """The first pass shared by several waves a candidate (grow_bin_kernel,
+    every slices-th template chunk each; with 3 slices of a 2-chunk template
+    one wave has none), the walk from their summed histogram with no annulus
+    cache: the oracle's radii, reasons and counts."""
+    x, y, z, m = field
+    rung, factor, t = 9.0, 2.0, np.float32(-0.8)
+    rs = rung + max((factor - 1.0) * rung, 3.0 * BOX / 32)
+    tpl = oracle.OracleTemplate(MESH, np.float32(BOX) / np.float32(MESH), rung, rs).arrays()
+    assert 64 < len(tpl["dx"]) <= 128
+    vol = (np.float32(4) / np.float32(3)) * np.float32(3.14159265358979323846) * \
+        (np.float32(len(x)) / (np.float32(BOX) ** 3))
+    centres = np.random.default_rng(17).uniform(0, BOX, (24, 3)).astype(np.float32)
+    r, why, nin = run_emu(emu, m, tpl, centres, rung, rs, t, vol, slices=slices)
+    om = oracle.OracleMesh(m, MESH, BOX)
+    orr, owhy, onin = oracle.find_radius(om, centres, rung, rs, t, vol)
+    np.testing.assert_array_equal(why, owhy)
+    np.testing.assert_array_equal(r.view(np.uint32), orr.view(np.uint32))
+    np.testing.assert_array_equal(nin[r > 0], onin[orr > 0])
+    assert (r > 0).sum() >= 1 and len(set(why.tolist())) >= 2
 
 
 def annulus_sizes(x, y, z, centres, rung, rs):
