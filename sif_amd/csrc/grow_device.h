@@ -277,21 +277,41 @@ __device__ __forceinline__ void query_cell(const GrowParams& P, const Query& q,
   *ez = cz;
 }
 
-constexpr int PARTICLE_UNROLL = 8;
+// Steps of 64 particles a wave has in flight: their loads are issued
+// together.
+constexpr int PARTICLE_STEPS = 8;
 
-// Walks the template 64 entries at a time. `want(type, bound)` says whether
-// an entry is visited; `whole(type, count, cell)` consumes a whole cell
-// without reading its particles (returns true when it did); `visit(d2, w,
-// type, valid)` sees every particle of the remaining cells, with its weight
-// when WT (1 otherwise). visit is called by every lane the same number of
-// times -- `valid` false on a lane with no particle that round -- so a
-// visitor may use wave-wide operations.
-// The template is read in chunks of 64 entries, from chunk `first`, every
-// `stride`-th chunk.
+// A template entry's fields as lane `l` holds them (l uniform), uniform over
+// the wave. The CPU emulator of the kernel supplies its own, one rendezvous
+// for the six.
+#ifndef GROW_ENTRY_FIELDS
+__device__ __forceinline__ void entry_fields(int l, uint32_t cnt, uint32_t start,
+  uint32_t type, float ex, float ey, float ez, uint32_t* o_cnt, uint32_t* o_start,
+  uint32_t* o_type, float* o_x, float* o_y, float* o_z) {
+  *o_cnt = (uint32_t)__builtin_amdgcn_readlane((int)cnt, l);
+  *o_start = (uint32_t)__builtin_amdgcn_readlane((int)start, l);
+  *o_type = (uint32_t)__builtin_amdgcn_readlane((int)type, l);
+  *o_x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex), l));
+  *o_y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey), l));
+  *o_z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ez), l));
+}
+#endif
+
+// Walks the template 64 entries at a time, from chunk `first`, every
+// `stride`-th chunk. `want(type, bound)` says whether an entry is visited;
+// `whole(type, count, cell)` consumes a whole cell without reading its
+// particles (returns true when it did); `visit(d2, w, type, valid)` sees
+// every particle of the remaining cells, with its weight when WT (1
+// otherwise). The wave takes one cell at a time, 64 of its particles a step
+// (lane k the step's k-th; lanes past the cell's end have valid false), with
+// the cell's start, image of the centre and type uniform over the wave.
+// visit is called by every lane the same number of times, with `type`
+// uniform, so a visitor may use wave-wide operations and branch on type.
 template <bool WT, class Want, class Whole, class Visit>
 __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& q,
   WaveLDS& S, int lane, Want want, Whole whole, Visit visit, uint32_t first = 0,
   uint32_t stride = 1) {
+  (void)S;
   for (uint32_t base = first * 64; base < P.tcount; base += stride * 64) {
     const uint32_t e = base + lane;
     uint32_t cnt = 0, start = 0, type = 0;
@@ -306,57 +326,74 @@ __device__ __forceinline__ void for_particles(const GrowParams& P, const Query& 
           cnt = 0;
       }
     }
-    const uint32_t incl = wave_incl_scan_u32(cnt, lane);
-    const uint32_t total = uni(__shfl(incl, 63, 64));
-    if (total == 0)
-      continue;
-    S.t_start[lane] = start;
-    S.t_scan[lane] = incl;
-    S.t_ex[lane] = ex;
-    S.t_ey[lane] = ey;
-    S.t_ez[lane] = ez;
-    S.t_type[lane] = type;
-    wave_sync();
-    // particle j of the chunk goes to lane j % 64; a lane's j only grows, so
-    // its entry (the first whose inclusive prefix exceeds j) is found by
-    // stepping forward from the last one. PARTICLE_UNROLL particles per lane
-    // per round, their loads issued together.
-    int ent = 0;
-    for (uint32_t j0 = 0; j0 < total; j0 += 64 * PARTICLE_UNROLL) {
-      uint32_t pi[PARTICLE_UNROLL];
-      int en[PARTICLE_UNROLL];
+    // the entries with particles to read, lowest lane first; the current
+    // one's fields are read from its lane once
+    uint64_t live = uni((uint64_t)__ballot(cnt != 0));
+    int cur = live ? __builtin_ctzll(live) : 64;  // uniform: the entry read next
+    uint32_t k = 0;                               // uniform: particles of it done
+    uint32_t c_cnt = 0, c_start = 0, c_type = 0;
+    float c_x = 0.0f, c_y = 0.0f, c_z = 0.0f;
+    auto take_entry = [&]() {
+      if (cur < 64)
+        entry_fields(cur, cnt, start, type, ex, ey, ez, &c_cnt, &c_start, &c_type, &c_x, &c_y,
+          &c_z);
+    };
+    take_entry();
+    while (cur < 64) {
+      uint32_t st_start[PARTICLE_STEPS], st_n[PARTICLE_STEPS], st_type[PARTICLE_STEPS];
+      float st_x[PARTICLE_STEPS], st_y[PARTICLE_STEPS], st_z[PARTICLE_STEPS];
+      // the next steps: (entry, first particle, particles), all uniform
 #pragma unroll
-      for (int u = 0; u < PARTICLE_UNROLL; u++) {
-        const uint32_t j = j0 + (uint32_t)u * 64 + lane;
-        while (ent < 63 && S.t_scan[ent] <= j)
-          ent++;
-        const uint32_t before = ent ? S.t_scan[ent - 1] : 0u;
-        en[u] = ent;
-        pi[u] = j < total ? S.t_start[ent] + (j - before) : 0xffffffffu;
+      for (int u = 0; u < PARTICLE_STEPS; u++) {
+        st_n[u] = 0;
+        st_start[u] = 0;
+        st_type[u] = 0;
+        st_x[u] = st_y[u] = st_z[u] = 0.0f;
+        if (cur < 64) {
+          st_start[u] = c_start + k;
+          st_n[u] = c_cnt - k < 64u ? c_cnt - k : 64u;
+          st_type[u] = c_type;
+          st_x[u] = c_x;
+          st_y[u] = c_y;
+          st_z[u] = c_z;
+          k += 64;
+          if (k >= c_cnt) {
+            live &= live - 1;
+            cur = live ? __builtin_ctzll(live) : 64;
+            k = 0;
+            take_entry();
+          }
+        }
       }
-      float px[PARTICLE_UNROLL], py[PARTICLE_UNROLL], pz[PARTICLE_UNROLL];
-      float pw[PARTICLE_UNROLL];
+      float px[PARTICLE_STEPS], py[PARTICLE_STEPS], pz[PARTICLE_STEPS];
+      float pw[PARTICLE_STEPS];
 #pragma unroll
-      for (int u = 0; u < PARTICLE_UNROLL; u++) {
-        const uint32_t p = pi[u] != 0xffffffffu ? pi[u] : 0u;
-        px[u] = P.mx[p];
-        py[u] = P.my[p];
-        pz[u] = P.mz[p];
-        if constexpr (WT)
-          pw[u] = P.mw[p];
-        else
-          pw[u] = 1.0f;
+      for (int u = 0; u < PARTICLE_STEPS; u++) {
+        // lanes past the step's end read its last particle again, unused
+        const uint32_t off = (uint32_t)lane < st_n[u] ? (uint32_t)lane : st_n[u] - 1u;
+        if (st_n[u]) {
+          // uniform base, lane offset: one address register a step
+          px[u] = (P.mx + st_start[u])[off];
+          py[u] = (P.my + st_start[u])[off];
+          pz[u] = (P.mz + st_start[u])[off];
+          if constexpr (WT)
+            pw[u] = (P.mw + st_start[u])[off];
+          else
+            pw[u] = 1.0f;
+        } else {
+          px[u] = py[u] = pz[u] = 0.0f;
+          pw[u] = 0.0f;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < PARTICLE_UNROLL; u++) {
-        const bool valid = pi[u] != 0xffffffffu;
-        const int lo = en[u];
-        const float d2 =
-          valid ? dist2(px[u], py[u], pz[u], S.t_ex[lo], S.t_ey[lo], S.t_ez[lo]) : 0.0f;
-        visit(d2, valid ? pw[u] : 0.0f, valid ? S.t_type[lo] : 0u, valid);
+      for (int u = 0; u < PARTICLE_STEPS; u++) {
+        if (st_n[u]) {
+          const bool valid = (uint32_t)lane < st_n[u];
+          const float d2 = valid ? dist2(px[u], py[u], pz[u], st_x[u], st_y[u], st_z[u]) : 0.0f;
+          visit(d2, valid ? pw[u] : 0.0f, st_type[u], valid);
+        }
       }
     }
-    wave_sync();
   }
 }
 

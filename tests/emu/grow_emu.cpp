@@ -14,6 +14,30 @@ emu_dim3 blockDim, gridDim;
 static std::vector<uint32_t> g_lds;
 #define GROW_DYNAMIC_LDS(name) uint32_t* name = g_lds.data()
 
+// a template entry's six fields from lane l: one rendezvous of the wave
+#define GROW_ENTRY_FIELDS 1
+namespace sif_amd {
+struct EntrySlot {
+  uint32_t cnt, start, type;
+  float x, y, z;
+};
+static EntrySlot g_entry_slots[64];
+inline void entry_fields(int l, uint32_t cnt, uint32_t start, uint32_t type, float ex,
+  float ey, float ez, uint32_t* o_cnt, uint32_t* o_start, uint32_t* o_type, float* o_x,
+  float* o_y, float* o_z) {
+  g_entry_slots[emu_lane()] = EntrySlot{cnt, start, type, ex, ey, ez};
+  emu_wave().barrier("entry_fields");
+  const EntrySlot e = g_entry_slots[l & 63];
+  emu_wave().barrier("entry_fields");
+  *o_cnt = e.cnt;
+  *o_start = e.start;
+  *o_type = e.type;
+  *o_x = e.x;
+  *o_y = e.y;
+  *o_z = e.z;
+}
+}  // namespace sif_amd
+
 #include "grow_device.h"
 
 static EmuWave g_waves[sif_amd::GROW_WAVES];
@@ -21,21 +45,29 @@ static std::atomic<int> g_stuck{0};
 
 EmuWave& emu_wave() { return g_waves[threadIdx.x >> 6]; }
 
+// A generation barrier the lanes spin on, yielding the core: the 64 lane
+// threads outnumber the cores, and a rendezvous is far more frequent than a
+// sleep and wake-up of every thread can afford.
 void EmuWave::barrier(const char* site) {
-  std::unique_lock<std::mutex> lk(m);
-  const uint64_t gen = generation;
-  if (++arrived == 64) {
-    arrived = 0;
-    generation++;
-    cv.notify_all();
+  std::atomic<uint64_t>& gen_ref = generation;
+  std::atomic<int>& arrived_ref = arrived;
+  const uint64_t gen = gen_ref.load(std::memory_order_acquire);
+  if (arrived_ref.fetch_add(1, std::memory_order_acq_rel) == 63) {
+    arrived_ref.store(0, std::memory_order_relaxed);
+    gen_ref.store(gen + 1, std::memory_order_release);
     return;
   }
-  if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return generation != gen; })) {
-    std::fprintf(stderr,
-      "wave_emu: lanes of wave %u, block %u did not all reach %s (%d of 64 arrived)\n",
-      threadIdx.x >> 6, blockIdx.x, site, arrived);
-    g_stuck = 1;
-    std::_Exit(3);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spins = 1; gen_ref.load(std::memory_order_acquire) == gen; spins++) {
+    std::this_thread::yield();
+    if ((spins & 0xffffu) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+      std::fprintf(stderr,
+        "wave_emu: lanes of wave %u, block %u did not all reach %s (%d of 64 arrived)\n",
+        threadIdx.x >> 6, blockIdx.x, site, arrived_ref.load());
+      g_stuck = 1;
+      std::_Exit(3);
+    }
   }
 }
 

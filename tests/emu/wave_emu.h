@@ -40,10 +40,8 @@ inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 inline float2 make_float2(float a, float b) { return float2{a, b}; }
 
 struct EmuWave {
-  std::mutex m;
-  std::condition_variable cv;
-  int arrived = 0;
-  uint64_t generation = 0;
+  std::atomic<int> arrived{0};
+  std::atomic<uint64_t> generation{0};
   uint64_t slot[64];
   const char* where = "";
   void barrier(const char* site);
@@ -105,6 +103,7 @@ inline unsigned long long __ballot(int pred) {
 
 // The first active lane is lane 0: every call site runs with the whole wave.
 inline int __builtin_amdgcn_readfirstlane(int v) { return emu_exchange(v, 0, "readfirstlane"); }
+inline int __builtin_amdgcn_readlane(int v, int l) { return emu_exchange(v, l, "readlane"); }
 inline int __float_as_int(float v) { return emu_from<int>(emu_bits(v)); }
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_wave_barrier() emu_wave().barrier("wave_barrier")
