@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <immintrin.h>
 #include <atomic>
 #include <condition_variable>
 #include <functional>
@@ -158,7 +159,7 @@ class WorkPool {
   ~WorkPool() {
     {
       std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
+      stop_.store(true);
     }
     cv_.notify_all();
     for (std::thread& t : threads_)
@@ -169,23 +170,36 @@ class WorkPool {
       fn(0, n);
       return;
     }
+    job_ = &fn;
+    n_ = n;
+    next_.store(0, std::memory_order_relaxed);
+    active_.store((unsigned)threads_.size(), std::memory_order_relaxed);
     {
+      // under the lock: a worker about to sleep sees the new job
       std::lock_guard<std::mutex> lk(mu_);
-      job_ = &fn;
-      n_ = n;
-      next_.store(0);
-      active_ = (unsigned)threads_.size();
-      gen_++;
+      gen_.fetch_add(1, std::memory_order_release);
     }
-    cv_.notify_all();
+    if (sleeping_.load() > 0)
+      cv_.notify_all();
     work(fn, n);
-    std::unique_lock<std::mutex> lk(mu_);
-    done_.wait(lk, [this]() { return active_ == 0; });
+    for (uint32_t spins = 0; active_.load(std::memory_order_acquire) != 0; spins++) {
+      if (spins < SPIN) {
+        _mm_pause();
+        continue;
+      }
+      std::unique_lock<std::mutex> lk(mu_);
+      done_.wait(lk, [this]() { return active_.load() == 0; });
+      break;
+    }
     job_ = nullptr;
   }
 
  private:
   static constexpr uint64_t GRAIN = 64;
+  // Pauses a waiting thread spins before it sleeps (some tens of
+  // microseconds): the commit's pool runs come a few tens of microseconds
+  // apart, and a wake-up from sleep costs about as much.
+  static constexpr uint32_t SPIN = 2048;
   void work(const std::function<void(uint64_t, uint64_t)>& fn, uint64_t n) {
     for (;;) {
       const uint64_t b = next_.fetch_add(GRAIN);
@@ -197,21 +211,27 @@ class WorkPool {
   void loop() {
     uint64_t seen = 0;
     for (;;) {
-      const std::function<void(uint64_t, uint64_t)>* job;
-      uint64_t n;
-      {
+      uint64_t g;
+      for (uint32_t spins = 0;
+           (g = gen_.load(std::memory_order_acquire)) == seen && !stop_.load(); spins++) {
+        if (spins < SPIN) {
+          _mm_pause();
+          continue;
+        }
         std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
-        if (stop_)
-          return;
-        seen = gen_;
-        job = job_;
-        n = n_;
+        sleeping_++;
+        cv_.wait(lk, [&]() { return stop_.load() || gen_.load() != seen; });
+        sleeping_--;
+        spins = 0;
       }
-      work(*job, n);
-      std::lock_guard<std::mutex> lk(mu_);
-      if (--active_ == 0)
+      if (stop_.load())
+        return;
+      seen = g;
+      work(*job_, n_);
+      if (active_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        std::lock_guard<std::mutex> lk(mu_);
         done_.notify_one();
+      }
     }
   }
   std::vector<std::thread> threads_;
@@ -220,9 +240,10 @@ class WorkPool {
   const std::function<void(uint64_t, uint64_t)>* job_ = nullptr;
   uint64_t n_ = 0;
   std::atomic<uint64_t> next_{0};
-  unsigned active_ = 0;
-  uint64_t gen_ = 0;
-  bool stop_ = false;
+  std::atomic<unsigned> active_{0};
+  std::atomic<uint64_t> gen_{0};
+  std::atomic<int> sleeping_{0};
+  std::atomic<bool> stop_{false};
 };
 
 WorkPool& host_pool() {
