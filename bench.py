@@ -29,6 +29,9 @@ CONFIGS = {
     # the CPU baseline's bounded sample of plain512: one eighth of its volume
     # at the same particle density, grid spacing, mesh cell and ladder
     "sample256": dict(n_side=256, box=500.0, n_cells=256, mesh_cell=10.0, rmax=40.0, seed=2),
+    # eight times the volume at the same density and grid spacing, a 15 Mpc/h
+    # mesh cell (133^3) and radii to 60 Mpc/h
+    "plain1024": dict(n_side=1024, box=2000.0, n_cells=1024, mesh_cell=15.0, rmax=60.0, seed=5),
 }
 THRESHOLD = -0.8
 HBM_PEAK_GBS = 8000.0
@@ -42,7 +45,8 @@ def workload(cfg):
     from sif_amd import zeldovich
     c = CONFIGS[cfg]
     t0 = time.time()
-    x, y, z = zeldovich.generate(c["n_side"], c["box"], c["seed"])
+    x, y, z = zeldovich.generate(c["n_side"], c["box"], c["seed"],
+                                 progress=lambda m: log(f"[bench] {m}, {time.time() - t0:.1f}s"))
     log(f"[bench] generated {c['n_side']}^3 Zel'dovich particles in {time.time() - t0:.1f}s, "
         f"crc32 {zeldovich.crc32(x, y, z):08x}")
     cell = c["box"] / c["n_cells"]
@@ -159,6 +163,10 @@ def cpu_baseline(name="sample256"):
 ADJOINT = dict(n_side=256, box=500.0, n_cells=256, r_smooth=3.0, n_centres=100_000,
                r_lo=5.0, r_hi=40.0, ext=3.0, n_bins=16, seed=4, centre_seed=41,
                radius_seed=42, target_seed=43)
+# the same chain on the 1024^3 set of plain1024, Gaussian R = 6 Mpc/h, at the
+# 2e5 largest voids of its catalogue
+ADJOINT1024 = dict(n_side=1024, box=2000.0, n_cells=1024, r_smooth=6.0, n_centres=200_000,
+                   ext=3.0, n_bins=16, seed=5, target_seed=53)
 
 
 class AdjointChain:
@@ -167,18 +175,26 @@ class AdjointChain:
     gradient, the profiles' transpose, the same smoothing, the deposit's
     transpose). The adjoint stores no field of the forward."""
 
-    def __init__(self):
+    def __init__(self, a=ADJOINT, positions=None, voids=None):
+        """positions: the particles (x, y, z) when given, else a's seeded set;
+        voids: (n_centres, 4) centres and radii when given, else a's seeded
+        uniform centres and radii."""
         import ctypes as C
         from sif_amd import _native as N, zeldovich
         from sif_amd._native import DeviceArray
-        self.C, self.N = C, N
-        a = ADJOINT
-        x, y, z = zeldovich.generate(a["n_side"], a["box"], a["seed"])
+        self.C, self.N, self.a = C, N, a
+        if positions is None:
+            x, y, z = zeldovich.generate(a["n_side"], a["box"], a["seed"])
+        else:
+            x, y, z = positions
         self.n = len(x)
-        rng = np.random.default_rng(a["centre_seed"])
-        cen = rng.uniform(0, a["box"], (a["n_centres"], 3)).astype(np.float32)
-        rv = np.random.default_rng(a["radius_seed"]).uniform(
-            a["r_lo"], a["r_hi"], a["n_centres"]).astype(np.float32)
+        if voids is None:
+            rng = np.random.default_rng(a["centre_seed"])
+            cen = rng.uniform(0, a["box"], (a["n_centres"], 3)).astype(np.float32)
+            rv = np.random.default_rng(a["radius_seed"]).uniform(
+                a["r_lo"], a["r_hi"], a["n_centres"]).astype(np.float32)
+        else:
+            cen, rv = voids[:, :3].astype(np.float32), voids[:, 3].astype(np.float32)
         up = lambda v: DeviceArray.from_host(np.ascontiguousarray(v, np.float32))
         self.x, self.y, self.z = up(x), up(y), up(z)
         self.cx, self.cy, self.cz, self.rv = up(cen[:, 0]), up(cen[:, 1]), up(cen[:, 2]), up(rv)
@@ -200,14 +216,14 @@ class AdjointChain:
         self.target = DeviceArray.from_host((p0 + noise).astype(np.float32))
 
     def _smooth(self, src, dst):
-        a, N = ADJOINT, self.N
+        a, N = self.a, self.N
         N.check(N.lib.sif_amd_fft_r2c(src.ptr, a["n_cells"], self.spec.ptr, self.work.ptr))
         N.check(N.lib.sif_amd_fft_filter_c2r(self.spec.ptr, a["n_cells"], a["box"],
                                              N.FILTER_GAUSSIAN, a["r_smooth"], 0.0, 0, 0,
                                              dst.ptr, self.work.ptr))
 
     def forward(self):
-        a, N, C = ADJOINT, self.N, self.C
+        a, N, C = self.a, self.N, self.C
         N.check(N.lib.sif_amd_grid_assign_cic(self.x.ptr, self.y.ptr, self.z.ptr, None, self.n,
                                               a["n_cells"], a["box"], self.grid.ptr,
                                               self.work.ptr))
@@ -230,7 +246,7 @@ class AdjointChain:
         return loss.value
 
     def forward_adjoint(self):
-        a, N = ADJOINT, self.N
+        a, N = self.a, self.N
         self.forward()
         loss = self.loss(self.gp)  # dL/dp over the contrast's mean
         N.check(N.lib.sif_amd_grid_profiles_adjoint(self.gp.ptr, a["n_cells"], a["box"],
@@ -244,12 +260,12 @@ class AdjointChain:
         return loss
 
 
-def adjoint_form(steps, warmup):
+def adjoint_form(steps, warmup, a=ADJOINT, positions=None, voids=None, what=None):
     """Forward and forward+adjoint times of the profile chain, positions
     resident in HBM, each step synchronised."""
     from sif_amd import _native as N
     t0 = time.time()
-    ch = AdjointChain()
+    ch = AdjointChain(a, positions, voids)
     log(f"[bench] adjoint chain set up in {time.time() - t0:.1f}s")
 
     def timed(fn):
@@ -265,7 +281,7 @@ def adjoint_form(steps, warmup):
     t_fwd, loss = timed(ch.forward)
     t_both, _ = timed(ch.forward_adjoint)
     g = np.concatenate([v.get()[:4] for v in (ch.gx, ch.gy, ch.gz)])
-    a = ADJOINT
+    centres = what or (f"{a['n_centres']} centres radii {a['r_lo']:g}-{a['r_hi']:g}")
     log(f"[bench] adjoint chain: forward {t_fwd * 1e3:.2f} ms, forward+adjoint "
         f"{t_both * 1e3:.2f} ms, loss {loss:.6g}")
     return {"metric": "profile-chain particles/s, forward + adjoint",
@@ -275,7 +291,7 @@ def adjoint_form(steps, warmup):
             "loss": loss, "grad_sample": [float(v) for v in g],
             "config": {"workload": f"{a['n_side']}^3 Zel'dovich particles, L={a['box']:g}, "
                                    f"grid {a['n_cells']}^3, Gaussian R={a['r_smooth']:g}, "
-                                   f"{a['n_centres']} centres radii {a['r_lo']:g}-{a['r_hi']:g}, "
+                                   f"{centres}, "
                                    f"{a['n_bins']} cumulative bins to {a['ext']:g}R"}}
 
 
@@ -430,12 +446,29 @@ def main():
     log("[bench] stage times over the timed steps (ms, launches): " +
         ", ".join(f"{k} {v[0]:.1f}/{v[1]}" for k, v in timing.items()))
     roof = roofline(timing, args.config)
+    import ctypes as C
+    used, total = C.c_uint64(), C.c_uint64()
+    N.check(N.lib.sif_amd_device_memory(C.byref(used), C.byref(total)))
+    log(f"[bench] device memory in use after the steps: {used.value / 1e9:.1f} GB of "
+        f"{total.value / 1e9:.1f} GB")
 
     forms = {}
-    if not args.no_adjoint:
-        forms["adjoint"] = adjoint_form(args.steps, args.warmup)
-    if not args.no_batched:
-        forms["batched"] = batched_form()
+    if args.config == "plain1024":
+        # the chain on this set: the catalogue's largest voids as centres
+        a = ADJOINT1024
+        big = cat[np.argsort(-cat[:, 3], kind="stable")[:a["n_centres"]]]
+        a = dict(a, n_centres=len(big))
+        del step
+        if not args.no_adjoint:
+            forms["adjoint"] = adjoint_form(
+                args.steps, args.warmup, a, (x, y, z), big,
+                f"the {len(big)} largest voids of the plain catalogue "
+                f"(radii {big[-1, 3]:.2f}-{big[0, 3]:.2f})")
+    else:
+        if not args.no_adjoint:
+            forms["adjoint"] = adjoint_form(args.steps, args.warmup)
+        if not args.no_batched:
+            forms["batched"] = batched_form()
 
     cpu = None
     if not args.no_cpu:
@@ -454,6 +487,7 @@ def main():
                                f"{len(radii)} rungs {radii[-1]:.3f}-{radii[0]:.2f}",
                    "voids": int(len(cat)), "parallelism": "single GPU"},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timing.items()},
+        "hbm_footprint_gb": round(used.value / 1e9, 2),
         "roofline": roof,
         "forms": forms,
         "cpu_baseline": cpu,

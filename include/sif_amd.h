@@ -29,6 +29,10 @@ SIF_API int sif_amd_memcpy_h2d(void* d_dst, const void* src, uint64_t bytes);
 SIF_API int sif_amd_memcpy_d2h(void* dst, const void* d_src, uint64_t bytes);
 SIF_API int sif_amd_memset(void* d_dst, int value, uint64_t bytes);
 SIF_API int sif_amd_synchronize(void);
+/* Device memory in use on the library's device (every allocation of the
+ * process, the library's cached work buffers included) and its size, from
+ * hipMemGetInfo; no counterpart in the reference. */
+SIF_API int sif_amd_device_memory(uint64_t* used, uint64_t* total);
 /* A plain device-to-device copy kernel (16-byte loads and stores, bytes a
  * multiple of 16, both pointers 16-byte aligned): traffic of known size, the
  * yardstick rocprof's FETCH_SIZE and WRITE_SIZE counters are scaled by. */

@@ -78,6 +78,7 @@ _sig("sif_amd_memcpy_h2d", i32, P, P, u64)
 _sig("sif_amd_memcpy_d2h", i32, P, P, u64)
 _sig("sif_amd_memset", i32, P, i32, u64)
 _sig("sif_amd_synchronize", i32)
+_sig("sif_amd_device_memory", i32, C.POINTER(u64), C.POINTER(u64))
 _sig("sif_amd_copy", i32, P, P, u64)
 _sig("sif_amd_timing_enable", i32, i32)
 _sig("sif_amd_timing_reset", None)

@@ -305,6 +305,21 @@ int sif_amd_synchronize(void) {
   return SIF_OK;
 }
 
+int sif_amd_device_memory(uint64_t* used, uint64_t* total) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!used || !total) {
+    error_set(SIF_ERR_INVALID, "sif_amd_device_memory: NULL argument");
+    return SIF_ERR_INVALID;
+  }
+  size_t free_b = 0, total_b = 0;
+  SIF_HIP(hipMemGetInfo(&free_b, &total_b));
+  *used = (uint64_t)(total_b - free_b);
+  *total = (uint64_t)total_b;
+  return SIF_OK;
+}
+
 int sif_amd_timing_enable(int on) {
   int st = ensure_init();
   if (st != SIF_OK)

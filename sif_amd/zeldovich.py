@@ -53,14 +53,17 @@ def power(k):
     return amplitude() * k ** N_S * bbks_transfer(k) ** 2
 
 
-def generate(n_side, box, seed, workers=-1):
-    """(x, y, z) float32 arrays of n_side^3 particles in [0, box)."""
+def generate(n_side, box, seed, workers=-1, progress=None):
+    """(x, y, z) float32 arrays of n_side^3 particles in [0, box).
+    progress(message), when given, is told as each transform is done."""
+    say = progress or (lambda msg: None)
     n = int(n_side)
     L = float(box)
     rng = np.random.Generator(np.random.PCG64(int(seed)))
     white = rng.standard_normal((n, n, n), dtype=np.float32)
     wk = sfft.rfftn(white, workers=workers)
     del white
+    say(f"white noise transformed ({n}^3)")
     kf = 2.0 * np.pi / L
     kx = (sfft.fftfreq(n, 1.0 / n) * kf).astype(np.float32)
     kz = (sfft.rfftfreq(n, 1.0 / n) * kf).astype(np.float32)
@@ -78,6 +81,7 @@ def generate(n_side, box, seed, workers=-1):
     amp[:, :, nyq] = 0.0
     delta_k = wk * amp
     del wk, amp
+    say("density modes shaped")
     with np.errstate(divide="ignore", invalid="ignore"):
         inv_k2 = np.where(k2 > 0, 1.0 / k2, 0.0).astype(np.float32)
     del k, k2
@@ -98,6 +102,7 @@ def generate(n_side, box, seed, workers=-1):
         pos = np.mod(pos, L).astype(np.float32)
         pos[pos >= np.float32(L)] = 0.0
         out.append(np.ascontiguousarray(pos))
+        say(f"displacement {'xyz'[axis]} done")
     return tuple(out)
 
 

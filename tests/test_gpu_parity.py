@@ -170,6 +170,46 @@ def test_fft_round_trip(native, n):
     assert mag.max() <= 1e-4 * n ** 3
 
 
+def test_fft_1024_properties(native):
+    """n = 1024, where the column passes hold 73.8 KB of LDS a block: the
+    round trip to 1e-4 (test_fft.c:49-125), Parseval's identity over the
+    half spectrum, and a single mode in exactly one bin of magnitude n^3 / 2.
+    (The transform's values against scipy are pinned up to n = 512 above.)"""
+    from sif_amd import device as D
+    n = 1024
+    print("[fft] 1024^3 round trip", flush=True)
+    v = np.random.default_rng(n).standard_normal(n ** 3, dtype=np.float32)
+    dv = D.to_device(v)
+    spec = D.spectrum(dv, n)
+    s = spec.get().reshape(n, n, n // 2 + 1, 2)
+    # Parseval: sum |v|^2 = (1/n^3) sum over the full spectrum |F|^2; the half
+    # spectrum counts every kz but 0 and n/2 twice
+    power = 0.0
+    for ix in range(0, n, 64):
+        blk = s[ix:ix + 64].astype(np.float64)
+        m2 = blk[..., 0] ** 2 + blk[..., 1] ** 2
+        power += 2.0 * m2.sum() - m2[..., 0].sum() - m2[..., n // 2].sum()
+    energy = float(np.dot(v.astype(np.float64), v.astype(np.float64)))
+    assert abs(power / n ** 3 - energy) <= 1e-5 * energy
+    del s
+    back = D.smooth(spec, n, BOX, 0, 0.0, window_order=0).get()
+    assert np.abs(back - v).max() < 1e-4
+    del back, dv, spec
+    k = (1, 2, 3)
+    i = np.arange(n, dtype=np.int32)
+    phase = (k[0] * i[:, None, None] + k[1] * i[None, :, None] + k[2] * i[None, None, :]) % n
+    table = np.cos(2 * np.pi * np.arange(n) / n).astype(np.float32)
+    wave = table[phase].ravel()
+    del phase
+    s1 = D.spectrum(D.to_device(wave), n).get()
+    del wave
+    mag = np.hypot(s1[..., 0], s1[..., 1])
+    del s1
+    assert abs(mag[k] - n ** 3 / 2) <= 1e-4 * n ** 3
+    mag[k] = 0.0
+    assert mag.max() <= 1e-4 * n ** 3
+
+
 def test_top_hat_preserves_constant(native):
     """test_fft.c:129-173."""
     from sif_amd import device as D
