@@ -83,7 +83,7 @@ class Step:
 
 # stage -> (the kernels one timed launch of it runs, how many)
 STAGE_KERNELS = {
-    "grow": ("grow_kernel", 1),
+    "grow": ("grow_kernel (+ grow_bin_kernel on launches of under 2048 candidates)", 1),
     "smooth": ("cols_kernel x2 + c2r_rows_kernel (one filtered c2r)", 3),
     "cic": ("cic_count/bin/tile_kernel + cic_finish_kernel (tile binning, LDS deposit, flush)", 4),
     "mesh": ("mesh_key_kernel + radix sort + mesh_gather_kernel", 17),
