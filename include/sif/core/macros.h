@@ -7,6 +7,8 @@
  *   sif_real, sif_option      include/sif/core/macros.h:42, :504-534
  *   SIF_FINDER_* option bits  include/sif/core/macros.h:73-248
  *   SIF_MESH_* option bits    include/sif/core/macros.h:269-293
+ *   SIF_PBC_* option bits     include/sif/core/macros.h:52-54
+ *   SIF_PROFILES_* bits       include/sif/core/macros.h:310-332
  *   SIF_OK ... SIF_ERR_*      include/sif/core/macros.h:690-707
  */
 #ifndef SIF_CORE_MACROS_H
@@ -86,6 +88,23 @@ static inline float sif__finder_search_factor(uint32_t opt) {
 /* --- chain mesh options --- */
 #define SIF_MESH_DROP_INDICES (1u << 16)
 #define SIF_MESH_NO_CANONICAL (1u << 17)
+
+/* --- boundary (bit 0): a periodic box, or an open one with nothing outside --- */
+#define SIF_PBC_PERIODIC (0u << 0)
+#define SIF_PBC_OPEN     (1u << 0)
+#define SIF__PBC_MASK    (1u << 0)
+
+/* --- profile options --- */
+/* A density bin holds the contrast enclosed within its outer edge, or that
+ * of its own shell (bit 8). */
+#define SIF_PROFILES_CUMULATIVE   (0u << 8)
+#define SIF_PROFILES_DIFFERENTIAL (1u << 8)
+#define SIF__PROFILES_BIN_MASK    (1u << 8)
+/* A shell's radial velocity averaged over its tracers, or over their
+ * weights (bit 9). */
+#define SIF_PROFILES_VELOCITY_NUMBER   (0u << 9)
+#define SIF_PROFILES_VELOCITY_WEIGHTED (1u << 9)
+#define SIF__PROFILES_VELOCITY_MASK    (1u << 9)
 
 /* --- status codes --- */
 #define SIF_OK                0

@@ -41,11 +41,11 @@ SIF_API int sif_amd_copy(void* d_dst, const void* d_src, uint64_t bytes);
 /* Per-stage device time (HIP events on the library stream, recorded without
  * synchronising and resolved when read). Stage order:
  * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit,
- * retest, sort, host. commit, retest and host are host time: the ordered
- * accept loop, the re-tests that assemble each growth batch, and the rest of
- * the driver's own work per rung (the void index, the survivors' copy, the
- * filter tables). */
-#define SIF_AMD_TIMING_STAGES 12
+ * retest, sort, host, profiles. commit, retest and host are host time: the
+ * ordered accept loop, the re-tests that assemble each growth batch, and the
+ * rest of the driver's own work per rung (the void index, the survivors'
+ * copy, the filter tables). profiles is sif_profiles' binning. */
+#define SIF_AMD_TIMING_STAGES 13
 SIF_API int sif_amd_timing_enable(int on);
 SIF_API void sif_amd_timing_reset(void);
 SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);
@@ -183,6 +183,24 @@ SIF_API sif_catalogue_t* sif_amd_exodus_device_weighted(float* d_delta,
   uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
   sif_real threshold, sif_real overlap_fraction, sif_option options,
   int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats);
+
+/* --- profiles of a catalogue over the tracers --- */
+/* sif_profiles (src/measure/profiles.c:354-673) on device arrays: the chain
+ * mesh (d_offsets, n^3 + 1 entries; positions; d_mw weights or NULL; d_vx,
+ * d_vy, d_vz velocities, needed only with d_vel) of a box of side box and
+ * total weight total_weight (the tracer count when unweighted), and n_voids
+ * voids (d_cx, d_cy, d_cz, d_rv). ext (not positive: 5), n_bins and options
+ * (SIF_PBC_*, SIF_PROFILES_*) as sif_profiles takes them. d_dens and d_vel
+ * (n_voids * n_bins floats, either may be NULL) receive the rows; a void of
+ * non-positive radius leaves its rows as they were. An unweighted density
+ * profile is the reference's bit for bit; weighted sums and velocities are
+ * summed in double in a fixed order. */
+SIF_API int sif_amd_tracer_profiles(const uint64_t* d_offsets, const float* d_mx,
+  const float* d_my, const float* d_mz, const float* d_mw, const float* d_vx,
+  const float* d_vy, const float* d_vz, uint32_t mesh_cells, float box,
+  double total_weight, const float* d_cx, const float* d_cy, const float* d_cz,
+  const float* d_rv, uint32_t n_voids, float ext, uint32_t n_bins, sif_option options,
+  float* d_dens, float* d_vel);
 
 /* --- profiles of a grid, and the adjoint of the profile chain --- */
 /* sif_profiles (src/measure/profiles.c:354-673), density part, cumulative

@@ -43,7 +43,7 @@ class Template(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("n_cells", u32), ("box_length", f32), ("cell_length", f32),
                 ("offsets", P), ("x", P), ("y", P), ("z", P), ("w", P), ("cell_w", P),
-                ("total_weight", f64)]
+                ("total_weight", f64), ("vx", P), ("vy", P), ("vz", P)]
 
 
 class Catalogue(C.Structure):
@@ -271,11 +271,13 @@ class OracleMesh:
         self.keep = m
         w = m.get("w")
         cw = m.get("cell_weights")
+        v = [m.get(a) for a in ("vx", "vy", "vz")]
         self.s = Mesh(n_cells, box, np.float32(box) / np.float32(n_cells),
                       m["offsets"].ctypes.data, m["x"].ctypes.data, m["y"].ctypes.data,
                       m["z"].ctypes.data, None if w is None else w.ctypes.data,
                       None if cw is None else cw.ctypes.data,
-                      float(m.get("total_weight", len(m["x"]))))
+                      float(m.get("total_weight", len(m["x"]))),
+                      *[None if a is None else a.ctypes.data for a in v])
 
 
 class OracleTemplate:
@@ -322,6 +324,26 @@ def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, window=8,
         reason[i] = rs.value
         nin[i] = ni.value
     return out, reason, nin
+
+
+_sig("orc_profiles", i32, C.POINTER(Mesh), P, P, P, P, u64, f32, u32, u32, P, P, P)
+
+
+def profiles(mesh, voids, ext, n_bins, opt=0, dens=True, vel=False):
+    """sif_profiles restated (profiles.c:354-673): (dens, vel, count), each
+    (n_voids, n_bins) or None; voids (n, 4) centres and radii."""
+    v = np.ascontiguousarray(voids, dtype=np.float32).reshape(-1, 4)
+    cols = [np.ascontiguousarray(v[:, k]) for k in range(4)]
+    nv = len(v)
+    d = np.zeros((nv, n_bins), np.float32) if dens else None
+    u = np.zeros((nv, n_bins), np.float32) if vel else None
+    cnt = np.zeros((nv, n_bins), np.uint64)
+    st = lib.orc_profiles(C.byref(mesh.s), *[c.ctypes.data for c in cols], nv, ext, n_bins, opt,
+                          None if d is None else d.ctypes.data,
+                          None if u is None else u.ctypes.data, cnt.ctypes.data)
+    if st != 0:
+        raise ValueError("orc_profiles: bad arguments")
+    return d, u, cnt
 
 
 def brute_force_radius(x, y, z, box, centre, rung, r_search, threshold):

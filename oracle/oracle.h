@@ -119,7 +119,22 @@ typedef struct {
   const float* w;
   const float* cell_w;
   double total_weight;
+  /* velocities in mesh order, or NULL */
+  const float* vx;
+  const float* vy;
+  const float* vz;
 } orc_mesh_t;
+
+/* sif_profiles (src/measure/profiles.c:354-673) for voids (cx, cy, cz, rv)
+ * over the mesh: float accumulators per void, the reference's cell walk and
+ * order. opt: SIF_PBC_OPEN (bit 0), SIF_PROFILES_DIFFERENTIAL (bit 8),
+ * SIF_PROFILES_VELOCITY_WEIGHTED (bit 9). dens / vel: n_voids * n_bins or
+ * NULL; count (optional): the tracers binned, n_voids * n_bins. Rows of voids
+ * of non-positive radius are not written. Returns 0, or -1 on a bad
+ * argument. */
+int orc_profiles(const orc_mesh_t* m, const float* cx, const float* cy,
+  const float* cz, const float* rv, uint64_t n_voids, float ext,
+  uint32_t n_bins, uint32_t opt, float* dens, float* vel, uint64_t* count);
 
 #define ORC_RESCALE_OK         0
 #define ORC_RESCALE_EMPTY      1

@@ -114,7 +114,7 @@ int orc_exodus_pipeline(const float* x, const float* y, const float* z,
   t0 = t1;
 
   const orc_mesh_t mesh = {mesh_cells, box, box / (float)mesh_cells, offsets, mx,
-    my, mz, NULL, NULL, (double)n};
+    my, mz, NULL, NULL, (double)n, NULL, NULL, NULL};
   const int st = orc_exodus_run(fields, N, box, &mesh, n, radii, n_radii,
     threshold, 0.0f, 0, 0, 0, 1, 8, cat, NULL);
   t1 = now_s();

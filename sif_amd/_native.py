@@ -23,9 +23,9 @@ SIF_ERR_UNSUPPORTED = -5
 SIF_ERR_UNINITIALISED = -6
 
 FILTER_NONE, FILTER_TOP_HAT, FILTER_GAUSSIAN = 0, 1, 2
-N_STAGES = 12
+N_STAGES = 13
 STAGE_NAMES = ("cic", "contrast", "mesh", "fft_forward", "smooth", "scan",
-               "prefilter", "grow", "commit", "retest", "sort", "host")
+               "prefilter", "grow", "commit", "retest", "sort", "host", "profiles")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -112,6 +112,8 @@ _sig("sif_amd_exodus_device_weighted", C.POINTER(Catalogue), P, u32, f32, P, P, 
      f64, u32, u64, C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
      C.POINTER(RungStats))
 _sig("sif_amd_exodus_fields", i32, P, u32, f32, C.POINTER(f32), u32, u32, P)
+_sig("sif_amd_tracer_profiles", i32, P, P, P, P, P, P, P, P, u32, f32, f64, P, P, P, P, u32,
+     f32, u32, u32, P, P)
 _sig("sif_catalogue_free", None, C.POINTER(Catalogue))
 
 libc = C.CDLL(None)

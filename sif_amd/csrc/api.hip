@@ -18,6 +18,7 @@
 #include "internal.h"
 #include "sif/core/system.h"
 #include "sif/finder/exodus_finder.h"
+#include "sif/measure/profiles.h"
 #include "sif/structures/catalogue.h"
 #include "sif/structures/chain_mesh.h"
 #include "sif/structures/field.h"
@@ -1051,6 +1052,186 @@ uint32_t sif_finder_grid_cells_from_size(uint64_t n_particles, sif_real box_leng
     return 0;
   }
   return cells < 1.0 ? 1u : (uint32_t)cells;
+}
+
+// --- stacked profiles (profiles.h, profiles.c:24-92, 354-673) ---
+
+void sif_density_profiles_free(sif_density_profiles_t* p) {
+  if (!p)
+    return;
+  free(p->r_edges);
+  free(p->profiles);
+  free(p);
+}
+
+void sif_velocity_profiles_free(sif_velocity_profiles_t* p) {
+  if (!p)
+    return;
+  free(p->r_edges);
+  free(p->v_rad);
+  free(p);
+}
+
+int sif_profiles(const sif_catalogue_t* cat, const sif_chain_mesh_t* mesh, sif_real ext,
+  uint32_t n_bins, sif_option opt, sif_density_profiles_t** out_dens,
+  sif_velocity_profiles_t** out_vel) {
+  // profiles.c:97-142, in its order
+  if (!cat || cat->n_voids == 0) {
+    error_set(SIF_ERR_INVALID, "no voids to profile");
+    return SIF_ERR_INVALID;
+  }
+  if (!mesh || mesh->n_particles == 0 || !(mesh->box_length > 0.0f)) {
+    error_set(SIF_ERR_INVALID, "invalid or empty chain mesh");
+    return SIF_ERR_INVALID;
+  }
+  if (!out_dens && !out_vel) {
+    error_set(SIF_ERR_INVALID, "both outputs are NULL; there is nothing to compute");
+    return SIF_ERR_INVALID;
+  }
+  if (n_bins == 0) {
+    error_set(SIF_ERR_INVALID, "n_bins must be non-zero");
+    return SIF_ERR_INVALID;
+  }
+  if (out_vel && !mesh->vx) {
+    error_set(SIF_ERR_INVALID,
+      "velocity profiles were requested but the mesh carries no velocities");
+    return SIF_ERR_INVALID;
+  }
+  if (cat->units != SIF_COORDINATES_CARTESIAN) {
+    error_set(SIF_ERR_INVALID,
+      "the catalogue holds sky coordinates; profiles are measured around Cartesian "
+      "centres, before sif_catalogue_to_sky()");
+    return SIF_ERR_INVALID;
+  }
+  int st = need_init();
+  if (st != SIF_OK)
+    return st;
+  TracerProfileSpec S;
+  tracer_profile_spec(&S, mesh->n_cells, mesh->box_length, mesh->total_weight, ext, n_bins,
+    opt, mesh->weights != nullptr, out_vel != nullptr);
+  S.cell_length = mesh->cell_length;
+  const uint64_t nv = cat->n_voids, rows = nv * n_bins;
+  // a set the caller passed is refilled; NULL asks for a new one, zeroed
+  // (profiles.c:393-412)
+  bool made_dens = false, made_vel = false;
+  auto fail = [&](int status) {
+    if (out_dens && *out_dens) {
+      sif_density_profiles_free(*out_dens);
+      *out_dens = nullptr;
+    }
+    if (out_vel && *out_vel) {
+      sif_velocity_profiles_free(*out_vel);
+      *out_vel = nullptr;
+    }
+    return status;
+  };
+  if (out_dens && !*out_dens) {
+    sif_density_profiles_t* p =
+      (sif_density_profiles_t*)calloc(1, sizeof(sif_density_profiles_t));
+    if (p) {
+      p->n_voids = nv;
+      p->n_bins = n_bins;
+      p->ext = S.ext;
+      p->differential = S.differential != 0;
+      p->r_edges = (sif_real*)aligned_alloc_bytes((n_bins + 1) * sizeof(sif_real));
+      p->profiles = (sif_real*)aligned_alloc_bytes(rows * sizeof(sif_real));
+      if (p->profiles)
+        memset(p->profiles, 0, rows * sizeof(sif_real));
+    }
+    if (!p || !p->r_edges || !p->profiles) {
+      sif_density_profiles_free(p);
+      error_set(SIF_ERR_ALLOC, "OOM allocating density profiles");
+      return SIF_ERR_ALLOC;
+    }
+    *out_dens = p;
+    made_dens = true;
+  }
+  if (out_vel && !*out_vel) {
+    sif_velocity_profiles_t* p =
+      (sif_velocity_profiles_t*)calloc(1, sizeof(sif_velocity_profiles_t));
+    if (p) {
+      p->n_voids = nv;
+      p->n_bins = n_bins;
+      p->ext = S.ext;
+      p->r_edges = (sif_real*)aligned_alloc_bytes((n_bins + 1) * sizeof(sif_real));
+      p->v_rad = (sif_real*)aligned_alloc_bytes(rows * sizeof(sif_real));
+      if (p->v_rad)
+        memset(p->v_rad, 0, rows * sizeof(sif_real));
+    }
+    if (!p || !p->r_edges || !p->v_rad) {
+      sif_velocity_profiles_free(p);
+      error_set(SIF_ERR_ALLOC, "OOM allocating velocity profiles");
+      return fail(SIF_ERR_ALLOC);
+    }
+    *out_vel = p;
+    made_vel = true;
+  }
+  (void)made_dens;
+  (void)made_vel;
+  if (out_dens)
+    memcpy((*out_dens)->r_edges, S.edges.data(), (n_bins + 1) * sizeof(sif_real));
+  if (out_vel) {
+    // velocity edges are the cumulative binning's (profiles.c:431-432)
+    for (uint32_t j = 0; j < n_bins; j++)
+      (*out_vel)->r_edges[j] = (float)j * S.bin_width;
+    (*out_vel)->r_edges[n_bins] = S.ext;
+  }
+
+  const uint64_t n = mesh->n_particles, cells = mesh->total_cells;
+  Dev doff, dmx, dmy, dmz, dmw, dvx, dvy, dvz, dcx, dcy, dcz, drv, dd, dv;
+  st = doff.upload(mesh->cell_offsets, (cells + 1) * 8);
+  if (st == SIF_OK) st = dmx.upload(mesh->x, n * 4);
+  if (st == SIF_OK) st = dmy.upload(mesh->y, n * 4);
+  if (st == SIF_OK) st = dmz.upload(mesh->z, n * 4);
+  if (st == SIF_OK && mesh->weights) st = dmw.upload(mesh->weights, n * 4);
+  if (st == SIF_OK && out_vel) st = dvx.upload(mesh->vx, n * 4);
+  if (st == SIF_OK && out_vel) st = dvy.upload(mesh->vy, n * 4);
+  if (st == SIF_OK && out_vel) st = dvz.upload(mesh->vz, n * 4);
+  if (st == SIF_OK) st = dcx.upload(cat->cx, nv * 4);
+  if (st == SIF_OK) st = dcy.upload(cat->cy, nv * 4);
+  if (st == SIF_OK) st = dcz.upload(cat->cz, nv * 4);
+  if (st == SIF_OK) st = drv.upload(cat->radii, nv * 4);
+  // the rows as they are, so that those of voids without a radius stay so
+  if (st == SIF_OK && out_dens) st = dd.upload((*out_dens)->profiles, rows * 4);
+  if (st == SIF_OK && out_vel) st = dv.upload((*out_vel)->v_rad, rows * 4);
+  if (st != SIF_OK)
+    return fail(st);
+  S.offsets = doff.as<uint64_t>();
+  S.mx = dmx.as<float>();
+  S.my = dmy.as<float>();
+  S.mz = dmz.as<float>();
+  S.mw = mesh->weights ? dmw.as<float>() : nullptr;
+  S.vx = out_vel ? dvx.as<float>() : nullptr;
+  S.vy = out_vel ? dvy.as<float>() : nullptr;
+  S.vz = out_vel ? dvz.as<float>() : nullptr;
+  S.cx = dcx.as<float>();
+  S.cy = dcy.as<float>();
+  S.cz = dcz.as<float>();
+  S.rv = drv.as<float>();
+  S.n_voids = (uint32_t)nv;
+  S.dens = out_dens ? dd.as<float>() : nullptr;
+  S.vel = out_vel ? dv.as<float>() : nullptr;
+  if (nv > 0xffffffffull) {
+    error_set(SIF_ERR_RANGE, "profiles: %llu voids (at most 2^32 - 1)", (unsigned long long)nv);
+    return fail(SIF_ERR_RANGE);
+  }
+  st = tracer_profiles(S);
+  if (st == SIF_OK && out_dens &&
+      hipMemcpyAsync((*out_dens)->profiles, dd.p, rows * 4, hipMemcpyDeviceToHost, stream()) !=
+        hipSuccess)
+    st = SIF_ERR_INVALID;
+  if (st == SIF_OK && out_vel &&
+      hipMemcpyAsync((*out_vel)->v_rad, dv.p, rows * 4, hipMemcpyDeviceToHost, stream()) !=
+        hipSuccess)
+    st = SIF_ERR_INVALID;
+  if (st == SIF_OK && hipStreamSynchronize(stream()) != hipSuccess)
+    st = SIF_ERR_INVALID;
+  if (st != SIF_OK) {
+    if (st == SIF_ERR_INVALID)
+      error_set(SIF_ERR_INVALID, "profiles: the device run failed");
+    return fail(st);
+  }
+  return SIF_OK;
 }
 
 }  // extern "C"

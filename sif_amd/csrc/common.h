@@ -74,5 +74,6 @@ enum {
   SIF_AMD_STAGE_RETEST,
   SIF_AMD_STAGE_SORT,
   SIF_AMD_STAGE_HOST,
+  SIF_AMD_STAGE_PROFILES,
   SIF_AMD_N_STAGES
 };

@@ -141,6 +141,32 @@ int check_options(sif_option opt);
 // positive total. The host form reads host weights, the device form device
 // ones.
 int check_weights_host(const float* w, uint64_t n, const float* cell_w, double total);
+
+// sif_profiles on device arrays (tracer_profiles.hip). The spec's geometry,
+// bin edges and volumes, mean density and flags come from
+// tracer_profile_spec() (profiles.c:332-352, 144-150, 408-470); the caller
+// sets the arrays. vx NULL: no velocities; dens or vel NULL: not written.
+// Rows of voids of non-positive radius are left as they are.
+struct TracerProfileSpec {
+  const uint64_t* offsets = nullptr;
+  const float *mx = nullptr, *my = nullptr, *mz = nullptr, *mw = nullptr;
+  const float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  uint32_t mesh_n = 0;
+  float box = 0.0f, cell_length = 0.0f;
+  int periodic = 1;
+  const float *cx = nullptr, *cy = nullptr, *cz = nullptr, *rv = nullptr;
+  uint32_t n_voids = 0;
+  float ext = 0.0f, bin_width = 0.0f;
+  uint32_t n_bins = 0;
+  int differential = 0, vel_weighted = 0;
+  float mean_dens = 1.0f;
+  std::vector<float> edges, vols;  // n_bins + 1 edges; n_bins volumes, in void radii
+  const float* bin_vols = nullptr;
+  float *dens = nullptr, *vel = nullptr;
+};
+void tracer_profile_spec(TracerProfileSpec* S, uint32_t mesh_n, float box, double total_weight,
+  float ext, uint32_t n_bins, sif_option opt, bool weighted, bool want_vel);
+int tracer_profiles(const TracerProfileSpec& S);
 int check_weights_device(const float* d_w, uint64_t n, const float* d_cell_w, double total);
 
 }  // namespace sif_amd

@@ -124,6 +124,29 @@ def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, mass_facto
     return dr.get()[:k], drs.get()[:k], dn.get()[:k]
 
 
+def tracer_profiles(mesh, voids, ext, n_bins, options=0, dens=True, velocities=None):
+    """sif_profiles on device arrays: (dens, vel), each (n_voids, n_bins) or
+    None. voids: (n, 4) centres and radii (host); velocities: (vx, vy, vz)
+    device arrays in mesh order, or None for no velocity profile."""
+    v = np.ascontiguousarray(voids, dtype=np.float32).reshape(-1, 4)
+    nv = len(v)
+    cols = [to_device(np.ascontiguousarray(v[:, k])) for k in range(4)]
+    dd = DeviceArray((max(nv * n_bins, 1),), np.float32) if dens else None
+    dv = DeviceArray((max(nv * n_bins, 1),), np.float32) if velocities is not None else None
+    for a in (dd, dv):
+        if a is not None:
+            check(lib.sif_amd_memset(a.ptr, 0, a.nbytes))
+    vx, vy, vz = velocities if velocities is not None else (None, None, None)
+    check(lib.sif_amd_tracer_profiles(
+        mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+        mesh.w.ptr if mesh.w is not None else None, vx.ptr if vx else None,
+        vy.ptr if vy else None, vz.ptr if vz else None, mesh.n_cells, mesh.box,
+        mesh.total_weight, cols[0].ptr, cols[1].ptr, cols[2].ptr, cols[3].ptr, nv, ext, n_bins,
+        options, dd.ptr if dd else None, dv.ptr if dv else None))
+    out = [a.get()[:nv * n_bins].reshape(nv, n_bins) if a is not None else None for a in (dd, dv)]
+    return out[0], out[1]
+
+
 def exodus(ddelta, n_cells, box, mesh, radii, threshold, overlap=0.0, options=0,
            restore=True):
     """The exodus finder on device data: (catalogue (n, 4), n_in, stats); on
