@@ -295,6 +295,38 @@ def adjoint_form(steps, warmup, a=ADJOINT, positions=None, voids=None, what=None
                                    f"{a['n_bins']} cumulative bins to {a['ext']:g}R"}}
 
 
+def profiles_form(step, cat, steps, warmup):
+    """sif_profiles over the plain catalogue: every void, the reference's
+    default reach of 5 void radii, 16 bins, density and radial velocity
+    (seeded velocities), over the same 512^3 tracers on their mesh."""
+    from sif_amd import _native as N
+    D = step.D
+    c = step.c
+    mesh = D.Mesh(step.dx, step.dy, step.dz, step.n, step.mesh_cells, c["box"])
+    o = mesh.orig.get().astype(np.int64)
+    rng = np.random.default_rng(77)
+    vel = tuple(D.to_device(rng.normal(0.0, 300.0, step.n).astype(np.float32)[o])
+                for _ in range(3))
+    del o
+
+    def run():
+        return D.tracer_profiles(mesh, cat, 5.0, 16, 0, velocities=vel)
+
+    for _ in range(warmup):
+        run()
+    N.check(N.lib.sif_amd_synchronize())
+    t = time.perf_counter()
+    for _ in range(steps):
+        dens, v = run()
+    N.check(N.lib.sif_amd_synchronize())
+    dt = (time.perf_counter() - t) / steps
+    log(f"[bench] profiles of {len(cat)} voids: {dt * 1e3:.1f} ms")
+    return {"metric": "stacked profiles of the plain catalogue, voids/s", "unit": "voids/s",
+            "value": round(len(cat) / dt, 1), "ms": round(dt * 1e3, 2),
+            "config": {"workload": f"{len(cat)} voids, {step.n} tracers on the {step.mesh_cells}^3 "
+                                   f"mesh, ext 5, 16 bins, density and radial velocity"}}
+
+
 # many boxes of uneven size: 96 seeded Zel'dovich sets, N^3 particles with N
 # log-uniform in 64-256 (a multiple of 8), boxes at the fixed mean separation
 # 100/64 Mpc/h (100-400 Mpc/h a side), grids at the power of two nearest N
@@ -465,6 +497,8 @@ def main():
                 f"the {len(big)} largest voids of the plain catalogue "
                 f"(radii {big[-1, 3]:.2f}-{big[0, 3]:.2f})")
     else:
+        forms["profiles"] = profiles_form(step, cat, args.steps, args.warmup)
+        del step
         if not args.no_adjoint:
             forms["adjoint"] = adjoint_form(args.steps, args.warmup)
         if not args.no_batched:
