@@ -340,6 +340,23 @@ static int catalogue_resize(sif_catalogue_t* c, uint64_t cap) {
     memcpy(block + 2 * stride, c->cz, keep * sizeof(sif_real));
     memcpy(block + 3 * stride, c->radii, keep * sizeof(sif_real));
   }
+  sif_real* fblock = nullptr;
+  if (c->_footprint_block) {  // the footprint columns follow the same stride
+    fblock = (sif_real*)aligned_alloc_bytes(2 * stride * sizeof(sif_real));
+    if (!fblock) {
+      free(block);
+      error_set(SIF_ERR_ALLOC, "catalogue: cannot grow to %llu voids", (unsigned long long)cap);
+      return SIF_ERR_ALLOC;
+    }
+    if (keep) {
+      memcpy(fblock, c->footprint, keep * sizeof(sif_real));
+      memcpy(fblock + stride, c->footprint_shell, keep * sizeof(sif_real));
+    }
+    free(c->_footprint_block);
+    c->_footprint_block = fblock;
+    c->footprint = fblock;
+    c->footprint_shell = fblock + stride;
+  }
   free(c->_block);
   c->_block = block;
   c->cx = block;
@@ -388,6 +405,32 @@ int sif_catalogue_append(sif_catalogue_t* c, sif_real x, sif_real y, sif_real z,
   c->cy[k] = y;
   c->cz[k] = z;
   c->radii[k] = r;
+  if (c->footprint) {  // catalogue.h:224-236: not measured yet
+    c->footprint[k] = SIF_CATALOGUE_FOOTPRINT_UNKNOWN;
+    c->footprint_shell[k] = SIF_CATALOGUE_FOOTPRINT_UNKNOWN;
+  }
+  return SIF_OK;
+}
+
+int sif_catalogue_reserve_footprint(sif_catalogue_t* c) {
+  if (!c) {
+    error_set(SIF_ERR_INVALID, "catalogue: NULL");
+    return SIF_ERR_INVALID;
+  }
+  if (c->footprint)
+    return SIF_OK;
+  const uint64_t per_line = LINE / sizeof(sif_real);
+  const uint64_t stride = (c->capacity + per_line - 1) & ~(per_line - 1);
+  sif_real* fblock = (sif_real*)aligned_alloc_bytes(2 * stride * sizeof(sif_real));
+  if (!fblock) {
+    error_set(SIF_ERR_ALLOC, "catalogue: cannot allocate the footprint");
+    return SIF_ERR_ALLOC;
+  }
+  c->_footprint_block = fblock;
+  c->footprint = fblock;
+  c->footprint_shell = fblock + stride;
+  for (uint64_t i = 0; i < c->n_voids; i++)
+    c->footprint[i] = c->footprint_shell[i] = SIF_CATALOGUE_FOOTPRINT_UNKNOWN;
   return SIF_OK;
 }
 

@@ -15,6 +15,9 @@ struct Columns {
 };
 
 // exodus_checkpoint.c:57-70: CRC32 of a catalogue's four columns
+// The running CRC32 of utils/crc32.h: start from 0xFFFFFFFF, update over
+// each buffer in turn, finish by XOR with 0xFFFFFFFF.
+uint32_t crc32_update(uint32_t crc, const void* data, size_t n);
 uint32_t catalogue_crc(const float* x, const float* y, const float* z, const float* r,
   uint64_t n);
 // exodus-<16 hex digits>.ckpt, a name no earlier run has used

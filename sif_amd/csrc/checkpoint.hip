@@ -74,6 +74,10 @@ int read_failed(const char* path, const char* what) {
 
 }  // namespace
 
+uint32_t crc32_update(uint32_t crc, const void* data, size_t n) {
+  return crc_table().update(crc, data, n);
+}
+
 uint32_t catalogue_crc(const float* x, const float* y, const float* z, const float* r,
   uint64_t n) {
   const Crc32& t = crc_table();

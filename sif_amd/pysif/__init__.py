@@ -89,7 +89,9 @@ _sig("sif_chain_mesh_free", None, C.POINTER(_Mesh))
 _sig("sif_catalogue_alloc", C.POINTER(_Catalogue), u64)
 _sig("sif_catalogue_append", i32, C.POINTER(_Catalogue), f32, f32, f32, f32)
 _sig("sif_catalogue_trim", i32, C.POINTER(_Catalogue))
+_sig("sif_catalogue_reserve_footprint", i32, C.POINTER(_Catalogue))
 
+SIF_COORDINATES_SKY = 1  # macros.h: sif_coordinates_t
 SIF_MESH_DROP_INDICES = 1 << 16
 SIF_MESH_NO_CANONICAL = 1 << 17
 _ASSIGNMENTS = {0: None, 1: "ngp", 2: "cic", 3: "tsc", 4: "pcs"}
@@ -332,31 +334,75 @@ class Catalogue:
     def radii(self):
         return _view(self, self._p.contents.radii, (self.n_voids,))
 
+    @property
+    def units(self):
+        """'cartesian', or 'sky' for centres as ra, dec (degrees) and redshift."""
+        return "sky" if self._p.contents.units == SIF_COORDINATES_SKY else "cartesian"
+
+    @property
+    def footprint(self):
+        """The fraction of each void's sphere inside a survey footprint, a
+        read-only view, or None for a catalogue with none (-1: not measured)."""
+        return _view(self, C.cast(self._p.contents.footprint, C.POINTER(f32)), (self.n_voids,))
+
+    @property
+    def footprint_shell(self):
+        """The same fraction over the shell of one to two radii, or None."""
+        return _view(self, C.cast(self._p.contents.footprint_shell, C.POINTER(f32)),
+                     (self.n_voids,))
+
     def to_numpy(self):
-        """A structured array of (x, y, z, radius)."""
-        out = np.zeros(self.n_voids, dtype=[("x", real), ("y", real), ("z", real),
-                                            ("radius", real)])
+        """A structured array, a copy, with the fields named as the file
+        headers name them: cx, cy, cz, r (ra, dec, z, r on the sky), and
+        footprint, footprint_shell when carried."""
+        names = ["ra", "dec", "z"] if self.units == "sky" else ["cx", "cy", "cz"]
+        extra = ["footprint", "footprint_shell"] if self._p.contents.footprint else []
+        out = np.zeros(self.n_voids, dtype=[(k, real) for k in names + ["r"] + extra])
         cen = self.centres
-        out["x"], out["y"], out["z"] = cen[:, 0], cen[:, 1], cen[:, 2]
-        out["radius"] = self.radii if self.n_voids else []
+        for k, name in enumerate(names):
+            out[name] = cen[:, k]
+        out["r"] = self.radii if self.n_voids else []
+        for name in extra:
+            out[name] = getattr(self, name) if self.n_voids else []
         return out
 
     def _append(self, x, y, z, r):
         _check(L.sif_catalogue_append(self._p, float(x), float(y), float(z), float(r)))
 
 
-def catalogue_from_numpy(cx=None, cy=None, cz=None, radii=None):
-    if cx is None or cy is None or cz is None or radii is None:
-        raise ValueError("catalogue_from_numpy: cx, cy, cz and radii are required")
-    cols = [_col(cx)]
-    cols += [_col(a, len(cols[0])) for a in (cy, cz, radii)]
+def catalogue_from_numpy(cx=None, cy=None, cz=None, r=None, *, ra=None, dec=None, z=None,
+                         footprint=None, footprint_shell=None, radii=None):
+    """A new Catalogue with its voids copied out of NumPy arrays: cx, cy, cz
+    for Cartesian centres or ra, dec, z for a catalogue on the sky; r (or
+    radii) the radii; footprint and footprint_shell both or neither."""
+    if r is None:
+        r = radii
+    sky = [a is not None for a in (ra, dec, z)]
+    box = [a is not None for a in (cx, cy, cz)]
+    if any(sky) and any(box):
+        raise ValueError("catalogue_from_numpy: Cartesian and sky names are mixed")
+    centre = (ra, dec, z) if any(sky) else (cx, cy, cz)
+    if any(a is None for a in centre) or r is None:
+        raise ValueError("catalogue_from_numpy: the three centre columns and r are required")
+    if (footprint is None) != (footprint_shell is None):
+        raise ValueError("catalogue_from_numpy: footprint and footprint_shell go together")
+    cols = [_col(centre[0])]
+    cols += [_col(a, len(cols[0])) for a in (centre[1], centre[2], r)]
     cat = Catalogue(len(cols[0]))
     for i in range(len(cols[0])):
         cat._append(cols[0][i], cols[1][i], cols[2][i], cols[3][i])
+    if any(sky):
+        cat._p.contents.units = SIF_COORDINATES_SKY
+    if footprint is not None:
+        fp = [_col(a, len(cols[0])) for a in (footprint, footprint_shell)]
+        _check(L.sif_catalogue_reserve_footprint(cat._p))
+        n = len(cols[0])
+        for k, name in enumerate(("footprint", "footprint_shell")):
+            C.memmove(getattr(cat._p.contents, name), fp[k].ctypes.data, n * 4)
     return cat
 
 
-from . import finders  # noqa: E402  (uses the types above)
+from . import finders, io  # noqa: E402  (use the types above)
 
 __all__ = ["system_init", "system_finalise", "init", "finalise", "Field", "field_from_numpy",
-           "Grid", "ChainMesh", "Catalogue", "catalogue_from_numpy", "finders", "real"]
+           "Grid", "ChainMesh", "Catalogue", "catalogue_from_numpy", "finders", "io", "real"]

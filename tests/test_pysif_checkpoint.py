@@ -137,7 +137,7 @@ def test_views_are_read_only_and_keep_their_owner():
     with pytest.raises(ValueError):
         pysif.field_from_numpy(x, x[:10], x)
     cat = pysif.catalogue_from_numpy([1.0, 2.0], [3.0, 4.0], [5.0, 6.0], [7.0, 8.0])
-    assert cat.n_voids == 2 and cat.to_numpy()["radius"].tolist() == [7.0, 8.0]
+    assert cat.n_voids == 2 and cat.to_numpy()["r"].tolist() == [7.0, 8.0]
 
 
 # ---- GPU --------------------------------------------------------------------
