@@ -107,6 +107,15 @@ SIF_API int sif_amd_scan_candidates(const float* d_delta,
   const uint64_t* d_mask_words, uint64_t total_cells, float threshold,
   float* d_out_delta, uint32_t* d_out_flat, uint64_t capacity, uint64_t* count);
 
+/* sif__finder_scan_refined with minima_only and no refinement
+ * (src/finder/utils.c:228-262, 406-552), the SIF_FINDER_SEEDS_MINIMA scan: of
+ * those cells, the local minima of the periodic n_cells^3 field -- strictly
+ * below the 26 neighbours, an equal neighbour of lower flat index winning --
+ * on their nodes, sorted (delta, flat). Same output contract. */
+SIF_API int sif_amd_scan_minima(const float* d_delta, const uint64_t* d_mask_words,
+  uint32_t n_cells, float threshold, float* d_out_delta, uint32_t* d_out_flat,
+  uint64_t capacity, uint64_t* count);
+
 /* --- sphere growth --- */
 /* sif__exodus_find_radius (src/finder/exodus_rescale.c:642-1027), unweighted,
  * one result per centre: radius (or -1), reason (0 ok, 1 empty, 2 beyond the

@@ -82,6 +82,7 @@ _sig("orc_mode_factor", f32, P, P, u32, u32, u32, u32)
 _sig("orc_fft_r2c", i32, P, u32, P)
 _sig("orc_fft_c2r", i32, P, u32, P)
 _sig("orc_scan_candidates", u64, P, P, u64, f32, P)
+_sig("orc_scan_minima", u64, P, u32, P, f32, P)
 _sig("orc_sort_candidates_ref", None, P, u64)
 _sig("orc_sort_candidates_flat", None, P, u64)
 _sig("orc_template_build", i32, C.POINTER(Template), u32, f32, f32, f32)
@@ -232,6 +233,20 @@ def scan_candidates(delta, threshold, mask=None, tie_rule=0):
     d = f32a(delta)
     buf = (Candidate * max(d.size, 1))()
     k = lib.orc_scan_candidates(_p(d), _p(mask), d.size, threshold, buf)
+    if tie_rule == 0:
+        lib.orc_sort_candidates_ref(buf, k)
+    else:
+        lib.orc_sort_candidates_flat(buf, k)
+    arr = np.ctypeslib.as_array(buf)[:k]
+    return arr["delta"].copy(), arr["flat_idx"].copy()
+
+
+def scan_minima(delta, n, threshold, mask=None, tie_rule=1):
+    """The SIF_FINDER_SEEDS_MINIMA candidates of an n^3 field, sorted."""
+    d = f32a(delta)
+    assert d.size == n ** 3
+    buf = (Candidate * max(d.size, 1))()
+    k = lib.orc_scan_minima(_p(d), n, _p(mask), threshold, buf)
     if tie_rule == 0:
         lib.orc_sort_candidates_ref(buf, k)
     else:

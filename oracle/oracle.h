@@ -188,6 +188,14 @@ typedef struct {
   uint64_t candidates, masked, proxy, overlap, witness, rescale_fail, accepted;
 } orc_rung_stats_t;
 
+/* utils.c:406-552 with minima_only and no refinement: the cells of
+ * orc_scan_candidates that are local minima of the periodic n^3 field. */
+uint64_t orc_scan_minima(const float* delta, uint32_t n, const uint64_t* mask,
+  float t, orc_candidate_t* out);
+
+/* search_opt bit 9 (SIF_FINDER_SEEDS_MINIMA): the candidates are the minima */
+#define ORC_SEEDS_MINIMA (1u << 9)
+
 /* fields[i] is the smoothed density contrast for the i-th radius of the
  * descending ladder (n_cells^3 floats each). tie_rule 0 = reference
  * quicksort, 1 = (delta, flat). use_witness mirrors the reference's witness

@@ -37,6 +37,49 @@ uint64_t orc_scan_candidates(const float* delta, const uint64_t* mask,
   return k;
 }
 
+/* --- minima: src/finder/utils.c:228-262, 406-552 (minima_only, no
+ * refinement) ---
+ *
+ * The candidates of SIF_FINDER_SEEDS_MINIMA: of the cells above, those whose
+ * value is below each of the 26 periodic neighbours', a neighbour of lower
+ * flat index winning a tie. On their nodes, at their own depth. */
+static int orc_is_minimum(const float* f, uint32_t n, uint64_t i) {
+  const uint64_t nn = (uint64_t)n * n;
+  const int64_t c[3] = {(int64_t)(i / nn), (int64_t)((i / n) % n), (int64_t)(i % n)};
+  const float v = f[i];
+  for (int a = -1; a <= 1; a++)
+    for (int b = -1; b <= 1; b++)
+      for (int d = -1; d <= 1; d++) {
+        if (!a && !b && !d)
+          continue;
+        const uint64_t x = (uint64_t)((c[0] + a + n) % n);
+        const uint64_t y = (uint64_t)((c[1] + b + n) % n);
+        const uint64_t z = (uint64_t)((c[2] + d + n) % n);
+        const uint64_t j = x * nn + y * n + z;
+        const float u = f[j];
+        if (j < i ? !(v < u) : !(v <= u))
+          return 0;
+      }
+  return 1;
+}
+
+uint64_t orc_scan_minima(const float* delta, uint32_t n, const uint64_t* mask,
+  float t, orc_candidate_t* out) {
+  const uint64_t total = (uint64_t)n * n * n;
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < total; i++) {
+    if (delta[i] <= t && !(mask && bit_get(mask, i)) && orc_is_minimum(delta, n, i)) {
+      if (out) {
+        out[k].delta = delta[i];
+        out[k].refined = 0;
+        out[k].flat_idx = i;
+      }
+      k++;
+    }
+  }
+  return k;
+}
+
 /* include/sif/utils/sort.h:56-99 (three-way partition, middle pivot, the
  * serial permutation that the task-parallel form also produces, :101-145),
  * with the predicate a.delta < b.delta of utils.c:24. */
@@ -941,7 +984,9 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
     orc_rung_stats_t st;
     memset(&st, 0, sizeof(st));
     const float* field = fields[i];
-    const uint64_t nc = orc_scan_candidates(field, mask, total, t, cand);
+    const uint64_t nc = (search_opt & ORC_SEEDS_MINIMA)
+                          ? orc_scan_minima(field, n, mask, t, cand)
+                          : orc_scan_candidates(field, mask, total, t, cand);
     st.candidates = nc;
     if (tie_rule == 0)
       orc_sort_candidates_ref(cand, nc);

@@ -96,6 +96,7 @@ _sig("sif_amd_spectrum_bytes", u64, u32)
 _sig("sif_amd_fft_r2c", i32, P, u32, P, P)
 _sig("sif_amd_fft_filter_c2r", i32, P, u32, f32, i32, f32, f32, i32, i32, P, P)
 _sig("sif_amd_scan_candidates", i32, P, P, u64, f32, P, P, u64, C.POINTER(u64))
+_sig("sif_amd_scan_minima", i32, P, P, u32, f32, P, P, u64, C.POINTER(u64))
 _sig("sif_amd_mark_spheres", i32, P, u32, f32, P, P, P, P, u32, f32)
 _sig("sif_amd_grid_profiles", i32, P, u32, f32, P, P, P, P, u32, f32, u32, P)
 _sig("sif_amd_grid_profiles_adjoint", i32, P, u32, f32, P, P, P, P, u32, f32, u32, P)

@@ -641,7 +641,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           break;
       }
       st = scan_candidates_flat(field, g_dev.mask.as<uint64_t>(), total, R.threshold,
-        g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap, &nc);
+        (R.opt & SIF_FINDER_SEEDS_MINIMA) ? n : 0u, g_dev.keys.as<uint32_t>(),
+        g_dev.flat.as<uint32_t>(), cand_cap, &nc);
       if (st != SIF_OK || nc <= cand_cap)
         break;
       cand_cap = (uint64_t)((double)nc * 1.2) + 1;
@@ -1151,11 +1152,11 @@ int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_
   return SIF_OK;
 }
 
-// exodus_finder.c:1112-1138: what this build can run.
+// exodus_finder.c:1112-1138: what this build can run. SIF_FINDER_SEEDS_MINIMA
+// runs (minima on their nodes, on the rung's own field).
 int check_options(sif_option opt) {
   const sif_option refine = opt & SIF__FINDER_REFINE_MASK;
-  const char* missing = (opt & SIF_FINDER_SEEDS_MINIMA)         ? "SIF_FINDER_SEEDS_MINIMA"
-                        : refine == SIF_FINDER_REFINE_GAUSSIAN  ? "SIF_FINDER_REFINE_GAUSSIAN"
+  const char* missing = refine == SIF_FINDER_REFINE_GAUSSIAN    ? "SIF_FINDER_REFINE_GAUSSIAN"
                         : refine == SIF_FINDER_REFINE_TRACERS   ? "SIF_FINDER_REFINE_TRACERS"
                         : refine                                ? "SIF_FINDER_REFINE_HILL"
                         : (opt & SIF_FINDER_EXTEND_REACH)       ? "SIF_FINDER_EXTEND_REACH"
@@ -1163,7 +1164,7 @@ int check_options(sif_option opt) {
                                                                 : nullptr;
   if (missing) {
     error_set(SIF_ERR_UNSUPPORTED,
-      "options: %s is not in this build (cells on their nodes, no displacement)", missing);
+      "options: %s is not in this build (seeds on their nodes, no displacement)", missing);
     return SIF_ERR_UNSUPPORTED;
   }
   return SIF_OK;

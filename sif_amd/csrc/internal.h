@@ -39,11 +39,13 @@ void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
 void build_window_axis(uint32_t n, int order, int mode, std::vector<double>& axis);
 
 // scan.hip
+// minima_n: 0 for every cell below the threshold, the grid's side for the
+// local minima among them (SIF_FINDER_SEEDS_MINIMA).
 int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
 // The same candidates in flat order, unsorted; then the sort on its own.
 int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
 int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n);
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
 

@@ -1,5 +1,7 @@
 // Candidate scan: every cell with delta <= threshold whose mask bit is clear,
-// compacted in cell order and sorted deepest first.
+// compacted in cell order and sorted deepest first. Under
+// SIF_FINDER_SEEDS_MINIMA only the local minima among them (on the 27 nodes
+// around each) are kept.
 //
 // Replaces sif__finder_scan_candidates and sort_candidates
 // (src/finder/utils.c:123-213, include/sif/utils/sort.h:56-145) of
@@ -32,9 +34,39 @@ constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
 constexpr uint64_t SC_TILE = (uint64_t)SC_THREADS * SC_ITEMS;
 
-// Bit j of the result: cell first + j is a candidate (delta <= t, mask clear).
+// Node i is a local minimum of the periodic n^3 field: strictly below each of
+// its 26 neighbours, an equal neighbour winning when its flat index is lower,
+// so of a run of equal values only the lowest-indexed node is a minimum
+// (utils.c:228-262). A NaN neighbour makes no minimum. Only below-threshold
+// cells come here, a few per cent of the grid; their stencils are L2 hits.
+__device__ __forceinline__ bool node_is_minimum(const float* __restrict__ f, uint32_t n,
+  uint64_t i) {
+  const uint64_t nn = (uint64_t)n * n;
+  const uint32_t ix = (uint32_t)(i / nn), iy = (uint32_t)((i / n) % n), iz = (uint32_t)(i % n);
+  const float v = f[i];
+  const uint32_t xs[3] = {ix ? ix - 1 : n - 1, ix, ix + 1 == n ? 0 : ix + 1};
+  const uint32_t ys[3] = {iy ? iy - 1 : n - 1, iy, iy + 1 == n ? 0 : iy + 1};
+  const uint32_t zs[3] = {iz ? iz - 1 : n - 1, iz, iz + 1 == n ? 0 : iz + 1};
+  bool ok = true;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      const uint64_t row = (uint64_t)xs[a] * nn + (uint64_t)ys[b] * n;
+      for (int c = 0; c < 3; c++) {
+        const uint64_t j = row + zs[c];
+        if (a == 1 && b == 1 && c == 1)
+          continue;
+        const float u = f[j];
+        ok = ok && (j < i ? v < u : v <= u);
+      }
+    }
+  return ok;
+}
+
+// Bit j of the result: cell first + j is a candidate (delta <= t, mask clear,
+// and a local minimum when minima_n, the grid's side, is not 0).
 __device__ __forceinline__ uint32_t cand_bits(const float* __restrict__ delta,
-  const uint64_t* __restrict__ mask, uint64_t total, float t, uint64_t first) {
+  const uint64_t* __restrict__ mask, uint64_t total, float t, uint64_t first,
+  uint32_t minima_n) {
   uint32_t bits = 0;
   if (first + SC_ITEMS <= total) {
     const float4* d4 = reinterpret_cast<const float4*>(delta + first);
@@ -54,6 +86,12 @@ __device__ __forceinline__ uint32_t cand_bits(const float* __restrict__ delta,
         bits |= 1u << j;
     }
   }
+  if (minima_n)
+    for (uint32_t b = bits; b; b &= b - 1) {
+      const int j = __ffs(b) - 1;
+      if (!node_is_minimum(delta, minima_n, first + j))
+        bits &= ~(1u << j);
+    }
   return bits;
 }
 
@@ -83,9 +121,9 @@ __device__ __forceinline__ uint32_t sc_block_scan(uint32_t v, uint32_t* total) {
 
 __global__ void __launch_bounds__(SC_THREADS) cand_count_kernel(
   const float* __restrict__ delta, const uint64_t* __restrict__ mask, uint64_t total,
-  float t, uint64_t* __restrict__ tile_counts) {
+  float t, uint32_t minima_n, uint64_t* __restrict__ tile_counts) {
   const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
-  const uint32_t c = (uint32_t)__popc(cand_bits(delta, mask, total, t, first));
+  const uint32_t c = (uint32_t)__popc(cand_bits(delta, mask, total, t, first, minima_n));
   uint32_t all;
   sc_block_scan(c, &all);
   if (threadIdx.x == 0)
@@ -94,10 +132,10 @@ __global__ void __launch_bounds__(SC_THREADS) cand_count_kernel(
 
 __global__ void __launch_bounds__(SC_THREADS) cand_write_kernel(
   const float* __restrict__ delta, const uint64_t* __restrict__ mask, uint64_t total,
-  float t, const uint64_t* __restrict__ tile_offsets, uint32_t* __restrict__ keys,
-  uint32_t* __restrict__ flat) {
+  float t, uint32_t minima_n, const uint64_t* __restrict__ tile_offsets,
+  uint32_t* __restrict__ keys, uint32_t* __restrict__ flat) {
   const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
-  uint32_t bits = cand_bits(delta, mask, total, t, first);
+  uint32_t bits = cand_bits(delta, mask, total, t, first, minima_n);
   uint32_t all;
   uint64_t pos = tile_offsets[blockIdx.x] + sc_block_scan((uint32_t)__popc(bits), &all);
   while (bits) {
@@ -124,9 +162,12 @@ DeviceBuffer g_tiles, g_tile_pos, g_count;
 // Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)
 // and their number to *count; keys are the order-preserving integer images of
 // the depths. When the count exceeds `cap` nothing is written and the caller
-// grows the buffers and calls again.
+// grows the buffers and calls again. minima_n: 0 for every cell below the
+// threshold; the grid's side (total = minima_n^3) for the local minima alone
+// (sif__finder_scan_refined with minima_only and no refinement,
+// utils.c:406-552).
 int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
   if (total >= 0x80000000ull) {
     error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^31 - 1)",
       (unsigned long long)total);
@@ -134,6 +175,11 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
   }
   if (reinterpret_cast<uintptr_t>(delta) % 16 != 0) {
     error_set(SIF_ERR_INVALID, "scan: the grid must be 16-byte aligned");
+    return SIF_ERR_INVALID;
+  }
+  if (minima_n && (uint64_t)minima_n * minima_n * minima_n != total) {
+    error_set(SIF_ERR_INVALID, "scan: minima on a %u^3 grid of %llu cells", minima_n,
+      (unsigned long long)total);
     return SIF_ERR_INVALID;
   }
   const uint64_t tiles = (total + SC_TILE - 1) / SC_TILE;
@@ -149,7 +195,7 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SCAN);
   cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
-    g_tiles.as<uint64_t>());
+    minima_n, g_tiles.as<uint64_t>());
   SIF_HIP(hipGetLastError());
   st = exclusive_scan_u64(g_tiles.as<uint64_t>(), tiles, g_tile_pos.as<uint64_t>(),
     g_count.as<uint64_t>());
@@ -161,7 +207,7 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
   *count = n;
   if (n <= cap && n > 0) {
     cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
-      g_tile_pos.as<uint64_t>(), keys, flat);
+      minima_n, g_tile_pos.as<uint64_t>(), keys, flat);
     SIF_HIP(hipGetLastError());
   }
   timing_end(SIF_AMD_STAGE_SCAN);
@@ -185,8 +231,8 @@ int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n) {
 
 // The sorted candidates (sif__finder_scan_candidates + sort_candidates).
 int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
-  int st = scan_candidates_flat(delta, mask, total, t, keys, flat, cap, count);
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
+  int st = scan_candidates_flat(delta, mask, total, t, minima_n, keys, flat, cap, count);
   if (st != SIF_OK || *count > cap)
     return st;
   return sort_candidates(keys, flat, *count);
@@ -219,8 +265,31 @@ int sif_amd_scan_candidates(const float* delta, const uint64_t* mask_words,
   // out_delta doubles as the key buffer (same size); keys become depths at
   // the end.
   uint32_t* keys = reinterpret_cast<uint32_t*>(out_delta);
-  st = scan_candidates(delta, mask_words, total_cells, threshold, keys, out_flat,
+  st = scan_candidates(delta, mask_words, total_cells, threshold, 0, keys, out_flat,
     capacity, count);
+  if (st != SIF_OK)
+    return st;
+  if (*count <= capacity)
+    st = keys_to_delta(keys, *count, out_delta);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+int sif_amd_scan_minima(const float* delta, const uint64_t* mask_words, uint32_t n_cells,
+  float threshold, float* out_delta, uint32_t* out_flat, uint64_t capacity, uint64_t* count) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!delta || !out_delta || !out_flat || !count || n_cells == 0) {
+    error_set(SIF_ERR_INVALID, "sif_amd_scan_minima: invalid arguments");
+    return SIF_ERR_INVALID;
+  }
+  uint32_t* keys = reinterpret_cast<uint32_t*>(out_delta);
+  const uint64_t total = (uint64_t)n_cells * n_cells * n_cells;
+  st = scan_candidates(delta, mask_words, total, threshold, n_cells, keys, out_flat, capacity,
+    count);
   if (st != SIF_OK)
     return st;
   if (*count <= capacity)

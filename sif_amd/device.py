@@ -100,6 +100,20 @@ def scan_candidates(ddelta, total, threshold, dmask=None, capacity=None):
     return d_delta.get()[:k], d_flat.get()[:k]
 
 
+def scan_minima(ddelta, n, threshold, dmask=None, capacity=None):
+    """The SIF_FINDER_SEEDS_MINIMA candidates of an n^3 device field:
+    (depths, flat indices), sorted (depth, flat)."""
+    total = n ** 3
+    cap = capacity or total
+    d_delta = DeviceArray((max(cap, 1),), np.float32)
+    d_flat = DeviceArray((max(cap, 1),), np.uint32)
+    count = C.c_uint64()
+    check(lib.sif_amd_scan_minima(ddelta.ptr, dmask.ptr if dmask else None, n, threshold,
+                                  d_delta.ptr, d_flat.ptr, cap, C.byref(count)))
+    k = count.value
+    return d_delta.get()[:k], d_flat.get()[:k]
+
+
 def find_radius(mesh, centres, rmin, r_search, threshold, vol_factor, mass_factor=None):
     """Grows spheres at (k, 3) float32 centres; returns radius, reason, n_in.
     On a weighted mesh the density tests use mass_factor."""

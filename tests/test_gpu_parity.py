@@ -347,8 +347,11 @@ def _exodus_setup(dev, particles, n=32):
     return g
 
 
-@pytest.mark.parametrize("overlap,opts", [(0.0, 0), (0.0, 1 << 18), (0.5, 0)])
+@pytest.mark.parametrize("overlap,opts", [(0.0, 0), (0.0, 1 << 18), (0.5, 0), (0.0, 1 << 9),
+                                          (0.5, 1 << 9)])
 def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap, opts):
+    """Cells, Lagrangian overlap, a partial overlap, and the minima seeds
+    (SIF_FINDER_SEEDS_MINIMA, bit 9)."""
     import oracle
     D, dx, dy, dz = dev
     x, y, z = particles
@@ -365,7 +368,7 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
     for witness in (True, False):
         ocat, onin, orung, ostats = oracle.exodus_run(
             list(fields), n, BOX, om, len(x), radii, THRESH, overlap=overlap,
-            lagrangian=lag, tie_rule=1, witness=witness)
+            lagrangian=lag, search_opt=opts & (1 << 9), tie_rule=1, witness=witness)
         np.testing.assert_array_equal(cat.view(np.uint32), ocat.view(np.uint32))
         np.testing.assert_array_equal(nin, onin)
     assert len(cat) > 0
