@@ -195,6 +195,9 @@ uint64_t orc_scan_minima(const float* delta, uint32_t n, const uint64_t* mask,
 
 /* search_opt bit 9 (SIF_FINDER_SEEDS_MINIMA): the candidates are the minima */
 #define ORC_SEEDS_MINIMA (1u << 9)
+/* search_opt bit 21 (SIF_FINDER_EXTEND_REACH): a growth that ends beyond the
+ * rung's reach is measured again at the extended levels */
+#define ORC_EXTEND_REACH (1u << 21)
 
 /* fields[i] is the smoothed density contrast for the i-th radius of the
  * descending ladder (n_cells^3 floats each). tie_rule 0 = reference

@@ -979,6 +979,39 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   float max_r = 0.0f;
   int status = 0;
 
+  /* exodus_finder.c:234-244, 1670-1702, 1342-1353: the extended reach's
+   * levels, from the run's largest search radius down past the bottom rung
+   * in steps of 1.5; level k's core 1.2 below level k - 1's reach */
+  uint32_t n_ext = 0;
+  float ext_reach[32], ext_rmin[32];
+  orc_template_t etpl[32];
+  if (search_opt & ORC_EXTEND_REACH) {
+    float r_max = radii[0];
+    for (uint32_t i = 1; i < n_radii; i++)
+      if (radii[i] > r_max)
+        r_max = radii[i];
+    const float r_ext = r_max + fmaxf((sf - 1.0f) * r_max, 3.0f * l);
+    const float r_low = radii[n_radii - 1];
+    while (n_ext < 32u && (n_ext == 0 || r_ext / powf(1.5f, (float)(n_ext - 1)) > r_low))
+      n_ext++;
+    for (uint32_t k = 0; k < n_ext; k++)
+      ext_reach[k] = r_ext / powf(1.5f, (float)(n_ext - 1 - k));
+    for (uint32_t k = 0; k < n_ext; k++) {
+      ext_rmin[k] = (k ? ext_reach[k - 1] : ext_reach[0] / 1.5f) / 1.2f;
+      if (orc_template_build(&etpl[k], mesh->n_cells, mesh->cell_length, ext_rmin[k],
+            ext_reach[k]) != 0) {
+        for (uint32_t j = 0; j < k; j++)
+          orc_template_free(&etpl[j]);
+        free(mask);
+        free(cand);
+        free(vi);
+        free(ev);
+        free(bidx);
+        return -3;
+      }
+    }
+  }
+
   for (uint32_t i = 0; i < n_radii && status == 0; i++) {
     const float radius = radii[i];
     orc_rung_stats_t st;
@@ -1005,6 +1038,13 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
     const int32_t proxy =
       (int32_t)(scale * radius * (1.0f - f) / l) - 1;
     const float rc2 = rmin * rmin;
+    /* exodus_finder.c:1824-1832: the first level this rung enters */
+    uint32_t ext0 = n_ext;
+    for (uint32_t k = 0; k < n_ext && ext0 == n_ext; k++)
+      if (ext_reach[k] >= 1.2f * r_search)
+        ext0 = k;
+    if (ext0 == n_ext && n_ext > 0 && ext_reach[n_ext - 1] > r_search)
+      ext0 = n_ext - 1;
 
     uint64_t k = 0;
     while (k < nc) {
@@ -1055,11 +1095,21 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
         }
         uint8_t reason;
         uint64_t n_in = 0;
-        const float r = weighted
+        float r = weighted
           ? orc_find_radius_weighted(mesh, cx, cy, cz, r_search, t, vol, mass,
               rmin, &tpl, window, &reason, &n_in)
           : orc_find_radius(mesh, cx, cy, cz, r_search, t, vol, rmin, &tpl,
               window, &reason, &n_in);
+        /* exodus_finder.c:2060-2072: beyond the reach, the next level */
+        for (uint32_t x = ext0; x < n_ext && r < 0.0f && reason == ORC_RESCALE_BEYOND;
+             x++) {
+          n_in = 0;
+          r = weighted
+            ? orc_find_radius_weighted(mesh, cx, cy, cz, ext_reach[x], t, vol,
+                mass, ext_rmin[x], &etpl[x], window, &reason, &n_in)
+            : orc_find_radius(mesh, cx, cy, cz, ext_reach[x], t, vol,
+                ext_rmin[x], &etpl[x], window, &reason, &n_in);
+        }
         e->status = r < 0.0f ? 2 : 3;
         e->r = r;
         e->n_in = n_in;
@@ -1120,6 +1170,8 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
     if (stats)
       stats[i] = st;
   }
+  for (uint32_t k = 0; k < n_ext; k++)
+    orc_template_free(&etpl[k]);
 
   free(mask);
   free(cand);

@@ -523,6 +523,32 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     (R.opt & SIF_FINDER_LAGRANGIAN_OVERLAP) ? (float)cbrt(1.0 + (double)R.threshold) : 1.0f;
   const float sf = sif__finder_search_factor(R.opt);
 
+  // SIF_FINDER_EXTEND_REACH (exodus_finder.c:1670-1702): a crossing past a
+  // rung's reach is measured again, level by level, the reach growing by
+  // EXT_GROWTH from the bottom rung to the largest search radius of the run;
+  // level k's core ends EXT_SLACK below level k - 1's reach.
+  constexpr float EXT_GROWTH = 1.5f, EXT_SLACK = 1.2f;
+  constexpr uint32_t EXT_MAX_LEVELS = 32;
+  std::vector<float> ext_reach, ext_rmin;
+  if (R.opt & SIF_FINDER_EXTEND_REACH) {
+    float r_max = R.radii[0];
+    for (float r : R.radii)
+      r_max = r > r_max ? r : r_max;
+    const float r_ext = r_max + fmaxf((sf - 1.0f) * r_max, 3.0f * l);
+    const float r_low = R.radii.back();
+    uint32_t n_ext = 0;
+    while (n_ext < EXT_MAX_LEVELS &&
+           (n_ext == 0 || r_ext / powf(EXT_GROWTH, (float)(n_ext - 1)) > r_low))
+      n_ext++;
+    ext_reach.resize(n_ext);
+    ext_rmin.resize(n_ext);
+    for (uint32_t k = 0; k < n_ext; k++)
+      ext_reach[k] = r_ext / powf(EXT_GROWTH, (float)(n_ext - 1 - k));
+    for (uint32_t k = 0; k < n_ext; k++)
+      ext_rmin[k] = (k ? ext_reach[k - 1] : ext_reach[0] / EXT_GROWTH) / EXT_SLACK;
+  }
+  const uint32_t n_ext = (uint32_t)ext_reach.size();
+
   int st = exodus_prepare_spectrum(R.d_delta, n);
   const double t_lut = now_ms();
   if (st == SIF_OK)
@@ -678,6 +704,14 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if (i > 0 && R.radii[i - 1] > r_search)
       r_search = R.radii[i - 1];
     const int32_t proxy = (int32_t)(scale * radius * (1.0f - R.f) / l) - 1;
+    // the first extended level this rung enters (exodus_finder.c:1824-1832):
+    // the lowest EXT_SLACK past its own reach, or the top if none is
+    uint32_t ext0 = n_ext;
+    for (uint32_t k = 0; k < n_ext && ext0 == n_ext; k++)
+      if (ext_reach[k] >= EXT_SLACK * r_search)
+        ext0 = k;
+    if (ext0 == n_ext && n_ext > 0 && ext_reach[n_ext - 1] > r_search)
+      ext0 = n_ext - 1;
 
     // prefilter against the catalogue as the rung begins, in flat order (the
     // verdicts do not depend on the order; neighbouring candidates share
@@ -813,6 +847,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, rmin, r_search,
           R.threshold, vol, B.d_rad.as<float>(), B.d_reason.as<uint32_t>(),
           B.d_nin.as<uint64_t>(), algo ? B.d_algo.as<uint64_t>() : nullptr, &mw);
+      if (e == SIF_OK && ext0 < n_ext)
+        e = grow_extend(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
+          B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, ext_rmin.data(),
+          ext_reach.data(), ext0, n_ext, R.threshold, vol, B.d_rad.as<float>(),
+          B.d_reason.as<uint32_t>(), B.d_nin.as<uint64_t>(), &mw);
       if (e == SIF_OK)
         e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
           B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
@@ -1153,13 +1192,13 @@ int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_
 }
 
 // exodus_finder.c:1112-1138: what this build can run. SIF_FINDER_SEEDS_MINIMA
-// runs (minima on their nodes, on the rung's own field).
+// (minima on their nodes, on the rung's own field) and
+// SIF_FINDER_EXTEND_REACH run.
 int check_options(sif_option opt) {
   const sif_option refine = opt & SIF__FINDER_REFINE_MASK;
   const char* missing = refine == SIF_FINDER_REFINE_GAUSSIAN    ? "SIF_FINDER_REFINE_GAUSSIAN"
                         : refine == SIF_FINDER_REFINE_TRACERS   ? "SIF_FINDER_REFINE_TRACERS"
                         : refine                                ? "SIF_FINDER_REFINE_HILL"
-                        : (opt & SIF_FINDER_EXTEND_REACH)       ? "SIF_FINDER_EXTEND_REACH"
                         : (opt & SIF_FINDER_DISPLACE)           ? "SIF_FINDER_DISPLACE"
                                                                 : nullptr;
   if (missing) {

@@ -21,6 +21,7 @@
 // buffer (the reference resolves a window of W bins too, and its result does
 // not depend on W). A single bin too full for the buffer is split into 64
 // sub-ranges of squared distance, resolved top down.
+#include <memory>
 #include <vector>
 
 #include "common.h"
@@ -315,7 +316,15 @@ struct TemplateCache {
   uint32_t count = 0;
   DeviceBuffer cells, bounds;
 };
-TemplateCache g_tpl;
+// The templates on the device, one per (mesh, core, reach): a rung's own and,
+// under SIF_FINDER_EXTEND_REACH, one per level of the extended reach, each
+// built once per run. At most TEMPLATE_SLOTS are kept; the oldest goes first.
+constexpr size_t TEMPLATE_SLOTS = 48;
+std::vector<std::unique_ptr<TemplateCache>> g_tpls;
+// the scratch of the extended-reach passes: the candidates still beyond their
+// reach, gathered, their results, and the count
+DeviceBuffer g_ext_idx, g_ext_cx, g_ext_cy, g_ext_cz, g_ext_rad, g_ext_reason, g_ext_nin,
+  g_ext_count;
 DeviceBuffer g_pf_counts;
 // the split first pass's partial histograms and counts
 DeviceBuffer g_part_bins, g_part_counts;
@@ -328,29 +337,40 @@ constexpr uint32_t WAVE_SLOTS = 4096;
 constexpr uint32_t MAX_SLICES = 16;
 }  // namespace
 
-static int template_device(uint32_t mesh_n, float l, float rmin, float r_search) {
-  if (g_tpl.mesh_n == mesh_n && g_tpl.l == l && g_tpl.rmin == rmin &&
-      g_tpl.r_search == r_search)
-    return SIF_OK;
+static int template_device(uint32_t mesh_n, float l, float rmin, float r_search,
+  const TemplateCache** out) {
+  for (const auto& t : g_tpls)
+    if (t->mesh_n == mesh_n && t->l == l && t->rmin == rmin && t->r_search == r_search) {
+      *out = t.get();
+      return SIF_OK;
+    }
   std::vector<int4> cells;
   std::vector<float2> bounds;
   int st = template_build(mesh_n, l, rmin, r_search, cells, bounds);
-  if (st == SIF_OK)
-    st = g_tpl.cells.reserve(cells.size() * sizeof(int4));
-  if (st == SIF_OK)
-    st = g_tpl.bounds.reserve(bounds.size() * sizeof(float2));
   if (st != SIF_OK)
     return st;
-  SIF_HIP(hipMemcpyAsync(g_tpl.cells.ptr, cells.data(), cells.size() * sizeof(int4),
+  if (g_tpls.size() == TEMPLATE_SLOTS) {
+    SIF_HIP(hipStreamSynchronize(stream()));  // no launch still reads it
+    g_tpls.erase(g_tpls.begin());
+  }
+  auto t = std::make_unique<TemplateCache>();
+  st = t->cells.reserve(cells.size() * sizeof(int4));
+  if (st == SIF_OK)
+    st = t->bounds.reserve(bounds.size() * sizeof(float2));
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemcpyAsync(t->cells.ptr, cells.data(), cells.size() * sizeof(int4),
     hipMemcpyHostToDevice, stream()));
-  SIF_HIP(hipMemcpyAsync(g_tpl.bounds.ptr, bounds.data(), bounds.size() * sizeof(float2),
+  SIF_HIP(hipMemcpyAsync(t->bounds.ptr, bounds.data(), bounds.size() * sizeof(float2),
     hipMemcpyHostToDevice, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
-  g_tpl.mesh_n = mesh_n;
-  g_tpl.l = l;
-  g_tpl.rmin = rmin;
-  g_tpl.r_search = r_search;
-  g_tpl.count = (uint32_t)cells.size();
+  t->mesh_n = mesh_n;
+  t->l = l;
+  t->rmin = rmin;
+  t->r_search = r_search;
+  t->count = (uint32_t)cells.size();
+  *out = t.get();
+  g_tpls.push_back(std::move(t));
   return SIF_OK;
 }
 
@@ -361,7 +381,8 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   uint64_t* algo_bytes, const MeshWeights* weights) {
   const float l = box / (float)mesh_n;
   const bool weighted = weights && weights->w;
-  int st = template_device(mesh_n, l, rmin, r_search);
+  const TemplateCache* tpl = nullptr;
+  int st = template_device(mesh_n, l, rmin, r_search, &tpl);
   if (st != SIF_OK)
     return st;
   if (n == 0)
@@ -374,9 +395,9 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   P.mesh_n = (int32_t)mesh_n;
   P.box = box;
   P.inv_l = 1.0f / l;
-  P.tcell = g_tpl.cells.as<int4>();
-  P.tbound = g_tpl.bounds.as<float2>();
-  P.tcount = g_tpl.count;
+  P.tcell = tpl->cells.as<int4>();
+  P.tbound = tpl->bounds.as<float2>();
+  P.tcount = tpl->count;
   P.rmin = rmin;
   P.r_search = r_search;
   P.threshold = threshold;
@@ -426,6 +447,81 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
     grow_kernel<false><<<(unsigned)blocks, 64 * GROW_WAVES, lds_bytes, stream()>>>(P);
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_GROW);
+  return SIF_OK;
+}
+
+// --- SIF_FINDER_EXTEND_REACH: the candidates still beyond their reach, again
+// at the next level (exodus_finder.c:2060-2072) ---
+
+__global__ void ext_gather_kernel(const float* __restrict__ radius,
+  const uint32_t* __restrict__ reason, uint32_t n, const float* __restrict__ cx,
+  const float* __restrict__ cy, const float* __restrict__ cz, uint32_t* __restrict__ idx,
+  float* __restrict__ gx, float* __restrict__ gy, float* __restrict__ gz,
+  uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(radius[i] < 0.0f) || reason[i] != R_BEYOND)
+    return;
+  // the order of the gathered list is free: every candidate's result is its own
+  const uint32_t k = atomicAdd(count, 1u);
+  idx[k] = i;
+  gx[k] = cx[i];
+  gy[k] = cy[i];
+  gz[k] = cz[i];
+}
+
+__global__ void ext_scatter_kernel(const uint32_t* __restrict__ idx, uint32_t m,
+  const float* __restrict__ gr, const uint32_t* __restrict__ greason,
+  const uint64_t* __restrict__ gnin, float* __restrict__ radius, uint32_t* __restrict__ reason,
+  uint64_t* __restrict__ n_in) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m)
+    return;
+  const uint32_t i = idx[k];
+  radius[i] = gr[k];
+  reason[i] = greason[k];
+  if (n_in)
+    n_in[i] = gnin[k];
+}
+
+int grow_extend(const uint64_t* offsets, const float* mx, const float* my, const float* mz,
+  uint32_t mesh_n, float box, const float* cx, const float* cy, const float* cz, uint32_t n,
+  const float* ext_rmin, const float* ext_reach, uint32_t first, uint32_t levels,
+  float threshold, float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
+  const MeshWeights* weights) {
+  if (n == 0 || first >= levels)
+    return SIF_OK;
+  int st = g_ext_idx.reserve((size_t)n * 4);
+  if (st == SIF_OK) st = g_ext_cx.reserve((size_t)n * 4);
+  if (st == SIF_OK) st = g_ext_cy.reserve((size_t)n * 4);
+  if (st == SIF_OK) st = g_ext_cz.reserve((size_t)n * 4);
+  if (st == SIF_OK) st = g_ext_rad.reserve((size_t)n * 4);
+  if (st == SIF_OK) st = g_ext_reason.reserve((size_t)n * 4);
+  if (st == SIF_OK) st = g_ext_nin.reserve((size_t)n * 8);
+  if (st == SIF_OK) st = g_ext_count.reserve(4);
+  if (st != SIF_OK)
+    return st;
+  for (uint32_t e = first; e < levels; e++) {
+    SIF_HIP(hipMemsetAsync(g_ext_count.ptr, 0, 4, stream()));
+    ext_gather_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(radius, reason, n, cx, cy, cz,
+      g_ext_idx.as<uint32_t>(), g_ext_cx.as<float>(), g_ext_cy.as<float>(),
+      g_ext_cz.as<float>(), g_ext_count.as<uint32_t>());
+    SIF_HIP(hipGetLastError());
+    uint32_t m = 0;
+    SIF_HIP(hipMemcpyAsync(&m, g_ext_count.ptr, 4, hipMemcpyDeviceToHost, stream()));
+    SIF_HIP(hipStreamSynchronize(stream()));
+    if (m == 0)
+      break;
+    st = grow_batch(offsets, mx, my, mz, mesh_n, box, g_ext_cx.as<float>(),
+      g_ext_cy.as<float>(), g_ext_cz.as<float>(), m, ext_rmin[e], ext_reach[e], threshold,
+      vol_factor, g_ext_rad.as<float>(), g_ext_reason.as<uint32_t>(),
+      g_ext_nin.as<uint64_t>(), nullptr, weights);
+    if (st != SIF_OK)
+      return st;
+    ext_scatter_kernel<<<grid_for(m, 256), 256, 0, stream()>>>(g_ext_idx.as<uint32_t>(), m,
+      g_ext_rad.as<float>(), g_ext_reason.as<uint32_t>(), g_ext_nin.as<uint64_t>(), radius,
+      reason, n_in);
+    SIF_HIP(hipGetLastError());
+  }
   return SIF_OK;
 }
 

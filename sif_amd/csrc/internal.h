@@ -79,6 +79,15 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* cz, uint32_t n, float rmin, float r_search, float threshold,
   float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
   uint64_t* algo_bytes = nullptr, const MeshWeights* weights = nullptr);
+// SIF_FINDER_EXTEND_REACH (exodus_finder.c:2060-2072): the candidates whose
+// growth ended beyond its reach, grown again at the levels first .. levels-1
+// (core ext_rmin[e], reach ext_reach[e], host arrays) until none is; their
+// radius, reason and count are overwritten in place.
+int grow_extend(const uint64_t* offsets, const float* mx, const float* my, const float* mz,
+  uint32_t mesh_n, float box, const float* cx, const float* cy, const float* cz, uint32_t n,
+  const float* ext_rmin, const float* ext_reach, uint32_t first, uint32_t levels,
+  float threshold, float vol_factor, float* radius, uint32_t* reason, uint64_t* n_in,
+  const MeshWeights* weights);
 int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
   float* cx, float* cy, float* cz);
 // keep[k] and the refusal counts (masked, proxy, overlap) added to d_counts.

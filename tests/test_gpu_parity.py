@@ -348,10 +348,11 @@ def _exodus_setup(dev, particles, n=32):
 
 
 @pytest.mark.parametrize("overlap,opts", [(0.0, 0), (0.0, 1 << 18), (0.5, 0), (0.0, 1 << 9),
-                                          (0.5, 1 << 9)])
+                                          (0.5, 1 << 9), (1.0, (1 << 9) | (1 << 21))])
 def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap, opts):
-    """Cells, Lagrangian overlap, a partial overlap, and the minima seeds
-    (SIF_FINDER_SEEDS_MINIMA, bit 9)."""
+    """Cells, Lagrangian overlap, a partial overlap, the minima seeds
+    (SIF_FINDER_SEEDS_MINIMA, bit 9) and the extended reach
+    (SIF_FINDER_EXTEND_REACH, bit 21)."""
     import oracle
     D, dx, dy, dz = dev
     x, y, z = particles
@@ -368,7 +369,8 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
     for witness in (True, False):
         ocat, onin, orung, ostats = oracle.exodus_run(
             list(fields), n, BOX, om, len(x), radii, THRESH, overlap=overlap,
-            lagrangian=lag, search_opt=opts & (1 << 9), tie_rule=1, witness=witness)
+            lagrangian=lag, search_opt=opts & ((1 << 9) | (1 << 21)), tie_rule=1,
+            witness=witness)
         np.testing.assert_array_equal(cat.view(np.uint32), ocat.view(np.uint32))
         np.testing.assert_array_equal(nin, onin)
     assert len(cat) > 0
@@ -379,6 +381,8 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
         rs = rung + max(0.5 * rung, 3.0 * BOX / n)
         if k > 0:
             rs = max(rs, float(radii[k - 1]))
+        if opts & (1 << 21):  # the extended reach: up to the top rung's
+            rs = max(rs, float(radii[0]) + max(0.5 * float(radii[0]), 3.0 * BOX / n))
         want = oracle.brute_force_radius(x, y, z, BOX, (cx, cy, cz), rung, rs, THRESH)
         assert abs(r - want) <= 1e-5 * want
 
