@@ -54,7 +54,8 @@ class Catalogue(C.Structure):
 
 class RungStats(C.Structure):
     _fields_ = [("candidates", u64), ("masked", u64), ("proxy", u64), ("overlap", u64),
-                ("witness", u64), ("rescale_fail", u64), ("accepted", u64)]
+                ("witness", u64), ("rescale_fail", u64), ("accepted", u64),
+                ("displaced", u64)]
 
 
 def _sig(name, res, *args):

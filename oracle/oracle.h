@@ -185,7 +185,8 @@ typedef struct {
 } orc_catalogue_t;
 
 typedef struct {
-  uint64_t candidates, masked, proxy, overlap, witness, rescale_fail, accepted;
+  uint64_t candidates, masked, proxy, overlap, witness, rescale_fail, accepted,
+    displaced;
 } orc_rung_stats_t;
 
 /* utils.c:406-552 with minima_only and no refinement: the cells of
@@ -198,6 +199,9 @@ uint64_t orc_scan_minima(const float* delta, uint32_t n, const uint64_t* mask,
 /* search_opt bit 21 (SIF_FINDER_EXTEND_REACH): a growth that ends beyond the
  * rung's reach is measured again at the extended levels */
 #define ORC_EXTEND_REACH (1u << 21)
+/* search_opt bit 22 (SIF_FINDER_DISPLACE): a void is refused only by a larger
+ * one and displaces the smaller ones it collides with */
+#define ORC_DISPLACE (1u << 22)
 
 /* fields[i] is the smoothed density contrast for the i-th radius of the
  * descending ladder (n_cells^3 floats each). tie_rule 0 = reference

@@ -693,9 +693,10 @@ int orc_overlap_quick(const uint64_t* mask, uint32_t n, uint32_t ix,
   return 0;
 }
 
-/* utils.c:965-1033, set form */
-void orc_mark_sphere(uint64_t* mask, float cx, float cy, float cz, float r,
-  uint32_t n, float l) {
+/* utils.c:965-1039: the nodes of a sphere set (sif__mark_sphere) or cleared
+ * (sif__clear_sphere) */
+static void paint_nodes(uint64_t* mask, float cx, float cy, float cz, float r,
+  uint32_t n, float l, int set) {
   const float r2 = r * r;
   const float inv_l = 1.0f / l;
   const float L = (float)n * l;
@@ -726,11 +727,19 @@ void orc_mark_sphere(uint64_t* mask, float cx, float cy, float cz, float r,
         if (dxy2 + ddz * ddz <= r2) {
           const uint64_t cell =
             ((uint64_t)wx * n + wy) * (uint64_t)n + wrap_pbc(gz, n);
-          mask[cell >> 6] |= 1ull << (cell & 63);
+          if (set)
+            mask[cell >> 6] |= 1ull << (cell & 63);
+          else
+            mask[cell >> 6] &= ~(1ull << (cell & 63));
         }
       }
     }
   }
+}
+
+void orc_mark_sphere(uint64_t* mask, float cx, float cy, float cz, float r,
+  uint32_t n, float l) {
+  paint_nodes(mask, cx, cy, cz, r, n, l, 1);
 }
 
 /* --- void index: a cell linked list over a 32^3 grid,
@@ -765,6 +774,32 @@ static void vindex_insert(void_index_t* vi, uint64_t k, float x, float y,
     vcell(z, vi->inv_l);
   vi->next[k] = vi->head[c];
   vi->head[c] = (int32_t)k;
+}
+
+/* cell_linked_list.c:212-232: unlinked from its cell's chain */
+static void vindex_remove(void_index_t* vi, uint64_t k, float x, float y, float z) {
+  const int32_t c =
+    (vcell(x, vi->inv_l) * VOID_CELLS + vcell(y, vi->inv_l)) * VOID_CELLS +
+    vcell(z, vi->inv_l);
+  int32_t* link = &vi->head[c];
+  while (*link != -1 && *link != (int32_t)k)
+    link = &vi->next[*link];
+  if (*link != -1)
+    *link = vi->next[k];
+}
+
+/* The window of 32^3 cells within `reach` of x, whole along an axis it spans
+ * (exodus_finder.c:706-717) */
+static void vwindow(const void_index_t* vi, const float x[3], float reach,
+  int32_t lo[3], int32_t hi[3]) {
+  for (int a = 0; a < 3; a++) {
+    lo[a] = (int32_t)floorf((x[a] - reach) * vi->inv_l);
+    hi[a] = (int32_t)floorf((x[a] + reach) * vi->inv_l);
+    if (hi[a] - lo[a] >= VOID_CELLS) {
+      lo[a] = 0;
+      hi[a] = VOID_CELLS - 1;
+    }
+  }
 }
 
 /* utils.c:850-960 */
@@ -931,6 +966,105 @@ void orc_catalogue_free(orc_catalogue_t* c) {
   memset(c, 0, sizeof(*c));
 }
 
+/* --- displacement: exodus_finder.c:684-1037 --- */
+
+/* find_conflicts (:696-750): the voids a sphere collides with, in the
+ * window's cell order and each chain's order, and the largest radius among
+ * them (-1 for none). */
+static uint64_t find_conflicts(const orc_catalogue_t* cat, const void_index_t* vi,
+  float cx, float cy, float cz, float r, float f, float box, float max_r,
+  float scale, uint64_t** list, uint64_t* cap, float* largest) {
+  const float half = 0.5f * box;
+  const float reach = scale * (r + max_r - f * (r < max_r ? r : max_r));
+  const float x[3] = {cx, cy, cz};
+  int32_t lo[3], hi[3];
+  vwindow(vi, x, reach, lo, hi);
+  uint64_t m = 0;
+  *largest = -1.0f;
+  for (int32_t a = lo[0]; a <= hi[0]; a++)
+    for (int32_t b = lo[1]; b <= hi[1]; b++)
+      for (int32_t c = lo[2]; c <= hi[2]; c++) {
+        const uint32_t cell = (wrap_pbc(a, VOID_CELLS) * VOID_CELLS +
+                                wrap_pbc(b, VOID_CELLS)) * VOID_CELLS +
+                              wrap_pbc(c, VOID_CELLS);
+        for (int32_t i = vi->head[cell]; i != -1; i = vi->next[i]) {
+          float dx = fabsf(cx - cat->cx[i]);
+          float dy = fabsf(cy - cat->cy[i]);
+          float dz = fabsf(cz - cat->cz[i]);
+          dx = dx > half ? box - dx : dx;
+          dy = dy > half ? box - dy : dy;
+          dz = dz > half ? box - dz : dz;
+          const float ri = cat->r[i];
+          const float tot = scale * (r + ri - f * (r < ri ? r : ri));
+          if (dx * dx + dy * dy + dz * dz >= tot * tot)
+            continue;
+          if (m == *cap) {
+            *cap = *cap ? 2 * *cap : 64;
+            *list = realloc(*list, sizeof(uint64_t) * *cap);
+          }
+          (*list)[m++] = (uint64_t)i;
+          if (ri > *largest)
+            *largest = ri;
+        }
+      }
+  return m;
+}
+
+/* displace_void (:946-1002): out of the index, its radius negated, its nodes
+ * cleared, then the nodes of every standing void it intersects set again. */
+static void displace_void(orc_catalogue_t* cat, void_index_t* vi, uint64_t* mask,
+  uint64_t i, float max_r, float scale, uint32_t n, float l, float box) {
+  const float cx = cat->cx[i], cy = cat->cy[i], cz = cat->cz[i], r = cat->r[i];
+  vindex_remove(vi, i, cx, cy, cz);
+  cat->r[i] = -r;
+  paint_nodes(mask, cx, cy, cz, scale * r, n, l, 0);
+  const float half = 0.5f * box;
+  const float x[3] = {cx, cy, cz};
+  int32_t lo[3], hi[3];
+  vwindow(vi, x, scale * (r + max_r), lo, hi);
+  for (int32_t a = lo[0]; a <= hi[0]; a++)
+    for (int32_t b = lo[1]; b <= hi[1]; b++)
+      for (int32_t c = lo[2]; c <= hi[2]; c++) {
+        const uint32_t cell = (wrap_pbc(a, VOID_CELLS) * VOID_CELLS +
+                                wrap_pbc(b, VOID_CELLS)) * VOID_CELLS +
+                              wrap_pbc(c, VOID_CELLS);
+        for (int32_t j = vi->head[cell]; j != -1; j = vi->next[j]) {
+          float dx = fabsf(cx - cat->cx[j]);
+          float dy = fabsf(cy - cat->cy[j]);
+          float dz = fabsf(cz - cat->cz[j]);
+          dx = dx > half ? box - dx : dx;
+          dy = dy > half ? box - dy : dy;
+          dz = dz > half ? box - dz : dz;
+          const float t = scale * (r + cat->r[j]);
+          if (dx * dx + dy * dy + dz * dz < t * t)
+            paint_nodes(mask, cat->cx[j], cat->cy[j], cat->cz[j], scale * cat->r[j],
+              n, l, 1);
+        }
+      }
+}
+
+/* compact_catalogue (:1004-1032): the displaced voids dropped, the rest
+ * re-indexed in catalogue order. */
+static void compact_catalogue(orc_catalogue_t* cat, void_index_t* vi) {
+  uint64_t j = 0;
+  for (uint64_t i = 0; i < cat->n; i++) {
+    if (!(cat->r[i] > 0.0f))
+      continue;
+    cat->cx[j] = cat->cx[i];
+    cat->cy[j] = cat->cy[i];
+    cat->cz[j] = cat->cz[i];
+    cat->r[j] = cat->r[i];
+    cat->n_in[j] = cat->n_in[i];
+    cat->rung[j] = cat->rung[i];
+    j++;
+  }
+  cat->n = j;
+  for (int32_t c = 0; c < VOID_CELLS * VOID_CELLS * VOID_CELLS; c++)
+    vi->head[c] = -1;
+  for (uint64_t i = 0; i < cat->n; i++)
+    vindex_insert(vi, i, cat->cx[i], cat->cy[i], cat->cz[i]);
+}
+
 /* A candidate's speculative evaluation against the frozen catalogue. */
 typedef struct {
   uint8_t status; /* 1 refused (final), 2 rescale failed, 3 radius found */
@@ -978,6 +1112,13 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   const float sf = factors[(search_opt >> 12) & 3u];
   float max_r = 0.0f;
   int status = 0;
+  /* the witness is a shortcut of the commit; under displacement the commit
+   * runs in full, without it */
+  const int displace = (search_opt & ORC_DISPLACE) != 0;
+  if (displace)
+    use_witness = 0;
+  uint64_t* conflicts = NULL;
+  uint64_t conflicts_cap = 0;
 
   /* exodus_finder.c:234-244, 1670-1702, 1342-1353: the extended reach's
    * levels, from the run's largest search radius down past the bottom rung
@@ -1046,7 +1187,69 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
     if (ext0 == n_ext && n_ext > 0 && ext_reach[n_ext - 1] > r_search)
       ext0 = n_ext - 1;
 
-    uint64_t k = 0;
+    uint64_t k = displace ? nc : 0;
+    /* SIF_FINDER_DISPLACE (exodus_finder.c:1871-1880, 2210-2275): no
+     * prefilter; each candidate, in order, tested at the rung against the
+     * catalogue as it stands, grown, and then refused only by a void at least
+     * as large, displacing the smaller ones it collides with. The reference's
+     * batches end at each displacement, so its commit is this sequence. */
+    for (uint64_t q = 0; displace && q < nc; q++) {
+      const uint64_t flat = cand[q].flat_idx;
+      if (bit_get(mask, flat)) {
+        st.masked++;
+        continue;
+      }
+      const uint32_t ix = (uint32_t)(flat / ((uint64_t)n * n));
+      const uint32_t iy = (uint32_t)((flat / n) % n);
+      const uint32_t iz = (uint32_t)(flat % n);
+      if (proxy > 0 && orc_overlap_quick(mask, n, ix, iy, iz, (uint32_t)proxy)) {
+        st.proxy++;
+        continue;
+      }
+      const float cx = (float)ix * l, cy = (float)iy * l, cz = (float)iz * l;
+      if (overlap_exact(cat, vi, cx, cy, cz, radius, max_r, box, f, scale)) {
+        st.overlap++;
+        continue;
+      }
+      uint8_t reason;
+      uint64_t n_in = 0;
+      float r = weighted
+        ? orc_find_radius_weighted(mesh, cx, cy, cz, r_search, t, vol, mass, rmin,
+            &tpl, window, &reason, &n_in)
+        : orc_find_radius(mesh, cx, cy, cz, r_search, t, vol, rmin, &tpl, window,
+            &reason, &n_in);
+      for (uint32_t x = ext0; x < n_ext && r < 0.0f && reason == ORC_RESCALE_BEYOND;
+           x++) {
+        n_in = 0;
+        r = weighted
+          ? orc_find_radius_weighted(mesh, cx, cy, cz, ext_reach[x], t, vol, mass,
+              ext_rmin[x], &etpl[x], window, &reason, &n_in)
+          : orc_find_radius(mesh, cx, cy, cz, ext_reach[x], t, vol, ext_rmin[x],
+              &etpl[x], window, &reason, &n_in);
+      }
+      if (r < 0.0f) {
+        st.rescale_fail++;
+        continue;
+      }
+      float largest;
+      const uint64_t m = find_conflicts(cat, vi, cx, cy, cz, r, f, box, max_r, scale,
+        &conflicts, &conflicts_cap, &largest);
+      if (largest >= r) {
+        st.overlap++;
+        continue;
+      }
+      for (uint64_t c = 0; c < m; c++)
+        displace_void(cat, vi, mask, conflicts[c], max_r, scale, n, l, box);
+      st.displaced += m;
+      cat_append(cat, cx, cy, cz, r, n_in, i);
+      vindex_insert(vi, cat->n - 1, cx, cy, cz);
+      orc_mark_sphere(mask, cx, cy, cz, scale * r, n, l);
+      if (r > max_r)
+        max_r = r;
+      st.accepted++;
+    }
+    if (displace)
+      compact_catalogue(cat, vi);
     while (k < nc) {
       /* phase 0: masked nodes, sequentially */
       uint32_t bc = 0;
@@ -1172,6 +1375,7 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   }
   for (uint32_t k = 0; k < n_ext; k++)
     orc_template_free(&etpl[k]);
+  free(conflicts);
 
   free(mask);
   free(cand);

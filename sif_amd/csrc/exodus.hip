@@ -138,7 +138,133 @@ struct Voids {
     }
     return false;
   }
+
+  // --- SIF_FINDER_DISPLACE (exodus_finder.c:684-1032) ---
+
+  // The cell window within `reach` of a point, whole along an axis it spans
+  // (exodus_finder.c:706-717).
+  void window(float x, float y, float z, float reach, int32_t lo[3], int32_t hi[3]) const {
+    const float p[3] = {x, y, z};
+    for (int a = 0; a < 3; a++) {
+      lo[a] = (int32_t)floorf((p[a] - reach) * inv_l);
+      hi[a] = (int32_t)floorf((p[a] + reach) * inv_l);
+      if (hi[a] - lo[a] >= (int32_t)VOID_CELLS) {
+        lo[a] = 0;
+        hi[a] = VOID_CELLS - 1;
+      }
+    }
+  }
+  // Visits the listed voids of the window in the reference's order: cells in
+  // window order, each chain from its head.
+  template <class F>
+  void visit(float x, float y, float z, float reach, F&& fn) {
+    link();
+    int32_t lo[3], hi[3];
+    window(x, y, z, reach, lo, hi);
+    for (int32_t a = lo[0]; a <= hi[0]; a++)
+      for (int32_t b = lo[1]; b <= hi[1]; b++)
+        for (int32_t c = lo[2]; c <= hi[2]; c++) {
+          const uint32_t cell = (void_wrap(a) * VOID_CELLS + void_wrap(b)) * VOID_CELLS +
+                                void_wrap(c);
+          for (int32_t i = head[cell]; i != -1; i = next[i])
+            fn((uint32_t)i);
+        }
+  }
+  // folded axis distances to void i
+  void dist(float x, float y, float z, uint32_t i, float* dx, float* dy, float* dz) const {
+    const float half = 0.5f * box;
+    float ax = fabsf(x - cx[i]), ay = fabsf(y - cy[i]), az = fabsf(z - cz[i]);
+    *dx = ax > half ? box - ax : ax;
+    *dy = ay > half ? box - ay : ay;
+    *dz = az > half ? box - az : az;
+  }
+  // find_conflicts (:696-750): the voids a sphere collides with, in visiting
+  // order, and the largest radius among them (-1 for none)
+  float conflicts(float x, float y, float z, float rr, float f, float scale,
+    std::vector<uint32_t>* out) {
+    out->clear();
+    float largest = -1.0f;
+    const float reach = scale * (rr + max_r - f * (rr < max_r ? rr : max_r));
+    visit(x, y, z, reach, [&](uint32_t i) {
+      float dx, dy, dz;
+      dist(x, y, z, i, &dx, &dy, &dz);
+      const float ri = r[i];
+      const float tot = scale * (rr + ri - f * (rr < ri ? rr : ri));
+      if (dx * dx + dy * dy + dz * dz >= tot * tot)
+        return;
+      out->push_back(i);
+      if (ri > largest)
+        largest = ri;
+    });
+    return largest;
+  }
+  // cell_linked_list.c:212-232
+  void unlink(uint32_t i) {
+    link();
+    int32_t* at = &head[cell_of(cx[i], cy[i], cz[i])];
+    while (*at != -1 && *at != (int32_t)i)
+      at = &next[*at];
+    if (*at != -1)
+      *at = next[i];
+  }
+  // compact_catalogue (:1004-1032): the displaced voids (radius negated)
+  // dropped, the rest re-listed in catalogue order. The largest radius is
+  // kept: the reference never lowers it.
+  void compact() {
+    size_t j = 0;
+    for (size_t i = 0; i < cx.size(); i++) {
+      if (!(r[i] > 0.0f))
+        continue;
+      cx[j] = cx[i];
+      cy[j] = cy[i];
+      cz[j] = cz[i];
+      r[j] = r[i];
+      n_in[j] = n_in[i];
+      j++;
+    }
+    cx.resize(j);
+    cy.resize(j);
+    cz.resize(j);
+    r.resize(j);
+    n_in.resize(j);
+    head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
+    linked = 0;
+    link();
+  }
 };
+
+// sif__mark_sphere / sif__clear_sphere (utils.c:965-1039) on a host mask: the
+// reference's walk of the cube of nodes around the centre, with paint.h's
+// float steps.
+void host_paint(std::vector<uint64_t>& mask, float x, float y, float z, float r, float l,
+  uint32_t n, bool set) {
+  PaintSphere P;
+  paint_sphere(P, x, y, z, r, l, n);
+  for (int32_t gx = P.ox - P.c; gx <= P.ox + P.c; gx++) {
+    const float ddx = paint_dist(P, gx, P.cx);
+    const float dx2 = ddx * ddx;
+    if (dx2 > P.r2)
+      continue;
+    const uint64_t wx = paint_wrap(gx, P.n);
+    for (int32_t gy = P.oy - P.c; gy <= P.oy + P.c; gy++) {
+      const float ddy = paint_dist(P, gy, P.cy);
+      const float dxy2 = dx2 + ddy * ddy;
+      if (dxy2 > P.r2)
+        continue;
+      const uint64_t wy = paint_wrap(gy, P.n);
+      for (int32_t gz = P.oz - P.c; gz <= P.oz + P.c; gz++) {
+        const float ddz = paint_dist(P, gz, P.cz);
+        if (dxy2 + ddz * ddz <= P.r2) {
+          const uint64_t cell = (wx * n + wy) * n + paint_wrap(gz, P.n);
+          if (set)
+            mask[cell >> 6] |= 1ull << (cell & 63);
+          else
+            mask[cell >> 6] &= ~(1ull << (cell & 63));
+        }
+      }
+    }
+  }
+}
 
 struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
@@ -146,6 +272,22 @@ struct DeviceState {
   DeviceBuffer field2;     // the second smoothed field (rungs alternate)
 };
 DeviceState g_dev;
+
+// A displacement rung's growth window on the device.
+struct DisplaceBuffers {
+  DeviceBuffer flat, cx, cy, cz, rad, reason, nin;
+  int reserve(uint32_t n) {
+    int st = flat.reserve((size_t)n * 4);
+    if (st == SIF_OK) st = cx.reserve((size_t)n * 4);
+    if (st == SIF_OK) st = cy.reserve((size_t)n * 4);
+    if (st == SIF_OK) st = cz.reserve((size_t)n * 4);
+    if (st == SIF_OK) st = rad.reserve((size_t)n * 4);
+    if (st == SIF_OK) st = reason.reserve((size_t)n * 4);
+    if (st == SIF_OK) st = nin.reserve((size_t)n * 8);
+    return st;
+  }
+};
+DisplaceBuffers disp_bufs;
 
 // A fork-join pool for the host's independent per-candidate tests: run(n,
 // fn) calls fn(begin, end) over [0, n) in chunks from every worker and the
@@ -617,6 +759,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
 
   std::vector<uint32_t> surv;
   uint64_t cand_cap = 0;
+  // under SIF_FINDER_DISPLACE the mask is kept on the host (voids leave it)
+  const bool displace = (R.opt & SIF_FINDER_DISPLACE) != 0;
+  std::vector<uint64_t> hmask;
 
   // Two smoothed fields in turn: the caller's grid and a second buffer.
   st = g_dev.field2.reserve(total * sizeof(float));
@@ -712,6 +857,174 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         ext0 = k;
     if (ext0 == n_ext && n_ext > 0 && ext_reach[n_ext - 1] > r_search)
       ext0 = n_ext - 1;
+
+    if (displace) {
+      // SIF_FINDER_DISPLACE (exodus_finder.c:1871-1880, 2210-2275): no
+      // prefilter; each candidate in order is tested at the rung against the
+      // catalogue as it stands, grown, and refused only by a void at least as
+      // large, the smaller ones it collides with displaced. The reference
+      // ends its batch at each displacement, so its commit is this sequence.
+      // Candidates are grown ahead in windows on the device (a growth
+      // depends on the candidate alone); a candidate that a displacement
+      // frees after its window was formed starts the next window.
+      if (hmask.empty()) {  // the mask as the run begins: prior voids painted
+        hmask.resize((total + 63) / 64);
+        SIF_HIP(hipMemcpyAsync(hmask.data(), g_dev.mask.ptr, hmask.size() * 8,
+          hipMemcpyDeviceToHost, stream()));
+      }
+      st = sort_candidates(g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), nc);
+      if (st != SIF_OK)
+        break;
+      surv.resize(nc);
+      if (nc)
+        SIF_HIP(hipMemcpyAsync(surv.data(), g_dev.flat.ptr, nc * 4, hipMemcpyDeviceToHost,
+          stream()));
+      SIF_HIP(hipStreamSynchronize(stream()));
+      const uint32_t nn = n, nn2 = n * n;
+      auto masked_at = [&](uint32_t flat) { return (hmask[flat >> 6] >> (flat & 63)) & 1u; };
+      // 0 pass, 1 masked, 2 proxy, 3 overlap (exodus_finder.c:2220-2240)
+      auto tests = [&](uint32_t flat) -> int {
+        if (masked_at(flat))
+          return 1;
+        const uint32_t ix = flat / nn2, iy = (flat / nn) % nn, iz = flat % nn;
+        if (proxy > 0) {  // sif__overlap_quick (utils.c:819-848)
+          const int32_t p = proxy, ni = (int32_t)n;
+          auto w = [&](int32_t v) { return (uint32_t)(((v % ni) + ni) % ni); };
+          const uint32_t probes[6] = {(w((int32_t)ix + p) * nn + iy) * nn + iz,
+            (w((int32_t)ix - p) * nn + iy) * nn + iz, (ix * nn + w((int32_t)iy + p)) * nn + iz,
+            (ix * nn + w((int32_t)iy - p)) * nn + iz, (ix * nn + iy) * nn + w((int32_t)iz + p),
+            (ix * nn + iy) * nn + w((int32_t)iz - p)};
+          for (uint32_t q : probes)
+            if (masked_at(q))
+              return 2;
+        }
+        if (V.overlap_exact((float)ix * l, (float)iy * l, (float)iz * l, radius, R.f, scale))
+          return 3;
+        return 0;
+      };
+      std::vector<uint32_t> grow_flat, conflicts;
+      std::vector<int32_t> slot;  // a window candidate's growth result, or -1
+      std::vector<float> hrad;
+      std::vector<uint32_t> hreason;
+      std::vector<uint64_t> hnin;
+      const uint32_t window = BATCH_FLOOR;
+      uint64_t k = 0;
+      while (st == SIF_OK && k < nc) {
+        // the window: candidates from k, those passing as things stand grown
+        grow_flat.clear();
+        slot.clear();
+        uint64_t end = k;
+        while (end < nc && grow_flat.size() < window) {
+          const int why = tests(surv[end]);
+          slot.push_back(why ? -1 : (int32_t)grow_flat.size());
+          if (!why)
+            grow_flat.push_back(surv[end]);
+          end++;
+        }
+        const uint32_t gc = (uint32_t)grow_flat.size();
+        if (gc) {
+          st = disp_bufs.reserve(gc);
+          if (st != SIF_OK)
+            break;
+          SIF_HIP(hipMemcpyAsync(disp_bufs.flat.ptr, grow_flat.data(), gc * 4,
+            hipMemcpyHostToDevice, stream()));
+          st = node_centres(disp_bufs.flat.as<uint32_t>(), gc, n, l, disp_bufs.cx.as<float>(),
+            disp_bufs.cy.as<float>(), disp_bufs.cz.as<float>());
+          if (st == SIF_OK)
+            st = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
+              disp_bufs.cx.as<float>(), disp_bufs.cy.as<float>(), disp_bufs.cz.as<float>(), gc,
+              rmin, r_search, R.threshold, vol, disp_bufs.rad.as<float>(),
+              disp_bufs.reason.as<uint32_t>(), disp_bufs.nin.as<uint64_t>(), nullptr, &mw);
+          if (st == SIF_OK && ext0 < n_ext)
+            st = grow_extend(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
+              disp_bufs.cx.as<float>(), disp_bufs.cy.as<float>(), disp_bufs.cz.as<float>(), gc,
+              ext_rmin.data(), ext_reach.data(), ext0, n_ext, R.threshold, vol,
+              disp_bufs.rad.as<float>(), disp_bufs.reason.as<uint32_t>(),
+              disp_bufs.nin.as<uint64_t>(), &mw);
+          if (st != SIF_OK)
+            break;
+          hrad.resize(gc);
+          hreason.resize(gc);
+          hnin.resize(gc);
+          SIF_HIP(hipMemcpyAsync(hrad.data(), disp_bufs.rad.ptr, gc * 4, hipMemcpyDeviceToHost,
+            stream()));
+          SIF_HIP(hipMemcpyAsync(hreason.data(), disp_bufs.reason.ptr, gc * 4,
+            hipMemcpyDeviceToHost, stream()));
+          SIF_HIP(hipMemcpyAsync(hnin.data(), disp_bufs.nin.ptr, gc * 8,
+            hipMemcpyDeviceToHost, stream()));
+          SIF_HIP(hipStreamSynchronize(stream()));
+          S.evaluated += gc;
+        }
+        // the commit, in order, against the catalogue as it stands
+        uint64_t q = k;
+        for (; q < end; q++) {
+          const uint32_t flat = surv[q];
+          const int why = tests(flat);
+          if (why) {
+            if (why == 1) S.masked++;
+            else if (why == 2) S.proxy++;
+            else S.overlap++;
+            continue;
+          }
+          const int32_t g = slot[q - k];
+          if (g < 0)
+            break;  // freed by a displacement since the window formed
+          if (hreason[g] == 5) {
+            error_set(SIF_ERR_RANGE,
+              "exodus: a search shell at rung %g holds more tracers in one histogram "
+              "bin than the refinement buffer takes",
+              (double)radius);
+            st = SIF_ERR_RANGE;
+            break;
+          }
+          const float rr = hrad[g];
+          if (!(rr >= 0.0f)) {
+            S.rescale_fail++;
+            continue;
+          }
+          const float x = (float)(flat / nn2) * l, y = (float)((flat / nn) % nn) * l,
+                      z = (float)(flat % nn) * l;
+          const float largest = V.conflicts(x, y, z, rr, R.f, scale, &conflicts);
+          if (largest >= rr) {
+            S.overlap++;
+            continue;
+          }
+          for (uint32_t j : conflicts) {  // displace_void (:946-1002)
+            const float jx = V.cx[j], jy = V.cy[j], jz = V.cz[j], jr = V.r[j];
+            V.unlink(j);
+            V.r[j] = -jr;
+            host_paint(hmask, jx, jy, jz, scale * jr, l, n, false);
+            V.visit(jx, jy, jz, scale * (jr + V.max_r), [&](uint32_t m) {
+              float dx, dy, dz;
+              V.dist(jx, jy, jz, m, &dx, &dy, &dz);
+              const float t = scale * (jr + V.r[m]);
+              if (dx * dx + dy * dy + dz * dz < t * t)
+                host_paint(hmask, V.cx[m], V.cy[m], V.cz[m], scale * V.r[m], l, n, true);
+            });
+          }
+          // accept_void (:656-683)
+          V.add(x, y, z, rr, hnin[g]);
+          host_paint(hmask, x, y, z, scale * rr, l, n, true);
+          if (rr / radius > 2.0f)  // exodus_finder.c:2321-2328
+            mismatched++;
+          S.accepted++;
+        }
+        k = q;
+      }
+      if (st != SIF_OK)
+        break;
+      // the rung's survivors re-listed, the mask to the device for the next scan
+      V.compact();
+      SIF_HIP(hipMemcpyAsync(g_dev.mask.ptr, hmask.data(), hmask.size() * 8,
+        hipMemcpyHostToDevice, stream()));
+      SIF_HIP(hipStreamSynchronize(stream()));
+      if (stats_out)
+        stats_out[i] = S;
+      if (R.on_rung)
+        R.on_rung((uint32_t)(i + 1), V.cx.data(), V.cy.data(), V.cz.data(), V.r.data(),
+          (uint64_t)V.cx.size(), mismatched);
+      continue;
+    }
 
     // prefilter against the catalogue as the rung begins, in flat order (the
     // verdicts do not depend on the order; neighbouring candidates share
@@ -1192,18 +1505,17 @@ int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_
 }
 
 // exodus_finder.c:1112-1138: what this build can run. SIF_FINDER_SEEDS_MINIMA
-// (minima on their nodes, on the rung's own field) and
-// SIF_FINDER_EXTEND_REACH run.
+// (minima on their nodes, on the rung's own field), SIF_FINDER_EXTEND_REACH
+// and SIF_FINDER_DISPLACE run: the excursion-peak variant whole.
 int check_options(sif_option opt) {
   const sif_option refine = opt & SIF__FINDER_REFINE_MASK;
   const char* missing = refine == SIF_FINDER_REFINE_GAUSSIAN    ? "SIF_FINDER_REFINE_GAUSSIAN"
                         : refine == SIF_FINDER_REFINE_TRACERS   ? "SIF_FINDER_REFINE_TRACERS"
                         : refine                                ? "SIF_FINDER_REFINE_HILL"
-                        : (opt & SIF_FINDER_DISPLACE)           ? "SIF_FINDER_DISPLACE"
                                                                 : nullptr;
   if (missing) {
     error_set(SIF_ERR_UNSUPPORTED,
-      "options: %s is not in this build (seeds on their nodes, no displacement)", missing);
+      "options: %s is not in this build (seeds stay on their nodes)", missing);
     return SIF_ERR_UNSUPPORTED;
   }
   return SIF_OK;

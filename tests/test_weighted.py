@@ -194,8 +194,8 @@ def test_api_refuses_unusable_weights(particles):
     assert st == N.SIF_ERR_INVALID and "sum to zero" in msg
     w = base.copy()
     w[5] = -0.0
-    st, msg = run(w, opt=finders.NO_CHECKPOINT | finders.DISPLACE)
-    assert st == N.SIF_ERR_UNSUPPORTED and "DISPLACE" in msg
+    st, msg = run(w, opt=finders.NO_CHECKPOINT | finders.REFINE["gaussian"])
+    assert st == N.SIF_ERR_UNSUPPORTED and "REFINE_GAUSSIAN" in msg
 
 
 # ---- GPU --------------------------------------------------------------------
