@@ -75,5 +75,6 @@ enum {
   SIF_AMD_STAGE_SORT,
   SIF_AMD_STAGE_HOST,
   SIF_AMD_STAGE_PROFILES,
+  SIF_AMD_STAGE_COMMIT_ORDER,
   SIF_AMD_N_STAGES
 };

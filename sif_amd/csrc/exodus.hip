@@ -307,6 +307,9 @@ class WorkPool {
     for (std::thread& t : threads_)
       t.join();
   }
+  // A run ends when its chunks are done, not when every worker has come by:
+  // a worker still waking from sleep when the caller has taken the last
+  // chunk never joins, and the caller waits only for the workers inside.
   void run(uint64_t n, const std::function<void(uint64_t, uint64_t)>& fn) {
     if (threads_.empty() || n <= GRAIN) {
       fn(0, n);
@@ -315,30 +318,28 @@ class WorkPool {
     job_ = &fn;
     n_ = n;
     next_.store(0, std::memory_order_relaxed);
-    active_.store((unsigned)threads_.size(), std::memory_order_relaxed);
+    const uint64_t g = gen_.load(std::memory_order_relaxed) + 1;
+    open_.store(g, std::memory_order_seq_cst);
     {
       // under the lock: a worker about to sleep sees the new job
       std::lock_guard<std::mutex> lk(mu_);
-      gen_.fetch_add(1, std::memory_order_release);
+      gen_.store(g, std::memory_order_release);
     }
     if (sleeping_.load() > 0)
       cv_.notify_all();
     work(fn, n);
-    for (uint32_t spins = 0; active_.load(std::memory_order_acquire) != 0; spins++) {
-      if (spins < SPIN) {
-        _mm_pause();
-        continue;
-      }
-      std::unique_lock<std::mutex> lk(mu_);
-      done_.wait(lk, [this]() { return active_.load() == 0; });
-      break;
-    }
+    // closed: a worker that arrives now sees it and leaves without the job;
+    // one that arrived before is inside and is waited for (each side writes
+    // its flag before reading the other's, both sequentially consistent)
+    open_.store(0, std::memory_order_seq_cst);
+    while (inside_.load(std::memory_order_seq_cst) != 0)
+      _mm_pause();
     job_ = nullptr;
   }
 
  private:
   static constexpr uint64_t GRAIN = 64;
-  // Pauses a waiting thread spins before it sleeps (some tens of
+  // Pauses a waiting worker spins before it sleeps (some tens of
   // microseconds): the commit's pool runs come a few tens of microseconds
   // apart, and a wake-up from sleep costs about as much.
   static constexpr uint32_t SPIN = 2048;
@@ -369,27 +370,36 @@ class WorkPool {
       if (stop_.load())
         return;
       seen = g;
-      work(*job_, n_);
-      if (active_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
-        std::lock_guard<std::mutex> lk(mu_);
-        done_.notify_one();
-      }
+      inside_.fetch_add(1, std::memory_order_seq_cst);
+      if (open_.load(std::memory_order_seq_cst) == g)
+        work(*job_, n_);
+      inside_.fetch_sub(1, std::memory_order_seq_cst);
     }
   }
   std::vector<std::thread> threads_;
   std::mutex mu_;
-  std::condition_variable cv_, done_;
+  std::condition_variable cv_;
   const std::function<void(uint64_t, uint64_t)>* job_ = nullptr;
   uint64_t n_ = 0;
   std::atomic<uint64_t> next_{0};
-  std::atomic<unsigned> active_{0};
+  std::atomic<uint64_t> open_{0};  // the generation taking workers, 0 when closed
+  std::atomic<unsigned> inside_{0};
   std::atomic<uint64_t> gen_{0};
   std::atomic<int> sleeping_{0};
   std::atomic<bool> stop_{false};
 };
 
+// The pool's workers beside the calling thread: SIF_AMD_HOST_THREADS - 1 when
+// set (1 to 64 threads in all), else one fewer than the hardware threads, at
+// most 15.
 WorkPool& host_pool() {
-  static WorkPool pool(std::min(std::max(std::thread::hardware_concurrency(), 2u) - 1u, 15u));
+  static WorkPool pool([]() {
+    const char* env = getenv("SIF_AMD_HOST_THREADS");
+    const long want = env ? strtol(env, nullptr, 10) : 0;
+    if (want >= 1 && want <= 64)
+      return (unsigned)(want - 1);
+    return std::min(std::max(std::thread::hardware_concurrency(), 2u) - 1u, 15u);
+  }());
   return pool;
 }
 
@@ -1295,6 +1305,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       // In order. The voids accepted join the catalogue at once; the list of
       // this commit's voids (CL) is filled only when a candidate with more
       // effects than were listed needs it.
+      const double t3 = now_ms();
       CL.clear();
       size_t cl_filled = 0;
       taken.clear();
@@ -1381,7 +1392,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       S.accepted += accepted;
       *accepted_out = accepted;
       if (timing_active()) {
-        timing_add(SIF_AMD_STAGE_COMMIT, now_ms() - t2);
+        const double t4 = now_ms();
+        timing_add(SIF_AMD_STAGE_COMMIT, t3 - t2);
+        timing_add(SIF_AMD_STAGE_COMMIT_ORDER, t4 - t3);
         if (B.algo) {
           double bytes = 0.0;
           for (uint32_t b = 0; b < B.count; b++)
