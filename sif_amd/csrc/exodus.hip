@@ -93,6 +93,10 @@ struct Voids {
   // utils.c:850-960
   bool overlap_exact(float x, float y, float z, float rr, float f, float scale) {
     link();
+    return overlap_linked(x, y, z, rr, f, scale);
+  }
+  // the same, with every void already linked (callable from several threads)
+  bool overlap_linked(float x, float y, float z, float rr, float f, float scale) const {
     const float half = box * 0.5f;
     const float sr = scale * (rr + max_r - f * (rr < max_r ? rr : max_r));
     const int32_t lo[3] = {(int32_t)floorf((x - sr) * inv_l), (int32_t)floorf((y - sr) * inv_l),
@@ -893,6 +897,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       const uint32_t nn = n, nn2 = n * n;
       auto masked_at = [&](uint32_t flat) { return (hmask[flat >> 6] >> (flat & 63)) & 1u; };
       // 0 pass, 1 masked, 2 proxy, 3 overlap (exodus_finder.c:2220-2240)
+      // (V linked before: the tests only read the catalogue and the mask, and
+      // run on the pool while a window forms)
       auto tests = [&](uint32_t flat) -> int {
         if (masked_at(flat))
           return 1;
@@ -908,10 +914,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
             if (masked_at(q))
               return 2;
         }
-        if (V.overlap_exact((float)ix * l, (float)iy * l, (float)iz * l, radius, R.f, scale))
+        if (V.overlap_linked((float)ix * l, (float)iy * l, (float)iz * l, radius, R.f, scale))
           return 3;
         return 0;
       };
+      std::vector<uint8_t> why_ahead;
       std::vector<uint32_t> grow_flat, conflicts;
       std::vector<int32_t> slot;  // a window candidate's growth result, or -1
       std::vector<float> hrad;
@@ -920,16 +927,26 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       const uint32_t window = BATCH_FLOOR;
       uint64_t k = 0;
       while (st == SIF_OK && k < nc) {
-        // the window: candidates from k, those passing as things stand grown
+        // the window: candidates from k, those passing as things stand grown;
+        // the tests of a stretch of candidates run on the pool at once
         grow_flat.clear();
         slot.clear();
         uint64_t end = k;
+        V.link();
         while (end < nc && grow_flat.size() < window) {
-          const int why = tests(surv[end]);
-          slot.push_back(why ? -1 : (int32_t)grow_flat.size());
-          if (!why)
-            grow_flat.push_back(surv[end]);
-          end++;
+          const uint64_t m = std::min<uint64_t>(nc - end, 2 * window);
+          why_ahead.resize(m);
+          const uint64_t e0 = end;
+          host_pool().run(m, [&](uint64_t a, uint64_t b) {
+            for (uint64_t t = a; t < b; t++)
+              why_ahead[t] = (uint8_t)tests(surv[e0 + t]);
+          });
+          for (uint64_t t = 0; t < m && grow_flat.size() < window; t++, end++) {
+            const int why = why_ahead[t];
+            slot.push_back(why ? -1 : (int32_t)grow_flat.size());
+            if (!why)
+              grow_flat.push_back(surv[end]);
+          }
         }
         const uint32_t gc = (uint32_t)grow_flat.size();
         if (gc) {
@@ -969,6 +986,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         uint64_t q = k;
         for (; q < end; q++) {
           const uint32_t flat = surv[q];
+          V.link();
           const int why = tests(flat);
           if (why) {
             if (why == 1) S.masked++;
