@@ -811,6 +811,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
 
     // the candidates in flat order, growing the buffers when they are short
     uint64_t nc = 0;
+    uint32_t key_differ = 0xffffffffu;  // the bits on which the candidates' keys differ
     auto reserve_candidates = [&]() {
       int e = g_dev.keys.reserve(cand_cap * 4);
       if (e == SIF_OK) e = g_dev.flat.reserve(cand_cap * 4);
@@ -827,7 +828,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       }
       st = scan_candidates_flat(field, g_dev.mask.as<uint64_t>(), total, R.threshold,
         (R.opt & SIF_FINDER_SEEDS_MINIMA) ? n : 0u, g_dev.keys.as<uint32_t>(),
-        g_dev.flat.as<uint32_t>(), cand_cap, &nc);
+        g_dev.flat.as<uint32_t>(), cand_cap, &nc, &key_differ);
       if (st != SIF_OK || nc <= cand_cap)
         break;
       cand_cap = (uint64_t)((double)nc * 1.2) + 1;
@@ -886,7 +887,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         SIF_HIP(hipMemcpyAsync(hmask.data(), g_dev.mask.ptr, hmask.size() * 8,
           hipMemcpyDeviceToHost, stream()));
       }
-      st = sort_candidates(g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), nc);
+      st = sort_candidates(g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), nc,
+        key_differ);
       if (st != SIF_OK)
         break;
       surv.resize(nc);
@@ -1077,7 +1079,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if (st != SIF_OK)
       break;
     timing_end(SIF_AMD_STAGE_PREFILTER);
-    st = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), ns);
+    st = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), ns,
+      key_differ);  // the survivors are a subset of the scanned candidates
     if (st != SIF_OK)
       break;
     th = now_ms();

@@ -44,9 +44,12 @@ void build_window_axis(uint32_t n, int order, int mode, std::vector<double>& axi
 int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
   float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
 // The same candidates in flat order, unsorted; then the sort on its own.
+// key_differ (optional) receives the bits on which the listed keys differ.
 int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count);
-int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n);
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count,
+  uint32_t* key_differ = nullptr);
+int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n,
+  uint32_t key_differ = 0xffffffffu);
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
 
 // grow.hip

@@ -229,7 +229,7 @@ namespace {
 DeviceBuffer g_rs_hist, g_rs_offs, g_rs_k, g_rs_v;
 }
 
-int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits) {
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t differ) {
   if (n <= 1)
     return SIF_OK;
   const uint32_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
@@ -247,6 +247,8 @@ int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits) {
   int passes = (bits + 3) / 4;
   for (int p = 0; p < passes; p++) {
     const int shift = 4 * p;
+    if (((differ >> shift) & (RS_RADIX - 1)) == 0)
+      continue;  // one digit value in every key: the identity permutation
     radix_hist<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, n, shift, n_tiles,
       g_rs_hist.as<uint32_t>());
     SIF_HIP(hipGetLastError());

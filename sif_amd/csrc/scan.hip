@@ -119,15 +119,69 @@ __device__ __forceinline__ uint32_t sc_block_scan(uint32_t v, uint32_t* total) {
   return before + x - v;
 }
 
+// Also the AND and the OR of each tile's candidate keys (tile_span[2 tile],
+// [2 tile + 1]): the sort skips the digits on which every key agrees.
 __global__ void __launch_bounds__(SC_THREADS) cand_count_kernel(
   const float* __restrict__ delta, const uint64_t* __restrict__ mask, uint64_t total,
-  float t, uint32_t minima_n, uint64_t* __restrict__ tile_counts) {
+  float t, uint32_t minima_n, uint64_t* __restrict__ tile_counts,
+  uint32_t* __restrict__ tile_span) {
+  __shared__ uint32_t w_and[SC_THREADS / 64], w_or[SC_THREADS / 64];
   const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
-  const uint32_t c = (uint32_t)__popc(cand_bits(delta, mask, total, t, first, minima_n));
+  const uint32_t bits = cand_bits(delta, mask, total, t, first, minima_n);
+  uint32_t k_and = 0xffffffffu, k_or = 0u;
+  for (uint32_t b = bits; b; b &= b - 1) {
+    const uint32_t k = float_key(delta[first + (__ffs(b) - 1)]);
+    k_and &= k;
+    k_or |= k;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    k_and &= __shfl_xor(k_and, off, 64);
+    k_or |= __shfl_xor(k_or, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    w_and[threadIdx.x >> 6] = k_and;
+    w_or[threadIdx.x >> 6] = k_or;
+  }
   uint32_t all;
-  sc_block_scan(c, &all);
-  if (threadIdx.x == 0)
+  sc_block_scan((uint32_t)__popc(bits), &all);  // its barriers publish w_and, w_or
+  if (threadIdx.x == 0) {
     tile_counts[blockIdx.x] = all;
+    uint32_t a = 0xffffffffu, o = 0u;
+    for (int w = 0; w < SC_THREADS / 64; w++) {
+      a &= w_and[w];
+      o |= w_or[w];
+    }
+    tile_span[2 * (uint64_t)blockIdx.x] = a;
+    tile_span[2 * (uint64_t)blockIdx.x + 1] = o;
+  }
+}
+
+// The tiles' spans into one (span[0] AND, span[1] OR), one block.
+__global__ void __launch_bounds__(1024) span_reduce_kernel(const uint32_t* __restrict__ tile_span,
+  uint64_t tiles, uint32_t* __restrict__ span) {
+  __shared__ uint32_t w_and[16], w_or[16];
+  uint32_t a = 0xffffffffu, o = 0u;
+  for (uint64_t i = threadIdx.x; i < tiles; i += blockDim.x) {
+    a &= tile_span[2 * i];
+    o |= tile_span[2 * i + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a &= __shfl_xor(a, off, 64);
+    o |= __shfl_xor(o, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    w_and[threadIdx.x >> 6] = a;
+    w_or[threadIdx.x >> 6] = o;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      a &= w_and[w];
+      o |= w_or[w];
+    }
+    span[0] = a;
+    span[1] = o;
+  }
 }
 
 __global__ void __launch_bounds__(SC_THREADS) cand_write_kernel(
@@ -156,7 +210,7 @@ __global__ void cand_unkey_kernel(const uint32_t* __restrict__ keys, uint64_t n,
 }
 
 namespace {
-DeviceBuffer g_tiles, g_tile_pos, g_count;
+DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span;
 }
 
 // Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)
@@ -167,7 +221,8 @@ DeviceBuffer g_tiles, g_tile_pos, g_count;
 // (sif__finder_scan_refined with minima_only and no refinement,
 // utils.c:406-552).
 int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count,
+  uint32_t* key_differ) {
   if (total >= 0x80000000ull) {
     error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^31 - 1)",
       (unsigned long long)total);
@@ -187,24 +242,34 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
   if (st == SIF_OK)
     st = g_tile_pos.reserve((tiles ? tiles : 1) * 8);
   if (st == SIF_OK)
-    st = g_count.reserve(8);
+    st = g_count.reserve(16);  // the count, then the keys' AND and OR
+  if (st == SIF_OK)
+    st = g_tile_span.reserve((tiles ? tiles : 1) * 8);
   if (st != SIF_OK)
     return st;
   *count = 0;
+  if (key_differ)
+    *key_differ = 0xffffffffu;
   if (total == 0)
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SCAN);
   cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
-    minima_n, g_tiles.as<uint64_t>());
+    minima_n, g_tiles.as<uint64_t>(), g_tile_span.as<uint32_t>());
+  SIF_HIP(hipGetLastError());
+  span_reduce_kernel<<<1, 1024, 0, stream()>>>(g_tile_span.as<uint32_t>(), tiles,
+    g_count.as<uint32_t>() + 2);
   SIF_HIP(hipGetLastError());
   st = exclusive_scan_u64(g_tiles.as<uint64_t>(), tiles, g_tile_pos.as<uint64_t>(),
     g_count.as<uint64_t>());
   if (st != SIF_OK)
     return st;
-  uint64_t n = 0;
-  SIF_HIP(hipMemcpyAsync(&n, g_count.ptr, 8, hipMemcpyDeviceToHost, stream()));
+  uint64_t back[2] = {0, 0};  // the count; the keys' AND (low) and OR (high)
+  SIF_HIP(hipMemcpyAsync(back, g_count.ptr, 16, hipMemcpyDeviceToHost, stream()));
   SIF_HIP(hipStreamSynchronize(stream()));
+  const uint64_t n = back[0];
   *count = n;
+  if (key_differ)
+    *key_differ = (uint32_t)back[1] ^ (uint32_t)(back[1] >> 32);
   if (n <= cap && n > 0) {
     cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
       minima_n, g_tile_pos.as<uint64_t>(), keys, flat);
@@ -219,12 +284,13 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
 
 // Deepest first; equal depths keep their order (a stable radix sort), so a
 // list in flat order comes out ordered by (depth, flat index). Bit 31 of a
-// flat value rides along untouched.
-int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n) {
+// flat value rides along untouched. key_differ: the bits on which the keys
+// may differ (from the scan that listed them, or a superset of them).
+int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n, uint32_t key_differ) {
   if (n == 0)
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SORT);
-  const int st = radix_sort_u32(keys, flat, (uint32_t)n, 32);
+  const int st = radix_sort_u32(keys, flat, (uint32_t)n, 32, key_differ);
   timing_end(SIF_AMD_STAGE_SORT);
   return st;
 }
@@ -232,10 +298,12 @@ int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n) {
 // The sorted candidates (sif__finder_scan_candidates + sort_candidates).
 int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
   float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count) {
-  int st = scan_candidates_flat(delta, mask, total, t, minima_n, keys, flat, cap, count);
+  uint32_t differ = 0xffffffffu;
+  int st = scan_candidates_flat(delta, mask, total, t, minima_n, keys, flat, cap, count,
+    &differ);
   if (st != SIF_OK || *count > cap)
     return st;
-  return sort_candidates(keys, flat, *count);
+  return sort_candidates(keys, flat, *count, differ);
 }
 
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out) {
