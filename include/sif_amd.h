@@ -249,6 +249,16 @@ SIF_API int sif_amd_cic_adjoint(const float* d_x, const float* d_y,
 SIF_API int sif_amd_squared_residual(const float* d_p, const float* d_target,
   uint64_t n, float scale, float* d_grad, double* loss);
 
+/* The speculation window of the exodus runs that follow in this process
+ * (exodus_finder.c:183-213, 2331-2349: a batch that accepts nothing widens
+ * the window by the floor, one that accepts divides it down, within
+ * [floor, ceiling]). The reference fixes both at compile time and the
+ * catalogue does not depend on them; this build's defaults are 4096 and
+ * 32768, sized to fill the GPU. 1 <= floor <= ceiling <= 32768; (0, 0)
+ * restores the defaults. Lets small inputs cross batch boundaries. Not
+ * synchronised, like the reference's settings table. */
+SIF_API int sif_amd_set_batch_window(uint32_t floor, uint32_t ceiling);
+
 /* Per-rung smoothed fields as the finder scans them, for parity checks:
  * rung i (descending ladder) into d_fields + i * n_cells^3. */
 SIF_API int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells,

@@ -114,6 +114,7 @@ _sig("sif_amd_exodus_device_weighted", C.POINTER(Catalogue), P, u32, f32, P, P, 
      f64, u32, u64, C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
      C.POINTER(RungStats))
 _sig("sif_amd_exodus_fields", i32, P, u32, f32, C.POINTER(f32), u32, u32, P)
+_sig("sif_amd_set_batch_window", i32, u32, u32)
 _sig("sif_amd_tracer_profiles", i32, P, P, P, P, P, P, P, P, u32, f32, f64, P, P, P, P, u32,
      f32, u32, u32, P, P)
 _sig("sif_catalogue_free", None, C.POINTER(Catalogue))

@@ -191,6 +191,12 @@ def exodus(ddelta, n_cells, box, mesh, radii, threshold, overlap=0.0, options=0,
     return out, n_in, st
 
 
+def set_batch_window(floor=0, ceiling=0):
+    """The growth batch window of later exodus runs in this process; (0, 0)
+    restores the defaults."""
+    check(lib.sif_amd_set_batch_window(floor, ceiling))
+
+
 def exodus_fields(ddelta, n_cells, box, radii, options=0):
     radii = np.ascontiguousarray(radii, dtype=np.float32)
     total = n_cells ** 3

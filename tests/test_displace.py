@@ -176,6 +176,18 @@ def test_device_zeldovich_displacement(native, f, opt):
     np.testing.assert_array_equal(cat.view(np.uint32), ocat.view(np.uint32))
     np.testing.assert_array_equal(nin, onin)
     assert colliding_pairs(cat, box, f) == []
+    # growth windows of 16 and 100 candidates: many windows a rung, and
+    # displacements that free candidates of a window already grown
+    try:
+        for floor in (16, 100):
+            D.set_batch_window(floor, floor)
+            g2 = D.cic(dx, dy, dz, len(x), n, box)
+            D.density_contrast(g2, n ** 3)
+            cat2, nin2, _ = D.exodus(g2, n, box, m, radii, -0.7, overlap=f, options=opt)
+            np.testing.assert_array_equal(cat2.view(np.uint32), ocat.view(np.uint32))
+            np.testing.assert_array_equal(nin2, onin)
+    finally:
+        D.set_batch_window(0, 0)
 
 
 @pytest.mark.gpu

@@ -347,12 +347,20 @@ def _exodus_setup(dev, particles, n=32):
     return g
 
 
+# growth batch windows (floor, ceiling) small enough that a 32^3 rung spans
+# many batches: the pipelined slots, the pair effects between a batch's
+# candidates, the re-tests at batch assembly and the rung-start index are
+# all crossed
+SMALL_WINDOWS = [(16, 16), (64, 256), (256, 1024)]
+
+
 @pytest.mark.parametrize("overlap,opts", [(0.0, 0), (0.0, 1 << 18), (0.5, 0), (0.0, 1 << 9),
                                           (0.5, 1 << 9), (1.0, (1 << 9) | (1 << 21))])
 def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap, opts):
     """Cells, Lagrangian overlap, a partial overlap, the minima seeds
     (SIF_FINDER_SEEDS_MINIMA, bit 9) and the extended reach
-    (SIF_FINDER_EXTEND_REACH, bit 21)."""
+    (SIF_FINDER_EXTEND_REACH, bit 21); with the default batch window (one
+    batch a rung at this size) and with small ones (many)."""
     import oracle
     D, dx, dy, dz = dev
     x, y, z = particles
@@ -375,6 +383,23 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
         np.testing.assert_array_equal(nin, onin)
     assert len(cat) > 0
     assert sum(s["accepted"] for s in stats) == len(cat)
+    batches_one = sum(s["evaluated"] for s in stats)
+    try:
+        for floor, ceiling in SMALL_WINDOWS:
+            D.set_batch_window(floor, ceiling)
+            g2 = _exodus_setup(dev, particles, n)  # the same bits as g before the run
+            cat2, nin2, stats2 = D.exodus(g2, n, BOX, m, radii, THRESH, overlap=overlap,
+                                          options=opts)
+            np.testing.assert_array_equal(cat2.view(np.uint32), ocat.view(np.uint32))
+            np.testing.assert_array_equal(nin2, onin)
+            for a, b in zip(stats, stats2):
+                for key in ("candidates", "accepted", "masked", "proxy", "overlap",
+                            "rescale_fail"):
+                    assert a[key] == b[key], (floor, key, a, b)
+            if floor == 16:  # a speculative window that small wastes growths on accepts
+                assert sum(s["evaluated"] for s in stats2) >= batches_one
+    finally:
+        D.set_batch_window(0, 0)
     # radii against brute force at the rung they came from
     for (cx, cy, cz, r), k in zip(cat[:20], orung[:20]):
         rung = float(radii[k])

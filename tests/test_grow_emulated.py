@@ -12,26 +12,21 @@ spheres); against an FP64 brute force over every tracer; and all-ones
 weights against the unweighted kernel (exodus_finder.c:1634-1638).
 """
 import ctypes as C
-import os
-import subprocess
 
 import numpy as np
 import pytest
 
+import emu_lib
 import oracle
 from sif_amd import zeldovich
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-EMU_DIR = os.path.join(HERE, "emu")
-EMU_LIB = os.path.join(EMU_DIR, "libgrow_emu.so")
 
 BOX, MESH = 100.0, 10
 
 
 @pytest.fixture(scope="module")
 def emu():
-    subprocess.run(["make", "-s", "-C", EMU_DIR], check=True)
-    lib = C.CDLL(EMU_LIB)
+    lib = emu_lib.load("grow")
     lib.emu_find_radius.restype = C.c_int
     return lib
 

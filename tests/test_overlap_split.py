@@ -19,22 +19,18 @@ catalogue's largest, scans that wrap the box, f in {0, 0.5, 1} and the
 Lagrangian scale.
 """
 import ctypes as C
-import os
-import subprocess
 
 import numpy as np
 import pytest
 
+import emu_lib
 import oracle
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-EMU_DIR = os.path.join(HERE, "emu")
 
 
 @pytest.fixture(scope="module")
 def emu():
-    subprocess.run(["make", "-s", "-C", EMU_DIR], check=True)
-    lib = C.CDLL(os.path.join(EMU_DIR, "liboverlap_emu.so"))
+    lib = emu_lib.load("overlap")
     lib.ovl_states.restype = None
     return lib
 

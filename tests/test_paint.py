@@ -11,22 +11,18 @@ catalogue), small and large, wider than half the box, and wrapping the box on
 every axis.
 """
 import ctypes as C
-import os
-import subprocess
 
 import numpy as np
 import pytest
 
+import emu_lib
 import oracle
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-EMU_DIR = os.path.join(HERE, "emu")
 
 
 @pytest.fixture(scope="module")
 def emu():
-    subprocess.run(["make", "-s", "-C", EMU_DIR], check=True)
-    lib = C.CDLL(os.path.join(EMU_DIR, "libpaint_emu.so"))
+    lib = emu_lib.load("paint")
     lib.paint_nodes.restype = None
     lib.rv_probe_many.restype = None
     return lib
