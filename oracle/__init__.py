@@ -68,8 +68,9 @@ def _sig(name, res, *args):
 _sig("orc_box_fold", f32, f32, f32)
 _sig("orc_check_in_box", u64, P, P, P, u64, f32, C.POINTER(u64), C.POINTER(u64))
 _sig("orc_cic_float", None, P, P, P, P, u64, u32, f32, u64, P)
-_sig("orc_cic_fixed", None, P, P, P, P, u64, u32, f32, P)
-_sig("orc_cic_fixed_density", None, P, u32, f32, P, C.POINTER(f64))
+_sig("orc_cic_unit_exponent", i32, u64, f32)
+_sig("orc_cic_fixed", i32, P, P, P, P, u64, u32, f32, P)
+_sig("orc_cic_fixed_density", None, P, u32, f32, i32, P, C.POINTER(f64))
 _sig("orc_cic_brute_double", None, P, P, P, P, u64, u32, f32, P)
 _sig("orc_density_contrast", i32, P, u64, C.POINTER(f64))
 _sig("orc_density_contrast_with_mean", None, P, u64, f64)
@@ -128,16 +129,21 @@ def cic_float(x, y, z, n_cells, box, w=None, tile=0):
     return out
 
 
-def cic_fixed(x, y, z, n_cells, box, w=None):
-    """(density float32, fixed-point int64 accumulators, fixed-point mean)."""
+def cic_fixed(x, y, z, n_cells, box, w=None, with_exponent=False):
+    """(density float32, fixed-point int64 accumulators in 2^-s units,
+    fixed-point mean[, s])."""
     x, y, z = f32a(x), f32a(y), f32a(z)
     w = None if w is None else f32a(w)
     acc = np.empty(n_cells ** 3, np.int64)
-    lib.orc_cic_fixed(_p(x), _p(y), _p(z), _p(w), len(x), n_cells, box, _p(acc))
+    s = lib.orc_cic_fixed(_p(x), _p(y), _p(z), _p(w), len(x), n_cells, box, _p(acc))
     out = np.empty(n_cells ** 3, np.float32)
     mean = C.c_double()
-    lib.orc_cic_fixed_density(_p(acc), n_cells, box, _p(out), C.byref(mean))
-    return out, acc, mean.value
+    lib.orc_cic_fixed_density(_p(acc), n_cells, box, s, _p(out), C.byref(mean))
+    return (out, acc, mean.value, s) if with_exponent else (out, acc, mean.value)
+
+
+def cic_unit_exponent(n, wmax):
+    return lib.orc_cic_unit_exponent(n, wmax)
 
 
 def cic_brute(x, y, z, n_cells, box, w=None):

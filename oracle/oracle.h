@@ -33,11 +33,13 @@ uint64_t orc_check_in_box(const float* x, const float* y, const float* z,
 void orc_cic_float(const float* x, const float* y, const float* z,
   const float* w, uint64_t n, uint32_t n_cells, float box, uint64_t tile,
   float* out);
-/* Fixed-point deposit: per-corner float weight rounded to 2^-32 units and
- * summed exactly in int64, the model of the GPU kernel. */
-void orc_cic_fixed(const float* x, const float* y, const float* z,
+/* Fixed-point deposit: per-corner float weight rounded to 2^-s units and
+ * summed exactly in int64, the model of the GPU kernel; s from the count and
+ * the largest |w| (orc_cic_unit_exponent), returned. */
+int orc_cic_unit_exponent(uint64_t n, float wmax);
+int orc_cic_fixed(const float* x, const float* y, const float* z,
   const float* w, uint64_t n, uint32_t n_cells, float box, int64_t* acc);
-void orc_cic_fixed_density(const int64_t* acc, uint32_t n_cells, float box,
+void orc_cic_fixed_density(const int64_t* acc, uint32_t n_cells, float box, int s,
   float* out, double* mean_out);
 /* Textbook double-precision CIC, no shared arithmetic with the above. */
 void orc_cic_brute_double(const float* x, const float* y, const float* z,

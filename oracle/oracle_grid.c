@@ -229,11 +229,35 @@ void orc_cic_float(const float* x, const float* y, const float* z,
 
 /* The GPU deposit's arithmetic: the same eight float corner values as
  * grid.c:585-650, each rounded to a multiple of 2^-32 and added in int64. */
-void orc_cic_fixed(const float* x, const float* y, const float* z,
+/* The fixed-point unit of a deposit, 2^-s: n particles whose largest |w| is
+ * wmax put at most n wmax in a cell; with n wmax <= 2^e, s = 62 - e keeps
+ * every corner and every cell sum below 2^62. 32 when nothing is deposited. */
+int orc_cic_unit_exponent(uint64_t n, float wmax) {
+  const double bound = (double)n * (double)wmax;
+  if (!(bound > 0.0))
+    return 32;
+  int e;
+  (void)frexp(bound, &e);
+  return 62 - e;
+}
+
+/* The deposit in fixed point: each corner value rounded once to 2^-s units
+ * (s by orc_cic_unit_exponent over the weights' largest magnitude) and summed
+ * exactly in int64. Returns s. */
+int orc_cic_fixed(const float* x, const float* y, const float* z,
   const float* w, uint64_t n, uint32_t N, float box, int64_t* acc) {
   const float idx = (float)N / box;
   const uint64_t plane = (uint64_t)N * N;
   memset(acc, 0, sizeof(int64_t) * plane * N);
+  float wmax = 1.0f;
+  if (w) {
+    wmax = 0.0f;
+    for (uint64_t p = 0; p < n; p++)
+      if (fabsf(w[p]) > wmax)
+        wmax = fabsf(w[p]);
+  }
+  const int s = orc_cic_unit_exponent(n, wmax);
+  const double unit_inv = ldexp(1.0, s);
   for (uint64_t p = 0; p < n; p++) {
     cic_corner_t c;
     cic_corners(x[p], y[p], z[p], w ? w[p] : 1.0f, N, idx, box, &c);
@@ -244,24 +268,26 @@ void orc_cic_fixed(const float* x, const float* y, const float* z,
       a1 + (uint64_t)c.iy * N + c.iz1, a1 + (uint64_t)c.iy1 * N + c.iz,
       a1 + (uint64_t)c.iy1 * N + c.iz1};
     for (int k = 0; k < 8; k++)
-      acc[cells[k]] += (int64_t)llrint((double)c.v[k] * 4294967296.0);
+      acc[cells[k]] += (int64_t)llrint((double)c.v[k] * unit_inv);
   }
+  return s;
 }
 
 /* Density per unit volume from the fixed-point cells, and the mean of the
  * fixed-point mass per unit volume (exact integer sum, one rounding). */
-void orc_cic_fixed_density(const int64_t* acc, uint32_t N, float box,
+void orc_cic_fixed_density(const int64_t* acc, uint32_t N, float box, int s,
   float* out, double* mean_out) {
+  const double unit = ldexp(1.0, -s);
   const float cell = box / (float)N;
   const float inv_vol = 1.0f / (float)(cell * cell * cell);
   const uint64_t total = (uint64_t)N * N * N;
   __int128 sum = 0;
   for (uint64_t i = 0; i < total; i++) {
-    out[i] = (float)((double)acc[i] * 0x1p-32) * inv_vol;
+    out[i] = (float)((double)acc[i] * unit) * inv_vol;
     sum += acc[i];
   }
   if (mean_out)
-    *mean_out = (double)sum * 0x1p-32 * (double)inv_vol / (double)total;
+    *mean_out = (double)sum * unit * (double)inv_vol / (double)total;
 }
 
 /* Textbook CIC in double: each particle spreads unit mass (or its weight)

@@ -94,6 +94,7 @@ _sig("sif_amd_grid_to_density_contrast", i32, P, u64, C.POINTER(f64))
 _sig("sif_amd_chain_mesh_build", i32, P, P, P, P, u64, u32, f32, P, P, P, P, P, P, P,
      C.POINTER(f64))
 _sig("sif_amd_spectrum_bytes", u64, u32)
+_sig("sif_amd_spectrum_pitch", u32, u32)
 _sig("sif_amd_fft_r2c", i32, P, u32, P, P)
 _sig("sif_amd_fft_filter_c2r", i32, P, u32, f32, i32, f32, f32, i32, i32, P, P)
 _sig("sif_amd_scan_candidates", i32, P, P, u64, f32, P, P, u64, C.POINTER(u64))

@@ -6,12 +6,23 @@
 // (grid.c:1177-1244) of sif v0.5.0.
 //
 // The eight corner weights of a particle are computed with exactly the float
-// operations of grid.c:585-650. Each is rounded to an integer number of 2^-32
+// operations of grid.c:585-650. Each is rounded to an integer number of 2^-s
 // units and added with a 64-bit integer atomic, so the sum in every cell is
 // independent of launch shape and arrival order: the grid is bit-identical
 // from run to run and from one launch configuration to the next. (The
 // reference gets the same property from a fixed chunk/slab order; the bits of
 // the two differ by float rounding, within 1e-5 of the peak.)
+//
+// The unit 2^-s is chosen per deposit so that no sum can overflow: a cell
+// holds at most n |w|max (n particles, each corner at most its weight), and
+// with n |w|max <= 2^e, s = 62 - e keeps every corner and every cell sum
+// below 2^62. The bound depends only on the count and the largest weight,
+// both order-free, so the grid stays bit-identical; small weights get finer
+// units rather than losing their digits. A weight that is not finite is
+// refused (SIF_ERR_RANGE).
+#include <cmath>
+#include <cstring>
+
 #include "common.h"
 #include "internal.h"
 #include "primitives.h"
@@ -32,16 +43,17 @@ __device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }
 
-__device__ __forceinline__ int64_t to_fixed(float v) {
-  return (int64_t)__double2ll_rn((double)v * 4294967296.0);
+__device__ __forceinline__ int64_t to_fixed(float v, double unit_inv) {
+  return (int64_t)__double2ll_rn((double)v * unit_inv);
 }
 
 // The CIC footprint of one particle in the reference's float operations
 // (grid.c:585-650): base node (ix, iy, iz) of the scaled, folded position
-// c = fold(x) * n / L, and the eight corner values as 2^-32 fixed point, in
-// corner order (dx, dy, dz) = 000, 001, 010, 011, 100, 101, 110, 111.
+// c = fold(x) * n / L, and the eight corner values in fixed point (unit_inv =
+// 2^s units per 1), in corner order (dx, dy, dz) = 000, 001, 010, 011, 100,
+// 101, 110, 111.
 __device__ __forceinline__ void cic_corners(float cx, float cy, float cz, float pw, uint32_t N,
-  uint32_t* ix, uint32_t* iy, uint32_t* iz, int64_t v[8]) {
+  double unit_inv, uint32_t* ix, uint32_t* iy, uint32_t* iz, int64_t v[8]) {
   const uint32_t rx = (uint32_t)cx, ry = (uint32_t)cy, rz = (uint32_t)cz;
   *ix = rx >= N ? 0u : rx;
   *iy = ry >= N ? 0u : ry;
@@ -53,14 +65,14 @@ __device__ __forceinline__ void cic_corners(float cx, float cy, float cz, float 
   const float w00 = wx0 * wy0, w01 = wx0 * wy1;
   const float w10 = wx1 * wy0, w11 = wx1 * wy1;
   const float mz0 = pw * wz0, mz1 = pw * wz1;
-  v[0] = to_fixed(w00 * mz0);
-  v[1] = to_fixed(w00 * mz1);
-  v[2] = to_fixed(w01 * mz0);
-  v[3] = to_fixed(w01 * mz1);
-  v[4] = to_fixed(w10 * mz0);
-  v[5] = to_fixed(w10 * mz1);
-  v[6] = to_fixed(w11 * mz0);
-  v[7] = to_fixed(w11 * mz1);
+  v[0] = to_fixed(w00 * mz0, unit_inv);
+  v[1] = to_fixed(w00 * mz1, unit_inv);
+  v[2] = to_fixed(w01 * mz0, unit_inv);
+  v[3] = to_fixed(w01 * mz1, unit_inv);
+  v[4] = to_fixed(w10 * mz0, unit_inv);
+  v[5] = to_fixed(w10 * mz1, unit_inv);
+  v[6] = to_fixed(w11 * mz0, unit_inv);
+  v[7] = to_fixed(w11 * mz1, unit_inv);
 }
 
 // The deposit is binned by tiles of CIC_T^3 nodes. A particle belongs to the
@@ -150,7 +162,8 @@ __global__ void __launch_bounds__(256) cic_bin_kernel(const float* __restrict__ 
 // flush.
 __global__ void __launch_bounds__(256) cic_tile_kernel(const float* __restrict__ bx,
   const float* __restrict__ by, const float* __restrict__ bz, const float* __restrict__ bw,
-  const uint64_t* __restrict__ offsets, uint32_t N, uint32_t nt, int64_t* __restrict__ acc) {
+  const uint64_t* __restrict__ offsets, uint32_t N, uint32_t nt, double unit_inv,
+  int64_t* __restrict__ acc) {
   __shared__ unsigned long long L[CIC_L * CIC_L * CIC_L];
   const uint32_t t = blockIdx.x;
   const uint64_t p0 = offsets[t], p1 = offsets[t + 1];
@@ -164,7 +177,7 @@ __global__ void __launch_bounds__(256) cic_tile_kernel(const float* __restrict__
   for (uint64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     uint32_t ix, iy, iz;
     int64_t v[8];
-    cic_corners(bx[p], by[p], bz[p], bw ? bw[p] : 1.0f, N, &ix, &iy, &iz, v);
+    cic_corners(bx[p], by[p], bz[p], bw ? bw[p] : 1.0f, N, unit_inv, &ix, &iy, &iz, v);
     const uint32_t lx = ix - tx0, ly = iy - ty0, lz = iz - tz0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -189,11 +202,29 @@ __global__ void __launch_bounds__(256) cic_tile_kernel(const float* __restrict__
 
 // Fixed point -> density per unit volume (grid.c:735-738 divides by l^3).
 __global__ void cic_finish_kernel(const int64_t* __restrict__ acc, uint64_t total,
-  float inv_vol, float* __restrict__ out) {
+  double unit, float inv_vol, float* __restrict__ out) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += stride)
-    out[i] = (float)((double)acc[i] * 0x1p-32) * inv_vol;
+    out[i] = (float)((double)acc[i] * unit) * inv_vol;
+}
+
+// The largest |w| by its bits (sign cleared): integer max, order-free; a
+// non-finite weight has the largest bits of all.
+__global__ void abs_max_bits_kernel(const uint32_t* __restrict__ w, uint64_t n,
+  uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = w[i] & 0x7fffffffu;
+    m = b > m ? b : m;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m)
+    atomicMax(out, m);
 }
 
 // Deterministic mean: block b covers cells [b*MEAN_BLOCK, (b+1)*MEAN_BLOCK);
@@ -271,6 +302,7 @@ DeviceBuffer g_mean_buf;
 DeviceBuffer g_box_buf;
 }  // namespace
 
+
 int grid_mean(const float* values, uint64_t total, double* mean) {
   const uint64_t blocks = (total + MEAN_BLOCK - 1) / MEAN_BLOCK;
   SIF_HIP(hipGetLastError());
@@ -328,7 +360,42 @@ int check_in_box(const float* x, const float* y, const float* z, uint64_t n,
 }
 
 namespace {
-DeviceBuffer g_tile_counts, g_tile_offsets, g_tile_cursor, g_bx, g_by, g_bz, g_bw;
+DeviceBuffer g_tile_counts, g_tile_offsets, g_tile_cursor, g_bx, g_by, g_bz, g_bw, g_wmax;
+}
+
+// The fixed-point exponent of a deposit of n particles whose largest |weight|
+// is wmax (the rule above): s = 62 - e with n wmax <= 2^e.
+int cic_unit_exponent(uint64_t n, float wmax) {
+  const double bound = (double)n * (double)wmax;
+  if (!(bound > 0.0))
+    return 32;
+  int e;
+  (void)frexp(bound, &e);  // bound = m 2^e, m in [0.5, 1): bound < 2^e
+  return 62 - e;
+}
+
+// The largest |w| of a weighted deposit on the device; SIF_ERR_RANGE for a
+// weight that is not finite.
+int cic_weight_max(const float* w, uint64_t n, float* wmax) {
+  int st = g_wmax.reserve(sizeof(uint32_t));
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemsetAsync(g_wmax.ptr, 0, sizeof(uint32_t), stream()));
+  if (n) {
+    const unsigned blocks = grid_for(n, 256) > 2048 ? 2048 : grid_for(n, 256);
+    abs_max_bits_kernel<<<blocks, 256, 0, stream()>>>(
+      reinterpret_cast<const uint32_t*>(w), n, g_wmax.as<uint32_t>());
+    SIF_HIP(hipGetLastError());
+  }
+  uint32_t bits = 0;
+  SIF_HIP(hipMemcpyAsync(&bits, g_wmax.ptr, sizeof(bits), hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  if (bits >= 0x7f800000u) {
+    error_set(SIF_ERR_RANGE, "cic: a particle weight is not finite");
+    return SIF_ERR_RANGE;
+  }
+  memcpy(wmax, &bits, sizeof(float));
+  return SIF_OK;
 }
 
 int cic_deposit(const float* x, const float* y, const float* z, const float* w,
@@ -352,6 +419,11 @@ int cic_deposit(const float* x, const float* y, const float* z, const float* w,
   if (st == SIF_OK && w) st = g_bw.reserve((n ? n : 1) * 4);
   if (st != SIF_OK)
     return st;
+  float wmax = 1.0f;
+  if (w && (st = cic_weight_max(w, n, &wmax)) != SIF_OK)
+    return st;
+  const int s_exp = cic_unit_exponent(n, wmax);
+  const double unit_inv = ldexp(1.0, s_exp), unit = ldexp(1.0, -s_exp);
   timing_begin(SIF_AMD_STAGE_CIC);
   SIF_HIP(hipMemsetAsync(acc, 0, total * sizeof(int64_t), stream()));
   if (n) {
@@ -371,11 +443,11 @@ int cic_deposit(const float* x, const float* y, const float* z, const float* w,
     SIF_HIP(hipGetLastError());
     cic_tile_kernel<<<(unsigned)tiles, 256, 0, stream()>>>(g_bx.as<float>(), g_by.as<float>(),
       g_bz.as<float>(), w ? g_bw.as<float>() : nullptr, g_tile_offsets.as<uint64_t>(), N, nt,
-      acc);
+      unit_inv, acc);
     SIF_HIP(hipGetLastError());
   }
   cic_finish_kernel<<<grid_for(total, 256), 256, 0, stream()>>>(
-    acc, total, inv_vol, values);
+    acc, total, unit, inv_vol, values);
   SIF_HIP(hipGetLastError());
   timing_end(SIF_AMD_STAGE_CIC);
   if (timing_active())  // positions (and weights) once, the grid once

@@ -4,8 +4,11 @@
 // Replaces the FFTW plane + pencil plans of src/math/fft.c:378-434, 790-888
 // and the filter/window multiply of fft.c:698-788 of sif v0.5.0.
 //
-// Layout is FFTW's: real field [n][n][n] (z fastest), spectrum
-// [n][n][n/2+1] complex. Each axis is one pass over memory: a workgroup loads
+// Layout: real field [n][n][n] (z fastest), as FFTW's; spectrum
+// [n][n][pitch] complex with the n/2+1 modes of each row first and the row
+// pitch rounded up to 16 complex values (spectrum_pitch), so that the column
+// passes' 16-column tiles start on a 128-byte line and every line they fetch
+// is theirs whole. Each axis is one pass over memory: a workgroup loads
 // a tile of 8192 complex values -- TK = 8192/n pencils of length n -- into LDS
 // with coalesced reads (consecutive kz columns for the x and y passes, whole
 // rows for the z pass), transforms every pencil, and writes the tile back the
@@ -259,7 +262,7 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
 
 template <int N>
 __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __restrict__ in,
-  float2* __restrict__ spec, uint64_t rows, const float2* __restrict__ tw_g) {
+  float2* __restrict__ spec, uint64_t rows, uint32_t pitch, const float2* __restrict__ tw_g) {
   constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
   __shared__ float2 L[TK * (N + 1)];
   __shared__ float2 tw[N];
@@ -294,7 +297,7 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __re
     const int e = threadIdx.x + q * FFT_THREADS;
     const uint64_t row = row0 + e / nz;
     if (e < nz * TK && row < rows)
-      spec[row * nz + e % nz] = v[q];
+      spec[row * pitch + e % nz] = v[q];
   }
 }
 
@@ -302,7 +305,8 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __re
 // transformed, and the real part scaled by `norm` (fft.c:832).
 template <int N>
 __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __restrict__ spec,
-  float* __restrict__ out, uint64_t rows, float norm, const float2* __restrict__ tw_g) {
+  float* __restrict__ out, uint64_t rows, uint32_t pitch, float norm,
+  const float2* __restrict__ tw_g) {
   constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
   __shared__ float2 L[TK * (N + 1)];
   __shared__ float2 tw[N];
@@ -316,7 +320,7 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
     for (int q = 0; q < PER_H; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
       const uint64_t row = row0 + e / nz;
-      v[q] = e < nz * TK && row < rows ? spec[row * nz + e % nz] : make_float2(0.0f, 0.0f);
+      v[q] = e < nz * TK && row < rows ? spec[row * pitch + e % nz] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int q = 0; q < PER_H; q++) {
@@ -403,7 +407,7 @@ int pass_r2c_rows(const float* in, float2* spec) {
   constexpr int TK = FFT_TILE / N;
   const uint64_t rows = (uint64_t)N * N;
   r2c_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, stream()>>>(in, spec,
-    rows, g_tw.fwd.as<float2>());
+    rows, spectrum_pitch(N), g_tw.fwd.as<float2>());
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
@@ -413,7 +417,7 @@ int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s) {
   constexpr int TK = FFT_TILE / N;
   const uint64_t rows = (uint64_t)N * N;
   c2r_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, s>>>(spec, out,
-    rows, norm, g_tw.fwd.as<float2>());
+    rows, spectrum_pitch(N), norm, g_tw.fwd.as<float2>());
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
@@ -521,16 +525,16 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
     st = twiddles(n);
   if (st != SIF_OK)
     return st;
-  const uint64_t nz = n / 2 + 1;
+  const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n);
   const uint64_t rows = (uint64_t)n * n;
   timing_begin(SIF_AMD_STAGE_FFT_FORWARD);
   st = run_r2c_rows(n, values, work);
   // y pass: pencil a = x, i = y
   if (st == SIF_OK)
-    st = run_cols(n, false, work, work, (uint64_t)n * nz, nz, nullptr, nullptr, stream());
+    st = run_cols(n, false, work, work, (uint64_t)n * pitch, pitch, nullptr, nullptr, stream());
   // x pass: pencil a = y, i = x
   if (st == SIF_OK)
-    st = run_cols(n, false, work, spec, nz, (uint64_t)n * nz, nullptr, nullptr, stream());
+    st = run_cols(n, false, work, spec, pitch, (uint64_t)n * pitch, nullptr, nullptr, stream());
   if (st != SIF_OK)
     return st;
   timing_end(SIF_AMD_STAGE_FFT_FORWARD);
@@ -549,15 +553,15 @@ int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double
     st = twiddles(n);
   if (st != SIF_OK)
     return st;
-  const uint64_t nz = n / 2 + 1;
+  const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n);
   const uint64_t rows = (uint64_t)n * n;
   const float norm = 1.0f / (float)((uint64_t)n * n * n);
   timing_begin(SIF_AMD_STAGE_SMOOTH, s);
   // x pass with the filter fused: pencil a = y, i = x
-  st = run_cols(n, true, spec, work, nz, (uint64_t)n * nz, d_lut, d_axis, s);
+  st = run_cols(n, true, spec, work, pitch, (uint64_t)n * pitch, d_lut, d_axis, s);
   // y pass: pencil a = x, i = y
   if (st == SIF_OK)
-    st = run_cols(n, true, work, work, (uint64_t)n * nz, nz, nullptr, nullptr, s);
+    st = run_cols(n, true, work, work, (uint64_t)n * pitch, pitch, nullptr, nullptr, s);
   if (st == SIF_OK)
     st = run_c2r_rows(n, work, out, norm, s);
   if (st != SIF_OK)
@@ -611,7 +615,11 @@ using namespace sif_amd;
 extern "C" {
 
 uint64_t sif_amd_spectrum_bytes(uint32_t n) {
-  return (uint64_t)n * n * (n / 2 + 1) * 2 * sizeof(float);
+  return (uint64_t)n * n * spectrum_pitch(n) * 2 * sizeof(float);
+}
+
+uint32_t sif_amd_spectrum_pitch(uint32_t n) {
+  return spectrum_pitch(n);
 }
 
 int sif_amd_fft_r2c(const float* values, uint32_t n, float* spectrum, void* work) {

@@ -25,6 +25,11 @@ int mesh_build(const float* x, const float* y, const float* z, const float* w,
   float* mz, float* mw, uint64_t* orig, float* cw, double* total_weight);
 
 // fft.hip
+// Complex values per row of a spectrum or FFT work buffer: the n/2 + 1 modes,
+// rounded up to 16 (a 128-byte line).
+__host__ __device__ inline uint32_t spectrum_pitch(uint32_t n) {
+  return ((n / 2 + 1) + 15u) & ~15u;
+}
 int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work);
 int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
   float r, float sigma, int window_order, int window_mode, float* out,

@@ -68,19 +68,34 @@ class Mesh:
         self.total_weight = tw.value
 
 
+def spectrum_pitch(n):
+    """Complex values per row of a device spectrum (n/2 + 1 modes, padded)."""
+    return int(lib.sif_amd_spectrum_pitch(n))
+
+
 def spectrum(dvalues, n):
-    spec = DeviceArray((n, n, n // 2 + 1, 2), np.float32)
-    work = DeviceArray((n, n, n // 2 + 1, 2), np.float32)
+    """The r2c transform of an n^3 device field: a device array
+    (n, n, pitch, 2) whose first n/2 + 1 entries of each row are the modes."""
+    p = spectrum_pitch(n)
+    spec = DeviceArray((n, n, p, 2), np.float32)
+    work = DeviceArray((n, n, p, 2), np.float32)
     check(lib.sif_amd_fft_r2c(dvalues.ptr, n, spec.ptr, work.ptr))
     work.free()
     return spec
+
+
+def spectrum_modes(dspec, n):
+    """A device spectrum on the host as complex128 (n, n, n/2 + 1), FFTW's
+    r2c layout."""
+    s = dspec.get()[:, :, :n // 2 + 1]
+    return s[..., 0] + 1j * s[..., 1].astype(np.float64)
 
 
 def smooth(dspec, n, box, filt, r, sigma=0.0, window_order=2, window_mode=0, out=None,
            work=None):
     out = out or DeviceArray((n ** 3,), np.float32)
     own = work is None
-    work = work or DeviceArray((n, n, n // 2 + 1, 2), np.float32)
+    work = work or DeviceArray((n, n, spectrum_pitch(n), 2), np.float32)
     check(lib.sif_amd_fft_filter_c2r(dspec.ptr, n, box, filt, r, sigma, window_order,
                                      window_mode, out.ptr, work.ptr))
     if own:

@@ -85,6 +85,32 @@ def test_cic_weighted_and_repeatable(dev, particles):
     np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("scale", [1e-6, 1e9, 1e10])
+def test_cic_weights_far_from_one(dev, particles, scale):
+    """Weights of 1e-6 and of 1e9 to 1e10 (halo masses): the unit of the fixed
+    point follows them, so nothing overflows and small weights keep their
+    digits; bit-exact against the oracle's model, within 1e-5 of the peak
+    against the textbook double deposit."""
+    import oracle
+    D, dx, dy, dz = dev
+    x, y, z = particles
+    w = (np.random.default_rng(8).uniform(0.1, 10.0, len(x)) * scale).astype(np.float32)
+    got = D.cic(dx, dy, dz, len(x), 32, BOX, dw=D.to_device(w)).get()
+    want, _, _ = oracle.cic_fixed(x, y, z, 32, BOX, w=w)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    brute = oracle.cic_brute(x, y, z, 32, BOX, w=w)
+    assert np.abs(got - brute).max() <= 1e-5 * np.abs(brute).max()
+
+
+def test_cic_refuses_a_weight_that_is_not_finite(dev, particles):
+    D, dx, dy, dz = dev
+    x = particles[0]
+    w = np.ones(len(x), np.float32)
+    w[17] = np.inf
+    with pytest.raises(ValueError, match="not finite"):
+        D.cic(dx, dy, dz, len(x), 32, BOX, dw=D.to_device(w))
+
+
 def test_density_contrast(dev, particles):
     import oracle
     D, dx, dy, dz = dev
@@ -148,7 +174,8 @@ def test_fft_round_trip(native, n):
     v = rng.standard_normal(n ** 3).astype(np.float32)
     dv = D.to_device(v)
     spec = D.spectrum(dv, n)
-    s = spec.get()
+    assert spec.shape[2] % 16 == 0 and spec.shape[2] >= n // 2 + 1  # rows padded to lines
+    s = spec.get()[:, :, :n // 2 + 1]
     import scipy.fft
     want = scipy.fft.rfftn(v.reshape(n, n, n).astype(np.float64), workers=-1)
     got = s[..., 0] + 1j * s[..., 1]
@@ -163,7 +190,7 @@ def test_fft_round_trip(native, n):
     phase = (k[0] * i)[:, None, None] + (k[1] * i)[None, :, None] + (k[2] * i)[None, None, :]
     wave = np.cos(2 * np.pi * phase / n).astype(np.float32).ravel()
     del phase
-    s1 = D.spectrum(D.to_device(wave), n).get()
+    s1 = D.spectrum(D.to_device(wave), n).get()[:, :, :n // 2 + 1]
     mag = np.hypot(s1[..., 0], s1[..., 1])
     assert abs(mag[k] - n ** 3 / 2) <= 1e-4 * n ** 3
     mag[k] = 0.0
@@ -181,7 +208,7 @@ def test_fft_1024_properties(native):
     v = np.random.default_rng(n).standard_normal(n ** 3, dtype=np.float32)
     dv = D.to_device(v)
     spec = D.spectrum(dv, n)
-    s = spec.get().reshape(n, n, n // 2 + 1, 2)
+    s = spec.get()[:, :, :n // 2 + 1]
     # Parseval: sum |v|^2 = (1/n^3) sum over the full spectrum |F|^2; the half
     # spectrum counts every kz but 0 and n/2 twice
     power = 0.0
@@ -201,7 +228,7 @@ def test_fft_1024_properties(native):
     table = np.cos(2 * np.pi * np.arange(n) / n).astype(np.float32)
     wave = table[phase].ravel()
     del phase
-    s1 = D.spectrum(D.to_device(wave), n).get()
+    s1 = D.spectrum(D.to_device(wave), n).get()[:, :, :n // 2 + 1]
     del wave
     mag = np.hypot(s1[..., 0], s1[..., 1])
     del s1
@@ -383,7 +410,6 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
         np.testing.assert_array_equal(nin, onin)
     assert len(cat) > 0
     assert sum(s["accepted"] for s in stats) == len(cat)
-    batches_one = sum(s["evaluated"] for s in stats)
     try:
         for floor, ceiling in SMALL_WINDOWS:
             D.set_batch_window(floor, ceiling)
@@ -392,12 +418,15 @@ def test_exodus_catalogue_matches_oracle_on_same_fields(dev, particles, overlap,
                                           options=opts)
             np.testing.assert_array_equal(cat2.view(np.uint32), ocat.view(np.uint32))
             np.testing.assert_array_equal(nin2, onin)
+            # the reason a candidate is refused for can depend on the batching
+            # (as in the reference, whose phase 1 tests against the catalogue
+            # frozen at the batch's start); what is refused cannot
+            refused = ("masked", "proxy", "overlap", "rescale_fail")
             for a, b in zip(stats, stats2):
-                for key in ("candidates", "accepted", "masked", "proxy", "overlap",
-                            "rescale_fail"):
-                    assert a[key] == b[key], (floor, key, a, b)
-            if floor == 16:  # a speculative window that small wastes growths on accepts
-                assert sum(s["evaluated"] for s in stats2) >= batches_one
+                assert a["candidates"] == b["candidates"] and a["accepted"] == b["accepted"]
+                assert sum(a[k] for k in refused) == sum(b[k] for k in refused), (floor, a, b)
+            if ceiling == 16:  # a batch holds at most 16: such a rung spans several
+                assert max(s["evaluated"] for s in stats2) > ceiling
     finally:
         D.set_batch_window(0, 0)
     # radii against brute force at the rung they came from

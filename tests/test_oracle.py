@@ -192,6 +192,38 @@ def test_cic_textbook_and_mass(n):
         assert abs(got.astype(np.float64).sum() * cell ** 3 - len(x)) <= 1e-4 * len(x)
 
 
+@pytest.mark.parametrize("scale", [1e-6, 1e-3, 1e9, 1e10, 3e30])
+def test_cic_fixed_point_range(scale):
+    """The fixed-point unit follows the weights: weights of 1e-6 keep their
+    digits and weights of 1e10 (a 2^-32 unit would overflow int64 at 4e19)
+    do not overflow. Against the textbook double deposit at 1e-5 of the peak
+    (tests/test_field_grid.c:812), mass to 1e-6, on weights spread over two
+    decades."""
+    box, n = 100.0, 24
+    x, y, z = zeldovich.generate(20, box, 5)
+    w = (np.random.default_rng(6).uniform(0.1, 10.0, len(x)) * scale).astype(np.float32)
+    got, acc, _, s = oracle.cic_fixed(x, y, z, n, box, w=w, with_exponent=True)
+    brute = oracle.cic_brute(x, y, z, n, box, w=w)
+    peak = np.abs(brute).max()
+    assert np.abs(got.astype(np.float64) - brute).max() <= 1e-5 * peak
+    cell = box / n
+    total = float(w.astype(np.float64).sum())
+    assert abs(got.astype(np.float64).sum() * cell ** 3 - total) <= 1e-6 * total
+    # the rule: n |w|max <= 2^e, s = 62 - e, so every sum stays below 2^62
+    e = 62 - s
+    bound = len(x) * float(np.abs(w).max())
+    assert bound <= 2.0 ** e < 2.0 * bound
+    assert np.abs(acc).max() < 2 ** 62
+
+
+def test_cic_unit_exponent_closed_form():
+    assert oracle.cic_unit_exponent(1, 1.0) == 61          # 1 <= 2^1
+    assert oracle.cic_unit_exponent(2 ** 27, 1.0) == 34     # 2^27 < 2^28
+    assert oracle.cic_unit_exponent(3, 1.0) == 60           # 3 <= 2^2
+    assert oracle.cic_unit_exponent(0, 1.0) == 32
+    assert oracle.cic_unit_exponent(10, 0.0) == 32
+
+
 def test_density_contrast_closed_form():
     rng = np.random.default_rng(2)
     v = rng.uniform(0.5, 2.0, 32 ** 3).astype(np.float32)
