@@ -76,6 +76,18 @@ SIF_API int sif_amd_grid_mean(const float* d_values, uint64_t total_cells,
 SIF_API int sif_amd_grid_to_density_contrast(float* d_values,
   uint64_t total_cells, double* mean_out);
 
+/* No counterpart in the reference: sif_amd_grid_to_density_contrast for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The mean's block
+ * sums go to a fixed scratch array in device memory that the library holds
+ * per device until the process ends, so grids of at most 2^31 cells
+ * (SIF_ERR_RANGE beyond); sif_system_init() must have run
+ * (SIF_ERR_UNINITIALISED). Same mean and contrast, bit for bit, as
+ * sif_amd_grid_to_density_contrast. Calls on one device share the scratch:
+ * calls that overlap on a device race. */
+SIF_API int sif_amd_grid_to_density_contrast_noalloc(float* d_values,
+  uint64_t total_cells, double* mean_out);
+
 /* --- chain mesh --- */
 /* chain_mesh_create + canonical order (src/structures/chain_mesh.c:680-832,
  * :364-542): d_offsets has n_cells^3 + 1 entries; d_w, d_mw,

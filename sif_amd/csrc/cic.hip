@@ -22,6 +22,7 @@
 // refused (SIF_ERR_RANGE).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 #include "internal.h"
@@ -302,6 +303,52 @@ DeviceBuffer g_mean_buf;
 DeviceBuffer g_box_buf;
 }  // namespace
 
+// --- the density contrast with no allocation ---
+//
+// The block sums of the mean live in `scratch`, a fixed array in device
+// memory (one per device the library runs on, held until the process ends),
+// for callers that cannot allocate. Its last slot receives their ordered sum.
+// Calls on one device share it: calls that overlap on a device race.
+constexpr uint32_t NOALLOC_MEAN_BLOCKS = 32768;  // grids of up to 2^31 cells
+static __device__ __align__(256) double scratch[NOALLOC_MEAN_BLOCKS + 1];
+
+// The block sums added in block order by one thread: the same additions, in
+// the same order, as grid_mean's loop on the host.
+__global__ void ordered_sum_kernel(const double* __restrict__ blocks, uint64_t n,
+  double* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double sum = 0.0;
+    for (uint64_t b = 0; b < n; b++)
+      sum += blocks[b];
+    *out = sum;
+  }
+}
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call there (the name is a host-side stand-in, not an address).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  SIF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_DEVICES) {
+    error_set(SIF_ERR_UNSUPPORTED, "device %d: beyond the %d devices this build serves", dev,
+      SCRATCH_DEVICES);
+    return SIF_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SIF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SIF_OK;
+}
+}  // namespace
 
 int grid_mean(const float* values, uint64_t total, double* mean) {
   const uint64_t blocks = (total + MEAN_BLOCK - 1) / MEAN_BLOCK;
@@ -516,6 +563,52 @@ int sif_amd_grid_to_density_contrast(float* values, uint64_t total_cells,
   }
   st = density_contrast(values, total_cells, mean);
   timing_end(SIF_AMD_STAGE_CONTRAST);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+int sif_amd_grid_to_density_contrast_noalloc(float* values, uint64_t total_cells,
+  double* mean_out) {
+  if (!initialised()) {
+    error_set(SIF_ERR_UNINITIALISED,
+      "sif_amd_grid_to_density_contrast_noalloc: call sif_system_init first");
+    return SIF_ERR_UNINITIALISED;
+  }
+  if (!values || total_cells == 0) {
+    error_set(SIF_ERR_INVALID, "sif_amd_grid_to_density_contrast_noalloc: no grid");
+    return SIF_ERR_INVALID;
+  }
+  const uint64_t blocks = (total_cells + MEAN_BLOCK - 1) / MEAN_BLOCK;
+  if (blocks > NOALLOC_MEAN_BLOCKS) {
+    error_set(SIF_ERR_RANGE,
+      "sif_amd_grid_to_density_contrast_noalloc: %llu cells (at most %llu without "
+      "allocation)", (unsigned long long)total_cells,
+      (unsigned long long)NOALLOC_MEAN_BLOCKS * MEAN_BLOCK);
+    return SIF_ERR_RANGE;
+  }
+  double* d_scratch = nullptr;
+  int st = noalloc_scratch(&d_scratch);
+  if (st != SIF_OK)
+    return st;
+  // every block sum the call reads is written by it first: nothing to clear
+  mean_blocks_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(values, total_cells, d_scratch);
+  SIF_HIP(hipGetLastError());
+  ordered_sum_kernel<<<1, 64, 0, stream()>>>(d_scratch, blocks, d_scratch + NOALLOC_MEAN_BLOCKS);
+  SIF_HIP(hipGetLastError());
+  double sum = 0.0;
+  SIF_HIP(hipMemcpyAsync(&sum, d_scratch + NOALLOC_MEAN_BLOCKS, sizeof(double),
+    hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  const double mean = sum / (double)total_cells;
+  if (mean_out)
+    *mean_out = mean;
+  if (!(mean > 0.0)) {
+    error_set(SIF_ERR_RANGE, "grid: mean density %g (expected > 0)", mean);
+    return SIF_ERR_RANGE;
+  }
+  st = density_contrast(values, total_cells, mean);
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipStreamSynchronize(stream()));
