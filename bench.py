@@ -99,21 +99,25 @@ TRAFFIC = {("plain512", "grow"): "profiles/grow_kernel_hbm_traffic_plain512.md"}
 
 
 def measured_traffic(config, stage):
+    """(bytes per launch, the summary under profiles/ that holds it) or
+    (None, None)."""
     path = TRAFFIC.get((config, stage))
     if not path or not os.path.exists(os.path.join(ROOT, path)):
-        return None
+        return None, None
     for line in open(os.path.join(ROOT, path)):
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
+            return int(line.split(":")[1]), path
+    return None, None
 
 
 def roofline(timing, config):
     """Achieved HBM rate of each timed stage whose algorithmic bytes are
     known (bytes / event-timed duration over the timed region), and that of
-    the dominant kernel: the stage with the most device time per kernel it
-    runs (a stage of several kernels is timed as one region; the smoothing's
-    runs on a second stream beside the commit)."""
+    the dominant one: the stage with the most device time per step (a stage
+    of several kernels is timed as one region; the smoothing's runs on a
+    second stream beside the commit). Its measured traffic is read from the
+    rocprof counter summary named in `traffic_source` (an earlier profiled
+    run of this workload, not this run)."""
     stages = {}
     for name, (ms, launches, by) in timing.items():
         if name not in STAGE_KERNELS or launches == 0 or by <= 0 or ms <= 0:
@@ -123,11 +127,12 @@ def roofline(timing, config):
                         "avg_launch_ms": round(ms / launches, 4),
                         "bytes_per_launch": round(by / launches),
                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    top = max(stages, key=lambda k: timing[k][0] / STAGE_KERNELS[k][1])
+    top = max(stages, key=lambda k: timing[k][0])
     d = stages[top]
-    return {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved"],
+    traffic, source = measured_traffic(config, top)
+    return {"bound": "hbm", "kernel": d["kernel"], "stage": top, "achieved": d["achieved"],
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
-            "traffic": measured_traffic(config, top),
+            "traffic": traffic, "traffic_source": source,
             "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
             "stages": stages}
 
@@ -293,6 +298,36 @@ def adjoint_form(steps, warmup, a=ADJOINT, positions=None, voids=None, what=None
                                    f"grid {a['n_cells']}^3, Gaussian R={a['r_smooth']:g}, "
                                    f"{centres}, "
                                    f"{a['n_bins']} cumulative bins to {a['ext']:g}R"}}
+
+
+def dropin_form(c, x, y, z, radii, mesh_cells, want_voids):
+    """The drop-in path through the host structs of include/sif/ as a caller
+    of sif v0.5.0 uses it (sif_field_*, sif_grid_assign_cic,
+    sif_grid_to_density_contrast, sif_chain_mesh_alloc, sif_finder_exodus_raw),
+    from host arrays to a host catalogue: every call moves its inputs to the
+    device and its results back (PCIe included), one run."""
+    import sif_amd.pysif as pysif
+    from sif_amd.pysif import finders
+    pysif.system_init()
+    t = time.perf_counter()
+    field = pysif.field_from_numpy(x, y, z)
+    grid = pysif.Grid(c["n_cells"], c["box"])
+    grid.assign_cic(field)
+    grid.to_density_contrast()
+    mesh = pysif.ChainMesh(mesh_cells, c["box"], field, drop_indices=True)
+    cat = finders.exodus_raw(grid, mesh, radii, THRESHOLD, 0.0, checkpoint=False,
+                             consume_grid=True)
+    dt = time.perf_counter() - t
+    log(f"[bench] drop-in (host structs): {dt:.2f}s, {cat.n_voids} voids")
+    if cat.n_voids != want_voids:
+        raise RuntimeError(f"drop-in catalogue has {cat.n_voids} voids, the device path "
+                           f"{want_voids}")
+    return {"metric": "exodus particles/s through the host-struct C API (PCIe included)",
+            "value": round(len(x) / dt, 1), "unit": "particles/s", "seconds": round(dt, 3),
+            "voids": int(cat.n_voids),
+            "config": {"workload": "the same particles from host arrays: sif_field, "
+                                   "sif_grid_assign_cic, sif_grid_to_density_contrast, "
+                                   "sif_chain_mesh_alloc, sif_finder_exodus_raw, one run"}}
 
 
 def profiles_form(step, cat, steps, warmup):
@@ -499,6 +534,7 @@ def main():
     else:
         forms["profiles"] = profiles_form(step, cat, args.steps, args.warmup)
         del step
+        forms["dropin"] = dropin_form(c, x, y, z, radii, mesh_cells, len(cat))
         if not args.no_adjoint:
             forms["adjoint"] = adjoint_form(args.steps, args.warmup)
         if not args.no_batched:

@@ -303,7 +303,7 @@ static int window_order_of(sif_option opt) {
 }
 
 int exodus_prepare_spectrum(const float* d_delta, uint32_t n) {
-  const uint64_t sb = (uint64_t)n * n * spectrum_pitch(n) * sizeof(float2);
+  const uint64_t sb = (uint64_t)n * spectrum_plane(n) * sizeof(float2);
   int st = g_dev.spec.reserve(sb);
   if (st == SIF_OK)
     st = g_dev.work.reserve(sb);

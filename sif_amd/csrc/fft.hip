@@ -37,6 +37,22 @@ namespace sif_amd {
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_TILE = 8192;  // complex values per workgroup tile
 
+// LDS layout of a tile: pencil p's element i at p * pitch + i + (i >> SH).
+// The Stockham writes of the early stages stride the pencil by R or R NS
+// elements, which would put a wavefront's 64 stores on a few of the 32
+// banks; one padding slot every 2^SH elements spreads them. The pitch is
+// 1 mod 16 slots at the lengths used, so the column phases (16 pencils side
+// by side) stay conflict-free. SH = 3 where the tile and its twiddles then
+// still fit two workgroups a CU (80 KB each), else 4.
+constexpr int pad_shift(int len, int tw_entries) {
+  return (FFT_TILE / len) * (len + len / 8 + 1) * 8 + tw_entries * 8 <= 80 * 1024 ? 3 : 4;
+}
+template <int N, int SH>
+struct Lay {
+  static constexpr int pitch = N + (N >> SH) + 1;
+  static __device__ __forceinline__ int at(int i) { return i + (i >> SH); }
+};
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -109,16 +125,16 @@ __device__ __forceinline__ void dft(float2* v) {
 // stages whose radices multiply to NS. Thread t of the pencil's N/8 threads
 // does 8/R butterflies j = t + b N/8: inputs X[j + r N/R], twiddles
 // w^(r (j mod NS) N / (NS R)), outputs Y[(j / NS) NS R + j mod NS + r NS].
-template <int N, int R, int NS, bool INV>
+template <int N, int SH, int R, int NS, bool INV>
 __device__ __forceinline__ void stage_read(const float2* X, int t, float2* v) {
   constexpr int T = N / 8;
 #pragma unroll
   for (int b = 0; b < 8 / R; b++)
 #pragma unroll
     for (int r = 0; r < R; r++)
-      v[b * R + r] = X[t + b * T + r * (N / R)];
+      v[b * R + r] = X[Lay<N, SH>::at(t + b * T + r * (N / R))];
 }
-template <int N, int R, int NS, bool INV>
+template <int N, int SH, int R, int NS, bool INV>
 __device__ __forceinline__ void stage_write(float2* Y, int t, float2* v, const float2* tw) {
   constexpr int T = N / 8;
 #pragma unroll
@@ -138,7 +154,7 @@ __device__ __forceinline__ void stage_write(float2* Y, int t, float2* v, const f
     const int d = (j / NS) * NS * R + jm;
 #pragma unroll
     for (int r = 0; r < R; r++)
-      Y[d + r * NS] = v[b * R + r];
+      Y[Lay<N, SH>::at(d + r * NS)] = v[b * R + r];
   }
 }
 
@@ -146,42 +162,43 @@ __device__ __forceinline__ void stage_write(float2* Y, int t, float2* v, const f
 // the rest are radix 8.
 template <int N>
 struct Plan {
-  static constexpr int L = N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6 : N == 128 ? 7 :
-                           N == 256 ? 8 : N == 512 ? 9 : 10;
+  static constexpr int L = N == 8 ? 3 : N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6 :
+                           N == 128 ? 7 : N == 256 ? 8 : N == 512 ? 9 : 10;
   static constexpr int R0 = (L % 3 == 0) ? 8 : (1 << (L % 3));
 };
 
-// Every pencil of the tile (TK pencils, pitch N + 1) through all stages. Each
-// of the 256 threads serves four pencils; the tile is transformed in place,
-// a workgroup barrier between the reads and the writes of a stage.
-template <int N, int R, int NS, bool INV>
+// Every pencil of the tile (FFT_TILE / N pencils, layout Lay<N, SH>) through
+// all stages. The 256 threads serve the pencils in groups; the tile is
+// transformed in place, a workgroup barrier between the reads and the writes
+// of a stage.
+template <int N, int SH, int R, int NS, bool INV>
 __device__ __forceinline__ void tile_stage(float2* L, const float2* tw) {
   constexpr int T = N / 8, PG = FFT_THREADS / T, PP = FFT_TILE / N / PG;
   const int t = threadIdx.x % T, g = threadIdx.x / T;
   float2 v[PP][8];
 #pragma unroll
   for (int p = 0; p < PP; p++)
-    stage_read<N, R, NS, INV>(L + (p * PG + g) * (N + 1), t, v[p]);
+    stage_read<N, SH, R, NS, INV>(L + (p * PG + g) * Lay<N, SH>::pitch, t, v[p]);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PP; p++)
-    stage_write<N, R, NS, INV>(L + (p * PG + g) * (N + 1), t, v[p], tw);
+    stage_write<N, SH, R, NS, INV>(L + (p * PG + g) * Lay<N, SH>::pitch, t, v[p], tw);
   __syncthreads();
 }
 
-template <int N, bool INV, int NS>
+template <int N, int SH, bool INV, int NS>
 __device__ __forceinline__ void tile_rest(float2* L, const float2* tw) {
   if constexpr (NS < N) {
-    tile_stage<N, 8, NS, INV>(L, tw);
-    tile_rest<N, INV, NS * 8>(L, tw);
+    tile_stage<N, SH, 8, NS, INV>(L, tw);
+    tile_rest<N, SH, INV, NS * 8>(L, tw);
   }
 }
 
-template <int N, bool INV>
+template <int N, int SH, bool INV>
 __device__ __forceinline__ void tile_fft(float2* L, const float2* tw) {
   constexpr int R0 = Plan<N>::R0;
-  tile_stage<N, R0, 1, INV>(L, tw);
-  tile_rest<N, INV, R0>(L, tw);
+  tile_stage<N, SH, R0, 1, INV>(L, tw);
+  tile_rest<N, SH, INV, R0>(L, tw);
 }
 
 template <int N>
@@ -200,8 +217,9 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
   float2* __restrict__ out, int nz, uint64_t stride_a, uint64_t stride_i,
   const float2* __restrict__ tw_g, const float* __restrict__ lut,
   const double* __restrict__ axis) {
-  constexpr int TK = FFT_TILE / N;
-  __shared__ float2 L[TK * (N + 1)];
+  constexpr int TK = FFT_TILE / N, SH = pad_shift(N, N);
+  using LY = Lay<N, SH>;
+  __shared__ float2 L[TK * LY::pitch];
   __shared__ float2 tw[N];
   const int tiles = (nz + TK - 1) / TK;
   const int a = blockIdx.x / tiles;
@@ -210,6 +228,25 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
   load_twiddles<N>(tw, tw_g);
   const uint64_t base = (uint64_t)a * stride_a + kz0;
   constexpr int half = N / 2;
+  // i = x index, a = y index, kz0 + c = z index (fft.c:753-780). The filter
+  // factor depends on the x index through kx^2 and the window axis, which is
+  // even (axis[i] = axis[N - i] bit for bit), so the tile's factors are built
+  // once per |kx| (half the table gathers) into the LDS the values go to
+  // next. Their table reads are issued first, so that their latency hides
+  // under the tile's loads.
+  constexpr int FT = (TK * (half + 1) + FFT_THREADS - 1) / FFT_THREADS;
+  const int ky = a > half ? a - N : a;
+  float fs[FT];
+  if (lut) {
+#pragma unroll
+    for (int t = 0; t < FT; t++) {
+      const int f = threadIdx.x + t * FFT_THREADS;
+      const int c = f / (half + 1), kx = f % (half + 1);
+      const int kz = kz0 + c < nz ? kz0 + c : nz - 1;
+      const uint32_t k2 = (uint32_t)(kx * kx) + (uint32_t)(ky * ky) + (uint32_t)(kz * kz);
+      fs[t] = f < TK * (half + 1) ? lut[k2] : 0.0f;
+    }
+  }
   // every load of the thread issued before the first is used
   constexpr int PER = N * TK / FFT_THREADS;
   float2 v[PER];
@@ -220,34 +257,43 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
     v[q] = c < cols ? in[base + (uint64_t)i * stride_i + c] : make_float2(0.0f, 0.0f);
   }
   if (lut) {
-    // i = x index, a = y index, kz0 + c = z index (fft.c:753-780)
-    const int ky = a > half ? a - N : a;
+    float* F = reinterpret_cast<float*>(L);
+    const double ay = axis ? axis[a] : 1.0;
+#pragma unroll
+    for (int t = 0; t < FT; t++) {
+      const int f = threadIdx.x + t * FFT_THREADS;
+      if (f < TK * (half + 1)) {
+        float w = fs[t];
+        if (axis) {
+          const int c = f / (half + 1), kx = f % (half + 1);
+          const int kz = kz0 + c < nz ? kz0 + c : nz - 1;
+          const double cxy = axis[kx] * ay;
+          w = (float)((double)w * cxy * axis[kz]);
+        }
+        F[f] = w;
+      }
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
       const int i = e / TK, c = e % TK;
-      const int kx = i > half ? i - N : i;
-      const int kz = kz0 + c < nz ? kz0 + c : nz - 1;
-      const uint32_t k2 = (uint32_t)(kx * kx) + (uint32_t)(ky * ky) + (uint32_t)(kz * kz);
-      float s = lut[k2];
-      if (axis) {
-        const double cxy = axis[i] * axis[a];
-        s = (float)((double)s * cxy * axis[kz]);
-      }
-      v[q] = make_float2(v[q].x * s, v[q].y * s);
+      const float fs = F[c * (half + 1) + (i > half ? N - i : i)];
+      v[q] = make_float2(v[q].x * fs, v[q].y * fs);
     }
+    __syncthreads();  // the table's LDS takes the tile next
   }
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int e = threadIdx.x + q * FFT_THREADS;
-    L[(e % TK) * (N + 1) + e / TK] = v[q];
+    L[(e % TK) * LY::pitch + LY::at(e / TK)] = v[q];
   }
   __syncthreads();
-  tile_fft<N, INV>(L, tw);
+  tile_fft<N, SH, INV>(L, tw);
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int e = threadIdx.x + q * FFT_THREADS;
-    v[q] = L[(e % TK) * (N + 1) + e / TK];
+    v[q] = L[(e % TK) * LY::pitch + LY::at(e / TK)];
   }
 #pragma unroll
   for (int q = 0; q < PER; q++) {
@@ -258,95 +304,153 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
   }
 }
 
-// --- z pass: TK whole rows ---
+// --- z pass: whole rows, as complex transforms of half the length ---
+//
+// A real row x of length N is transformed as the complex sequence
+// z[m] = x[2m] + i x[2m+1] of length M = N/2, which halves the work of the
+// pass. With Z the length-M DFT of z and W = exp(-2 pi i / N):
+//
+//   forward   X[k] = E[k] + W^k (-i) O[k],  k = 0 .. M,
+//             E[k] = (Z[k] + conj Z[M-k]) / 2,  O[k] = (Z[k] - conj Z[M-k]) / 2,
+//             Z[M] taken as Z[0];
+//   inverse   Z[k] = A[k] + i B[k],  k = 0 .. M-1,
+//             A[k] = X[k] + conj X[M-k],  B[k] = (X[k] - conj X[M-k]) W^-k,
+//             and the length-M inverse of Z is x[2m] + i x[2m+1]
+//
+// (X[k + M] = conj X[M - k] for a real row). Rows are TK2 = FFT_TILE / M to
+// a workgroup, laid out as Lay<M, SH> in LDS (element M of a row, X[M], fits
+// in its padding); the twiddles of the length-M transform (W^2m) and the
+// split's (W^k, k <= M) sit side by side in LDS.
+
+template <int N>
+__device__ __forceinline__ void load_row_twiddles(float2* tw, const float2* __restrict__ g) {
+  constexpr int M = N / 2;
+  for (int i = threadIdx.x; i < M; i += FFT_THREADS)
+    tw[i] = g[2 * i];
+  for (int i = threadIdx.x; i <= M; i += FFT_THREADS)
+    tw[M + i] = g[i];
+}
 
 template <int N>
 __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __restrict__ in,
-  float2* __restrict__ spec, uint64_t rows, uint32_t pitch, const float2* __restrict__ tw_g) {
-  constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
-  __shared__ float2 L[TK * (N + 1)];
-  __shared__ float2 tw[N];
-  load_twiddles<N>(tw, tw_g);
-  const uint64_t row0 = (uint64_t)blockIdx.x * TK;
-  constexpr int PER = N * TK / FFT_THREADS;
-  constexpr int PER_H = (nz * TK + FFT_THREADS - 1) / FFT_THREADS;
+  float2* __restrict__ spec, uint64_t rows, uint32_t pitch, uint64_t plane,
+  const float2* __restrict__ tw_g) {
+  constexpr int M = N / 2, TK2 = FFT_TILE / M, nz = M + 1, SH = pad_shift(M, 2 * M + 1);
+  using LY = Lay<M, SH>;
+  __shared__ float2 L[TK2 * LY::pitch];
+  __shared__ float2 tw[2 * M + 1];
+  load_row_twiddles<N>(tw, tw_g);
+  const uint64_t row0 = (uint64_t)blockIdx.x * TK2;
+  constexpr int PER = M * TK2 / FFT_THREADS;
+  constexpr int PER_H = (nz * TK2 + FFT_THREADS - 1) / FFT_THREADS;
+  const float2* in2 = reinterpret_cast<const float2*>(in);
   {
-    float r[PER];
+    float2 z[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
-      const uint64_t row = row0 + e / N;
-      r[q] = row < rows ? in[row * N + e % N] : 0.0f;
+      const uint64_t row = row0 + e / M;
+      z[q] = row < rows ? in2[row * M + e % M] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
-      L[(e / N) * (N + 1) + e % N] = make_float2(r[q], 0.0f);
+      L[(e / M) * LY::pitch + LY::at(e % M)] = z[q];
     }
   }
   __syncthreads();
-  tile_fft<N, false>(L, tw);
+  tile_fft<M, SH, false>(L, tw);
   float2 v[PER_H];
 #pragma unroll
   for (int q = 0; q < PER_H; q++) {
     const int e = threadIdx.x + q * FFT_THREADS;
-    v[q] = e < nz * TK ? L[(e / nz) * (N + 1) + e % nz] : make_float2(0.0f, 0.0f);
+    v[q] = make_float2(0.0f, 0.0f);
+    if (e < nz * TK2) {
+      const int r = e / nz, k = e % nz;
+      const float2 zk = L[r * LY::pitch + LY::at(k == M ? 0 : k)];
+      const float2 zm = L[r * LY::pitch + LY::at(k == 0 ? 0 : M - k)];
+      const float2 ev = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      const float2 od = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y + zm.y));
+      const float2 w = tw[M + k];
+      v[q] = cadd(ev, cmul(w, make_float2(od.y, -od.x)));
+    }
   }
 #pragma unroll
   for (int q = 0; q < PER_H; q++) {
     const int e = threadIdx.x + q * FFT_THREADS;
     const uint64_t row = row0 + e / nz;
-    if (e < nz * TK && row < rows)
-      spec[row * pitch + e % nz] = v[q];
+    if (e < nz * TK2 && row < rows)
+      spec[(row / N) * plane + (row % N) * pitch + e % nz] = v[q];
   }
 }
 
-// The Hermitian half row is extended to the full row (v(N-k) = conj v(k)),
-// transformed, and the real part scaled by `norm` (fft.c:832).
+// The Hermitian half row (n/2 + 1 modes) to the real row, scaled by `norm`
+// (fft.c:832).
 template <int N>
 __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __restrict__ spec,
-  float* __restrict__ out, uint64_t rows, uint32_t pitch, float norm,
+  float* __restrict__ out, uint64_t rows, uint32_t pitch, uint64_t plane, float norm,
   const float2* __restrict__ tw_g) {
-  constexpr int TK = FFT_TILE / N, nz = N / 2 + 1;
-  __shared__ float2 L[TK * (N + 1)];
-  __shared__ float2 tw[N];
-  load_twiddles<N>(tw, tw_g);
-  const uint64_t row0 = (uint64_t)blockIdx.x * TK;
-  constexpr int PER = N * TK / FFT_THREADS;
-  constexpr int PER_H = (nz * TK + FFT_THREADS - 1) / FFT_THREADS;
+  constexpr int M = N / 2, TK2 = FFT_TILE / M, nz = M + 1, SH = pad_shift(M, 2 * M + 1);
+  using LY = Lay<M, SH>;
+  __shared__ float2 L[TK2 * LY::pitch];
+  __shared__ float2 tw[2 * M + 1];
+  load_row_twiddles<N>(tw, tw_g);
+  const uint64_t row0 = (uint64_t)blockIdx.x * TK2;
+  constexpr int PER = M * TK2 / FFT_THREADS;
+  constexpr int PER_H = (nz * TK2 + FFT_THREADS - 1) / FFT_THREADS;
   {
     float2 v[PER_H];
 #pragma unroll
     for (int q = 0; q < PER_H; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
       const uint64_t row = row0 + e / nz;
-      v[q] = e < nz * TK && row < rows ? spec[row * pitch + e % nz] : make_float2(0.0f, 0.0f);
+      v[q] = e < nz * TK2 && row < rows ? spec[(row / N) * plane + (row % N) * pitch + e % nz]
+                                        : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int q = 0; q < PER_H; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
-      if (e < nz * TK) {
-        const int p = e / nz, k = e % nz;
-        L[p * (N + 1) + k] = v[q];
-        if (k > 0 && k < N - k)
-          L[p * (N + 1) + N - k] = make_float2(v[q].x, -v[q].y);
-      }
+      if (e < nz * TK2)
+        L[(e / nz) * LY::pitch + LY::at(e % nz)] = v[q];
     }
   }
   __syncthreads();
-  tile_fft<N, true>(L, tw);
-  float r[PER];
-#pragma unroll
-  for (int q = 0; q < PER; q++) {
-    const int e = threadIdx.x + q * FFT_THREADS;
-    r[q] = L[(e / N) * (N + 1) + e % N].x * norm;
+  // Z[k] and Z[M - k] from X[k] and X[M - k], each pair by one thread, in
+  // place (X[M] sits past the last Z and is read by the pair k = 0 only)
+  constexpr int PAIRS = M / 2 + 1;
+  for (int e = threadIdx.x; e < PAIRS * TK2; e += FFT_THREADS) {
+    const int r = e / PAIRS, k = e % PAIRS;
+    float2* row = L + r * LY::pitch;
+    const float2 xk = row[LY::at(k)], xm = row[LY::at(M - k)];
+    // W^-k and W^-(M-k) = -W^k conjugated: the forward table conjugated
+    const float2 wk = make_float2(tw[M + k].x, -tw[M + k].y);
+    const float2 a = make_float2(xk.x + xm.x, xk.y - xm.y);
+    const float2 b = cmul(make_float2(xk.x - xm.x, xk.y + xm.y), wk);
+    const float2 zk = make_float2(a.x - b.y, a.y + b.x);
+    if (k > 0 && k < M - k) {
+      const float2 wm = make_float2(tw[2 * M - k].x, -tw[2 * M - k].y);
+      const float2 a2 = make_float2(xm.x + xk.x, xm.y - xk.y);
+      const float2 b2 = cmul(make_float2(xm.x - xk.x, xm.y + xk.y), wm);
+      row[LY::at(M - k)] = make_float2(a2.x - b2.y, a2.y + b2.x);
+    }
+    row[LY::at(k)] = zk;
   }
+  __syncthreads();
+  tile_fft<M, SH, true>(L, tw);
+  float2 r[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int e = threadIdx.x + q * FFT_THREADS;
-    const uint64_t row = row0 + e / N;
+    const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
+    r[q] = make_float2(z.x * norm, z.y * norm);
+  }
+  float2* out2 = reinterpret_cast<float2*>(out);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + q * FFT_THREADS;
+    const uint64_t row = row0 + e / M;
     if (row < rows)
-      out[row * N + e % N] = r[q];
+      out2[row * M + e % M] = r[q];
   }
 }
 
@@ -404,20 +508,20 @@ int pass_cols(bool inv, const float2* in, float2* out, uint64_t stride_a, uint64
 
 template <int N>
 int pass_r2c_rows(const float* in, float2* spec) {
-  constexpr int TK = FFT_TILE / N;
+  constexpr int TK2 = FFT_TILE / (N / 2);
   const uint64_t rows = (uint64_t)N * N;
-  r2c_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, stream()>>>(in, spec,
-    rows, spectrum_pitch(N), g_tw.fwd.as<float2>());
+  r2c_rows_kernel<N><<<(unsigned)((rows + TK2 - 1) / TK2), FFT_THREADS, 0, stream()>>>(in,
+    spec, rows, spectrum_pitch(N), spectrum_plane(N), g_tw.fwd.as<float2>());
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
 
 template <int N>
 int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s) {
-  constexpr int TK = FFT_TILE / N;
+  constexpr int TK2 = FFT_TILE / (N / 2);
   const uint64_t rows = (uint64_t)N * N;
-  c2r_rows_kernel<N><<<(unsigned)((rows + TK - 1) / TK), FFT_THREADS, 0, s>>>(spec, out,
-    rows, spectrum_pitch(N), norm, g_tw.fwd.as<float2>());
+  c2r_rows_kernel<N><<<(unsigned)((rows + TK2 - 1) / TK2), FFT_THREADS, 0, s>>>(spec, out,
+    rows, spectrum_pitch(N), spectrum_plane(N), norm, g_tw.fwd.as<float2>());
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
@@ -525,16 +629,16 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
     st = twiddles(n);
   if (st != SIF_OK)
     return st;
-  const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n);
+  const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n), plane = spectrum_plane(n);
   const uint64_t rows = (uint64_t)n * n;
   timing_begin(SIF_AMD_STAGE_FFT_FORWARD);
   st = run_r2c_rows(n, values, work);
   // y pass: pencil a = x, i = y
   if (st == SIF_OK)
-    st = run_cols(n, false, work, work, (uint64_t)n * pitch, pitch, nullptr, nullptr, stream());
+    st = run_cols(n, false, work, work, plane, pitch, nullptr, nullptr, stream());
   // x pass: pencil a = y, i = x
   if (st == SIF_OK)
-    st = run_cols(n, false, work, spec, pitch, (uint64_t)n * pitch, nullptr, nullptr, stream());
+    st = run_cols(n, false, work, spec, pitch, plane, nullptr, nullptr, stream());
   if (st != SIF_OK)
     return st;
   timing_end(SIF_AMD_STAGE_FFT_FORWARD);
@@ -553,15 +657,15 @@ int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double
     st = twiddles(n);
   if (st != SIF_OK)
     return st;
-  const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n);
+  const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n), plane = spectrum_plane(n);
   const uint64_t rows = (uint64_t)n * n;
   const float norm = 1.0f / (float)((uint64_t)n * n * n);
   timing_begin(SIF_AMD_STAGE_SMOOTH, s);
   // x pass with the filter fused: pencil a = y, i = x
-  st = run_cols(n, true, spec, work, pitch, (uint64_t)n * pitch, d_lut, d_axis, s);
+  st = run_cols(n, true, spec, work, pitch, plane, d_lut, d_axis, s);
   // y pass: pencil a = x, i = y
   if (st == SIF_OK)
-    st = run_cols(n, true, work, work, (uint64_t)n * pitch, pitch, nullptr, nullptr, s);
+    st = run_cols(n, true, work, work, plane, pitch, nullptr, nullptr, s);
   if (st == SIF_OK)
     st = run_c2r_rows(n, work, out, norm, s);
   if (st != SIF_OK)
@@ -570,6 +674,10 @@ int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double
   if (timing_active())  // the spectrum once, the field once
     timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + 4.0 * (double)rows * n);
   return SIF_OK;
+}
+
+uint64_t spectrum_plane(uint32_t n) {
+  return (uint64_t)n * spectrum_pitch(n);
 }
 
 uint64_t filter_lut_size(uint32_t n) {
@@ -615,7 +723,7 @@ using namespace sif_amd;
 extern "C" {
 
 uint64_t sif_amd_spectrum_bytes(uint32_t n) {
-  return (uint64_t)n * n * spectrum_pitch(n) * 2 * sizeof(float);
+  return (uint64_t)n * spectrum_plane(n) * 2 * sizeof(float);
 }
 
 uint32_t sif_amd_spectrum_pitch(uint32_t n) {

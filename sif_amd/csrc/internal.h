@@ -30,6 +30,9 @@ int mesh_build(const float* x, const float* y, const float* z, const float* w,
 __host__ __device__ inline uint32_t spectrum_pitch(uint32_t n) {
   return ((n / 2 + 1) + 15u) & ~15u;
 }
+// Complex values per x-plane of a spectrum (n rows of the pitch, and a pad
+// that keeps successive planes off the same memory channels).
+uint64_t spectrum_plane(uint32_t n);
 int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work);
 int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
   float r, float sigma, int window_order, int window_mode, float* out,

@@ -4,13 +4,22 @@
 // canonical per-cell order (chain_mesh.c:364-542) and the weight sums
 // (chain_mesh.c:624-672) of sif v0.5.0.
 //
-// The reference claims slots with atomics and then sorts each cell by source
-// index. Here the ordering is made deterministic from the start: a particle's
-// slot is offsets[cell] + (the number of particles with a smaller index in the
-// same cell), obtained with a stable LSD radix sort on the cell key, whose
-// input is in source order. That is the reference's canonical order exactly.
-// Cells per side are at most 1024 (SIF_FINDER_MESH_MAX_CELLS), so keys fit in
-// 30 bits; the sort runs only over the bits the mesh needs.
+// The order is the reference's canonical one: a particle's slot is
+// offsets[cell] + (the number of particles with a smaller source index in the
+// same cell). It is reached in three passes over the particles:
+//
+//   count    each particle's cell, the counts per cell (a wavefront's run of
+//            equal cells adds its length once);
+//   scatter  the folded positions (and weights, and the source index) to a
+//            slot of their cell claimed with an atomic cursor, in whatever
+//            order the wavefronts arrive;
+//   order    one wavefront per cell puts its slots in source order: each
+//            element's rank among the cell's source indices (compared in LDS)
+//            is its place, so the result does not depend on the arrival order.
+//
+// Cells of more than MESH_ORDER_CAP particles are ordered by the stable LSD
+// radix sort of (cell, source index) instead (the whole mesh takes that path
+// then). Source indices are 32-bit below 2^32 particles and 64-bit above.
 #include "common.h"
 #include "internal.h"
 #include "primitives.h"
@@ -103,50 +112,262 @@ __global__ void weight_blocks_kernel(const double* __restrict__ cd, uint64_t cel
   out[b] = sum;
 }
 
-namespace {
-DeviceBuffer g_keys, g_vals, g_counts, g_cd, g_blocks;
+// --- count, scatter, order ---
+
+constexpr uint32_t MESH_ORDER_CAP = 1024;  // particles a cell, for the ordering pass
+constexpr int MESH_ORDER_WAVES = 4;        // cells a workgroup (one wavefront each)
+
+__device__ __forceinline__ uint32_t mesh_cell_of(float x, float y, float z, uint32_t nc,
+  float inv_l, float box) {
+  int32_t ix = (int32_t)(box_fold_m(x, box) * inv_l);
+  int32_t iy = (int32_t)(box_fold_m(y, box) * inv_l);
+  int32_t iz = (int32_t)(box_fold_m(z, box) * inv_l);
+  const int32_t m = (int32_t)nc;
+  ix = ix < 0 ? 0 : (ix >= m ? m - 1 : ix);
+  iy = iy < 0 ? 0 : (iy >= m ? m - 1 : iy);
+  iz = iz < 0 ? 0 : (iz >= m ? m - 1 : iz);
+  return ((uint32_t)ix * nc + (uint32_t)iy) * nc + (uint32_t)iz;
 }
 
-int mesh_build(const float* x, const float* y, const float* z, const float* w,
-  uint64_t n64, uint32_t nc, float box, uint64_t* offsets, float* mx, float* my,
-  float* mz, float* mw, uint64_t* orig, float* cw, double* total_weight) {
-  if (n64 >= 0xffffffffull) {
-    error_set(SIF_ERR_RANGE, "chain mesh: %llu particles (at most 2^32 - 2)",
-      (unsigned long long)n64);
-    return SIF_ERR_RANGE;
+// A wavefront's lanes holding the same key in a row form a run: its first
+// lane and (on that lane) its length.
+__device__ __forceinline__ int mesh_run(uint32_t key, bool valid, int lane, uint32_t* len) {
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const uint64_t heads = __ballot(head);
+  const uint64_t live = __ballot(valid);
+  const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int first = 63 - __clzll((long long)(heads & upto));
+  if (head) {
+    const uint64_t later = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    const int end = later ? __ffsll((long long)later) - 1 : 64 - __clzll((long long)live);
+    *len = (uint32_t)(end - lane);
   }
-  const uint32_t n = (uint32_t)n64;
-  const uint64_t cells = (uint64_t)nc * nc * nc;
-  const float cell = box / (float)nc;
-  const float inv_l = 1.0f / cell;
+  return first;
+}
+
+__global__ void __launch_bounds__(256) mesh_count_kernel(const float* __restrict__ xs,
+  const float* __restrict__ ys, const float* __restrict__ zs, uint64_t n, uint32_t nc,
+  float inv_l, float box, uint32_t* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < n;
+  const uint32_t key = valid ? mesh_cell_of(xs[i], ys[i], zs[i], nc, inv_l, box) : 0xffffffffu;
+  uint32_t len = 0;
+  const int first = mesh_run(key, valid, lane, &len);
+  if (valid && first == lane)
+    atomicAdd(&counts[key], len);
+}
+
+template <class IdxT>
+__global__ void __launch_bounds__(256) mesh_scatter_kernel(const float* __restrict__ xs,
+  const float* __restrict__ ys, const float* __restrict__ zs, const float* __restrict__ ws,
+  uint64_t n, uint32_t nc, float inv_l, float box, const uint64_t* __restrict__ offsets,
+  uint32_t* __restrict__ cursor, float* __restrict__ mx, float* __restrict__ my,
+  float* __restrict__ mz, float* __restrict__ mw, IdxT* __restrict__ src) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < n;
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  uint32_t key = 0xffffffffu;
+  if (valid) {
+    x = xs[i];
+    y = ys[i];
+    z = zs[i];
+    key = mesh_cell_of(x, y, z, nc, inv_l, box);
+  }
+  uint32_t len = 0;
+  const int first = mesh_run(key, valid, lane, &len);
+  uint32_t base = 0;
+  if (valid && first == lane)
+    base = atomicAdd(&cursor[key], len);
+  base = __shfl(base, first, 64);
+  if (valid) {
+    const uint64_t slot = offsets[key] + base + (uint32_t)(lane - first);
+    mx[slot] = box_fold_m(x, box);
+    my[slot] = box_fold_m(y, box);
+    mz[slot] = box_fold_m(z, box);
+    if (mw)
+      mw[slot] = ws[i];
+    src[slot] = (IdxT)i;
+  }
+}
+
+// One wavefront per cell (of at most MESH_ORDER_CAP particles): every element
+// goes to its rank among the cell's source indices, which are distinct.
+// Positions are read into registers before any is written (the cell's slots
+// are rewritten in place).
+template <class IdxT>
+__global__ void __launch_bounds__(64 * MESH_ORDER_WAVES) mesh_order_kernel(
+  const uint64_t* __restrict__ offsets, uint64_t cells, const IdxT* __restrict__ src,
+  float* mx, float* my, float* mz, float* mw, uint64_t* __restrict__ orig) {
+  constexpr int PER = MESH_ORDER_CAP / 64;
+  __shared__ IdxT ids[MESH_ORDER_WAVES][MESH_ORDER_CAP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t c = (uint64_t)blockIdx.x * MESH_ORDER_WAVES + wave;
+  uint64_t b = 0;
+  uint32_t cnt = 0;
+  if (c < cells) {
+    b = offsets[c];
+    cnt = (uint32_t)(offsets[c + 1] - b);
+  }
+  const uint32_t nt = (cnt + 63) / 64;  // element slots in use, uniform over the wave
+  IdxT own[PER];
+  float px[PER], py[PER], pz[PER], pw[PER];
+#pragma unroll
+  for (int t = 0; t < PER; t++) {
+    const uint32_t j = lane + 64u * t;
+    own[t] = 0;
+    px[t] = py[t] = pz[t] = pw[t] = 0.0f;
+    if (j < cnt) {
+      own[t] = src[b + j];
+      ids[wave][j] = own[t];
+      px[t] = mx[b + j];
+      py[t] = my[b + j];
+      pz[t] = mz[b + j];
+      pw[t] = mw ? mw[b + j] : 0.0f;
+    }
+  }
+  __syncthreads();
+  uint32_t rank[PER];
+#pragma unroll
+  for (int t = 0; t < PER; t++)
+    rank[t] = 0;
+  for (uint32_t k = 0; k < cnt; k++) {
+    const IdxT v = ids[wave][k];
+#pragma unroll
+    for (int t = 0; t < PER; t++)
+      if ((uint32_t)t < nt)
+        rank[t] += (v < own[t]) ? 1u : 0u;
+  }
+#pragma unroll
+  for (int t = 0; t < PER; t++) {
+    const uint32_t j = lane + 64u * t;
+    if (j < cnt) {
+      const uint64_t at = b + rank[t];
+      mx[at] = px[t];
+      my[at] = py[t];
+      mz[at] = pz[t];
+      if (mw)
+        mw[at] = pw[t];
+      if (orig)
+        orig[at] = (uint64_t)own[t];
+    }
+  }
+}
+
+// The largest cell count.
+__global__ void count_max_kernel(const uint32_t* __restrict__ counts, uint64_t cells,
+  uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < cells;
+       c += (uint64_t)gridDim.x * blockDim.x)
+    m = counts[c] > m ? counts[c] : m;
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m)
+    atomicMax(out, m);
+}
+
+namespace {
+DeviceBuffer g_keys, g_vals, g_counts, g_cd, g_blocks, g_cursor, g_src, g_max;
+}
+
+// The stable radix path (cells of more than MESH_ORDER_CAP particles):
+// (cell, source index) pairs sorted by cell, then the positions gathered.
+static int mesh_by_radix(const float* x, const float* y, const float* z, const float* w,
+  uint32_t n, uint32_t nc, float inv_l, float box, uint64_t cells, float* mx, float* my,
+  float* mz, float* mw, uint64_t* orig) {
   int st = g_keys.reserve((uint64_t)(n ? n : 1) * 4);
   if (st == SIF_OK)
     st = g_vals.reserve((uint64_t)(n ? n : 1) * 4);
-  if (st == SIF_OK)
-    st = g_counts.reserve(cells * 4);
   if (st != SIF_OK)
     return st;
-  timing_begin(SIF_AMD_STAGE_MESH);
-  SIF_HIP(hipMemsetAsync(g_counts.ptr, 0, cells * 4, stream()));
-  if (n) {
-    mesh_key_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(x, y, z, n, nc,
-      inv_l, box, g_keys.as<uint32_t>(), g_vals.as<uint32_t>(),
-      g_counts.as<uint32_t>());
-    SIF_HIP(hipGetLastError());
-  }
-  st = exclusive_scan_u32(g_counts.as<uint32_t>(), cells, offsets, offsets + cells);
-  if (st != SIF_OK)
-    return st;
+  // the counts are known already: this pass only writes the keys
+  SIF_HIP(hipMemsetAsync(g_cursor.ptr, 0, cells * 4, stream()));
+  mesh_key_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(x, y, z, n, nc, inv_l, box,
+    g_keys.as<uint32_t>(), g_vals.as<uint32_t>(), g_cursor.as<uint32_t>());
+  SIF_HIP(hipGetLastError());
   int bits = 1;
   while (bits < 32 && (1ull << bits) < cells)
     bits++;
   st = radix_sort_u32(g_keys.as<uint32_t>(), g_vals.as<uint32_t>(), n, bits);
   if (st != SIF_OK)
     return st;
+  mesh_gather_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(
+    g_vals.as<uint32_t>(), n, x, y, z, w, box, mx, my, mz, w ? mw : nullptr, orig);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+template <class IdxT>
+static int mesh_scatter_order(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t nc, float inv_l, float box, uint64_t cells, const uint64_t* offsets,
+  float* mx, float* my, float* mz, float* mw, uint64_t* orig) {
+  int st = g_src.reserve((n ? n : 1) * sizeof(IdxT));
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipMemsetAsync(g_cursor.ptr, 0, cells * 4, stream()));
+  mesh_scatter_kernel<IdxT><<<grid_for(n, 256), 256, 0, stream()>>>(x, y, z, w, n, nc, inv_l,
+    box, offsets, g_cursor.as<uint32_t>(), mx, my, mz, w ? mw : nullptr, g_src.as<IdxT>());
+  SIF_HIP(hipGetLastError());
+  const uint64_t groups = (cells + MESH_ORDER_WAVES - 1) / MESH_ORDER_WAVES;
+  mesh_order_kernel<IdxT><<<(unsigned)groups, 64 * MESH_ORDER_WAVES, 0, stream()>>>(offsets,
+    cells, g_src.as<IdxT>(), mx, my, mz, w ? mw : nullptr, orig);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+int mesh_build(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t nc, float box, uint64_t* offsets, float* mx, float* my,
+  float* mz, float* mw, uint64_t* orig, float* cw, double* total_weight) {
+  const uint64_t cells = (uint64_t)nc * nc * nc;
+  const float cell = box / (float)nc;
+  const float inv_l = 1.0f / cell;
+  int st = g_counts.reserve(cells * 4);
+  if (st == SIF_OK)
+    st = g_cursor.reserve(cells * 4);
+  if (st == SIF_OK)
+    st = g_max.reserve(4);
+  if (st != SIF_OK)
+    return st;
+  timing_begin(SIF_AMD_STAGE_MESH);
+  SIF_HIP(hipMemsetAsync(g_counts.ptr, 0, cells * 4, stream()));
+  SIF_HIP(hipMemsetAsync(g_max.ptr, 0, 4, stream()));
   if (n) {
-    mesh_gather_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(
-      g_vals.as<uint32_t>(), n, x, y, z, w, box, mx, my, mz, w ? mw : nullptr, orig);
+    mesh_count_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(x, y, z, n, nc, inv_l, box,
+      g_counts.as<uint32_t>());
     SIF_HIP(hipGetLastError());
+  }
+  st = exclusive_scan_u32(g_counts.as<uint32_t>(), cells, offsets, offsets + cells);
+  if (st != SIF_OK)
+    return st;
+  if (n) {
+    const unsigned mb = grid_for(cells, 256) > 1024 ? 1024 : grid_for(cells, 256);
+    count_max_kernel<<<mb, 256, 0, stream()>>>(g_counts.as<uint32_t>(), cells,
+      g_max.as<uint32_t>());
+    SIF_HIP(hipGetLastError());
+    uint32_t largest = 0;
+    SIF_HIP(hipMemcpyAsync(&largest, g_max.ptr, 4, hipMemcpyDeviceToHost, stream()));
+    SIF_HIP(hipStreamSynchronize(stream()));
+    if (largest <= MESH_ORDER_CAP) {
+      st = n < 0xffffffffull
+             ? mesh_scatter_order<uint32_t>(x, y, z, w, n, nc, inv_l, box, cells, offsets, mx,
+                 my, mz, mw, orig)
+             : mesh_scatter_order<uint64_t>(x, y, z, w, n, nc, inv_l, box, cells, offsets, mx,
+                 my, mz, mw, orig);
+    } else if (n < 0xffffffffull) {
+      st = mesh_by_radix(x, y, z, w, (uint32_t)n, nc, inv_l, box, cells, mx, my, mz, mw, orig);
+    } else {
+      error_set(SIF_ERR_RANGE, "chain mesh: a cell of %u particles among %llu (cells of more "
+        "than %u need fewer than 2^32 particles)", largest, (unsigned long long)n,
+        MESH_ORDER_CAP);
+      st = SIF_ERR_RANGE;
+    }
+    if (st != SIF_OK)
+      return st;
   }
   if (w && cw) {
     st = g_cd.reserve(cells * sizeof(double));

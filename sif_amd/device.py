@@ -73,29 +73,40 @@ def spectrum_pitch(n):
     return int(lib.sif_amd_spectrum_pitch(n))
 
 
+def spectrum_plane(n):
+    """Complex values per x-plane of a device spectrum (n padded rows and a
+    plane pad)."""
+    return int(lib.sif_amd_spectrum_bytes(n)) // (8 * n)
+
+
+def spectrum_buffer(n):
+    return DeviceArray((n, spectrum_plane(n), 2), np.float32)
+
+
 def spectrum(dvalues, n):
-    """The r2c transform of an n^3 device field: a device array
-    (n, n, pitch, 2) whose first n/2 + 1 entries of each row are the modes."""
-    p = spectrum_pitch(n)
-    spec = DeviceArray((n, n, p, 2), np.float32)
-    work = DeviceArray((n, n, p, 2), np.float32)
+    """The r2c transform of an n^3 device field as a device array
+    (n, plane, 2): x-plane i holds rows of spectrum_pitch(n) complex values,
+    the first n/2 + 1 of each row the modes (spectrum_modes reads them)."""
+    spec = spectrum_buffer(n)
+    work = spectrum_buffer(n)
     check(lib.sif_amd_fft_r2c(dvalues.ptr, n, spec.ptr, work.ptr))
     work.free()
     return spec
 
 
 def spectrum_modes(dspec, n):
-    """A device spectrum on the host as complex128 (n, n, n/2 + 1), FFTW's
-    r2c layout."""
-    s = dspec.get()[:, :, :n // 2 + 1]
-    return s[..., 0] + 1j * s[..., 1].astype(np.float64)
+    """A device spectrum on the host as float32 (n, n, n/2 + 1, 2), FFTW's
+    r2c layout (real, imaginary)."""
+    p = spectrum_pitch(n)
+    s = dspec.get()[:, :n * p].reshape(n, n, p, 2)
+    return s[:, :, :n // 2 + 1]
 
 
 def smooth(dspec, n, box, filt, r, sigma=0.0, window_order=2, window_mode=0, out=None,
            work=None):
     out = out or DeviceArray((n ** 3,), np.float32)
     own = work is None
-    work = work or DeviceArray((n, n, spectrum_pitch(n), 2), np.float32)
+    work = work or spectrum_buffer(n)
     check(lib.sif_amd_fft_filter_c2r(dspec.ptr, n, box, filt, r, sigma, window_order,
                                      window_mode, out.ptr, work.ptr))
     if own:
