@@ -8,6 +8,7 @@ include/sif/ and include/sif_amd.h.
 """
 import concurrent.futures
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -43,12 +44,55 @@ def hipcc():
     return path
 
 
+LABEL = re.compile(r"^([A-Za-z_.$][\w.$]*):")
+
+
+def return_address_hazards(asm_text):
+    """Device functions (not kernels) that jump through s[30:31], the
+    register pair holding their return address: the long-branch sequence
+    `s_getpc_b64 s[30:31]; ...; s_setpc_b64 s[30:31]` that the compiler emits
+    for a branch past the short branch's reach overwrites it, and the
+    function's own return then jumps back into itself, forever. Returns the
+    names of such functions in an AMDGPU assembly listing."""
+    bad, name, is_func, body = [], None, False, []
+    functions = set()
+    for line in asm_text.splitlines():
+        t = line.strip()
+        if t.startswith(".type") and t.endswith("@function"):
+            functions.add(t.split()[1].split(",")[0])
+            continue
+        m = LABEL.match(t)
+        head = m.group(1) if m else None
+        if head and head in functions:
+            if is_func and "s_getpc_b64 s[30:31]" in body and "s_setpc_b64 s[30:31]" in body:
+                bad.append(name)
+            name, body = head, []
+            is_func = True
+            continue
+        if t.startswith(".amdhsa_kernel"):
+            is_func = False  # a kernel's descriptor: the function before it was the kernel
+        if is_func:
+            body.append(" ".join(t.split()))
+    if is_func and "s_getpc_b64 s[30:31]" in body and "s_setpc_b64 s[30:31]" in body:
+        bad.append(name)
+    return bad
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [hipcc(), *FLAGS, "-c", src, "-o", obj]
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    # -save-temps=obj leaves the device assembly beside the object (under
+    # lib/obj, not shipped): it is checked for the hazard above
+    cmd = [hipcc(), *FLAGS, "-save-temps=obj", "-c", src, "-o", obj]
+    res = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{res.stderr}")
+    base = os.path.splitext(os.path.basename(src))[0]
+    asm = os.path.join(OBJ, f"{base}-hip-amdgcn-amd-amdhsa-{ARCH}.s")
+    if os.path.exists(asm):
+        bad = return_address_hazards(open(asm).read())
+        if bad:
+            raise RuntimeError(f"{src}: device functions that branch through their return "
+                               f"address (hang on return): {', '.join(bad)}")
     return obj
 
 

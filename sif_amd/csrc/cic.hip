@@ -20,6 +20,7 @@
 // both order-free, so the grid stays bit-identical; small weights get finer
 // units rather than losing their digits. A weight that is not finite is
 // refused (SIF_ERR_RANGE).
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -82,14 +83,14 @@ __device__ __forceinline__ void cic_corners(float cx, float cy, float cz, float 
 // (CIC_T + 1)^3 nodes, and the tile is flushed with one 64-bit integer atomic
 // per node it touched. Integer sums do not depend on order: the grid is the
 // same bits for any launch shape or particle order.
-constexpr uint32_t CIC_T = 8;
-constexpr uint32_t CIC_L = CIC_T + 1;
-
+// The tile edge T is 8 or 16 nodes (cic_tile_edge(), a tuning: any edge
+// gives the same bits); the LDS accumulators hold (T + 1)^3 int64.
+template <uint32_t T>
 __device__ __forceinline__ uint32_t cic_tile(float cx, float cy, float cz, uint32_t N,
   uint32_t nt) {
   const uint32_t rx = (uint32_t)cx, ry = (uint32_t)cy, rz = (uint32_t)cz;
   const uint32_t ix = rx >= N ? 0u : rx, iy = ry >= N ? 0u : ry, iz = rz >= N ? 0u : rz;
-  return ((ix / CIC_T) * nt + iy / CIC_T) * nt + iz / CIC_T;
+  return ((ix / T) * nt + iy / T) * nt + iz / T;
 }
 
 // Lanes holding the same key in a row (common: neighbouring particles share
@@ -109,6 +110,7 @@ __device__ __forceinline__ int key_run(uint32_t key, bool valid, int lane, uint3
   return first;
 }
 
+template <uint32_t T>
 __global__ void __launch_bounds__(256) cic_count_kernel(const float* __restrict__ xs,
   const float* __restrict__ ys, const float* __restrict__ zs, uint64_t n, uint32_t N,
   float idx, float box, uint32_t nt, uint32_t* __restrict__ counts) {
@@ -117,7 +119,7 @@ __global__ void __launch_bounds__(256) cic_count_kernel(const float* __restrict_
   const bool valid = p < n;
   uint32_t key = 0xffffffffu;
   if (valid)
-    key = cic_tile(box_fold(xs[p], box) * idx, box_fold(ys[p], box) * idx,
+    key = cic_tile<T>(box_fold(xs[p], box) * idx, box_fold(ys[p], box) * idx,
       box_fold(zs[p], box) * idx, N, nt);
   uint32_t len = 0;
   const int first = key_run(key, valid, lane, &len);
@@ -126,6 +128,7 @@ __global__ void __launch_bounds__(256) cic_count_kernel(const float* __restrict_
 }
 
 // Particles into tile order: the scaled, folded coordinates (and weights).
+template <uint32_t T>
 __global__ void __launch_bounds__(256) cic_bin_kernel(const float* __restrict__ xs,
   const float* __restrict__ ys, const float* __restrict__ zs, const float* __restrict__ ws,
   uint64_t n, uint32_t N, float idx, float box, uint32_t nt,
@@ -141,7 +144,7 @@ __global__ void __launch_bounds__(256) cic_bin_kernel(const float* __restrict__ 
     cx = box_fold(xs[p], box) * idx;
     cy = box_fold(ys[p], box) * idx;
     cz = box_fold(zs[p], box) * idx;
-    key = cic_tile(cx, cy, cz, N, nt);
+    key = cic_tile<T>(cx, cy, cz, N, nt);
   }
   uint32_t len = 0;
   const int first = key_run(key, valid, lane, &len);
@@ -159,12 +162,14 @@ __global__ void __launch_bounds__(256) cic_bin_kernel(const float* __restrict__ 
   }
 }
 
-// One workgroup per tile: LDS accumulation over (CIC_T + 1)^3 nodes, then the
+// One workgroup per tile: LDS accumulation over (T + 1)^3 nodes, then the
 // flush.
+template <uint32_t T>
 __global__ void __launch_bounds__(256) cic_tile_kernel(const float* __restrict__ bx,
   const float* __restrict__ by, const float* __restrict__ bz, const float* __restrict__ bw,
   const uint64_t* __restrict__ offsets, uint32_t N, uint32_t nt, double unit_inv,
   int64_t* __restrict__ acc) {
+  constexpr uint32_t CIC_T = T, CIC_L = T + 1;
   __shared__ unsigned long long L[CIC_L * CIC_L * CIC_L];
   const uint32_t t = blockIdx.x;
   const uint64_t p0 = offsets[t], p1 = offsets[t + 1];
@@ -299,8 +304,8 @@ __global__ void check_box_kernel(const float* __restrict__ xs,
 }
 
 namespace {
-DeviceBuffer g_mean_buf;
-DeviceBuffer g_box_buf;
+thread_local DeviceBuffer g_mean_buf;
+thread_local DeviceBuffer g_box_buf;
 }  // namespace
 
 // --- the density contrast with no allocation ---
@@ -407,7 +412,7 @@ int check_in_box(const float* x, const float* y, const float* z, uint64_t n,
 }
 
 namespace {
-DeviceBuffer g_tile_counts, g_tile_offsets, g_tile_cursor, g_bx, g_by, g_bz, g_bw, g_wmax;
+thread_local DeviceBuffer g_tile_counts, g_tile_offsets, g_tile_cursor, g_bx, g_by, g_bz, g_bw, g_wmax;
 }
 
 // The fixed-point exponent of a deposit of n particles whose largest |weight|
@@ -445,7 +450,32 @@ int cic_weight_max(const float* w, uint64_t n, float* wmax) {
   return SIF_OK;
 }
 
+namespace {
+std::atomic<uint32_t> g_cic_tile{8};
+}
+
+uint32_t cic_tile_edge() { return g_cic_tile.load(std::memory_order_relaxed); }
+int set_cic_tile_edge(uint32_t t) {
+  if (t != 8 && t != 16) {
+    error_set(SIF_ERR_INVALID, "cic tile edge %u (8 or 16)", t);
+    return SIF_ERR_INVALID;
+  }
+  g_cic_tile.store(t, std::memory_order_relaxed);
+  return SIF_OK;
+}
+
+template <uint32_t CIC_T>
+static int cic_deposit_t(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t N, float box, float* values, int64_t* acc);
+
 int cic_deposit(const float* x, const float* y, const float* z, const float* w,
+  uint64_t n, uint32_t N, float box, float* values, int64_t* acc) {
+  return cic_tile_edge() == 16 ? cic_deposit_t<16>(x, y, z, w, n, N, box, values, acc)
+                               : cic_deposit_t<8>(x, y, z, w, n, N, box, values, acc);
+}
+
+template <uint32_t CIC_T>
+static int cic_deposit_t(const float* x, const float* y, const float* z, const float* w,
   uint64_t n, uint32_t N, float box, float* values, int64_t* acc) {
   const uint64_t total = (uint64_t)N * N * N;
   const float idx = (float)N / box;
@@ -453,8 +483,10 @@ int cic_deposit(const float* x, const float* y, const float* z, const float* w,
   const float inv_vol = 1.0f / (float)(cell * cell * cell);
   const uint32_t nt = (N + CIC_T - 1) / CIC_T;
   const uint64_t tiles = (uint64_t)nt * nt * nt;
-  if (n >= 0xffffffffull || tiles >= 0xffffffffull) {
-    error_set(SIF_ERR_RANGE, "cic: %llu particles (at most 2^32 - 2)", (unsigned long long)n);
+  // slots and offsets are 64-bit; a tile's count is 32-bit
+  if (tiles >= 0xffffffffull) {
+    error_set(SIF_ERR_RANGE, "cic: %llu tiles of %u^3 nodes (at most 2^32 - 2)",
+      (unsigned long long)tiles, CIC_T);
     return SIF_ERR_RANGE;
   }
   int st = g_tile_counts.reserve((tiles + 1) * 4);
@@ -477,18 +509,18 @@ int cic_deposit(const float* x, const float* y, const float* z, const float* w,
     SIF_HIP(hipMemsetAsync(g_tile_counts.ptr, 0, (tiles + 1) * 4, stream()));
     SIF_HIP(hipMemsetAsync(g_tile_cursor.ptr, 0, tiles * 4, stream()));
     const unsigned g = grid_for(n, 256);
-    cic_count_kernel<<<g, 256, 0, stream()>>>(x, y, z, n, N, idx, box, nt,
+    cic_count_kernel<CIC_T><<<g, 256, 0, stream()>>>(x, y, z, n, N, idx, box, nt,
       g_tile_counts.as<uint32_t>());
     SIF_HIP(hipGetLastError());
     st = exclusive_scan_u32(g_tile_counts.as<uint32_t>(), tiles + 1,
       g_tile_offsets.as<uint64_t>(), nullptr);
     if (st != SIF_OK)
       return st;
-    cic_bin_kernel<<<g, 256, 0, stream()>>>(x, y, z, w, n, N, idx, box, nt,
+    cic_bin_kernel<CIC_T><<<g, 256, 0, stream()>>>(x, y, z, w, n, N, idx, box, nt,
       g_tile_offsets.as<uint64_t>(), g_tile_cursor.as<uint32_t>(), g_bx.as<float>(),
       g_by.as<float>(), g_bz.as<float>(), w ? g_bw.as<float>() : nullptr);
     SIF_HIP(hipGetLastError());
-    cic_tile_kernel<<<(unsigned)tiles, 256, 0, stream()>>>(g_bx.as<float>(), g_by.as<float>(),
+    cic_tile_kernel<CIC_T><<<(unsigned)tiles, 256, 0, stream()>>>(g_bx.as<float>(), g_by.as<float>(),
       g_bz.as<float>(), w ? g_bw.as<float>() : nullptr, g_tile_offsets.as<uint64_t>(), N, nt,
       unit_inv, acc);
     SIF_HIP(hipGetLastError());

@@ -23,6 +23,10 @@ hipStream_t stream();
 hipStream_t aux_stream();
 bool initialised();
 int ensure_init();  // lazy device init for the sif_amd_* device entry points
+// The calling thread's own stream pair (null, null: the library's), and
+// whether it has one.
+void set_thread_streams(hipStream_t s, hipStream_t aux);
+bool on_own_streams();
 
 // Per-stage device timing (HIP events on the library stream).
 void timing_begin(int stage);

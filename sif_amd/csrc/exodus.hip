@@ -72,7 +72,7 @@ struct DeviceState {
   DeviceBuffer pf_counts;  // per rung: masked, proxy, overlap
   DeviceBuffer field2;     // the second smoothed field (rungs alternate)
 };
-DeviceState g_dev;
+thread_local DeviceState g_dev;
 
 // A displacement rung's growth window on the device.
 struct DisplaceBuffers {
@@ -88,12 +88,18 @@ struct DisplaceBuffers {
     return st;
   }
 };
-DisplaceBuffers disp_bufs;
+thread_local DisplaceBuffers disp_bufs;
 
 // The pool's workers beside the calling thread: SIF_AMD_HOST_THREADS - 1 when
 // set (1 to 64 threads in all), else one fewer than the hardware threads, at
 // most 15.
 WorkPool& host_pool() {
+  // a thread on streams of its own is one of several runs side by side:
+  // its per-candidate tests run inline
+  if (on_own_streams()) {
+    static thread_local WorkPool inline_pool(0);
+    return inline_pool;
+  }
   static WorkPool pool([]() {
     const char* env = getenv("SIF_AMD_HOST_THREADS");
     const long want = env ? strtol(env, nullptr, 10) : 0;
@@ -119,7 +125,7 @@ struct BatchSlot {
   uint32_t count = 0;
   bool algo = false;
 };
-BatchSlot g_slots[2];
+thread_local BatchSlot g_slots[2];
 
 int host_alloc(void** p, size_t bytes) {
   SIF_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
@@ -174,7 +180,7 @@ struct DeviceVoids {
   uint32_t hcap = 0;
   float class_max[VOID_CLASSES] = {-1.0f, -1.0f, -1.0f, -1.0f};
 };
-DeviceVoids g_voids;
+thread_local DeviceVoids g_voids;
 
 int grow_keep(DeviceBuffer& b, size_t used, size_t want) {
   if (want <= b.bytes && b.ptr)
@@ -479,7 +485,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   if (st != SIF_OK)
     return st;
   float* const fields[2] = {R.d_delta, g_dev.field2.as<float>()};
-  static hipEvent_t ev_smoothed[2] = {nullptr, nullptr}, ev_scanned[2] = {nullptr, nullptr};
+  static thread_local hipEvent_t ev_smoothed[2] = {nullptr, nullptr},
+    ev_scanned[2] = {nullptr, nullptr};
   for (int b = 0; b < 2; b++) {
     if (!ev_smoothed[b])
       SIF_HIP(hipEventCreateWithFlags(&ev_smoothed[b], hipEventDisableTiming));
@@ -1318,7 +1325,7 @@ int weights_verdict(uint64_t negative, uint64_t nonfinite, uint64_t n, double to
   return SIF_OK;
 }
 
-DeviceBuffer g_weight_counts;
+thread_local DeviceBuffer g_weight_counts;
 
 }  // namespace
 

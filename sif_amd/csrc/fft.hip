@@ -23,7 +23,7 @@
 // The per-mode smoothing factor (float)(LUT[k^2] * w(kx) w(ky) w(kz)) of
 // fft.c:753-780 is applied as the spectrum is first read by the inverse x
 // pass, so filtering costs no extra pass over memory. Power-of-two n from 16
-// to 1024.
+// to 1024; other lengths through the mixed-radix kernels below.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -454,6 +454,244 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
   }
 }
 
+// --- any length: mixed-radix Stockham with a plan chosen at run time ---
+//
+// Lengths n = 2^a 3^b 5^c ... whose prime factors are at most 31 (every
+// multiple of 8 from 64 to 256, 100, 384, 1536 ...). The radices are 8, 4
+// and 2 for the power of two, then each odd prime; the stage formulation is
+// the one above (any order of radices gives the natural order). A tile of
+// TK pencils (pitch n + 1) is transformed in two LDS buffers in turn, one
+// butterfly per thread and step; radix 2, 4 and 8 use the butterflies above,
+// the odd primes a direct DFT whose roots come from the length's twiddle
+// table. Twiddles are read from global memory: tw[k * tws] is
+// exp(-2 pi i k / L) for a transform of length L over a table of L * tws.
+
+constexpr int GEN_MAX_STAGES = 24;
+constexpr int GEN_MAX_PRIME = 31;
+constexpr int GEN_LDS_BYTES = 64 * 1024;
+
+struct GenPlan {
+  int len;  // transform length
+  int stages;
+  int radix[GEN_MAX_STAGES];
+};
+
+// One stage of radix R over the tile's `pencils` pencils (length L, pitch
+// `pitch`): X -> Y. Radix 2, 4 and 8 transform in registers; an odd prime's
+// outputs out[m] = sum_r v[r] w^(r m) (w = exp(-+2 pi i / R), the table's
+// entry (L / R) * tws) are formed and stored one at a time, in a loop kept
+// rolled so that the kernels stay small.
+template <int R, bool INV>
+__device__ __forceinline__ void gen_stage(const float2* X, float2* Y, int L, int NS, int pitch,
+  int pencils, const float2* __restrict__ tw, int tws) {
+  const int nb = L / R;
+  const int twmul = L / (NS * R);
+  for (int b = threadIdx.x; b < pencils * nb; b += FFT_THREADS) {
+    const int p = b / nb, j = b - p * nb;
+    const float2* x = X + p * pitch;
+    float2* y = Y + p * pitch;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      v[r] = x[j + r * nb];
+    const int jm = j % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        float2 w = tw[(jm * r * twmul) * tws];
+        if (INV)
+          w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    const int d = (j / NS) * NS * R + jm;
+    if constexpr (R == 2 || R == 4 || R == 8) {
+      dft<R, INV>(v);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        y[d + r * NS] = v[r];
+    } else {
+      const int step = (L / R) * tws;
+#pragma unroll 1
+      for (int m = 0; m < R; m++) {
+        float2 acc = v[0];
+        int e = 0;  // (r m) mod R, stepped
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          e += m;
+          e = e >= R ? e - R : e;
+          float2 w = tw[e * step];
+          if (INV)
+            w.y = -w.y;
+          acc = cadd(acc, cmul(v[r], w));
+        }
+        y[d + m * NS] = acc;
+      }
+    }
+  }
+}
+
+// The plan's stages over the tile; returns the buffer holding the result.
+// Inlined into each kernel: no call (and no return address) in the stage
+// loop's barriers' path.
+template <bool INV>
+__device__ __forceinline__ float2* gen_fft(float2* X, float2* Y, const GenPlan& P, int pitch,
+  int pencils, const float2* __restrict__ tw, int tws) {
+  int ns = 1;
+  for (int s = 0; s < P.stages; s++) {
+    const int R = P.radix[s];
+    switch (R) {
+    case 2: gen_stage<2, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 3: gen_stage<3, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 4: gen_stage<4, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 5: gen_stage<5, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 7: gen_stage<7, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 8: gen_stage<8, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 11: gen_stage<11, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 13: gen_stage<13, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 17: gen_stage<17, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 19: gen_stage<19, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 23: gen_stage<23, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    case 29: gen_stage<29, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    default: gen_stage<31, INV>(X, Y, P.len, ns, pitch, pencils, tw, tws); break;
+    }
+    __syncthreads();
+    float2* t = X;
+    X = Y;
+    Y = t;
+    ns *= R;
+  }
+  return X;
+}
+
+// x and y passes for any n: TK columns a tile (as cols_kernel), the filter
+// and window fused into the first inverse pass.
+template <bool INV>
+__global__ void __launch_bounds__(FFT_THREADS) gen_cols_kernel(const float2* __restrict__ in,
+  float2* __restrict__ out, int nz, int TK, uint64_t stride_a, uint64_t stride_i, GenPlan P,
+  const float2* __restrict__ tw, const float* __restrict__ lut,
+  const double* __restrict__ axis) {
+  extern __shared__ float2 sm[];
+  const int N = P.len, pitch = N + 1, half = N / 2;
+  float2* X = sm;
+  float2* Y = sm + TK * pitch;
+  const int tiles = (nz + TK - 1) / TK;
+  const int a = blockIdx.x / tiles;
+  const int kz0 = (blockIdx.x % tiles) * TK;
+  const int cols = nz - kz0 < TK ? nz - kz0 : TK;
+  const uint64_t base = (uint64_t)a * stride_a + kz0;
+  const int ky = a > half ? a - N : a;
+  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+    const int i = e / TK, c = e % TK;
+    float2 v = c < cols ? in[base + (uint64_t)i * stride_i + c] : make_float2(0.0f, 0.0f);
+    if (lut && c < cols) {  // fft.c:753-780
+      const int kx = i > half ? i - N : i;
+      const int kz = kz0 + c;
+      float f = lut[(uint32_t)(kx * kx) + (uint32_t)(ky * ky) + (uint32_t)(kz * kz)];
+      if (axis)
+        f = (float)((double)f * (axis[i] * axis[a]) * axis[kz]);
+      v = make_float2(v.x * f, v.y * f);
+    }
+    X[c * pitch + i] = v;
+  }
+  __syncthreads();
+  const float2* R = gen_fft<INV>(X, Y, P, pitch, TK, tw, 1);
+  for (int e = threadIdx.x; e < N * TK; e += FFT_THREADS) {
+    const int i = e / TK, c = e % TK;
+    if (c < cols)
+      out[base + (uint64_t)i * stride_i + c] = R[c * pitch + i];
+  }
+}
+
+// z pass for any n (n even: the half-length transform of the rows above,
+// over plan P of length n / 2; n odd: the whole row, plan P of length n),
+// TK2 rows a workgroup.
+__global__ void __launch_bounds__(FFT_THREADS) gen_r2c_rows_kernel(const float* __restrict__ in,
+  float2* __restrict__ spec, uint64_t rows, int n, int TK2, uint32_t pitch_s, uint64_t plane,
+  GenPlan P, const float2* __restrict__ tw) {
+  extern __shared__ float2 sm[];
+  const bool even = (n & 1) == 0;
+  const int L = P.len, pitch = L + 1, nz = n / 2 + 1;
+  float2* X = sm;
+  float2* Y = sm + TK2 * pitch;
+  const uint64_t row0 = (uint64_t)blockIdx.x * TK2;
+  for (int e = threadIdx.x; e < L * TK2; e += FFT_THREADS) {
+    const uint64_t row = row0 + e / L;
+    const int m = e % L;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (row < rows)
+      z = even ? reinterpret_cast<const float2*>(in + row * n)[m]
+               : make_float2(in[row * n + m], 0.0f);
+    X[(e / L) * pitch + m] = z;
+  }
+  __syncthreads();
+  const float2* Z = gen_fft<false>(X, Y, P, pitch, TK2, tw, even ? 2 : 1);
+  for (int e = threadIdx.x; e < nz * TK2; e += FFT_THREADS) {
+    const int r = e / nz, k = e % nz;
+    const uint64_t row = row0 + r;
+    if (row >= rows)
+      continue;
+    float2 v;
+    if (even) {  // the split above, W^k = tw[k]
+      const float2 zk = Z[r * pitch + (k == L ? 0 : k)];
+      const float2 zm = Z[r * pitch + (k == 0 ? 0 : L - k)];
+      const float2 ev = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      const float2 od = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y + zm.y));
+      v = cadd(ev, cmul(tw[k], make_float2(od.y, -od.x)));
+    } else {
+      v = Z[r * pitch + k];
+    }
+    spec[(row / n) * plane + (row % n) * pitch_s + k] = v;
+  }
+}
+
+__global__ void __launch_bounds__(FFT_THREADS) gen_c2r_rows_kernel(const float2* __restrict__ spec,
+  float* __restrict__ out, uint64_t rows, int n, int TK2, uint32_t pitch_s, uint64_t plane,
+  float norm, GenPlan P, const float2* __restrict__ tw) {
+  extern __shared__ float2 sm[];
+  const bool even = (n & 1) == 0;
+  const int L = P.len, pitch = L + 1, nz = n / 2 + 1;
+  float2* X = sm;
+  float2* Y = sm + TK2 * pitch;
+  const uint64_t row0 = (uint64_t)blockIdx.x * TK2;
+  // the half row into Y (n even: X[M] at index M, in the pitch's spare slot)
+  for (int e = threadIdx.x; e < nz * TK2; e += FFT_THREADS) {
+    const int r = e / nz, k = e % nz;
+    const uint64_t row = row0 + r;
+    Y[r * pitch + k] = row < rows ? spec[(row / n) * plane + (row % n) * pitch_s + k]
+                                  : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < L * TK2; e += FFT_THREADS) {
+    const int r = e / L, k = e % L;
+    const float2* h = Y + r * pitch;
+    float2 z;
+    if (even) {  // Z[k] = A + i B, as in c2r_rows_kernel
+      const float2 xk = h[k], xm = h[L - k];
+      const float2 wk = make_float2(tw[k].x, -tw[k].y);
+      const float2 A = make_float2(xk.x + xm.x, xk.y - xm.y);
+      const float2 B = cmul(make_float2(xk.x - xm.x, xk.y + xm.y), wk);
+      z = make_float2(A.x - B.y, A.y + B.x);
+    } else {  // the Hermitian row whole: v(n - k) = conj v(k)
+      z = k < nz ? h[k] : make_float2(h[n - k].x, -h[n - k].y);
+    }
+    X[r * pitch + k] = z;
+  }
+  __syncthreads();
+  const float2* R = gen_fft<true>(X, Y, P, pitch, TK2, tw, even ? 2 : 1);
+  for (int e = threadIdx.x; e < L * TK2; e += FFT_THREADS) {
+    const int r = e / L, m = e % L;
+    const uint64_t row = row0 + r;
+    if (row >= rows)
+      continue;
+    const float2 z = R[r * pitch + m];
+    if (even)
+      reinterpret_cast<float2*>(out + row * n)[m] = make_float2(z.x * norm, z.y * norm);
+    else
+      out[row * n + m] = z.x * norm;
+  }
+}
+
 // --- host side ---
 
 namespace {
@@ -462,10 +700,10 @@ struct Twiddles {
   int n = 0;
   DeviceBuffer fwd;  // exp(-2 pi i m / n), m < n
 };
-Twiddles g_tw;
-DeviceBuffer g_lut, g_axis;
-std::vector<float> g_lut_host;
-std::vector<double> g_axis_host;
+thread_local Twiddles g_tw;
+thread_local DeviceBuffer g_lut, g_axis;
+thread_local std::vector<float> g_lut_host;
+thread_local std::vector<double> g_axis_host;
 
 int log2_exact(uint32_t n) {
   int l = 0;
@@ -537,25 +775,124 @@ int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s) {
   default: { constexpr int N = 1024; return call; }   \
   }
 
+int gen_run_cols(uint32_t n, bool inv, const float2* in, float2* out, uint64_t stride_a,
+  uint64_t stride_i, const float* lut, const double* axis, hipStream_t s);
+int gen_rows(uint32_t n, bool inv, const float* rin, const float2* cin, float2* cout,
+  float* rout, float norm, hipStream_t s);
+bool native_pow2(uint32_t n);
+
 int run_cols(uint32_t n, bool inv, const float2* in, float2* out, uint64_t stride_a,
   uint64_t stride_i, const float* lut, const double* axis, hipStream_t s) {
+  if (!native_pow2(n))
+    return gen_run_cols(n, inv, in, out, stride_a, stride_i, lut, axis, s);
   SIF_FFT_DISPATCH(n, pass_cols<N>(inv, in, out, stride_a, stride_i, lut, axis, s))
 }
 int run_r2c_rows(uint32_t n, const float* in, float2* spec) {
+  if (!native_pow2(n))
+    return gen_rows(n, false, in, nullptr, spec, nullptr, 0.0f, stream());
   SIF_FFT_DISPATCH(n, pass_r2c_rows<N>(in, spec))
 }
 int run_c2r_rows(uint32_t n, const float2* spec, float* out, float norm, hipStream_t s) {
+  if (!native_pow2(n))
+    return gen_rows(n, true, nullptr, spec, nullptr, out, norm, s);
   SIF_FFT_DISPATCH(n, pass_c2r_rows<N>(spec, out, norm, s))
+}
+
+// The radices of a length: 8s, a 4, a 2, then each odd prime up to 31.
+bool gen_plan(int len, GenPlan* P) {
+  P->len = len;
+  P->stages = 0;
+  int m = len;
+  auto push = [&](int r) {
+    if (P->stages < GEN_MAX_STAGES)
+      P->radix[P->stages] = r;
+    P->stages++;
+  };
+  while (m % 8 == 0) {
+    push(8);
+    m /= 8;
+  }
+  if (m % 4 == 0) {
+    push(4);
+    m /= 4;
+  }
+  if (m % 2 == 0) {
+    push(2);
+    m /= 2;
+  }
+  for (int q = 3; q <= GEN_MAX_PRIME; q += 2)
+    while (m % q == 0) {
+      push(q);
+      m /= q;
+    }
+  return m == 1 && P->stages <= GEN_MAX_STAGES;
+}
+
+// Pencils a tile for the general kernels: up to `want`, two buffers within
+// GEN_LDS_BYTES.
+int gen_tile(int len, int want) {
+  int t = want;
+  while (t > 1 && 2 * t * (len + 1) * (int)sizeof(float2) > GEN_LDS_BYTES)
+    t--;
+  return 2 * t * (len + 1) * (int)sizeof(float2) <= GEN_LDS_BYTES ? t : 0;
+}
+
+bool native_pow2(uint32_t n) {
+  return n >= 16 && n <= 1024 && (n & (n - 1)) == 0;
 }
 
 int check_n(uint32_t n, int* log2n) {
   *log2n = log2_exact(n);
-  if (*log2n < 4 || n > 1024) {
+  if (native_pow2(n))
+    return SIF_OK;
+  GenPlan P;
+  if (n < 2 || !gen_plan((int)n, &P) || (n % 2 == 0 && !gen_plan((int)n / 2, &P)) ||
+      gen_tile((int)n, 1) == 0) {
     error_set(SIF_ERR_UNSUPPORTED,
-      "fft: n = %u cells per side (this build transforms powers of two from 16 to 1024)",
-      n);
+      "fft: n = %u cells per side (this build transforms n >= 2 whose prime factors are at "
+      "most %d, n below 4096)", n, GEN_MAX_PRIME);
     return SIF_ERR_UNSUPPORTED;
   }
+  return SIF_OK;
+}
+
+// The three passes of the general kernels.
+int gen_run_cols(uint32_t n, bool inv, const float2* in, float2* out, uint64_t stride_a,
+  uint64_t stride_i, const float* lut, const double* axis, hipStream_t s) {
+  GenPlan P;
+  gen_plan((int)n, &P);
+  const int nz = (int)n / 2 + 1;
+  const int tk = gen_tile((int)n, 16);
+  const int tiles = (nz + tk - 1) / tk;
+  const size_t lds = 2 * (size_t)tk * (n + 1) * sizeof(float2);
+  if (inv)
+    gen_cols_kernel<true><<<n * tiles, FFT_THREADS, lds, s>>>(in, out, nz, tk, stride_a,
+      stride_i, P, g_tw.fwd.as<float2>(), lut, axis);
+  else
+    gen_cols_kernel<false><<<n * tiles, FFT_THREADS, lds, s>>>(in, out, nz, tk, stride_a,
+      stride_i, P, g_tw.fwd.as<float2>(), lut, axis);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+int gen_rows(uint32_t n, bool inv, const float* rin, const float2* cin, float2* cout,
+  float* rout, float norm, hipStream_t s) {
+  GenPlan P;
+  const int len = n % 2 == 0 ? (int)n / 2 : (int)n;
+  gen_plan(len, &P);
+  int want = 8192 / (len + 1);
+  want = want < 1 ? 1 : (want > 256 ? 256 : want);
+  const int tk2 = gen_tile(len, want);
+  const uint64_t rows = (uint64_t)n * n;
+  const unsigned blocks = (unsigned)((rows + tk2 - 1) / tk2);
+  const size_t lds = 2 * (size_t)tk2 * (len + 1) * sizeof(float2);
+  if (inv)
+    gen_c2r_rows_kernel<<<blocks, FFT_THREADS, lds, s>>>(cin, rout, rows, (int)n, tk2,
+      spectrum_pitch(n), spectrum_plane(n), norm, P, g_tw.fwd.as<float2>());
+  else
+    gen_r2c_rows_kernel<<<blocks, FFT_THREADS, lds, s>>>(rin, cout, rows, (int)n, tk2,
+      spectrum_pitch(n), spectrum_plane(n), P, g_tw.fwd.as<float2>());
+  SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
 

@@ -24,6 +24,8 @@
 #include <memory>
 #include <vector>
 
+#include <atomic>
+
 #include "common.h"
 #include "internal.h"
 #include "primitives.h"
@@ -320,22 +322,35 @@ struct TemplateCache {
 // under SIF_FINDER_EXTEND_REACH, one per level of the extended reach, each
 // built once per run. At most TEMPLATE_SLOTS are kept; the oldest goes first.
 constexpr size_t TEMPLATE_SLOTS = 48;
-std::vector<std::unique_ptr<TemplateCache>> g_tpls;
+thread_local std::vector<std::unique_ptr<TemplateCache>> g_tpls;
 // the scratch of the extended-reach passes: the candidates still beyond their
 // reach, gathered, their results, and the count
-DeviceBuffer g_ext_idx, g_ext_cx, g_ext_cy, g_ext_cz, g_ext_rad, g_ext_reason, g_ext_nin,
+thread_local DeviceBuffer g_ext_idx, g_ext_cx, g_ext_cy, g_ext_cz, g_ext_rad, g_ext_reason, g_ext_nin,
   g_ext_count;
-DeviceBuffer g_pf_counts;
+thread_local DeviceBuffer g_pf_counts;
 // the split first pass's partial histograms and counts
-DeviceBuffer g_part_bins, g_part_counts;
+thread_local DeviceBuffer g_part_bins, g_part_counts;
 
 // A launch of fewer candidates than the chip holds waves (256 CUs, 16 waves
 // each at this kernel's registers and LDS) gives each candidate's first pass
-// to several waves, up to this many, so that one large candidate does not
-// hold the launch for the time a single wave takes over its sphere.
+// to several waves, up to MAX_SLICES, so that one large candidate does not
+// hold the launch for the time a single wave takes over its sphere. The wave
+// count aimed at is a tuning (several runs side by side share the chip, and
+// each should aim lower); the split changes no bit (integer partial sums).
 constexpr uint32_t WAVE_SLOTS = 4096;
 constexpr uint32_t MAX_SLICES = 16;
+std::atomic<uint32_t> g_split_waves{WAVE_SLOTS};
 }  // namespace
+
+uint32_t grow_split_waves() { return g_split_waves.load(std::memory_order_relaxed); }
+int set_grow_split_waves(uint32_t waves) {
+  if (waves < 64 || waves > 65536) {
+    error_set(SIF_ERR_INVALID, "growth split target %u waves (64 to 65536)", waves);
+    return SIF_ERR_INVALID;
+  }
+  g_split_waves.store(waves, std::memory_order_relaxed);
+  return SIF_OK;
+}
 
 static int template_device(uint32_t mesh_n, float l, float rmin, float r_search,
   const TemplateCache** out) {
@@ -423,8 +438,9 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
     grow_lds_words(nb, 0, &P.lds_bins, &P.lds_cache);
   const size_t lds_bytes =
     (size_t)GROW_WAVES * grow_wave_words(P.lds_bins, P.lds_cache, weighted) * sizeof(uint32_t);
-  if (!weighted && n < WAVE_SLOTS / 2) {
-    const uint32_t slices = std::min<uint32_t>(MAX_SLICES, WAVE_SLOTS / n);
+  const uint32_t split_waves = grow_split_waves();
+  if (!weighted && n < split_waves / 2) {
+    const uint32_t slices = std::min<uint32_t>(MAX_SLICES, split_waves / n);
     int st2 = g_part_bins.reserve((size_t)n * slices * nb * sizeof(uint32_t));
     if (st2 == SIF_OK)
       st2 = g_part_counts.reserve((size_t)n * slices * PART_COUNTS * sizeof(uint64_t));
@@ -584,7 +600,7 @@ __global__ void void_scatter_kernel(const uint32_t* __restrict__ slot, uint32_t 
 }
 
 namespace {
-DeviceBuffer g_vi_slot, g_vi_counts, g_vi_pos, g_vi_cursor;
+thread_local DeviceBuffer g_vi_slot, g_vi_counts, g_vi_pos, g_vi_cursor;
 }
 
 int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,

@@ -17,6 +17,9 @@ int check_in_box(const float* x, const float* y, const float* z, uint64_t n,
 int cic_deposit(const float* x, const float* y, const float* z, const float* w,
   uint64_t n, uint32_t N, float box, float* values, int64_t* acc);
 int grid_mean(const float* values, uint64_t total, double* mean);
+// The CIC tile edge (8 or 16 nodes): a tuning that changes no bit.
+uint32_t cic_tile_edge();
+int set_cic_tile_edge(uint32_t t);
 int density_contrast(float* values, uint64_t total, double mean);
 
 // mesh.hip
@@ -85,6 +88,10 @@ struct MeshWeights {
   const float* cell = nullptr;
   float mass_factor = 0.0f;
 };
+// The wave count a small growth launch's split first pass aims at (a
+// tuning that changes no bit).
+uint32_t grow_split_waves();
+int set_grow_split_waves(uint32_t waves);
 int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   const float* mz, uint32_t mesh_n, float box, const float* cx, const float* cy,
   const float* cz, uint32_t n, float rmin, float r_search, float threshold,

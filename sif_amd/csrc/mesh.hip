@@ -272,7 +272,7 @@ __global__ void count_max_kernel(const uint32_t* __restrict__ counts, uint64_t c
 }
 
 namespace {
-DeviceBuffer g_keys, g_vals, g_counts, g_cd, g_blocks, g_cursor, g_src, g_max;
+thread_local DeviceBuffer g_keys, g_vals, g_counts, g_cd, g_blocks, g_cursor, g_src, g_max;
 }
 
 // The stable radix path (cells of more than MESH_ORDER_CAP particles):

@@ -131,7 +131,7 @@ static int scan_u64_rec(const uint64_t* in, uint64_t n, uint64_t* out,
 }
 
 namespace {
-DeviceBuffer g_scan_scratch[4];
+thread_local DeviceBuffer g_scan_scratch[4];
 }
 
 int exclusive_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out,
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(RS_THREADS) radix_scatter(const uint32_t* __re
 }
 
 namespace {
-DeviceBuffer g_rs_hist, g_rs_offs, g_rs_k, g_rs_v;
+thread_local DeviceBuffer g_rs_hist, g_rs_offs, g_rs_k, g_rs_v;
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t differ) {
@@ -283,7 +283,7 @@ __global__ void compact_scatter(const uint32_t* __restrict__ in,
 }
 
 namespace {
-DeviceBuffer g_cpos, g_ccount;
+thread_local DeviceBuffer g_cpos, g_ccount;
 }
 
 __global__ void compact2_scatter(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,

@@ -438,8 +438,8 @@ __global__ void __launch_bounds__(RESID_THREADS) residual_kernel(const float* __
     block_sums[blockIdx.x] = s[0];
 }
 
-DeviceBuffer g_pair_count, g_pair_offset, g_pair_keys, g_pair_vals, g_tile_start, g_tile_end;
-DeviceBuffer g_resid;
+thread_local DeviceBuffer g_pair_count, g_pair_offset, g_pair_keys, g_pair_vals, g_tile_start, g_tile_end;
+thread_local DeviceBuffer g_resid;
 
 bool bins_supported(uint32_t n_bins) { return n_bins >= 1 && n_bins <= 32; }
 

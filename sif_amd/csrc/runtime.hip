@@ -82,9 +82,18 @@ int ensure_init() {
   return init_locked();
 }
 
+// A thread may run the library on streams of its own (the batched form's
+// workers): its calls then use them, and stage timing is off there.
+thread_local hipStream_t t_stream = nullptr, t_aux = nullptr;
+
 bool initialised() { return g_init; }
-hipStream_t stream() { return g_stream; }
-hipStream_t aux_stream() { return g_aux; }
+hipStream_t stream() { return t_stream ? t_stream : g_stream; }
+hipStream_t aux_stream() { return t_aux ? t_aux : g_aux; }
+bool on_own_streams() { return t_stream != nullptr; }
+void set_thread_streams(hipStream_t s, hipStream_t aux) {
+  t_stream = s;
+  t_aux = aux;
+}
 
 int DeviceBuffer::reserve(size_t want) {
   if (want <= bytes && ptr)
@@ -140,11 +149,11 @@ static void timing_resolve() {
   g_pending.clear();
 }
 
-void timing_begin(int s) { timing_begin(s, g_stream); }
-void timing_end(int s) { timing_end(s, g_stream); }
+void timing_begin(int s) { timing_begin(s, stream()); }
+void timing_end(int s) { timing_end(s, stream()); }
 
 void timing_begin(int s, hipStream_t st) {
-  if (!g_timing)
+  if (!g_timing || t_stream)
     return;
   hipEvent_t e = take_event();
   if (!e)
@@ -157,7 +166,7 @@ void timing_begin(int s, hipStream_t st) {
 }
 
 void timing_end(int s, hipStream_t st) {
-  if (!g_timing || !g_open[s])
+  if (!g_timing || t_stream || !g_open[s])
     return;
   hipEvent_t e = take_event();
   if (!e)
@@ -169,14 +178,19 @@ void timing_end(int s, hipStream_t st) {
     timing_resolve();
 }
 
-bool timing_active() { return g_timing; }
+bool timing_active() { return g_timing && !t_stream; }
 
 void timing_add(int s, double ms) {
+  if (t_stream)
+    return;
   g_ms[s] += ms;
   g_count[s]++;
 }
 
-void timing_bytes(int s, double bytes) { g_bytes[s] += bytes; }
+void timing_bytes(int s, double bytes) {
+  if (!t_stream)
+    g_bytes[s] += bytes;
+}
 
 // grid-stride copy, one 16-byte element per thread per step
 __global__ void __launch_bounds__(256) copy_kernel(uint4* __restrict__ dst,

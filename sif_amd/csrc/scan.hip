@@ -210,7 +210,7 @@ __global__ void cand_unkey_kernel(const uint32_t* __restrict__ keys, uint64_t n,
 }
 
 namespace {
-DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span;
+thread_local DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span;
 }
 
 // Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)

@@ -349,7 +349,7 @@ __global__ void tracer_profile_finish_kernel(const float* __restrict__ rv, uint3
   }
 }
 
-DeviceBuffer g_tp_count, g_tp_mass, g_tp_vrad, g_tp_vols;
+thread_local DeviceBuffer g_tp_count, g_tp_mass, g_tp_vrad, g_tp_vols;
 
 }  // namespace
 

@@ -198,6 +198,75 @@ def test_fft_round_trip(native, n):
     assert mag.max() <= 1e-4 * n ** 3
 
 
+@pytest.mark.parametrize("n", [15, 24, 40, 72, 100, 136, 200, 248, 384])
+def test_fft_any_length(native, n):
+    """Lengths that are not powers of two (the mixed-radix kernels: radices 8,
+    4, 2 and odd primes up to 31; n odd through whole rows): against scipy's
+    FP64 transform, the round trip of test_fft.c:49-125 (which round-trips
+    n = 15) and a single mode in one bin."""
+    import scipy.fft
+    from sif_amd import device as D
+    rng = np.random.default_rng(n)
+    v = rng.standard_normal(n ** 3).astype(np.float32)
+    spec = D.spectrum(D.to_device(v), n)
+    s = D.spectrum_modes(spec, n)
+    want = scipy.fft.rfftn(v.reshape(n, n, n).astype(np.float64), workers=-1)
+    got = s[..., 0] + 1j * s[..., 1]
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+    back = D.smooth(spec, n, BOX, 0, 0.0, window_order=0).get()
+    assert np.abs(back - v).max() < 1e-4
+    k = (1, 2, 3)
+    i = np.arange(n, dtype=np.float64)
+    phase = (k[0] * i)[:, None, None] + (k[1] * i)[None, :, None] + (k[2] * i)[None, None, :]
+    wave = np.cos(2 * np.pi * phase / n).astype(np.float32).ravel()
+    s1 = D.spectrum_modes(D.spectrum(D.to_device(wave), n), n)
+    mag = np.hypot(s1[..., 0], s1[..., 1])
+    assert abs(mag[k] - n ** 3 / 2) <= 1e-4 * n ** 3
+    mag[k] = 0.0
+    assert mag.max() <= 1e-4 * n ** 3
+
+
+@pytest.mark.parametrize("n,radius", [(72, 6.0), (100, 9.0), (136, 12.5)])
+def test_smoothing_any_length_against_fp64(native, n, radius):
+    """The filtered c2r (top hat, CIC window divided out) of a field on a
+    grid that is not a power of two, against the same factors on an FP64
+    transform; 1e-5 of the field's range."""
+    import oracle
+    from sif_amd import device as D
+    v = np.random.default_rng(n + 1).standard_normal(n ** 3).astype(np.float32)
+    spec = D.spectrum(D.to_device(v), n)
+    got = D.smooth(spec, n, BOX, 1, radius, window_order=2).get()
+    fac = oracle.mode_factors(oracle.filter_lut(1, radius, 0.0, BOX, n),
+                              oracle.window_axis(n, 2, 0), n).astype(np.float64)
+    want = np.fft.irfftn(np.fft.rfftn(v.reshape(n, n, n).astype(np.float64)) * fac,
+                         s=(n, n, n), axes=(0, 1, 2)).reshape(-1)
+    assert np.abs(got - want).max() <= 1e-5 * (want.max() - want.min())
+
+
+def test_exodus_on_a_grid_that_is_not_a_power_of_two(native):
+    """40^3 Zel'dovich particles on a 40^3 grid (the grid matching the
+    lattice) in a 62.5 Mpc/h box: the catalogue equals the oracle's on the
+    same smoothed fields."""
+    import oracle
+    from sif_amd import device as D
+    from sif_amd import zeldovich
+    n, box, mc = 40, 62.5, 6
+    x, y, z = zeldovich.generate(40, box, 31)
+    dx, dy, dz = D.to_device(x), D.to_device(y), D.to_device(z)
+    g = D.cic(dx, dy, dz, len(x), n, box)
+    D.density_contrast(g, n ** 3)
+    radii = zeldovich.ladder(2.0 * box / n, 10.0, 0.05)
+    fields = D.exodus_fields(g, n, box, radii)
+    m = D.Mesh(dx, dy, dz, len(x), mc, box)
+    cat, nin, _ = D.exodus(g, n, box, m, radii, THRESH)
+    om = oracle.OracleMesh(oracle.chain_mesh(x, y, z, mc, box), mc, box)
+    ocat, onin, _, _ = oracle.exodus_run(list(fields), n, box, om, len(x), radii, THRESH,
+                                        tie_rule=1)
+    assert len(cat) > 10
+    np.testing.assert_array_equal(cat.view(np.uint32), ocat.view(np.uint32))
+    np.testing.assert_array_equal(nin, onin)
+
+
 def test_fft_1024_properties(native):
     """n = 1024, where the column passes hold 73.8 KB of LDS a block: the
     round trip to 1e-4 (test_fft.c:49-125), Parseval's identity over the
