@@ -364,9 +364,9 @@ def profiles_form(step, cat, steps, warmup):
 
 # many boxes of uneven size: 96 seeded Zel'dovich sets, N^3 particles with N
 # log-uniform in 64-256 (a multiple of 8), boxes at the fixed mean separation
-# 100/64 Mpc/h (100-400 Mpc/h a side), grids at the power of two nearest N
-# (the FFT here takes powers of two), mesh cell 10 Mpc/h, radii from 3 Mpc/h
-# to min(60, L/4) Mpc/h (the growth template must stay inside the mesh),
+# 100/64 Mpc/h (100-400 Mpc/h a side), each on the grid matching its particle
+# lattice (N^3 cells), mesh cell 10 Mpc/h, radii from 3 Mpc/h to
+# min(60, L/4) Mpc/h (the growth template must stay inside the mesh),
 # threshold -0.8; box b uses seed 3000 + b
 BATCHED = dict(n_boxes=96, side_lo=64, side_hi=256, spacing=100.0 / 64.0, mesh_cell=10.0,
                r_lo=3.0, r_hi=60.0, seed0=3000, size_seed=3)
@@ -380,20 +380,20 @@ def batched_boxes():
     for b, side in enumerate(sides):
         n_side = int(min(max(round(side / 8.0) * 8, a["side_lo"]), a["side_hi"]))
         box = n_side * a["spacing"]
-        n_cells = 1 << int(round(np.log2(n_side)))
         mesh_cells = int(round(box / a["mesh_cell"]))
         r_hi = min(a["r_hi"], box / 4.0)
-        out.append(dict(seed=a["seed0"] + b, n_side=n_side, box=box, n_cells=n_cells,
+        out.append(dict(seed=a["seed0"] + b, n_side=n_side, box=box, n_cells=n_side,
                         mesh_cells=mesh_cells, r_hi=r_hi))
     return out
 
 
 def batched_form():
     """Sum of particles over the 96 boxes / the time of all their catalogues
-    (deposit, contrast, mesh, every rung), positions resident in HBM, one box
-    after another on the library stream."""
+    (deposit, contrast, mesh, every rung), positions resident in HBM, through
+    sif_amd_exodus_batch: the boxes side by side under the tuning the
+    autotuner picked (before the timed region); the same boxes one after
+    another on one worker, timed beside it."""
     from sif_amd import device as D, zeldovich
-    from sif_amd._native import DeviceArray
     t0 = time.time()
     boxes = batched_boxes()
     for bx in boxes:
@@ -401,45 +401,37 @@ def batched_form():
         bx["n"] = len(x)
         bx["d"] = (D.to_device(x), D.to_device(y), D.to_device(z))
         bx["radii"] = zeldovich.ladder(BATCHED["r_lo"], bx["r_hi"], 0.05)
-    n_max = max(bx["n_cells"] for bx in boxes) ** 3
-    grid = DeviceArray((n_max,), np.float32)
-    work = DeviceArray((n_max,), np.int64)
     total = sum(bx["n"] for bx in boxes)
     log(f"[bench] batched: {len(boxes)} boxes, {total} particles "
-        f"({min(bx['n_side'] for bx in boxes)}^3..{max(bx['n_side'] for bx in boxes)}^3), "
-        f"set up in {time.time() - t0:.1f}s")
-
-    def one(bx):
-        dx, dy, dz = bx["d"]
-        nc = bx["n_cells"]
-        D.cic(dx, dy, dz, bx["n"], nc, bx["box"], out=grid, work=work)
-        D.density_contrast(grid, nc ** 3)
-        mesh = D.Mesh(dx, dy, dz, bx["n"], bx["mesh_cells"], bx["box"], indices=False)
-        cat, _, _ = D.exodus(grid, nc, bx["box"], mesh, bx["radii"], THRESHOLD, restore=False)
-        return len(cat)
-
-    one(boxes[0])  # warm-up: the library's buffers at the smallest sizes
-    from sif_amd import _native as N
-    N.check(N.lib.sif_amd_synchronize())
+        f"({min(bx['n_side'] for bx in boxes)}^3..{max(bx['n_side'] for bx in boxes)}^3, "
+        f"grids matching), set up in {time.time() - t0:.1f}s")
+    t1 = time.perf_counter()
+    tuning = D.batch_autotune(boxes, THRESHOLD, probe=12)
+    t_tune = time.perf_counter() - t1
+    log(f"[bench] batched: autotuned in {t_tune:.1f}s: {tuning}")
+    D.exodus_batch(boxes, THRESHOLD, tuning=tuning)  # every worker's buffers at full size
     t = time.perf_counter()
-    per, voids = [], 0
-    for bx in boxes:
-        tb = time.perf_counter()
-        voids += one(bx)
-        per.append(time.perf_counter() - tb)
-    N.check(N.lib.sif_amd_synchronize())
+    cats = D.exodus_batch(boxes, THRESHOLD, tuning=tuning)
     dt = time.perf_counter() - t
-    log(f"[bench] batched: {dt:.2f}s for {len(boxes)} catalogues, {voids} voids")
-    per = np.array(per)
+    voids = sum(len(c) for c in cats)
+    log(f"[bench] batched: {dt:.2f}s for {len(boxes)} catalogues side by side, {voids} voids")
+    serial = dict(tuning, workers=1)
+    D.exodus_batch(boxes[:1], THRESHOLD, tuning=serial)
+    t = time.perf_counter()
+    cats1 = D.exodus_batch(boxes, THRESHOLD, tuning=serial)
+    dt1 = time.perf_counter() - t
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(cats, cats1))
+    log(f"[bench] batched: {dt1:.2f}s one after another, catalogues identical: {same}")
+    if not same:
+        raise RuntimeError("the batched catalogues differ from the boxes run one by one")
     return {"metric": "exodus particles/s over 96 boxes of uneven size",
             "value": round(total / dt, 1), "unit": "particles/s", "seconds": round(dt, 3),
-            "voids": int(voids), "particles": int(total),
-            "ms_per_box": {"min": round(per.min() * 1e3, 2), "median": round(float(np.median(per)) * 1e3, 2),
-                           "max": round(per.max() * 1e3, 2)},
-            "config": {"workload": "96 Zel'dovich boxes, N^3 log-uniform in 64^3-256^3, "
-                                   "L = N * 100/64 Mpc/h, grid the power of two nearest N, "
-                                   "mesh cell 10 Mpc/h, radii 3-min(60, L/4) Mpc/h, t=-0.8, "
-                                   "one box after another"}}
+            "seconds_one_after_another": round(dt1, 3), "voids": int(voids),
+            "particles": int(total), "tuning": tuning, "autotune_seconds": round(t_tune, 2),
+            "config": {"workload": "96 Zel'dovich boxes, N^3 log-uniform in 64^3-256^3 "
+                                   "(N a multiple of 8), L = N * 100/64 Mpc/h, grid N^3 (the "
+                                   "particle lattice), mesh cell 10 Mpc/h, radii "
+                                   "3-min(60, L/4) Mpc/h, t=-0.8, sif_amd_exodus_batch"}}
 
 
 def main():

@@ -275,6 +275,55 @@ SIF_API int sif_amd_squared_residual(const float* d_p, const float* d_target,
  * synchronised, like the reference's settings table. */
 SIF_API int sif_amd_set_batch_window(uint32_t floor, uint32_t ceiling);
 
+/* --- the batched form (no reference counterpart: many independent boxes,
+ * each the catalogue sif_finder_exodus_raw would give it) --- */
+
+/* One box: its particles (device, n_particles each), its grid and mesh
+ * resolutions (grid cells per side: any n whose prime factors are at most
+ * 31; mesh cells per side at most 1024), its side and its radius ladder
+ * (host, any order). Unweighted. */
+typedef struct {
+  const float* d_x;
+  const float* d_y;
+  const float* d_z;
+  uint64_t n_particles;
+  uint32_t n_cells;
+  uint32_t mesh_cells;
+  float box;
+  const sif_real* radii;
+  uint32_t n_radii;
+} sif_amd_box_t;
+
+/* How a batch runs; no field changes any bit of any catalogue. 0 in a field
+ * keeps its default: workers 8 (boxes side by side, each on its own stream
+ * pair and buffers, 1 to 64), cic_tile 8 (the CIC deposit's tile edge, 8 or
+ * 16 nodes), split_waves 4096 (the wave count a growth launch of few
+ * candidates splits their first passes towards, 64 to 65536), batch_floor
+ * and batch_ceiling as sif_amd_set_batch_window. */
+typedef struct {
+  uint32_t workers;
+  uint32_t cic_tile;
+  uint32_t split_waves;
+  uint32_t batch_floor;
+  uint32_t batch_ceiling;
+} sif_amd_tuning_t;
+
+/* Every box's exodus catalogue (CIC deposit, density contrast, chain mesh,
+ * sif__exodus_run of exodus_finder.c:1585-2492 over its ladder), each the
+ * same bits as the box run alone: catalogues[b] receives box b's (the
+ * caller frees each with sif_catalogue_free). tuning may be NULL. On an
+ * error no catalogue is returned and the message names the box. */
+SIF_API int sif_amd_exodus_batch(const sif_amd_box_t* boxes, uint32_t n_boxes,
+  sif_real threshold, sif_real overlap_fraction, sif_option options,
+  const sif_amd_tuning_t* tuning, sif_catalogue_t** catalogues);
+
+/* Times a few tunings (boxes side by side, the growth split, the CIC tile)
+ * on `probe` boxes spread over the sizes (0: 8), checks that each gives
+ * the same catalogues, and returns the fastest. */
+SIF_API int sif_amd_batch_autotune(const sif_amd_box_t* boxes, uint32_t n_boxes,
+  sif_real threshold, sif_real overlap_fraction, sif_option options, uint32_t probe,
+  sif_amd_tuning_t* best);
+
 /* Per-rung smoothed fields as the finder scans them, for parity checks:
  * rung i (descending ladder) into d_fields + i * n_cells^3. */
 SIF_API int sif_amd_exodus_fields(const float* d_delta, uint32_t n_cells,

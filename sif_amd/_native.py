@@ -116,6 +116,22 @@ _sig("sif_amd_exodus_device_weighted", C.POINTER(Catalogue), P, u32, f32, P, P, 
      C.POINTER(RungStats))
 _sig("sif_amd_exodus_fields", i32, P, u32, f32, C.POINTER(f32), u32, u32, P)
 _sig("sif_amd_set_batch_window", i32, u32, u32)
+
+
+class Box(C.Structure):  # sif_amd_box_t
+    _fields_ = [("d_x", P), ("d_y", P), ("d_z", P), ("n_particles", u64), ("n_cells", u32),
+                ("mesh_cells", u32), ("box", f32), ("radii", C.POINTER(f32)), ("n_radii", u32)]
+
+
+class Tuning(C.Structure):  # sif_amd_tuning_t
+    _fields_ = [("workers", u32), ("cic_tile", u32), ("split_waves", u32),
+                ("batch_floor", u32), ("batch_ceiling", u32)]
+
+
+_sig("sif_amd_exodus_batch", i32, C.POINTER(Box), u32, f32, f32, u32, C.POINTER(Tuning),
+     C.POINTER(C.POINTER(Catalogue)))
+_sig("sif_amd_batch_autotune", i32, C.POINTER(Box), u32, f32, f32, u32, u32,
+     C.POINTER(Tuning))
 _sig("sif_amd_tracer_profiles", i32, P, P, P, P, P, P, P, P, u32, f32, f64, P, P, P, P, u32,
      f32, u32, u32, P, P)
 _sig("sif_catalogue_free", None, C.POINTER(Catalogue))

@@ -1233,6 +1233,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   return SIF_OK;
 }
 
+int exodus_batch_window(uint32_t* floor, uint32_t* ceiling) {
+  *floor = g_batch_floor;
+  *ceiling = g_batch_max;
+  return SIF_OK;
+}
+
 int sorted_ladder(const sif_real* radii, uint32_t n_radii, std::vector<float>& out) {
   out.assign(radii, radii + n_radii);
   qsort(out.data(), out.size(), sizeof(float), cmp_desc);

@@ -163,6 +163,9 @@ struct RunSpec {
 int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   sif_amd_rung_stats_t* stats_out);
 int sorted_ladder(const sif_real* radii, uint32_t n_radii, std::vector<float>& out);
+int check_options(sif_option opt);
+// The growth batch window now in force (sif_amd_set_batch_window).
+int exodus_batch_window(uint32_t* floor, uint32_t* ceiling);
 int check_args(const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real f);
 int check_options(sif_option opt);
 // exodus_finder.c:1051-1106: a weighted mesh the rescaling can use -- a

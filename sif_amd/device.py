@@ -223,6 +223,43 @@ def set_batch_window(floor=0, ceiling=0):
     check(lib.sif_amd_set_batch_window(floor, ceiling))
 
 
+def _boxes(boxes):
+    """ctypes boxes from dicts: d (dx, dy, dz device arrays), n, n_cells,
+    mesh_cells, box, radii; the ladders are kept alive with the array."""
+    arr = (N.Box * len(boxes))()
+    keep = []
+    for b, bx in enumerate(boxes):
+        r = np.ascontiguousarray(bx["radii"], dtype=np.float32)
+        keep.append(r)
+        dx, dy, dz = bx["d"]
+        arr[b] = N.Box(dx.ptr, dy.ptr, dz.ptr, bx["n"], bx["n_cells"], bx["mesh_cells"],
+                       bx["box"], r.ctypes.data_as(C.POINTER(C.c_float)), len(r))
+    return arr, keep
+
+
+TUNING_FIELDS = ("workers", "cic_tile", "split_waves", "batch_floor", "batch_ceiling")
+
+
+def exodus_batch(boxes, threshold, overlap=0.0, options=0, tuning=None):
+    """Every box's exodus catalogue (sif_amd_exodus_batch): a list of (n, 4)
+    arrays in box order. tuning: a dict of TUNING_FIELDS (missing: defaults)."""
+    arr, keep = _boxes(boxes)
+    t = N.Tuning(*[int((tuning or {}).get(k, 0)) for k in TUNING_FIELDS])
+    cats = (C.POINTER(N.Catalogue) * len(boxes))()
+    check(lib.sif_amd_exodus_batch(arr, len(boxes), threshold, overlap, options, C.byref(t),
+                                   cats))
+    return [N.catalogue_to_numpy(c) for c in cats]
+
+
+def batch_autotune(boxes, threshold, overlap=0.0, options=0, probe=8):
+    """The fastest tuning on `probe` of the boxes (sif_amd_batch_autotune)."""
+    arr, keep = _boxes(boxes)
+    t = N.Tuning()
+    check(lib.sif_amd_batch_autotune(arr, len(boxes), threshold, overlap, options, probe,
+                                     C.byref(t)))
+    return {k: getattr(t, k) for k in TUNING_FIELDS}
+
+
 def exodus_fields(ddelta, n_cells, box, radii, options=0):
     radii = np.ascontiguousarray(radii, dtype=np.float32)
     total = n_cells ** 3
