@@ -13,13 +13,17 @@
 //   scatter  the folded positions (and weights, and the source index) to a
 //            slot of their cell claimed with an atomic cursor, in whatever
 //            order the wavefronts arrive;
-//   order    one wavefront per cell puts its slots in source order: each
-//            element's rank among the cell's source indices (compared in LDS)
-//            is its place, so the result does not depend on the arrival order.
+//   order    a workgroup per cell puts its slots in source order: each
+//            element's rank among the cell's source indices (sorted in LDS,
+//            then searched) is its place, so the result does not depend on
+//            the arrival order. One wavefront takes a cell of up to
+//            MESH_ORDER_CAP particles; a larger one gets MESH_BIG_THREADS
+//            threads and up to MESH_BIG_BYTES of indices (the count pass
+//            lists those cells).
 //
-// Cells of more than MESH_ORDER_CAP particles are ordered by the stable LSD
-// radix sort of (cell, source index) instead (the whole mesh takes that path
-// then). Source indices are 32-bit below 2^32 particles and 64-bit above.
+// A cell past what a workgroup's LDS holds sends the whole mesh through the
+// stable LSD radix sort of (cell, source index) instead. Source indices are
+// 32-bit below 2^32 particles and 64-bit above.
 #include "common.h"
 #include "internal.h"
 #include "primitives.h"
@@ -115,7 +119,13 @@ __global__ void weight_blocks_kernel(const double* __restrict__ cd, uint64_t cel
 // --- count, scatter, order ---
 
 constexpr uint32_t MESH_ORDER_CAP = 1024;  // particles a cell, for the ordering pass
-constexpr int MESH_ORDER_WAVES = 4;        // cells a workgroup (one wavefront each)
+// The large cells' pass: a bigger workgroup a cell, its source indices in LDS.
+constexpr uint32_t MESH_BIG_THREADS = 1024;
+constexpr uint32_t MESH_BIG_BYTES = 64 * 1024;
+template <class IdxT>
+constexpr uint32_t mesh_big_cap() {
+  return MESH_BIG_BYTES / (uint32_t)sizeof(IdxT);
+}
 
 __device__ __forceinline__ uint32_t mesh_cell_of(float x, float y, float z, uint32_t nc,
   float inv_l, float box) {
@@ -193,76 +203,107 @@ __global__ void __launch_bounds__(256) mesh_scatter_kernel(const float* __restri
   }
 }
 
-// One wavefront per cell (of at most MESH_ORDER_CAP particles): every element
-// goes to its rank among the cell's source indices, which are distinct.
-// Positions are read into registers before any is written (the cell's slots
-// are rewritten in place).
-template <class IdxT>
-__global__ void __launch_bounds__(64 * MESH_ORDER_WAVES) mesh_order_kernel(
-  const uint64_t* __restrict__ offsets, uint64_t cells, const IdxT* __restrict__ src,
-  float* mx, float* my, float* mz, float* mw, uint64_t* __restrict__ orig) {
-  constexpr int PER = MESH_ORDER_CAP / 64;
-  __shared__ IdxT ids[MESH_ORDER_WAVES][MESH_ORDER_CAP];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint64_t c = (uint64_t)blockIdx.x * MESH_ORDER_WAVES + wave;
-  uint64_t b = 0;
-  uint32_t cnt = 0;
-  if (c < cells) {
-    b = offsets[c];
-    cnt = (uint32_t)(offsets[c + 1] - b);
-  }
-  const uint32_t nt = (cnt + 63) / 64;  // element slots in use, uniform over the wave
-  IdxT own[PER];
-  float px[PER], py[PER], pz[PER], pw[PER];
-#pragma unroll
-  for (int t = 0; t < PER; t++) {
-    const uint32_t j = lane + 64u * t;
-    own[t] = 0;
-    px[t] = py[t] = pz[t] = pw[t] = 0.0f;
-    if (j < cnt) {
-      own[t] = src[b + j];
-      ids[wave][j] = own[t];
-      px[t] = mx[b + j];
-      py[t] = my[b + j];
-      pz[t] = mz[b + j];
-      pw[t] = mw ? mw[b + j] : 0.0f;
+// The ordering pass: a workgroup of THREADS threads a cell of at most CAP
+// particles -- cells q = blockIdx.x, blockIdx.x + gridDim.x, ... below
+// `count`: cell q, or big[q] when a list is given.
+// Every element's place is its rank among the cell's source indices, which
+// are distinct: the indices are sorted in LDS (bitonic, padded to a power of
+// two with the largest value) and each element finds its own there by binary
+// search, so the result does not depend on the order the scatter left. Every
+// thread holds its own elements (thread t the elements t, t + THREADS, ...)
+// in registers before any is written back (the cell's slots are rewritten in
+// place). A cell of more than CAP particles is left to the other launch.
+template <class IdxT, uint32_t THREADS, uint32_t CAP>
+__global__ void __launch_bounds__(THREADS) mesh_order_kernel(
+  const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ big, uint64_t count,
+  const IdxT* __restrict__ src, float* mx, float* my, float* mz, float* mw,
+  uint64_t* __restrict__ orig) {
+  constexpr int PER = (int)(CAP / THREADS);
+  __shared__ IdxT ids[CAP];
+  for (uint64_t q = blockIdx.x; q < count; q += gridDim.x) {
+    const uint64_t c = big ? big[q] : q;
+    const uint64_t b = offsets[c];
+    uint32_t cnt = (uint32_t)(offsets[c + 1] - b);
+    if (cnt > CAP)
+      cnt = 0;
+    uint32_t p2 = 1;
+    while (p2 < cnt)
+      p2 <<= 1;
+    const uint32_t tid = threadIdx.x;
+    IdxT own[PER];
+    float px[PER], py[PER], pz[PER], pw[PER];
+  #pragma unroll
+    for (int t = 0; t < PER; t++) {
+      const uint32_t j = tid + THREADS * (uint32_t)t;
+      own[t] = 0;
+      px[t] = py[t] = pz[t] = pw[t] = 0.0f;
+      if (j < cnt) {
+        own[t] = src[b + j];
+        px[t] = mx[b + j];
+        py[t] = my[b + j];
+        pz[t] = mz[b + j];
+        pw[t] = mw ? mw[b + j] : 0.0f;
+      }
+      if (j < p2)
+        ids[j] = j < cnt ? own[t] : (IdxT)~(IdxT)0;
     }
-  }
-  __syncthreads();
-  uint32_t rank[PER];
-#pragma unroll
-  for (int t = 0; t < PER; t++)
-    rank[t] = 0;
-  for (uint32_t k = 0; k < cnt; k++) {
-    const IdxT v = ids[wave][k];
-#pragma unroll
-    for (int t = 0; t < PER; t++)
-      if ((uint32_t)t < nt)
-        rank[t] += (v < own[t]) ? 1u : 0u;
-  }
-#pragma unroll
-  for (int t = 0; t < PER; t++) {
-    const uint32_t j = lane + 64u * t;
-    if (j < cnt) {
-      const uint64_t at = b + rank[t];
-      mx[at] = px[t];
-      my[at] = py[t];
-      mz[at] = pz[t];
-      if (mw)
-        mw[at] = pw[t];
-      if (orig)
-        orig[at] = (uint64_t)own[t];
+    __syncthreads();
+    for (uint32_t k = 2; k <= p2; k <<= 1) {
+      for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+        for (uint32_t i = tid; i < p2; i += THREADS) {
+          const uint32_t l = i ^ h;
+          if (l > i) {
+            const IdxT u = ids[i], v = ids[l];
+            const bool up = (i & k) == 0;
+            if (up ? (u > v) : (u < v)) {
+              ids[i] = v;
+              ids[l] = u;
+            }
+          }
+        }
+        __syncthreads();
+      }
     }
+  #pragma unroll
+    for (int t = 0; t < PER; t++) {
+      const uint32_t j = tid + THREADS * (uint32_t)t;
+      if (j < cnt) {
+        uint32_t lo = 0, hi = cnt;  // the first place holding own[t] or more
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (ids[mid] < own[t])
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        const uint64_t at = b + lo;
+        mx[at] = px[t];
+        my[at] = py[t];
+        mz[at] = pz[t];
+        if (mw)
+          mw[at] = pw[t];
+        if (orig)
+          orig[at] = (uint64_t)own[t];
+      }
+    }
+    __syncthreads();  // the next cell rewrites ids
   }
 }
 
-// The largest cell count.
+// The largest cell count, and the cells of more than MESH_ORDER_CAP particles
+// (out[1] of them; the first `big_cap` are listed in `big`, in no order).
 __global__ void count_max_kernel(const uint32_t* __restrict__ counts, uint64_t cells,
-  uint32_t* __restrict__ out) {
+  uint32_t* __restrict__ out, uint64_t* __restrict__ big, uint32_t big_cap) {
   uint32_t m = 0;
   for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < cells;
-       c += (uint64_t)gridDim.x * blockDim.x)
+       c += (uint64_t)gridDim.x * blockDim.x) {
     m = counts[c] > m ? counts[c] : m;
+    if (counts[c] > MESH_ORDER_CAP) {
+      const uint32_t k = atomicAdd(&out[1], 1u);
+      if (k < big_cap)
+        big[k] = c;
+    }
+  }
   for (int off = 32; off > 0; off >>= 1) {
     const uint32_t o = __shfl_xor(m, off, 64);
     m = o > m ? o : m;
@@ -272,10 +313,10 @@ __global__ void count_max_kernel(const uint32_t* __restrict__ counts, uint64_t c
 }
 
 namespace {
-thread_local DeviceBuffer g_keys, g_vals, g_counts, g_cd, g_blocks, g_cursor, g_src, g_max;
+thread_local DeviceBuffer g_keys, g_vals, g_counts, g_cd, g_blocks, g_cursor, g_src, g_max, g_big;
 }
 
-// The stable radix path (cells of more than MESH_ORDER_CAP particles):
+// The stable radix path (a cell past what a workgroup's LDS holds):
 // (cell, source index) pairs sorted by cell, then the positions gathered.
 static int mesh_by_radix(const float* x, const float* y, const float* z, const float* w,
   uint32_t n, uint32_t nc, float inv_l, float box, uint64_t cells, float* mx, float* my,
@@ -305,7 +346,7 @@ static int mesh_by_radix(const float* x, const float* y, const float* z, const f
 template <class IdxT>
 static int mesh_scatter_order(const float* x, const float* y, const float* z, const float* w,
   uint64_t n, uint32_t nc, float inv_l, float box, uint64_t cells, const uint64_t* offsets,
-  float* mx, float* my, float* mz, float* mw, uint64_t* orig) {
+  float* mx, float* my, float* mz, float* mw, uint64_t* orig, uint32_t n_big) {
   int st = g_src.reserve((n ? n : 1) * sizeof(IdxT));
   if (st != SIF_OK)
     return st;
@@ -313,10 +354,16 @@ static int mesh_scatter_order(const float* x, const float* y, const float* z, co
   mesh_scatter_kernel<IdxT><<<grid_for(n, 256), 256, 0, stream()>>>(x, y, z, w, n, nc, inv_l,
     box, offsets, g_cursor.as<uint32_t>(), mx, my, mz, w ? mw : nullptr, g_src.as<IdxT>());
   SIF_HIP(hipGetLastError());
-  const uint64_t groups = (cells + MESH_ORDER_WAVES - 1) / MESH_ORDER_WAVES;
-  mesh_order_kernel<IdxT><<<(unsigned)groups, 64 * MESH_ORDER_WAVES, 0, stream()>>>(offsets,
-    cells, g_src.as<IdxT>(), mx, my, mz, w ? mw : nullptr, orig);
+  const uint64_t blocks = cells < (1ull << 22) ? cells : (1ull << 22);
+  mesh_order_kernel<IdxT, 64, MESH_ORDER_CAP><<<(unsigned)blocks, 64, 0, stream()>>>(offsets,
+    nullptr, cells, g_src.as<IdxT>(), mx, my, mz, w ? mw : nullptr, orig);
   SIF_HIP(hipGetLastError());
+  if (n_big) {
+    mesh_order_kernel<IdxT, MESH_BIG_THREADS, mesh_big_cap<IdxT>()>
+      <<<n_big, MESH_BIG_THREADS, 0, stream()>>>(offsets, g_big.as<uint64_t>(), n_big,
+        g_src.as<IdxT>(), mx, my, mz, w ? mw : nullptr, orig);
+    SIF_HIP(hipGetLastError());
+  }
   return SIF_OK;
 }
 
@@ -330,12 +377,17 @@ int mesh_build(const float* x, const float* y, const float* z, const float* w,
   if (st == SIF_OK)
     st = g_cursor.reserve(cells * 4);
   if (st == SIF_OK)
-    st = g_max.reserve(4);
+    st = g_max.reserve(8);
+  // cells of more than MESH_ORDER_CAP particles: at most n / (cap + 1)
+  const uint64_t big_room = n / (MESH_ORDER_CAP + 1) + 1;
+  const uint32_t big_cap = (uint32_t)(big_room < cells ? big_room : cells);
+  if (st == SIF_OK)
+    st = g_big.reserve((uint64_t)big_cap * 8);
   if (st != SIF_OK)
     return st;
   timing_begin(SIF_AMD_STAGE_MESH);
   SIF_HIP(hipMemsetAsync(g_counts.ptr, 0, cells * 4, stream()));
-  SIF_HIP(hipMemsetAsync(g_max.ptr, 0, 4, stream()));
+  SIF_HIP(hipMemsetAsync(g_max.ptr, 0, 8, stream()));
   if (n) {
     mesh_count_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(x, y, z, n, nc, inv_l, box,
       g_counts.as<uint32_t>());
@@ -347,23 +399,24 @@ int mesh_build(const float* x, const float* y, const float* z, const float* w,
   if (n) {
     const unsigned mb = grid_for(cells, 256) > 1024 ? 1024 : grid_for(cells, 256);
     count_max_kernel<<<mb, 256, 0, stream()>>>(g_counts.as<uint32_t>(), cells,
-      g_max.as<uint32_t>());
+      g_max.as<uint32_t>(), g_big.as<uint64_t>(), big_cap);
     SIF_HIP(hipGetLastError());
-    uint32_t largest = 0;
-    SIF_HIP(hipMemcpyAsync(&largest, g_max.ptr, 4, hipMemcpyDeviceToHost, stream()));
+    uint32_t got[2] = {0, 0};  // the largest cell, the cells past MESH_ORDER_CAP
+    SIF_HIP(hipMemcpyAsync(got, g_max.ptr, 8, hipMemcpyDeviceToHost, stream()));
     SIF_HIP(hipStreamSynchronize(stream()));
-    if (largest <= MESH_ORDER_CAP) {
-      st = n < 0xffffffffull
-             ? mesh_scatter_order<uint32_t>(x, y, z, w, n, nc, inv_l, box, cells, offsets, mx,
-                 my, mz, mw, orig)
-             : mesh_scatter_order<uint64_t>(x, y, z, w, n, nc, inv_l, box, cells, offsets, mx,
-                 my, mz, mw, orig);
-    } else if (n < 0xffffffffull) {
+    const uint32_t largest = got[0], n_big = got[1];
+    const bool narrow = n < 0xffffffffull;
+    const uint32_t cap = narrow ? mesh_big_cap<uint32_t>() : mesh_big_cap<uint64_t>();
+    if (largest <= cap && n_big <= big_cap) {
+      st = narrow ? mesh_scatter_order<uint32_t>(x, y, z, w, n, nc, inv_l, box, cells, offsets,
+                      mx, my, mz, mw, orig, n_big)
+                  : mesh_scatter_order<uint64_t>(x, y, z, w, n, nc, inv_l, box, cells, offsets,
+                      mx, my, mz, mw, orig, n_big);
+    } else if (narrow) {
       st = mesh_by_radix(x, y, z, w, (uint32_t)n, nc, inv_l, box, cells, mx, my, mz, mw, orig);
     } else {
       error_set(SIF_ERR_RANGE, "chain mesh: a cell of %u particles among %llu (cells of more "
-        "than %u need fewer than 2^32 particles)", largest, (unsigned long long)n,
-        MESH_ORDER_CAP);
+        "than %u need fewer than 2^32 particles)", largest, (unsigned long long)n, cap);
       st = SIF_ERR_RANGE;
     }
     if (st != SIF_OK)

@@ -125,7 +125,7 @@ def test_density_contrast(dev, particles):
     assert abs(ref_mean - mean) <= 1e-12 * mean
 
 
-@pytest.mark.parametrize("mesh_cells", [1, 7, 10, 16])
+@pytest.mark.parametrize("mesh_cells", [1, 2, 3, 7, 10, 16])
 def test_chain_mesh_bit_exact(dev, particles, mesh_cells):
     import oracle
     D, dx, dy, dz = dev
@@ -139,13 +139,16 @@ def test_chain_mesh_bit_exact(dev, particles, mesh_cells):
     assert m.total_weight == len(x)
 
 
-def test_chain_mesh_weighted(dev, particles):
+@pytest.mark.parametrize("mesh_cells", [3, 10])
+def test_chain_mesh_weighted(dev, particles, mesh_cells):
     import oracle
     D, dx, dy, dz = dev
     x, y, z = particles
     w = (np.random.default_rng(5).integers(1, 8, len(x)) * 0.25).astype(np.float32)
-    m = D.Mesh(dx, dy, dz, len(x), 10, BOX, dw=D.to_device(w))
-    want = oracle.chain_mesh(x, y, z, 10, BOX, w=w)
+    m = D.Mesh(dx, dy, dz, len(x), mesh_cells, BOX, dw=D.to_device(w), indices=True)
+    want = oracle.chain_mesh(x, y, z, mesh_cells, BOX, w=w)
+    np.testing.assert_array_equal(m.orig.get(), want["orig"])
+    np.testing.assert_array_equal(m.x.get(), want["x"])
     np.testing.assert_array_equal(m.w.get(), want["w"])
     np.testing.assert_array_equal(m.cell_weights.get(), want["cell_weights"])
     assert m.total_weight == want["total_weight"]
