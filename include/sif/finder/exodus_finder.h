@@ -56,6 +56,36 @@ SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid,
   sif_real threshold, sif_real overlap_fraction, sif_option opt,
   const sif_finder_exodus_extras_t* extras);
 
+/* exodus_finder.h:305: sif_finder_exodus_survey_raw with a variant's
+ * settings (excursion-set or lagrangian-overlap). */
+SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus_survey(sif_grid_t* data_grid,
+  sif_grid_t* random_grid, const sif_chain_mesh_t* data_mesh,
+  const sif_chain_mesh_t* random_mesh, const sif_real offset[3], const sif_real* radii,
+  uint32_t n_radii, sif_real threshold, sif_exodus_variant_t variant,
+  const sif_finder_exodus_extras_t* extras);
+
+/* exodus_finder.h:402: the exodus finder on a survey whose geometry comes
+ * from a random catalogue. data_grid holds the data's CIC densities (NOT a
+ * density contrast) and is smoothed and restored like a box run's grid;
+ * random_grid the randoms' on the same cells (only read here); the meshes
+ * share the box, their cells need not match. Every tracer must sit the
+ * largest search sphere plus a grid cell (or half the sphere and a mesh cell,
+ * whichever is more) from every face (SIF_ERR_INVALID otherwise).
+ * SIF_FINDER_SEEDS_CELLS on their nodes only. The catalogue carries the
+ * footprint columns and is moved back by -offset (NULL: not moved). */
+SIF_API SIF_NODISCARD sif_catalogue_t* sif_finder_exodus_survey_raw(
+  sif_grid_t* data_grid, sif_grid_t* random_grid, const sif_chain_mesh_t* data_mesh,
+  const sif_chain_mesh_t* random_mesh, const sif_real offset[3], const sif_real* radii,
+  uint32_t n_radii, sif_real threshold, sif_real overlap_fraction, sif_option opt,
+  const sif_finder_exodus_extras_t* extras);
+
+/* exodus_finder.h:601: the cubic box a survey must be searched in (the
+ * padding sized for grid cells up to the smallest radius) and the offset
+ * that centres data and randoms in it. */
+SIF_API SIF_NODISCARD int sif_finder_exodus_survey_box(const sif_field_t* data,
+  const sif_field_t* randoms, const sif_real* radii, uint32_t n_radii, sif_option opt,
+  sif_real offset[3], sif_real* box_length);
+
 /* exodus_finder.h:409-531: the checkpoint a run writes after every rung. A
  * 192-byte header, then in host byte order the ladder largest first (n_radii
  * floats) and the voids as four columns of n_voids floats (every x, then every

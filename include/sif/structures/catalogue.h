@@ -35,6 +35,9 @@ SIF_API void sif_catalogue_free(sif_catalogue_t* catalogue);
 SIF_API int sif_catalogue_append(
   sif_catalogue_t* catalogue, sif_real x, sif_real y, sif_real z, sif_real r);
 SIF_API int sif_catalogue_trim(sif_catalogue_t* catalogue);
+/* catalogue.h:166: offset added to every centre; SIF_ERR_INVALID on a sky
+ * catalogue. */
+SIF_API int sif_catalogue_translate(sif_catalogue_t* catalogue, const sif_real offset[3]);
 /* catalogue.h:224-237: footprint columns for the catalogue, every void
  * already in it at SIF_CATALOGUE_FOOTPRINT_UNKNOWN (as is every void
  * appended later); nothing when it has them. */

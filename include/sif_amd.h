@@ -157,6 +157,57 @@ SIF_API int sif_amd_find_radius_weighted(const uint64_t* d_offsets, const float*
   uint32_t n, float rmin, float r_search, float threshold, float vol_factor,
   float mass_factor, float* d_radius, uint32_t* d_reason, uint64_t* d_n_in);
 
+/* --- survey (src/finder/exodus_survey.c) --- */
+
+/* One chain mesh on the device, as sif_amd_chain_mesh_build leaves it:
+ * offsets (n_cells^3 + 1), positions, and for a weighted mesh the tracer and
+ * cell weights (NULL on an unweighted one). */
+typedef struct {
+  const uint64_t* offsets;
+  const float* x;
+  const float* y;
+  const float* z;
+  const float* weights;
+  const float* cell_weights;
+  double total_weight;
+  uint32_t n_cells;
+  uint64_t n_particles;
+} sif_amd_mesh_t;
+
+/* sif__exodus_find_radius_survey (src/finder/exodus_rescale.c:2172-2317):
+ * the largest data distance d inside r_search with D(<d) <= K R(<d), D and R
+ * the data and random weight enclosed (R interpolated in d^2 within each
+ * histogram bin), K = (1 + threshold) alpha. Reasons as sif_amd_find_radius's,
+ * 6 for no randoms inside r_search; n_in (may be NULL) the data tracers
+ * enclosed. Both meshes share the box; their cells need not match. */
+SIF_API int sif_amd_find_radius_survey(const sif_amd_mesh_t* data,
+  const sif_amd_mesh_t* randoms, float box, const float* d_cx, const float* d_cy,
+  const float* d_cz, uint32_t n, float rmin, float r_search, float K, float* d_radius,
+  uint32_t* d_reason, uint64_t* d_n_in);
+
+/* sif__exodus_survey_occupancy (src/finder/exodus_survey.c:148-207): the
+ * nodes of an n_cells^3 grid over `box` that the n positions reach (node
+ * nearest each), closed over (2k + 1)^3 nodes, k in [1, 4] the smallest that
+ * expects 128 randoms at lambda randoms a node. d_occupied: n_cells^3 bytes,
+ * 0 or 1, flat (ix n + iy) n + iz. k (0 when nothing is marked), lambda,
+ * marked and observed node counts may each be NULL. */
+SIF_API int sif_amd_survey_occupancy(const float* d_x, const float* d_y, const float* d_z,
+  uint64_t n, uint32_t n_cells, float box, uint8_t* d_occupied, int32_t* k, double* lambda,
+  uint64_t* marked, uint64_t* observed);
+
+/* sif__exodus_survey_contrast (exodus_survey.c:223-234), in place:
+ * data / (alpha randoms) - 1 on observed nodes with randoms expected, 1e30
+ * elsewhere. */
+SIF_API int sif_amd_survey_contrast(float* d_data, const float* d_randoms,
+  const uint8_t* d_occupied, float alpha, uint64_t total_cells);
+
+/* sif__exodus_survey_footprint (exodus_survey.c:247-318): the observed
+ * fraction of each void's sphere and of its shell out to twice its radius.
+ * Void columns and results are host arrays. */
+SIF_API int sif_amd_survey_footprint(const uint8_t* d_occupied, uint32_t n_cells, float box,
+  const float* cx, const float* cy, const float* cz, const float* r, uint64_t n_voids,
+  float* inside, float* shell);
+
 /* sif__exodus_template_build (exodus_rescale.c:175-283), host only: the
  * per-rung stencil the growth kernel walks, as (dx, dy, dz, class) and
  * (min d^2, max d^2) per entry; *count is the total, entries past capacity
@@ -195,6 +246,20 @@ SIF_API sif_catalogue_t* sif_amd_exodus_device(float* d_delta,
   uint64_t n_particles, const sif_real* radii, uint32_t n_radii,
   sif_real threshold, sif_real overlap_fraction, sif_option options,
   int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats);
+
+/* sif_finder_exodus_survey_raw (src/finder/exodus_survey.c:386-535) on
+ * device-resident data: d_data the data's CIC density grid (NOT a density
+ * contrast; smoothed in place, restored when `restore`), d_randoms the
+ * randoms' on the same grid (read only), the two meshes over the same box
+ * (their cells need not match). The survey must sit at least the largest
+ * search sphere plus a cell from every face (SIF_ERR_INVALID otherwise, the
+ * padding needed in the message). Seeds on their nodes only. The catalogue
+ * carries the footprint columns; coordinates are the box's. */
+SIF_API sif_catalogue_t* sif_amd_exodus_survey_device(float* d_data,
+  const float* d_randoms, uint32_t n_cells, float box, const sif_amd_mesh_t* data,
+  const sif_amd_mesh_t* randoms, const sif_real* radii, uint32_t n_radii,
+  sif_real threshold, sif_real overlap_fraction, sif_option options, int restore,
+  uint64_t** n_in, sif_amd_rung_stats_t* stats);
 
 /* The same on weighted tracers (sif__exodus_find_radius_weighted,
  * exodus_rescale.c:1029-1037; the density tests against the mean weight

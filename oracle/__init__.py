@@ -420,3 +420,103 @@ def exodus_pipeline(x, y, z, n_cells, box, mesh_cells, radii, threshold, max_run
     lib.orc_catalogue_free(C.byref(cat))
     return out[0], out[1], stage
 
+
+
+# --- survey (src/finder/exodus_survey.c, exodus_rescale.c:1909-2317) -------------
+
+_sig("orc_survey_node", u32, f32, f32, u32)
+_sig("orc_survey_randoms_per_node", f64, f64)
+_sig("orc_survey_close", None, P, u32, C.c_int32)
+_sig("orc_survey_occupancy", i32, P, P, P, u64, u32, f32, P, C.POINTER(f64), C.POINTER(u64),
+     C.POINTER(u64))
+_sig("orc_survey_contrast", None, P, P, P, f32, u64)
+_sig("orc_survey_void_footprint", None, P, u32, f32, f32, f32, f32, f32, f32,
+     C.POINTER(f32), C.POINTER(f32))
+_sig("orc_find_radius_survey", f32, C.POINTER(Mesh), C.POINTER(Mesh), C.POINTER(Template),
+     C.POINTER(Template), f32, f32, f32, f32, f32, f32, u32, C.POINTER(C.c_uint8),
+     C.POINTER(u64))
+_sig("orc_exodus_run_survey", i32, C.POINTER(P), u32, f32, C.POINTER(Mesh), C.POINTER(Mesh),
+     f32, P, u32, f32, f32, i32, u32, i32, u32, C.POINTER(Catalogue), C.POINTER(RungStats))
+
+
+def survey_randoms_per_node(m):
+    return lib.orc_survey_randoms_per_node(float(m))
+
+
+def survey_close(occ, n, k):
+    """The (2k + 1)^3 closing of a 0/1 byte mask of n^3 nodes, in a copy."""
+    o = np.ascontiguousarray(occ, dtype=np.uint8).copy()
+    lib.orc_survey_close(_p(o), n, k)
+    return o
+
+
+def survey_occupancy(rx, ry, rz, n, box):
+    """(mask of n^3 bytes, k, lambda, marked, observed) for the randoms on a
+    grid of n cells over `box` (cell length box / n in float)."""
+    rx, ry, rz = f32a(rx), f32a(ry), f32a(rz)
+    occ = np.empty(n ** 3, np.uint8)
+    lam, marked, observed = C.c_double(), C.c_uint64(), C.c_uint64()
+    k = lib.orc_survey_occupancy(_p(rx), _p(ry), _p(rz), len(rx), n,
+                                 np.float32(box) / np.float32(n), _p(occ), C.byref(lam),
+                                 C.byref(marked), C.byref(observed))
+    return occ, k, lam.value, marked.value, observed.value
+
+
+def survey_contrast(data, randoms, occ, alpha):
+    d = np.ascontiguousarray(data, dtype=np.float32).copy()
+    r = f32a(randoms)
+    o = np.ascontiguousarray(occ, dtype=np.uint8)
+    lib.orc_survey_contrast(_p(d), _p(r), _p(o), alpha, d.size)
+    return d
+
+
+def survey_footprint(occ, n, box, voids):
+    """(inside, shell) observed fractions of voids (k, 4)."""
+    o = np.ascontiguousarray(occ, dtype=np.uint8)
+    v = np.asarray(voids, np.float32).reshape(-1, 4)
+    l = np.float32(box) / np.float32(n)
+    fin = np.empty(len(v), np.float32)
+    fsh = np.empty(len(v), np.float32)
+    a, b = C.c_float(), C.c_float()
+    for i, (x, y, z, r) in enumerate(v):
+        lib.orc_survey_void_footprint(_p(o), n, l, box, x, y, z, r, C.byref(a), C.byref(b))
+        fin[i], fsh[i] = a.value, b.value
+    return fin, fsh
+
+
+def find_radius_survey(dmesh, rmesh, centres, rmin, r_search, K, window=8):
+    """Survey radii of `centres` (reason, enclosed data tracers beside)."""
+    dt = OracleTemplate(dmesh.s.n_cells, dmesh.s.cell_length, rmin, r_search)
+    rt = OracleTemplate(rmesh.s.n_cells, rmesh.s.cell_length, rmin, r_search)
+    c = np.asarray(centres, np.float32).reshape(-1, 3)
+    out = np.empty(len(c), np.float32)
+    reason = np.empty(len(c), np.uint8)
+    nin = np.zeros(len(c), np.uint64)
+    for i, (x, y, z) in enumerate(c):
+        rs, ni = C.c_uint8(), C.c_uint64()  # n_in is written only when a radius is found
+        out[i] = lib.orc_find_radius_survey(C.byref(dmesh.s), C.byref(rmesh.s), C.byref(dt.t),
+                                            C.byref(rt.t), x, y, z, r_search, K, rmin, window,
+                                            C.byref(rs), C.byref(ni))
+        reason[i] = rs.value
+        nin[i] = ni.value
+    return out, reason, nin
+
+
+def exodus_run_survey(fields, n_cells, box, dmesh, rmesh, survey_K, radii_desc, threshold,
+                      overlap=0.0, lagrangian=False, search_opt=0, tie_rule=0, window=8):
+    """Sequential survey commit on given per-rung survey contrasts."""
+    fields = [np.ascontiguousarray(f, dtype=np.float32) for f in fields]
+    ptrs = (P * len(fields))(*[f.ctypes.data for f in fields])
+    radii = np.ascontiguousarray(radii_desc, dtype=np.float32)
+    cat = Catalogue()
+    stats = (RungStats * len(radii))()
+    st = lib.orc_exodus_run_survey(ptrs, n_cells, box, C.byref(dmesh.s), C.byref(rmesh.s),
+                                   survey_K, radii.ctypes.data, len(radii), threshold, overlap,
+                                   1 if lagrangian else 0, search_opt, tie_rule, window,
+                                   C.byref(cat), stats)
+    if st != 0:
+        lib.orc_catalogue_free(C.byref(cat))
+        raise ValueError(f"oracle survey exodus failed ({st})")
+    out = _catalogue(cat)
+    lib.orc_catalogue_free(C.byref(cat))
+    return out + ([dict((k, getattr(s, k)) for k, _ in RungStats._fields_) for s in stats],)

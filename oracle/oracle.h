@@ -143,6 +143,7 @@ int orc_profiles(const orc_mesh_t* m, const float* cx, const float* cy,
 #define ORC_RESCALE_BEYOND     2
 #define ORC_RESCALE_BELOW_RUNG 3
 #define ORC_RESCALE_DEGENERATE 4
+#define ORC_RESCALE_NO_RANDOMS 6
 /* Unweighted find_exact_radius_impl. window = bins resolved per pass. n_in
  * (optional) receives the enclosed count at the crossing. */
 float orc_find_radius(const orc_mesh_t* m, float cx, float cy, float cz,
@@ -224,6 +225,31 @@ int orc_exodus_pipeline(const float* x, const float* y, const float* z,
   uint64_t n, uint32_t n_cells, float box, uint32_t mesh_cells,
   const float* radii, uint32_t n_radii, float threshold, uint32_t max_rungs,
   orc_catalogue_t* cat, double* stage_s);
+
+/* --- survey: src/finder/exodus_survey.c, exodus_rescale.c:1909-2317 ---
+ * Occupancy masks are one byte a grid node (0/1), flat (ix n + iy) n + iz. */
+uint32_t orc_survey_node(float p, float inv_l, uint32_t n);
+double orc_survey_randoms_per_node(double m);
+/* The closing of occ over a (2k + 1)^3 box, nodes past a face unset. */
+void orc_survey_close(uint8_t* occ, uint32_t n, int32_t k);
+/* The footprint of the randoms, closed; returns k (0: nothing marked). */
+int orc_survey_occupancy(const float* rx, const float* ry, const float* rz, uint64_t nr,
+  uint32_t n, float cell_length, uint8_t* occ, double* lambda, uint64_t* marked,
+  uint64_t* observed);
+void orc_survey_contrast(float* data, const float* randoms, const uint8_t* occ, float alpha,
+  uint64_t total);
+void orc_survey_void_footprint(const uint8_t* occ, uint32_t n, float l, float box, float cx,
+  float cy, float cz, float r, float* inside, float* shell);
+/* The survey radius against the random mesh rm (its own template rtpl);
+ * n_in: the data tracers enclosed. */
+float orc_find_radius_survey(const orc_mesh_t* dm, const orc_mesh_t* rm,
+  const orc_template_t* dtpl, const orc_template_t* rtpl, float cx, float cy, float cz,
+  float r_search, float K, float rmin, uint32_t window, uint8_t* reason, uint64_t* n_in);
+int orc_exodus_run_survey(float* const* fields, uint32_t n, float box,
+  const orc_mesh_t* data_mesh, const orc_mesh_t* random_mesh, float survey_K,
+  const float* radii_desc, uint32_t n_radii, float threshold, float overlap_fraction,
+  int lagrangian, uint32_t search_opt, int tie_rule, uint32_t window, orc_catalogue_t* cat,
+  orc_rung_stats_t* stats);
 
 int orc_num_threads(void);
 

@@ -5,6 +5,7 @@
  * oracle.h for who may use it. Built with -ffp-contract=off.
  */
 #include "oracle.h"
+#include "oracle_rescale.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -202,54 +203,6 @@ void orc_template_free(orc_template_t* t) {
 
 /* --- rescaling: exodus_rescale.c:295-369, 391-507, 642-1012 --- */
 
-typedef struct {
-  const orc_mesh_t* m;
-  const orc_template_t* t;
-  float cx, cy, cz;
-  int32_t ix, iy, iz;
-} query_t;
-
-/* exodus_rescale.c:295-342 */
-static int query_cell(const query_t* q, uint32_t i, uint64_t* s, uint64_t* e,
-  float* ex, float* ey, float* ez) {
-  const int32_t N = (int32_t)q->m->n_cells;
-  const float L = q->m->box_length;
-  int32_t ix = q->ix + q->t->dx[i];
-  float cx = q->cx;
-  if (ix < 0) {
-    ix += N;
-    cx += L;
-  } else if (ix >= N) {
-    ix -= N;
-    cx -= L;
-  }
-  int32_t iy = q->iy + q->t->dy[i];
-  float cy = q->cy;
-  if (iy < 0) {
-    iy += N;
-    cy += L;
-  } else if (iy >= N) {
-    iy -= N;
-    cy -= L;
-  }
-  int32_t iz = q->iz + q->t->dz[i];
-  float cz = q->cz;
-  if (iz < 0) {
-    iz += N;
-    cz += L;
-  } else if (iz >= N) {
-    iz -= N;
-    cz -= L;
-  }
-  const uint64_t flat = (uint64_t)ix * N * N + (uint64_t)iy * N + iz;
-  *s = q->m->offsets[flat];
-  *e = q->m->offsets[flat + 1];
-  *ex = cx;
-  *ey = cy;
-  *ez = cz;
-  return *e > *s;
-}
-
 /* The mesh cell of template entry i, as query_cell() finds it. */
 static uint64_t orc_query_flat(const query_t* q, uint32_t i, int32_t N) {
   int32_t ix = q->ix + q->t->dx[i], iy = q->iy + q->t->dy[i],
@@ -260,44 +213,9 @@ static uint64_t orc_query_flat(const query_t* q, uint32_t i, int32_t N) {
   return (uint64_t)ix * N * N + (uint64_t)iy * N + iz;
 }
 
-/* exodus_rescale.c:349-355 */
-static inline float dist2(float px, float py, float pz, float cx, float cy,
-  float cz) {
-  const float dx = px - cx, dy = py - cy, dz = pz - cz;
-  return dx * dx + dy * dy + dz * dz;
-}
-
-/* exodus_rescale.c:360-369 */
-static inline uint32_t bin_of(float d2, float rc2, float inv_w, uint32_t nb) {
-  const int32_t b = (int32_t)((d2 - rc2) * inv_w);
-  if (b < 0)
-    return 0;
-  if ((uint32_t)b >= nb)
-    return nb - 1;
-  return (uint32_t)b;
-}
-
 static int cmp_float(const void* pa, const void* pb) {
   const float a = *(const float*)pa, b = *(const float*)pb;
   return (a > b) - (a < b);
-}
-
-typedef struct {
-  float d2, w;
-} pair_t;
-
-/* exodus_rescale.c:29 sorts the pairs on d2 alone, so the order of equal
- * distances is its quicksort's. Equal distances give one radius whichever
- * passes first (the walk's enclosed weight only falls through them); here
- * they are ordered by weight too, the order the device sort gives, which
- * fixes the reported enclosed count. */
-static int cmp_pair(const void* pa, const void* pb) {
-  const pair_t* a = pa;
-  const pair_t* b = pb;
-  if (a->d2 != b->d2)
-    return (a->d2 > b->d2) - (a->d2 < b->d2);
-  const float wa = a->w + 0.0f, wb = b->w + 0.0f; /* -0 as +0 */
-  return (wa > wb) - (wa < wb);
 }
 
 /* exodus_rescale.c:391-507. Returns 1 found, 0 exhausted. cur_w is read
@@ -1081,10 +999,12 @@ typedef struct {
  * began (exodus_finder.c:1969-2084) and committed in order against the
  * catalogue as it stands (2181-2329), with every rung-radius test repeated
  * at commit; the catalogue is the one a strictly sequential pass gives. */
-int orc_exodus_run(float* const* fields, uint32_t n, float box,
-  const orc_mesh_t* mesh, uint64_t n_particles, const float* radii,
-  uint32_t n_radii, float t, float f, int lagrangian, uint32_t search_opt,
-  int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
+/* rmesh set: a survey run (exodus_finder.c:1592, 2045-2049), each radius
+ * grown against the randoms by orc_find_radius_survey with K, no witness. */
+static int exodus_run_impl(float* const* fields, uint32_t n, float box,
+  const orc_mesh_t* mesh, const orc_mesh_t* rmesh, float survey_K, uint64_t n_particles,
+  const float* radii, uint32_t n_radii, float t, float f, int lagrangian,
+  uint32_t search_opt, int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
   orc_rung_stats_t* stats) {
   const uint64_t total = (uint64_t)n * n * n;
   const float l = box / (float)n;
@@ -1105,7 +1025,7 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   const float mass =
     weighted ? (4.0f / 3.0f) * PI_F * ((float)mesh->total_weight / box_volume)
              : vol;
-  if (weighted)
+  if (weighted || rmesh)
     use_witness = 0;
   const float scale = lagrangian ? (float)cbrt(1.0 + (double)t) : 1.0f;
   const float factors[4] = {1.5f, 1.25f, 1.75f, 2.0f};
@@ -1170,9 +1090,17 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
     float r_search = radius + fmaxf((sf - 1.0f) * radius, 3.0f * l);
     if (i > 0 && radii[i - 1] > r_search)
       r_search = radii[i - 1];
-    orc_template_t tpl;
+    orc_template_t tpl, rtpl;
+    memset(&rtpl, 0, sizeof(rtpl));
     if (orc_template_build(&tpl, mesh->n_cells, mesh->cell_length, rmin,
           r_search) != 0) {
+      status = -3;
+      break;
+    }
+    /* exodus_finder.c:1834-1840: the random mesh's own template */
+    if (rmesh && orc_template_build(&rtpl, rmesh->n_cells, rmesh->cell_length, rmin,
+                   r_search) != 0) {
+      orc_template_free(&tpl);
       status = -3;
       break;
     }
@@ -1298,7 +1226,10 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
         }
         uint8_t reason;
         uint64_t n_in = 0;
-        float r = weighted
+        float r = rmesh
+          ? orc_find_radius_survey(mesh, rmesh, &tpl, &rtpl, cx, cy, cz, r_search,
+              survey_K, rmin, window, &reason, &n_in)
+          : weighted
           ? orc_find_radius_weighted(mesh, cx, cy, cz, r_search, t, vol, mass,
               rmin, &tpl, window, &reason, &n_in)
           : orc_find_radius(mesh, cx, cy, cz, r_search, t, vol, rmin, &tpl,
@@ -1370,6 +1301,8 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
       }
     }
     orc_template_free(&tpl);
+    if (rmesh)
+      orc_template_free(&rtpl);
     if (stats)
       stats[i] = st;
   }
@@ -1384,4 +1317,31 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   free(ev);
   free(bidx);
   return status;
+}
+
+int orc_exodus_run(float* const* fields, uint32_t n, float box,
+  const orc_mesh_t* mesh, uint64_t n_particles, const float* radii,
+  uint32_t n_radii, float t, float f, int lagrangian, uint32_t search_opt,
+  int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
+  orc_rung_stats_t* stats) {
+  return exodus_run_impl(fields, n, box, mesh, NULL, 0.0f, n_particles, radii, n_radii, t,
+    f, lagrangian, search_opt, tie_rule, use_witness, window, cat, stats);
+}
+
+/* sif__exodus_run for a survey (exodus_finder.c:1585-2492 with a random
+ * mesh): fields[i] holds the survey contrast of rung i (orc_survey_contrast
+ * of the smoothed data and randoms); K = (1 + t) alpha, alpha the data's
+ * total weight over the randoms' (exodus_finder.c:1649-1653). Cells seeds on
+ * their nodes, no extended reach or displacement (exodus_finder.c:1114-1137).
+ * The footprint columns are orc_survey_void_footprint's, per void. */
+int orc_exodus_run_survey(float* const* fields, uint32_t n, float box,
+  const orc_mesh_t* data_mesh, const orc_mesh_t* random_mesh, float survey_K,
+  const float* radii_desc, uint32_t n_radii, float threshold, float overlap_fraction,
+  int lagrangian, uint32_t search_opt, int tie_rule, uint32_t window, orc_catalogue_t* cat,
+  orc_rung_stats_t* stats) {
+  if (search_opt & (ORC_SEEDS_MINIMA | ORC_EXTEND_REACH | ORC_DISPLACE))
+    return -2;
+  return exodus_run_impl(fields, n, box, data_mesh, random_mesh, survey_K, 0, radii_desc,
+    n_radii, threshold, overlap_fraction, lagrangian, search_opt, tie_rule, 0, window, cat,
+    stats);
 }

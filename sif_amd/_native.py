@@ -115,6 +115,23 @@ _sig("sif_amd_exodus_device_weighted", C.POINTER(Catalogue), P, u32, f32, P, P, 
      f64, u32, u64, C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
      C.POINTER(RungStats))
 _sig("sif_amd_exodus_fields", i32, P, u32, f32, C.POINTER(f32), u32, u32, P)
+
+
+class MeshDesc(C.Structure):  # sif_amd_mesh_t
+    _fields_ = [("offsets", P), ("x", P), ("y", P), ("z", P), ("weights", P),
+                ("cell_weights", P), ("total_weight", f64), ("n_cells", u32),
+                ("n_particles", u64)]
+
+
+_sig("sif_amd_survey_occupancy", i32, P, P, P, u64, u32, f32, P, C.POINTER(C.c_int32),
+     C.POINTER(f64), C.POINTER(u64), C.POINTER(u64))
+_sig("sif_amd_survey_contrast", i32, P, P, P, f32, u64)
+_sig("sif_amd_survey_footprint", i32, P, u32, f32, P, P, P, P, u64, P, P)
+_sig("sif_amd_find_radius_survey", i32, C.POINTER(MeshDesc), C.POINTER(MeshDesc), f32, P, P, P,
+     u32, f32, f32, f32, P, P, P)
+_sig("sif_amd_exodus_survey_device", C.POINTER(Catalogue), P, P, u32, f32, C.POINTER(MeshDesc),
+     C.POINTER(MeshDesc), C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
+     C.POINTER(RungStats))
 _sig("sif_amd_set_batch_window", i32, u32, u32)
 
 
@@ -212,20 +229,25 @@ class DeviceArray:
             pass
 
 
-def catalogue_to_numpy(cat_ptr):
-    """Copies a sif_catalogue_t* into an (n, 4) array [cx, cy, cz, r] and frees it."""
+def catalogue_to_numpy(cat_ptr, footprint=False):
+    """Copies a sif_catalogue_t* into an (n, 4) array [cx, cy, cz, r] and frees
+    it; with `footprint`, also its (n, 2) footprint columns (inside, shell)."""
     if not cat_ptr:
         raise error_for(lib.sif_error_status() or SIF_ERR_INVALID)
     c = cat_ptr.contents
     n = int(c.n_voids)
     out = np.zeros((n, 4), np.float32)
+    fp = np.zeros((n, 2), np.float32)
     if n:
         out[:, 0] = np.ctypeslib.as_array(c.cx, (n,))
         out[:, 1] = np.ctypeslib.as_array(c.cy, (n,))
         out[:, 2] = np.ctypeslib.as_array(c.cz, (n,))
         out[:, 3] = np.ctypeslib.as_array(c.radii, (n,))
+        if footprint and c.footprint:
+            fp[:, 0] = np.ctypeslib.as_array(C.cast(c.footprint, C.POINTER(f32)), (n,))
+            fp[:, 1] = np.ctypeslib.as_array(C.cast(c.footprint_shell, C.POINTER(f32)), (n,))
     lib.sif_catalogue_free(cat_ptr)
-    return out
+    return (out, fp) if footprint else out
 
 
 def timing_read():

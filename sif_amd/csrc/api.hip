@@ -865,6 +865,314 @@ struct Checkpointing {
   sif_finder_exodus_checkpoint_t header;
 };
 
+}  // extern "C"
+
+namespace {
+
+// What a survey run adds to the drop-in (exodus_survey.c:386-535).
+struct SurveyIn {
+  sif_grid_t* random_grid;
+  const sif_chain_mesh_t* random_mesh;
+  const sif_real* offset;  // NULL: (0, 0, 0)
+};
+
+// exodus_finder.c:1145-1191 (sif__exodus_check_extras): a prior catalogue
+// the finder can place -- Cartesian, radii positive and finite, centres
+// (moved by the offset) inside [0, box).
+int check_extras(const sif_finder_exodus_extras_t* extras, const sif_real* offset, float box) {
+  const sif_catalogue_t* prior = extras ? extras->prior : nullptr;
+  if (!prior)
+    return SIF_OK;
+  if (prior->units != SIF_COORDINATES_CARTESIAN) {
+    error_set(SIF_ERR_INVALID, "prior: centres on the sky (expected Cartesian coordinates)");
+    return SIF_ERR_INVALID;
+  }
+  for (uint64_t i = 0; i < prior->n_voids; i++) {
+    const sif_real r = prior->radii[i];
+    if (!(r > 0.0f) || !std::isfinite(r)) {
+      error_set(SIF_ERR_INVALID, "prior: void %llu: radius %g (expected positive and finite)",
+        (unsigned long long)i, (double)r);
+      return SIF_ERR_INVALID;
+    }
+    const sif_real c[3] = {prior->cx[i] + (offset ? offset[0] : 0.0f),
+      prior->cy[i] + (offset ? offset[1] : 0.0f), prior->cz[i] + (offset ? offset[2] : 0.0f)};
+    for (int k = 0; k < 3; k++)
+      if (!(c[k] >= 0.0f && c[k] < box)) {
+        error_set(SIF_ERR_INVALID, "prior: void %llu: centre (%g, %g, %g)%s outside the box "
+          "[0, %g)", (unsigned long long)i, (double)prior->cx[i], (double)prior->cy[i],
+          (double)prior->cz[i], offset ? ", moved by the offset," : "", (double)box);
+        return SIF_ERR_INVALID;
+      }
+  }
+  return SIF_OK;
+}
+
+// sif__exodus_run's drop-in half (exodus_finder.c:1435-1575, 2430-2492):
+// where the run starts (a checkpoint resumed, or the prior voids), the
+// upload of the host grid(s) and mesh(es), the run on the device, the
+// checkpoint after every rung, and the grid handed back. `sv` set: a survey.
+sif_catalogue_t* exodus_dropin(sif_grid_t* grid, const sif_chain_mesh_t* mesh,
+  const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real overlap_fraction,
+  sif_option options, const sif_finder_exodus_extras_t* extras, const SurveyIn* sv) {
+  const char* named = extras ? extras->checkpoint : nullptr;
+  if (named && (options & SIF_FINDER_NO_CHECKPOINT)) {
+    error_set(SIF_ERR_INVALID, "checkpoint: %s, with SIF_FINDER_NO_CHECKPOINT set", named);
+    return nullptr;
+  }
+  if (named && strlen(named) + 32 >= 4096) {
+    error_set(SIF_ERR_INVALID, "checkpoint: path of %zu characters (expected under %d)",
+      strlen(named), 4096 - 32);
+    return nullptr;
+  }
+  int st = need_init();
+  if (st != SIF_OK)
+    return nullptr;
+  const sif_real* offset = sv ? sv->offset : nullptr;
+
+  // exodus_finder.c:1435-1575: where the run starts. A checkpoint the caller
+  // named and that exists is resumed once its header describes this run;
+  // otherwise the run starts from the prior voids (moved into the box by the
+  // offset) and writes its first checkpoint now (only a named one is worth
+  // failing over). Checkpointed voids are in the box's frame.
+  std::vector<float> sorted;
+  sorted_ladder(radii, n_radii, sorted);
+  const sif_catalogue_t* prior = extras ? extras->prior : nullptr;
+  sif_catalogue_t* in_box = nullptr;  // the prior voids, moved by the offset
+  if (prior && offset) {
+    in_box = sif_catalogue_alloc(prior->n_voids ? prior->n_voids : 1);
+    if (!in_box) {
+      error_set(SIF_ERR_ALLOC, "prior: cannot allocate %llu voids",
+        (unsigned long long)prior->n_voids);
+      return nullptr;
+    }
+    for (uint64_t i = 0; i < prior->n_voids; i++)
+      sif_catalogue_append(in_box, prior->cx[i] + offset[0], prior->cy[i] + offset[1],
+        prior->cz[i] + offset[2], prior->radii[i]);
+  }
+  const sif_catalogue_t* start = in_box ? in_box : prior;
+  Checkpointing ck;
+  ck.keep = extras && extras->keep_checkpoint;
+  sif_catalogue_t* resumed = nullptr;
+  uint32_t first_rung = 0;
+  uint64_t mismatched0 = 0;
+  auto fail = [&]() -> sif_catalogue_t* {
+    sif_catalogue_free(resumed);
+    sif_catalogue_free(in_box);
+    return nullptr;
+  };
+  if (!(options & SIF_FINDER_NO_CHECKPOINT)) {
+    sif_finder_exodus_checkpoint_t& h = ck.header;
+    memset(&h, 0, sizeof(h));
+    h.is_survey = sv ? 1u : 0u;
+    h.is_weighted = mesh->weights != nullptr;
+    h.options = options & (SIF__FINDER_SEARCH_MASK | SIF_FINDER_KEEP_CIC_WINDOW |
+                            SIF__FINDER_SEEDS_MASK | SIF__FINDER_REFINE_MASK |
+                            SIF_FINDER_EXTEND_REACH | SIF_FINDER_DISPLACE |
+                            SIF_FINDER_ALIASING_WINDOW | SIF_FINDER_LAGRANGIAN_OVERLAP);
+    h.n_radii = n_radii;
+    h.grid_n_cells = grid->n_cells;
+    h.assignment = ((uint32_t)grid->assignment & 0xffu) | ((grid->interlaced ? 1u : 0u) << 8);
+    h.box_length = (double)grid->box_length;
+    h.threshold = (double)threshold;
+    h.overlap_fraction = (double)overlap_fraction;
+    h.n_particles = mesh->n_particles;
+    h.total_weight = mesh->total_weight;
+    if (sv) {  // exodus_finder.c:1478-1484
+      h.random_n_particles = sv->random_mesh->n_particles;
+      h.random_total_weight = sv->random_mesh->total_weight;
+      for (int k = 0; offset && k < 3; k++)
+        h.offset[k] = (double)offset[k];
+    }
+    if (prior && prior->n_voids > 0) {
+      h.n_prior = prior->n_voids;
+      h.prior_crc32 = catalogue_crc(prior->cx, prior->cy, prior->cz, prior->radii,
+        prior->n_voids);
+    }
+    if (named)
+      snprintf(ck.path, sizeof(ck.path), "%s", named);
+    else
+      checkpoint_name(ck.path, sizeof(ck.path));
+    if (named && access(named, F_OK) == 0) {
+      sif_finder_exodus_checkpoint_t got;
+      std::vector<float> got_radii, cols[4];
+      st = checkpoint_load(named, &got, &got_radii, cols);
+      if (st == SIF_OK)
+        st = checkpoint_match(named, h, sorted.data(), got, got_radii.data());
+      if (st == SIF_OK) {
+        resumed = sif_catalogue_alloc(got.n_voids ? got.n_voids : 1);
+        if (!resumed) {
+          error_set(SIF_ERR_ALLOC, "checkpoint: %s: cannot allocate its voids", named);
+          st = SIF_ERR_ALLOC;
+        }
+        for (uint64_t v = 0; st == SIF_OK && v < got.n_voids; v++)
+          st = sif_catalogue_append(resumed, cols[0][v], cols[1][v], cols[2][v], cols[3][v]);
+      }
+      if (st != SIF_OK)
+        return fail();
+      first_rung = got.next_rung;
+      mismatched0 = got.n_mismatched;
+      h.next_rung = got.next_rung;
+      h.n_mismatched = got.n_mismatched;
+      ck.active = ck.on_disk = ck.progress = true;
+    } else {
+      const uint64_t np = start ? start->n_voids : 0;
+      Columns v{{start ? start->cx : nullptr, start ? start->cy : nullptr,
+                  start ? start->cz : nullptr, start ? start->radii : nullptr}, np};
+      st = checkpoint_write(ck.path, &h, sorted.data(), v);
+      if (st == SIF_OK) {
+        ck.active = ck.on_disk = true;
+      } else if (named) {
+        return fail();
+      } else {
+        sif_error_clear();  // running without a checkpoint
+        st = SIF_OK;
+      }
+    }
+  }
+
+  const uint64_t cells = grid->total_cells;
+  const uint64_t mcells = mesh->total_cells;
+  const uint64_t n = mesh->n_particles;
+  Dev dv, doff, dmx, dmy, dmz, dmw, dcw;
+  Dev rv, roff, rmx, rmy, rmz, rmw, rcw;  // a survey's randoms
+  st = dv.upload(grid->values, cells * 4);
+  if (st == SIF_OK) st = doff.upload(mesh->cell_offsets, (mcells + 1) * 8);
+  if (st == SIF_OK) st = dmx.upload(mesh->x, n * 4);
+  if (st == SIF_OK) st = dmy.upload(mesh->y, n * 4);
+  if (st == SIF_OK) st = dmz.upload(mesh->z, n * 4);
+  if (st == SIF_OK && mesh->weights) st = dmw.upload(mesh->weights, n * 4);
+  if (st == SIF_OK && mesh->weights) st = dcw.upload(mesh->cell_weights, mcells * 4);
+  if (sv) {
+    const sif_chain_mesh_t* rm = sv->random_mesh;
+    const uint64_t rn = rm->n_particles, rcells = rm->total_cells;
+    if (st == SIF_OK) st = rv.upload(sv->random_grid->values, cells * 4);
+    if (st == SIF_OK) st = roff.upload(rm->cell_offsets, (rcells + 1) * 8);
+    if (st == SIF_OK) st = rmx.upload(rm->x, rn * 4);
+    if (st == SIF_OK) st = rmy.upload(rm->y, rn * 4);
+    if (st == SIF_OK) st = rmz.upload(rm->z, rn * 4);
+    if (st == SIF_OK && rm->weights) st = rmw.upload(rm->weights, rn * 4);
+    if (st == SIF_OK && rm->weights) st = rcw.upload(rm->cell_weights, rcells * 4);
+  }
+  RunSpec R;
+  R.d_delta = dv.as<float>();
+  R.n = grid->n_cells;
+  R.box = grid->box_length;
+  R.d_offsets = doff.as<uint64_t>();
+  R.d_mx = dmx.as<float>();
+  R.d_my = dmy.as<float>();
+  R.d_mz = dmz.as<float>();
+  if (mesh->weights) {
+    R.d_mw = dmw.as<float>();
+    R.d_cw = dcw.as<float>();
+  }
+  R.total_weight = mesh->weights ? mesh->total_weight : (double)n;
+  R.mesh_n = mesh->n_cells;
+  R.n_particles = n;
+  sorted_ladder(radii, n_radii, R.radii);
+  R.threshold = threshold;
+  R.f = overlap_fraction;
+  R.opt = options;
+  R.restore = !(options & SIF_FINDER_CONSUME_GRID);
+  R.prior = resumed ? resumed : start;
+  R.first_rung = first_rung;
+  R.mismatched0 = mismatched0;
+  if (st == SIF_OK && sv) {
+    const sif_chain_mesh_t* rm = sv->random_mesh;
+    sif_amd_mesh_t dm{doff.as<uint64_t>(), dmx.as<float>(), dmy.as<float>(), dmz.as<float>(),
+      mesh->weights ? dmw.as<float>() : nullptr, mesh->weights ? dcw.as<float>() : nullptr,
+      mesh->weights ? mesh->total_weight : (double)n, mesh->n_cells, n};
+    sif_amd_mesh_t rmd{roff.as<uint64_t>(), rmx.as<float>(), rmy.as<float>(), rmz.as<float>(),
+      rm->weights ? rmw.as<float>() : nullptr, rm->weights ? rcw.as<float>() : nullptr,
+      rm->weights ? rm->total_weight : (double)rm->n_particles, rm->n_cells,
+      rm->n_particles};
+    st = survey_prepare(R, dm, rmd, rv.as<float>(), radii, n_radii, options);
+  }
+  if (st != SIF_OK) {
+    if (ck.on_disk && !ck.progress)
+      unlink(ck.path);
+    return fail();
+  }
+  if (ck.active)
+    R.on_rung = [&](uint32_t done, const float* x, const float* y, const float* z,
+                  const float* r, uint64_t nv, uint64_t mism) {
+      // a failed write leaves the previous checkpoint standing; the next rung
+      // tries again
+      ck.header.next_rung = done;
+      ck.header.n_mismatched = mism;
+      if (checkpoint_write(ck.path, &ck.header, R.radii.data(), Columns{{x, y, z, r}, nv}) ==
+          SIF_OK)
+        ck.progress = true;
+      else
+        sif_error_clear();
+    };
+  sif_catalogue_t* cat = nullptr;
+  st = exodus_run(R, &cat, nullptr, nullptr);
+  sif_catalogue_free(resumed);
+  resumed = nullptr;
+  // exodus_finder.c:1560-1575: a finished run has no use for its checkpoint
+  // (unless the caller keeps it); a failed one keeps it once it holds a rung
+  if (ck.on_disk && !(st != SIF_OK && ck.progress) && !(st == SIF_OK && ck.keep))
+    unlink(ck.path);
+  // the grid comes back restored (or holding the last smoothing under
+  // SIF_FINDER_CONSUME_GRID), on failure too; a survey's random grid is only
+  // ever read
+  if (hipMemcpyAsync(grid->values, dv.p, cells * 4, hipMemcpyDeviceToHost, stream()) !=
+        hipSuccess ||
+      hipStreamSynchronize(stream()) != hipSuccess) {
+    if (st == SIF_OK)
+      error_set(SIF_ERR_INVALID, "exodus: the grid could not be copied back");
+    sif_catalogue_free(cat);
+    return fail();
+  }
+  if (st != SIF_OK) {
+    sif_catalogue_free(cat);
+    return fail();
+  }
+  sif_catalogue_trim(cat);
+  // exodus_survey.c:538-560: back to the caller's frame, the prior voids as
+  // the caller gave them (x + o - o need not be x)
+  if (offset) {
+    const sif_real back[3] = {-offset[0], -offset[1], -offset[2]};
+    sif_catalogue_translate(cat, back);
+    if (prior && prior->n_voids > 0) {
+      const size_t bytes = (size_t)prior->n_voids * sizeof(sif_real);
+      memcpy(cat->cx, prior->cx, bytes);
+      memcpy(cat->cy, prior->cy, bytes);
+      memcpy(cat->cz, prior->cz, bytes);
+    }
+  }
+  sif_catalogue_free(in_box);
+  return cat;
+}
+
+// exodus_survey.c:321-342 on a host mesh, and the padding check of 367-385
+int survey_check_margin_host(const sif_chain_mesh_t* m, const char* what, float margin,
+  const char* hint) {
+  float lo[3] = {m->box_length, m->box_length, m->box_length}, hi[3] = {0.0f, 0.0f, 0.0f};
+  for (uint64_t p = 0; p < m->n_particles; p++) {
+    lo[0] = fminf(lo[0], m->x[p]);
+    lo[1] = fminf(lo[1], m->y[p]);
+    lo[2] = fminf(lo[2], m->z[p]);
+    hi[0] = fmaxf(hi[0], m->x[p]);
+    hi[1] = fmaxf(hi[1], m->y[p]);
+    hi[2] = fmaxf(hi[2], m->z[p]);
+  }
+  const float box = m->box_length;
+  for (int k = 0; k < 3; k++)
+    if (lo[k] < margin || hi[k] > box - margin) {
+      error_set(SIF_ERR_INVALID,
+        "%s: span [%g, %g] on axis %d of a box of %g, %g from its face (needs %g of padding "
+        "on every side%s)", what, (double)lo[k], (double)hi[k], k, (double)box,
+        (double)fminf(lo[k], box - hi[k]), (double)margin, hint);
+      return SIF_ERR_INVALID;
+    }
+  return SIF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t* mesh,
   const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real overlap_fraction,
   sif_option options, const sif_finder_exodus_extras_t* extras) {
@@ -891,173 +1199,204 @@ sif_catalogue_t* sif_finder_exodus_raw(sif_grid_t* grid, const sif_chain_mesh_t*
     return nullptr;
   if ((st = check_options(options)) != SIF_OK)
     return nullptr;
-  const char* named = extras ? extras->checkpoint : nullptr;
-  if (named && (options & SIF_FINDER_NO_CHECKPOINT)) {
-    error_set(SIF_ERR_INVALID, "checkpoint: %s, with SIF_FINDER_NO_CHECKPOINT set", named);
+  if (check_extras(extras, nullptr, grid->box_length) != SIF_OK)
     return nullptr;
-  }
-  if (named && strlen(named) + 32 >= 4096) {
-    error_set(SIF_ERR_INVALID, "checkpoint: path of %zu characters (expected under %d)",
-      strlen(named), 4096 - 32);
-    return nullptr;
-  }
-  if ((st = need_init()) != SIF_OK)
-    return nullptr;
+  return exodus_dropin(grid, mesh, radii, n_radii, threshold, overlap_fraction, options,
+    extras, nullptr);
+}
 
-  // exodus_finder.c:1435-1575: where the run starts. A checkpoint the caller
-  // named and that exists is resumed once its header describes this run;
-  // otherwise the run starts from the prior voids and writes its first
-  // checkpoint now (only a named one is worth failing over).
-  std::vector<float> sorted;
-  sorted_ladder(radii, n_radii, sorted);
-  const sif_catalogue_t* prior = extras ? extras->prior : nullptr;
-  Checkpointing ck;
-  ck.keep = extras && extras->keep_checkpoint;
-  sif_catalogue_t* resumed = nullptr;
-  uint32_t first_rung = 0;
-  uint64_t mismatched0 = 0;
-  if (!(options & SIF_FINDER_NO_CHECKPOINT)) {
-    sif_finder_exodus_checkpoint_t& h = ck.header;
-    memset(&h, 0, sizeof(h));
-    h.is_weighted = mesh->weights != nullptr;
-    h.options = options & (SIF__FINDER_SEARCH_MASK | SIF_FINDER_KEEP_CIC_WINDOW |
-                            SIF__FINDER_SEEDS_MASK | SIF__FINDER_REFINE_MASK |
-                            SIF_FINDER_EXTEND_REACH | SIF_FINDER_DISPLACE |
-                            SIF_FINDER_ALIASING_WINDOW | SIF_FINDER_LAGRANGIAN_OVERLAP);
-    h.n_radii = n_radii;
-    h.grid_n_cells = grid->n_cells;
-    h.assignment = ((uint32_t)grid->assignment & 0xffu) | ((grid->interlaced ? 1u : 0u) << 8);
-    h.box_length = (double)grid->box_length;
-    h.threshold = (double)threshold;
-    h.overlap_fraction = (double)overlap_fraction;
-    h.n_particles = mesh->n_particles;
-    h.total_weight = mesh->total_weight;
-    if (prior && prior->n_voids > 0) {
-      h.n_prior = prior->n_voids;
-      h.prior_crc32 = catalogue_crc(prior->cx, prior->cy, prior->cz, prior->radii,
-        prior->n_voids);
-    }
-    if (named)
-      snprintf(ck.path, sizeof(ck.path), "%s", named);
-    else
-      checkpoint_name(ck.path, sizeof(ck.path));
-    if (named && access(named, F_OK) == 0) {
-      sif_finder_exodus_checkpoint_t got;
-      std::vector<float> got_radii, cols[4];
-      st = checkpoint_load(named, &got, &got_radii, cols);
-      if (st == SIF_OK)
-        st = checkpoint_match(named, h, sorted.data(), got, got_radii.data());
-      if (st == SIF_OK) {
-        resumed = sif_catalogue_alloc(got.n_voids ? got.n_voids : 1);
-        if (!resumed) {
-          error_set(SIF_ERR_ALLOC, "checkpoint: %s: cannot allocate its voids", named);
-          st = SIF_ERR_ALLOC;
-        }
-        for (uint64_t v = 0; st == SIF_OK && v < got.n_voids; v++)
-          st = sif_catalogue_append(resumed, cols[0][v], cols[1][v], cols[2][v], cols[3][v]);
-      }
-      if (st != SIF_OK) {
-        sif_catalogue_free(resumed);
-        return nullptr;
-      }
-      first_rung = got.next_rung;
-      mismatched0 = got.n_mismatched;
-      h.next_rung = got.next_rung;
-      h.n_mismatched = got.n_mismatched;
-      ck.active = ck.on_disk = ck.progress = true;
-    } else {
-      std::vector<float> px, py, pz, pr;
-      const uint64_t np = prior ? prior->n_voids : 0;
-      Columns v{{prior ? prior->cx : nullptr, prior ? prior->cy : nullptr,
-                  prior ? prior->cz : nullptr, prior ? prior->radii : nullptr}, np};
-      st = checkpoint_write(ck.path, &h, sorted.data(), v);
-      if (st == SIF_OK) {
-        ck.active = ck.on_disk = true;
-      } else if (named) {
-        return nullptr;
-      } else {
-        sif_error_clear();  // running without a checkpoint
-        st = SIF_OK;
-      }
-    }
+// exodus_survey.c:386-535, in the reference's order of checks
+sif_catalogue_t* sif_finder_exodus_survey_raw(sif_grid_t* data_grid, sif_grid_t* random_grid,
+  const sif_chain_mesh_t* data_mesh, const sif_chain_mesh_t* random_mesh,
+  const sif_real offset[3], const sif_real* radii, uint32_t n_radii, sif_real threshold,
+  sif_real overlap_fraction, sif_option options, const sif_finder_exodus_extras_t* extras) {
+  const char* missing = !data_grid                          ? "data grid: NULL"
+                        : !data_grid->values                ? "data grid: no values"
+                        : !random_grid                      ? "random grid: NULL"
+                        : !random_grid->values              ? "random grid: no values"
+                        : !data_mesh                        ? "data mesh: NULL"
+                        : !random_mesh                      ? "random mesh: NULL"
+                        : !data_mesh->x || data_mesh->n_particles == 0 ? "data mesh: no particles"
+                        : !random_mesh->x || random_mesh->n_particles == 0
+                          ? "random mesh: no particles"
+                          : nullptr;
+  if (missing) {
+    error_set(SIF_ERR_INVALID, "%s", missing);
+    return nullptr;
   }
+  if (check_args(radii, n_radii, threshold, overlap_fraction) != SIF_OK ||
+      survey_check_options(options) != SIF_OK)
+    return nullptr;
+  for (int k = 0; offset && k < 3; k++)
+    if (!std::isfinite(offset[k])) {
+      error_set(SIF_ERR_INVALID, "offset[%d]: %g, expected finite", k, (double)offset[k]);
+      return nullptr;
+    }
+  // the contrast is formed against the randoms: a grid already turned into
+  // one against the box mean is the wrong input
+  if (data_grid->content == SIF_GRID_DENSITY_CONTRAST ||
+      random_grid->content == SIF_GRID_DENSITY_CONTRAST) {
+    error_set(SIF_ERR_INVALID, "%s grid: holds a density contrast; a survey takes the CIC "
+      "densities as assign_cic leaves them",
+      data_grid->content == SIF_GRID_DENSITY_CONTRAST ? "data" : "random");
+    return nullptr;
+  }
+  if (data_grid->n_cells != random_grid->n_cells ||
+      data_grid->box_length != random_grid->box_length) {
+    error_set(SIF_ERR_INVALID, "data grid: %u cells over %g, random grid: %u over %g (must "
+      "match)", data_grid->n_cells, (double)data_grid->box_length, random_grid->n_cells,
+      (double)random_grid->box_length);
+    return nullptr;
+  }
+  if (data_mesh->box_length != data_grid->box_length ||
+      random_mesh->box_length != data_grid->box_length) {
+    error_set(SIF_ERR_INVALID, "meshes: boxes of %g (data) and %g (randoms), grids: %g (must "
+      "match)", (double)data_mesh->box_length, (double)random_mesh->box_length,
+      (double)data_grid->box_length);
+    return nullptr;
+  }
+  if ((data_mesh->weights && check_weights_host(data_mesh->weights, data_mesh->n_particles,
+                               data_mesh->cell_weights, data_mesh->total_weight) != SIF_OK) ||
+      (random_mesh->weights &&
+        check_weights_host(random_mesh->weights, random_mesh->n_particles,
+          random_mesh->cell_weights, random_mesh->total_weight) != SIF_OK))
+    return nullptr;
+  if (!(data_mesh->total_weight > 0.0) || !(random_mesh->total_weight > 0.0)) {
+    error_set(SIF_ERR_INVALID, "%s mesh: total weight is zero",
+      data_mesh->total_weight > 0.0 ? "random" : "data");
+    return nullptr;
+  }
+  const float cell = data_grid->box_length / (float)data_grid->n_cells;
+  float r_max = radii[0], r_min = radii[0];
+  for (uint32_t i = 1; i < n_radii; i++) {
+    r_max = radii[i] > r_max ? radii[i] : r_max;
+    r_min = radii[i] < r_min ? radii[i] : r_min;
+  }
+  const float sf = sif__finder_search_factor(options);
+  const float r_search = r_max + fmaxf((sf - 1.0f) * r_max, 3.0f * cell);
+  const float mesh_cell = fmaxf(data_mesh->box_length / (float)data_mesh->n_cells,
+    random_mesh->box_length / (float)random_mesh->n_cells);
+  const float margin = fmaxf(r_search, 0.5f * (r_search + mesh_cell)) + cell;
+  char hint[192] = ", as survey_box sizes it";
+  if (cell > r_min)
+    snprintf(hint, sizeof(hint),
+      "; survey_box allows for grid cells up to the smallest radius, %g, and these are %g",
+      (double)r_min, (double)cell);
+  if (survey_check_margin_host(random_mesh, "randoms", margin, hint) != SIF_OK ||
+      survey_check_margin_host(data_mesh, "data", margin, hint) != SIF_OK)
+    return nullptr;
+  if (check_extras(extras, offset, data_grid->box_length) != SIF_OK)
+    return nullptr;
+  SurveyIn sv{random_grid, random_mesh, offset};
+  return exodus_dropin(data_grid, data_mesh, radii, n_radii, threshold, overlap_fraction,
+    options, extras, &sv);
+}
 
-  const uint64_t cells = grid->total_cells;
-  const uint64_t mcells = mesh->total_cells;
-  const uint64_t n = mesh->n_particles;
-  Dev dv, doff, dmx, dmy, dmz, dmw, dcw;
-  st = dv.upload(grid->values, cells * 4);
-  if (st == SIF_OK) st = doff.upload(mesh->cell_offsets, (mcells + 1) * 8);
-  if (st == SIF_OK) st = dmx.upload(mesh->x, n * 4);
-  if (st == SIF_OK) st = dmy.upload(mesh->y, n * 4);
-  if (st == SIF_OK) st = dmz.upload(mesh->z, n * 4);
-  if (st == SIF_OK && mesh->weights) st = dmw.upload(mesh->weights, n * 4);
-  if (st == SIF_OK && mesh->weights) st = dcw.upload(mesh->cell_weights, mcells * 4);
-  if (st != SIF_OK) {
-    if (ck.on_disk && !ck.progress)
-      unlink(ck.path);
-    sif_catalogue_free(resumed);
+sif_catalogue_t* sif_finder_exodus_survey(sif_grid_t* data_grid, sif_grid_t* random_grid,
+  const sif_chain_mesh_t* data_mesh, const sif_chain_mesh_t* random_mesh,
+  const sif_real offset[3], const sif_real* radii, uint32_t n_radii, sif_real threshold,
+  sif_exodus_variant_t variant, const sif_finder_exodus_extras_t* extras) {
+  sif_real f;
+  sif_option opt;
+  if (sif_finder_exodus_preset(variant, &f, &opt) != SIF_OK) {
+    error_set(SIF_ERR_INVALID, "variant: %d", (int)variant);
     return nullptr;
   }
-  RunSpec R;
-  R.d_delta = dv.as<float>();
-  R.n = grid->n_cells;
-  R.box = grid->box_length;
-  R.d_offsets = doff.as<uint64_t>();
-  R.d_mx = dmx.as<float>();
-  R.d_my = dmy.as<float>();
-  R.d_mz = dmz.as<float>();
-  if (mesh->weights) {
-    R.d_mw = dmw.as<float>();
-    R.d_cw = dcw.as<float>();
+  return sif_finder_exodus_survey_raw(data_grid, random_grid, data_mesh, random_mesh, offset,
+    radii, n_radii, threshold, f, opt, extras);
+}
+
+// exodus_survey.c:602-703: a box just large enough for the survey, centred,
+// with the padding the finder asks for at grid cells up to the smallest radius.
+int sif_finder_exodus_survey_box(const sif_field_t* data, const sif_field_t* randoms,
+  const sif_real* radii, uint32_t n_radii, sif_option opt, sif_real offset[3],
+  sif_real* box_length) {
+  const char* names[2] = {"data", "randoms"};
+  const sif_field_t* given[2] = {data, randoms};
+  for (int i = 0; i < 2; i++) {
+    const sif_field_t* f = given[i];
+    if (!f || !f->x || !f->y || !f->z || f->n_particles == 0) {
+      error_set(SIF_ERR_INVALID, "%s: %s", names[i],
+        !f ? "NULL" : f->n_particles == 0 ? "no particles" : "no positions");
+      return SIF_ERR_INVALID;
+    }
+    if (f->units != SIF_COORDINATES_CARTESIAN) {
+      error_set(SIF_ERR_INVALID, "%s: positions on the sky (expected Cartesian coordinates)",
+        names[i]);
+      return SIF_ERR_INVALID;
+    }
   }
-  R.total_weight = mesh->weights ? mesh->total_weight : (double)n;
-  R.mesh_n = mesh->n_cells;
-  R.n_particles = n;
-  sorted_ladder(radii, n_radii, R.radii);
-  R.threshold = threshold;
-  R.f = overlap_fraction;
-  R.opt = options;
-  R.restore = !(options & SIF_FINDER_CONSUME_GRID);
-  R.prior = resumed ? resumed : prior;
-  R.first_rung = first_rung;
-  R.mismatched0 = mismatched0;
-  if (ck.active)
-    R.on_rung = [&](uint32_t done, const float* x, const float* y, const float* z,
-                  const float* r, uint64_t nv, uint64_t mism) {
-      // a failed write leaves the previous checkpoint standing; the next rung
-      // tries again
-      ck.header.next_rung = done;
-      ck.header.n_mismatched = mism;
-      if (checkpoint_write(ck.path, &ck.header, R.radii.data(), Columns{{x, y, z, r}, nv}) ==
-          SIF_OK)
-        ck.progress = true;
-      else
-        sif_error_clear();
-    };
-  sif_catalogue_t* cat = nullptr;
-  st = exodus_run(R, &cat, nullptr, nullptr);
-  sif_catalogue_free(resumed);
-  // exodus_finder.c:1560-1575: a finished run has no use for its checkpoint
-  // (unless the caller keeps it); a failed one keeps it once it holds a rung
-  if (ck.on_disk && !(st != SIF_OK && ck.progress) && !(st == SIF_OK && ck.keep))
-    unlink(ck.path);
-  // the grid comes back restored (or holding the last smoothing under
-  // SIF_FINDER_CONSUME_GRID), on failure too
-  if (hipMemcpyAsync(grid->values, dv.p, cells * 4, hipMemcpyDeviceToHost, stream()) !=
-        hipSuccess ||
-      hipStreamSynchronize(stream()) != hipSuccess) {
-    if (st == SIF_OK)
-      error_set(SIF_ERR_INVALID, "exodus: the grid could not be copied back");
-    sif_catalogue_free(cat);
-    return nullptr;
+  if (!radii || n_radii == 0 || !offset || !box_length) {
+    error_set(SIF_ERR_INVALID, "%s",
+      !radii || n_radii == 0 ? "radii: none given" : "offset, box_length: NULL");
+    return SIF_ERR_INVALID;
   }
-  if (st != SIF_OK) {
-    sif_catalogue_free(cat);
-    return nullptr;
+  for (uint32_t i = 0; i < n_radii; i++)
+    if (!(radii[i] > 0.0f) || !std::isfinite(radii[i])) {
+      error_set(SIF_ERR_INVALID, "radii[%u] = %g: not a positive radius", i,
+        (double)radii[i]);
+      return SIF_ERR_INVALID;
+    }
+  // exodus_survey.c:567-600: the bounding box, a coordinate that is not finite
+  // counted (fminf and fmaxf would pass over it)
+  auto extent = [&](const sif_field_t* f, const char* what, float lo[3], float hi[3]) {
+    uint64_t bad = 0;
+    for (uint64_t p = 0; p < f->n_particles; p++) {
+      const float v[3] = {f->x[p], f->y[p], f->z[p]};
+      bad += !(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]));
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(lo[k], v[k]);
+        hi[k] = fmaxf(hi[k], v[k]);
+      }
+    }
+    if (bad) {
+      error_set(SIF_ERR_INVALID, "%s: %llu of %llu with a coordinate that is not finite", what,
+        (unsigned long long)bad, (unsigned long long)f->n_particles);
+      return SIF_ERR_INVALID;
+    }
+    return SIF_OK;
+  };
+  float lo[3] = {randoms->x[0], randoms->y[0], randoms->z[0]};
+  float hi[3] = {lo[0], lo[1], lo[2]};
+  if (extent(randoms, "randoms", lo, hi) != SIF_OK)
+    return SIF_ERR_INVALID;
+  if (extent(data, "data", lo, hi) != SIF_OK)
+    return SIF_ERR_INVALID;
+  float ext = 0.0f, r_min = radii[0], r_max = radii[0];
+  for (int k = 0; k < 3; k++)
+    ext = fmaxf(ext, hi[k] - lo[k]);
+  for (uint32_t i = 1; i < n_radii; i++) {
+    r_min = fminf(r_min, radii[i]);
+    r_max = fmaxf(r_max, radii[i]);
   }
-  sif_catalogue_trim(cat);
-  return cat;
+  // exodus_survey.c:655-690: margin = r_search + cell at a cell of r_min,
+  // with 1% slack on each side
+  const float f = sif__finder_search_factor(opt);
+  const float cell = r_min;
+  const float margin = r_max + fmaxf((f - 1.0f) * r_max, 3.0f * cell) + cell;
+  const float L = ext + 2.0f * 1.01f * margin;
+  for (int k = 0; k < 3; k++)
+    offset[k] = 0.5f * L - 0.5f * (lo[k] + hi[k]);
+  *box_length = L;
+  return SIF_OK;
+}
+
+int sif_catalogue_translate(sif_catalogue_t* c, const sif_real offset[3]) {
+  if (!c || !offset) {
+    error_set(SIF_ERR_INVALID, "%s: NULL", !c ? "catalogue" : "offset");
+    return SIF_ERR_INVALID;
+  }
+  if (c->units != SIF_COORDINATES_CARTESIAN) {
+    error_set(SIF_ERR_INVALID, "catalogue: sky coordinates (ra, dec, z), not positions; a "
+      "translation would move angles");
+    return SIF_ERR_INVALID;
+  }
+  for (uint64_t i = 0; i < c->n_voids; i++) {
+    c->cx[i] += offset[0];
+    c->cy[i] += offset[1];
+    c->cz[i] += offset[2];
+  }
+  return SIF_OK;
 }
 
 uint32_t sif_finder_grid_cells_from_size(uint64_t n_particles, sif_real box_length,

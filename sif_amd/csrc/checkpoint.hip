@@ -302,6 +302,18 @@ int checkpoint_match(const char* path, const sif_finder_exodus_checkpoint_t& wan
   if (got.n_particles != want.n_particles || !weight_ok)
     return mismatch("%" PRIu64 " tracers of total weight %.17g (expected %" PRIu64 " of %.17g)",
       got.n_particles, got.total_weight, want.n_particles, want.total_weight);
+  if (want.is_survey) {  // exodus_checkpoint.c:417-430
+    const bool rw_ok = std::fabs(got.random_total_weight - want.random_total_weight) <=
+                       1e-9 * std::fmax(std::fabs(got.random_total_weight),
+                                std::fabs(want.random_total_weight));
+    if (got.random_n_particles != want.random_n_particles || !rw_ok)
+      return mismatch("%" PRIu64 " randoms of total weight %.17g (expected %" PRIu64
+        " of %.17g)", got.random_n_particles, got.random_total_weight,
+        want.random_n_particles, want.random_total_weight);
+    for (int k = 0; k < 3; k++)
+      if (got.offset[k] != want.offset[k])
+        return mismatch("offset[%d] %g (expected %g)", k, got.offset[k], want.offset[k]);
+  }
   if (got.n_prior != want.n_prior || got.prior_crc32 != want.prior_crc32)
     return mismatch("started from %" PRIu64 " prior voids, checksum %08x (this run's: %" PRIu64
       ", %08x)", got.n_prior, got.prior_crc32, want.n_prior, want.prior_crc32);

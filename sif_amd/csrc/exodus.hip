@@ -71,6 +71,7 @@ struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
   DeviceBuffer pf_counts;  // per rung: masked, proxy, overlap
   DeviceBuffer field2;     // the second smoothed field (rungs alternate)
+  DeviceBuffer rspec, rfield;  // a survey's randoms: spectrum, smoothed grid
 };
 thread_local DeviceState g_dev;
 
@@ -408,7 +409,26 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   }
   const uint32_t n_ext = (uint32_t)ext_reach.size();
 
+  // a survey (exodus_finder.c:1649-1653, 1729-1737): the randoms smoothed
+  // beside the data each rung, the scanned field their contrast
+  const bool survey = R.d_random != nullptr;
+  const float survey_K = (1.0f + R.threshold) * R.alpha;
+  SurveyMesh data_mesh;
+  data_mesh.offsets = R.d_offsets;
+  data_mesh.x = R.d_mx;
+  data_mesh.y = R.d_my;
+  data_mesh.z = R.d_mz;
+  data_mesh.w = R.d_mw;
+  data_mesh.cw = R.d_cw;
+  data_mesh.mesh_n = R.mesh_n;
   int st = exodus_prepare_spectrum(R.d_delta, n);
+  if (st == SIF_OK && survey) {
+    st = g_dev.rspec.reserve((uint64_t)n * spectrum_plane(n) * sizeof(float2));
+    if (st == SIF_OK)
+      st = g_dev.rfield.reserve(total * sizeof(float));
+    if (st == SIF_OK)
+      st = fft_r2c(R.d_random, n, g_dev.rspec.as<float2>(), g_dev.work.as<float2>());
+  }
   const double t_lut = now_ms();
   if (st == SIF_OK)
     st = exodus_prepare_luts(n, R.box, R.radii, R.opt);
@@ -418,6 +438,19 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     return st;
   const uint64_t lut_size = filter_lut_size(n);
   const bool windowed = window_order_of(R.opt) > 0;
+  // rung j's scanned field into out, on stream s: the smoothed contrast, or
+  // for a survey the smoothed data over the smoothed randoms
+  auto smooth = [&](size_t j, float* out, hipStream_t s) -> int {
+    const float* lut = g_dev.luts.as<float>() + j * lut_size;
+    const double* axis = windowed ? g_dev.axis.as<double>() : nullptr;
+    int e = fft_c2r_lut(g_dev.spec.as<float2>(), n, lut, axis, out, g_dev.work.as<float2>(), s);
+    if (e == SIF_OK && survey)
+      e = fft_c2r_lut(g_dev.rspec.as<float2>(), n, lut, axis, g_dev.rfield.as<float>(),
+        g_dev.work.as<float2>(), s);
+    if (e == SIF_OK && survey)
+      e = survey_contrast(out, g_dev.rfield.as<float>(), R.d_occupied, R.alpha, total, s);
+    return e;
+  };
 
   // The mask of accepted-void nodes lives on the device: painted there, at
   // the end of each rung, with that rung's voids. Within a rung the host
@@ -506,8 +539,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // committed (the first rung run on the library stream).
     float* const field = fields[i & 1];
     if (i == R.first_rung)
-      st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>() + i * lut_size,
-        windowed ? g_dev.axis.as<double>() : nullptr, field, g_dev.work.as<float2>(), stream());
+      st = smooth(i, field, stream());
     else
       SIF_HIP(hipStreamWaitEvent(stream(), ev_smoothed[i & 1], 0));
     if (st != SIF_OK)
@@ -555,9 +587,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       // and every earlier use of the work buffer are behind it
       const size_t j = i + 1;
       SIF_HIP(hipStreamWaitEvent(aux_stream(), ev_scanned[i & 1], 0));
-      st = fft_c2r_lut(g_dev.spec.as<float2>(), n, g_dev.luts.as<float>() + j * lut_size,
-        windowed ? g_dev.axis.as<double>() : nullptr, fields[j & 1], g_dev.work.as<float2>(),
-        aux_stream());
+      st = smooth(j, fields[j & 1], aux_stream());
       if (st != SIF_OK)
         break;
       SIF_HIP(hipEventRecord(ev_smoothed[j & 1], aux_stream()));
@@ -904,11 +934,15 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // growth and the grown-radius overlap state, results to pinned memory
     auto launch = [&](BatchSlot& B) -> int {
       const uint32_t bc = B.count;
-      const bool algo = timing_active();
+      const bool algo = timing_active() && !survey;
       SIF_HIP(hipMemcpyAsync(B.d_flat.ptr, B.h_flat, bc * 4, hipMemcpyHostToDevice, stream()));
       int e = node_centres(B.d_flat.as<uint32_t>(), bc, n, l, B.d_cx.as<float>(),
         B.d_cy.as<float>(), B.d_cz.as<float>());
-      if (e == SIF_OK)
+      if (e == SIF_OK && survey)  // exodus_finder.c:2045-2049
+        e = grow_survey(data_mesh, R.rmesh, R.box, B.d_cx.as<float>(), B.d_cy.as<float>(),
+          B.d_cz.as<float>(), bc, rmin, r_search, survey_K, B.d_rad.as<float>(),
+          B.d_reason.as<uint32_t>(), B.d_nin.as<uint64_t>());
+      else if (e == SIF_OK)
         e = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
           B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, rmin, r_search,
           R.threshold, vol, B.d_rad.as<float>(), B.d_reason.as<uint32_t>(),
@@ -1224,6 +1258,17 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   }
   for (uint64_t v = 0; v < nv; v++)
     sif_catalogue_append(cat, V.cx[v], V.cy[v], V.cz[v], V.r[v]);
+  // exodus_finder.c:2477-2483: how much of each void the survey observed
+  if (survey) {
+    st = sif_catalogue_reserve_footprint(cat);
+    if (st == SIF_OK)
+      st = survey_footprint(R.d_occupied, n, R.box, cat->cx, cat->cy, cat->cz, cat->radii, nv,
+        cat->footprint, cat->footprint_shell);
+    if (st != SIF_OK) {
+      sif_catalogue_free(cat);
+      return st;
+    }
+  }
   if (n_in_out) {
     *n_in_out = (uint64_t*)malloc(sizeof(uint64_t) * (nv ? nv : 1));
     if (*n_in_out)
@@ -1374,6 +1419,117 @@ int check_weights_device(const float* d_w, uint64_t n, const float* d_cell_w, do
   return weights_verdict(counts[0], counts[1], n, total);
 }
 
+// exodus_finder.c:1114-1137: a survey runs SIF_FINDER_SEEDS_CELLS on their
+// nodes with the rules of 0.4, for now; the rest is refused as invalid.
+int survey_check_options(sif_option opt) {
+  const sif_option refine = opt & SIF__FINDER_REFINE_MASK;
+  const char* box_only = (opt & SIF_FINDER_SEEDS_MINIMA)          ? "SIF_FINDER_SEEDS_MINIMA"
+                         : refine == SIF_FINDER_REFINE_GAUSSIAN   ? "SIF_FINDER_REFINE_GAUSSIAN"
+                         : refine == SIF_FINDER_REFINE_TRACERS    ? "SIF_FINDER_REFINE_TRACERS"
+                         : refine                                 ? "SIF_FINDER_REFINE_HILL"
+                         : (opt & SIF_FINDER_EXTEND_REACH)        ? "SIF_FINDER_EXTEND_REACH"
+                         : (opt & SIF_FINDER_DISPLACE)            ? "SIF_FINDER_DISPLACE"
+                                                                  : nullptr;
+  if (box_only) {
+    error_set(SIF_ERR_INVALID,
+      "options: %s: a survey (expected SIF_FINDER_SEEDS_CELLS on their nodes, for now)",
+      box_only);
+    return SIF_ERR_INVALID;
+  }
+  return check_options(opt);
+}
+
+namespace {
+
+thread_local DeviceBuffer g_occupied;
+
+// exodus_survey.c:367-385: every tracer at least `margin` from every face
+// (a survey too close to its faces is invalid input, exodus_survey.c:520-523).
+int survey_check_margin(const float* x, const float* y, const float* z, uint64_t n, float box,
+  const char* what, float margin, const char* hint) {
+  float lo[3], hi[3];
+  int st = mesh_extent(x, y, z, n, lo, hi);
+  if (st != SIF_OK)
+    return st;
+  for (int k = 0; k < 3; k++)
+    if (lo[k] < margin || hi[k] > box - margin) {
+      error_set(SIF_ERR_INVALID,
+        "%s: span [%g, %g] on axis %d of a box of %g, %g from its face (needs %g of padding "
+        "on every side%s)", what, (double)lo[k], (double)hi[k], k, (double)box,
+        (double)fminf(lo[k], box - hi[k]), (double)margin, hint);
+      return SIF_ERR_INVALID;
+    }
+  return SIF_OK;
+}
+
+}  // namespace
+
+// exodus_survey.c:417-535 (sif_finder_exodus_survey_raw, from the checks on
+// the meshes on): the meshes' weights and totals, the padding, and the
+// survey's own inputs to R -- the observed nodes, alpha, the randoms.
+int survey_prepare(RunSpec& R, const sif_amd_mesh_t& data, const sif_amd_mesh_t& randoms,
+  const float* d_randoms, const sif_real* radii, uint32_t n_radii, sif_option opt) {
+  int st = SIF_OK;
+  if (data.weights)
+    st = check_weights_device(data.weights, data.n_particles, data.cell_weights,
+      data.total_weight);
+  if (st == SIF_OK && randoms.weights)
+    st = check_weights_device(randoms.weights, randoms.n_particles, randoms.cell_weights,
+      randoms.total_weight);
+  if (st != SIF_OK)
+    return st;
+  const double wd = data.weights ? data.total_weight : (double)data.n_particles;
+  const double wr = randoms.weights ? randoms.total_weight : (double)randoms.n_particles;
+  if (!(wd > 0.0) || !(wr > 0.0)) {
+    error_set(SIF_ERR_INVALID, "%s mesh: total weight is zero", wd > 0.0 ? "random" : "data");
+    return SIF_ERR_INVALID;
+  }
+  // exodus_survey.c:511-535: r_search + cell (or more), the hint naming the
+  // likeliest reason it falls short
+  const float cell = R.box / (float)R.n;
+  float r_max = radii[0], r_min = radii[0];
+  for (uint32_t i = 1; i < n_radii; i++) {
+    r_max = radii[i] > r_max ? radii[i] : r_max;
+    r_min = radii[i] < r_min ? radii[i] : r_min;
+  }
+  const float sf = sif__finder_search_factor(opt);
+  const float r_search = r_max + fmaxf((sf - 1.0f) * r_max, 3.0f * cell);
+  const float mesh_cell =
+    fmaxf(R.box / (float)data.n_cells, R.box / (float)randoms.n_cells);
+  const float margin = fmaxf(r_search, 0.5f * (r_search + mesh_cell)) + cell;
+  char hint[192] = ", as survey_box sizes it";
+  if (cell > r_min)
+    snprintf(hint, sizeof(hint),
+      "; survey_box allows for grid cells up to the smallest radius, %g, and these are %g",
+      (double)r_min, (double)cell);
+  st = survey_check_margin(randoms.x, randoms.y, randoms.z, randoms.n_particles, R.box,
+    "randoms", margin, hint);
+  if (st == SIF_OK)
+    st = survey_check_margin(data.x, data.y, data.z, data.n_particles, R.box, "data", margin,
+      hint);
+  if (st != SIF_OK)
+    return st;
+  const uint64_t total = (uint64_t)R.n * R.n * R.n;
+  if ((st = g_occupied.reserve(total)) != SIF_OK)
+    return st;
+  int32_t k = 0;
+  st = survey_occupancy(randoms.x, randoms.y, randoms.z, randoms.n_particles, R.n, R.box,
+    g_occupied.as<uint8_t>(), &k, nullptr, nullptr, nullptr);
+  if (st != SIF_OK)
+    return st;
+  R.d_random = d_randoms;
+  R.rmesh.offsets = randoms.offsets;
+  R.rmesh.x = randoms.x;
+  R.rmesh.y = randoms.y;
+  R.rmesh.z = randoms.z;
+  R.rmesh.w = randoms.weights;
+  R.rmesh.cw = randoms.cell_weights;
+  R.rmesh.mesh_n = randoms.n_cells;
+  R.d_occupied = g_occupied.as<uint8_t>();
+  R.alpha = (float)(wd / wr);  // exodus_finder.c:1651-1652
+  return SIF_OK;
+}
+
 }  // namespace sif_amd
 
 using namespace sif_amd;
@@ -1427,6 +1583,59 @@ sif_catalogue_t* exodus_device(float* d_delta, uint32_t n_cells, float box,
 }  // namespace
 
 extern "C" {
+
+sif_catalogue_t* sif_amd_exodus_survey_device(float* d_data, const float* d_randoms,
+  uint32_t n_cells, float box, const sif_amd_mesh_t* data, const sif_amd_mesh_t* randoms,
+  const sif_real* radii, uint32_t n_radii, sif_real threshold, sif_real overlap_fraction,
+  sif_option options, int restore, uint64_t** n_in, sif_amd_rung_stats_t* stats) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return nullptr;
+  const char* missing = !d_data ? "data grid: NULL"
+                        : !d_randoms ? "random grid: NULL"
+                        : !data ? "data mesh: NULL"
+                        : !randoms ? "random mesh: NULL"
+                        : !data->x || data->n_particles == 0 ? "data mesh: no particles"
+                        : !randoms->x || randoms->n_particles == 0 ? "random mesh: no particles"
+                                                                   : nullptr;
+  if (missing) {
+    error_set(SIF_ERR_INVALID, "%s", missing);
+    return nullptr;
+  }
+  if (n_cells == 0 || !(box > 0.0f) || !data->offsets || !randoms->offsets ||
+      data->n_cells == 0 || randoms->n_cells == 0 || !data->y || !data->z || !randoms->y ||
+      !randoms->z) {
+    error_set(SIF_ERR_INVALID, "sif_amd_exodus_survey_device: invalid arguments");
+    return nullptr;
+  }
+  if ((st = check_args(radii, n_radii, threshold, overlap_fraction)) != SIF_OK ||
+      (st = survey_check_options(options)) != SIF_OK)
+    return nullptr;
+  RunSpec R;
+  R.d_delta = d_data;
+  R.n = n_cells;
+  R.box = box;
+  R.d_offsets = data->offsets;
+  R.d_mx = data->x;
+  R.d_my = data->y;
+  R.d_mz = data->z;
+  R.d_mw = data->weights;
+  R.d_cw = data->weights ? data->cell_weights : nullptr;
+  R.total_weight = data->weights ? data->total_weight : (double)data->n_particles;
+  R.mesh_n = data->n_cells;
+  R.n_particles = data->n_particles;
+  sorted_ladder(radii, n_radii, R.radii);
+  R.threshold = threshold;
+  R.f = overlap_fraction;
+  R.opt = options;
+  R.restore = restore != 0;
+  R.prior = nullptr;
+  if ((st = survey_prepare(R, *data, *randoms, d_randoms, radii, n_radii, options)) != SIF_OK)
+    return nullptr;
+  sif_catalogue_t* cat = nullptr;
+  st = exodus_run(R, &cat, n_in, stats);
+  return st == SIF_OK ? cat : nullptr;
+}
 
 sif_catalogue_t* sif_amd_exodus_device_weighted(float* d_delta, uint32_t n_cells, float box,
   const uint64_t* d_offsets, const float* d_mx, const float* d_my, const float* d_mz,

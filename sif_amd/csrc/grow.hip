@@ -31,6 +31,7 @@
 #include "primitives.h"
 #include "sif_amd.h"
 #include "grow_device.h"
+#include "grow_survey.h"
 #include "overlap.h"
 #include "paint.h"
 
@@ -466,6 +467,114 @@ int grow_batch(const uint64_t* offsets, const float* mx, const float* my,
   return SIF_OK;
 }
 
+// --- the survey rescaling (grow_survey.h) ---
+
+__global__ void u32_max_kernel(const uint32_t* __restrict__ v, uint32_t n,
+  uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = v[i] > m ? v[i] : m;
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m)
+    atomicMax(out, m);
+}
+
+namespace {
+thread_local DeviceBuffer g_survey_nb, g_survey_max;
+
+GrowParams survey_params(const SurveyMesh& M, float box, const TemplateCache* t) {
+  GrowParams P{};
+  P.offsets = M.offsets;
+  P.mx = M.x;
+  P.my = M.y;
+  P.mz = M.z;
+  P.mesh_n = (int32_t)M.mesh_n;
+  P.box = box;
+  P.inv_l = 1.0f / (box / (float)M.mesh_n);
+  P.tcell = t->cells.as<int4>();
+  P.tbound = t->bounds.as<float2>();
+  P.tcount = t->count;
+  P.mw = M.w;
+  P.cw = M.w ? M.cw : nullptr;
+  P.slices = 1;
+  return P;
+}
+
+template <bool DW, bool RW>
+void survey_launch(const GrowParams& PD, const GrowParams& PR, const SurveyParams& Q,
+  bool second, size_t lds) {
+  if (!second)
+    survey_totals_kernel<DW, RW><<<Q.n_cand, 64, 0, stream()>>>(PD, PR, Q);
+  else
+    survey_grow_kernel<DW, RW><<<Q.n_cand, 64, lds, stream()>>>(PD, PR, Q);
+}
+}  // namespace
+
+int grow_survey(const SurveyMesh& D, const SurveyMesh& Rm, float box, const float* cx,
+  const float* cy, const float* cz, uint32_t n, float rmin, float r_search, float K,
+  float* radius, uint32_t* reason, uint64_t* n_in) {
+  const TemplateCache *td = nullptr, *tr = nullptr;
+  int st = template_device(D.mesh_n, box / (float)D.mesh_n, rmin, r_search, &td);
+  if (st == SIF_OK)
+    st = template_device(Rm.mesh_n, box / (float)Rm.mesh_n, rmin, r_search, &tr);
+  // the second build may have evicted the first from the cache: found again
+  // (or rebuilt, evicting an older one than the random mesh's)
+  if (st == SIF_OK)
+    st = template_device(D.mesh_n, box / (float)D.mesh_n, rmin, r_search, &td);
+  if (st != SIF_OK)
+    return st;
+  if (n == 0)
+    return SIF_OK;
+  if ((st = g_survey_nb.reserve((size_t)n * 4)) != SIF_OK ||
+      (st = g_survey_max.reserve(4)) != SIF_OK)
+    return st;
+  const GrowParams PD = survey_params(D, box, td);
+  const GrowParams PR = survey_params(Rm, box, tr);
+  SurveyParams Q{};
+  Q.cx = cx;
+  Q.cy = cy;
+  Q.cz = cz;
+  Q.n_cand = n;
+  Q.rmin = rmin;
+  Q.r_search = r_search;
+  Q.K = K;
+  Q.radius = radius;
+  Q.reason = reason;
+  Q.n_in = n_in;
+  Q.nb = g_survey_nb.as<uint32_t>();
+  Q.nb_max = 0;
+  auto launch = [&](bool second, size_t lds) {
+    if (D.w && Rm.w)
+      survey_launch<true, true>(PD, PR, Q, second, lds);
+    else if (D.w)
+      survey_launch<true, false>(PD, PR, Q, second, lds);
+    else if (Rm.w)
+      survey_launch<false, true>(PD, PR, Q, second, lds);
+    else
+      survey_launch<false, false>(PD, PR, Q, second, lds);
+  };
+  timing_begin(SIF_AMD_STAGE_GROW);
+  launch(false, 0);
+  SIF_HIP(hipGetLastError());
+  SIF_HIP(hipMemsetAsync(g_survey_max.ptr, 0, 4, stream()));
+  u32_max_kernel<<<grid_for(n, 256) > 256 ? 256 : grid_for(n, 256), 256, 0, stream()>>>(
+    Q.nb, n, g_survey_max.as<uint32_t>());
+  SIF_HIP(hipGetLastError());
+  uint32_t nb_max = 0;
+  SIF_HIP(hipMemcpyAsync(&nb_max, g_survey_max.ptr, 4, hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  if (nb_max > 0) {
+    Q.nb_max = nb_max;
+    launch(true, (size_t)survey_wave_words(nb_max) * sizeof(uint32_t));
+    SIF_HIP(hipGetLastError());
+  }
+  timing_end(SIF_AMD_STAGE_GROW);
+  return SIF_OK;
+}
+
 // --- SIF_FINDER_EXTEND_REACH: the candidates still beyond their reach, again
 // at the next level (exodus_finder.c:2060-2072) ---
 
@@ -713,6 +822,42 @@ int sif_amd_find_radius(const uint64_t* offsets, const float* mx, const float* m
   }
   st = grow_batch(offsets, mx, my, mz, mesh_cells, box, cx, cy, cz, n, rmin,
     r_search, threshold, vol_factor, radius, reason, n_in);
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  return SIF_OK;
+}
+
+int sif_amd_find_radius_survey(const sif_amd_mesh_t* data, const sif_amd_mesh_t* randoms,
+  float box, const float* cx, const float* cy, const float* cz, uint32_t n, float rmin,
+  float r_search, float K, float* radius, uint32_t* reason, uint64_t* n_in) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  if (!data || !randoms || !data->offsets || !data->x || !data->y || !data->z ||
+      !randoms->offsets || !randoms->x || !randoms->y || !randoms->z || data->n_cells == 0 ||
+      randoms->n_cells == 0 || (data->weights && !data->cell_weights) ||
+      (randoms->weights && !randoms->cell_weights) || (n && (!cx || !cy || !cz)) ||
+      !radius || !reason || !(box > 0.0f)) {
+    error_set(SIF_ERR_INVALID, "sif_amd_find_radius_survey: invalid arguments");
+    return SIF_ERR_INVALID;
+  }
+  SurveyMesh D, Rm;
+  D.offsets = data->offsets;
+  D.x = data->x;
+  D.y = data->y;
+  D.z = data->z;
+  D.w = data->weights;
+  D.cw = data->cell_weights;
+  D.mesh_n = data->n_cells;
+  Rm.offsets = randoms->offsets;
+  Rm.x = randoms->x;
+  Rm.y = randoms->y;
+  Rm.z = randoms->z;
+  Rm.w = randoms->weights;
+  Rm.cw = randoms->cell_weights;
+  Rm.mesh_n = randoms->n_cells;
+  st = grow_survey(D, Rm, box, cx, cy, cz, n, rmin, r_search, K, radius, reason, n_in);
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipStreamSynchronize(stream()));

@@ -88,6 +88,41 @@ struct MeshWeights {
   const float* cell = nullptr;
   float mass_factor = 0.0f;
 };
+// One mesh of a survey run, on the device: cell offsets, positions, and the
+// tracer and cell weights (NULL on an unweighted mesh).
+struct SurveyMesh {
+  const uint64_t* offsets = nullptr;
+  const float *x = nullptr, *y = nullptr, *z = nullptr;
+  const float* w = nullptr;
+  const float* cw = nullptr;
+  uint32_t mesh_n = 0;
+};
+// --- survey (survey.hip) ---
+// What an unobserved node reads as on the scanned field (utils.h:161-166):
+// far above any threshold, never a candidate.
+constexpr float SURVEY_OUTSIDE = 1e30f;
+double survey_randoms_per_node(double m);
+// The observed nodes of an nn^3 grid over `box` (one byte a node): those the
+// n positions reach, closed over (2k + 1)^3 nodes.
+int survey_occupancy(const float* x, const float* y, const float* z, uint64_t n, uint32_t nn,
+  float box, uint8_t* occ, int32_t* k, double* lambda, uint64_t* marked, uint64_t* observed);
+// data <- data / (alpha randoms) - 1 on observed nodes, SURVEY_OUTSIDE elsewhere.
+int survey_contrast(float* data, const float* randoms, const uint8_t* occ, float alpha,
+  uint64_t total, hipStream_t s);
+// The observed fraction of each void's sphere and of its shell to 2r (host
+// columns in and out).
+int survey_footprint(const uint8_t* occ, uint32_t nn, float box, const float* cx,
+  const float* cy, const float* cz, const float* r, uint64_t nv, float* inside, float* shell);
+// The bounding box of n device positions.
+int mesh_extent(const float* x, const float* y, const float* z, uint64_t n, float lo[3],
+  float hi[3]);
+
+// sif__exodus_find_radius_survey (exodus_rescale.c:2172-2317) for n centres
+// against the data mesh D and the random mesh Rm, K = (1 + t) alpha; reasons
+// as grow_batch's, 6 for no randoms inside r_search.
+int grow_survey(const SurveyMesh& D, const SurveyMesh& Rm, float box, const float* cx,
+  const float* cy, const float* cz, uint32_t n, float rmin, float r_search, float K,
+  float* radius, uint32_t* reason, uint64_t* n_in);
 // The wave count a small growth launch's split first pass aims at (a
 // tuning that changes no bit).
 uint32_t grow_split_waves();
@@ -159,6 +194,14 @@ struct RunSpec {
   // (added voids more than twice the rung that found them)
   std::function<void(uint32_t, const float*, const float*, const float*, const float*,
     uint64_t, uint64_t)> on_rung;
+  // A survey run (exodus_finder.c:1592-1660): the randoms' CIC grid (device,
+  // read only; d_delta then holds the data's CIC density, not a contrast),
+  // their mesh, the observed nodes and alpha, the data weight over the
+  // randoms'. The catalogue then carries the footprint columns.
+  const float* d_random = nullptr;
+  SurveyMesh rmesh;
+  const uint8_t* d_occupied = nullptr;
+  float alpha = 0.0f;
 };
 int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   sif_amd_rung_stats_t* stats_out);
@@ -173,6 +216,14 @@ int check_options(sif_option opt);
 // positive total. The host form reads host weights, the device form device
 // ones.
 int check_weights_host(const float* w, uint64_t n, const float* cell_w, double total);
+// exodus_finder.c:1114-1137 for a survey: seeds on their nodes only (invalid
+// otherwise), then check_options.
+int survey_check_options(sif_option opt);
+// The survey's checks on its meshes (weights, totals, padding) and its inputs
+// to R: the observed nodes (from the random mesh), alpha, the random grid and
+// mesh. R's grid, data mesh and box are set before.
+int survey_prepare(RunSpec& R, const sif_amd_mesh_t& data, const sif_amd_mesh_t& randoms,
+  const float* d_randoms, const sif_real* radii, uint32_t n_radii, sif_option opt);
 
 // sif_profiles on device arrays (tracer_profiles.hip). The spec's geometry,
 // bin edges and volumes, mean density and flags come from

@@ -217,6 +217,85 @@ def exodus(ddelta, n_cells, box, mesh, radii, threshold, overlap=0.0, options=0,
     return out, n_in, st
 
 
+# --- survey (src/finder/exodus_survey.c) ---
+
+
+def mesh_desc(mesh):
+    """The sif_amd_mesh_t of a device Mesh."""
+    w = mesh.w is not None
+    return N.MeshDesc(mesh.offsets.ptr, mesh.x.ptr, mesh.y.ptr, mesh.z.ptr,
+                      mesh.w.ptr if w else None, mesh.cell_weights.ptr if w else None,
+                      mesh.total_weight if w else float(mesh.n), mesh.n_cells, mesh.n)
+
+
+def survey_occupancy(dx, dy, dz, n, n_cells, box):
+    """(occupied nodes as an n_cells^3 uint8 device array, k, lambda, marked,
+    observed) for positions on the device."""
+    occ = DeviceArray((n_cells ** 3,), np.uint8)
+    k, lam, marked, observed = C.c_int32(), C.c_double(), C.c_uint64(), C.c_uint64()
+    check(lib.sif_amd_survey_occupancy(dx.ptr, dy.ptr, dz.ptr, n, n_cells, box, occ.ptr,
+                                       C.byref(k), C.byref(lam), C.byref(marked),
+                                       C.byref(observed)))
+    return occ, k.value, lam.value, marked.value, observed.value
+
+
+def survey_contrast(ddata, drandoms, docc, alpha):
+    """In place on ddata: data / (alpha randoms) - 1 on observed nodes, 1e30 elsewhere."""
+    check(lib.sif_amd_survey_contrast(ddata.ptr, drandoms.ptr, docc.ptr, alpha,
+                                      int(np.prod(ddata.shape))))
+
+
+def survey_footprint(docc, n_cells, box, voids):
+    """(inside, shell) observed fractions of voids (k, 4) (host arrays)."""
+    v = np.ascontiguousarray(np.asarray(voids, np.float32).reshape(-1, 4).T)
+    k = v.shape[1]
+    fin = np.empty(k, np.float32)
+    fsh = np.empty(k, np.float32)
+    check(lib.sif_amd_survey_footprint(docc.ptr, n_cells, box, v[0].ctypes.data,
+                                       v[1].ctypes.data, v[2].ctypes.data, v[3].ctypes.data, k,
+                                       fin.ctypes.data, fsh.ctypes.data))
+    return fin, fsh
+
+
+def find_radius_survey(dmesh, rmesh, centres, rmin, r_search, K):
+    """Survey radii of host `centres` (k, 3): (radius, reason, n_in)."""
+    c = np.ascontiguousarray(np.asarray(centres, np.float32).reshape(-1, 3).T)
+    k = c.shape[1]
+    dc = [DeviceArray.from_host(c[a]) for a in range(3)]
+    rad = DeviceArray((max(k, 1),), np.float32)
+    rea = DeviceArray((max(k, 1),), np.uint32)
+    nin = DeviceArray((max(k, 1),), np.uint64)
+    dd, rd = mesh_desc(dmesh), mesh_desc(rmesh)
+    check(lib.sif_amd_find_radius_survey(C.byref(dd), C.byref(rd), dmesh.box, dc[0].ptr,
+                                         dc[1].ptr, dc[2].ptr, k, rmin, r_search, K, rad.ptr,
+                                         rea.ptr, nin.ptr))
+    return rad.get()[:k], rea.get()[:k], nin.get()[:k]
+
+
+def exodus_survey(ddata, drandoms, n_cells, box, dmesh, rmesh, radii, threshold, overlap=0.0,
+                  options=0, restore=True):
+    """The survey finder on device data: (catalogue (n, 4), footprint (n, 2),
+    n_in, stats). ddata holds the data's CIC densities, drandoms the randoms'."""
+    radii = np.ascontiguousarray(radii, dtype=np.float32)
+    nr = len(radii)
+    stats = (N.RungStats * nr)()
+    nin = C.POINTER(C.c_uint64)()
+    dd, rd = mesh_desc(dmesh), mesh_desc(rmesh)
+    cat = lib.sif_amd_exodus_survey_device(
+        ddata.ptr, drandoms.ptr, n_cells, box, C.byref(dd), C.byref(rd),
+        radii.ctypes.data_as(C.POINTER(C.c_float)), nr, threshold, overlap, options,
+        1 if restore else 0, C.byref(nin), stats)
+    if not cat:
+        raise N.error_for(lib.sif_error_status())
+    out, fp = N.catalogue_to_numpy(cat, footprint=True)
+    n_in = np.ctypeslib.as_array(nin, (max(len(out), 1),))[:len(out)].copy()
+    N.libc.free(C.cast(nin, C.c_void_p))
+    st = [dict(radius=s.radius, candidates=s.candidates, masked=s.masked, proxy=s.proxy,
+               overlap=s.overlap, rescale_fail=s.rescale_fail, accepted=s.accepted,
+               evaluated=s.evaluated) for s in stats]
+    return out, fp, n_in, st
+
+
 def set_batch_window(floor=0, ceiling=0):
     """The growth batch window of later exodus runs in this process; (0, 0)
     restores the defaults."""

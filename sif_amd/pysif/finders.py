@@ -1,6 +1,8 @@
 """pysif.finders for the exodus path (python/pysif/finders.pyi of sif
-v0.5.0): exodus(), exodus_raw(), read_checkpoint(), suggest_mesh_cells()
-and grid_cells_from_size(), over sif_finder_exodus, sif_finder_exodus_raw,
+v0.5.0): exodus(), exodus_raw(), exodus_survey(), exodus_survey_raw(),
+survey_box(), read_checkpoint(), suggest_mesh_cells() and
+grid_cells_from_size(), over sif_finder_exodus(_raw),
+sif_finder_exodus_survey(_raw), sif_finder_exodus_survey_box,
 sif_finder_exodus_checkpoint_read and sif_finder_grid_cells_from_size.
 """
 import ctypes as C
@@ -10,7 +12,7 @@ import os
 import numpy as np
 
 from .. import _native as N
-from . import Catalogue, ChainMesh, Grid, _Grid, _Mesh, _Catalogue, _check, real
+from . import Catalogue, ChainMesh, Grid, _Field, _Grid, _Mesh, _Catalogue, _check, real
 
 P = C.c_void_p
 f32, f64, u32, u64, i32 = C.c_float, C.c_double, C.c_uint32, C.c_uint64, C.c_int
@@ -135,6 +137,82 @@ def exodus_raw(grid, mesh, radii, threshold, overlap_fraction=0.0, *, overlap_ty
     ptr = L.sif_finder_exodus_raw(grid._p, mesh._p, r.ctypes.data_as(C.POINTER(f32)), len(r),
                                   float(threshold), float(overlap_fraction), opt, C.byref(ex))
     return _catalogue(ptr)
+
+
+SURVEY_VARIANTS = ("excursion-set", "lagrangian-overlap")
+_RP = C.POINTER(f32)
+L.sif_finder_exodus_survey.restype = C.POINTER(_Catalogue)
+L.sif_finder_exodus_survey.argtypes = [C.POINTER(_Grid), C.POINTER(_Grid), C.POINTER(_Mesh),
+                                       C.POINTER(_Mesh), _RP, _RP, u32, f32, i32,
+                                       C.POINTER(_Extras)]
+L.sif_finder_exodus_survey_raw.restype = C.POINTER(_Catalogue)
+L.sif_finder_exodus_survey_raw.argtypes = [C.POINTER(_Grid), C.POINTER(_Grid),
+                                           C.POINTER(_Mesh), C.POINTER(_Mesh), _RP, _RP, u32,
+                                           f32, f32, u32, C.POINTER(_Extras)]
+L.sif_finder_exodus_survey_box.restype = i32
+L.sif_finder_exodus_survey_box.argtypes = [C.POINTER(_Field), C.POINTER(_Field), _RP, u32,
+                                           u32, _RP, _RP]
+
+
+def _offset(offset):
+    o = [float(v) for v in offset]
+    if len(o) != 3:
+        raise ValueError(f"offset: {offset!r}, expected 3 numbers")
+    return (f32 * 3)(*o)
+
+
+def exodus_survey(data_grid, random_grid, data_mesh, random_mesh, offset, radii, threshold,
+                  variant="excursion-set", *, checkpoint=None, prior=None,
+                  keep_checkpoint=False, save_memory=False):
+    """Find voids in a survey whose geometry a random catalogue describes;
+    the voids come back in the caller's frame, with their footprint."""
+    if variant not in SURVEY_VARIANTS:
+        raise ValueError(f"variant: {variant!r}, expected one of {list(SURVEY_VARIANTS)}")
+    r = _radii(radii)
+    ex = _extras(checkpoint, prior, keep_checkpoint, save_memory)
+    ptr = L.sif_finder_exodus_survey(data_grid._p, random_grid._p, data_mesh._p,
+                                     random_mesh._p, _offset(offset),
+                                     r.ctypes.data_as(_RP), len(r), float(threshold),
+                                     VARIANTS[variant], C.byref(ex))
+    return _catalogue(ptr)
+
+
+def exodus_survey_raw(data_grid, random_grid, data_mesh, random_mesh, offset, radii, threshold,
+                      overlap_fraction=0.0, *, overlap_type="eulerian", seeds="cells",
+                      search_factor=1.5, deconvolution="full", consume_grid=False,
+                      checkpoint=None, prior=None, keep_checkpoint=False, save_memory=False):
+    """exodus_survey(), every setting taken one by one; seeds='cells' only."""
+    if overlap_type not in ("eulerian", "lagrangian"):
+        raise ValueError(f"overlap_type: {overlap_type!r}")
+    if seeds not in ("cells", "minima"):
+        raise ValueError(f"seeds: {seeds!r}")
+    if deconvolution not in ("full", "aliasing", "none"):
+        raise ValueError(f"deconvolution: {deconvolution!r}")
+    opt = SEARCH[snap_search_factor(search_factor)]
+    opt |= LAGRANGIAN_OVERLAP if overlap_type == "lagrangian" else 0
+    opt |= SEEDS_MINIMA if seeds == "minima" else 0
+    opt |= {"full": 0, "aliasing": ALIASING_WINDOW, "none": KEEP_CIC_WINDOW}[deconvolution]
+    opt |= CONSUME_GRID if consume_grid else 0
+    opt |= NO_CHECKPOINT if checkpoint is False else 0
+    r = _radii(radii)
+    ex = _extras(checkpoint, prior, keep_checkpoint, save_memory)
+    ptr = L.sif_finder_exodus_survey_raw(data_grid._p, random_grid._p, data_mesh._p,
+                                         random_mesh._p, _offset(offset),
+                                         r.ctypes.data_as(_RP), len(r), float(threshold),
+                                         float(overlap_fraction), opt, C.byref(ex))
+    return _catalogue(ptr)
+
+
+def survey_box(data, randoms, radii, search_factor=1.5):
+    """(offset, box_length): the box a survey has to be searched in and the
+    offset that centres it there."""
+    r = _radii(radii)
+    off = (f32 * 3)()
+    box = f32()
+    _check(L.sif_finder_exodus_survey_box(data._p, randoms._p, r.ctypes.data_as(_RP), len(r),
+                                          SEARCH[snap_search_factor(search_factor)], off,
+                                          C.byref(box)))
+    return np.array(list(off), dtype=real), float(box.value)
 
 
 def read_checkpoint(path):
