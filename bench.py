@@ -434,6 +434,93 @@ def batched_form():
                                    "3-min(60, L/4) Mpc/h, t=-0.8, sif_amd_exodus_batch"}}
 
 
+SURVEY = dict(box=1000.0, n_cells=256, centre=500.0, foot_r=380.0, n_data=2_000_000,
+              n_randoms=20_000_000, n_holes=300, hole_lo=10.0, hole_hi=35.0, kept=0.03,
+              r_lo=8.0, r_hi=40.0, seed=17)
+
+
+def survey_tracers(a, rng, n, carve, holes):
+    """n points uniform over a ball footprint; with `carve`, all but a few
+    per cent of what falls in a hole dropped."""
+    out, got = [], 0
+    while got < n:
+        m = 2 * (n - got) + 65536
+        p = a["centre"] + (2.0 * rng.random((m, 3)) - 1.0) * a["foot_r"]
+        keep = ((p - a["centre"]) ** 2).sum(axis=1) <= a["foot_r"] ** 2
+        p = p[keep]
+        if carve:
+            from scipy.spatial import cKDTree
+            hit = np.zeros(len(p), bool)
+            tree = cKDTree(p)
+            for h in holes:
+                hit[tree.query_ball_point(h[:3], h[3])] = True
+            p = p[~hit | (rng.random(len(p)) < a["kept"])]
+        p = p[: n - got]
+        out.append(p.astype(np.float32))
+        got += len(p)
+    p = np.vstack(out)
+    return [np.ascontiguousarray(p[:, k]) for k in range(3)]
+
+
+def survey_form(steps, warmup):
+    """The survey finder (sif_amd_exodus_survey_device) on a ball footprint
+    of 2 M galaxies and 20 M randoms in a 1000 Mpc/h box, 300 carved holes:
+    both CIC grids, both meshes and the run (occupancy, both fields smoothed
+    each rung, the two-mesh growth, footprints) per step, positions resident
+    in HBM."""
+    from sif_amd import _native as N, device as D, zeldovich
+    a = SURVEY
+    rng = np.random.default_rng(a["seed"])
+    t0 = time.time()
+    r = a["foot_r"] - a["hole_hi"] - 20.0
+    hc = a["centre"] + rng.uniform(-1.0, 1.0, (4 * a["n_holes"], 3)) * r
+    hc = hc[((hc - a["centre"]) ** 2).sum(axis=1) <= r * r][: a["n_holes"]]
+    holes = np.c_[hc, rng.uniform(a["hole_lo"], a["hole_hi"], len(hc))]
+    data = survey_tracers(a, rng, a["n_data"], True, holes)
+    randoms = survey_tracers(a, rng, a["n_randoms"], False, holes)
+    radii = zeldovich.ladder(a["r_lo"], a["r_hi"], 0.05)
+    n, box = a["n_cells"], a["box"]
+    dd = [D.to_device(v) for v in data]
+    rd = [D.to_device(v) for v in randoms]
+    md = max(8, int(min((a["n_data"] / 30.0) ** (1.0 / 3.0), 1024)))
+    mr = max(8, int(min((a["n_randoms"] / 30.0) ** (1.0 / 3.0), 1024)))
+    log(f"[bench] survey: {a['n_data']} galaxies, {a['n_randoms']} randoms, {len(holes)} holes, "
+        f"set up in {time.time() - t0:.1f}s")
+
+    def run():
+        gd = D.cic(*dd, a["n_data"], n, box)
+        gr = D.cic(*rd, a["n_randoms"], n, box)
+        mesh_d = D.Mesh(*dd, a["n_data"], md, box, indices=False)
+        mesh_r = D.Mesh(*rd, a["n_randoms"], mr, box, indices=False)
+        cat, fp, _, _ = D.exodus_survey(gd, gr, n, box, mesh_d, mesh_r, radii, THRESHOLD,
+                                        restore=False)
+        return cat, fp
+
+    for _ in range(warmup):
+        cat, fp = run()
+    N.check(N.lib.sif_amd_synchronize())
+    t = time.perf_counter()
+    for _ in range(steps):
+        cat, fp = run()
+    N.check(N.lib.sif_amd_synchronize())
+    dt = (time.perf_counter() - t) / steps
+    found = 0
+    for h in holes:
+        d = np.sqrt(((cat[:, :3] - h[:3]) ** 2).sum(axis=1)) if len(cat) else np.array([np.inf])
+        found += d.min() <= 0.5 * h[3]
+    log(f"[bench] survey: {dt * 1e3:.1f} ms a catalogue, {len(cat)} voids, {found} of "
+        f"{len(holes)} holes within half their radius, mean footprint {fp[:, 0].mean():.3f}")
+    return {"metric": "survey exodus galaxies/s", "value": round(a["n_data"] / dt, 1),
+            "unit": "galaxies/s", "seconds": round(dt, 4), "voids": int(len(cat)),
+            "holes_recovered": int(found), "holes": int(len(holes)),
+            "config": {"workload": f"ball footprint r={a['foot_r']:g} in L={box:g}, "
+                                   f"{a['n_data']} galaxies (300 carved holes of r "
+                                   f"{a['hole_lo']:g}-{a['hole_hi']:g}), {a['n_randoms']} "
+                                   f"randoms, grid {n}^3, meshes {md}^3/{mr}^3, "
+                                   f"{len(radii)} rungs {a['r_lo']:g}-{a['r_hi']:g}, "
+                                   f"t={THRESHOLD}, sif_amd_exodus_survey_device"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,6 +532,8 @@ def main():
                     help="skip the differentiable profile chain's leg")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 96-box leg")
+    ap.add_argument("--no-survey", action="store_true",
+                    help="skip the survey finder's leg")
     ap.add_argument("--forms-only", action="store_true",
                     help="run only the adjoint and batched legs (for profiling them)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -476,6 +565,8 @@ def main():
             forms["adjoint"] = adjoint_form(args.steps, args.warmup)
         if not args.no_batched:
             forms["batched"] = batched_form()
+        if not args.no_survey:
+            forms["survey"] = survey_form(args.steps, args.warmup)
         print(json.dumps({"forms": forms}), flush=True)
         return
 
@@ -531,6 +622,8 @@ def main():
             forms["adjoint"] = adjoint_form(args.steps, args.warmup)
         if not args.no_batched:
             forms["batched"] = batched_form()
+        if not args.no_survey:
+            forms["survey"] = survey_form(args.steps, args.warmup)
 
     cpu = None
     if not args.no_cpu:

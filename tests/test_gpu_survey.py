@@ -303,3 +303,34 @@ def test_survey_refusals(native):
     with pytest.raises(ValueError, match="holds a density contrast"):
         finders.exodus_survey_raw(gd, gr, md, mr, (0, 0, 0), RADII, THRESHOLD,
                                   checkpoint=False)
+
+
+def test_survey_checkpoint_kept_resumed_and_refused(native, tmp_path):
+    """A survey's checkpoint (exodus_finder.c:1435-1575, exodus_checkpoint.c:
+    417-430): kept on request, read back in the caller's frame, resumed to the
+    same catalogue, refused for another offset."""
+    from sif_amd import pysif
+    from sif_amd.pysif import finders
+    pysif.system_init()
+    rng = np.random.default_rng(90)
+    data = draw(40000, True, True, rng)
+    randoms = draw(400000, True, False, rng)
+    fd, gd, md = _pysif_tracers(pysif, data, BOX)
+    fr, gr, mr = _pysif_tracers(pysif, randoms, BOX)
+    off = (0.0, 0.0, 0.0)
+    path = tmp_path / "survey.ckpt"
+    cat = finders.exodus_survey_raw(gd, gr, md, mr, off, RADII, THRESHOLD, checkpoint=str(path),
+                                    keep_checkpoint=True)
+    full = np.c_[cat.centres, cat.radii]
+    fp = np.c_[cat.footprint, cat.footprint_shell]
+    assert path.exists() and len(full) >= 3
+    voids, info = finders.read_checkpoint(path)
+    assert info["mode"] == "survey" and info["rungs_done"] == len(RADII)
+    np.testing.assert_array_equal(np.c_[voids.centres, voids.radii], full)
+    again = finders.exodus_survey_raw(gd, gr, md, mr, off, RADII, THRESHOLD,
+                                      checkpoint=str(path), keep_checkpoint=True)
+    np.testing.assert_array_equal(np.c_[again.centres, again.radii], full)
+    np.testing.assert_array_equal(np.c_[again.footprint, again.footprint_shell], fp)
+    with pytest.raises(ValueError, match="offset"):
+        finders.exodus_survey_raw(gd, gr, md, mr, (1.0, 0.0, 0.0), RADII, THRESHOLD,
+                                  checkpoint=str(path))
