@@ -92,20 +92,22 @@ STAGE_KERNELS = {
 }
 
 
-# HBM traffic per launch of a kernel, measured with rocprof's FETCH_SIZE and
+# HBM traffic per launch of a stage, measured with rocprof's FETCH_SIZE and
 # WRITE_SIZE counters scaled against a copy kernel of known size; the summary
-# that holds it names the commands
-TRAFFIC = {("plain512", "grow"): "profiles/grow_kernel_hbm_traffic_plain512.md"}
+# that holds it names the commands and carries a line
+# "traffic_bytes_per_launch[<stage>]: <bytes>" per stage measured
+TRAFFIC = {"plain512": "profiles/hbm_traffic_plain512.md"}
 
 
 def measured_traffic(config, stage):
     """(bytes per launch, the summary under profiles/ that holds it) or
     (None, None)."""
-    path = TRAFFIC.get((config, stage))
+    path = TRAFFIC.get(config)
     if not path or not os.path.exists(os.path.join(ROOT, path)):
         return None, None
+    key = f"traffic_bytes_per_launch[{stage}]:"
     for line in open(os.path.join(ROOT, path)):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return int(line.split(":")[1]), path
     return None, None
 
@@ -534,6 +536,8 @@ def main():
                     help="skip the 96-box leg")
     ap.add_argument("--no-survey", action="store_true",
                     help="skip the survey finder's leg")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="time the plain catalogue only (for profiling it)")
     ap.add_argument("--forms-only", action="store_true",
                     help="run only the adjoint and batched legs (for profiling them)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -614,7 +618,7 @@ def main():
                 args.steps, args.warmup, a, (x, y, z), big,
                 f"the {len(big)} largest voids of the plain catalogue "
                 f"(radii {big[-1, 3]:.2f}-{big[0, 3]:.2f})")
-    else:
+    elif not args.no_forms:
         forms["profiles"] = profiles_form(step, cat, args.steps, args.warmup)
         del step
         forms["dropin"] = dropin_form(c, x, y, z, radii, mesh_cells, len(cat))
