@@ -101,8 +101,10 @@ _sig("orc_mark_sphere", None, P, f32, f32, f32, f32, u32, f32)
 _sig("orc_exodus_run", i32, C.POINTER(P), u32, f32, C.POINTER(Mesh), u64, P, u32, f32,
      f32, i32, u32, i32, i32, u32, C.POINTER(Catalogue), C.POINTER(RungStats))
 _sig("orc_catalogue_free", None, C.POINTER(Catalogue))
-_sig("orc_exodus_pipeline", i32, P, P, P, u64, u32, f32, u32, P, u32, f32, u32,
+_sig("orc_exodus_pipeline", i32, P, P, P, u64, u32, f32, u32, P, u32, f32, u32, i32,
      C.POINTER(Catalogue), P)
+_sig("orc_fftf_r2c", i32, P, u32, P)
+_sig("orc_fftf_filter_c2r", i32, P, u32, P, P, f32, P, P)
 _sig("orc_num_threads", i32)
 
 
@@ -233,6 +235,30 @@ def fft_c2r(spec, n):
     out = np.empty(n ** 3, np.float64)
     if lib.orc_fft_c2r(_p(s), n, _p(out)) != 0:
         raise ValueError("n must be a power of two")
+    return out
+
+
+def fftf_r2c(values, n):
+    """Float forward transform (orc_fftf_r2c): the half spectrum as complex64."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    spec = np.empty((n, n, n // 2 + 1, 2), np.float32)
+    if lib.orc_fftf_r2c(_p(v), n, _p(spec)) != 0:
+        raise ValueError("n must be a power of two of at least 4")
+    return spec[..., 0] + 1j * spec[..., 1]
+
+
+def fftf_filter_c2r(spec, n, lut=None, axis=None, norm=1.0):
+    """Float inverse transform (orc_fftf_filter_c2r) of spec times the mode
+    factors of lut and axis, times norm."""
+    s = np.empty((n, n, n // 2 + 1, 2), np.float32)
+    s[..., 0] = spec.real
+    s[..., 1] = spec.imag
+    work = np.empty_like(s)
+    out = np.empty(n ** 3, np.float32)
+    lut = None if lut is None else np.ascontiguousarray(lut, np.float32)
+    axis = None if axis is None else np.ascontiguousarray(axis, np.float64)
+    if lib.orc_fftf_filter_c2r(_p(s), n, _p(lut), _p(axis), norm, _p(work), _p(out)) != 0:
+        raise ValueError("n must be a power of two of at least 4")
     return out
 
 
@@ -405,14 +431,18 @@ def exodus_run(fields, n_cells, box, mesh, n_particles, radii_desc, threshold,
     return out + ([dict((k, getattr(s, k)) for k, _ in RungStats._fields_) for s in stats],)
 
 
-def exodus_pipeline(x, y, z, n_cells, box, mesh_cells, radii, threshold, max_rungs=0):
+def exodus_pipeline(x, y, z, n_cells, box, mesh_cells, radii, threshold, max_rungs=0,
+                    fp32=False):
+    """The whole CPU path from particles (orc_exodus_pipeline): transforms in
+    double (each field rounded to float) or, with fp32, in float as the
+    reference's FFTW plans run. Returns (catalogue, counts, stage seconds)."""
     x, y, z = f32a(x), f32a(y), f32a(z)
     radii = np.ascontiguousarray(radii, dtype=np.float32)
     cat = Catalogue()
     stage = np.zeros(6, np.float64)
     st = lib.orc_exodus_pipeline(_p(x), _p(y), _p(z), len(x), n_cells, box, mesh_cells,
-                                 _p(radii), len(radii), threshold, max_rungs, C.byref(cat),
-                                 _p(stage))
+                                 _p(radii), len(radii), threshold, max_rungs, int(fp32),
+                                 C.byref(cat), _p(stage))
     if st != 0:
         lib.orc_catalogue_free(C.byref(cat))
         raise ValueError(f"oracle pipeline failed ({st})")

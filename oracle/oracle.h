@@ -77,6 +77,13 @@ float orc_mode_factor(const float* lut, const double* axis, uint32_t n,
 int orc_fft_r2c(const double* in, uint32_t n, double* spec /* 2 per mode */);
 int orc_fft_c2r(const double* spec, uint32_t n, double* out /* unscaled */);
 
+/* --- 3D FFT in float (power-of-two n >= 4), the CPU baseline's: spectrum
+ * [n][n][n/2+1] interleaved complex; the c2r multiplies by the mode factors
+ * (lut null for none) and by norm --- */
+int orc_fftf_r2c(const float* in, uint32_t n, float* spec);
+int orc_fftf_filter_c2r(const float* spec, uint32_t n, const float* lut, const double* axis,
+  float norm, float* work, float* out);
+
 /* --- candidates --- */
 typedef struct {
   float delta;
@@ -218,13 +225,26 @@ int orc_exodus_run(float* const* fields, uint32_t n_cells, float box,
 void orc_catalogue_free(orc_catalogue_t* cat);
 
 /* Whole CPU path on the host cores (OpenMP): CIC (reference float order),
- * contrast, mesh, r2c, per rung top-hat + window + c2r (double, rounded to
- * float), then orc_exodus_run. Per-stage seconds go to stage_s[6]:
- * cic, contrast, mesh, forward fft, smoothing (all rungs), finder. */
+ * contrast, mesh, r2c, per rung top-hat + window + c2r, then orc_exodus_run.
+ * fp32 0: the transforms in double, each field rounded to float, all rungs
+ * smoothed before the finder; fp32 1: the transforms in float (the
+ * reference's precision) and each rung smoothed as the finder reaches it,
+ * into one buffer, as the reference does. Per-stage seconds go to
+ * stage_s[6]: cic, contrast, mesh, forward fft, smoothing (all rungs),
+ * finder (less the smoothing). */
 int orc_exodus_pipeline(const float* x, const float* y, const float* z,
   uint64_t n, uint32_t n_cells, float box, uint32_t mesh_cells,
   const float* radii, uint32_t n_radii, float threshold, uint32_t max_rungs,
-  orc_catalogue_t* cat, double* stage_s);
+  int fp32, orc_catalogue_t* cat, double* stage_s);
+
+/* orc_exodus_run with each rung's field made when the run reaches it:
+ * field(ctx, i) returns rung i's field (valid until the next call). */
+typedef const float* (*orc_field_fn)(void* ctx, uint32_t rung);
+int orc_exodus_run_lazy(orc_field_fn field, void* ctx, uint32_t n, float box,
+  const orc_mesh_t* mesh, uint64_t n_particles, const float* radii,
+  uint32_t n_radii, float t, float f, int lagrangian, uint32_t search_opt,
+  int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
+  orc_rung_stats_t* stats);
 
 /* --- survey: src/finder/exodus_survey.c, exodus_rescale.c:1909-2317 ---
  * Occupancy masks are one byte a grid node (0/1), flat (ix n + iy) n + iz. */

@@ -1001,7 +1001,8 @@ typedef struct {
  * at commit; the catalogue is the one a strictly sequential pass gives. */
 /* rmesh set: a survey run (exodus_finder.c:1592, 2045-2049), each radius
  * grown against the randoms by orc_find_radius_survey with K, no witness. */
-static int exodus_run_impl(float* const* fields, uint32_t n, float box,
+static int exodus_run_impl(float* const* fields, orc_field_fn field_fn, void* field_ctx,
+  uint32_t n, float box,
   const orc_mesh_t* mesh, const orc_mesh_t* rmesh, float survey_K, uint64_t n_particles,
   const float* radii, uint32_t n_radii, float t, float f, int lagrangian,
   uint32_t search_opt, int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
@@ -1077,7 +1078,11 @@ static int exodus_run_impl(float* const* fields, uint32_t n, float box,
     const float radius = radii[i];
     orc_rung_stats_t st;
     memset(&st, 0, sizeof(st));
-    const float* field = fields[i];
+    const float* field = fields ? fields[i] : field_fn(field_ctx, i);
+    if (!field) {
+      status = -5;
+      break;
+    }
     const uint64_t nc = (search_opt & ORC_SEEDS_MINIMA)
                           ? orc_scan_minima(field, n, mask, t, cand)
                           : orc_scan_candidates(field, mask, total, t, cand);
@@ -1324,8 +1329,17 @@ int orc_exodus_run(float* const* fields, uint32_t n, float box,
   uint32_t n_radii, float t, float f, int lagrangian, uint32_t search_opt,
   int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
   orc_rung_stats_t* stats) {
-  return exodus_run_impl(fields, n, box, mesh, NULL, 0.0f, n_particles, radii, n_radii, t,
-    f, lagrangian, search_opt, tie_rule, use_witness, window, cat, stats);
+  return exodus_run_impl(fields, NULL, NULL, n, box, mesh, NULL, 0.0f, n_particles, radii,
+    n_radii, t, f, lagrangian, search_opt, tie_rule, use_witness, window, cat, stats);
+}
+
+int orc_exodus_run_lazy(orc_field_fn field, void* ctx, uint32_t n, float box,
+  const orc_mesh_t* mesh, uint64_t n_particles, const float* radii,
+  uint32_t n_radii, float t, float f, int lagrangian, uint32_t search_opt,
+  int tie_rule, int use_witness, uint32_t window, orc_catalogue_t* cat,
+  orc_rung_stats_t* stats) {
+  return exodus_run_impl(NULL, field, ctx, n, box, mesh, NULL, 0.0f, n_particles, radii,
+    n_radii, t, f, lagrangian, search_opt, tie_rule, use_witness, window, cat, stats);
 }
 
 /* sif__exodus_run for a survey (exodus_finder.c:1585-2492 with a random
@@ -1341,7 +1355,8 @@ int orc_exodus_run_survey(float* const* fields, uint32_t n, float box,
   orc_rung_stats_t* stats) {
   if (search_opt & (ORC_SEEDS_MINIMA | ORC_EXTEND_REACH | ORC_DISPLACE))
     return -2;
-  return exodus_run_impl(fields, n, box, data_mesh, random_mesh, survey_K, 0, radii_desc,
+  return exodus_run_impl(fields, NULL, NULL, n, box, data_mesh, random_mesh, survey_K, 0,
+    radii_desc,
     n_radii, threshold, overlap_fraction, lagrangian, search_opt, tie_rule, 0, window, cat,
     stats);
 }

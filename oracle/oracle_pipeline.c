@@ -25,9 +25,34 @@ static int cmp_desc(const void* pa, const void* pb) {
   return (a < b) - (a > b);
 }
 
+/* The float path's per-rung smoothing, made when the finder reaches a rung
+ * (the reference smooths each rung into the one grid buffer the same way,
+ * exodus_finder.c:1704-1760). */
+typedef struct {
+  const float* spec;
+  float* work;
+  float* field;
+  float* lut;
+  const double* axis;
+  const float* radii;
+  uint32_t n;
+  float box;
+  double seconds;
+} lazy_smooth_t;
+
+static const float* lazy_field(void* ctx, uint32_t rung) {
+  lazy_smooth_t* L = ctx;
+  const double t0 = now_s();
+  orc_filter_lut(ORC_FILTER_TOP_HAT, L->radii[rung], 0.0f, L->box, L->n, L->lut);
+  const float norm = 1.0f / (float)((uint64_t)L->n * L->n * L->n);
+  const int st = orc_fftf_filter_c2r(L->spec, L->n, L->lut, L->axis, norm, L->work, L->field);
+  L->seconds += now_s() - t0;
+  return st == 0 ? L->field : NULL;
+}
+
 int orc_exodus_pipeline(const float* x, const float* y, const float* z,
   uint64_t n, uint32_t N, float box, uint32_t mesh_cells, const float* radii_in,
-  uint32_t n_radii, float threshold, uint32_t max_rungs, orc_catalogue_t* cat,
+  uint32_t n_radii, float threshold, uint32_t max_rungs, int fp32, orc_catalogue_t* cat,
   double* stage_s) {
   const uint64_t total = (uint64_t)N * N * N;
   const uint32_t nz = N / 2 + 1;
@@ -63,72 +88,84 @@ int orc_exodus_pipeline(const float* x, const float* y, const float* z,
   stage_s[2] = t1 - t0;
   t0 = t1;
 
-  double* dgrid = malloc(sizeof(double) * total);
-  for (uint64_t i = 0; i < total; i++)
-    dgrid[i] = (double)grid[i];
-  double* spec = malloc(sizeof(double) * 2 * modes);
-  if (orc_fft_r2c(dgrid, N, spec) != 0) {
-    free(grid);
-    free(dgrid);
-    free(spec);
-    free(offsets);
-    free(mx);
-    free(my);
-    free(mz);
-    return -5;
-  }
-  double* axis = malloc(sizeof(double) * N);
-  orc_window_axis(N, 2, 0, axis);
-  t1 = now_s();
-  stage_s[3] = t1 - t0;
-  t0 = t1;
-
   float* radii = malloc(sizeof(float) * n_radii);
   memcpy(radii, radii_in, sizeof(float) * n_radii);
   qsort(radii, n_radii, sizeof(float), cmp_desc);
   if (max_rungs && max_rungs < n_radii)
     n_radii = max_rungs;
-
-  float** fields = malloc(sizeof(float*) * n_radii);
-  double* work = malloc(sizeof(double) * 2 * modes);
+  double* axis = malloc(sizeof(double) * N);
+  orc_window_axis(N, 2, 0, axis);
   float* lut = malloc(sizeof(float) * orc_filter_lut_size(N));
-  const double norm = 1.0 / (double)total;
-  for (uint32_t r = 0; r < n_radii; r++) {
-    orc_filter_lut(ORC_FILTER_TOP_HAT, radii[r], 0.0f, box, N, lut);
-#pragma omp parallel for schedule(static)
-    for (uint64_t m = 0; m < modes; m++) {
-      const uint32_t ix = (uint32_t)(m / ((uint64_t)N * nz));
-      const uint32_t iy = (uint32_t)((m / nz) % N);
-      const uint32_t iz = (uint32_t)(m % nz);
-      const double s = (double)orc_mode_factor(lut, axis, N, ix, iy, iz);
-      work[2 * m] = spec[2 * m] * s;
-      work[2 * m + 1] = spec[2 * m + 1] * s;
-    }
-    orc_fft_c2r(work, N, dgrid);
-    fields[r] = malloc(sizeof(float) * total);
-    for (uint64_t i = 0; i < total; i++)
-      fields[r][i] = (float)(dgrid[i] * norm);
-  }
-  t1 = now_s();
-  stage_s[4] = t1 - t0;
-  t0 = t1;
-
   const orc_mesh_t mesh = {mesh_cells, box, box / (float)mesh_cells, offsets, mx,
     my, mz, NULL, NULL, (double)n, NULL, NULL, NULL};
-  const int st = orc_exodus_run(fields, N, box, &mesh, n, radii, n_radii,
-    threshold, 0.0f, 0, 0, 0, 1, 8, cat, NULL);
-  t1 = now_s();
-  stage_s[5] = t1 - t0;
+  int st = 0;
 
-  for (uint32_t r = 0; r < n_radii; r++)
-    free(fields[r]);
-  free(fields);
-  free(work);
+  if (fp32) {
+    float* fspec = malloc(sizeof(float) * 2 * modes);
+    float* fwork = malloc(sizeof(float) * 2 * modes);
+    st = fspec && fwork ? orc_fftf_r2c(grid, N, fspec) : -4;
+    t1 = now_s();
+    stage_s[3] = t1 - t0;
+    t0 = t1;
+    if (st == 0) {
+      lazy_smooth_t L = {fspec, fwork, grid, lut, axis, radii, N, box, 0.0};
+      st = orc_exodus_run_lazy(lazy_field, &L, N, box, &mesh, n, radii, n_radii, threshold,
+        0.0f, 0, 0, 0, 1, 8, cat, NULL);
+      t1 = now_s();
+      stage_s[4] = L.seconds;
+      stage_s[5] = t1 - t0 - L.seconds;
+    } else {
+      st = -5;
+    }
+    free(fspec);
+    free(fwork);
+  } else {
+    double* dgrid = malloc(sizeof(double) * total);
+    for (uint64_t i = 0; i < total; i++)
+      dgrid[i] = (double)grid[i];
+    double* spec = malloc(sizeof(double) * 2 * modes);
+    st = orc_fft_r2c(dgrid, N, spec) != 0 ? -5 : 0;
+    t1 = now_s();
+    stage_s[3] = t1 - t0;
+    t0 = t1;
+    float** fields = calloc(n_radii, sizeof(float*));
+    double* work = malloc(sizeof(double) * 2 * modes);
+    const double norm = 1.0 / (double)total;
+    for (uint32_t r = 0; r < n_radii && st == 0; r++) {
+      orc_filter_lut(ORC_FILTER_TOP_HAT, radii[r], 0.0f, box, N, lut);
+#pragma omp parallel for schedule(static)
+      for (uint64_t m = 0; m < modes; m++) {
+        const uint32_t ix = (uint32_t)(m / ((uint64_t)N * nz));
+        const uint32_t iy = (uint32_t)((m / nz) % N);
+        const uint32_t iz = (uint32_t)(m % nz);
+        const double s = (double)orc_mode_factor(lut, axis, N, ix, iy, iz);
+        work[2 * m] = spec[2 * m] * s;
+        work[2 * m + 1] = spec[2 * m + 1] * s;
+      }
+      orc_fft_c2r(work, N, dgrid);
+      fields[r] = malloc(sizeof(float) * total);
+      for (uint64_t i = 0; i < total; i++)
+        fields[r][i] = (float)(dgrid[i] * norm);
+    }
+    t1 = now_s();
+    stage_s[4] = t1 - t0;
+    t0 = t1;
+    if (st == 0)
+      st = orc_exodus_run(fields, N, box, &mesh, n, radii, n_radii, threshold, 0.0f, 0, 0, 0,
+        1, 8, cat, NULL);
+    t1 = now_s();
+    stage_s[5] = t1 - t0;
+    for (uint32_t r = 0; r < n_radii; r++)
+      free(fields[r]);
+    free(fields);
+    free(work);
+    free(spec);
+    free(dgrid);
+  }
+
   free(lut);
   free(axis);
   free(radii);
-  free(spec);
-  free(dgrid);
   free(grid);
   free(offsets);
   free(mx);

@@ -107,8 +107,13 @@ def holes():
 
 def test_four_holes_pipeline(holes):
     x, y, z, mc, _, _ = holes
-    cat, _, _ = oracle.exodus_pipeline(x, y, z, N_GRID4, BOX4, mc, RADII4, -0.7)
+    cat, nin, _ = oracle.exodus_pipeline(x, y, z, N_GRID4, BOX4, mc, RADII4, -0.7)
     check_four_holes(cat, 0.9, 1.5)
+    # the float path (the CPU baseline's) finds the same catalogue here
+    cat32, nin32, _ = oracle.exodus_pipeline(x, y, z, N_GRID4, BOX4, mc, RADII4, -0.7,
+                                             fp32=True)
+    np.testing.assert_array_equal(cat32.view(np.uint32), cat.view(np.uint32))
+    np.testing.assert_array_equal(nin32, nin)
 
 
 @pytest.mark.parametrize("overlap,lag", [(0.0, False), (0.2, False), (0.0, True)])
@@ -265,6 +270,37 @@ def test_fft_matches_numpy_and_round_trips():
     np.testing.assert_allclose(spec, np.fft.rfftn(v.reshape(n, n, n)), rtol=0, atol=1e-10)
     back = oracle.fft_c2r(spec, n) / n ** 3
     np.testing.assert_allclose(back, v, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [4, 16, 32, 64])
+def test_float_fft_against_numpy(n):
+    """The CPU baseline's single-precision transform: forward against
+    numpy's FP64 pocketfft, the round trip, and the filtered inverse (top-hat
+    and CIC window factors) against pocketfft's inverse of the same product,
+    each to float accuracy (error bound ~ log2(n) ulps of the largest value);
+    a mistake in a twiddle, the bit reversal or the half-length split is
+    off by O(1)."""
+    rng = np.random.default_rng(n)
+    v = rng.standard_normal(n ** 3).astype(np.float32)
+    want = np.fft.rfftn(v.reshape(n, n, n).astype(np.float64))
+    got = oracle.fftf_r2c(v, n)
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+    back = oracle.fftf_filter_c2r(got, n, norm=1.0 / n ** 3)
+    assert np.abs(back - v).max() <= 1e-5 * np.abs(v).max()
+    lut = oracle.filter_lut(1, 3.0, 0.0, 100.0, n)
+    axis = oracle.window_axis(n, 2, 0)
+    fac = oracle.mode_factors(lut, axis, n)
+    ref = np.fft.irfftn(want * fac, s=(n, n, n), axes=(0, 1, 2)).ravel()
+    sm = oracle.fftf_filter_c2r(want.astype(np.complex64), n, lut, axis, 1.0 / n ** 3)
+    assert np.abs(sm - ref).max() <= 1e-5 * np.abs(ref).max()
+    # one mode alone: a plane wave along each axis
+    for ax in range(3):
+        s = np.zeros((n, n, n // 2 + 1), np.complex64)
+        idx = [0, 0, 0]
+        idx[ax] = 1
+        s[tuple(idx)] = 1.0
+        w = np.fft.irfftn(s.astype(np.complex128), s=(n, n, n), axes=(0, 1, 2)).ravel()
+        assert np.abs(oracle.fftf_filter_c2r(s, n, norm=1.0 / n ** 3) - w).max() <= 1e-6
 
 
 def test_corner_gains():
