@@ -41,13 +41,15 @@ SIF_API int sif_amd_copy(void* d_dst, const void* d_src, uint64_t bytes);
 /* Per-stage device time (HIP events on the library stream, recorded without
  * synchronising and resolved when read). Stage order:
  * cic, contrast, mesh, fft_forward, smooth, scan, prefilter, grow, commit,
- * retest, sort, host, profiles, commit_order. commit, retest, host and
- * commit_order are host time: the commit's per-candidate tests against the
- * rung's voids and the batch (on the host pool), the re-tests that assemble
- * each growth batch, the rest of the driver's own work per rung (the void
- * index, the survivors' copy, the filter tables), and the commit's ordered
- * accept loop. profiles is sif_profiles' binning. */
-#define SIF_AMD_TIMING_STAGES 14
+ * retest, sort, host, profiles, commit_order, commit_index. commit, retest,
+ * host, commit_order and commit_index are host time: the commit's
+ * per-candidate tests against the rung's voids and the batch (on the host
+ * pool), the re-tests that assemble each growth batch, the rest of the
+ * driver's own work per rung (the void index, the survivors' copy, the
+ * filter tables), the commit's ordered accept loop, and the commit's
+ * preparation (the candidates' nodes and the index of the batch's own
+ * voids). profiles is sif_profiles' binning. */
+#define SIF_AMD_TIMING_STAGES 15
 SIF_API int sif_amd_timing_enable(int on);
 SIF_API void sif_amd_timing_reset(void);
 SIF_API int sif_amd_timing_read(double* ms, uint64_t* launches, int n_stages);

@@ -23,10 +23,10 @@ SIF_ERR_UNSUPPORTED = -5
 SIF_ERR_UNINITIALISED = -6
 
 FILTER_NONE, FILTER_TOP_HAT, FILTER_GAUSSIAN = 0, 1, 2
-N_STAGES = 14
+N_STAGES = 15
 STAGE_NAMES = ("cic", "contrast", "mesh", "fft_forward", "smooth", "scan",
                "prefilter", "grow", "commit", "retest", "sort", "host", "profiles",
-               "commit_order")
+               "commit_order", "commit_index")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

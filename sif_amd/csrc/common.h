@@ -57,6 +57,16 @@ struct DeviceBuffer {
   template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// The order-preserving integer image of a float (the candidate sort's key)
+// and back: a lower key is a lower value, -0 just below +0.
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block) {
   uint64_t g = (n + block - 1) / block;
   return (unsigned)(g > 0x7fffffffull ? 0x7fffffffull : (g ? g : 1));
@@ -80,5 +90,6 @@ enum {
   SIF_AMD_STAGE_HOST,
   SIF_AMD_STAGE_PROFILES,
   SIF_AMD_STAGE_COMMIT_ORDER,
+  SIF_AMD_STAGE_COMMIT_INDEX,
   SIF_AMD_N_STAGES
 };

@@ -71,6 +71,8 @@ struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
   DeviceBuffer pf_counts;  // per rung: masked, proxy, overlap
   DeviceBuffer field2;     // the second smoothed field (rungs alternate)
+  // or, when only scanned, the two rungs' cell lists in their place
+  DeviceBuffer lkeys[2], llocal[2], lcount[2];
   DeviceBuffer rspec, rfield;  // a survey's randoms: spectrum, smoothed grid
 };
 thread_local DeviceState g_dev;
@@ -438,11 +440,39 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     return st;
   const uint64_t lut_size = filter_lut_size(n);
   const bool windowed = window_order_of(R.opt) > 0;
-  // rung j's scanned field into out, on stream s: the smoothed contrast, or
-  // for a survey the smoothed data over the smoothed randoms
+  // A field that is only scanned for the cells at or below the threshold is
+  // not written: the smoothing lists those cells (CellLists), and the scan
+  // reads the lists. The grid is then never smoothed in place, and comes
+  // back as it was. Minima seeds read each cell's neighbours and a survey
+  // divides two fields: those write the fields.
+  const bool minima = (R.opt & SIF_FINDER_SEEDS_MINIMA) != 0;
+  const bool use_lists = !survey && !minima && fft_lists_supported(n);
+  CellLists lists[2];
+  if (use_lists) {
+    const uint64_t blocks = list_blocks(n);
+    for (int b = 0; b < 2 && st == SIF_OK; b++) {
+      st = g_dev.lkeys[b].reserve(blocks * LIST_BLOCK * sizeof(uint32_t));
+      if (st == SIF_OK)
+        st = g_dev.llocal[b].reserve(blocks * LIST_BLOCK * sizeof(uint16_t));
+      if (st == SIF_OK)
+        st = g_dev.lcount[b].reserve(blocks * sizeof(uint32_t));
+      lists[b].keys = g_dev.lkeys[b].as<uint32_t>();
+      lists[b].local = g_dev.llocal[b].as<uint16_t>();
+      lists[b].count = g_dev.lcount[b].as<uint32_t>();
+      lists[b].blocks = blocks;
+    }
+    if (st != SIF_OK)
+      return st;
+  }
+  // rung j's scanned field into out (or its lists into lists[j & 1]), on
+  // stream s: the smoothed contrast, or for a survey the smoothed data over
+  // the smoothed randoms
   auto smooth = [&](size_t j, float* out, hipStream_t s) -> int {
     const float* lut = g_dev.luts.as<float>() + j * lut_size;
     const double* axis = windowed ? g_dev.axis.as<double>() : nullptr;
+    if (use_lists)
+      return fft_c2r_lut_lists(g_dev.spec.as<float2>(), n, lut, axis, R.threshold, lists[j & 1],
+        g_dev.work.as<float2>(), s);
     int e = fft_c2r_lut(g_dev.spec.as<float2>(), n, lut, axis, out, g_dev.work.as<float2>(), s);
     if (e == SIF_OK && survey)
       e = fft_c2r_lut(g_dev.rspec.as<float2>(), n, lut, axis, g_dev.rfield.as<float>(),
@@ -514,9 +544,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   std::vector<uint64_t> hmask;
 
   // Two smoothed fields in turn: the caller's grid and a second buffer.
-  st = g_dev.field2.reserve(total * sizeof(float));
-  if (st != SIF_OK)
-    return st;
+  if (!use_lists) {
+    st = g_dev.field2.reserve(total * sizeof(float));
+    if (st != SIF_OK)
+      return st;
+  }
   float* const fields[2] = {R.d_delta, g_dev.field2.as<float>()};
   static thread_local hipEvent_t ev_smoothed[2] = {nullptr, nullptr},
     ev_scanned[2] = {nullptr, nullptr};
@@ -562,9 +594,13 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         if ((st = reserve_candidates()) != SIF_OK)
           break;
       }
-      st = scan_candidates_flat(field, g_dev.mask.as<uint64_t>(), total, R.threshold,
-        (R.opt & SIF_FINDER_SEEDS_MINIMA) ? n : 0u, g_dev.keys.as<uint32_t>(),
-        g_dev.flat.as<uint32_t>(), cand_cap, &nc, &key_differ);
+      st = use_lists
+             ? scan_lists_flat(lists[i & 1], g_dev.mask.as<uint64_t>(), total,
+                 g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap, &nc,
+                 &key_differ)
+             : scan_candidates_flat(field, g_dev.mask.as<uint64_t>(), total, R.threshold,
+                 minima ? n : 0u, g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(),
+                 cand_cap, &nc, &key_differ);
       if (st != SIF_OK || nc <= cand_cap)
         break;
       cand_cap = (uint64_t)((double)nc * 1.2) + 1;
@@ -992,24 +1028,24 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
             (double)radius);
           return SIF_ERR_RANGE;
         }
+      // the nodes and the batch's own index, on this thread: a few
+      // nanoseconds a candidate, less than a pool run's wake-up
       nodes.resize(B.count);
-      host_pool().run(B.count, [&](uint64_t b0, uint64_t b1) {
-        for (uint64_t b = b0; b < b1; b++) {
-          Node& N = nodes[b];
-          node_of(B.h_flat[b], &N.ix, &N.iy, &N.iz);
-          N.x = (float)N.ix * l;
-          N.y = (float)N.iy * l;
-          N.z = (float)N.iz * l;
-        }
-      });
       BI.clear();
       bi_place.clear();
-      for (uint32_t b = 0; b < B.count; b++)
+      for (uint32_t b = 0; b < B.count; b++) {
+        Node& N = nodes[b];
+        node_of(B.h_flat[b], &N.ix, &N.iy, &N.iz);
+        N.x = (float)N.ix * l;
+        N.y = (float)N.iy * l;
+        N.z = (float)N.iz * l;
         if (B.h_rad[b] >= 0.0f) {
-          BI.add(nodes[b].x, nodes[b].y, nodes[b].z, B.h_rad[b]);
+          BI.add(N.x, N.y, N.z, B.h_rad[b]);
           bi_place.push_back(b);
         }
+      }
       const float cat_max_r = V.max_r;
+      const double t_idx = now_ms();
       pre.resize(B.count);
       host_pool().run(B.count, [&](uint64_t b0, uint64_t b1) {
         for (uint64_t b = b0; b < b1; b++) {
@@ -1170,7 +1206,8 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       *accepted_out = accepted;
       if (timing_active()) {
         const double t4 = now_ms();
-        timing_add(SIF_AMD_STAGE_COMMIT, t3 - t2);
+        timing_add(SIF_AMD_STAGE_COMMIT_INDEX, t_idx - t2);
+        timing_add(SIF_AMD_STAGE_COMMIT, t3 - t_idx);
         timing_add(SIF_AMD_STAGE_COMMIT_ORDER, t4 - t3);
         if (B.algo) {
           double bytes = 0.0;
@@ -1230,7 +1267,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
 
   // the auxiliary stream may still smooth a rung the loop left (or read work)
   (void)hipStreamSynchronize(aux_stream());
-  if (R.restore && st == SIF_OK)
+  if (R.restore && st == SIF_OK && !use_lists)  // listed rungs left the grid as it was
     st = fft_filter_c2r(g_dev.spec.as<float2>(), n, R.box, SIF_AMD_FILTER_NONE, 0.0f, 0.0f,
       0, 0, R.d_delta, g_dev.work.as<float2>());
   // a failed rung may leave a batch in flight: drain the stream either way

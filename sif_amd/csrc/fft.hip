@@ -385,11 +385,15 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __re
 }
 
 // The Hermitian half row (n/2 + 1 modes) to the real row, scaled by `norm`
-// (fft.c:832).
-template <int N>
+// (fft.c:832). With LIST the real rows are not written: the block's cells at
+// or below t (TK2 rows of N, LIST_BLOCK cells) go to its list in cell order
+// (CellLists), the candidate scan's first half done where the values are
+// made.
+template <int N, bool LIST>
 __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __restrict__ spec,
   float* __restrict__ out, uint64_t rows, uint32_t pitch, uint64_t plane, float norm,
-  const float2* __restrict__ tw_g) {
+  const float2* __restrict__ tw_g, float t, uint32_t* __restrict__ lkeys,
+  uint16_t* __restrict__ llocal, uint32_t* __restrict__ lcount) {
   constexpr int M = N / 2, TK2 = FFT_TILE / M, nz = M + 1, SH = pad_shift(M, 2 * M + 1);
   using LY = Lay<M, SH>;
   __shared__ float2 L[TK2 * LY::pitch];
@@ -437,20 +441,87 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
   }
   __syncthreads();
   tile_fft<M, SH, true>(L, tw);
-  float2 r[PER];
+  if constexpr (!LIST) {
+    float2 r[PER];
 #pragma unroll
-  for (int q = 0; q < PER; q++) {
-    const int e = threadIdx.x + q * FFT_THREADS;
-    const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
-    r[q] = make_float2(z.x * norm, z.y * norm);
-  }
-  float2* out2 = reinterpret_cast<float2*>(out);
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
+      r[q] = make_float2(z.x * norm, z.y * norm);
+    }
+    float2* out2 = reinterpret_cast<float2*>(out);
 #pragma unroll
-  for (int q = 0; q < PER; q++) {
-    const int e = threadIdx.x + q * FFT_THREADS;
-    const uint64_t row = row0 + e / M;
-    if (row < rows)
-      out2[row * M + e % M] = r[q];
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const uint64_t row = row0 + e / M;
+      if (row < rows)
+        out2[row * M + e % M] = r[q];
+    }
+  } else {
+    // Thread element e holds cells 2e and 2e + 1 of the block, so the
+    // block's cell order is (q, wave, lane, pair half). Each wave counts its
+    // cells at or below t for every q by ballot, one wave turns the
+    // (q, wave) counts into offsets, and each cell's place is its run's
+    // offset plus the cells before it in the run. The values are read from
+    // the tile in LDS in both passes (no register array, no ballot held
+    // across the loop).
+    static_assert(PER * (FFT_THREADS / 64) <= 128, "one offset pair a lane");
+    constexpr int WAVES = FFT_THREADS / 64;
+    __shared__ uint32_t run_off[PER * WAVES + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll 2
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const bool in = row0 + e / M < rows;
+      const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
+      const uint64_t m0 = __ballot(in && z.x * norm <= t), m1 = __ballot(in && z.y * norm <= t);
+      if (lane == 0)
+        run_off[q * WAVES + wave] = (uint32_t)(__popcll(m0) + __popcll(m1));
+    }
+    __syncthreads();
+    if (wave == 0) {  // exclusive scan of the PER * WAVES run counts
+      const int i0 = 2 * lane, i1 = 2 * lane + 1;
+      const uint32_t c0 = i0 < PER * WAVES ? run_off[i0] : 0u;
+      const uint32_t c1 = i1 < PER * WAVES ? run_off[i1] : 0u;
+      uint32_t x = c0 + c1;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+          x += y;
+      }
+      const uint32_t ex = x - c0 - c1;
+      if (i0 < PER * WAVES)
+        run_off[i0] = ex;
+      if (i1 < PER * WAVES)
+        run_off[i1] = ex + c0;
+      if (lane == 63)
+        run_off[PER * WAVES] = x;
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * LIST_BLOCK;
+#pragma unroll 2
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * FFT_THREADS;
+      const bool in = row0 + e / M < rows;
+      const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
+      const float vx = z.x * norm, vy = z.y * norm;
+      const bool b0 = in && vx <= t, b1 = in && vy <= t;
+      const uint64_t m0 = __ballot(b0), m1 = __ballot(b1);
+      uint32_t pos = run_off[q * WAVES + wave] +
+                     (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below));
+      if (b0) {
+        lkeys[base + pos] = float_key(vx);
+        llocal[base + pos] = (uint16_t)(2 * e);
+        pos++;
+      }
+      if (b1) {
+        lkeys[base + pos] = float_key(vy);
+        llocal[base + pos] = (uint16_t)(2 * e + 1);
+      }
+    }
+    if (threadIdx.x == 0)
+      lcount[blockIdx.x] = run_off[PER * WAVES];
   }
 }
 
@@ -755,11 +826,20 @@ int pass_r2c_rows(const float* in, float2* spec) {
 }
 
 template <int N>
-int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s) {
+int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s,
+  const CellLists* lists, float t) {
   constexpr int TK2 = FFT_TILE / (N / 2);
+  static_assert((uint64_t)TK2 * N == LIST_BLOCK, "a rows block is one list block");
   const uint64_t rows = (uint64_t)N * N;
-  c2r_rows_kernel<N><<<(unsigned)((rows + TK2 - 1) / TK2), FFT_THREADS, 0, s>>>(spec, out,
-    rows, spectrum_pitch(N), spectrum_plane(N), norm, g_tw.fwd.as<float2>());
+  const unsigned blocks = (unsigned)((rows + TK2 - 1) / TK2);
+  if (lists)
+    c2r_rows_kernel<N, true><<<blocks, FFT_THREADS, 0, s>>>(spec, nullptr, rows,
+      spectrum_pitch(N), spectrum_plane(N), norm, g_tw.fwd.as<float2>(), t, lists->keys,
+      lists->local, lists->count);
+  else
+    c2r_rows_kernel<N, false><<<blocks, FFT_THREADS, 0, s>>>(spec, out, rows,
+      spectrum_pitch(N), spectrum_plane(N), norm, g_tw.fwd.as<float2>(), 0.0f, nullptr,
+      nullptr, nullptr);
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }
@@ -792,10 +872,11 @@ int run_r2c_rows(uint32_t n, const float* in, float2* spec) {
     return gen_rows(n, false, in, nullptr, spec, nullptr, 0.0f, stream());
   SIF_FFT_DISPATCH(n, pass_r2c_rows<N>(in, spec))
 }
-int run_c2r_rows(uint32_t n, const float2* spec, float* out, float norm, hipStream_t s) {
+int run_c2r_rows(uint32_t n, const float2* spec, float* out, float norm, hipStream_t s,
+  const CellLists* lists = nullptr, float t = 0.0f) {
   if (!native_pow2(n))
     return gen_rows(n, true, nullptr, spec, nullptr, out, norm, s);
-  SIF_FFT_DISPATCH(n, pass_c2r_rows<N>(spec, out, norm, s))
+  SIF_FFT_DISPATCH(n, pass_c2r_rows<N>(spec, out, norm, s, lists, t))
 }
 
 // The radices of a length: 8s, a 4, a 2, then each odd prime up to 31.
@@ -985,15 +1066,20 @@ int fft_r2c(const float* values, uint32_t n, float2* spec, float2* work) {
 }
 
 // The three inverse passes with a device-resident filter LUT (indexed by
-// |k|^2; null for none) and window axis (null for none).
-int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
-  float* out, float2* work, hipStream_t s) {
+// |k|^2; null for none) and window axis (null for none); the rows pass
+// writes the field (out) or, with lists, the lists of its cells <= t.
+static int c2r_lut_impl(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
+  float* out, const CellLists* lists, float t, float2* work, hipStream_t s) {
   int log2n;
   int st = check_n(n, &log2n);
   if (st == SIF_OK)
     st = twiddles(n);
   if (st != SIF_OK)
     return st;
+  if (lists && !native_pow2(n)) {
+    error_set(SIF_ERR_UNSUPPORTED, "fft: cell lists for n = %u (powers of two 16 to 1024)", n);
+    return SIF_ERR_UNSUPPORTED;
+  }
   const uint64_t nz = n / 2 + 1, pitch = spectrum_pitch(n), plane = spectrum_plane(n);
   const uint64_t rows = (uint64_t)n * n;
   const float norm = 1.0f / (float)((uint64_t)n * n * n);
@@ -1004,13 +1090,33 @@ int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double
   if (st == SIF_OK)
     st = run_cols(n, true, work, work, plane, pitch, nullptr, nullptr, s);
   if (st == SIF_OK)
-    st = run_c2r_rows(n, work, out, norm, s);
+    st = run_c2r_rows(n, work, out, norm, s, lists, t);
   if (st != SIF_OK)
     return st;
   timing_end(SIF_AMD_STAGE_SMOOTH, s);
-  if (timing_active())  // the spectrum once, the field once
-    timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + 4.0 * (double)rows * n);
+  // the spectrum once, and the field once (without lists; the lists' bytes
+  // are known only on the device)
+  if (timing_active())
+    timing_bytes(SIF_AMD_STAGE_SMOOTH, 8.0 * (double)rows * nz + (lists ? 0.0 : 4.0 * (double)rows * n));
   return SIF_OK;
+}
+
+int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
+  float* out, float2* work, hipStream_t s) {
+  return c2r_lut_impl(spec, n, d_lut, d_axis, out, nullptr, 0.0f, work, s);
+}
+
+int fft_c2r_lut_lists(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
+  float t, const CellLists& lists, float2* work, hipStream_t s) {
+  return c2r_lut_impl(spec, n, d_lut, d_axis, nullptr, &lists, t, work, s);
+}
+
+bool fft_lists_supported(uint32_t n) {
+  return native_pow2(n);
+}
+
+uint64_t list_blocks(uint32_t n) {
+  return ((uint64_t)n * n * n + LIST_BLOCK - 1) / LIST_BLOCK;
 }
 
 uint64_t spectrum_plane(uint32_t n) {

@@ -44,6 +44,26 @@ int fft_filter_c2r(const float2* spec, uint32_t n, float box, int filter,
 // entries) and window axis (n doubles); either may be null.
 int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
   float* out, float2* work, hipStream_t s);
+// A smoothing whose field is only scanned need not be written: the rows
+// pass lists each block's cells at or below the threshold instead, in cell
+// order. Block b covers cells [b LIST_BLOCK, (b + 1) LIST_BLOCK) of the flat
+// grid; its list holds count[b] entries, the cell's offset in the block
+// (local) and the order-preserving key of its value (float_key), at
+// [b LIST_BLOCK, b LIST_BLOCK + count[b]) of keys and local.
+constexpr uint32_t LIST_BLOCK = 16384;
+struct CellLists {
+  uint32_t* keys = nullptr;
+  uint16_t* local = nullptr;
+  uint32_t* count = nullptr;
+  uint64_t blocks = 0;
+};
+// Whether the rows pass can list cells for an n^3 grid (the power-of-two
+// lengths of the register kernels, 16 to 1024).
+bool fft_lists_supported(uint32_t n);
+uint64_t list_blocks(uint32_t n);
+// fft_c2r_lut with the field replaced by the lists of cells <= t.
+int fft_c2r_lut_lists(const float2* spec, uint32_t n, const float* d_lut, const double* d_axis,
+  float t, const CellLists& lists, float2* work, hipStream_t s);
 uint64_t filter_lut_size(uint32_t n);
 void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
   std::vector<float>& lut);
@@ -61,6 +81,10 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
   uint32_t* key_differ = nullptr);
 int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n,
   uint32_t key_differ = 0xffffffffu);
+// scan_candidates_flat (minima_n 0) from the lists a listing smoothing made
+// of the same field: the listed cells whose mask bit is clear.
+int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total,
+  uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count, uint32_t* key_differ);
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
 
 // grow.hip

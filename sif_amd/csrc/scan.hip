@@ -16,15 +16,6 @@
 
 namespace sif_amd {
 
-__device__ __forceinline__ uint32_t float_key(float f) {
-  const uint32_t b = __float_as_uint(f);
-  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-
-__device__ __forceinline__ float key_float(uint32_t k) {
-  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
-}
-
 // The scan is a two-pass compaction in tiles of 4096 consecutive cells: each
 // thread owns 16 consecutive cells (four float4 loads and one 16-bit slice of
 // the mask), a tile's count and then its in-tile offsets come from a block
@@ -279,6 +270,143 @@ int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t tota
   if (timing_active())  // the field and the mask once, the candidates once
     timing_bytes(SIF_AMD_STAGE_SCAN, 4.0 * (double)total + (double)total / 8.0 +
       (n <= cap ? 8.0 * (double)n : 0.0));
+  return SIF_OK;
+}
+
+// --- the same scan from cell lists (CellLists, a listing smoothing's) ---
+//
+// One workgroup per list block: the listed cells whose mask bit is clear,
+// counted (with their keys' AND and OR), then written in list order -- cell
+// order -- at the block's offset among all blocks. Every list is read twice,
+// nothing of the grid's size.
+__global__ void __launch_bounds__(SC_THREADS) list_count_kernel(
+  const uint32_t* __restrict__ lkeys, const uint16_t* __restrict__ llocal,
+  const uint32_t* __restrict__ lcount, const uint64_t* __restrict__ mask,
+  uint64_t* __restrict__ tile_counts, uint32_t* __restrict__ tile_span,
+  unsigned long long* __restrict__ listed) {
+  __shared__ uint32_t w_and[SC_THREADS / 64], w_or[SC_THREADS / 64], w_cnt[SC_THREADS / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * LIST_BLOCK;
+  const uint32_t c = lcount[blockIdx.x];
+  uint32_t cnt = 0, k_and = 0xffffffffu, k_or = 0u;
+  for (uint32_t i = threadIdx.x; i < c; i += SC_THREADS) {
+    const uint64_t cell = base + llocal[base + i];
+    if (!((mask[cell >> 6] >> (cell & 63)) & 1ull)) {
+      const uint32_t k = lkeys[base + i];
+      cnt++;
+      k_and &= k;
+      k_or |= k;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt += __shfl_xor(cnt, off, 64);
+    k_and &= __shfl_xor(k_and, off, 64);
+    k_or |= __shfl_xor(k_or, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    w_and[threadIdx.x >> 6] = k_and;
+    w_or[threadIdx.x >> 6] = k_or;
+    w_cnt[threadIdx.x >> 6] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = 0xffffffffu, o = 0u, all = 0;
+    for (int w = 0; w < SC_THREADS / 64; w++) {
+      a &= w_and[w];
+      o |= w_or[w];
+      all += w_cnt[w];
+    }
+    tile_counts[blockIdx.x] = all;
+    tile_span[2 * (uint64_t)blockIdx.x] = a;
+    tile_span[2 * (uint64_t)blockIdx.x + 1] = o;
+    if (c)
+      atomicAdd(listed, (unsigned long long)c);
+  }
+}
+
+__global__ void __launch_bounds__(SC_THREADS) list_write_kernel(
+  const uint32_t* __restrict__ lkeys, const uint16_t* __restrict__ llocal,
+  const uint32_t* __restrict__ lcount, const uint64_t* __restrict__ mask,
+  const uint64_t* __restrict__ tile_offsets, uint32_t* __restrict__ keys,
+  uint32_t* __restrict__ flat) {
+  const uint64_t base = (uint64_t)blockIdx.x * LIST_BLOCK;
+  const uint32_t c = lcount[blockIdx.x];
+  uint64_t pos = tile_offsets[blockIdx.x];
+  // chunks of SC_THREADS entries in list order; a block scan places each
+  // kept one (every thread runs every chunk: the scan's barriers)
+  for (uint32_t i0 = 0; i0 < c; i0 += SC_THREADS) {
+    const uint32_t i = i0 + threadIdx.x;
+    uint64_t cell = 0;
+    bool keep = false;
+    if (i < c) {
+      cell = base + llocal[base + i];
+      keep = !((mask[cell >> 6] >> (cell & 63)) & 1ull);
+    }
+    uint32_t all;
+    const uint32_t at = sc_block_scan(keep ? 1u : 0u, &all);
+    if (keep) {
+      keys[pos + at] = lkeys[base + i];
+      flat[pos + at] = (uint32_t)cell;
+    }
+    pos += all;
+  }
+}
+
+int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total,
+  uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count, uint32_t* key_differ) {
+  if (total >= 0x80000000ull) {
+    error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^31 - 1)",
+      (unsigned long long)total);
+    return SIF_ERR_RANGE;
+  }
+  const uint64_t tiles = lists.blocks;
+  int st = g_tiles.reserve((tiles ? tiles : 1) * 8);
+  if (st == SIF_OK)
+    st = g_tile_pos.reserve((tiles ? tiles : 1) * 8);
+  if (st == SIF_OK)
+    st = g_count.reserve(32);  // the count, the keys' AND and OR, the cells listed
+  if (st == SIF_OK)
+    st = g_tile_span.reserve((tiles ? tiles : 1) * 8);
+  if (st != SIF_OK)
+    return st;
+  *count = 0;
+  if (key_differ)
+    *key_differ = 0xffffffffu;
+  if (tiles == 0)
+    return SIF_OK;
+  timing_begin(SIF_AMD_STAGE_SCAN);
+  SIF_HIP(hipMemsetAsync(g_count.as<uint64_t>() + 2, 0, 8, stream()));
+  list_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(lists.keys, lists.local,
+    lists.count, mask, g_tiles.as<uint64_t>(), g_tile_span.as<uint32_t>(),
+    reinterpret_cast<unsigned long long*>(g_count.as<uint64_t>() + 2));
+  SIF_HIP(hipGetLastError());
+  span_reduce_kernel<<<1, 1024, 0, stream()>>>(g_tile_span.as<uint32_t>(), tiles,
+    g_count.as<uint32_t>() + 2);
+  SIF_HIP(hipGetLastError());
+  st = exclusive_scan_u64(g_tiles.as<uint64_t>(), tiles, g_tile_pos.as<uint64_t>(),
+    g_count.as<uint64_t>());
+  if (st != SIF_OK)
+    return st;
+  // the count; the keys' AND (low) and OR (high); the cells listed
+  uint64_t back[3] = {0, 0, 0};
+  SIF_HIP(hipMemcpyAsync(back, g_count.ptr, 24, hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipStreamSynchronize(stream()));
+  const uint64_t n = back[0];
+  *count = n;
+  if (key_differ)
+    *key_differ = (uint32_t)back[1] ^ (uint32_t)(back[1] >> 32);
+  if (n <= cap && n > 0) {
+    list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(lists.keys, lists.local,
+      lists.count, mask, g_tile_pos.as<uint64_t>(), keys, flat);
+    SIF_HIP(hipGetLastError());
+  }
+  timing_end(SIF_AMD_STAGE_SCAN);
+  if (timing_active()) {
+    // the lists twice and the candidates once; the smoothing that made the
+    // lists wrote them once
+    const double listed = (double)back[2];
+    timing_bytes(SIF_AMD_STAGE_SCAN, 12.0 * listed + (n <= cap ? 8.0 * (double)n : 0.0));
+    timing_bytes(SIF_AMD_STAGE_SMOOTH, 6.0 * listed);
+  }
   return SIF_OK;
 }
 
