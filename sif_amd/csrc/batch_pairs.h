@@ -37,6 +37,41 @@ struct PairEffect {
   uint8_t state : 2, state_r : 2;
 };
 
+// A candidate's effects count up to BATCH_EFF_FULL, which stands for "that
+// many or more": such a candidate, or one whose effects run past the batch's
+// list, probes the commit's accepted voids instead.
+constexpr uint32_t BATCH_EFF_FULL = 255;
+// The batch's list holds this many effects a candidate on average (a few
+// times what a batch of Zel'dovich candidates needs).
+constexpr uint32_t BATCH_EFF_PER = 4;
+
+// What batch_pairs() needs of the run and the rung.
+struct BatchPairParams {
+  uint32_t n;        // grid side
+  float l, box, f, scale;
+  float radius;      // the rung's
+  float cat_max_r;   // the catalogue's largest radius as the batch is launched
+  float inv_l;       // VOID_CELLS / box
+  int32_t proxy;     // the rung's pole distance in nodes
+};
+
+// Device scratch and results of batch_pairs(): per candidate its count
+// (cnt8, read by the host) and offset into eff (the exclusive sum of the
+// counts in batch order, which the host can sum again from cnt8).
+struct BatchPairScratch {
+  float* set_max_r;   // 1: the batch's largest grown radius
+  uint32_t* cnt32;    // count
+  uint8_t* cnt8;      // count
+  uint64_t* offset;   // count
+  PairEffect* eff;    // capacity
+  uint64_t capacity;
+};
+
+// For each of `count` grown candidates (device: flat nodes, grown radii, the
+// grown-overlap states), its effects from the earlier ones (batch_pairs.hip).
+int batch_pairs(const uint32_t* flat, const float* rad, const uint32_t* grown_state,
+  uint32_t count, const BatchPairParams& P, BatchPairScratch& W);
+
 struct PairQuery {
   uint32_t b;              // the candidate's place in the batch
   uint32_t ix, iy, iz;     // its node

@@ -117,6 +117,11 @@ WorkPool& host_pool() {
 // inputs and results, and the event that marks the results in host memory.
 struct BatchSlot {
   DeviceBuffer d_flat, d_cx, d_cy, d_cz, d_rad, d_reason, d_nin, d_state, d_algo;
+  // the batch's pair effects (batch_pairs.hip)
+  DeviceBuffer d_eff, d_cnt8, d_cnt32, d_off, d_setmax;
+  PairEffect* h_eff = nullptr;
+  uint8_t* h_cnt = nullptr;
+  uint64_t eff_capacity = 0;  // effects downloaded with the batch
   uint32_t* h_flat = nullptr;
   float* h_rad = nullptr;
   uint32_t* h_reason = nullptr;
@@ -149,6 +154,14 @@ int reserve_slots() {
     if (e == SIF_OK) e = B.d_nin.reserve(BATCH_ROOM * 8);
     if (e == SIF_OK) e = B.d_state.reserve(BATCH_ROOM * 4);
     if (e == SIF_OK) e = B.d_algo.reserve(BATCH_ROOM * 8);
+    if (e == SIF_OK) e = B.d_eff.reserve((size_t)BATCH_ROOM * BATCH_EFF_PER * sizeof(PairEffect));
+    if (e == SIF_OK) e = B.d_cnt8.reserve(BATCH_ROOM);
+    if (e == SIF_OK) e = B.d_cnt32.reserve(BATCH_ROOM * 4);
+    if (e == SIF_OK) e = B.d_off.reserve(BATCH_ROOM * 8);
+    if (e == SIF_OK) e = B.d_setmax.reserve(sizeof(float));
+    if (e == SIF_OK)
+      e = host_alloc((void**)&B.h_eff, (size_t)BATCH_ROOM * BATCH_EFF_PER * sizeof(PairEffect));
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_cnt, BATCH_ROOM);
     if (e == SIF_OK) e = host_alloc((void**)&B.h_flat, BATCH_ROOM * 4);
     if (e == SIF_OK) e = host_alloc((void**)&B.h_rad, BATCH_ROOM * 4);
     if (e == SIF_OK) e = host_alloc((void**)&B.h_reason, BATCH_ROOM * 4);
@@ -492,25 +505,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   RungVoids CL;  // the voids accepted by the commit under way
   CL.init(R.box);
   CL.columns = false;
-  RungVoids BI;  // the batch's grown candidates, as voids (batch_pairs.h)
-  BI.init(R.box);
-  BI.columns = false;
-  std::vector<uint32_t> bi_place;
-  struct Node {
-    uint32_t ix, iy, iz;
-    float x, y, z;
-  };
-  std::vector<Node> nodes;      // a batch's candidates: node and centre
   std::vector<uint32_t> taken;  // a commit's accepted candidates, in order
-  // A candidate's tests against the rung's voids as its commit begins, and
-  // the effects of the batch's earlier candidates on it (n_eff < 0: more
-  // than fit, probed in order instead)
-  constexpr int EFF_CAP = 12;
+  // A candidate's tests against the rung's voids as its commit begins
   struct PreProbe {
     bool node, pole, pole_r;
     int8_t state, state_r;
-    int8_t n_eff;
-    PairEffect eff[EFF_CAP];
   };
   std::vector<PreProbe> pre;
   std::vector<uint8_t> is_taken;  // accepted, by batch place
@@ -992,8 +991,33 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
           B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
           g_dev.mask.as<uint64_t>(), n, l, B.d_state.as<uint32_t>());
+      if (e == SIF_OK) {
+        BatchPairParams bp;
+        bp.n = n;
+        bp.l = l;
+        bp.box = R.box;
+        bp.f = R.f;
+        bp.scale = scale;
+        bp.radius = radius;
+        bp.cat_max_r = V.max_r;
+        bp.inv_l = V.inv_l;
+        bp.proxy = proxy;
+        BatchPairScratch W;
+        W.set_max_r = B.d_setmax.as<float>();
+        W.cnt32 = B.d_cnt32.as<uint32_t>();
+        W.cnt8 = B.d_cnt8.as<uint8_t>();
+        W.offset = B.d_off.as<uint64_t>();
+        W.eff = B.d_eff.as<PairEffect>();
+        W.capacity = (uint64_t)bc * BATCH_EFF_PER;
+        B.eff_capacity = W.capacity;
+        e = batch_pairs(B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), B.d_state.as<uint32_t>(),
+          bc, bp, W);
+      }
       if (e != SIF_OK)
         return e;
+      SIF_HIP(hipMemcpyAsync(B.h_cnt, B.d_cnt8.ptr, bc, hipMemcpyDeviceToHost, stream()));
+      SIF_HIP(hipMemcpyAsync(B.h_eff, B.d_eff.ptr, B.eff_capacity * sizeof(PairEffect),
+        hipMemcpyDeviceToHost, stream()));
       SIF_HIP(hipMemcpyAsync(B.h_rad, B.d_rad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
       SIF_HIP(hipMemcpyAsync(B.h_reason, B.d_reason.ptr, bc * 4, hipMemcpyDeviceToHost,
         stream()));
@@ -1028,40 +1052,27 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
             (double)radius);
           return SIF_ERR_RANGE;
         }
-      // the nodes and the batch's own index, on this thread: a few
-      // nanoseconds a candidate, less than a pool run's wake-up
-      nodes.resize(B.count);
-      BI.clear();
-      bi_place.clear();
-      for (uint32_t b = 0; b < B.count; b++) {
-        Node& N = nodes[b];
-        node_of(B.h_flat[b], &N.ix, &N.iy, &N.iz);
-        N.x = (float)N.ix * l;
-        N.y = (float)N.iy * l;
-        N.z = (float)N.iz * l;
-        if (B.h_rad[b] >= 0.0f) {
-          BI.add(N.x, N.y, N.z, B.h_rad[b]);
-          bi_place.push_back(b);
-        }
-      }
       const float cat_max_r = V.max_r;
       const double t_idx = now_ms();
+      // The rung's newer voids (RV) for every candidate at once. Each thread
+      // decodes the nodes it needs itself: a table of nodes written by one
+      // thread and read by the others costs a cache line's trip between
+      // cores per few candidates.
       pre.resize(B.count);
       host_pool().run(B.count, [&](uint64_t b0, uint64_t b1) {
         for (uint64_t b = b0; b < b1; b++) {
-          const Node& N = nodes[b];
-          const bool edge = B.edge[b] != 0;
+          uint32_t ix, iy, iz;
+          node_of(B.h_flat[b], &ix, &iy, &iz);
+          const float x = (float)ix * l, y = (float)iy * l, z = (float)iz * l;
           OverlapQuery q;
-          overlap_query(q, N.x, N.y, N.z, radius, cat_max_r, R.box, R.f, scale, V.inv_l);
-          const RungProbe p1 =
-            RV.probe(&q, N.ix, N.iy, N.iz, N.x, N.y, N.z, true, proxy, scale, l, n);
-          PreProbe& P = pre[b];
+          overlap_query(q, x, y, z, radius, cat_max_r, R.box, R.f, scale, V.inv_l);
+          const RungProbe p1 = RV.probe(&q, ix, iy, iz, x, y, z, true, proxy, scale, l, n);
+          PreProbe P;
           P.node = p1.node;
           P.pole = p1.pole;
           P.state = (int8_t)p1.state;
           P.pole_r = false;
           P.state_r = 0;
-          P.n_eff = 0;
           const float r = B.h_rad[b];
           const int32_t pole_r = r >= 0.0f ? (int32_t)(scale * r * (1.0f - R.f) / l) - 1 : 0;
           const uint32_t gs = B.h_state[b];
@@ -1072,72 +1083,55 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
               P.pole_r = true;
             } else {
               OverlapQuery q2;
-              overlap_query(q2, N.x, N.y, N.z, r, cat_max_r, R.box, R.f, scale, V.inv_l);
-              const RungProbe p2 = RV.probe((gs & 3u) == 1u ? nullptr : &q2, N.ix, N.iy, N.iz,
-                N.x, N.y, N.z, false, pole_r, scale, l, n);
+              overlap_query(q2, x, y, z, r, cat_max_r, R.box, R.f, scale, V.inv_l);
+              const RungProbe p2 =
+                RV.probe((gs & 3u) == 1u ? nullptr : &q2, ix, iy, iz, x, y, z, false, pole_r,
+                  scale, l, n);
               P.pole_r = p2.pole;
               P.state_r = (int8_t)p2.state;
             }
           }
-          // what the batch's earlier candidates can still change, test by
-          // test in the commit's order: node, pole, overlap, then at the
-          // grown radius pole and overlap
-          uint8_t need = 0;
-          if (!P.node) {
-            need |= NEED_NODE;
-            if (!(proxy > 0 && P.pole)) {
-              if (proxy > 0)
-                need |= NEED_POLE;
-              if (!edge && P.state != 1)
-                need |= NEED_STATE;
-              if (r >= 0.0f && P.state != 1) {
-                const bool pole_refused = pole_r > 0 && ((gs & GROWN_POLE_BIT) || P.pole_r);
-                if (pole_r > 0 && !pole_refused)
-                  need |= NEED_POLE_R;
-                if (!pole_refused && (gs & 3u) == 0u && P.state_r != 1)
-                  need |= NEED_STATE_R;
-              }
-            }
-          }
-          PairQuery pq;
-          pq.b = (uint32_t)b;
-          pq.ix = N.ix;
-          pq.iy = N.iy;
-          pq.iz = N.iz;
-          pq.x = N.x;
-          pq.y = N.y;
-          pq.z = N.z;
-          pq.r = r;
-          pq.pole = proxy;
-          pq.pole_r = pole_r;
-          pq.need = need;
-          P.n_eff = (int8_t)pair_effects(BI, bi_place.data(), pq, radius, cat_max_r, R.f, scale,
-            l, n, P.eff, EFF_CAP);
+          pre[b] = P;
         }
       });
-      // In order. The voids accepted join the catalogue at once; the list of
-      // this commit's voids (CL) is filled only when a candidate with more
-      // effects than were listed needs it.
+      // In order. The voids accepted join the catalogue at once; the effects
+      // of the batch's earlier candidates come listed from the device
+      // (batch_pairs.hip); the list of this commit's voids (CL) is filled only
+      // when a candidate with more effects than were listed needs it.
       const double t3 = now_ms();
       CL.clear();
       size_t cl_filled = 0;
       taken.clear();
       is_taken.assign(B.count, 0);
       uint64_t accepted = 0;
+      uint64_t eff_at = 0;  // the batch's effects list: this candidate's offset
+      auto centre_of = [&](uint32_t b, uint32_t* ix, uint32_t* iy, uint32_t* iz, float* x,
+                         float* y, float* z) {
+        node_of(B.h_flat[b], ix, iy, iz);
+        *x = (float)*ix * l;
+        *y = (float)*iy * l;
+        *z = (float)*iz * l;
+      };
       for (uint32_t b = 0; b < B.count; b++) {
-        const PreProbe& P = pre[b];
+        const PreProbe P = pre[b];
+        const uint32_t n_eff = B.h_cnt[b];
+        const uint64_t eff0 = eff_at;
+        eff_at += n_eff;
         if (P.node) {
           count_refusal(1);
           continue;
         }
-        const Node& N = nodes[b];
+        uint32_t ix, iy, iz;
+        float x, y, z;
+        centre_of(b, &ix, &iy, &iz, &x, &y, &z);
         const float r = B.h_rad[b];
         const int32_t pole = r >= 0.0f ? (int32_t)(scale * r * (1.0f - R.f) / l) - 1 : 0;
         // the effects of this commit's voids: listed, or probed when too many
         RungProbe c1, c2;
-        if (P.n_eff >= 0) {
-          for (int e = 0; e < P.n_eff; e++) {
-            const PairEffect& E = P.eff[e];
+        if (n_eff < BATCH_EFF_FULL && eff0 + n_eff <= B.eff_capacity) {
+          const PairEffect* eff = B.h_eff + eff0;
+          for (uint32_t e = 0; e < n_eff; e++) {
+            const PairEffect E = eff[e];
             if (!is_taken[E.a])
               continue;
             c1.node |= E.node;
@@ -1149,19 +1143,22 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         } else {
           for (; cl_filled < taken.size(); cl_filled++) {
             const uint32_t a = taken[cl_filled];
-            CL.add(nodes[a].x, nodes[a].y, nodes[a].z, B.h_rad[a]);
+            uint32_t jx, jy, jz;
+            float ax, ay, az;
+            centre_of(a, &jx, &jy, &jz, &ax, &ay, &az);
+            CL.add(ax, ay, az, B.h_rad[a]);
           }
           const bool pole_refuses = proxy > 0 && P.pole;
           OverlapQuery q;
-          overlap_query(q, N.x, N.y, N.z, radius, V.max_r, R.box, R.f, scale, V.inv_l);
-          c1 = CL.probe(pole_refuses || P.state == 1 ? nullptr : &q, N.ix, N.iy, N.iz, N.x, N.y,
-            N.z, true, pole_refuses ? 0 : proxy, scale, l, n);
+          overlap_query(q, x, y, z, radius, V.max_r, R.box, R.f, scale, V.inv_l);
+          c1 = CL.probe(pole_refuses || P.state == 1 ? nullptr : &q, ix, iy, iz, x, y, z, true,
+            pole_refuses ? 0 : proxy, scale, l, n);
           if (!c1.node && !(proxy > 0 && (P.pole || c1.pole)) && r >= 0.0f) {
             const uint32_t gs = B.h_state[b];
             OverlapQuery q2;
-            overlap_query(q2, N.x, N.y, N.z, r, V.max_r, R.box, R.f, scale, V.inv_l);
-            c2 = CL.probe((gs & 3u) == 1u || P.state_r == 1 ? nullptr : &q2, N.ix, N.iy, N.iz,
-              N.x, N.y, N.z, false, pole, scale, l, n);
+            overlap_query(q2, x, y, z, r, V.max_r, R.box, R.f, scale, V.inv_l);
+            c2 = CL.probe((gs & 3u) == 1u || P.state_r == 1 ? nullptr : &q2, ix, iy, iz, x, y,
+              z, false, pole, scale, l, n);
           }
         }
         // the rung-radius tests (mask, pole probe, overlap)
@@ -1173,8 +1170,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           count_refusal(2);
           continue;
         }
-        if (exact_overlap(N.x, N.y, N.z, radius, B.edge[b] ? 2u : 0u,
-              combine(P.state, c1.state))) {
+        if (exact_overlap(x, y, z, radius, B.edge[b] ? 2u : 0u, combine(P.state, c1.state))) {
           count_refusal(3);
           continue;
         }
@@ -1188,11 +1184,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           S.proxy++;
           continue;
         }
-        if (exact_overlap(N.x, N.y, N.z, r, gs & 3u, combine(P.state_r, c2.state))) {
+        if (exact_overlap(x, y, z, r, gs & 3u, combine(P.state_r, c2.state))) {
           S.overlap++;
           continue;
         }
-        V.add(N.x, N.y, N.z, r, B.h_nin[b]);
+        V.add(x, y, z, r, B.h_nin[b]);
         if (r / radius > 2.0f)  // exodus_finder.c:2321-2328
           mismatched++;
         is_taken[b] = 1;
@@ -1200,8 +1196,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         accepted++;
       }
       // the rung's list, for the next commits' probes and the mask
-      for (uint32_t a : taken)
-        RV.add(nodes[a].x, nodes[a].y, nodes[a].z, B.h_rad[a]);
+      for (uint32_t a : taken) {
+        uint32_t jx, jy, jz;
+        float ax, ay, az;
+        centre_of(a, &jx, &jy, &jz, &ax, &ay, &az);
+        RV.add(ax, ay, az, B.h_rad[a]);
+      }
       S.accepted += accepted;
       *accepted_out = accepted;
       if (timing_active()) {

@@ -26,7 +26,7 @@ struct RungProbe {
 // The six nodes `pole` away from (ix, iy, iz) along the axes, wrapped.
 struct PoleNodes {
   uint32_t px[2], py[2], pz[2];
-  void set(uint32_t ix, uint32_t iy, uint32_t iz, int32_t pole, uint32_t n) {
+  SIF_HD void set(uint32_t ix, uint32_t iy, uint32_t iz, int32_t pole, uint32_t n) {
     px[0] = paint_wrap((int32_t)ix + pole, (int32_t)n);
     px[1] = paint_wrap((int32_t)ix - pole, (int32_t)n);
     py[0] = paint_wrap((int32_t)iy + pole, (int32_t)n);
@@ -34,7 +34,7 @@ struct PoleNodes {
     pz[0] = paint_wrap((int32_t)iz + pole, (int32_t)n);
     pz[1] = paint_wrap((int32_t)iz - pole, (int32_t)n);
   }
-  bool painted(const PaintSphere& P, uint32_t ix, uint32_t iy, uint32_t iz) const {
+  SIF_HD bool painted(const PaintSphere& P, uint32_t ix, uint32_t iy, uint32_t iz) const {
     return paint_hits(P, px[0], iy, iz) || paint_hits(P, px[1], iy, iz) ||
            paint_hits(P, ix, py[0], iz) || paint_hits(P, ix, py[1], iz) ||
            paint_hits(P, ix, iy, pz[0]) || paint_hits(P, ix, iy, pz[1]);
@@ -44,8 +44,8 @@ struct PoleNodes {
 // Cells of the 32^3 list, per axis before wrapping: a union of reaches.
 struct CellRegion {
   int32_t lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
-  bool empty() const { return hi[0] < lo[0]; }
-  void merge(const int32_t l3[3], const int32_t h3[3]) {
+  SIF_HD bool empty() const { return hi[0] < lo[0]; }
+  SIF_HD void merge(const int32_t l3[3], const int32_t h3[3]) {
     for (int a = 0; a < 3; a++) {
       if (hi[a] < lo[a]) {
         lo[a] = l3[a];
@@ -62,14 +62,14 @@ struct CellRegion {
   }
   // every void whose overlap with q may hold, for a set of largest radius
   // set_max_r (overlap.h)
-  void add_overlap(const OverlapQuery& q, float set_max_r, float inv_l) {
+  SIF_HD void add_overlap(const OverlapQuery& q, float set_max_r, float inv_l) {
     int32_t l3[3], h3[3];
     superset_scan(q, set_max_r, inv_l, l3, h3);
     merge(l3, h3);
   }
   // every void that may paint a node within `reach` of (x, y, z) on each
   // axis, widened by a margin far above float rounding
-  void add_paint(float x, float y, float z, float reach, float box, float inv_l) {
+  SIF_HD void add_paint(float x, float y, float z, float reach, float box, float inv_l) {
     const float s = reach + reach * 1e-5f + box * 1e-4f;
     const float c3[3] = {x, y, z};
     int32_t l3[3], h3[3];
@@ -78,6 +78,17 @@ struct CellRegion {
       h3[a] = (int32_t)floorf((c3[a] + s) * inv_l);
     }
     merge(l3, h3);
+  }
+  // Does the region hold wrapped cell (wa, wb, wc)? The same cells as a scan
+  // of the region visits.
+  SIF_HD bool holds(uint32_t wa, uint32_t wb, uint32_t wc) const {
+    if (empty())
+      return false;
+    const uint32_t w[3] = {wa, wb, wc};
+    for (int a = 0; a < 3; a++)
+      if (((w[a] - void_wrap(lo[a])) & (uint32_t)(VOID_CELLS - 1)) > (uint32_t)(hi[a] - lo[a]))
+        return false;
+    return true;
   }
 };
 
