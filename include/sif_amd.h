@@ -237,8 +237,11 @@ SIF_API int sif_amd_mark_spheres(uint64_t* d_mask_words, uint32_t n_cells, float
 
 /* sif__exodus_run (src/finder/exodus_finder.c:1585-2492) for a periodic box,
  * SIF_FINDER_SEEDS_CELLS on their nodes, unweighted: the grid d_delta (a
- * density contrast, n_cells^3, left as the last rung's smoothing unless
- * restore) and the mesh arrays are device memory. n_in (optional, host,
+ * density contrast, n_cells^3) and the mesh arrays are device memory. On a
+ * power-of-two grid (16 to 1024) without SIF_FINDER_SEEDS_MINIMA the rungs
+ * list their candidate cells instead of smoothing the grid, which is left as
+ * it was; otherwise it is left as the last rung's smoothing unless restore
+ * (then transformed back, to float precision). n_in (optional, host,
  * capacity of the result) receives each void's enclosed count. stats
  * (optional, host, n_radii entries) receives the per-rung counters in ladder
  * order (largest first). */
