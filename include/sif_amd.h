@@ -344,6 +344,13 @@ SIF_API int sif_amd_squared_residual(const float* d_p, const float* d_target,
  * restores the defaults. Lets small inputs cross batch boundaries. Not
  * synchronised, like the reference's settings table. */
 SIF_API int sif_amd_set_batch_window(uint32_t floor, uint32_t ceiling);
+/* Whether later exodus runs in this process may list each rung's candidate
+ * cells in the smoothing instead of writing the smoothed field (on by
+ * default; power-of-two grids without SIF_FINDER_SEEDS_MINIMA). The
+ * catalogue is the same either way; off, the grid is smoothed in place and
+ * restored as the reference does. No reference counterpart. Not
+ * synchronised. */
+SIF_API int sif_amd_set_cell_lists(int on);
 
 /* --- the batched form (no reference counterpart: many independent boxes,
  * each the catalogue sif_finder_exodus_raw would give it) --- */

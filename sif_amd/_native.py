@@ -133,6 +133,7 @@ _sig("sif_amd_exodus_survey_device", C.POINTER(Catalogue), P, P, u32, f32, C.POI
      C.POINTER(MeshDesc), C.POINTER(f32), u32, f32, f32, u32, i32, C.POINTER(C.POINTER(u64)),
      C.POINTER(RungStats))
 _sig("sif_amd_set_batch_window", i32, u32, u32)
+_sig("sif_amd_set_cell_lists", i32, i32)
 
 
 class Box(C.Structure):  # sif_amd_box_t

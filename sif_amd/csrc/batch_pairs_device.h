@@ -1,0 +1,119 @@
+// The device text of the batch's pair effects (batch_pairs.hip launches it;
+// tests/emu runs the same text on the CPU under a wave emulator). One
+// wavefront per candidate b, its lanes over the earlier candidates a < b.
+#pragma once
+#include <cstdint>
+
+#include "batch_pairs.h"
+
+namespace sif_amd {
+
+constexpr int BP_WAVES = 4;
+
+// COUNT: each candidate's number of effects (at most BATCH_EFF_FULL, which
+// stands for "that many or more"). WRITE: its effects, in the order of the
+// earlier candidates, at its offset (the exclusive sum of the counts) in one
+// list of `capacity` effects; a candidate at BATCH_EFF_FULL, or whose effects
+// run past the capacity, writes none.
+template <bool WRITE>
+__global__ void __launch_bounds__(64 * BP_WAVES) batch_pairs_kernel(
+  const uint32_t* __restrict__ flat, const float* __restrict__ rad,
+  const uint32_t* __restrict__ grown_state, uint32_t count, BatchPairParams P,
+  const float* __restrict__ set_max_r, uint32_t* __restrict__ cnt32,
+  uint8_t* __restrict__ cnt8, const uint64_t* __restrict__ offset,
+  PairEffect* __restrict__ eff, uint64_t capacity) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t b = blockIdx.x * BP_WAVES + (threadIdx.x >> 6);
+  if (b >= count)
+    return;
+  uint64_t at0 = 0;
+  if constexpr (WRITE) {
+    at0 = offset[b];
+    const uint32_t c = cnt8[b];
+    if (c == 0 || c >= BATCH_EFF_FULL || at0 + c > capacity)
+      return;
+  }
+  const uint32_t n = P.n, nn = n * n;
+  const uint32_t fb = flat[b];
+  const uint32_t ix = fb / nn, iy = (fb / n) % n, iz = fb % n;
+  const float x = (float)ix * P.l, y = (float)iy * P.l, z = (float)iz * P.l;
+  const float r = rad[b];
+  const uint32_t gs = grown_state[b];
+  const float bmax = *set_max_r;
+  const int32_t pole_r = r >= 0.0f ? (int32_t)(P.scale * r * (1.0f - P.f) / P.l) - 1 : 0;
+  // every test b may still need (batch_pairs.h NEED_*), as far as the
+  // device can tell
+  const bool want_node = true;
+  const bool want_pole = P.proxy > 0;
+  const bool need_state = true;
+  const bool want_pole_r = r >= 0.0f && pole_r > 0 && !(gs & GROWN_POLE_BIT);
+  const bool need_state_r = r >= 0.0f && (gs & 3u) == 0u;
+  OverlapQuery q, qr;
+  CellRegion R;
+  if (bmax >= 0.0f) {
+    if (need_state) {
+      overlap_query(q, x, y, z, P.radius, P.cat_max_r, P.box, P.f, P.scale, P.inv_l);
+      R.add_overlap(q, bmax, P.inv_l);
+    }
+    if (need_state_r) {
+      overlap_query(qr, x, y, z, r, P.cat_max_r, P.box, P.f, P.scale, P.inv_l);
+      R.add_overlap(qr, bmax, P.inv_l);
+    }
+    const float paint = P.scale * bmax + 2.0f * P.l;
+    if (want_node || want_pole)
+      R.add_paint(x, y, z, paint + (want_pole ? (float)P.proxy * P.l : 0.0f), P.box, P.inv_l);
+    if (want_pole_r)
+      R.add_paint(x, y, z, paint + (float)pole_r * P.l, P.box, P.inv_l);
+  }
+  PoleNodes poles, poles_r;
+  if (want_pole)
+    poles.set(ix, iy, iz, P.proxy, n);
+  if (want_pole_r)
+    poles_r.set(ix, iy, iz, pole_r, n);
+  uint32_t found = 0;  // wave-uniform
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t a0 = 0; a0 < b && found < BATCH_EFF_FULL && !R.empty(); a0 += 64) {
+    const uint32_t a = a0 + lane;
+    PairEffect e{};
+    bool any = false;
+    if (a < b) {
+      const float ra = rad[a];
+      if (ra >= 0.0f) {
+        const uint32_t fa = flat[a];
+        const uint32_t jx = fa / nn, jy = (fa / n) % n, jz = fa % n;
+        const float ax = (float)jx * P.l, ay = (float)jy * P.l, az = (float)jz * P.l;
+        const uint32_t wa = void_axis_cell(ax, P.inv_l), wb = void_axis_cell(ay, P.inv_l),
+                       wc = void_axis_cell(az, P.inv_l);
+        if (R.holds(wa, wb, wc)) {
+          e.a = a;
+          if (need_state)
+            e.state = (uint8_t)pair_state(q, wa, wb, wc, ax, ay, az, ra);
+          if (need_state_r)
+            e.state_r = (uint8_t)pair_state(qr, wa, wb, wc, ax, ay, az, ra);
+          PaintSphere S;
+          paint_sphere(S, ax, ay, az, P.scale * ra, P.l, n);
+          e.node = want_node && paint_hits(S, ix, iy, iz);
+          e.pole = want_pole && poles.painted(S, ix, iy, iz);
+          e.pole_r = want_pole_r && poles_r.painted(S, ix, iy, iz);
+          any = e.node | e.pole | e.pole_r | e.state | e.state_r;
+        }
+      }
+    }
+    const uint64_t m = __ballot(any);
+    if constexpr (WRITE) {
+      const uint32_t at = found + (uint32_t)__popcll(m & below);
+      if (any)
+        eff[at0 + at] = e;
+    }
+    found += (uint32_t)__popcll(m);
+  }
+  if constexpr (!WRITE) {
+    if (lane == 0) {
+      const uint32_t c = found < BATCH_EFF_FULL ? found : BATCH_EFF_FULL;
+      cnt32[b] = c;
+      cnt8[b] = (uint8_t)c;
+    }
+  }
+}
+
+}  // namespace sif_amd

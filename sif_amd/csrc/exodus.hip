@@ -57,6 +57,9 @@ constexpr uint32_t BATCH_FLOOR_DEFAULT = 4096;
 constexpr uint32_t BATCH_ROOM = 32768;
 uint32_t g_batch_floor = BATCH_FLOOR_DEFAULT;
 uint32_t g_batch_max = BATCH_ROOM;
+// Smoothings that only feed the scan list their candidate cells
+// (sif_amd_set_cell_lists).
+bool g_cell_lists = true;
 
 // A scan of the void cell list made while voids were left unlinked would have
 // missed them: the run fails rather than return a catalogue built on it.
@@ -186,7 +189,7 @@ double now_ms() {
 // next rung is built from it and the mask is painted from it.
 struct DeviceVoids {
   DeviceBuffer x, y, z, r;
-  DeviceBuffer start, items;
+  DeviceBuffer start, items, occupied;
   uint32_t n = 0, cap = 0;
   // pinned staging for an append
   float* hx = nullptr;
@@ -288,6 +291,8 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   int st = D.start.reserve((size_t)(slots + 1) * 4);
   if (st == SIF_OK)
     st = D.items.reserve((size_t)(D.cap ? D.cap : 1) * 4);
+  if (st == SIF_OK)
+    st = D.occupied.reserve((size_t)slots / 32 * 4);
   if (st == SIF_OK && D.n == 0) {  // keep the pointers valid
     st = D.x.reserve(4);
     if (st == SIF_OK) st = D.y.reserve(4);
@@ -296,7 +301,7 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   }
   if (st == SIF_OK)
     st = build_void_index(D.x.as<float>(), D.y.as<float>(), D.z.as<float>(), D.r.as<float>(),
-      D.n, V.box, D.start.as<uint32_t>(), D.items.as<uint32_t>());
+      D.n, V.box, D.start.as<uint32_t>(), D.items.as<uint32_t>(), D.occupied.as<uint32_t>());
   if (st != SIF_OK)
     return st;
   out->cx = D.x.as<float>();
@@ -305,6 +310,7 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   out->r = D.r.as<float>();
   out->cell_start = D.start.as<uint32_t>();
   out->items = D.items.as<uint32_t>();
+  out->occupied = D.occupied.as<uint32_t>();
   out->n = D.n;
   out->inv_l = V.inv_l;
   out->max_r = V.max_r;
@@ -459,7 +465,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   // back as it was. Minima seeds read each cell's neighbours and a survey
   // divides two fields: those write the fields.
   const bool minima = (R.opt & SIF_FINDER_SEEDS_MINIMA) != 0;
-  const bool use_lists = !survey && !minima && fft_lists_supported(n);
+  const bool use_lists = g_cell_lists && !survey && !minima && fft_lists_supported(n);
   CellLists lists[2];
   if (use_lists) {
     const uint64_t blocks = list_blocks(n);
@@ -1697,6 +1703,11 @@ sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float b
   return exodus_device(d_delta, n_cells, box, d_offsets, d_mx, d_my, d_mz, nullptr, nullptr,
     0.0, mesh_cells, n_particles, radii, n_radii, threshold, overlap_fraction, options,
     restore, n_in, stats);
+}
+
+int sif_amd_set_cell_lists(int on) {
+  g_cell_lists = on != 0;
+  return SIF_OK;
 }
 
 int sif_amd_set_batch_window(uint32_t floor, uint32_t ceiling) {

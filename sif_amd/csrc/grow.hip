@@ -70,6 +70,19 @@ __device__ bool overlap_quick(const uint64_t* mask, uint32_t n, uint32_t ix,
 // The overlap state (overlap.h) of a query against the CSR void index: the
 // index lists each void under the reference's cell, so a scan of the
 // superset cells sees every void that could collide.
+// The cells of one wrapped row (32 along z) that the z range [lo, hi] of a
+// scan covers, as a mask over the row's occupancy word (hi - lo < 32, or
+// the whole axis).
+__device__ __forceinline__ uint32_t row_zmask(int32_t lo, int32_t hi) {
+  const uint32_t span = (uint32_t)(hi - lo);
+  if (span + 1 >= (uint32_t)VOID_CELLS)
+    return 0xffffffffu;
+  const uint64_t m = ((1ull << (span + 1)) - 1ull) << void_wrap(lo);
+  return (uint32_t)(m | (m >> VOID_CELLS));
+}
+
+// The scan visits only the occupied cells of each row (the occupancy words);
+// the state is an OR over voids, so the order they come in changes nothing.
 __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
   int state = 0;
   for (int cls = VOID_CLASSES - 1; cls >= 0; cls--) {
@@ -78,13 +91,16 @@ __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
     int32_t lo[3], hi[3];
     superset_scan(q, V.class_max_r[cls], V.inv_l, lo, hi);
     const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+    const uint32_t* occ = V.occupied + (uint32_t)cls * VOID_CELLS * VOID_CELLS;
+    const uint32_t zmask = row_zmask(lo[2], hi[2]);
     for (int32_t a = lo[0]; a <= hi[0]; a++) {
       const uint32_t wa = void_wrap(a);
       for (int32_t b = lo[1]; b <= hi[1]; b++) {
         const uint32_t wb = void_wrap(b);
-        for (int32_t c = lo[2]; c <= hi[2]; c++) {
-          const uint32_t wc = void_wrap(c);
-          const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wc;
+        const uint32_t row = wa * VOID_CELLS + wb;
+        for (uint32_t cells = occ[row] & zmask; cells; cells &= cells - 1u) {
+          const uint32_t wc = (uint32_t)(__ffs(cells) - 1);
+          const uint32_t cell = row * VOID_CELLS + wc;
           const uint32_t end = start[cell + 1];
           for (uint32_t k = start[cell]; k < end; k++) {
             const uint32_t i = V.items[k];
@@ -184,23 +200,28 @@ __global__ void __launch_bounds__(64 * GO_WAVES) grown_overlap_kernel(
         continue;
       int32_t lo[3], hi[3];
       superset_scan(q, V.class_max_r[cls], V.inv_l, lo, hi);
-      const uint32_t na = (uint32_t)(hi[0] - lo[0] + 1), nb = (uint32_t)(hi[1] - lo[1] + 1),
-                     nc = (uint32_t)(hi[2] - lo[2] + 1);
+      const uint32_t na = (uint32_t)(hi[0] - lo[0] + 1), nb = (uint32_t)(hi[1] - lo[1] + 1);
       const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
-      for (uint32_t idx = lane; idx < na * nb * nc && st != 1; idx += 64) {
-        const uint32_t wa = void_wrap(lo[0] + (int32_t)(idx / (nb * nc)));
-        const uint32_t wb = void_wrap(lo[1] + (int32_t)((idx / nc) % nb));
-        const uint32_t wc = void_wrap(lo[2] + (int32_t)(idx % nc));
-        const uint32_t cell = (wa * VOID_CELLS + wb) * VOID_CELLS + wc;
-        const uint32_t end = start[cell + 1];
-        for (uint32_t j = start[cell]; j < end; j++) {
-          const uint32_t i = V.items[j];
-          const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
-          if (s == 1) {
-            st = 1;
-            break;
+      const uint32_t* occ = V.occupied + (uint32_t)cls * VOID_CELLS * VOID_CELLS;
+      const uint32_t zmask = row_zmask(lo[2], hi[2]);
+      // lanes over the rows, each row's occupied cells in turn
+      for (uint32_t idx = lane; idx < na * nb && st != 1; idx += 64) {
+        const uint32_t wa = void_wrap(lo[0] + (int32_t)(idx / nb));
+        const uint32_t wb = void_wrap(lo[1] + (int32_t)(idx % nb));
+        const uint32_t row = wa * VOID_CELLS + wb;
+        for (uint32_t cells = occ[row] & zmask; cells && st != 1; cells &= cells - 1u) {
+          const uint32_t wc = (uint32_t)(__ffs(cells) - 1);
+          const uint32_t cell = row * VOID_CELLS + wc;
+          const uint32_t end = start[cell + 1];
+          for (uint32_t j = start[cell]; j < end; j++) {
+            const uint32_t i = V.items[j];
+            const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
+            if (s == 1) {
+              st = 1;
+              break;
+            }
+            st |= s;
           }
-          st |= s;
         }
       }
     }
@@ -687,13 +708,21 @@ __global__ void void_slot_kernel(const float* __restrict__ vx, const float* __re
   atomicAdd(&counts[sl], 1u);
 }
 
+// Also the occupancy words: slot s's bit is bit (s mod 32) of word s / 32,
+// set when the slot lists a void (slots is a multiple of 64, so each wave's
+// two words lie within it or wholly past it).
 __global__ void void_start_kernel(const uint64_t* __restrict__ pos, uint32_t slots,
-  uint32_t* __restrict__ start, uint32_t* __restrict__ cursor) {
+  uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+  uint32_t* __restrict__ occupied) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s <= slots)
     start[s] = (uint32_t)pos[s];
   if (s < slots)
     cursor[s] = 0;
+  const uint64_t m = __ballot(s < slots && pos[s + 1] > pos[s]);
+  const uint32_t lane = threadIdx.x & 63;
+  if ((lane & 31) == 0 && s < slots)
+    occupied[s >> 5] = (uint32_t)(m >> lane);
 }
 
 // Within a cell the order of the voids is not fixed; nothing that reads the
@@ -713,8 +742,10 @@ thread_local DeviceBuffer g_vi_slot, g_vi_counts, g_vi_pos, g_vi_cursor;
 }
 
 int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
-  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items) {
+  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items, uint32_t* d_occupied) {
   const uint32_t slots = VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+  static_assert((VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS) % 64 == 0,
+    "whole waves of slots");
   int st = g_vi_slot.reserve((size_t)(nv ? nv : 1) * 4);
   if (st == SIF_OK) st = g_vi_counts.reserve((size_t)(slots + 1) * 4);
   if (st == SIF_OK) st = g_vi_pos.reserve((size_t)(slots + 1) * 8);
@@ -733,7 +764,7 @@ int build_void_index(const float* vx, const float* vy, const float* vz, const fl
   if (st != SIF_OK)
     return st;
   void_start_kernel<<<grid_for(slots + 1, 256), 256, 0, stream()>>>(g_vi_pos.as<uint64_t>(),
-    slots, d_start, g_vi_cursor.as<uint32_t>());
+    slots, d_start, g_vi_cursor.as<uint32_t>(), d_occupied);
   SIF_HIP(hipGetLastError());
   if (nv)
     void_scatter_kernel<<<grid_for(nv, 256), 256, 0, stream()>>>(g_vi_slot.as<uint32_t>(), nv,

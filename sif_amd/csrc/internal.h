@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.h"
+#include "overlap.h"
 #include "sif/structures/catalogue.h"
 #include "sif_amd.h"
 
@@ -100,6 +101,9 @@ struct VoidIndex {
   const float* r;
   const uint32_t* cell_start;  // VOID_CLASSES * 32^3 + 1 entries, class-major
   const uint32_t* items;       // void ids, grouped by (class, reference cell)
+  // one bit a (class, cell): set when the cell lists a void of the class;
+  // word (class 32 + wa) 32 + wb holds the row's 32 cells along z
+  const uint32_t* occupied;
   uint32_t n;                  // voids
   float inv_l;                 // (float)32 / box
   float max_r;                 // largest radius in the index
@@ -174,10 +178,11 @@ int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
 // The class-split CSR of nv device-resident voids: d_start has
 // VOID_CLASSES * 32^3 + 1 entries, d_items nv.
 int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
-  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items);
+  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items,
+  uint32_t* d_occupied);
 // Per grown sphere: its overlap state against the index (bits 0-1) and, in
-// GROWN_POLE_BIT, whether the pole probe at the grown radius hits the mask.
-constexpr uint32_t GROWN_POLE_BIT = 16u;
+// GROWN_POLE_BIT (overlap.h), whether the pole probe at the grown radius hits
+// the mask.
 int grown_overlap(const float* cx, const float* cy, const float* cz, const uint32_t* flat,
   const float* radius, uint32_t n, const VoidIndex& V, float max_r, float box, float f,
   float scale, const uint64_t* mask, uint32_t grid_n, float l, uint32_t* state);

@@ -42,6 +42,11 @@ namespace sif_amd {
 // exodus_finder.c:215: the void cell list is 32 cells a side.
 constexpr int32_t VOID_CELLS = 32;
 
+// A grown candidate's state word from the device (grown_overlap): the
+// overlap state against the catalogue as the rung began in bits 0-1, and
+// this bit when the pole probe at its grown radius hits the mask.
+constexpr uint32_t GROWN_POLE_BIT = 16u;
+
 SIF_HD uint32_t void_wrap(int32_t v) {
   const int32_t w = v % VOID_CELLS;
   return (uint32_t)(w < 0 ? w + VOID_CELLS : w);

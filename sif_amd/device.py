@@ -296,6 +296,13 @@ def exodus_survey(ddata, drandoms, n_cells, box, dmesh, rmesh, radii, threshold,
     return out, fp, n_in, st
 
 
+def set_cell_lists(on=True):
+    """Whether later exodus runs list each rung's candidate cells in the
+    smoothing (the default) or write the smoothed field; the catalogue is
+    the same."""
+    check(lib.sif_amd_set_cell_lists(1 if on else 0))
+
+
 def set_batch_window(floor=0, ceiling=0):
     """The growth batch window of later exodus runs in this process; (0, 0)
     restores the defaults."""

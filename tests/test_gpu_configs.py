@@ -95,6 +95,23 @@ def test_config1_catalogue_equals_oracle(config1):
         acc = np.array([s["accepted"] for s in stats])
         np.testing.assert_array_equal(np.repeat(np.arange(len(radii)), acc), orung)
     assert len(ocat) > 1000
+    # the smoothed fields written and scanned instead of the cell lists:
+    # the same catalogue, and the grid restored to float precision
+    D.set_cell_lists(False)
+    try:
+        g = _grid(D, dx, dy, dz, len(x), c["n"], c["box"])
+        delta0 = g.get()
+        cat, nin, _ = D.exodus(g, c["n"], c["box"], m, radii, THRESH)
+    finally:
+        D.set_cell_lists(True)
+    np.testing.assert_array_equal(cat.view(np.uint32), ocat.view(np.uint32))
+    np.testing.assert_array_equal(nin, onin)
+    assert np.abs(g.get() - delta0).max() < 1e-4 * max(1.0, np.abs(delta0).max())
+    # with the lists the grid is never written
+    g = _grid(D, dx, dy, dz, len(x), c["n"], c["box"])
+    delta0 = g.get()
+    D.exodus(g, c["n"], c["box"], m, radii, THRESH)
+    np.testing.assert_array_equal(g.get().view(np.uint32), delta0.view(np.uint32))
 
 
 def _match(cat_a, cat_b):
@@ -258,6 +275,15 @@ def test_config2_catalogue_repeatable_and_radii_brute_force(config2):
     _say(f"512^3 catalogue: {len(cat)} voids")
     np.testing.assert_array_equal(cat.view(np.uint32), cat2.view(np.uint32))
     np.testing.assert_array_equal(nin, nin2)
+    # the same catalogue from smoothed fields written and scanned
+    D.set_cell_lists(False)
+    try:
+        g = _grid(D, dx, dy, dz, len(x), n, box)
+        cat3, nin3, _ = D.exodus(g, n, box, m, radii, THRESH, restore=False)
+    finally:
+        D.set_cell_lists(True)
+    np.testing.assert_array_equal(cat3.view(np.uint32), cat.view(np.uint32))
+    np.testing.assert_array_equal(nin3, nin)
     del m
     # 200 voids against a brute force over every particle of the 3^3 coarse
     # cells around them (62.5 Mpc/h cells, wider than any search radius)

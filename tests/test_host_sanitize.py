@@ -41,7 +41,7 @@ def test_emulated_device_code_under_asan_ubsan():
     env = dict(os.environ, SIF_EMU_VARIANT="asan", LD_PRELOAD=runtime,
                ASAN_OPTIONS="detect_leaks=0:verify_asan_link_order=0")
     tests = [os.path.join(HERE, t) for t in ("test_grow_emulated.py", "test_overlap_split.py",
-                                            "test_paint.py")]
+                                            "test_paint.py", "test_batch_pairs.py")]
     res = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
                           "-m", "not gpu", *tests], capture_output=True, text=True, env=env,
                          cwd=os.path.dirname(HERE), timeout=900)
