@@ -73,6 +73,8 @@ int all_reads_linked(const Voids& V) {
 struct DeviceState {
   DeviceBuffer spec, work, mask, keys, flat, keep, surv, skeys, luts, axis;
   DeviceBuffer pf_counts;  // per rung: masked, proxy, overlap
+  DeviceBuffer why, since;  // the prefilter's verdicts; the candidates masked since (4)
+  uint64_t* h_since = nullptr;  // pinned copy of `since`
   DeviceBuffer field2;     // the second smoothed field (rungs alternate)
   // or, when only scanned, the two rungs' cell lists in their place
   DeviceBuffer lkeys[2], llocal[2], lcount[2];
@@ -119,23 +121,76 @@ WorkPool& host_pool() {
 // One growth batch in flight: device buffers, pinned host copies of its
 // inputs and results, and the event that marks the results in host memory.
 struct BatchSlot {
-  DeviceBuffer d_flat, d_cx, d_cy, d_cz, d_rad, d_reason, d_nin, d_state, d_algo;
-  // the batch's pair effects (batch_pairs.hip)
-  DeviceBuffer d_eff, d_cnt8, d_cnt32, d_off, d_setmax;
-  PairEffect* h_eff = nullptr;
-  uint8_t* h_cnt = nullptr;
-  uint64_t eff_capacity = 0;  // effects downloaded with the batch
-  uint32_t* h_flat = nullptr;
+  DeviceBuffer d_flat, d_cx, d_cy, d_cz;
+  // the batch's pair effects' scratch (batch_pairs.hip)
+  DeviceBuffer d_cnt32, d_off, d_setmax;
+  // The launch's results in one block, copied to host memory at once:
+  // radius, reason, state, effect count, enclosed count, algorithmic bytes,
+  // then the effects list. The typed pointers into it are set per launch
+  // (regions sized by its count), on the device and in the pinned copy.
+  DeviceBuffer d_out;
+  uint8_t* h_out = nullptr;
+  float* p_rad = nullptr;
+  uint32_t* p_reason = nullptr;
+  uint32_t* p_state = nullptr;
+  uint8_t* p_cnt = nullptr;
+  uint64_t* p_nin = nullptr;
+  uint64_t* p_algo = nullptr;
+  PairEffect* p_eff = nullptr;
+  uint64_t out_bytes = 0;
   float* h_rad = nullptr;
   uint32_t* h_reason = nullptr;
-  uint64_t* h_nin = nullptr;
   uint32_t* h_state = nullptr;
+  uint8_t* h_cnt = nullptr;
+  uint64_t* h_nin = nullptr;
   uint64_t* h_algo = nullptr;
+  PairEffect* h_eff = nullptr;
+  uint64_t eff_capacity = 0;  // effects downloaded with the batch
+  uint32_t* h_flat = nullptr;
   std::vector<uint8_t> edge;
   hipEvent_t done = nullptr;
   uint32_t count = 0;
   bool algo = false;
+  // The regions for a launch of bc candidates.
+  void place(uint32_t bc) {
+    auto a8 = [](uint64_t v) { return (v + 7) & ~(uint64_t)7; };
+    uint64_t o = 0;
+    const uint64_t o_rad = o;
+    o = a8(o + 4ull * bc);
+    const uint64_t o_reason = o;
+    o = a8(o + 4ull * bc);
+    const uint64_t o_state = o;
+    o = a8(o + 4ull * bc);
+    const uint64_t o_cnt = o;
+    o = a8(o + bc);
+    const uint64_t o_nin = o;
+    o += 8ull * bc;
+    const uint64_t o_algo = o;
+    o += 8ull * bc;
+    const uint64_t o_eff = o;
+    eff_capacity = (uint64_t)bc * BATCH_EFF_PER;
+    o += eff_capacity * sizeof(PairEffect);
+    out_bytes = o;
+    uint8_t* d = d_out.as<uint8_t>();
+    p_rad = reinterpret_cast<float*>(d + o_rad);
+    p_reason = reinterpret_cast<uint32_t*>(d + o_reason);
+    p_state = reinterpret_cast<uint32_t*>(d + o_state);
+    p_cnt = d + o_cnt;
+    p_nin = reinterpret_cast<uint64_t*>(d + o_nin);
+    p_algo = reinterpret_cast<uint64_t*>(d + o_algo);
+    p_eff = reinterpret_cast<PairEffect*>(d + o_eff);
+    h_rad = reinterpret_cast<float*>(h_out + o_rad);
+    h_reason = reinterpret_cast<uint32_t*>(h_out + o_reason);
+    h_state = reinterpret_cast<uint32_t*>(h_out + o_state);
+    h_cnt = h_out + o_cnt;
+    h_nin = reinterpret_cast<uint64_t*>(h_out + o_nin);
+    h_algo = reinterpret_cast<uint64_t*>(h_out + o_algo);
+    h_eff = reinterpret_cast<PairEffect*>(h_out + o_eff);
+  }
 };
+// a launch's result block at its largest
+constexpr uint64_t BATCH_OUT_BYTES =
+  (uint64_t)BATCH_ROOM * (4 + 4 + 4 + 1 + 8 + 8 + BATCH_EFF_PER * sizeof(PairEffect)) + 64;
 thread_local BatchSlot g_slots[2];
 
 int host_alloc(void** p, size_t bytes) {
@@ -152,25 +207,12 @@ int reserve_slots() {
     if (e == SIF_OK) e = B.d_cx.reserve(BATCH_ROOM * 4);
     if (e == SIF_OK) e = B.d_cy.reserve(BATCH_ROOM * 4);
     if (e == SIF_OK) e = B.d_cz.reserve(BATCH_ROOM * 4);
-    if (e == SIF_OK) e = B.d_rad.reserve(BATCH_ROOM * 4);
-    if (e == SIF_OK) e = B.d_reason.reserve(BATCH_ROOM * 4);
-    if (e == SIF_OK) e = B.d_nin.reserve(BATCH_ROOM * 8);
-    if (e == SIF_OK) e = B.d_state.reserve(BATCH_ROOM * 4);
-    if (e == SIF_OK) e = B.d_algo.reserve(BATCH_ROOM * 8);
-    if (e == SIF_OK) e = B.d_eff.reserve((size_t)BATCH_ROOM * BATCH_EFF_PER * sizeof(PairEffect));
-    if (e == SIF_OK) e = B.d_cnt8.reserve(BATCH_ROOM);
     if (e == SIF_OK) e = B.d_cnt32.reserve(BATCH_ROOM * 4);
     if (e == SIF_OK) e = B.d_off.reserve(BATCH_ROOM * 8);
     if (e == SIF_OK) e = B.d_setmax.reserve(sizeof(float));
-    if (e == SIF_OK)
-      e = host_alloc((void**)&B.h_eff, (size_t)BATCH_ROOM * BATCH_EFF_PER * sizeof(PairEffect));
-    if (e == SIF_OK) e = host_alloc((void**)&B.h_cnt, BATCH_ROOM);
+    if (e == SIF_OK) e = B.d_out.reserve(BATCH_OUT_BYTES);
+    if (e == SIF_OK) e = host_alloc((void**)&B.h_out, BATCH_OUT_BYTES);
     if (e == SIF_OK) e = host_alloc((void**)&B.h_flat, BATCH_ROOM * 4);
-    if (e == SIF_OK) e = host_alloc((void**)&B.h_rad, BATCH_ROOM * 4);
-    if (e == SIF_OK) e = host_alloc((void**)&B.h_reason, BATCH_ROOM * 4);
-    if (e == SIF_OK) e = host_alloc((void**)&B.h_nin, BATCH_ROOM * 8);
-    if (e == SIF_OK) e = host_alloc((void**)&B.h_state, BATCH_ROOM * 4);
-    if (e == SIF_OK) e = host_alloc((void**)&B.h_algo, BATCH_ROOM * 8);
     if (e != SIF_OK)
       return e;
     B.edge.assign(BATCH_ROOM, 0);
@@ -197,6 +239,7 @@ struct DeviceVoids {
   float* hz = nullptr;
   float* hr = nullptr;
   uint32_t hcap = 0;
+  hipEvent_t staged = nullptr;  // after the last append's copies from the staging
   float class_max[VOID_CLASSES] = {-1.0f, -1.0f, -1.0f, -1.0f};
 };
 thread_local DeviceVoids g_voids;
@@ -240,8 +283,12 @@ int append_voids(const float* x, const float* y, const float* z, const float* r,
       return st;
     D.cap = want;
   }
-  // the previous append's copies read the staging: let them finish
-  SIF_HIP(hipStreamSynchronize(stream()));
+  // the previous append's copies read the staging: let them finish (not the
+  // rest of the stream, which may hold a growth launch in flight)
+  if (!D.staged)
+    SIF_HIP(hipEventCreateWithFlags(&D.staged, hipEventDisableTiming));
+  else
+    SIF_HIP(hipEventSynchronize(D.staged));
   if (k > D.hcap) {
     for (float** p : {&D.hx, &D.hy, &D.hz, &D.hr})
       if (*p) {
@@ -267,6 +314,7 @@ int append_voids(const float* x, const float* y, const float* z, const float* r,
     stream()));
   SIF_HIP(hipMemcpyAsync(D.r.as<float>() + D.n, D.hr, (size_t)k * 4, hipMemcpyHostToDevice,
     stream()));
+  SIF_HIP(hipEventRecord(D.staged, stream()));
   const float cell_w = box / (float)VOID_CELLS;
   for (uint32_t i = 0; i < k; i++) {
     const uint32_t cls = r[i] < 0.25f * cell_w ? 0u
@@ -533,6 +581,12 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   if (st != SIF_OK)
     return st;
   SIF_HIP(hipMemsetAsync(g_dev.pf_counts.ptr, 0, n_rungs * 3 * sizeof(uint64_t), stream()));
+  std::vector<uint64_t> pf_since(n_rungs * 3, 0);  // prefilter refusals masked since
+  st = g_dev.since.reserve(4 * sizeof(uint64_t));
+  if (st != SIF_OK)
+    return st;
+  if (!g_dev.h_since)
+    SIF_HIP(hipHostMalloc((void**)&g_dev.h_since, 4 * sizeof(uint64_t), hipHostMallocDefault));
   if (R.prior && R.prior->n_voids) {
     for (uint64_t v = 0; v < R.prior->n_voids; v++)
       V.add(R.prior->cx[v], R.prior->cy[v], R.prior->cz[v], R.prior->radii[v], 0);
@@ -565,6 +619,174 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   }
 
 
+  // A rung's start -- its candidates scanned, prefiltered against the
+  // catalogue on the device, compacted and sorted, the survivors on the host
+  // -- runs in steps, so that the next rung's can be issued while this rung's
+  // last batches are grown and committed: its waits then fall after those
+  // commits instead of before the next rung. The next rung's prefilter sees
+  // the catalogue as it stands when it is issued; the voids this rung accepts
+  // after that start the next rung's list of newer voids (RV), which the
+  // commit's own tests cover as they cover the rung's.
+  struct RungStart {
+    size_t rung = 0;
+    int step = 0;  // 0 none, 1 counting, 2 prefilter issued, 3 survivors on the host
+    uint64_t nc = 0, ns = 0;
+    uint32_t key_differ = 0xffffffffu;
+    VoidIndex VI;
+    ScanJob job;
+    // Started ahead: the candidates that the voids accepted after its
+    // prefilter mask (the reference's scan, after them, never lists them),
+    // by prefilter verdict -- kept (the commit refuses them as masked),
+    // masked, proxy, overlap. Taken off the rung's counters.
+    uint64_t since[4] = {0, 0, 0, 0};
+  };
+  RungStart rs;
+  auto reserve_candidates = [&]() {
+    int e = g_dev.keys.reserve(cand_cap * 4);
+    if (e == SIF_OK) e = g_dev.flat.reserve(cand_cap * 4);
+    if (e == SIF_OK) e = g_dev.keep.reserve(cand_cap * 4);
+    if (e == SIF_OK) e = g_dev.surv.reserve(cand_cap * 4);
+    if (e == SIF_OK) e = g_dev.skeys.reserve(cand_cap * 4);
+    if (e == SIF_OK) e = g_dev.why.reserve(cand_cap);
+    return e;
+  };
+  // step 1: rung j's field (smoothed on the auxiliary stream, the first rung
+  // here), and the scan's counting pass
+  auto start_scan = [&](size_t j) -> int {
+    float* const fj = fields[j & 1];
+    if (j == R.first_rung) {
+      const int e = smooth(j, fj, stream());
+      if (e != SIF_OK)
+        return e;
+    } else {
+      SIF_HIP(hipStreamWaitEvent(stream(), ev_smoothed[j & 1], 0));
+    }
+    if (cand_cap == 0) {
+      cand_cap = total / 64 + 1024;
+      const int e = reserve_candidates();
+      if (e != SIF_OK)
+        return e;
+    }
+    rs = RungStart();
+    rs.rung = j;
+    rs.job.delta = fj;
+    rs.job.lists = use_lists ? &lists[j & 1] : nullptr;
+    rs.job.mask = g_dev.mask.as<uint64_t>();
+    rs.job.total = total;
+    rs.job.t = R.threshold;
+    rs.job.minima_n = minima ? n : 0u;
+    const int e = scan_begin(rs.job);
+    if (e == SIF_OK)
+      rs.step = 1;
+    return e;
+  };
+  // the candidates in flat order (the buffers grown and the scan run again
+  // when they are short); the scan was the field's last reader, so the next
+  // rung's field is smoothed into the other buffer now
+  auto finish_scan = [&]() -> int {
+    const size_t j = rs.rung;
+    int e = scan_finish(rs.job, g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap);
+    while (e == SIF_OK && rs.job.count > cand_cap) {
+      cand_cap = (uint64_t)((double)rs.job.count * 1.2) + 1;
+      g_dev.keys.release();
+      g_dev.flat.release();
+      g_dev.keep.release();
+      g_dev.surv.release();
+      g_dev.skeys.release();
+      g_dev.why.release();
+      e = reserve_candidates();
+      if (e == SIF_OK)
+        e = scan_begin(rs.job);
+      if (e == SIF_OK)
+        e = scan_finish(rs.job, g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap);
+    }
+    if (e != SIF_OK)
+      return e;
+    rs.nc = rs.job.count;
+    rs.key_differ = rs.job.key_differ;
+    SIF_HIP(hipEventRecord(ev_scanned[j & 1], stream()));
+    if (j + 1 < R.radii.size()) {
+      // after this scan: the other buffer's last reader (rung j - 1's scan)
+      // and every earlier use of the work buffer are behind it
+      const size_t jn = j + 1;
+      SIF_HIP(hipStreamWaitEvent(aux_stream(), ev_scanned[j & 1], 0));
+      e = smooth(jn, fields[jn & 1], aux_stream());
+      if (e != SIF_OK)
+        return e;
+      SIF_HIP(hipEventRecord(ev_smoothed[jn & 1], aux_stream()));
+    }
+    return SIF_OK;
+  };
+  // step 2: the prefilter against the catalogue on the device (the mask and
+  // the void index built now), in flat order (the verdicts do not depend on
+  // the order; neighbouring candidates share void cells), and the compaction
+  auto start_filter = [&]() -> int {
+    const size_t j = rs.rung;
+    const double th = now_ms();
+    int e = device_void_index(V, &rs.VI);
+    if (timing_active())
+      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
+    if (e != SIF_OK)
+      return e;
+    const int32_t proxy_j = (int32_t)(scale * R.radii[j] * (1.0f - R.f) / l) - 1;
+    timing_begin(SIF_AMD_STAGE_PREFILTER);
+    e = prefilter(g_dev.flat.as<uint32_t>(), (uint32_t)rs.nc, n, l, R.box,
+      g_dev.mask.as<uint64_t>(), proxy_j, R.radii[j], R.f, scale, rs.VI,
+      g_dev.keep.as<uint32_t>(), g_dev.pf_counts.as<uint64_t>() + 3 * j,
+      g_dev.why.as<uint8_t>());
+    if (e == SIF_OK)
+      e = compact2_u32_begin(g_dev.flat.as<uint32_t>(), g_dev.keys.as<uint32_t>(),
+        g_dev.keep.as<uint32_t>(), rs.nc, g_dev.surv.as<uint32_t>(), g_dev.skeys.as<uint32_t>());
+    if (e != SIF_OK)
+      return e;
+    timing_end(SIF_AMD_STAGE_PREFILTER);
+    rs.step = 2;
+    return SIF_OK;
+  };
+  // step 3: the survivors sorted deepest first, and on the host; for a start
+  // that went ahead, first the candidates masked since its prefilter (the
+  // mask now holds every void the rung starts with)
+  auto finish_filter = [&](bool went_ahead) -> int {
+    const double th = now_ms();
+    if (went_ahead && rs.nc) {
+      SIF_HIP(hipMemsetAsync(g_dev.since.ptr, 0, 4 * sizeof(uint64_t), stream()));
+      const int e = count_masked_since(g_dev.flat.as<uint32_t>(), g_dev.why.as<uint8_t>(), rs.nc,
+        g_dev.mask.as<uint64_t>(), g_dev.since.as<uint64_t>());
+      if (e != SIF_OK)
+        return e;
+      SIF_HIP(hipMemcpyAsync(g_dev.h_since, g_dev.since.ptr, 4 * sizeof(uint64_t),
+        hipMemcpyDeviceToHost, stream()));
+    }
+    int e = compact2_u32_finish(&rs.ns);
+    if (e == SIF_OK)  // the survivors are a subset of the scanned candidates
+      e = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), rs.ns,
+        rs.key_differ);
+    if (e != SIF_OK)
+      return e;
+    surv.resize(rs.ns);
+    if (rs.ns)
+      SIF_HIP(hipMemcpyAsync(surv.data(), g_dev.surv.ptr, rs.ns * 4, hipMemcpyDeviceToHost,
+        stream()));
+    SIF_HIP(hipStreamSynchronize(stream()));
+    if (went_ahead && rs.nc)
+      for (int w = 0; w < 4; w++)
+        rs.since[w] = g_dev.h_since[w];
+    if (timing_active())
+      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
+    rs.step = 3;
+    return SIF_OK;
+  };
+  // the rung's voids not yet on the device: appended and painted into the mask
+  size_t rv_appended = 0;
+  auto append_rung_voids = [&]() -> int {
+    if (rv_appended >= RV.cx.size())
+      return SIF_OK;
+    const size_t a = rv_appended;
+    rv_appended = RV.cx.size();
+    return append_voids(RV.cx.data() + a, RV.cy.data() + a, RV.cz.data() + a, RV.r.data() + a,
+      (uint32_t)(rv_appended - a), R.box, scale, l, n);
+  };
+
   uint64_t mismatched = R.mismatched0;
   for (size_t i = R.first_rung; i < R.radii.size() && st == SIF_OK; i++) {
     const float radius = R.radii[i];
@@ -572,67 +794,26 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     memset(&S, 0, sizeof(S));
     S.radius = radius;
 
-    // Rung i's field: smoothed on the auxiliary stream while rung i - 1 was
-    // committed (the first rung run on the library stream).
-    float* const field = fields[i & 1];
-    if (i == R.first_rung)
-      st = smooth(i, field, stream());
-    else
-      SIF_HIP(hipStreamWaitEvent(stream(), ev_smoothed[i & 1], 0));
-    if (st != SIF_OK)
-      break;
-
-    // the candidates in flat order, growing the buffers when they are short
-    uint64_t nc = 0;
-    uint32_t key_differ = 0xffffffffu;  // the bits on which the candidates' keys differ
-    auto reserve_candidates = [&]() {
-      int e = g_dev.keys.reserve(cand_cap * 4);
-      if (e == SIF_OK) e = g_dev.flat.reserve(cand_cap * 4);
-      if (e == SIF_OK) e = g_dev.keep.reserve(cand_cap * 4);
-      if (e == SIF_OK) e = g_dev.surv.reserve(cand_cap * 4);
-      if (e == SIF_OK) e = g_dev.skeys.reserve(cand_cap * 4);
-      return e;
-    };
-    for (;;) {
-      if (cand_cap == 0) {
-        cand_cap = total / 64 + 1024;
-        if ((st = reserve_candidates()) != SIF_OK)
-          break;
-      }
-      st = use_lists
-             ? scan_lists_flat(lists[i & 1], g_dev.mask.as<uint64_t>(), total,
-                 g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap, &nc,
-                 &key_differ)
-             : scan_candidates_flat(field, g_dev.mask.as<uint64_t>(), total, R.threshold,
-                 minima ? n : 0u, g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(),
-                 cand_cap, &nc, &key_differ);
-      if (st != SIF_OK || nc <= cand_cap)
-        break;
-      cand_cap = (uint64_t)((double)nc * 1.2) + 1;
-      g_dev.keys.release();
-      g_dev.flat.release();
-      g_dev.keep.release();
-      g_dev.surv.release();
-      g_dev.skeys.release();
-      if ((st = reserve_candidates()) != SIF_OK)
-        break;
-    }
-    if (st != SIF_OK)
-      break;
-    S.candidates = nc;
-    // The scan was the field's last reader: the next rung's field may be
-    // smoothed into the other buffer now, beside this rung's commit.
-    SIF_HIP(hipEventRecord(ev_scanned[i & 1], stream()));
-    if (i + 1 < R.radii.size()) {
-      // after this scan: the other buffer's last reader (rung i - 1's scan)
-      // and every earlier use of the work buffer are behind it
-      const size_t j = i + 1;
-      SIF_HIP(hipStreamWaitEvent(aux_stream(), ev_scanned[i & 1], 0));
-      st = smooth(j, fields[j & 1], aux_stream());
+    // The rung's start, unless the previous rung issued it ahead.
+    if (!(rs.step == 3 && rs.rung == i)) {
+      st = start_scan(i);
+      if (st == SIF_OK)
+        st = finish_scan();
+      if (st == SIF_OK && !displace)
+        st = start_filter();
+      if (st == SIF_OK && !displace)
+        st = finish_filter(false);
       if (st != SIF_OK)
         break;
-      SIF_HIP(hipEventRecord(ev_smoothed[j & 1], aux_stream()));
     }
+    const uint64_t nc = rs.nc;
+    const uint32_t key_differ = rs.key_differ;
+    // the reference's counters: the candidates its scan lists, the prefilter's
+    // refusals among them, and the commit's (less those masked since, below)
+    S.candidates = nc - rs.since[0] - rs.since[1] - rs.since[2] - rs.since[3];
+    for (int w = 1; w < 4; w++)
+      pf_since[3 * i + w - 1] = rs.since[w];
+    const uint64_t masked_since = rs.since[0];
 
     const float rmin = 1.00f * radius;
     float r_search = radius + fmaxf((sf - 1.0f) * radius, 3.0f * l);
@@ -835,41 +1016,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       continue;
     }
 
-    // prefilter against the catalogue as the rung begins, in flat order (the
-    // verdicts do not depend on the order; neighbouring candidates share
-    // void cells), then sort the survivors deepest first
-    VoidIndex VI;
-    double th = now_ms();
-    st = device_void_index(V, &VI);
-    if (st != SIF_OK)
-      break;
-    if (timing_active())
-      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
-    RV.clear();
-    timing_begin(SIF_AMD_STAGE_PREFILTER);
-    st = prefilter(g_dev.flat.as<uint32_t>(), (uint32_t)nc, n, l, R.box,
-      g_dev.mask.as<uint64_t>(), proxy, radius, R.f, scale, VI, g_dev.keep.as<uint32_t>(),
-      g_dev.pf_counts.as<uint64_t>() + 3 * i);
-    if (st != SIF_OK)
-      break;
-    uint64_t ns = 0;
-    st = compact2_u32(g_dev.flat.as<uint32_t>(), g_dev.keys.as<uint32_t>(),
-      g_dev.keep.as<uint32_t>(), nc, g_dev.surv.as<uint32_t>(), g_dev.skeys.as<uint32_t>(), &ns);
-    if (st != SIF_OK)
-      break;
-    timing_end(SIF_AMD_STAGE_PREFILTER);
-    st = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), ns,
-      key_differ);  // the survivors are a subset of the scanned candidates
-    if (st != SIF_OK)
-      break;
-    th = now_ms();
-    surv.resize(ns);
-    if (ns)
-      SIF_HIP(hipMemcpyAsync(surv.data(), g_dev.surv.ptr, ns * 4, hipMemcpyDeviceToHost,
-        stream()));
-    SIF_HIP(hipStreamSynchronize(stream()));
-    if (timing_active())
-      timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
+    const VoidIndex VI = rs.VI;
+    const uint64_t ns = rs.ns;
+    double th = 0.0;
 
     // Speculative batches, sequential commit. While the host commits one
     // batch, the GPU grows the next, assembled from the catalogue as it stood
@@ -976,27 +1125,25 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     auto launch = [&](BatchSlot& B) -> int {
       const uint32_t bc = B.count;
       const bool algo = timing_active() && !survey;
+      B.place(bc);
       SIF_HIP(hipMemcpyAsync(B.d_flat.ptr, B.h_flat, bc * 4, hipMemcpyHostToDevice, stream()));
       int e = node_centres(B.d_flat.as<uint32_t>(), bc, n, l, B.d_cx.as<float>(),
         B.d_cy.as<float>(), B.d_cz.as<float>());
       if (e == SIF_OK && survey)  // exodus_finder.c:2045-2049
         e = grow_survey(data_mesh, R.rmesh, R.box, B.d_cx.as<float>(), B.d_cy.as<float>(),
-          B.d_cz.as<float>(), bc, rmin, r_search, survey_K, B.d_rad.as<float>(),
-          B.d_reason.as<uint32_t>(), B.d_nin.as<uint64_t>());
+          B.d_cz.as<float>(), bc, rmin, r_search, survey_K, B.p_rad, B.p_reason, B.p_nin);
       else if (e == SIF_OK)
         e = grow_batch(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
           B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, rmin, r_search,
-          R.threshold, vol, B.d_rad.as<float>(), B.d_reason.as<uint32_t>(),
-          B.d_nin.as<uint64_t>(), algo ? B.d_algo.as<uint64_t>() : nullptr, &mw);
+          R.threshold, vol, B.p_rad, B.p_reason, B.p_nin, algo ? B.p_algo : nullptr, &mw);
       if (e == SIF_OK && ext0 < n_ext)
         e = grow_extend(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
           B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, ext_rmin.data(),
-          ext_reach.data(), ext0, n_ext, R.threshold, vol, B.d_rad.as<float>(),
-          B.d_reason.as<uint32_t>(), B.d_nin.as<uint64_t>(), &mw);
+          ext_reach.data(), ext0, n_ext, R.threshold, vol, B.p_rad, B.p_reason, B.p_nin, &mw);
       if (e == SIF_OK)
         e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
-          B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), bc, VI, V.max_r, R.box, R.f, scale,
-          g_dev.mask.as<uint64_t>(), n, l, B.d_state.as<uint32_t>());
+          B.d_flat.as<uint32_t>(), B.p_rad, bc, VI, V.max_r, R.box, R.f, scale,
+          g_dev.mask.as<uint64_t>(), n, l, B.p_state);
       if (e == SIF_OK) {
         BatchPairParams bp;
         bp.n = n;
@@ -1011,28 +1158,17 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         BatchPairScratch W;
         W.set_max_r = B.d_setmax.as<float>();
         W.cnt32 = B.d_cnt32.as<uint32_t>();
-        W.cnt8 = B.d_cnt8.as<uint8_t>();
+        W.cnt8 = B.p_cnt;
         W.offset = B.d_off.as<uint64_t>();
-        W.eff = B.d_eff.as<PairEffect>();
-        W.capacity = (uint64_t)bc * BATCH_EFF_PER;
-        B.eff_capacity = W.capacity;
-        e = batch_pairs(B.d_flat.as<uint32_t>(), B.d_rad.as<float>(), B.d_state.as<uint32_t>(),
-          bc, bp, W);
+        W.eff = B.p_eff;
+        W.capacity = B.eff_capacity;
+        e = batch_pairs(B.d_flat.as<uint32_t>(), B.p_rad, B.p_state, bc, bp, W);
       }
       if (e != SIF_OK)
         return e;
-      SIF_HIP(hipMemcpyAsync(B.h_cnt, B.d_cnt8.ptr, bc, hipMemcpyDeviceToHost, stream()));
-      SIF_HIP(hipMemcpyAsync(B.h_eff, B.d_eff.ptr, B.eff_capacity * sizeof(PairEffect),
-        hipMemcpyDeviceToHost, stream()));
-      SIF_HIP(hipMemcpyAsync(B.h_rad, B.d_rad.ptr, bc * 4, hipMemcpyDeviceToHost, stream()));
-      SIF_HIP(hipMemcpyAsync(B.h_reason, B.d_reason.ptr, bc * 4, hipMemcpyDeviceToHost,
+      // the results, one copy
+      SIF_HIP(hipMemcpyAsync(B.h_out, B.d_out.ptr, B.out_bytes, hipMemcpyDeviceToHost,
         stream()));
-      SIF_HIP(hipMemcpyAsync(B.h_nin, B.d_nin.ptr, bc * 8, hipMemcpyDeviceToHost, stream()));
-      SIF_HIP(hipMemcpyAsync(B.h_state, B.d_state.ptr, bc * 4, hipMemcpyDeviceToHost,
-        stream()));
-      if (algo)
-        SIF_HIP(hipMemcpyAsync(B.h_algo, B.d_algo.ptr, bc * 8, hipMemcpyDeviceToHost,
-          stream()));
       B.algo = algo;
       SIF_HIP(hipEventRecord(B.done, stream()));
       S.evaluated += bc;
@@ -1227,6 +1363,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
 
     BatchSlot* cur = &g_slots[0];
     BatchSlot* nxt = &g_slots[1];
+    // The next rung's start goes ahead once this rung's last batch is
+    // launched: its counting pass beside the next commit, its prefilter
+    // beside the one after (rv_mark: the voids its prefilter sees).
+    const bool ahead = i + 1 < R.radii.size();
+    size_t rv_mark = 0;
     assemble(*cur);
     if (cur->count)
       st = launch(*cur);
@@ -1237,9 +1378,22 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         if (nxt->count)
           st = launch(*nxt);
       }
+      if (st == SIF_OK && ahead && k >= ns && rs.rung == i) {
+        st = append_rung_voids();
+        if (st == SIF_OK)
+          st = start_scan(i + 1);
+      }
       uint64_t accepted = 0;
       if (st == SIF_OK)
         st = commit(*cur, &accepted);
+      if (st == SIF_OK && rs.rung == i + 1 && rs.step == 1) {
+        st = finish_scan();
+        if (st == SIF_OK)
+          st = append_rung_voids();
+        rv_mark = RV.cx.size();
+        if (st == SIF_OK)
+          st = start_filter();
+      }
       if (st != SIF_OK)
         break;
       // exodus_finder.c:2331-2349: widen after a quiet batch, back off after
@@ -1260,15 +1414,32 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // the rung's voids into the device mask, for the rungs below
     th = now_ms();
     if (st == SIF_OK)
-      st = append_voids(RV.cx.data(), RV.cy.data(), RV.cz.data(), RV.r.data(),
-        (uint32_t)RV.cx.size(), R.box, scale, l, n);
+      st = append_rung_voids();
     if (timing_active())
       timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
+    // the next rung's start, if it went ahead, brought to its end
+    const bool started = st == SIF_OK && rs.rung == i + 1 && rs.step == 2;
+    if (started)
+      st = finish_filter(true);
+    S.masked -= masked_since < S.masked ? masked_since : S.masked;
     if (stats_out)
       stats_out[i] = S;
     if (st == SIF_OK && R.on_rung)
       R.on_rung((uint32_t)(i + 1), V.cx.data(), V.cy.data(), V.cz.data(), V.r.data(),
         (uint64_t)V.cx.size(), mismatched);
+    // The next rung's newer voids: none, or, when its start went ahead, the
+    // voids of this rung its prefilter did not see (already on the device).
+    if (started && rv_mark < RV.cx.size()) {
+      const std::vector<float> tx(RV.cx.begin() + rv_mark, RV.cx.end()),
+        ty(RV.cy.begin() + rv_mark, RV.cy.end()), tz(RV.cz.begin() + rv_mark, RV.cz.end()),
+        tr(RV.r.begin() + rv_mark, RV.r.end());
+      RV.clear();
+      for (size_t v = 0; v < tx.size(); v++)
+        RV.add(tx[v], ty[v], tz[v], tr[v]);
+    } else {
+      RV.clear();
+    }
+    rv_appended = RV.cx.size();
   }
 
   // the auxiliary stream may still smooth a rung the loop left (or read work)
@@ -1286,9 +1457,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     SIF_HIP(hipMemcpy(pf.data(), g_dev.pf_counts.ptr, pf.size() * sizeof(uint64_t),
       hipMemcpyDeviceToHost));
     for (size_t r = 0; r < n_rungs; r++) {
-      stats_out[r].masked += pf[3 * r];
-      stats_out[r].proxy += pf[3 * r + 1];
-      stats_out[r].overlap += pf[3 * r + 2];
+      stats_out[r].masked += pf[3 * r] - pf_since[3 * r];
+      stats_out[r].proxy += pf[3 * r + 1] - pf_since[3 * r + 1];
+      stats_out[r].overlap += pf[3 * r + 2] - pf_since[3 * r + 2];
     }
   }
 

@@ -125,7 +125,8 @@ __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
 __global__ void __launch_bounds__(256) prefilter_kernel(uint32_t* __restrict__ flat,
   uint32_t n, uint32_t grid_n, float l, float box, const uint64_t* __restrict__ mask,
   int32_t proxy, float radius, float f, float scale, VoidIndex V,
-  uint32_t* __restrict__ keep, unsigned long long* __restrict__ counts) {
+  uint32_t* __restrict__ keep, unsigned long long* __restrict__ counts,
+  uint8_t* __restrict__ why_out) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t why = 0;  // 0 kept, 1 masked, 2 proxy, 3 overlap
   if (k < n) {
@@ -151,6 +152,8 @@ __global__ void __launch_bounds__(256) prefilter_kernel(uint32_t* __restrict__ f
       }
     }
     keep[k] = why == 0 ? 1u : 0u;
+    if (why_out)
+      why_out[k] = (uint8_t)why;
     if (edge)
       flat[k] = node | 0x80000000u;
   }
@@ -682,10 +685,36 @@ int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
 
 int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
-  const VoidIndex& V, uint32_t* keep, uint64_t* d_counts) {
+  const VoidIndex& V, uint32_t* keep, uint64_t* d_counts, uint8_t* why) {
   if (n)
     prefilter_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(flat, n, grid_n, l, box,
-      mask, proxy, radius, f, scale, V, keep, reinterpret_cast<unsigned long long*>(d_counts));
+      mask, proxy, radius, f, scale, V, keep, reinterpret_cast<unsigned long long*>(d_counts),
+      why);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
+__global__ void __launch_bounds__(256) masked_since_kernel(const uint32_t* __restrict__ flat,
+  const uint8_t* __restrict__ why, uint64_t n, const uint64_t* __restrict__ mask,
+  unsigned long long* __restrict__ out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t w = 4;  // none
+  if (k < n && mask_bit(mask, flat[k] & 0x7fffffffu))
+    w = why[k];
+  // one atomic per wavefront and verdict
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t r = 0; r < 4; r++) {
+    const uint64_t votes = __ballot(w == r);
+    if (lane == 0 && votes)
+      atomicAdd(&out[r], (unsigned long long)__popcll(votes));
+  }
+}
+
+int count_masked_since(const uint32_t* flat, const uint8_t* why, uint64_t n,
+  const uint64_t* mask, uint64_t* d_out) {
+  if (n)
+    masked_since_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(flat, why, n, mask,
+      reinterpret_cast<unsigned long long*>(d_out));
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }

@@ -86,6 +86,25 @@ int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n,
 // of the same field: the listed cells whose mask bit is clear.
 int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total,
   uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count, uint32_t* key_differ);
+// The same scans in two steps, so that the host can work while the counting
+// pass runs: scan_begin enqueues it and the copy of its results, scan_finish
+// waits for them and enqueues the writing pass when the count fits cap. One
+// job at a time per host thread.
+struct ScanJob {
+  const float* delta = nullptr;      // the field, when lists is null
+  const CellLists* lists = nullptr;  // or a listing smoothing's lists
+  const uint64_t* mask = nullptr;
+  uint64_t total = 0;
+  float t = 0.0f;
+  uint32_t minima_n = 0;
+  // results, after scan_finish
+  uint64_t count = 0;
+  uint32_t key_differ = 0xffffffffu;
+  uint64_t listed = 0;
+  bool begun = false;
+};
+int scan_begin(ScanJob& J);
+int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap);
 int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
 
 // grow.hip
@@ -174,7 +193,12 @@ int node_centres(const uint32_t* flat, uint32_t n, uint32_t grid_n, float l,
 // keep[k] and the refusal counts (masked, proxy, overlap) added to d_counts.
 int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
-  const VoidIndex& V, uint32_t* keep, uint64_t* d_counts);
+  const VoidIndex& V, uint32_t* keep, uint64_t* d_counts, uint8_t* why = nullptr);
+// Of n prefiltered candidates (flat nodes, bit 31 aside, and their prefilter
+// verdicts why: 0 kept, 1 masked, 2 proxy, 3 overlap), those whose node the
+// mask now covers, counted by verdict into d_out[4] (added to).
+int count_masked_since(const uint32_t* flat, const uint8_t* why, uint64_t n,
+  const uint64_t* mask, uint64_t* d_out);
 // The class-split CSR of nv device-resident voids: d_start has
 // VOID_CLASSES * 32^3 + 1 entries, d_items nv.
 int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
@@ -196,6 +220,12 @@ int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,
 // Two arrays compacted by the same keep flags with one scan.
 int compact2_u32(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uint64_t n,
   uint32_t* out_a, uint32_t* out_b, uint64_t* count);
+// compact2_u32 in two steps: the compaction and the copy of its count to
+// host memory enqueued, then the wait for the count. One at a time per host
+// thread.
+int compact2_u32_begin(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uint64_t n,
+  uint32_t* out_a, uint32_t* out_b);
+int compact2_u32_finish(uint64_t* count);
 
 // One exodus run on device-resident data (exodus.hip).
 struct RunSpec {

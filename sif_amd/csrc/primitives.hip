@@ -315,6 +315,40 @@ int compact2_u32(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uin
   return SIF_OK;
 }
 
+namespace {
+thread_local uint64_t* g_ccount_host = nullptr;  // pinned
+thread_local hipEvent_t g_ccount_ready = nullptr;
+}
+
+int compact2_u32_begin(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uint64_t n,
+  uint32_t* out_a, uint32_t* out_b) {
+  int st = g_cpos.reserve((n ? n : 1) * 8);
+  if (st == SIF_OK)
+    st = g_ccount.reserve(8);
+  if (st != SIF_OK)
+    return st;
+  if (!g_ccount_host)
+    SIF_HIP(hipHostMalloc((void**)&g_ccount_host, 8, hipHostMallocDefault));
+  if (!g_ccount_ready)
+    SIF_HIP(hipEventCreateWithFlags(&g_ccount_ready, hipEventDisableTiming));
+  st = exclusive_scan_u32(keep, n, g_cpos.as<uint64_t>(), g_ccount.as<uint64_t>());
+  if (st != SIF_OK)
+    return st;
+  if (n)
+    compact2_scatter<<<grid_for(n, 256), 256, 0, stream()>>>(a, b, keep,
+      g_cpos.as<uint64_t>(), n, out_a, out_b);
+  SIF_HIP(hipGetLastError());
+  SIF_HIP(hipMemcpyAsync(g_ccount_host, g_ccount.ptr, 8, hipMemcpyDeviceToHost, stream()));
+  SIF_HIP(hipEventRecord(g_ccount_ready, stream()));
+  return SIF_OK;
+}
+
+int compact2_u32_finish(uint64_t* count) {
+  SIF_HIP(hipEventSynchronize(g_ccount_ready));
+  *count = *g_ccount_host;
+  return SIF_OK;
+}
+
 int compact_u32(const uint32_t* in, const uint32_t* keep, uint64_t n,
   uint32_t* out, uint64_t* count) {
   int st = g_cpos.reserve((n ? n : 1) * 8);

@@ -204,75 +204,6 @@ namespace {
 thread_local DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span;
 }
 
-// Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)
-// and their number to *count; keys are the order-preserving integer images of
-// the depths. When the count exceeds `cap` nothing is written and the caller
-// grows the buffers and calls again. minima_n: 0 for every cell below the
-// threshold; the grid's side (total = minima_n^3) for the local minima alone
-// (sif__finder_scan_refined with minima_only and no refinement,
-// utils.c:406-552).
-int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
-  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count,
-  uint32_t* key_differ) {
-  if (total >= 0x80000000ull) {
-    error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^31 - 1)",
-      (unsigned long long)total);
-    return SIF_ERR_RANGE;
-  }
-  if (reinterpret_cast<uintptr_t>(delta) % 16 != 0) {
-    error_set(SIF_ERR_INVALID, "scan: the grid must be 16-byte aligned");
-    return SIF_ERR_INVALID;
-  }
-  if (minima_n && (uint64_t)minima_n * minima_n * minima_n != total) {
-    error_set(SIF_ERR_INVALID, "scan: minima on a %u^3 grid of %llu cells", minima_n,
-      (unsigned long long)total);
-    return SIF_ERR_INVALID;
-  }
-  const uint64_t tiles = (total + SC_TILE - 1) / SC_TILE;
-  int st = g_tiles.reserve((tiles ? tiles : 1) * 8);
-  if (st == SIF_OK)
-    st = g_tile_pos.reserve((tiles ? tiles : 1) * 8);
-  if (st == SIF_OK)
-    st = g_count.reserve(16);  // the count, then the keys' AND and OR
-  if (st == SIF_OK)
-    st = g_tile_span.reserve((tiles ? tiles : 1) * 8);
-  if (st != SIF_OK)
-    return st;
-  *count = 0;
-  if (key_differ)
-    *key_differ = 0xffffffffu;
-  if (total == 0)
-    return SIF_OK;
-  timing_begin(SIF_AMD_STAGE_SCAN);
-  cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
-    minima_n, g_tiles.as<uint64_t>(), g_tile_span.as<uint32_t>());
-  SIF_HIP(hipGetLastError());
-  span_reduce_kernel<<<1, 1024, 0, stream()>>>(g_tile_span.as<uint32_t>(), tiles,
-    g_count.as<uint32_t>() + 2);
-  SIF_HIP(hipGetLastError());
-  st = exclusive_scan_u64(g_tiles.as<uint64_t>(), tiles, g_tile_pos.as<uint64_t>(),
-    g_count.as<uint64_t>());
-  if (st != SIF_OK)
-    return st;
-  uint64_t back[2] = {0, 0};  // the count; the keys' AND (low) and OR (high)
-  SIF_HIP(hipMemcpyAsync(back, g_count.ptr, 16, hipMemcpyDeviceToHost, stream()));
-  SIF_HIP(hipStreamSynchronize(stream()));
-  const uint64_t n = back[0];
-  *count = n;
-  if (key_differ)
-    *key_differ = (uint32_t)back[1] ^ (uint32_t)(back[1] >> 32);
-  if (n <= cap && n > 0) {
-    cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(delta, mask, total, t,
-      minima_n, g_tile_pos.as<uint64_t>(), keys, flat);
-    SIF_HIP(hipGetLastError());
-  }
-  timing_end(SIF_AMD_STAGE_SCAN);
-  if (timing_active())  // the field and the mask once, the candidates once
-    timing_bytes(SIF_AMD_STAGE_SCAN, 4.0 * (double)total + (double)total / 8.0 +
-      (n <= cap ? 8.0 * (double)n : 0.0));
-  return SIF_OK;
-}
-
 // --- the same scan from cell lists (CellLists, a listing smoothing's) ---
 //
 // One workgroup per list block: the listed cells whose mask bit is clear,
@@ -351,14 +282,35 @@ __global__ void __launch_bounds__(SC_THREADS) list_write_kernel(
   }
 }
 
-int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total,
-  uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count, uint32_t* key_differ) {
-  if (total >= 0x80000000ull) {
+namespace {
+thread_local uint64_t* g_back = nullptr;  // pinned: the count, AND | OR << 32, cells listed
+thread_local hipEvent_t g_back_ready = nullptr;
+}
+
+// The counting pass of a scan job, and the copy of its results to host
+// memory (scan_finish reads them).
+int scan_begin(ScanJob& J) {
+  J.count = 0;
+  J.key_differ = 0xffffffffu;
+  J.listed = 0;
+  J.begun = false;
+  if (J.total >= 0x80000000ull) {
     error_set(SIF_ERR_RANGE, "scan: %llu cells (at most 2^31 - 1)",
-      (unsigned long long)total);
+      (unsigned long long)J.total);
     return SIF_ERR_RANGE;
   }
-  const uint64_t tiles = lists.blocks;
+  if (!J.lists) {
+    if (reinterpret_cast<uintptr_t>(J.delta) % 16 != 0) {
+      error_set(SIF_ERR_INVALID, "scan: the grid must be 16-byte aligned");
+      return SIF_ERR_INVALID;
+    }
+    if (J.minima_n && (uint64_t)J.minima_n * J.minima_n * J.minima_n != J.total) {
+      error_set(SIF_ERR_INVALID, "scan: minima on a %u^3 grid of %llu cells", J.minima_n,
+        (unsigned long long)J.total);
+      return SIF_ERR_INVALID;
+    }
+  }
+  const uint64_t tiles = J.lists ? J.lists->blocks : (J.total + SC_TILE - 1) / SC_TILE;
   int st = g_tiles.reserve((tiles ? tiles : 1) * 8);
   if (st == SIF_OK)
     st = g_tile_pos.reserve((tiles ? tiles : 1) * 8);
@@ -368,16 +320,22 @@ int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total
     st = g_tile_span.reserve((tiles ? tiles : 1) * 8);
   if (st != SIF_OK)
     return st;
-  *count = 0;
-  if (key_differ)
-    *key_differ = 0xffffffffu;
-  if (tiles == 0)
+  if (!g_back)
+    SIF_HIP(hipHostMalloc((void**)&g_back, 32, hipHostMallocDefault));
+  if (!g_back_ready)
+    SIF_HIP(hipEventCreateWithFlags(&g_back_ready, hipEventDisableTiming));
+  if (J.total == 0 || tiles == 0)
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SCAN);
-  SIF_HIP(hipMemsetAsync(g_count.as<uint64_t>() + 2, 0, 8, stream()));
-  list_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(lists.keys, lists.local,
-    lists.count, mask, g_tiles.as<uint64_t>(), g_tile_span.as<uint32_t>(),
-    reinterpret_cast<unsigned long long*>(g_count.as<uint64_t>() + 2));
+  if (J.lists) {
+    SIF_HIP(hipMemsetAsync(g_count.as<uint64_t>() + 2, 0, 8, stream()));
+    list_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
+      J.lists->local, J.lists->count, J.mask, g_tiles.as<uint64_t>(),
+      g_tile_span.as<uint32_t>(), reinterpret_cast<unsigned long long*>(g_count.as<uint64_t>() + 2));
+  } else {
+    cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
+      J.t, J.minima_n, g_tiles.as<uint64_t>(), g_tile_span.as<uint32_t>());
+  }
   SIF_HIP(hipGetLastError());
   span_reduce_kernel<<<1, 1024, 0, stream()>>>(g_tile_span.as<uint32_t>(), tiles,
     g_count.as<uint32_t>() + 2);
@@ -386,39 +344,165 @@ int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total
     g_count.as<uint64_t>());
   if (st != SIF_OK)
     return st;
-  // the count; the keys' AND (low) and OR (high); the cells listed
-  uint64_t back[3] = {0, 0, 0};
-  SIF_HIP(hipMemcpyAsync(back, g_count.ptr, 24, hipMemcpyDeviceToHost, stream()));
-  SIF_HIP(hipStreamSynchronize(stream()));
-  const uint64_t n = back[0];
-  *count = n;
-  if (key_differ)
-    *key_differ = (uint32_t)back[1] ^ (uint32_t)(back[1] >> 32);
-  if (n <= cap && n > 0) {
-    list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(lists.keys, lists.local,
-      lists.count, mask, g_tile_pos.as<uint64_t>(), keys, flat);
-    SIF_HIP(hipGetLastError());
-  }
+  SIF_HIP(hipMemcpyAsync(g_back, g_count.ptr, J.lists ? 24 : 16, hipMemcpyDeviceToHost,
+    stream()));
+  SIF_HIP(hipEventRecord(g_back_ready, stream()));
   timing_end(SIF_AMD_STAGE_SCAN);
+  J.begun = true;
+  return SIF_OK;
+}
+
+// Waits for the counting pass's results and, when the count fits cap,
+// enqueues the writing pass: the candidates in flat order with their keys.
+int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap) {
+  if (!J.begun)
+    return SIF_OK;
+  J.begun = false;
+  SIF_HIP(hipEventSynchronize(g_back_ready));
+  const uint64_t n = g_back[0];
+  J.count = n;
+  J.key_differ = (uint32_t)g_back[1] ^ (uint32_t)(g_back[1] >> 32);
+  J.listed = J.lists ? g_back[2] : 0;
+  const uint64_t tiles = J.lists ? J.lists->blocks : (J.total + SC_TILE - 1) / SC_TILE;
+  if (n <= cap && n > 0) {
+    timing_begin(SIF_AMD_STAGE_SCAN);
+    if (J.lists)
+      list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
+        J.lists->local, J.lists->count, J.mask, g_tile_pos.as<uint64_t>(), keys, flat);
+    else
+      cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
+        J.t, J.minima_n, g_tile_pos.as<uint64_t>(), keys, flat);
+    SIF_HIP(hipGetLastError());
+    timing_end(SIF_AMD_STAGE_SCAN);
+  }
   if (timing_active()) {
-    // the lists twice and the candidates once; the smoothing that made the
-    // lists wrote them once
-    const double listed = (double)back[2];
-    timing_bytes(SIF_AMD_STAGE_SCAN, 12.0 * listed + (n <= cap ? 8.0 * (double)n : 0.0));
-    timing_bytes(SIF_AMD_STAGE_SMOOTH, 6.0 * listed);
+    const double written = n <= cap ? 8.0 * (double)n : 0.0;
+    if (J.lists) {
+      // the lists twice and the candidates once; the smoothing that made
+      // the lists wrote them once
+      const double listed = (double)J.listed;
+      timing_bytes(SIF_AMD_STAGE_SCAN, 12.0 * listed + written);
+      timing_bytes(SIF_AMD_STAGE_SMOOTH, 6.0 * listed);
+    } else {  // the field and the mask once, the candidates once
+      timing_bytes(SIF_AMD_STAGE_SCAN, 4.0 * (double)J.total + (double)J.total / 8.0 + written);
+    }
   }
   return SIF_OK;
 }
 
-// Deepest first; equal depths keep their order (a stable radix sort), so a
-// list in flat order comes out ordered by (depth, flat index). Bit 31 of a
-// flat value rides along untouched. key_differ: the bits on which the keys
-// may differ (from the scan that listed them, or a superset of them).
+// Writes the candidates, in flat order, to keys/flat (device, capacity `cap`)
+// and their number to *count; keys are the order-preserving integer images of
+// the depths. When the count exceeds `cap` nothing is written and the caller
+// grows the buffers and calls again. minima_n: 0 for every cell below the
+// threshold; the grid's side (total = minima_n^3) for the local minima alone
+// (sif__finder_scan_refined with minima_only and no refinement,
+// utils.c:406-552).
+int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
+  float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count,
+  uint32_t* key_differ) {
+  ScanJob J;
+  J.delta = delta;
+  J.mask = mask;
+  J.total = total;
+  J.t = t;
+  J.minima_n = minima_n;
+  int st = scan_begin(J);
+  if (st == SIF_OK)
+    st = scan_finish(J, keys, flat, cap);
+  *count = J.count;
+  if (key_differ)
+    *key_differ = J.key_differ;
+  return st;
+}
+
+int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total,
+  uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count, uint32_t* key_differ) {
+  ScanJob J;
+  J.lists = &lists;
+  J.mask = mask;
+  J.total = total;
+  int st = scan_begin(J);
+  if (st == SIF_OK)
+    st = scan_finish(J, keys, flat, cap);
+  *count = J.count;
+  if (key_differ)
+    *key_differ = J.key_differ;
+  return st;
+}
+
+// A list of at most SMALL_SORT candidates sorts in one workgroup: each entry
+// becomes (key << 32 | its place in the list), every such value distinct, so
+// the ascending order is (key, place) -- the stable order -- whatever the
+// sorting network; a bitonic network in LDS, then each place's value
+// gathered. One launch instead of the radix sort's few per digit; the
+// network's LDS traffic grows as n log^2 n on one CU, so past 8192 entries
+// the radix sort over the whole device is the faster.
+constexpr uint32_t SMALL_SORT = 8192;
+constexpr uint32_t SMALL_SORT_THREADS = 1024;
+
+__global__ void __launch_bounds__(SMALL_SORT_THREADS) small_sort_kernel(
+  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n) {
+  __shared__ uint64_t s[SMALL_SORT];
+  uint32_t m = 1;
+  while (m < n)
+    m <<= 1;
+  for (uint32_t i = threadIdx.x; i < m; i += SMALL_SORT_THREADS)
+    s[i] = i < n ? ((uint64_t)keys[i] << 32) | i : ~0ull;
+  __syncthreads();
+  for (uint32_t k = 2; k <= m; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += SMALL_SORT_THREADS) {
+        const uint32_t p = i ^ j;
+        if (p > i) {
+          const uint64_t a = s[i], b = s[p];
+          if ((a > b) == ((i & k) == 0)) {
+            s[i] = b;
+            s[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  constexpr int PER = SMALL_SORT / SMALL_SORT_THREADS;
+  uint32_t kv[PER], vv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const uint32_t i = threadIdx.x + q * SMALL_SORT_THREADS;
+    if (i < n) {
+      const uint64_t e = s[i];
+      kv[q] = (uint32_t)(e >> 32);
+      vv[q] = vals[(uint32_t)e];
+    }
+  }
+  __syncthreads();  // every read of vals before any write
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const uint32_t i = threadIdx.x + q * SMALL_SORT_THREADS;
+    if (i < n) {
+      keys[i] = kv[q];
+      vals[i] = vv[q];
+    }
+  }
+}
+
+// Deepest first; equal depths keep their order (a stable radix sort, or the
+// one-workgroup sort of (key, place)), so a list in flat order comes out
+// ordered by (depth, flat index). Bit 31 of a flat value rides along
+// untouched. key_differ: the bits on which the keys may differ (from the scan
+// that listed them, or a superset of them).
 int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n, uint32_t key_differ) {
   if (n == 0)
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SORT);
-  const int st = radix_sort_u32(keys, flat, (uint32_t)n, 32, key_differ);
+  int st = SIF_OK;
+  if (n <= SMALL_SORT) {
+    small_sort_kernel<<<1, SMALL_SORT_THREADS, 0, stream()>>>(keys, flat, (uint32_t)n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      st = hip_check(e, "small_sort_kernel");
+  } else {
+    st = radix_sort_u32(keys, flat, (uint32_t)n, 32, key_differ);
+  }
   timing_end(SIF_AMD_STAGE_SORT);
   return st;
 }
