@@ -339,7 +339,7 @@ SIF_API int sif_amd_squared_residual(const float* d_p, const float* d_target,
  * (exodus_finder.c:183-213, 2331-2349: a batch that accepts nothing widens
  * the window by the floor, one that accepts divides it down, within
  * [floor, ceiling]). The reference fixes both at compile time and the
- * catalogue does not depend on them; this build's defaults are 4096 and
+ * catalogue does not depend on them; this build's defaults are 8192 and
  * 32768, sized to fill the GPU. 1 <= floor <= ceiling <= 32768; (0, 0)
  * restores the defaults. Lets small inputs cross batch boundaries. Not
  * synchronised, like the reference's settings table. */

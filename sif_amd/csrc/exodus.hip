@@ -49,11 +49,13 @@ namespace sif_amd {
 namespace {
 
 // Growth batches: the reference's adaptive rule (exodus_finder.c:2331-2349)
-// with a floor that fills the GPU (about 2048 wavefronts resident). The
-// window only groups candidates, so any floor and ceiling give the same
+// with a floor that keeps the GPU full while the batch's late, large spheres
+// finish (about 2048 wavefronts resident; a floor of 8192 grows a 512^3
+// step's candidates in fewer, fuller launches than 4096 did). The window
+// only groups candidates, so any floor and ceiling give the same
 // catalogue; sif_amd_set_batch_window() lowers them (BATCH_ROOM is what the
 // slots hold).
-constexpr uint32_t BATCH_FLOOR_DEFAULT = 4096;
+constexpr uint32_t BATCH_FLOOR_DEFAULT = 8192;
 constexpr uint32_t BATCH_ROOM = 32768;
 uint32_t g_batch_floor = BATCH_FLOOR_DEFAULT;
 uint32_t g_batch_max = BATCH_ROOM;
