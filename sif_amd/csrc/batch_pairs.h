@@ -55,11 +55,27 @@ struct BatchPairParams {
   int32_t proxy;     // the rung's pole distance in nodes
 };
 
+// A candidate's void cell (overlap.h), (wa 32 + wb) 32 + wc, as the pair
+// scan's region test reads it; BATCH_NO_CELL for one that did not grow, whose
+// void would paint and reach nothing.
+constexpr uint32_t BATCH_NO_CELL = 0xffffffffu;
+SIF_HD uint32_t batch_cell_code(uint32_t node, float r, const BatchPairParams& P) {
+  if (!(r >= 0.0f))
+    return BATCH_NO_CELL;
+  const uint32_t n = P.n, nn = n * n;
+  const uint32_t jx = node / nn, jy = (node / n) % n, jz = node % n;
+  const uint32_t wa = void_axis_cell((float)jx * P.l, P.inv_l),
+                 wb = void_axis_cell((float)jy * P.l, P.inv_l),
+                 wc = void_axis_cell((float)jz * P.l, P.inv_l);
+  return (wa << 10) | (wb << 5) | wc;
+}
+
 // Device scratch and results of batch_pairs(): per candidate its count
 // (cnt8, read by the host) and offset into eff (the exclusive sum of the
 // counts in batch order, which the host can sum again from cnt8).
 struct BatchPairScratch {
   float* set_max_r;   // 1: the batch's largest grown radius
+  uint32_t* cells;    // count: batch_cell_code
   uint32_t* cnt32;    // count
   uint8_t* cnt8;      // count
   uint64_t* offset;   // count

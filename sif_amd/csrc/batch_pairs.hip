@@ -27,13 +27,17 @@
 namespace sif_amd {
 
 // The largest grown radius of the batch (the set's max_r of its region
-// scans); -1 when none grew.
-__global__ void __launch_bounds__(1024) batch_max_radius_kernel(const float* __restrict__ rad,
-  uint32_t count, float* __restrict__ out) {
+// scans; -1 when none grew), and each candidate's cell code.
+__global__ void __launch_bounds__(1024) batch_prep_kernel(const uint32_t* __restrict__ flat,
+  const float* __restrict__ rad, uint32_t count, BatchPairParams P, float* __restrict__ out,
+  uint32_t* __restrict__ cells) {
   __shared__ float wmax[16];
   float m = -1.0f;
-  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
-    m = fmaxf(m, rad[i]);
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+    const float r = rad[i];
+    m = fmaxf(m, r);
+    cells[i] = batch_cell_code(flat[i], r, P);
+  }
   for (int off = 32; off > 0; off >>= 1)
     m = fmaxf(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0)
@@ -50,17 +54,17 @@ int batch_pairs(const uint32_t* flat, const float* rad, const uint32_t* grown_st
   uint32_t count, const BatchPairParams& P, BatchPairScratch& W) {
   if (count == 0)
     return SIF_OK;
-  batch_max_radius_kernel<<<1, 1024, 0, stream()>>>(rad, count, W.set_max_r);
+  batch_prep_kernel<<<1, 1024, 0, stream()>>>(flat, rad, count, P, W.set_max_r, W.cells);
   SIF_HIP(hipGetLastError());
   const unsigned blocks = (count + BP_WAVES - 1) / BP_WAVES;
   batch_pairs_kernel<false><<<blocks, 64 * BP_WAVES, 0, stream()>>>(flat, rad, grown_state,
-    count, P, W.set_max_r, W.cnt32, W.cnt8, nullptr, nullptr, 0);
+    count, P, W.set_max_r, W.cells, W.cnt32, W.cnt8, nullptr, nullptr, 0);
   SIF_HIP(hipGetLastError());
   int st = exclusive_scan_u32(W.cnt32, count, W.offset, nullptr);
   if (st != SIF_OK)
     return st;
   batch_pairs_kernel<true><<<blocks, 64 * BP_WAVES, 0, stream()>>>(flat, rad, grown_state,
-    count, P, W.set_max_r, nullptr, W.cnt8, W.offset, W.eff, W.capacity);
+    count, P, W.set_max_r, W.cells, nullptr, W.cnt8, W.offset, W.eff, W.capacity);
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }

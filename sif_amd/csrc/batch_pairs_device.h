@@ -19,7 +19,8 @@ template <bool WRITE>
 __global__ void __launch_bounds__(64 * BP_WAVES) batch_pairs_kernel(
   const uint32_t* __restrict__ flat, const float* __restrict__ rad,
   const uint32_t* __restrict__ grown_state, uint32_t count, BatchPairParams P,
-  const float* __restrict__ set_max_r, uint32_t* __restrict__ cnt32,
+  const float* __restrict__ set_max_r, const uint32_t* __restrict__ cells,
+  uint32_t* __restrict__ cnt32,
   uint8_t* __restrict__ cnt8, const uint64_t* __restrict__ offset,
   PairEffect* __restrict__ eff, uint64_t capacity) {
   const uint32_t lane = threadIdx.x & 63;
@@ -70,42 +71,56 @@ __global__ void __launch_bounds__(64 * BP_WAVES) batch_pairs_kernel(
     poles.set(ix, iy, iz, P.proxy, n);
   if (want_pole_r)
     poles_r.set(ix, iy, iz, pole_r, n);
+  // The region as one bit word per axis: most earlier candidates lie outside
+  // it, and a lane tells so from the candidate's cell code alone, four
+  // groups of 64 loaded at once; a group with no lane inside is passed over.
+  const uint32_t bx = R.axis_bits(0), by = R.axis_bits(1), bz = R.axis_bits(2);
+  constexpr uint32_t U = 4;
   uint32_t found = 0;  // wave-uniform
   const uint64_t below = (1ull << lane) - 1ull;
-  for (uint32_t a0 = 0; a0 < b && found < BATCH_EFF_FULL && !R.empty(); a0 += 64) {
-    const uint32_t a = a0 + lane;
-    PairEffect e{};
-    bool any = false;
-    if (a < b) {
-      const float ra = rad[a];
-      if (ra >= 0.0f) {
+  for (uint32_t a0 = 0; a0 < b && found < BATCH_EFF_FULL && !R.empty(); a0 += 64 * U) {
+    uint32_t code[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t a = a0 + u * 64 + lane;
+      code[u] = a < b ? cells[a] : BATCH_NO_CELL;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t c = code[u];
+      const bool near = c != BATCH_NO_CELL &&
+                        ((bx >> (c >> 10)) & (by >> ((c >> 5) & 31u)) & (bz >> (c & 31u)) & 1u);
+      if (__ballot(near) == 0ull)
+        continue;
+      const uint32_t a = a0 + u * 64 + lane;
+      PairEffect e{};
+      bool any = false;
+      if (near) {
+        const float ra = rad[a];
         const uint32_t fa = flat[a];
         const uint32_t jx = fa / nn, jy = (fa / n) % n, jz = fa % n;
         const float ax = (float)jx * P.l, ay = (float)jy * P.l, az = (float)jz * P.l;
-        const uint32_t wa = void_axis_cell(ax, P.inv_l), wb = void_axis_cell(ay, P.inv_l),
-                       wc = void_axis_cell(az, P.inv_l);
-        if (R.holds(wa, wb, wc)) {
-          e.a = a;
-          if (need_state)
-            e.state = (uint8_t)pair_state(q, wa, wb, wc, ax, ay, az, ra);
-          if (need_state_r)
-            e.state_r = (uint8_t)pair_state(qr, wa, wb, wc, ax, ay, az, ra);
-          PaintSphere S;
-          paint_sphere(S, ax, ay, az, P.scale * ra, P.l, n);
-          e.node = want_node && paint_hits(S, ix, iy, iz);
-          e.pole = want_pole && poles.painted(S, ix, iy, iz);
-          e.pole_r = want_pole_r && poles_r.painted(S, ix, iy, iz);
-          any = e.node | e.pole | e.pole_r | e.state | e.state_r;
-        }
+        const uint32_t wa = c >> 10, wb = (c >> 5) & 31u, wc = c & 31u;
+        e.a = a;
+        if (need_state)
+          e.state = (uint8_t)pair_state(q, wa, wb, wc, ax, ay, az, ra);
+        if (need_state_r)
+          e.state_r = (uint8_t)pair_state(qr, wa, wb, wc, ax, ay, az, ra);
+        PaintSphere S;
+        paint_sphere(S, ax, ay, az, P.scale * ra, P.l, n);
+        e.node = want_node && paint_hits(S, ix, iy, iz);
+        e.pole = want_pole && poles.painted(S, ix, iy, iz);
+        e.pole_r = want_pole_r && poles_r.painted(S, ix, iy, iz);
+        any = e.node | e.pole | e.pole_r | e.state | e.state_r;
       }
+      const uint64_t m = __ballot(any);
+      if constexpr (WRITE) {
+        const uint32_t at = found + (uint32_t)__popcll(m & below);
+        if (any)
+          eff[at0 + at] = e;
+      }
+      found += (uint32_t)__popcll(m);
     }
-    const uint64_t m = __ballot(any);
-    if constexpr (WRITE) {
-      const uint32_t at = found + (uint32_t)__popcll(m & below);
-      if (any)
-        eff[at0 + at] = e;
-    }
-    found += (uint32_t)__popcll(m);
   }
   if constexpr (!WRITE) {
     if (lane == 0) {

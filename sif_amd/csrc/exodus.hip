@@ -125,7 +125,7 @@ WorkPool& host_pool() {
 struct BatchSlot {
   DeviceBuffer d_flat, d_cx, d_cy, d_cz;
   // the batch's pair effects' scratch (batch_pairs.hip)
-  DeviceBuffer d_cnt32, d_off, d_setmax;
+  DeviceBuffer d_cnt32, d_off, d_setmax, d_cells;
   // The launch's results in one block, copied to host memory at once:
   // radius, reason, state, effect count, enclosed count, algorithmic bytes,
   // then the effects list. The typed pointers into it are set per launch
@@ -212,6 +212,7 @@ int reserve_slots() {
     if (e == SIF_OK) e = B.d_cnt32.reserve(BATCH_ROOM * 4);
     if (e == SIF_OK) e = B.d_off.reserve(BATCH_ROOM * 8);
     if (e == SIF_OK) e = B.d_setmax.reserve(sizeof(float));
+    if (e == SIF_OK) e = B.d_cells.reserve(BATCH_ROOM * 4);
     if (e == SIF_OK) e = B.d_out.reserve(BATCH_OUT_BYTES);
     if (e == SIF_OK) e = host_alloc((void**)&B.h_out, BATCH_OUT_BYTES);
     if (e == SIF_OK) e = host_alloc((void**)&B.h_flat, BATCH_ROOM * 4);
@@ -391,40 +392,111 @@ int exodus_prepare_spectrum(const float* d_delta, uint32_t n) {
 }
 
 // The top-hat LUT of every rung (fft.c:535-566, float libm on the host, as
-// the reference builds it), built by host threads while the device runs the
-// forward transform, and uploaded once; plus the window axis.
-static int exodus_prepare_luts(uint32_t n, float box, const std::vector<float>& radii,
-  sif_option opt) {
-  const uint64_t lsz = filter_lut_size(n);
-  std::vector<float> all(radii.size() * lsz);
-  const size_t workers = std::min<size_t>(
-    radii.size(), std::max<unsigned>(1u, std::min(std::thread::hardware_concurrency(), 16u)));
-  std::vector<std::thread> pool;
-  for (size_t w = 0; w < workers; w++)
-    pool.emplace_back([&, w]() {
-      std::vector<float> lut;
-      for (size_t i = w; i < radii.size(); i += workers) {
-        build_filter_lut(SIF_AMD_FILTER_TOP_HAT, radii[i], 0.0f, box, n, lut);
-        std::copy(lut.begin(), lut.end(), all.begin() + i * lsz);
-      }
-    });
-  for (std::thread& t : pool)
-    t.join();
-  int st = g_dev.luts.reserve(all.size() * sizeof(float));
-  if (st == SIF_OK)
-    st = g_dev.axis.reserve(n * sizeof(double));
+// the reference builds it). Host threads build the tables in rung order,
+// a chunk of entries at a time, into pinned memory; rung j's table goes to
+// the device on the stream of its smoothing as soon as its chunks are done.
+// The first smoothing then waits for one table rather than all of them, and
+// the rest are built while the device smooths and grows the first rungs. A
+// thread that needs a table not yet built builds its chunks itself.
+class LutBuilder {
+ public:
+  int start(uint32_t n, float box, const std::vector<float>& radii) {
+    box_ = box;
+    radii_ = &radii;
+    lsz_ = filter_lut_size(n);
+    per_ = (lsz_ + CHUNK - 1) / CHUNK;
+    total_ = per_ * radii.size();
+    int st = reserve_host(lsz_ * radii.size());
+    if (st != SIF_OK)
+      return st;
+    host_ = host_ptr();  // the workers' own thread_local slot is not this one
+    done_ = std::vector<std::atomic<uint64_t>>(radii.size());
+    for (auto& d : done_)
+      d.store(0, std::memory_order_relaxed);
+    next_.store(0, std::memory_order_relaxed);
+    stop_.store(false, std::memory_order_relaxed);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned workers = (unsigned)std::min<uint64_t>(total_, std::min(hw, 16u));
+    for (unsigned w = 0; w < workers; w++)
+      pool_.emplace_back([this]() {
+        while (!stop_.load(std::memory_order_relaxed) && claim()) {
+        }
+      });
+    return SIF_OK;
+  }
+  // Rung j's table to d_lut, the copy ordered before stream s's next work.
+  int issue(size_t j, float* d_lut, hipStream_t s) {
+    while (done_[j].load(std::memory_order_acquire) < per_)
+      if (!claim())
+        std::this_thread::yield();
+    SIF_HIP(hipMemcpyAsync(d_lut, host_ + j * lsz_, lsz_ * sizeof(float),
+      hipMemcpyHostToDevice, s));
+    return SIF_OK;
+  }
+  ~LutBuilder() {
+    stop_.store(true, std::memory_order_relaxed);
+    for (std::thread& t : pool_)
+      t.join();
+  }
+
+ private:
+  static constexpr uint64_t CHUNK = 16384;
+  // one chunk of the next table in rung order; false when none is left
+  bool claim() {
+    const uint64_t c = next_.fetch_add(1, std::memory_order_relaxed);
+    if (c >= total_)
+      return false;
+    const uint64_t j = c / per_, b = (c % per_) * CHUNK;
+    const uint64_t e = std::min(b + CHUNK, lsz_);
+    build_filter_lut_range(SIF_AMD_FILTER_TOP_HAT, (*radii_)[j], 0.0f, box_, (uint32_t)b,
+      (uint32_t)e, host_ + j * lsz_);
+    done_[j].fetch_add(1, std::memory_order_release);
+    return true;
+  }
+  // The pinned tables of this host thread's runs, kept between runs. Every
+  // copy from them is complete when a run returns (each precedes its rung's
+  // smoothing, which the run waits for), so a larger run may replace them.
+  static float*& host_ptr() {
+    static thread_local float* p = nullptr;
+    return p;
+  }
+  static uint64_t& host_cap() {
+    static thread_local uint64_t c = 0;
+    return c;
+  }
+  static int reserve_host(uint64_t floats) {
+    if (host_cap() >= floats)
+      return SIF_OK;
+    if (host_ptr()) {
+      SIF_HIP(hipHostFree(host_ptr()));
+      host_ptr() = nullptr;
+      host_cap() = 0;
+    }
+    SIF_HIP(hipHostMalloc((void**)&host_ptr(), floats * sizeof(float), hipHostMallocDefault));
+    host_cap() = floats;
+    return SIF_OK;
+  }
+  float box_ = 0.0f;
+  float* host_ = nullptr;  // the calling thread's pinned tables
+  const std::vector<float>* radii_ = nullptr;
+  uint64_t lsz_ = 0, per_ = 0, total_ = 0;
+  std::vector<std::atomic<uint64_t>> done_;
+  std::atomic<uint64_t> next_{0};
+  std::atomic<bool> stop_{false};
+  std::vector<std::thread> pool_;
+};
+
+// The window axis (fft.c:890-975), uploaded once a run.
+static int exodus_prepare_axis(uint32_t n, sif_option opt) {
+  int st = g_dev.axis.reserve(n * sizeof(double));
   if (st != SIF_OK)
     return st;
-  SIF_HIP(hipMemcpyAsync(g_dev.luts.ptr, all.data(), all.size() * sizeof(float),
-    hipMemcpyHostToDevice, stream()));
   const int order = window_order_of(opt);
   if (order > 0) {
     std::vector<double> axis;
     build_window_axis(n, order, (opt & SIF_FINDER_ALIASING_WINDOW) ? 1 : 0, axis);
-    SIF_HIP(hipMemcpyAsync(g_dev.axis.ptr, axis.data(), n * sizeof(double),
-      hipMemcpyHostToDevice, stream()));
+    SIF_HIP(hipMemcpy(g_dev.axis.ptr, axis.data(), n * sizeof(double), hipMemcpyHostToDevice));
   }
-  SIF_HIP(hipStreamSynchronize(stream()));
   return SIF_OK;
 }
 
@@ -500,14 +572,16 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if (st == SIF_OK)
       st = fft_r2c(R.d_random, n, g_dev.rspec.as<float2>(), g_dev.work.as<float2>());
   }
-  const double t_lut = now_ms();
+  LutBuilder luts;
+  const uint64_t lut_size = filter_lut_size(n);
   if (st == SIF_OK)
-    st = exodus_prepare_luts(n, R.box, R.radii, R.opt);
-  if (timing_active())
-    timing_add(SIF_AMD_STAGE_HOST, now_ms() - t_lut);
+    st = luts.start(n, R.box, R.radii);
+  if (st == SIF_OK)
+    st = g_dev.luts.reserve(R.radii.size() * lut_size * sizeof(float));
+  if (st == SIF_OK)
+    st = exodus_prepare_axis(n, R.opt);
   if (st != SIF_OK)
     return st;
-  const uint64_t lut_size = filter_lut_size(n);
   const bool windowed = window_order_of(R.opt) > 0;
   // A field that is only scanned for the cells at or below the threshold is
   // not written: the smoothing lists those cells (CellLists), and the scan
@@ -537,7 +611,13 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   // stream s: the smoothed contrast, or for a survey the smoothed data over
   // the smoothed randoms
   auto smooth = [&](size_t j, float* out, hipStream_t s) -> int {
-    const float* lut = g_dev.luts.as<float>() + j * lut_size;
+    float* lut = g_dev.luts.as<float>() + j * lut_size;
+    const double t_lut = now_ms();
+    int le = luts.issue(j, lut, s);
+    if (timing_active())
+      timing_add(SIF_AMD_STAGE_HOST, now_ms() - t_lut);
+    if (le != SIF_OK)
+      return le;
     const double* axis = windowed ? g_dev.axis.as<double>() : nullptr;
     if (use_lists)
       return fft_c2r_lut_lists(g_dev.spec.as<float2>(), n, lut, axis, R.threshold, lists[j & 1],
@@ -1159,6 +1239,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         bp.proxy = proxy;
         BatchPairScratch W;
         W.set_max_r = B.d_setmax.as<float>();
+        W.cells = B.d_cells.as<uint32_t>();
         W.cnt32 = B.d_cnt32.as<uint32_t>();
         W.cnt8 = B.p_cnt;
         W.offset = B.d_off.as<uint64_t>();

@@ -980,17 +980,15 @@ int gen_rows(uint32_t n, bool inv, const float* rin, const float2* cin, float2* 
 }  // namespace
 
 // src/math/fft.c:521-568, in float, on the host (the same libm as the
-// reference's own table), indexed by the integer |k|^2.
-void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
-  std::vector<float>& lut) {
-  const uint32_t h = n >> 1;
-  const uint32_t max_k2 = 3 * h * h;
-  lut.resize(max_k2 + 1);
+// reference's own table), indexed by the integer |k|^2: entries k2_begin to
+// k2_end - 1 of the table into lut[k2].
+void build_filter_lut_range(int filter, float r, float sigma, float box, uint32_t k2_begin,
+  uint32_t k2_end, float* lut) {
   const float factor = (float)(r * 2.0 * SIF_PI / box);
   const float factor2 = factor * factor;
   const float gfactor = (float)(sigma * 2.0 * SIF_PI / box);
   const float gfactor2 = gfactor * gfactor;
-  for (uint32_t k2 = 0; k2 <= max_k2; k2++) {
+  for (uint32_t k2 = k2_begin; k2 < k2_end; k2++) {
     float v;
     if (k2 == 0) {
       v = 1.0f;
@@ -1006,6 +1004,12 @@ void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
       v *= expf(-0.5f * gfactor2 * (float)k2);
     lut[k2] = v;
   }
+}
+
+void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
+  std::vector<float>& lut) {
+  lut.resize(filter_lut_size(n));
+  build_filter_lut_range(filter, r, sigma, box, 0, (uint32_t)lut.size(), lut.data());
 }
 
 // fft.c:890-975: per-axis window correction, 1/W (full) or W/S (aliasing).

@@ -68,6 +68,8 @@ int fft_c2r_lut_lists(const float2* spec, uint32_t n, const float* d_lut, const 
 uint64_t filter_lut_size(uint32_t n);
 void build_filter_lut(int filter, float r, float sigma, float box, uint32_t n,
   std::vector<float>& lut);
+void build_filter_lut_range(int filter, float r, float sigma, float box, uint32_t k2_begin,
+  uint32_t k2_end, float* lut);
 void build_window_axis(uint32_t n, int order, int mode, std::vector<double>& axis);
 
 // scan.hip

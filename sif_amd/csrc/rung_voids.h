@@ -79,6 +79,18 @@ struct CellRegion {
     }
     merge(l3, h3);
   }
+  // The region's wrapped cells on axis a as the bits of a word: holds()
+  // tests cell w of that axis as bit w.
+  SIF_HD uint32_t axis_bits(int a) const {
+    if (empty())
+      return 0u;
+    const uint32_t span = (uint32_t)(hi[a] - lo[a]);
+    if (span >= (uint32_t)(VOID_CELLS - 1))
+      return 0xffffffffu;
+    const uint32_t run = (2u << span) - 1u;  // span + 1 cells from lo
+    const uint32_t s = void_wrap(lo[a]);
+    return s ? (run << s) | (run >> (32u - s)) : run;
+  }
   // Does the region hold wrapped cell (wa, wb, wc)? The same cells as a scan
   // of the region visits.
   SIF_HD bool holds(uint32_t wa, uint32_t wb, uint32_t wc) const {

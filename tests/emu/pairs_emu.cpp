@@ -67,7 +67,8 @@ static sif_amd::BatchPairParams params(uint32_t n, float l, float box, float f, 
 
 template <bool WRITE>
 static void run_pass(const uint32_t* flat, const float* rad, const uint32_t* gs,
-  uint32_t count, const sif_amd::BatchPairParams& P, const float* bmax, uint32_t* cnt32,
+  uint32_t count, const sif_amd::BatchPairParams& P, const float* bmax, const uint32_t* cells,
+  uint32_t* cnt32,
   uint8_t* cnt8, const uint64_t* offset, PairEffect* eff, uint64_t capacity) {
   blockDim.x = 64 * sif_amd::BP_WAVES;
   const uint32_t blocks = (count + sif_amd::BP_WAVES - 1) / sif_amd::BP_WAVES;
@@ -78,8 +79,8 @@ static void run_pass(const uint32_t* flat, const float* rad, const uint32_t* gs,
       th.emplace_back([&, t, b] {
         threadIdx.x = t;
         blockIdx.x = b;
-        sif_amd::batch_pairs_kernel<WRITE>(flat, rad, gs, count, P, bmax, cnt32, cnt8, offset,
-          eff, capacity);
+        sif_amd::batch_pairs_kernel<WRITE>(flat, rad, gs, count, P, bmax, cells, cnt32, cnt8,
+          offset, eff, capacity);
       });
     for (auto& x : th)
       x.join();
@@ -94,15 +95,18 @@ extern "C" int emu_batch_pairs(const uint32_t* flat, const float* rad, const uin
   float cat_max_r, int32_t proxy, uint64_t capacity, int32_t* n_out, uint32_t* out,
   int cap) {
   const sif_amd::BatchPairParams P = params(n, l, box, f, scale, radius, cat_max_r, proxy);
-  float bmax = -1.0f;  // batch_max_radius_kernel's answer
-  for (uint32_t i = 0; i < count; i++)
+  float bmax = -1.0f;  // batch_prep_kernel's answers
+  std::vector<uint32_t> cells(count);
+  for (uint32_t i = 0; i < count; i++) {
     bmax = fmaxf(bmax, rad[i]);
+    cells[i] = sif_amd::batch_cell_code(flat[i], rad[i], P);
+  }
   std::vector<uint32_t> cnt32(count, 0xdeadbeefu);
   std::vector<uint8_t> cnt8(count, 0xaa);
   std::vector<uint64_t> offset(count);
   std::vector<PairEffect> eff(capacity + 1);
-  run_pass<false>(flat, rad, gs, count, P, &bmax, cnt32.data(), cnt8.data(), nullptr, nullptr,
-    0);
+  run_pass<false>(flat, rad, gs, count, P, &bmax, cells.data(), cnt32.data(), cnt8.data(),
+    nullptr, nullptr, 0);
   uint64_t at = 0;
   for (uint32_t b = 0; b < count; b++) {
     if (cnt32[b] != cnt8[b])
@@ -110,8 +114,8 @@ extern "C" int emu_batch_pairs(const uint32_t* flat, const float* rad, const uin
     offset[b] = at;
     at += cnt32[b];
   }
-  run_pass<true>(flat, rad, gs, count, P, &bmax, nullptr, cnt8.data(), offset.data(),
-    eff.data(), capacity);
+  run_pass<true>(flat, rad, gs, count, P, &bmax, cells.data(), nullptr, cnt8.data(),
+    offset.data(), eff.data(), capacity);
   for (uint32_t b = 0; b < count; b++) {
     const uint32_t c = cnt8[b];
     if (c >= sif_amd::BATCH_EFF_FULL || offset[b] + c > capacity || (int)c > cap) {

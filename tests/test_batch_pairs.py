@@ -69,17 +69,19 @@ def run(lib, flat, rad, gs, radius, f, scale, cat_max_r, cap=128, capacity=None)
     return dn, dout.reshape(count, cap), hn, hout.reshape(count, cap)
 
 
-@pytest.mark.parametrize("seed,f,lagrangian,radius,cat_max_r,near_face", [
-    (1, 0.0, False, 5.0, 14.0, False),
-    (2, 0.5, False, 4.0, 20.0, True),
-    (3, 0.0, True, 6.0, 0.0, False),
-    (4, 1.0, False, 3.5, 9.0, True),
-    (6, 0.0, False, 10.0, 14.0, False),
-    (7, 0.5, True, 12.0, 25.0, True),
+@pytest.mark.parametrize("seed,f,lagrangian,radius,cat_max_r,near_face,count", [
+    (1, 0.0, False, 5.0, 14.0, False, 96),
+    (2, 0.5, False, 4.0, 20.0, True, 96),
+    (3, 0.0, True, 6.0, 0.0, False, 96),
+    (4, 1.0, False, 3.5, 9.0, True, 96),
+    (6, 0.0, False, 10.0, 14.0, False, 96),
+    (7, 0.5, True, 12.0, 25.0, True, 96),
+    # past one round of the scan's four groups of 64 earlier candidates
+    (8, 0.0, False, 5.0, 14.0, True, 300),
 ])
 def test_device_lists_the_hosts_effects(lib, seed, f, lagrangian, radius, cat_max_r,
-                                        near_face):
-    flat, rad, gs = batch(seed, 96, near_face)
+                                        near_face, count):
+    flat, rad, gs = batch(seed, count, near_face)
     scale = float(np.cbrt(np.float64(1.0 - 0.8))) if lagrangian else 1.0
     dn, dout, hn, hout = run(lib, flat, rad, gs, radius, f, scale, cat_max_r)
     kinds = np.zeros(6, np.int64)
