@@ -10,6 +10,13 @@ namespace sif_amd {
 
 constexpr int BP_WAVES = 4;
 
+// The lanes of a wave see each other's LDS stores made before it.
+__device__ __forceinline__ void pairs_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // COUNT: each candidate's number of effects (at most BATCH_EFF_FULL, which
 // stands for "that many or more"). WRITE: its effects, in the order of the
 // earlier candidates, at its offset (the exclusive sum of the counts) in one
@@ -73,11 +80,49 @@ __global__ void __launch_bounds__(64 * BP_WAVES) batch_pairs_kernel(
     poles_r.set(ix, iy, iz, pole_r, n);
   // The region as one bit word per axis: most earlier candidates lie outside
   // it, and a lane tells so from the candidate's cell code alone, four
-  // groups of 64 loaded at once; a group with no lane inside is passed over.
+  // groups of 64 loaded at once. Those inside queue up in the wave's LDS, in
+  // batch order, and are tested 64 at a time, so the pair tests run on full
+  // waves.
+  __shared__ uint32_t queue_s[BP_WAVES][128];
+  uint32_t* queue = queue_s[threadIdx.x >> 6];
   const uint32_t bx = R.axis_bits(0), by = R.axis_bits(1), bz = R.axis_bits(2);
   constexpr uint32_t U = 4;
   uint32_t found = 0;  // wave-uniform
+  uint32_t queued = 0;  // wave-uniform
   const uint64_t below = (1ull << lane) - 1ull;
+  // the pair tests of queue[0 .. take), in queue order
+  auto test_queued = [&](uint32_t take) {
+    const bool act = lane < take;
+    PairEffect e{};
+    bool any = false;
+    if (act) {
+      const uint32_t a = queue[lane];
+      const uint32_t c = cells[a];
+      const float ra = rad[a];
+      const uint32_t fa = flat[a];
+      const uint32_t jx = fa / nn, jy = (fa / n) % n, jz = fa % n;
+      const float ax = (float)jx * P.l, ay = (float)jy * P.l, az = (float)jz * P.l;
+      const uint32_t wa = c >> 10, wb = (c >> 5) & 31u, wc = c & 31u;
+      e.a = a;
+      if (need_state)
+        e.state = (uint8_t)pair_state(q, wa, wb, wc, ax, ay, az, ra);
+      if (need_state_r)
+        e.state_r = (uint8_t)pair_state(qr, wa, wb, wc, ax, ay, az, ra);
+      PaintSphere S;
+      paint_sphere(S, ax, ay, az, P.scale * ra, P.l, n);
+      e.node = want_node && paint_hits(S, ix, iy, iz);
+      e.pole = want_pole && poles.painted(S, ix, iy, iz);
+      e.pole_r = want_pole_r && poles_r.painted(S, ix, iy, iz);
+      any = e.node | e.pole | e.pole_r | e.state | e.state_r;
+    }
+    const uint64_t m = __ballot(any);
+    if constexpr (WRITE) {
+      const uint32_t at = found + (uint32_t)__popcll(m & below);
+      if (any)
+        eff[at0 + at] = e;
+    }
+    found += (uint32_t)__popcll(m);
+  };
   for (uint32_t a0 = 0; a0 < b && found < BATCH_EFF_FULL && !R.empty(); a0 += 64 * U) {
     uint32_t code[U];
 #pragma unroll
@@ -90,38 +135,28 @@ __global__ void __launch_bounds__(64 * BP_WAVES) batch_pairs_kernel(
       const uint32_t c = code[u];
       const bool near = c != BATCH_NO_CELL &&
                         ((bx >> (c >> 10)) & (by >> ((c >> 5) & 31u)) & (bz >> (c & 31u)) & 1u);
-      if (__ballot(near) == 0ull)
+      const uint64_t m = __ballot(near);
+      if (m == 0ull)
         continue;
-      const uint32_t a = a0 + u * 64 + lane;
-      PairEffect e{};
-      bool any = false;
-      if (near) {
-        const float ra = rad[a];
-        const uint32_t fa = flat[a];
-        const uint32_t jx = fa / nn, jy = (fa / n) % n, jz = fa % n;
-        const float ax = (float)jx * P.l, ay = (float)jy * P.l, az = (float)jz * P.l;
-        const uint32_t wa = c >> 10, wb = (c >> 5) & 31u, wc = c & 31u;
-        e.a = a;
-        if (need_state)
-          e.state = (uint8_t)pair_state(q, wa, wb, wc, ax, ay, az, ra);
-        if (need_state_r)
-          e.state_r = (uint8_t)pair_state(qr, wa, wb, wc, ax, ay, az, ra);
-        PaintSphere S;
-        paint_sphere(S, ax, ay, az, P.scale * ra, P.l, n);
-        e.node = want_node && paint_hits(S, ix, iy, iz);
-        e.pole = want_pole && poles.painted(S, ix, iy, iz);
-        e.pole_r = want_pole_r && poles_r.painted(S, ix, iy, iz);
-        any = e.node | e.pole | e.pole_r | e.state | e.state_r;
+      if (near)
+        queue[queued + (uint32_t)__popcll(m & below)] = a0 + u * 64 + lane;
+      queued += (uint32_t)__popcll(m);
+      if (queued >= 64) {
+        pairs_wave_sync();
+        test_queued(64);
+        // the entries past the first 64 move to the front
+        const uint32_t rest = queued - 64;
+        const uint32_t moved = lane < rest ? queue[64 + lane] : 0u;
+        pairs_wave_sync();
+        if (lane < rest)
+          queue[lane] = moved;
+        queued = rest;
       }
-      const uint64_t m = __ballot(any);
-      if constexpr (WRITE) {
-        const uint32_t at = found + (uint32_t)__popcll(m & below);
-        if (any)
-          eff[at0 + at] = e;
-      }
-      found += (uint32_t)__popcll(m);
     }
   }
+  pairs_wave_sync();
+  if (queued > 0 && found < BATCH_EFF_FULL)
+    test_queued(queued);
   if constexpr (!WRITE) {
     if (lane == 0) {
       const uint32_t c = found < BATCH_EFF_FULL ? found : BATCH_EFF_FULL;

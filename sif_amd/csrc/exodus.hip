@@ -415,8 +415,11 @@ class LutBuilder {
       d.store(0, std::memory_order_relaxed);
     next_.store(0, std::memory_order_relaxed);
     stop_.store(false, std::memory_order_relaxed);
+    // a few threads: the tables are built far ahead of the rungs that use
+    // them, and more would take the cores the launching thread needs
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned workers = (unsigned)std::min<uint64_t>(total_, std::min(hw, 16u));
+    const unsigned workers =
+      (unsigned)std::min<uint64_t>(total_, std::min(std::max(1u, hw / 2), 6u));
     for (unsigned w = 0; w < workers; w++)
       pool_.emplace_back([this]() {
         while (!stop_.load(std::memory_order_relaxed) && claim()) {
