@@ -103,10 +103,12 @@ SIF_API int sif_amd_chain_mesh_build(const float* d_x, const float* d_y,
 #define SIF_AMD_FILTER_NONE     0
 #define SIF_AMD_FILTER_TOP_HAT  1
 #define SIF_AMD_FILTER_GAUSSIAN 2
-/* A spectrum is [n][n][pitch] complex64: row (x, y) holds the n/2 + 1 modes
- * kz = 0 .. n/2 of FFTW's r2c layout first, then padding up to the pitch
- * (n/2 + 1 rounded up to a multiple of 16, so rows start on 128-byte lines).
- * sif_amd_spectrum_bytes() is its size, also that of the FFT work buffer. */
+/* A spectrum is n x-planes of complex64: plane x holds n rows of the pitch,
+ * row (x, y) the n/2 + 1 modes kz = 0 .. n/2 of FFTW's r2c layout first, then
+ * padding up to the pitch (n/2 + 1 rounded up to a multiple of 16, so rows
+ * start on 128-byte lines); each plane ends with 48 more complex values of
+ * padding. sif_amd_spectrum_bytes() is its size (n planes), also that of the
+ * FFT work buffer. */
 SIF_API uint32_t sif_amd_spectrum_pitch(uint32_t n);
 SIF_API uint64_t sif_amd_spectrum_bytes(uint32_t n);
 /* sif__fft_grid_forward (src/math/fft.c:790-818). */

@@ -747,7 +747,10 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       SIF_HIP(hipStreamWaitEvent(stream(), ev_smoothed[j & 1], 0));
     }
     if (cand_cap == 0) {
-      cand_cap = total / 64 + 1024;
+      // a 64th of the cells, or the room an earlier run left
+      const uint64_t left = std::min({g_dev.keys.bytes / 4, g_dev.flat.bytes / 4,
+        g_dev.keep.bytes / 4, g_dev.surv.bytes / 4, g_dev.skeys.bytes / 4, g_dev.why.bytes});
+      cand_cap = std::max<uint64_t>(total / 64 + 1024, left);
       const int e = reserve_candidates();
       if (e != SIF_OK)
         return e;
@@ -772,7 +775,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     const size_t j = rs.rung;
     int e = scan_finish(rs.job, g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap);
     while (e == SIF_OK && rs.job.count > cand_cap) {
-      cand_cap = (uint64_t)((double)rs.job.count * 1.2) + 1;
+      // the counts stand (the mask is painted only outside a scan's two
+      // halves), so only the writing pass is issued again
+      cand_cap = (uint64_t)((double)rs.job.count * 1.5) + 1;
       g_dev.keys.release();
       g_dev.flat.release();
       g_dev.keep.release();
@@ -780,8 +785,6 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
       g_dev.skeys.release();
       g_dev.why.release();
       e = reserve_candidates();
-      if (e == SIF_OK)
-        e = scan_begin(rs.job);
       if (e == SIF_OK)
         e = scan_finish(rs.job, g_dev.keys.as<uint32_t>(), g_dev.flat.as<uint32_t>(), cand_cap);
     }

@@ -258,19 +258,36 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
   }
   if (lut) {
     float* F = reinterpret_cast<float*>(L);
-    const double ay = axis ? axis[a] : 1.0;
+    if (axis) {
+      // the window's factors of the tile's |kx| (times its y factor) and kz
+      // columns, staged in the LDS past the table: two coalesced reads a
+      // thread instead of two gathers a mode
+      constexpr int FB = (TK * (half + 1) + 1) / 2;  // float2 slots the table takes
+      double* cxy = reinterpret_cast<double*>(L + FB);
+      double* az = cxy + (half + 1);
+      static_assert(FB * 8 + (half + 1 + TK) * 8 <= TK * LY::pitch * 8, "factors fit the tile");
+      const double ay = axis[a];
+      for (int kx = threadIdx.x; kx <= half; kx += FFT_THREADS)
+        cxy[kx] = axis[kx] * ay;
+      if (threadIdx.x < TK) {
+        const int kz = kz0 + (int)threadIdx.x < nz ? kz0 + (int)threadIdx.x : nz - 1;
+        az[threadIdx.x] = axis[kz];
+      }
+      __syncthreads();
 #pragma unroll
-    for (int t = 0; t < FT; t++) {
-      const int f = threadIdx.x + t * FFT_THREADS;
-      if (f < TK * (half + 1)) {
-        float w = fs[t];
-        if (axis) {
+      for (int t = 0; t < FT; t++) {
+        const int f = threadIdx.x + t * FFT_THREADS;
+        if (f < TK * (half + 1)) {
           const int c = f / (half + 1), kx = f % (half + 1);
-          const int kz = kz0 + c < nz ? kz0 + c : nz - 1;
-          const double cxy = axis[kx] * ay;
-          w = (float)((double)w * cxy * axis[kz]);
+          F[f] = (float)((double)fs[t] * cxy[kx] * az[c]);
         }
-        F[f] = w;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < FT; t++) {
+        const int f = threadIdx.x + t * FFT_THREADS;
+        if (f < TK * (half + 1))
+          F[f] = fs[t];
       }
     }
     __syncthreads();
@@ -459,26 +476,39 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
     }
   } else {
     // Thread element e holds cells 2e and 2e + 1 of the block, so the
-    // block's cell order is (q, wave, lane, pair half). Each wave counts its
-    // cells at or below t for every q by ballot, one wave turns the
-    // (q, wave) counts into offsets, and each cell's place is its run's
-    // offset plus the cells before it in the run. The values are read from
-    // the tile in LDS in both passes (no register array, no ballot held
-    // across the loop).
-    static_assert(PER * (FFT_THREADS / 64) <= 128, "one offset pair a lane");
+    // block's cell order is (q, wave, lane, pair half). The values come out
+    // of LDS into registers once. Each wave counts its cells at or below t
+    // for every q by ballot (lane q keeps the count of step q), one wave
+    // turns the (q, wave) counts into offsets, and each cell's place is its
+    // run's offset plus the cells before it in the run. Every block is whole
+    // when N^2 rows are a multiple of its rows (N >= 32), and then no cell
+    // is tested for being in the grid.
+    static_assert(PER <= 64 && PER * (FFT_THREADS / 64) <= 128, "one offset pair a lane");
     constexpr int WAVES = FFT_THREADS / 64;
+    constexpr bool WHOLE = ((uint64_t)N * N) % TK2 == 0;
     __shared__ uint32_t run_off[PER * WAVES + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll 2
+    float2 z[PER];
+#pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
-      const bool in = row0 + e / M < rows;
-      const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
-      const uint64_t m0 = __ballot(in && z.x * norm <= t), m1 = __ballot(in && z.y * norm <= t);
-      if (lane == 0)
-        run_off[q * WAVES + wave] = (uint32_t)(__popcll(m0) + __popcll(m1));
+      const float2 w = L[(e / M) * LY::pitch + LY::at(e % M)];
+      z[q] = make_float2(w.x * norm, w.y * norm);
     }
+    auto inside = [&](int q) -> bool {
+      return WHOLE || row0 + (threadIdx.x + q * FFT_THREADS) / M < rows;
+    };
+    uint32_t my_count = 0;  // lane q: the wave's cells of step q
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const bool in = inside(q);
+      const uint64_t m0 = __ballot(in && z[q].x <= t), m1 = __ballot(in && z[q].y <= t);
+      const uint32_t c = (uint32_t)(__popcll(m0) + __popcll(m1));
+      my_count = lane == q ? c : my_count;
+    }
+    if (lane < PER)
+      run_off[lane * WAVES + wave] = my_count;
     __syncthreads();
     if (wave == 0) {  // exclusive scan of the PER * WAVES run counts
       const int i0 = 2 * lane, i1 = 2 * lane + 1;
@@ -499,24 +529,23 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
         run_off[PER * WAVES] = x;
     }
     __syncthreads();
+    const uint32_t my_off = lane < PER ? run_off[lane * WAVES + wave] : 0u;
     const uint64_t base = (uint64_t)blockIdx.x * LIST_BLOCK;
-#pragma unroll 2
+#pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = threadIdx.x + q * FFT_THREADS;
-      const bool in = row0 + e / M < rows;
-      const float2 z = L[(e / M) * LY::pitch + LY::at(e % M)];
-      const float vx = z.x * norm, vy = z.y * norm;
-      const bool b0 = in && vx <= t, b1 = in && vy <= t;
+      const bool in = inside(q);
+      const bool b0 = in && z[q].x <= t, b1 = in && z[q].y <= t;
       const uint64_t m0 = __ballot(b0), m1 = __ballot(b1);
-      uint32_t pos = run_off[q * WAVES + wave] +
+      uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)my_off, q) +
                      (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below));
       if (b0) {
-        lkeys[base + pos] = float_key(vx);
+        lkeys[base + pos] = float_key(z[q].x);
         llocal[base + pos] = (uint16_t)(2 * e);
         pos++;
       }
       if (b1) {
-        lkeys[base + pos] = float_key(vy);
+        lkeys[base + pos] = float_key(z[q].y);
         llocal[base + pos] = (uint16_t)(2 * e + 1);
       }
     }
@@ -1123,8 +1152,11 @@ uint64_t list_blocks(uint32_t n) {
   return ((uint64_t)n * n * n + LIST_BLOCK - 1) / LIST_BLOCK;
 }
 
+// n rows of the pitch and three 128-byte lines more: successive planes then
+// start on different memory channels, and the x pass, whose pencils stride
+// by a plane, streams faster (512^3: that access pattern alone 281 -> 250 us).
 uint64_t spectrum_plane(uint32_t n) {
-  return (uint64_t)n * spectrum_pitch(n);
+  return (uint64_t)n * spectrum_pitch(n) + 48;
 }
 
 uint64_t filter_lut_size(uint32_t n) {

@@ -354,17 +354,21 @@ int scan_begin(ScanJob& J) {
 
 // Waits for the counting pass's results and, when the count fits cap,
 // enqueues the writing pass: the candidates in flat order with their keys.
+// When it does not, the job stays begun: the caller grows its buffers and
+// finishes again, with the same counts (so the mask must not change between).
 int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap) {
   if (!J.begun)
     return SIF_OK;
-  J.begun = false;
   SIF_HIP(hipEventSynchronize(g_back_ready));
   const uint64_t n = g_back[0];
   J.count = n;
   J.key_differ = (uint32_t)g_back[1] ^ (uint32_t)(g_back[1] >> 32);
   J.listed = J.lists ? g_back[2] : 0;
+  if (n > cap)
+    return SIF_OK;
+  J.begun = false;
   const uint64_t tiles = J.lists ? J.lists->blocks : (J.total + SC_TILE - 1) / SC_TILE;
-  if (n <= cap && n > 0) {
+  if (n > 0) {
     timing_begin(SIF_AMD_STAGE_SCAN);
     if (J.lists)
       list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
@@ -376,7 +380,7 @@ int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap) {
     timing_end(SIF_AMD_STAGE_SCAN);
   }
   if (timing_active()) {
-    const double written = n <= cap ? 8.0 * (double)n : 0.0;
+    const double written = 8.0 * (double)n;
     if (J.lists) {
       // the lists twice and the candidates once; the smoothing that made
       // the lists wrote them once
