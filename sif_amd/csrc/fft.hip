@@ -35,7 +35,11 @@
 namespace sif_amd {
 
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_TILE = 8192;  // complex values per workgroup tile
+constexpr int FFT_TILE = 8192;  // complex values per workgroup tile of the x and y passes
+// The z pass's tile: fewer rows a workgroup, so that more workgroups share a
+// CU and hide each other's load-transform-store chain (the x and y passes
+// keep 16 columns of 128-byte lines; a row is contiguous at any count).
+constexpr int ROWS_TILE = 2048;
 
 // LDS layout of a tile: pencil p's element i at p * pitch + i + (i >> SH).
 // The Stockham writes of the early stages stride the pencil by R or R NS
@@ -44,8 +48,8 @@ constexpr int FFT_TILE = 8192;  // complex values per workgroup tile
 // 1 mod 16 slots at the lengths used, so the column phases (16 pencils side
 // by side) stay conflict-free. SH = 3 where the tile and its twiddles then
 // still fit two workgroups a CU (80 KB each), else 4.
-constexpr int pad_shift(int len, int tw_entries) {
-  return (FFT_TILE / len) * (len + len / 8 + 1) * 8 + tw_entries * 8 <= 80 * 1024 ? 3 : 4;
+constexpr int pad_shift(int len, int tw_entries, int tile = FFT_TILE) {
+  return (tile / len) * (len + len / 8 + 1) * 8 + tw_entries * 8 <= 80 * 1024 ? 3 : 4;
 }
 template <int N, int SH>
 struct Lay {
@@ -167,13 +171,14 @@ struct Plan {
   static constexpr int R0 = (L % 3 == 0) ? 8 : (1 << (L % 3));
 };
 
-// Every pencil of the tile (FFT_TILE / N pencils, layout Lay<N, SH>) through
+// Every pencil of the tile (TILE / N pencils, layout Lay<N, SH>) through
 // all stages. The 256 threads serve the pencils in groups; the tile is
 // transformed in place, a workgroup barrier between the reads and the writes
 // of a stage.
-template <int N, int SH, int R, int NS, bool INV>
+template <int N, int SH, int R, int NS, bool INV, int TILE>
 __device__ __forceinline__ void tile_stage(float2* L, const float2* tw) {
-  constexpr int T = N / 8, PG = FFT_THREADS / T, PP = FFT_TILE / N / PG;
+  constexpr int T = N / 8, PG = FFT_THREADS / T, PP = TILE / N / PG;
+  static_assert(PP >= 1, "a tile holds at least one pencil a thread group");
   const int t = threadIdx.x % T, g = threadIdx.x / T;
   float2 v[PP][8];
 #pragma unroll
@@ -186,19 +191,19 @@ __device__ __forceinline__ void tile_stage(float2* L, const float2* tw) {
   __syncthreads();
 }
 
-template <int N, int SH, bool INV, int NS>
+template <int N, int SH, bool INV, int NS, int TILE>
 __device__ __forceinline__ void tile_rest(float2* L, const float2* tw) {
   if constexpr (NS < N) {
-    tile_stage<N, SH, 8, NS, INV>(L, tw);
-    tile_rest<N, SH, INV, NS * 8>(L, tw);
+    tile_stage<N, SH, 8, NS, INV, TILE>(L, tw);
+    tile_rest<N, SH, INV, NS * 8, TILE>(L, tw);
   }
 }
 
-template <int N, int SH, bool INV>
+template <int N, int SH, bool INV, int TILE = FFT_TILE>
 __device__ __forceinline__ void tile_fft(float2* L, const float2* tw) {
   constexpr int R0 = Plan<N>::R0;
-  tile_stage<N, SH, R0, 1, INV>(L, tw);
-  tile_rest<N, SH, INV, R0>(L, tw);
+  tile_stage<N, SH, R0, 1, INV, TILE>(L, tw);
+  tile_rest<N, SH, INV, R0, TILE>(L, tw);
 }
 
 template <int N>
@@ -334,7 +339,7 @@ __global__ void __launch_bounds__(FFT_THREADS) cols_kernel(const float2* __restr
 //             A[k] = X[k] + conj X[M-k],  B[k] = (X[k] - conj X[M-k]) W^-k,
 //             and the length-M inverse of Z is x[2m] + i x[2m+1]
 //
-// (X[k + M] = conj X[M - k] for a real row). Rows are TK2 = FFT_TILE / M to
+// (X[k + M] = conj X[M - k] for a real row). Rows are TK2 = ROWS_TILE / M to
 // a workgroup, laid out as Lay<M, SH> in LDS (element M of a row, X[M], fits
 // in its padding); the twiddles of the length-M transform (W^2m) and the
 // split's (W^k, k <= M) sit side by side in LDS.
@@ -352,7 +357,8 @@ template <int N>
 __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __restrict__ in,
   float2* __restrict__ spec, uint64_t rows, uint32_t pitch, uint64_t plane,
   const float2* __restrict__ tw_g) {
-  constexpr int M = N / 2, TK2 = FFT_TILE / M, nz = M + 1, SH = pad_shift(M, 2 * M + 1);
+  constexpr int M = N / 2, TK2 = ROWS_TILE / M, nz = M + 1;
+  constexpr int SH = pad_shift(M, 2 * M + 1, ROWS_TILE);
   using LY = Lay<M, SH>;
   __shared__ float2 L[TK2 * LY::pitch];
   __shared__ float2 tw[2 * M + 1];
@@ -376,7 +382,7 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_rows_kernel(const float* __re
     }
   }
   __syncthreads();
-  tile_fft<M, SH, false>(L, tw);
+  tile_fft<M, SH, false, ROWS_TILE>(L, tw);
   float2 v[PER_H];
 #pragma unroll
   for (int q = 0; q < PER_H; q++) {
@@ -411,7 +417,8 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
   float* __restrict__ out, uint64_t rows, uint32_t pitch, uint64_t plane, float norm,
   const float2* __restrict__ tw_g, float t, uint32_t* __restrict__ lkeys,
   uint16_t* __restrict__ llocal, uint32_t* __restrict__ lcount) {
-  constexpr int M = N / 2, TK2 = FFT_TILE / M, nz = M + 1, SH = pad_shift(M, 2 * M + 1);
+  constexpr int M = N / 2, TK2 = ROWS_TILE / M, nz = M + 1;
+  constexpr int SH = pad_shift(M, 2 * M + 1, ROWS_TILE);
   using LY = Lay<M, SH>;
   __shared__ float2 L[TK2 * LY::pitch];
   __shared__ float2 tw[2 * M + 1];
@@ -457,7 +464,7 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_rows_kernel(const float2* __r
     row[LY::at(k)] = zk;
   }
   __syncthreads();
-  tile_fft<M, SH, true>(L, tw);
+  tile_fft<M, SH, true, ROWS_TILE>(L, tw);
   if constexpr (!LIST) {
     float2 r[PER];
 #pragma unroll
@@ -846,7 +853,7 @@ int pass_cols(bool inv, const float2* in, float2* out, uint64_t stride_a, uint64
 
 template <int N>
 int pass_r2c_rows(const float* in, float2* spec) {
-  constexpr int TK2 = FFT_TILE / (N / 2);
+  constexpr int TK2 = ROWS_TILE / (N / 2);
   const uint64_t rows = (uint64_t)N * N;
   r2c_rows_kernel<N><<<(unsigned)((rows + TK2 - 1) / TK2), FFT_THREADS, 0, stream()>>>(in,
     spec, rows, spectrum_pitch(N), spectrum_plane(N), g_tw.fwd.as<float2>());
@@ -857,7 +864,7 @@ int pass_r2c_rows(const float* in, float2* spec) {
 template <int N>
 int pass_c2r_rows(const float2* spec, float* out, float norm, hipStream_t s,
   const CellLists* lists, float t) {
-  constexpr int TK2 = FFT_TILE / (N / 2);
+  constexpr int TK2 = ROWS_TILE / (N / 2);
   static_assert((uint64_t)TK2 * N == LIST_BLOCK, "a rows block is one list block");
   const uint64_t rows = (uint64_t)N * N;
   const unsigned blocks = (unsigned)((rows + TK2 - 1) / TK2);

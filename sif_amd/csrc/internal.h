@@ -51,7 +51,7 @@ int fft_c2r_lut(const float2* spec, uint32_t n, const float* d_lut, const double
 // grid; its list holds count[b] entries, the cell's offset in the block
 // (local) and the order-preserving key of its value (float_key), at
 // [b LIST_BLOCK, b LIST_BLOCK + count[b]) of keys and local.
-constexpr uint32_t LIST_BLOCK = 16384;
+constexpr uint32_t LIST_BLOCK = 4096;
 struct CellLists {
   uint32_t* keys = nullptr;
   uint16_t* local = nullptr;

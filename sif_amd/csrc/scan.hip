@@ -206,26 +206,38 @@ thread_local DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span;
 
 // --- the same scan from cell lists (CellLists, a listing smoothing's) ---
 //
-// One workgroup per list block: the listed cells whose mask bit is clear,
-// counted (with their keys' AND and OR), then written in list order -- cell
-// order -- at the block's offset among all blocks. Every list is read twice,
-// nothing of the grid's size.
+// One workgroup per LISTS_PER_TILE list blocks (a tile), which it takes in
+// order: the listed cells whose mask bit is clear, counted (with their keys'
+// AND and OR), then written in list order -- cell order -- at the tile's
+// offset among all tiles. Every list is read twice, nothing of the grid's
+// size.
+constexpr uint32_t LISTS_PER_TILE = 4;
+
+uint64_t list_tiles(uint64_t blocks) {
+  return (blocks + LISTS_PER_TILE - 1) / LISTS_PER_TILE;
+}
 __global__ void __launch_bounds__(SC_THREADS) list_count_kernel(
   const uint32_t* __restrict__ lkeys, const uint16_t* __restrict__ llocal,
-  const uint32_t* __restrict__ lcount, const uint64_t* __restrict__ mask,
+  const uint32_t* __restrict__ lcount, uint64_t blocks, const uint64_t* __restrict__ mask,
   uint64_t* __restrict__ tile_counts, uint32_t* __restrict__ tile_span,
   unsigned long long* __restrict__ listed) {
   __shared__ uint32_t w_and[SC_THREADS / 64], w_or[SC_THREADS / 64], w_cnt[SC_THREADS / 64];
-  const uint64_t base = (uint64_t)blockIdx.x * LIST_BLOCK;
-  const uint32_t c = lcount[blockIdx.x];
-  uint32_t cnt = 0, k_and = 0xffffffffu, k_or = 0u;
-  for (uint32_t i = threadIdx.x; i < c; i += SC_THREADS) {
-    const uint64_t cell = base + llocal[base + i];
-    if (!((mask[cell >> 6] >> (cell & 63)) & 1ull)) {
-      const uint32_t k = lkeys[base + i];
-      cnt++;
-      k_and &= k;
-      k_or |= k;
+  uint32_t cnt = 0, k_and = 0xffffffffu, k_or = 0u, c_all = 0;
+  for (uint32_t j = 0; j < LISTS_PER_TILE; j++) {
+    const uint64_t b = (uint64_t)blockIdx.x * LISTS_PER_TILE + j;
+    if (b >= blocks)
+      break;
+    const uint64_t base = b * LIST_BLOCK;
+    const uint32_t c = lcount[b];
+    c_all += c;
+    for (uint32_t i = threadIdx.x; i < c; i += SC_THREADS) {
+      const uint64_t cell = base + llocal[base + i];
+      if (!((mask[cell >> 6] >> (cell & 63)) & 1ull)) {
+        const uint32_t k = lkeys[base + i];
+        cnt++;
+        k_and &= k;
+        k_or |= k;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -249,36 +261,41 @@ __global__ void __launch_bounds__(SC_THREADS) list_count_kernel(
     tile_counts[blockIdx.x] = all;
     tile_span[2 * (uint64_t)blockIdx.x] = a;
     tile_span[2 * (uint64_t)blockIdx.x + 1] = o;
-    if (c)
-      atomicAdd(listed, (unsigned long long)c);
+    if (c_all)
+      atomicAdd(listed, (unsigned long long)c_all);
   }
 }
 
 __global__ void __launch_bounds__(SC_THREADS) list_write_kernel(
   const uint32_t* __restrict__ lkeys, const uint16_t* __restrict__ llocal,
-  const uint32_t* __restrict__ lcount, const uint64_t* __restrict__ mask,
+  const uint32_t* __restrict__ lcount, uint64_t blocks, const uint64_t* __restrict__ mask,
   const uint64_t* __restrict__ tile_offsets, uint32_t* __restrict__ keys,
   uint32_t* __restrict__ flat) {
-  const uint64_t base = (uint64_t)blockIdx.x * LIST_BLOCK;
-  const uint32_t c = lcount[blockIdx.x];
   uint64_t pos = tile_offsets[blockIdx.x];
-  // chunks of SC_THREADS entries in list order; a block scan places each
-  // kept one (every thread runs every chunk: the scan's barriers)
-  for (uint32_t i0 = 0; i0 < c; i0 += SC_THREADS) {
-    const uint32_t i = i0 + threadIdx.x;
-    uint64_t cell = 0;
-    bool keep = false;
-    if (i < c) {
-      cell = base + llocal[base + i];
-      keep = !((mask[cell >> 6] >> (cell & 63)) & 1ull);
+  for (uint32_t j = 0; j < LISTS_PER_TILE; j++) {
+    const uint64_t b = (uint64_t)blockIdx.x * LISTS_PER_TILE + j;
+    if (b >= blocks)
+      break;
+    const uint64_t base = b * LIST_BLOCK;
+    const uint32_t c = lcount[b];
+    // chunks of SC_THREADS entries in list order; a block scan places each
+    // kept one (every thread runs every chunk: the scan's barriers)
+    for (uint32_t i0 = 0; i0 < c; i0 += SC_THREADS) {
+      const uint32_t i = i0 + threadIdx.x;
+      uint64_t cell = 0;
+      bool keep = false;
+      if (i < c) {
+        cell = base + llocal[base + i];
+        keep = !((mask[cell >> 6] >> (cell & 63)) & 1ull);
+      }
+      uint32_t all;
+      const uint32_t at = sc_block_scan(keep ? 1u : 0u, &all);
+      if (keep) {
+        keys[pos + at] = lkeys[base + i];
+        flat[pos + at] = (uint32_t)cell;
+      }
+      pos += all;
     }
-    uint32_t all;
-    const uint32_t at = sc_block_scan(keep ? 1u : 0u, &all);
-    if (keep) {
-      keys[pos + at] = lkeys[base + i];
-      flat[pos + at] = (uint32_t)cell;
-    }
-    pos += all;
   }
 }
 
@@ -310,7 +327,7 @@ int scan_begin(ScanJob& J) {
       return SIF_ERR_INVALID;
     }
   }
-  const uint64_t tiles = J.lists ? J.lists->blocks : (J.total + SC_TILE - 1) / SC_TILE;
+  const uint64_t tiles = J.lists ? list_tiles(J.lists->blocks) : (J.total + SC_TILE - 1) / SC_TILE;
   int st = g_tiles.reserve((tiles ? tiles : 1) * 8);
   if (st == SIF_OK)
     st = g_tile_pos.reserve((tiles ? tiles : 1) * 8);
@@ -330,7 +347,7 @@ int scan_begin(ScanJob& J) {
   if (J.lists) {
     SIF_HIP(hipMemsetAsync(g_count.as<uint64_t>() + 2, 0, 8, stream()));
     list_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
-      J.lists->local, J.lists->count, J.mask, g_tiles.as<uint64_t>(),
+      J.lists->local, J.lists->count, J.lists->blocks, J.mask, g_tiles.as<uint64_t>(),
       g_tile_span.as<uint32_t>(), reinterpret_cast<unsigned long long*>(g_count.as<uint64_t>() + 2));
   } else {
     cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
@@ -367,12 +384,13 @@ int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap) {
   if (n > cap)
     return SIF_OK;
   J.begun = false;
-  const uint64_t tiles = J.lists ? J.lists->blocks : (J.total + SC_TILE - 1) / SC_TILE;
+  const uint64_t tiles = J.lists ? list_tiles(J.lists->blocks) : (J.total + SC_TILE - 1) / SC_TILE;
   if (n > 0) {
     timing_begin(SIF_AMD_STAGE_SCAN);
     if (J.lists)
       list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
-        J.lists->local, J.lists->count, J.mask, g_tile_pos.as<uint64_t>(), keys, flat);
+        J.lists->local, J.lists->count, J.lists->blocks, J.mask, g_tile_pos.as<uint64_t>(),
+        keys, flat);
     else
       cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
         J.t, J.minima_n, g_tile_pos.as<uint64_t>(), keys, flat);
