@@ -726,6 +726,7 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     uint64_t since[4] = {0, 0, 0, 0};
   };
   RungStart rs;
+  size_t smoothed_next = R.first_rung + 1;  // the first rung not yet issued to be smoothed
   auto reserve_candidates = [&]() {
     int e = g_dev.keys.reserve(cand_cap * 4);
     if (e == SIF_OK) e = g_dev.flat.reserve(cand_cap * 4);
@@ -763,6 +764,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     rs.job.total = total;
     rs.job.t = R.threshold;
     rs.job.minima_n = minima ? n : 0u;
+    rs.job.keys = g_dev.keys.as<uint32_t>();
+    rs.job.flat = g_dev.flat.as<uint32_t>();
+    rs.job.cap = cand_cap;
     const int e = scan_begin(rs.job);
     if (e == SIF_OK)
       rs.step = 1;
@@ -793,15 +797,20 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     rs.nc = rs.job.count;
     rs.key_differ = rs.job.key_differ;
     SIF_HIP(hipEventRecord(ev_scanned[j & 1], stream()));
-    if (j + 1 < R.radii.size()) {
-      // after this scan: the other buffer's last reader (rung j - 1's scan)
-      // and every earlier use of the work buffer are behind it
-      const size_t jn = j + 1;
-      SIF_HIP(hipStreamWaitEvent(aux_stream(), ev_scanned[j & 1], 0));
+    // The next rungs' fields on the auxiliary stream, two rungs ahead, after
+    // this scan: rung j + 2 into the buffer this scan has just read, and rung
+    // j + 1, if the previous scan did not issue it, into the other one, whose
+    // last reader (rung j - 1's scan) is behind this one. Every earlier use of
+    // the work buffer is behind it too. The auxiliary stream then smooths
+    // rung after rung while the rungs that find nothing go by.
+    SIF_HIP(hipStreamWaitEvent(aux_stream(), ev_scanned[j & 1], 0));
+    for (size_t jn = std::max(j + 1, smoothed_next); jn <= j + 2 && jn < R.radii.size();
+         jn++) {
       e = smooth(jn, fields[jn & 1], aux_stream());
       if (e != SIF_OK)
         return e;
       SIF_HIP(hipEventRecord(ev_smoothed[jn & 1], aux_stream()));
+      smoothed_next = jn + 1;
     }
     return SIF_OK;
   };

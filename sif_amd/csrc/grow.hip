@@ -116,6 +116,50 @@ __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
   return state;
 }
 
+// overlap_state() by a whole wavefront for one query: the lanes
+// split the rows of each class's superset cells and walk the occupied cells
+// of their rows (the occupancy words). The state is an OR over voids, so
+// the order they come in changes nothing; the wave stops at the first
+// state 1. Every lane returns the same state.
+__device__ int wave_overlap_state(const VoidIndex& V, const OverlapQuery& q, uint32_t lane) {
+  int st = 0;
+  for (int cls = VOID_CLASSES - 1; cls >= 0; cls--) {
+    if (V.class_max_r[cls] < 0.0f)
+      continue;
+    int32_t lo[3], hi[3];
+    superset_scan(q, V.class_max_r[cls], V.inv_l, lo, hi);
+    const uint32_t na = (uint32_t)(hi[0] - lo[0] + 1), nb = (uint32_t)(hi[1] - lo[1] + 1);
+    const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
+    const uint32_t* occ = V.occupied + (uint32_t)cls * VOID_CELLS * VOID_CELLS;
+    const uint32_t zmask = row_zmask(lo[2], hi[2]);
+    for (uint32_t base = 0; base < na * nb; base += 64) {
+      const uint32_t idx = base + lane;
+      if (idx < na * nb) {
+        const uint32_t wa = void_wrap(lo[0] + (int32_t)(idx / nb));
+        const uint32_t wb = void_wrap(lo[1] + (int32_t)(idx % nb));
+        const uint32_t row = wa * VOID_CELLS + wb;
+        for (uint32_t cells = occ[row] & zmask; cells && st != 1; cells &= cells - 1u) {
+          const uint32_t wc = (uint32_t)(__ffs(cells) - 1);
+          const uint32_t cell = row * VOID_CELLS + wc;
+          const uint32_t end = start[cell + 1];
+          for (uint32_t j = start[cell]; j < end; j++) {
+            const uint32_t i = V.items[j];
+            const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
+            if (s == 1) {
+              st = 1;
+              break;
+            }
+            st |= s;
+          }
+        }
+      }
+      if (__ballot(st == 1))
+        return 1;
+    }
+  }
+  return __ballot(st == 2) ? 2 : 0;
+}
+
 // keep[k] = 0 refused (masked node, pole probe, or an overlap at the rung
 // radius with the catalogue as the rung began, decided as the reference
 // decides it), 1 kept. A kept candidate that some void collides with under
@@ -171,7 +215,7 @@ __global__ void __launch_bounds__(256) prefilter_kernel(uint32_t* __restrict__ f
 // as the rung began, for a reference max_r of the catalogue as the batch
 // began (exodus_finder.c:2302-2311, the test at the rescaled radius). One
 // wavefront per candidate: a grown sphere reaches many void cells, and the
-// lanes split them; no cross-lane operation until every lane is done.
+// lanes split them (wave_overlap_state).
 constexpr int GO_WAVES = 4;
 
 __global__ void __launch_bounds__(64 * GO_WAVES) grown_overlap_kernel(
@@ -195,43 +239,13 @@ __global__ void __launch_bounds__(64 * GO_WAVES) grown_overlap_kernel(
       pole_hit = overlap_quick(mask, grid_n, node / (grid_n * grid_n), (node / grid_n) % grid_n,
         node % grid_n, (uint32_t)pole);
   }
-  if (r >= 0.0f && V.n) {
+  if (r >= 0.0f && V.n) {  // r is the wave's own: uniform
     OverlapQuery q;
     overlap_query(q, cx[k], cy[k], cz[k], r, max_r, box, f, scale, V.inv_l);
-    for (int cls = VOID_CLASSES - 1; cls >= 0 && st != 1; cls--) {
-      if (V.class_max_r[cls] < 0.0f)
-        continue;
-      int32_t lo[3], hi[3];
-      superset_scan(q, V.class_max_r[cls], V.inv_l, lo, hi);
-      const uint32_t na = (uint32_t)(hi[0] - lo[0] + 1), nb = (uint32_t)(hi[1] - lo[1] + 1);
-      const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
-      const uint32_t* occ = V.occupied + (uint32_t)cls * VOID_CELLS * VOID_CELLS;
-      const uint32_t zmask = row_zmask(lo[2], hi[2]);
-      // lanes over the rows, each row's occupied cells in turn
-      for (uint32_t idx = lane; idx < na * nb && st != 1; idx += 64) {
-        const uint32_t wa = void_wrap(lo[0] + (int32_t)(idx / nb));
-        const uint32_t wb = void_wrap(lo[1] + (int32_t)(idx % nb));
-        const uint32_t row = wa * VOID_CELLS + wb;
-        for (uint32_t cells = occ[row] & zmask; cells && st != 1; cells &= cells - 1u) {
-          const uint32_t wc = (uint32_t)(__ffs(cells) - 1);
-          const uint32_t cell = row * VOID_CELLS + wc;
-          const uint32_t end = start[cell + 1];
-          for (uint32_t j = start[cell]; j < end; j++) {
-            const uint32_t i = V.items[j];
-            const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
-            if (s == 1) {
-              st = 1;
-              break;
-            }
-            st |= s;
-          }
-        }
-      }
-    }
+    st = wave_overlap_state(V, q, lane);
   }
-  const uint64_t ones = __ballot(st == 1), twos = __ballot(st == 2);
   if (lane == 0)
-    state[k] = (ones ? 1u : (twos ? 2u : 0u)) | (pole_hit ? GROWN_POLE_BIT : 0u);
+    state[k] = (uint32_t)st | (pole_hit ? GROWN_POLE_BIT : 0u);
 }
 
 // sif__mark_sphere (utils.c:965-1033) for a list of voids into the device

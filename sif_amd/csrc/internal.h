@@ -99,6 +99,12 @@ struct ScanJob {
   uint64_t total = 0;
   float t = 0.0f;
   uint32_t minima_n = 0;
+  // The writing pass's buffers, when known as the scan begins: it is then
+  // issued at once, and writes only if the count fits cap (checked on the
+  // device), so that scan_finish need not wait for the count to issue it.
+  uint32_t* keys = nullptr;
+  uint32_t* flat = nullptr;
+  uint64_t cap = 0;
   // results, after scan_finish
   uint64_t count = 0;
   uint32_t key_differ = 0xffffffffu;

@@ -178,7 +178,10 @@ __global__ void __launch_bounds__(1024) span_reduce_kernel(const uint32_t* __res
 __global__ void __launch_bounds__(SC_THREADS) cand_write_kernel(
   const float* __restrict__ delta, const uint64_t* __restrict__ mask, uint64_t total,
   float t, uint32_t minima_n, const uint64_t* __restrict__ tile_offsets,
-  uint32_t* __restrict__ keys, uint32_t* __restrict__ flat) {
+  uint32_t* __restrict__ keys, uint32_t* __restrict__ flat, const uint64_t* __restrict__ found,
+  uint64_t cap) {
+  if (*found > cap)  // the buffers are short: the caller grows them and writes again
+    return;
   const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
   uint32_t bits = cand_bits(delta, mask, total, t, first, minima_n);
   uint32_t all;
@@ -270,7 +273,9 @@ __global__ void __launch_bounds__(SC_THREADS) list_write_kernel(
   const uint32_t* __restrict__ lkeys, const uint16_t* __restrict__ llocal,
   const uint32_t* __restrict__ lcount, uint64_t blocks, const uint64_t* __restrict__ mask,
   const uint64_t* __restrict__ tile_offsets, uint32_t* __restrict__ keys,
-  uint32_t* __restrict__ flat) {
+  uint32_t* __restrict__ flat, const uint64_t* __restrict__ found, uint64_t cap) {
+  if (*found > cap)  // the buffers are short: the caller grows them and writes again
+    return;
   uint64_t pos = tile_offsets[blockIdx.x];
   for (uint32_t j = 0; j < LISTS_PER_TILE; j++) {
     const uint64_t b = (uint64_t)blockIdx.x * LISTS_PER_TILE + j;
@@ -306,6 +311,21 @@ thread_local hipEvent_t g_back_ready = nullptr;
 
 // The counting pass of a scan job, and the copy of its results to host
 // memory (scan_finish reads them).
+// The writing pass into keys/flat (capacity cap): writes nothing when the
+// count (on the device) exceeds cap.
+static int scan_write(const ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap,
+  uint64_t tiles) {
+  if (J.lists)
+    list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
+      J.lists->local, J.lists->count, J.lists->blocks, J.mask, g_tile_pos.as<uint64_t>(),
+      keys, flat, g_count.as<uint64_t>(), cap);
+  else
+    cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
+      J.t, J.minima_n, g_tile_pos.as<uint64_t>(), keys, flat, g_count.as<uint64_t>(), cap);
+  SIF_HIP(hipGetLastError());
+  return SIF_OK;
+}
+
 int scan_begin(ScanJob& J) {
   J.count = 0;
   J.key_differ = 0xffffffffu;
@@ -364,6 +384,11 @@ int scan_begin(ScanJob& J) {
   SIF_HIP(hipMemcpyAsync(g_back, g_count.ptr, J.lists ? 24 : 16, hipMemcpyDeviceToHost,
     stream()));
   SIF_HIP(hipEventRecord(g_back_ready, stream()));
+  if (J.keys && J.flat) {
+    st = scan_write(J, J.keys, J.flat, J.cap, tiles);
+    if (st != SIF_OK)
+      return st;
+  }
   timing_end(SIF_AMD_STAGE_SCAN);
   J.begun = true;
   return SIF_OK;
@@ -385,16 +410,12 @@ int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap) {
     return SIF_OK;
   J.begun = false;
   const uint64_t tiles = J.lists ? list_tiles(J.lists->blocks) : (J.total + SC_TILE - 1) / SC_TILE;
-  if (n > 0) {
+  const bool written = J.keys == keys && J.flat == flat && J.keys && J.flat && n <= J.cap;
+  if (n > 0 && !written) {
     timing_begin(SIF_AMD_STAGE_SCAN);
-    if (J.lists)
-      list_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
-        J.lists->local, J.lists->count, J.lists->blocks, J.mask, g_tile_pos.as<uint64_t>(),
-        keys, flat);
-    else
-      cand_write_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
-        J.t, J.minima_n, g_tile_pos.as<uint64_t>(), keys, flat);
-    SIF_HIP(hipGetLastError());
+    const int st = scan_write(J, keys, flat, cap, tiles);
+    if (st != SIF_OK)
+      return st;
     timing_end(SIF_AMD_STAGE_SCAN);
   }
   if (timing_active()) {
