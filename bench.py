@@ -99,19 +99,28 @@ STAGE_KERNELS = {
 # that holds it names the commands and carries a line
 # "traffic_bytes_per_launch[<stage>]: <bytes>" per stage measured
 TRAFFIC = {"plain512": "profiles/hbm_traffic_plain512.md"}
+# The kernels' own durations per launch of a stage from the rocprofv3 kernel
+# trace of the same command ("kernel_ms_per_launch[<stage>]: <ms>"): the HIP
+# events here bracket a stage's launches on its stream, which also holds the
+# waits of its kernels for CUs the other stream's kernels occupy.
+TRACE = {"plain512": "profiles/exodus_plain512_kernel_trace.md"}
+
+
+def summary_value(table, config, key, cast):
+    """(value, the summary under profiles/ that holds it) or (None, None)."""
+    path = table.get(config)
+    if not path or not os.path.exists(os.path.join(ROOT, path)):
+        return None, None
+    for line in open(os.path.join(ROOT, path)):
+        if line.startswith(key):
+            return cast(line.split(":")[1]), path
+    return None, None
 
 
 def measured_traffic(config, stage):
     """(bytes per launch, the summary under profiles/ that holds it) or
     (None, None)."""
-    path = TRAFFIC.get(config)
-    if not path or not os.path.exists(os.path.join(ROOT, path)):
-        return None, None
-    key = f"traffic_bytes_per_launch[{stage}]:"
-    for line in open(os.path.join(ROOT, path)):
-        if line.startswith(key):
-            return int(line.split(":")[1]), path
-    return None, None
+    return summary_value(TRAFFIC, config, f"traffic_bytes_per_launch[{stage}]:", int)
 
 
 def roofline(timing, config):
@@ -120,8 +129,9 @@ def roofline(timing, config):
     the dominant one: the stage with the most device time per step (a stage
     of several kernels is timed as one region; the smoothing's runs on a
     second stream beside the commit). Its measured traffic is read from the
-    rocprof counter summary named in `traffic_source` (an earlier profiled
-    run of this workload, not this run)."""
+    rocprof counter summary named in `traffic_source`, and its kernels' own
+    durations from the kernel trace named in `profiled_source` (earlier
+    profiled runs of this workload, not this run)."""
     stages = {}
     for name, (ms, launches, by) in timing.items():
         if name not in STAGE_KERNELS or launches == 0 or by <= 0 or ms <= 0:
@@ -134,10 +144,16 @@ def roofline(timing, config):
     top = max(stages, key=lambda k: timing[k][0])
     d = stages[top]
     traffic, source = measured_traffic(config, top)
+    kernel_ms, trace = summary_value(TRACE, config, f"kernel_ms_per_launch[{top}]:", float)
     return {"bound": "hbm", "kernel": d["kernel"], "stage": top, "achieved": d["achieved"],
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
             "traffic": traffic, "traffic_source": source,
             "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
+            # the kernels' own durations a launch in the rocprof trace of
+            # this command, and the rate the algorithmic bytes make over them
+            "profiled_kernel_ms": kernel_ms, "profiled_source": trace,
+            "profiled_achieved": (round(d["bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9, 1)
+                                  if kernel_ms else None),
             "stages": stages}
 
 

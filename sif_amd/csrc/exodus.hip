@@ -268,23 +268,43 @@ int grow_keep(DeviceBuffer& b, size_t used, size_t want) {
 // Appends voids (host arrays) to the device catalogue and paints them into
 // the device mask with radius scale * r. The host arrays may change as soon
 // as this returns (they are staged in pinned memory; the stream keeps order).
+// Room for `total` voids on the device. Growing moves the arrays (the
+// contents kept, the index's items not): an index built before is stale
+// after it, so appends come after the last kernel that reads the index.
+int reserve_voids(uint64_t total) {
+  DeviceVoids& D = g_voids;
+  if (total <= D.cap)
+    return SIF_OK;
+  if (total > 0xffffffffull) {
+    error_set(SIF_ERR_RANGE, "exodus: room for %llu voids (at most 2^32 - 1)",
+      (unsigned long long)total);
+    return SIF_ERR_RANGE;
+  }
+  uint64_t want = D.cap ? D.cap : 65536;
+  while (want < total)
+    want *= 2;
+  if (want > 0xffffffffull)
+    want = 0xffffffffull;
+  int st = grow_keep(D.x, (size_t)D.n * 4, (size_t)want * 4);
+  if (st == SIF_OK) st = grow_keep(D.y, (size_t)D.n * 4, (size_t)want * 4);
+  if (st == SIF_OK) st = grow_keep(D.z, (size_t)D.n * 4, (size_t)want * 4);
+  if (st == SIF_OK) st = grow_keep(D.r, (size_t)D.n * 4, (size_t)want * 4);
+  if (st == SIF_OK) st = D.items.reserve((size_t)want * 4);
+  if (st != SIF_OK)
+    return st;
+  D.cap = (uint32_t)want;
+  return SIF_OK;
+}
+
 int append_voids(const float* x, const float* y, const float* z, const float* r, uint32_t k,
   float box, float scale, float l, uint32_t grid_n) {
   if (k == 0)
     return SIF_OK;
   DeviceVoids& D = g_voids;
-  if (D.n + k > D.cap) {
-    uint32_t want = D.cap ? D.cap : 65536;
-    while (want < D.n + k)
-      want *= 2;
-    int st = grow_keep(D.x, (size_t)D.n * 4, (size_t)want * 4);
-    if (st == SIF_OK) st = grow_keep(D.y, (size_t)D.n * 4, (size_t)want * 4);
-    if (st == SIF_OK) st = grow_keep(D.z, (size_t)D.n * 4, (size_t)want * 4);
-    if (st == SIF_OK) st = grow_keep(D.r, (size_t)D.n * 4, (size_t)want * 4);
-    if (st == SIF_OK) st = D.items.reserve((size_t)want * 4);
+  if ((uint64_t)D.n + k > D.cap) {
+    const int st = reserve_voids((uint64_t)D.n + k);
     if (st != SIF_OK)
       return st;
-    D.cap = want;
   }
   // the previous append's copies read the staging: let them finish (not the
   // rest of the stream, which may hold a growth launch in flight)
