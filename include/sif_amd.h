@@ -353,6 +353,14 @@ SIF_API int sif_amd_set_batch_window(uint32_t floor, uint32_t ceiling);
  * restored as the reference does. No reference counterpart. Not
  * synchronised. */
 SIF_API int sif_amd_set_cell_lists(int on);
+/* The device copy of the catalogue (positions, radii and the overlap index)
+ * that exodus runs on this thread keep between runs: frees it, and sets the
+ * number of voids it is next allocated for (0: the default, 65536). It grows
+ * by doubling as runs need; a caller that knows the catalogue's size can
+ * reserve it whole, and a small room makes a run grow it many times. The
+ * catalogue does not depend on it. No reference counterpart. Call between
+ * runs only. */
+SIF_API int sif_amd_set_void_room(uint32_t voids);
 
 /* --- the batched form (no reference counterpart: many independent boxes,
  * each the catalogue sif_finder_exodus_raw would give it) --- */

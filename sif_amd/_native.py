@@ -134,6 +134,7 @@ _sig("sif_amd_exodus_survey_device", C.POINTER(Catalogue), P, P, u32, f32, C.POI
      C.POINTER(RungStats))
 _sig("sif_amd_set_batch_window", i32, u32, u32)
 _sig("sif_amd_set_cell_lists", i32, i32)
+_sig("sif_amd_set_void_room", i32, u32)
 
 
 class Box(C.Structure):  # sif_amd_box_t

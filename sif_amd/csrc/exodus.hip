@@ -236,6 +236,9 @@ struct DeviceVoids {
   DeviceBuffer x, y, z, r;
   DeviceBuffer start, items, occupied;
   uint32_t n = 0, cap = 0;
+  // Bumped each time the arrays move (reserve_voids): an index built at an
+  // earlier generation points at freed memory.
+  uint32_t gen = 0;
   // pinned staging for an append
   float* hx = nullptr;
   float* hy = nullptr;
@@ -246,6 +249,9 @@ struct DeviceVoids {
   float class_max[VOID_CLASSES] = {-1.0f, -1.0f, -1.0f, -1.0f};
 };
 thread_local DeviceVoids g_voids;
+// the voids the device catalogue is first allocated for (sif_amd_set_void_room)
+constexpr uint32_t VOID_ROOM_DEFAULT = 65536;
+thread_local uint32_t g_void_room = 0;
 
 int grow_keep(DeviceBuffer& b, size_t used, size_t want) {
   if (want <= b.bytes && b.ptr)
@@ -265,12 +271,10 @@ int grow_keep(DeviceBuffer& b, size_t used, size_t want) {
   return SIF_OK;
 }
 
-// Appends voids (host arrays) to the device catalogue and paints them into
-// the device mask with radius scale * r. The host arrays may change as soon
-// as this returns (they are staged in pinned memory; the stream keeps order).
 // Room for `total` voids on the device. Growing moves the arrays (the
-// contents kept, the index's items not): an index built before is stale
-// after it, so appends come after the last kernel that reads the index.
+// contents kept, the index's items not), which makes every index built
+// before stale: so each rung reserves, before its index is built, room for
+// every void appended until the next one is (start_filter).
 int reserve_voids(uint64_t total) {
   DeviceVoids& D = g_voids;
   if (total <= D.cap)
@@ -280,7 +284,7 @@ int reserve_voids(uint64_t total) {
       (unsigned long long)total);
     return SIF_ERR_RANGE;
   }
-  uint64_t want = D.cap ? D.cap : 65536;
+  uint64_t want = D.cap ? D.cap : (g_void_room ? g_void_room : VOID_ROOM_DEFAULT);
   while (want < total)
     want *= 2;
   if (want > 0xffffffffull)
@@ -293,9 +297,13 @@ int reserve_voids(uint64_t total) {
   if (st != SIF_OK)
     return st;
   D.cap = (uint32_t)want;
+  D.gen++;
   return SIF_OK;
 }
 
+// Appends voids (host arrays) to the device catalogue and paints them into
+// the device mask with radius scale * r. The host arrays may change as soon
+// as this returns (they are staged in pinned memory; the stream keeps order).
 int append_voids(const float* x, const float* y, const float* z, const float* r, uint32_t k,
   float box, float scale, float l, uint32_t grid_n) {
   if (k == 0)
@@ -383,6 +391,7 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   out->items = D.items.as<uint32_t>();
   out->occupied = D.occupied.as<uint32_t>();
   out->n = D.n;
+  out->gen = D.gen;
   out->inv_l = V.inv_l;
   out->max_r = V.max_r;
   for (int c = 0; c < VOID_CLASSES; c++)
@@ -840,7 +849,13 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   auto start_filter = [&]() -> int {
     const size_t j = rs.rung;
     const double th = now_ms();
-    int e = device_void_index(V, &rs.VI);
+    // Room for every void appended before the next rung's index is built:
+    // the rest of the rung under way (a start that goes ahead comes before
+    // its last commit, at most a batch) and this rung's own (at most its
+    // candidates). The index then stays valid for this rung's batches.
+    int e = reserve_voids((uint64_t)g_voids.n + rs.nc + 2ull * BATCH_ROOM);
+    if (e == SIF_OK)
+      e = device_void_index(V, &rs.VI);
     if (timing_active())
       timing_add(SIF_AMD_STAGE_HOST, now_ms() - th);
     if (e != SIF_OK)
@@ -1257,6 +1272,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         e = grow_extend(R.d_offsets, R.d_mx, R.d_my, R.d_mz, R.mesh_n, R.box,
           B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(), bc, ext_rmin.data(),
           ext_reach.data(), ext0, n_ext, R.threshold, vol, B.p_rad, B.p_reason, B.p_nin, &mw);
+      if (e == SIF_OK && VI.gen != g_voids.gen) {
+        error_set(SIF_ERR_INVALID, "exodus: internal: the rung's void index predates a move "
+          "of the device void arrays");
+        e = SIF_ERR_INVALID;
+      }
       if (e == SIF_OK)
         e = grown_overlap(B.d_cx.as<float>(), B.d_cy.as<float>(), B.d_cz.as<float>(),
           B.d_flat.as<uint32_t>(), B.p_rad, bc, VI, V.max_r, R.box, R.f, scale,
@@ -1992,6 +2012,21 @@ sif_catalogue_t* sif_amd_exodus_device(float* d_delta, uint32_t n_cells, float b
   return exodus_device(d_delta, n_cells, box, d_offsets, d_mx, d_my, d_mz, nullptr, nullptr,
     0.0, mesh_cells, n_particles, radii, n_radii, threshold, overlap_fraction, options,
     restore, n_in, stats);
+}
+
+int sif_amd_set_void_room(uint32_t voids) {
+  int st = ensure_init();
+  if (st != SIF_OK)
+    return st;
+  SIF_HIP(hipStreamSynchronize(stream()));
+  DeviceVoids& D = g_voids;
+  for (DeviceBuffer* b : {&D.x, &D.y, &D.z, &D.r, &D.start, &D.items, &D.occupied})
+    b->release();
+  D.n = 0;
+  D.cap = 0;
+  D.gen++;
+  g_void_room = voids;
+  return SIF_OK;
 }
 
 int sif_amd_set_cell_lists(int on) {

@@ -132,6 +132,7 @@ struct VoidIndex {
   // word (class 32 + wa) 32 + wb holds the row's 32 cells along z
   const uint32_t* occupied;
   uint32_t n;                  // voids
+  uint32_t gen;                // the arrays' generation it was built at (host check)
   float inv_l;                 // (float)32 / box
   float max_r;                 // largest radius in the index
   float class_max_r[VOID_CLASSES];  // < 0 for an empty class

@@ -303,6 +303,13 @@ def set_cell_lists(on=True):
     check(lib.sif_amd_set_cell_lists(1 if on else 0))
 
 
+def set_void_room(voids=0):
+    """Frees the device copy of the catalogue that exodus runs keep and sets
+    the voids it is next allocated for (0: the default); the catalogue does
+    not depend on it."""
+    check(lib.sif_amd_set_void_room(int(voids)))
+
+
 def set_batch_window(floor=0, ceiling=0):
     """The growth batch window of later exodus runs in this process; (0, 0)
     restores the defaults."""

@@ -114,6 +114,35 @@ def test_config1_catalogue_equals_oracle(config1):
     np.testing.assert_array_equal(g.get().view(np.uint32), delta0.view(np.uint32))
 
 
+def test_config1_catalogue_independent_of_void_room(config1):
+    """The device copy of the catalogue grown many times within a run (room
+    for 16 voids at first, doubled as needed) gives the catalogue of a run
+    whose room was allocated once, under the default batch window and a small
+    one (more rung starts issued ahead of their rung's last commits). Each
+    rung reserves its room before its overlap index is built, so the appends
+    that follow never move the arrays the index points at (a launch with an
+    index older than the arrays is refused)."""
+    D, (x, y, z), (dx, dy, dz), m, om, radii, fields = config1
+    c = C1
+    for window in (None, (64, 512)):
+        if window:
+            D.set_batch_window(*window)
+        try:
+            D.set_void_room(0)
+            g = _grid(D, dx, dy, dz, len(x), c["n"], c["box"])
+            ref, ref_nin, _ = D.exodus(g, c["n"], c["box"], m, radii, THRESH)
+            D.set_void_room(16)
+            g = _grid(D, dx, dy, dz, len(x), c["n"], c["box"])
+            cat, nin, _ = D.exodus(g, c["n"], c["box"], m, radii, THRESH)
+        finally:
+            D.set_void_room(0)
+            D.set_batch_window(0, 0)
+        _say(f"void room 16 (window {window}): {len(cat)} voids against {len(ref)}")
+        assert len(ref) > 1000
+        np.testing.assert_array_equal(cat.view(np.uint32), ref.view(np.uint32))
+        np.testing.assert_array_equal(nin, ref_nin)
+
+
 def _match(cat_a, cat_b):
     """Voids of two catalogues on the same node: (indices into a, into b).
     Centres are grid nodes (node index times the cell length, one float
