@@ -234,7 +234,7 @@ double now_ms() {
 // next rung is built from it and the mask is painted from it.
 struct DeviceVoids {
   DeviceBuffer x, y, z, r;
-  DeviceBuffer start, items, occupied;
+  DeviceBuffer start, packed, occupied;  // the index: cell starts, voids by cell, occupancy
   uint32_t n = 0, cap = 0;
   // Bumped each time the arrays move (reserve_voids): an index built at an
   // earlier generation points at freed memory.
@@ -272,7 +272,7 @@ int grow_keep(DeviceBuffer& b, size_t used, size_t want) {
 }
 
 // Room for `total` voids on the device. Growing moves the arrays (the
-// contents kept, the index's items not), which makes every index built
+// contents kept, the index's own arrays not), which makes every index built
 // before stale: so each rung reserves, before its index is built, room for
 // every void appended until the next one is (start_filter).
 int reserve_voids(uint64_t total) {
@@ -293,7 +293,7 @@ int reserve_voids(uint64_t total) {
   if (st == SIF_OK) st = grow_keep(D.y, (size_t)D.n * 4, (size_t)want * 4);
   if (st == SIF_OK) st = grow_keep(D.z, (size_t)D.n * 4, (size_t)want * 4);
   if (st == SIF_OK) st = grow_keep(D.r, (size_t)D.n * 4, (size_t)want * 4);
-  if (st == SIF_OK) st = D.items.reserve((size_t)want * 4);
+  if (st == SIF_OK) st = D.packed.reserve((size_t)want * sizeof(float4));
   if (st != SIF_OK)
     return st;
   D.cap = (uint32_t)want;
@@ -369,7 +369,7 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   const uint32_t slots = VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS;
   int st = D.start.reserve((size_t)(slots + 1) * 4);
   if (st == SIF_OK)
-    st = D.items.reserve((size_t)(D.cap ? D.cap : 1) * 4);
+    st = D.packed.reserve((size_t)(D.cap ? D.cap : 1) * sizeof(float4));
   if (st == SIF_OK)
     st = D.occupied.reserve((size_t)slots / 32 * 4);
   if (st == SIF_OK && D.n == 0) {  // keep the pointers valid
@@ -380,15 +380,11 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   }
   if (st == SIF_OK)
     st = build_void_index(D.x.as<float>(), D.y.as<float>(), D.z.as<float>(), D.r.as<float>(),
-      D.n, V.box, D.start.as<uint32_t>(), D.items.as<uint32_t>(), D.occupied.as<uint32_t>());
+      D.n, V.box, D.start.as<uint32_t>(), D.packed.as<float4>(), D.occupied.as<uint32_t>());
   if (st != SIF_OK)
     return st;
-  out->cx = D.x.as<float>();
-  out->cy = D.y.as<float>();
-  out->cz = D.z.as<float>();
-  out->r = D.r.as<float>();
   out->cell_start = D.start.as<uint32_t>();
-  out->items = D.items.as<uint32_t>();
+  out->voids = D.packed.as<float4>();
   out->occupied = D.occupied.as<uint32_t>();
   out->n = D.n;
   out->gen = D.gen;
@@ -2020,7 +2016,7 @@ int sif_amd_set_void_room(uint32_t voids) {
     return st;
   SIF_HIP(hipStreamSynchronize(stream()));
   DeviceVoids& D = g_voids;
-  for (DeviceBuffer* b : {&D.x, &D.y, &D.z, &D.r, &D.start, &D.items, &D.occupied})
+  for (DeviceBuffer* b : {&D.x, &D.y, &D.z, &D.r, &D.start, &D.packed, &D.occupied})
     b->release();
   D.n = 0;
   D.cap = 0;

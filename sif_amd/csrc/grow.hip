@@ -103,8 +103,8 @@ __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
           const uint32_t cell = row * VOID_CELLS + wc;
           const uint32_t end = start[cell + 1];
           for (uint32_t k = start[cell]; k < end; k++) {
-            const uint32_t i = V.items[k];
-            const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
+            const float4 v = V.voids[k];
+            const int s = pair_state(q, wa, wb, wc, v.x, v.y, v.z, v.w);
             if (s == 1)
               return 1;
             state |= s;
@@ -143,8 +143,8 @@ __device__ int wave_overlap_state(const VoidIndex& V, const OverlapQuery& q, uin
           const uint32_t cell = row * VOID_CELLS + wc;
           const uint32_t end = start[cell + 1];
           for (uint32_t j = start[cell]; j < end; j++) {
-            const uint32_t i = V.items[j];
-            const int s = pair_state(q, wa, wb, wc, V.cx[i], V.cy[i], V.cz[i], V.r[i]);
+            const float4 v = V.voids[j];
+            const int s = pair_state(q, wa, wb, wc, v.x, v.y, v.z, v.w);
             if (s == 1) {
               st = 1;
               break;
@@ -771,13 +771,14 @@ __global__ void void_start_kernel(const uint64_t* __restrict__ pos, uint32_t slo
 // Within a cell the order of the voids is not fixed; nothing that reads the
 // index depends on it (an overlap state is a yes/no over a set).
 __global__ void void_scatter_kernel(const uint32_t* __restrict__ slot, uint32_t nv,
-  const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
-  uint32_t* __restrict__ items) {
+  const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+  const float* __restrict__ vr, const uint32_t* __restrict__ start,
+  uint32_t* __restrict__ cursor, float4* __restrict__ voids) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv)
     return;
   const uint32_t sl = slot[i];
-  items[start[sl] + atomicAdd(&cursor[sl], 1u)] = i;
+  voids[start[sl] + atomicAdd(&cursor[sl], 1u)] = make_float4(vx[i], vy[i], vz[i], vr[i]);
 }
 
 namespace {
@@ -785,7 +786,7 @@ thread_local DeviceBuffer g_vi_slot, g_vi_counts, g_vi_pos, g_vi_cursor;
 }
 
 int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
-  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items, uint32_t* d_occupied) {
+  uint32_t nv, float box, uint32_t* d_start, float4* d_voids, uint32_t* d_occupied) {
   const uint32_t slots = VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS;
   static_assert((VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS) % 64 == 0,
     "whole waves of slots");
@@ -811,7 +812,7 @@ int build_void_index(const float* vx, const float* vy, const float* vz, const fl
   SIF_HIP(hipGetLastError());
   if (nv)
     void_scatter_kernel<<<grid_for(nv, 256), 256, 0, stream()>>>(g_vi_slot.as<uint32_t>(), nv,
-      d_start, g_vi_cursor.as<uint32_t>(), d_items);
+      vx, vy, vz, vr, d_start, g_vi_cursor.as<uint32_t>(), d_voids);
   SIF_HIP(hipGetLastError());
   return SIF_OK;
 }

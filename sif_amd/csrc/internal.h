@@ -122,12 +122,10 @@ int keys_to_delta(const uint32_t* keys, uint64_t n, float* out);
 // the reference's 32^3 cells.
 constexpr int VOID_CLASSES = 4;
 struct VoidIndex {
-  const float* cx;
-  const float* cy;
-  const float* cz;
-  const float* r;
   const uint32_t* cell_start;  // VOID_CLASSES * 32^3 + 1 entries, class-major
-  const uint32_t* items;       // void ids, grouped by (class, reference cell)
+  // each void's (x, y, z, r), grouped by (class, reference cell): a scan
+  // reads a void in one 16-byte load, with no id to look up first
+  const float4* voids;
   // one bit a (class, cell): set when the cell lists a void of the class;
   // word (class 32 + wa) 32 + wb holds the row's 32 cells along z
   const uint32_t* occupied;
@@ -209,9 +207,9 @@ int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
 int count_masked_since(const uint32_t* flat, const uint8_t* why, uint64_t n,
   const uint64_t* mask, uint64_t* d_out);
 // The class-split CSR of nv device-resident voids: d_start has
-// VOID_CLASSES * 32^3 + 1 entries, d_items nv.
+// VOID_CLASSES * 32^3 + 1 entries, d_voids nv (each void's x, y, z, r).
 int build_void_index(const float* vx, const float* vy, const float* vz, const float* vr,
-  uint32_t nv, float box, uint32_t* d_start, uint32_t* d_items,
+  uint32_t nv, float box, uint32_t* d_start, float4* d_voids,
   uint32_t* d_occupied);
 // Per grown sphere: its overlap state against the index (bits 0-1) and, in
 // GROWN_POLE_BIT (overlap.h), whether the pole probe at the grown radius hits
