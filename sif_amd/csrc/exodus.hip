@@ -293,7 +293,8 @@ int reserve_voids(uint64_t total) {
   if (st == SIF_OK) st = grow_keep(D.y, (size_t)D.n * 4, (size_t)want * 4);
   if (st == SIF_OK) st = grow_keep(D.z, (size_t)D.n * 4, (size_t)want * 4);
   if (st == SIF_OK) st = grow_keep(D.r, (size_t)D.n * 4, (size_t)want * 4);
-  if (st == SIF_OK) st = D.packed.reserve((size_t)want * sizeof(float4));
+  // four entries past the last void: the scans read voids four at a time
+  if (st == SIF_OK) st = D.packed.reserve((size_t)(want + 4) * sizeof(float4));
   if (st != SIF_OK)
     return st;
   D.cap = (uint32_t)want;
@@ -369,7 +370,7 @@ int device_void_index(const Voids& V, VoidIndex* out) {
   const uint32_t slots = VOID_CLASSES * VOID_CELLS * VOID_CELLS * VOID_CELLS;
   int st = D.start.reserve((size_t)(slots + 1) * 4);
   if (st == SIF_OK)
-    st = D.packed.reserve((size_t)(D.cap ? D.cap : 1) * sizeof(float4));
+    st = D.packed.reserve((size_t)(D.cap + 4) * sizeof(float4));
   if (st == SIF_OK)
     st = D.occupied.reserve((size_t)slots / 32 * 4);
   if (st == SIF_OK && D.n == 0) {  // keep the pointers valid

@@ -48,23 +48,28 @@ __device__ __forceinline__ bool mask_bit(const uint64_t* m, uint64_t i) {
   return (m[i >> 6] >> (i & 63)) & 1ull;
 }
 
-// utils.c:819-848
+// utils.c:819-848: any of the six nodes r away along the axes in the mask.
+// The six words are loaded together (the mask is far larger than the caches,
+// so each is a trip to memory) and then tested; the answer is their OR.
 __device__ bool overlap_quick(const uint64_t* mask, uint32_t n, uint32_t ix,
   uint32_t iy, uint32_t iz, uint32_t r) {
   const uint64_t nn = n;
-  if (mask_bit(mask, ((uint64_t)wrap_pbc((int32_t)(ix + r), n) * nn + iy) * nn + iz))
-    return true;
-  if (mask_bit(mask, ((uint64_t)wrap_pbc((int32_t)ix - (int32_t)r, n) * nn + iy) * nn + iz))
-    return true;
-  if (mask_bit(mask, ((uint64_t)ix * nn + wrap_pbc((int32_t)(iy + r), n)) * nn + iz))
-    return true;
-  if (mask_bit(mask, ((uint64_t)ix * nn + wrap_pbc((int32_t)iy - (int32_t)r, n)) * nn + iz))
-    return true;
-  if (mask_bit(mask, ((uint64_t)ix * nn + iy) * nn + wrap_pbc((int32_t)(iz + r), n)))
-    return true;
-  if (mask_bit(mask, ((uint64_t)ix * nn + iy) * nn + wrap_pbc((int32_t)iz - (int32_t)r, n)))
-    return true;
-  return false;
+  const uint64_t node[6] = {
+    ((uint64_t)wrap_pbc((int32_t)(ix + r), n) * nn + iy) * nn + iz,
+    ((uint64_t)wrap_pbc((int32_t)ix - (int32_t)r, n) * nn + iy) * nn + iz,
+    ((uint64_t)ix * nn + wrap_pbc((int32_t)(iy + r), n)) * nn + iz,
+    ((uint64_t)ix * nn + wrap_pbc((int32_t)iy - (int32_t)r, n)) * nn + iz,
+    ((uint64_t)ix * nn + iy) * nn + wrap_pbc((int32_t)(iz + r), n),
+    ((uint64_t)ix * nn + iy) * nn + wrap_pbc((int32_t)iz - (int32_t)r, n)};
+  uint64_t w[6];
+#pragma unroll
+  for (int p = 0; p < 6; p++)
+    w[p] = mask[node[p] >> 6];
+  uint64_t hit = 0;
+#pragma unroll
+  for (int p = 0; p < 6; p++)
+    hit |= (w[p] >> (node[p] & 63)) & 1ull;
+  return hit != 0;
 }
 
 // The overlap state (overlap.h) of a query against the CSR void index: the
@@ -81,8 +86,57 @@ __device__ __forceinline__ uint32_t row_zmask(int32_t lo, int32_t hi) {
   return (uint32_t)(m | (m >> VOID_CELLS));
 }
 
-// The scan visits only the occupied cells of each row (the occupancy words);
-// the state is an OR over voids, so the order they come in changes nothing.
+// The index lists a row's 32 cells along z one after another, so the voids of
+// the cells a scan covers in a row are one stretch of the index (two where
+// the z range wraps): a scan reads the row's occupancy word, skips an empty
+// row, and otherwise reads that stretch four voids at a time. A void's z cell
+// is that of its centre (void_axis_cell), the cell the index lists it under.
+// The state is an OR over voids, so the order they come in changes nothing.
+struct RowSpans {
+  uint32_t a0, b0, a1, b1;  // [a0, b0) and [a1, b1) of the row's z cells
+};
+__device__ __forceinline__ RowSpans row_spans(int32_t lo, int32_t hi) {
+  RowSpans r{0, 0, 0, 0};
+  const uint32_t span = (uint32_t)(hi - lo);
+  if (span + 1 >= (uint32_t)VOID_CELLS) {
+    r.b0 = VOID_CELLS;
+    return r;
+  }
+  const uint32_t w = void_wrap(lo);
+  r.a0 = w;
+  if (w + span < (uint32_t)VOID_CELLS) {
+    r.b0 = w + span + 1;
+  } else {
+    r.b0 = VOID_CELLS;
+    r.b1 = w + span + 1 - VOID_CELLS;
+  }
+  return r;
+}
+
+// The voids [k0, k1) of the index against q (row wa, wb): 1 at the first
+// state 1, else the OR of their states.
+__device__ __forceinline__ int stretch_state(const VoidIndex& V, const OverlapQuery& q,
+  uint32_t wa, uint32_t wb, uint32_t k0, uint32_t k1) {
+  int state = 0;
+  for (uint32_t k = k0; k < k1; k += 4) {
+    float4 v[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++)
+      v[u] = k + u < k1 ? V.voids[k + u] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+      if (k + u < k1) {
+        const uint32_t wc = void_axis_cell(v[u].z, V.inv_l);
+        const int s = pair_state(q, wa, wb, wc, v[u].x, v[u].y, v[u].z, v[u].w);
+        if (s == 1)
+          return 1;
+        state |= s;
+      }
+    }
+  }
+  return state;
+}
+
 __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
   int state = 0;
   for (int cls = VOID_CLASSES - 1; cls >= 0; cls--) {
@@ -93,34 +147,52 @@ __device__ int overlap_state(const VoidIndex& V, const OverlapQuery& q) {
     const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
     const uint32_t* occ = V.occupied + (uint32_t)cls * VOID_CELLS * VOID_CELLS;
     const uint32_t zmask = row_zmask(lo[2], hi[2]);
-    for (int32_t a = lo[0]; a <= hi[0]; a++) {
-      const uint32_t wa = void_wrap(a);
-      for (int32_t b = lo[1]; b <= hi[1]; b++) {
-        const uint32_t wb = void_wrap(b);
-        const uint32_t row = wa * VOID_CELLS + wb;
-        for (uint32_t cells = occ[row] & zmask; cells; cells &= cells - 1u) {
-          const uint32_t wc = (uint32_t)(__ffs(cells) - 1);
-          const uint32_t cell = row * VOID_CELLS + wc;
-          const uint32_t end = start[cell + 1];
-          for (uint32_t k = start[cell]; k < end; k++) {
-            const float4 v = V.voids[k];
-            const int s = pair_state(q, wa, wb, wc, v.x, v.y, v.z, v.w);
-            if (s == 1)
-              return 1;
-            state |= s;
-          }
-        }
+    const RowSpans z = row_spans(lo[2], hi[2]);
+    const uint32_t nb = (uint32_t)(hi[1] - lo[1] + 1);
+    const uint32_t rows = (uint32_t)(hi[0] - lo[0] + 1) * nb;
+    // four rows at a time: their occupancy words, then the stretch bounds of
+    // the occupied ones, each set loaded together, then the stretches
+    for (uint32_t r0 = 0; r0 < rows; r0 += 4) {
+      uint32_t wa[4], wb[4], o[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++) {
+        const uint32_t idx = r0 + u < rows ? r0 + u : r0;
+        wa[u] = void_wrap(lo[0] + (int32_t)(idx / nb));
+        wb[u] = void_wrap(lo[1] + (int32_t)(idx % nb));
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++)
+        o[u] = r0 + u < rows ? occ[wa[u] * VOID_CELLS + wb[u]] & zmask : 0u;
+      uint32_t k0[4], k1[4], k2[4], k3[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++) {
+        const uint32_t* rs = start + (wa[u] * VOID_CELLS + wb[u]) * VOID_CELLS;
+        k0[u] = o[u] ? rs[z.a0] : 0u;
+        k1[u] = o[u] ? rs[z.b0] : 0u;
+        k2[u] = o[u] && z.b1 > z.a1 ? rs[z.a1] : 0u;
+        k3[u] = o[u] && z.b1 > z.a1 ? rs[z.b1] : 0u;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++) {
+        if (!o[u])
+          continue;
+        const int s0 = stretch_state(V, q, wa[u], wb[u], k0[u], k1[u]);
+        if (s0 == 1)
+          return 1;
+        state |= s0;
+        const int s1 = stretch_state(V, q, wa[u], wb[u], k2[u], k3[u]);
+        if (s1 == 1)
+          return 1;
+        state |= s1;
       }
     }
   }
   return state;
 }
 
-// overlap_state() by a whole wavefront for one query: the lanes
-// split the rows of each class's superset cells and walk the occupied cells
-// of their rows (the occupancy words). The state is an OR over voids, so
-// the order they come in changes nothing; the wave stops at the first
-// state 1. Every lane returns the same state.
+// overlap_state() by a whole wavefront for one query: the lanes split the
+// rows of each class's superset cells. The wave stops at the first state 1.
+// Every lane returns the same state.
 __device__ int wave_overlap_state(const VoidIndex& V, const OverlapQuery& q, uint32_t lane) {
   int st = 0;
   for (int cls = VOID_CLASSES - 1; cls >= 0; cls--) {
@@ -132,25 +204,19 @@ __device__ int wave_overlap_state(const VoidIndex& V, const OverlapQuery& q, uin
     const uint32_t* start = V.cell_start + (uint32_t)cls * VOID_CELLS * VOID_CELLS * VOID_CELLS;
     const uint32_t* occ = V.occupied + (uint32_t)cls * VOID_CELLS * VOID_CELLS;
     const uint32_t zmask = row_zmask(lo[2], hi[2]);
+    const RowSpans z = row_spans(lo[2], hi[2]);
     for (uint32_t base = 0; base < na * nb; base += 64) {
       const uint32_t idx = base + lane;
-      if (idx < na * nb) {
+      if (idx < na * nb && st != 1) {
         const uint32_t wa = void_wrap(lo[0] + (int32_t)(idx / nb));
         const uint32_t wb = void_wrap(lo[1] + (int32_t)(idx % nb));
         const uint32_t row = wa * VOID_CELLS + wb;
-        for (uint32_t cells = occ[row] & zmask; cells && st != 1; cells &= cells - 1u) {
-          const uint32_t wc = (uint32_t)(__ffs(cells) - 1);
-          const uint32_t cell = row * VOID_CELLS + wc;
-          const uint32_t end = start[cell + 1];
-          for (uint32_t j = start[cell]; j < end; j++) {
-            const float4 v = V.voids[j];
-            const int s = pair_state(q, wa, wb, wc, v.x, v.y, v.z, v.w);
-            if (s == 1) {
-              st = 1;
-              break;
-            }
-            st |= s;
-          }
+        if (occ[row] & zmask) {
+          const uint32_t* rs = start + row * VOID_CELLS;
+          const int s0 = stretch_state(V, q, wa, wb, rs[z.a0], rs[z.b0]);
+          const int s1 = s0 != 1 && z.b1 > z.a1
+                           ? stretch_state(V, q, wa, wb, rs[z.a1], rs[z.b1]) : 0;
+          st = (s0 == 1 || s1 == 1) ? 1 : (st | s0 | s1);
         }
       }
       if (__ballot(st == 1))
@@ -166,49 +232,79 @@ __device__ int wave_overlap_state(const VoidIndex& V, const OverlapQuery& q, uin
 // the periodic distance but outside the reference's scan (overlap state 2)
 // gets bit 31 of its flat index set: the host re-tests it against the whole
 // catalogue.
+// The refusal counts of a workgroup, summed by wave and then by workgroup:
+// one atomic per workgroup and reason (same-address atomics from every wave
+// of a launch of millions of candidates serialise at the L2 and held the
+// launch's end).
+template <int R>
+__device__ __forceinline__ void block_add_counts(const uint32_t (&c)[R],
+  unsigned long long* __restrict__ out) {
+  __shared__ uint32_t part[R][256 / 64];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    uint32_t v = c[r];
+    for (int off = 32; off > 0; off >>= 1)
+      v += __shfl_xor(v, off, 64);
+    if (lane == 0)
+      part[r][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < R) {
+    uint64_t t = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++)
+      t += part[threadIdx.x][w];
+    if (t)
+      atomicAdd(&out[threadIdx.x], (unsigned long long)t);
+  }
+}
+
+// Workgroups of the candidate kernels below: enough to fill the device, each
+// striding over its share of the candidates (consecutive threads, consecutive
+// candidates).
+constexpr uint32_t CAND_BLOCKS = 2048;
+
 __global__ void __launch_bounds__(256) prefilter_kernel(uint32_t* __restrict__ flat,
   uint32_t n, uint32_t grid_n, float l, float box, const uint64_t* __restrict__ mask,
   int32_t proxy, float radius, float f, float scale, VoidIndex V,
   uint32_t* __restrict__ keep, unsigned long long* __restrict__ counts,
   uint8_t* __restrict__ why_out) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t why = 0;  // 0 kept, 1 masked, 2 proxy, 3 overlap
-  if (k < n) {
+  uint32_t refused[3] = {0u, 0u, 0u};  // masked, proxy, overlap
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += gridDim.x * blockDim.x) {
+    uint32_t why = 0;  // 0 kept, 1 masked, 2 proxy, 3 overlap
     const uint32_t node = flat[k];
     bool edge = false;
-    if (mask_bit(mask, node)) {
+    const uint32_t ix = node / (grid_n * grid_n);
+    const uint32_t iy = (node / grid_n) % grid_n;
+    const uint32_t iz = node % grid_n;
+    // the node's word and, with a pole distance, the poles' words loaded
+    // together; the tests then decide in the reference's order
+    const uint64_t own = mask[node >> 6];
+    const bool poles = proxy > 0 && overlap_quick(mask, grid_n, ix, iy, iz, (uint32_t)proxy);
+    if ((own >> (node & 63)) & 1ull) {
       why = 1;
-    } else {
-      const uint32_t ix = node / (grid_n * grid_n);
-      const uint32_t iy = (node / grid_n) % grid_n;
-      const uint32_t iz = node % grid_n;
-      if (proxy > 0 && overlap_quick(mask, grid_n, ix, iy, iz, (uint32_t)proxy)) {
-        why = 2;
-      } else if (V.n) {
-        OverlapQuery q;
-        overlap_query(q, (float)ix * l, (float)iy * l, (float)iz * l, radius, V.max_r, box,
-          f, scale, V.inv_l);
-        const int s = overlap_state(V, q);
-        if (s == 1)
-          why = 3;
-        else
-          edge = s == 2;
-      }
+    } else if (poles) {
+      why = 2;
+    } else if (V.n) {
+      OverlapQuery q;
+      overlap_query(q, (float)ix * l, (float)iy * l, (float)iz * l, radius, V.max_r, box,
+        f, scale, V.inv_l);
+      const int s = overlap_state(V, q);
+      if (s == 1)
+        why = 3;
+      else
+        edge = s == 2;
     }
     keep[k] = why == 0 ? 1u : 0u;
     if (why_out)
       why_out[k] = (uint8_t)why;
     if (edge)
       flat[k] = node | 0x80000000u;
+    if (why)
+      refused[why - 1]++;
   }
-  // refusal counts, one atomic per wavefront and reason (same-address
-  // atomics per candidate serialise at the L2)
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t r = 1; r <= 3; r++) {
-    const uint64_t votes = __ballot(why == r);
-    if (lane == 0 && votes)
-      atomicAdd(&counts[r - 1], (unsigned long long)__popcll(votes));
-  }
+  block_add_counts<3>(refused, counts);
 }
 
 // The overlap state of each grown sphere (radius >= 0) against the catalogue
@@ -701,7 +797,8 @@ int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
   const uint64_t* mask, int32_t proxy, float radius, float f, float scale,
   const VoidIndex& V, uint32_t* keep, uint64_t* d_counts, uint8_t* why) {
   if (n)
-    prefilter_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(flat, n, grid_n, l, box,
+    prefilter_kernel<<<std::min<unsigned>(grid_for(n, 256), CAND_BLOCKS), 256, 0, stream()>>>(
+      flat, n, grid_n, l, box,
       mask, proxy, radius, f, scale, V, keep, reinterpret_cast<unsigned long long*>(d_counts),
       why);
   SIF_HIP(hipGetLastError());
@@ -711,23 +808,19 @@ int prefilter(uint32_t* flat, uint32_t n, uint32_t grid_n, float l, float box,
 __global__ void __launch_bounds__(256) masked_since_kernel(const uint32_t* __restrict__ flat,
   const uint8_t* __restrict__ why, uint64_t n, const uint64_t* __restrict__ mask,
   unsigned long long* __restrict__ out) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t w = 4;  // none
-  if (k < n && mask_bit(mask, flat[k] & 0x7fffffffu))
-    w = why[k];
-  // one atomic per wavefront and verdict
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t r = 0; r < 4; r++) {
-    const uint64_t votes = __ballot(w == r);
-    if (lane == 0 && votes)
-      atomicAdd(&out[r], (unsigned long long)__popcll(votes));
-  }
+  uint32_t c[4] = {0u, 0u, 0u, 0u};
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    if (mask_bit(mask, flat[k] & 0x7fffffffu))
+      c[why[k]]++;
+  block_add_counts<4>(c, out);
 }
 
 int count_masked_since(const uint32_t* flat, const uint8_t* why, uint64_t n,
   const uint64_t* mask, uint64_t* d_out) {
   if (n)
-    masked_since_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(flat, why, n, mask,
+    masked_since_kernel<<<std::min<unsigned>(grid_for(n, 256), CAND_BLOCKS), 256, 0,
+      stream()>>>(flat, why, n, mask,
       reinterpret_cast<unsigned long long*>(d_out));
   SIF_HIP(hipGetLastError());
   return SIF_OK;
