@@ -148,30 +148,42 @@ __global__ void __launch_bounds__(SC_THREADS) cand_count_kernel(
 }
 
 // The tiles' spans into one (span[0] AND, span[1] OR), one block.
+// The keys' AND and OR over all tiles and, for a scan of cell lists, the
+// cells listed (the per-tile counts summed here, not by an atomic per tile).
 __global__ void __launch_bounds__(1024) span_reduce_kernel(const uint32_t* __restrict__ tile_span,
-  uint64_t tiles, uint32_t* __restrict__ span) {
+  uint64_t tiles, uint32_t* __restrict__ span, const uint32_t* __restrict__ tile_listed,
+  uint64_t* __restrict__ listed) {
   __shared__ uint32_t w_and[16], w_or[16];
+  __shared__ uint64_t w_sum[16];
   uint32_t a = 0xffffffffu, o = 0u;
+  uint64_t c = 0;
   for (uint64_t i = threadIdx.x; i < tiles; i += blockDim.x) {
     a &= tile_span[2 * i];
     o |= tile_span[2 * i + 1];
+    if (tile_listed)
+      c += tile_listed[i];
   }
   for (int off = 32; off > 0; off >>= 1) {
     a &= __shfl_xor(a, off, 64);
     o |= __shfl_xor(o, off, 64);
+    c += __shfl_xor(c, off, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     w_and[threadIdx.x >> 6] = a;
     w_or[threadIdx.x >> 6] = o;
+    w_sum[threadIdx.x >> 6] = c;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
       a &= w_and[w];
       o |= w_or[w];
+      c += w_sum[w];
     }
     span[0] = a;
     span[1] = o;
+    if (tile_listed)
+      *listed = c;
   }
 }
 
@@ -204,7 +216,7 @@ __global__ void cand_unkey_kernel(const uint32_t* __restrict__ keys, uint64_t n,
 }
 
 namespace {
-thread_local DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span;
+thread_local DeviceBuffer g_tiles, g_tile_pos, g_count, g_tile_span, g_tile_listed;
 }
 
 // --- the same scan from cell lists (CellLists, a listing smoothing's) ---
@@ -223,7 +235,7 @@ __global__ void __launch_bounds__(SC_THREADS) list_count_kernel(
   const uint32_t* __restrict__ lkeys, const uint16_t* __restrict__ llocal,
   const uint32_t* __restrict__ lcount, uint64_t blocks, const uint64_t* __restrict__ mask,
   uint64_t* __restrict__ tile_counts, uint32_t* __restrict__ tile_span,
-  unsigned long long* __restrict__ listed) {
+  uint32_t* __restrict__ tile_listed) {
   __shared__ uint32_t w_and[SC_THREADS / 64], w_or[SC_THREADS / 64], w_cnt[SC_THREADS / 64];
   uint32_t cnt = 0, k_and = 0xffffffffu, k_or = 0u, c_all = 0;
   for (uint32_t j = 0; j < LISTS_PER_TILE; j++) {
@@ -264,8 +276,7 @@ __global__ void __launch_bounds__(SC_THREADS) list_count_kernel(
     tile_counts[blockIdx.x] = all;
     tile_span[2 * (uint64_t)blockIdx.x] = a;
     tile_span[2 * (uint64_t)blockIdx.x + 1] = o;
-    if (c_all)
-      atomicAdd(listed, (unsigned long long)c_all);
+    tile_listed[blockIdx.x] = c_all;
   }
 }
 
@@ -355,6 +366,8 @@ int scan_begin(ScanJob& J) {
     st = g_count.reserve(32);  // the count, the keys' AND and OR, the cells listed
   if (st == SIF_OK)
     st = g_tile_span.reserve((tiles ? tiles : 1) * 8);
+  if (st == SIF_OK && J.lists)
+    st = g_tile_listed.reserve((tiles ? tiles : 1) * 4);
   if (st != SIF_OK)
     return st;
   if (!g_back)
@@ -365,17 +378,17 @@ int scan_begin(ScanJob& J) {
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SCAN);
   if (J.lists) {
-    SIF_HIP(hipMemsetAsync(g_count.as<uint64_t>() + 2, 0, 8, stream()));
     list_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.lists->keys,
       J.lists->local, J.lists->count, J.lists->blocks, J.mask, g_tiles.as<uint64_t>(),
-      g_tile_span.as<uint32_t>(), reinterpret_cast<unsigned long long*>(g_count.as<uint64_t>() + 2));
+      g_tile_span.as<uint32_t>(), g_tile_listed.as<uint32_t>());
   } else {
     cand_count_kernel<<<(unsigned)tiles, SC_THREADS, 0, stream()>>>(J.delta, J.mask, J.total,
       J.t, J.minima_n, g_tiles.as<uint64_t>(), g_tile_span.as<uint32_t>());
   }
   SIF_HIP(hipGetLastError());
   span_reduce_kernel<<<1, 1024, 0, stream()>>>(g_tile_span.as<uint32_t>(), tiles,
-    g_count.as<uint32_t>() + 2);
+    g_count.as<uint32_t>() + 2, J.lists ? g_tile_listed.as<uint32_t>() : nullptr,
+    g_count.as<uint64_t>() + 2);
   SIF_HIP(hipGetLastError());
   st = exclusive_scan_u64(g_tiles.as<uint64_t>(), tiles, g_tile_pos.as<uint64_t>(),
     g_count.as<uint64_t>());
