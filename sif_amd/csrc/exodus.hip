@@ -675,6 +675,21 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   struct PreProbe {
     bool node, pole, pole_r;
     int8_t state, state_r;
+    // the commit's verdict if no earlier candidate of the batch that it
+    // accepts affects this one (COMMIT_* below), or COMMIT_IN_ORDER when only
+    // the ordered pass can tell (a state 2 sends it to the reference's scan
+    // of the catalogue as it stands then)
+    uint8_t verdict;
+  };
+  enum : uint8_t {
+    COMMIT_ACCEPT = 0,
+    COMMIT_NODE,          // masked node
+    COMMIT_PROXY,         // pole probe at the rung
+    COMMIT_OVERLAP,       // overlap at the rung radius
+    COMMIT_RESCALE_FAIL,  // did not grow
+    COMMIT_GROWN_PROXY,   // pole probe at the grown radius
+    COMMIT_GROWN_OVERLAP, // overlap at the grown radius
+    COMMIT_IN_ORDER
   };
   std::vector<PreProbe> pre;
   std::vector<uint8_t> is_taken;  // accepted, by batch place
@@ -1368,6 +1383,32 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
               P.state_r = (int8_t)p2.state;
             }
           }
+          // The ordered pass's tests below with no effect from the batch:
+          // exact_overlap decided from the two states alone, or not at all
+          // here (a 2 on either side, then only the ordered pass may scan).
+          auto decided = [](uint32_t old_state, int new_state) -> int {
+            if (old_state == 1 || (old_state == 0 && new_state == 1))
+              return 1;
+            if (old_state == 0 && new_state == 0)
+              return 0;
+            return -1;
+          };
+          const int32_t pole = r >= 0.0f ? pole_r : 0;
+          int ov = 0;
+          if (P.node)
+            P.verdict = COMMIT_NODE;
+          else if (proxy > 0 && P.pole)
+            P.verdict = COMMIT_PROXY;
+          else if ((ov = decided(B.edge[b] ? 2u : 0u, P.state)) != 0)
+            P.verdict = ov > 0 ? COMMIT_OVERLAP : COMMIT_IN_ORDER;
+          else if (r < 0.0f)
+            P.verdict = COMMIT_RESCALE_FAIL;
+          else if (pole > 0 && ((gs & GROWN_POLE_BIT) || P.pole_r))
+            P.verdict = COMMIT_GROWN_PROXY;
+          else if ((ov = decided(gs & 3u, P.state_r)) != 0)
+            P.verdict = ov > 0 ? COMMIT_GROWN_OVERLAP : COMMIT_IN_ORDER;
+          else
+            P.verdict = COMMIT_ACCEPT;
           pre[b] = P;
         }
       });
@@ -1389,6 +1430,15 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         *y = (float)*iy * l;
         *z = (float)*iz * l;
       };
+      auto accept = [&](uint32_t b, float x, float y, float z) {
+        const float r = B.h_rad[b];
+        V.add(x, y, z, r, B.h_nin[b]);
+        if (r / radius > 2.0f)  // exodus_finder.c:2321-2328
+          mismatched++;
+        is_taken[b] = 1;
+        taken.push_back(b);
+        accepted++;
+      };
       for (uint32_t b = 0; b < B.count; b++) {
         const PreProbe P = pre[b];
         const uint32_t n_eff = B.h_cnt[b];
@@ -1397,6 +1447,32 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         if (P.node) {
           count_refusal(1);
           continue;
+        }
+        // No listed effect from a candidate accepted so far: the verdict
+        // found for every candidate at once stands (the tests below, every
+        // effect empty, give it).
+        if (P.verdict != COMMIT_IN_ORDER && n_eff < BATCH_EFF_FULL &&
+            eff0 + n_eff <= B.eff_capacity) {
+          const PairEffect* eff = B.h_eff + eff0;
+          bool touched = false;
+          for (uint32_t e = 0; e < n_eff && !touched; e++)
+            touched = is_taken[eff[e].a] != 0;
+          if (!touched) {
+            switch (P.verdict) {
+            case COMMIT_PROXY: count_refusal(2); break;
+            case COMMIT_OVERLAP: count_refusal(3); break;
+            case COMMIT_RESCALE_FAIL: S.rescale_fail++; break;
+            case COMMIT_GROWN_PROXY: S.proxy++; break;
+            case COMMIT_GROWN_OVERLAP: S.overlap++; break;
+            default: {  // COMMIT_ACCEPT
+              uint32_t ix, iy, iz;
+              float x, y, z;
+              centre_of(b, &ix, &iy, &iz, &x, &y, &z);
+              accept(b, x, y, z);
+            }
+            }
+            continue;
+          }
         }
         uint32_t ix, iy, iz;
         float x, y, z;
@@ -1465,20 +1541,20 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
           S.overlap++;
           continue;
         }
-        V.add(x, y, z, r, B.h_nin[b]);
-        if (r / radius > 2.0f)  // exodus_finder.c:2321-2328
-          mismatched++;
-        is_taken[b] = 1;
-        taken.push_back(b);
-        accepted++;
+        accept(b, x, y, z);
       }
       // the rung's list, for the next commits' probes and the mask
-      for (uint32_t a : taken) {
-        uint32_t jx, jy, jz;
-        float ax, ay, az;
-        centre_of(a, &jx, &jy, &jz, &ax, &ay, &az);
-        RV.add(ax, ay, az, B.h_rad[a]);
-      }
+      RV.add_many(
+        taken.size(),
+        [&](uint64_t i, float* x, float* y, float* z, float* r) {
+          const uint32_t a = taken[i];
+          uint32_t jx, jy, jz;
+          centre_of(a, &jx, &jy, &jz, x, y, z);
+          *r = B.h_rad[a];
+        },
+        [&](uint64_t m, const std::function<void(uint64_t, uint64_t)>& fn) {
+          host_pool().run(m, fn);
+        });
       S.accepted += accepted;
       *accepted_out = accepted;
       if (timing_active()) {

@@ -115,7 +115,6 @@ struct RungVoids {
   bool columns = true;                // kept only when asked for
   std::vector<Rec> rec;
   std::vector<int32_t> head;
-  std::vector<uint32_t> touched;
   std::vector<uint64_t> occupied;  // one bit per cell: a probe skips empty cells
   float box = 0.0f, inv_l = 0.0f, max_r = 0.0f;
   void init(float box_) {
@@ -126,11 +125,10 @@ struct RungVoids {
   }
   size_t size() const { return rec.size(); }
   void clear() {
-    for (uint32_t c : touched) {
-      head[c] = -1;
-      occupied[c >> 6] = 0ull;
+    for (const Rec& v : rec) {  // the cells holding a void, each at least once
+      head[v.cell] = -1;
+      occupied[v.cell >> 6] = 0ull;
     }
-    touched.clear();
     cx.clear();
     cy.clear();
     cz.clear();
@@ -149,14 +147,55 @@ struct RungVoids {
     const uint32_t c =
       (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
       void_axis_cell(z, inv_l);
-    if (head[c] == -1) {
-      touched.push_back(c);
-      occupied[c >> 6] |= 1ull << (c & 63);
-    }
+    occupied[c >> 6] |= 1ull << (c & 63);
     rec.push_back(Rec{x, y, z, rr, head[c], c});
     head[c] = k;
     if (rr > max_r)
       max_r = rr;
+  }
+
+  // Appends n voids at once, get(i, &x, &y, &z, &r) giving the i-th, over
+  // run(n, fn(begin, end)), a pool's parallel for. Each void's record and
+  // columns go to the slot its place fixes, and each cell's list is linked by
+  // an atomic exchange of the cell's head, so a cell lists its voids in no
+  // fixed order (walk() visits them in no particular order anyway). Added one
+  // by one, each void's writes go to lines that other cores have just read
+  // in probes, a transfer between cores apiece; spread over the pool, they
+  // overlap.
+  template <class Get, class Run>
+  void add_many(size_t n, Get get, Run run) {
+    if (n == 0)
+      return;
+    const size_t base = rec.size();
+    rec.resize(base + n);
+    if (columns) {
+      cx.resize(base + n);
+      cy.resize(base + n);
+      cz.resize(base + n);
+      r.resize(base + n);
+    }
+    run(n, [&](uint64_t b0, uint64_t b1) {
+      for (uint64_t i = b0; i < b1; i++) {
+        float x, y, z, rr;
+        get(i, &x, &y, &z, &rr);
+        const size_t k = base + i;
+        if (columns) {
+          cx[k] = x;
+          cy[k] = y;
+          cz[k] = z;
+          r[k] = rr;
+        }
+        const uint32_t c =
+          (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
+          void_axis_cell(z, inv_l);
+        const int32_t prev = __atomic_exchange_n(&head[c], (int32_t)k, __ATOMIC_RELAXED);
+        rec[k] = Rec{x, y, z, rr, prev, c};
+        __atomic_fetch_or(&occupied[c >> 6], 1ull << (c & 63), __ATOMIC_RELAXED);
+      }
+    });
+    for (size_t k = base; k < base + n; k++)
+      if (rec[k].r > max_r)
+        max_r = rec[k].r;
   }
 
   // Calls visit(void index, record, wa, wb, wc) for every void stored in the

@@ -15,6 +15,7 @@
 // success.
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <vector>
 
 #include "batch_pairs.h"
@@ -140,6 +141,69 @@ int main(int argc, char** argv) {
           "round %d candidate %u effect %d differs", round, b, e);
     }
   }
+  // The rung's voids added at once on the pool (add_many) and one by one:
+  // the same columns, the same largest radius and the same answers to the
+  // commit's probes, over two fillings (the second after clear()).
+  RungVoids one, many;
+  one.init(box);
+  many.init(box);
+  for (int fill = 0; fill < 2; fill++) {
+    one.clear();
+    many.clear();
+    const uint32_t nv = fill ? 700 : 2500;
+    std::vector<float> vx(nv), vy(nv), vz(nv), vr(nv);
+    for (uint32_t i = 0; i < nv; i++) {
+      vx[i] = rng.uniform(0, box);
+      vy[i] = rng.uniform(0, box);
+      vz[i] = i % 4 == 0 ? rng.uniform(0, 3) : rng.uniform(0, box);
+      vr[i] = rng.uniform(1.0f, 16.0f);
+      one.add(vx[i], vy[i], vz[i], vr[i]);
+    }
+    many.add_many(
+      nv,
+      [&](uint64_t i, float* x, float* y, float* z, float* r) {
+        *x = vx[i];
+        *y = vy[i];
+        *z = vz[i];
+        *r = vr[i];
+      },
+      [&](uint64_t m, const std::function<void(uint64_t, uint64_t)>& fn) { pool.run(m, fn); });
+    EXPECT(one.cx == many.cx && one.cy == many.cy && one.cz == many.cz && one.r == many.r,
+      "fill %d: columns differ", fill);
+    EXPECT(one.max_r == many.max_r, "fill %d: largest radius differs", fill);
+    const uint64_t m = 4000;
+    std::vector<RungProbe> a(m), b(m);
+    std::vector<uint32_t> cand(3 * m);
+    for (uint64_t t = 0; t < m; t++) {
+      cand[3 * t] = (uint32_t)(rng.next() % n);
+      cand[3 * t + 1] = (uint32_t)(rng.next() % n);
+      cand[3 * t + 2] = t % 3 == 0 ? (uint32_t)(rng.next() % 3) : (uint32_t)(rng.next() % n);
+    }
+    auto probe = [&](const RungVoids& RV, uint64_t t) {
+      const uint32_t ix = cand[3 * t], iy = cand[3 * t + 1], iz = cand[3 * t + 2];
+      const float x = (float)ix * l, y = (float)iy * l, z = (float)iz * l;
+      OverlapQuery oq;
+      overlap_query(oq, x, y, z, 6.0f, RV.max_r, box, f, scale, RV.inv_l);
+      return RV.probe(&oq, ix, iy, iz, x, y, z, true, 2, scale, l, n);
+    };
+    pool.run(m, [&](uint64_t lo, uint64_t hi) {
+      for (uint64_t t = lo; t < hi; t++) {
+        a[t] = probe(one, t);
+        b[t] = probe(many, t);
+      }
+    });
+    // the caller refuses on the first of node, pole, overlap that holds, and
+    // the probe stops once that can no longer change: what it decides on
+    // must agree (a cell lists its voids in another order in the two lists)
+    for (uint64_t t = 0; t < m; t++) {
+      bool same = a[t].node == b[t].node;
+      if (same && !a[t].node)
+        same = a[t].pole == b[t].pole && (a[t].pole || a[t].state == b[t].state);
+      EXPECT(same, "fill %d candidate %llu: probes of the two lists differ", fill,
+        (unsigned long long)t);
+    }
+  }
+
   EXPECT(V.unlinked_reads.load() == 0, "%llu unlinked reads",
     (unsigned long long)V.unlinked_reads.load());
 
