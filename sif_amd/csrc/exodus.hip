@@ -672,15 +672,18 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
   CL.columns = false;
   std::vector<uint32_t> taken;  // a commit's accepted candidates, in order
   // A candidate's tests against the rung's voids as its commit begins
+  // Two bytes a candidate: the pool threads write them and the ordered pass
+  // reads them from those threads' cores, a line at a time.
   struct PreProbe {
-    bool node, pole, pole_r;
-    int8_t state, state_r;
+    uint16_t node : 1, pole : 1, pole_r : 1;
+    uint16_t state : 2, state_r : 2;  // overlap states (overlap.h)
     // the commit's verdict if no earlier candidate of the batch that it
     // accepts affects this one (COMMIT_* below), or COMMIT_IN_ORDER when only
     // the ordered pass can tell (a state 2 sends it to the reference's scan
     // of the catalogue as it stands then)
-    uint8_t verdict;
+    uint16_t verdict : 3;
   };
+  static_assert(sizeof(PreProbe) == 2, "two bytes a candidate");
   enum : uint8_t {
     COMMIT_ACCEPT = 0,
     COMMIT_NODE,          // masked node
