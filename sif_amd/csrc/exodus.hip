@@ -1648,8 +1648,17 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         ty(RV.cy.begin() + rv_mark, RV.cy.end()), tz(RV.cz.begin() + rv_mark, RV.cz.end()),
         tr(RV.r.begin() + rv_mark, RV.r.end());
       RV.clear();
-      for (size_t v = 0; v < tx.size(); v++)
-        RV.add(tx[v], ty[v], tz[v], tr[v]);
+      RV.add_many(
+        tx.size(),
+        [&](uint64_t v, float* x, float* y, float* z, float* r) {
+          *x = tx[v];
+          *y = ty[v];
+          *z = tz[v];
+          *r = tr[v];
+        },
+        [&](uint64_t m, const std::function<void(uint64_t, uint64_t)>& fn) {
+          host_pool().run(m, fn);
+        });
     } else {
       RV.clear();
     }

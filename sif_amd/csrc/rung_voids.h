@@ -8,6 +8,7 @@
 // pass over the cells around the candidate. walk() visits the voids of any
 // such reach one by one, for callers that need each void's answer.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -114,20 +115,38 @@ struct RungVoids {
   std::vector<float> cx, cy, cz, r;  // the same voids as columns, for upload
   bool columns = true;                // kept only when asked for
   std::vector<Rec> rec;
-  std::vector<int32_t> head;
-  std::vector<uint64_t> occupied;  // one bit per cell: a probe skips empty cells
+  // Per cell, its first void as (generation << 32 | index), and per row of
+  // 32 cells along z, the cells holding a void as (generation << 32 | bits):
+  // an entry of an earlier generation stands for an empty cell or row, so
+  // clear() starts a generation instead of rewriting the cells (lines that
+  // other cores have just read in probes).
+  std::vector<uint64_t> head;
+  std::vector<uint64_t> rows;
+  uint32_t gen = 1;
   float box = 0.0f, inv_l = 0.0f, max_r = 0.0f;
   void init(float box_) {
     box = box_;
     inv_l = (float)VOID_CELLS / box;
-    head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, -1);
-    occupied.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS / 64, 0ull);
+    head.assign(VOID_CELLS * VOID_CELLS * VOID_CELLS, 0ull);
+    rows.assign(VOID_CELLS * VOID_CELLS, 0ull);
+    gen = 1;
   }
   size_t size() const { return rec.size(); }
+  // a cell's first void, or -1
+  int32_t first(uint32_t cell) const {
+    const uint64_t e = head[cell];
+    return (uint32_t)(e >> 32) == gen ? (int32_t)(uint32_t)e : -1;
+  }
+  // a row's cells holding a void, as bits
+  uint32_t row_cells(uint32_t row) const {
+    const uint64_t e = rows[row];
+    return (uint32_t)(e >> 32) == gen ? (uint32_t)e : 0u;
+  }
   void clear() {
-    for (const Rec& v : rec) {  // the cells holding a void, each at least once
-      head[v.cell] = -1;
-      occupied[v.cell >> 6] = 0ull;
+    if (++gen == 0) {  // after 2^32 - 1 generations: every entry reads empty again
+      std::fill(head.begin(), head.end(), 0ull);
+      std::fill(rows.begin(), rows.end(), 0ull);
+      gen = 1;
     }
     cx.clear();
     cy.clear();
@@ -147,9 +166,10 @@ struct RungVoids {
     const uint32_t c =
       (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
       void_axis_cell(z, inv_l);
-    occupied[c >> 6] |= 1ull << (c & 63);
-    rec.push_back(Rec{x, y, z, rr, head[c], c});
-    head[c] = k;
+    const uint64_t g = (uint64_t)gen << 32;
+    rows[c >> 5] = g | row_cells(c >> 5) | (1u << (c & 31u));
+    rec.push_back(Rec{x, y, z, rr, first(c), c});
+    head[c] = g | (uint32_t)k;
     if (rr > max_r)
       max_r = rr;
   }
@@ -188,9 +208,20 @@ struct RungVoids {
         const uint32_t c =
           (void_axis_cell(x, inv_l) * VOID_CELLS + void_axis_cell(y, inv_l)) * VOID_CELLS +
           void_axis_cell(z, inv_l);
-        const int32_t prev = __atomic_exchange_n(&head[c], (int32_t)k, __ATOMIC_RELAXED);
-        rec[k] = Rec{x, y, z, rr, prev, c};
-        __atomic_fetch_or(&occupied[c >> 6], 1ull << (c & 63), __ATOMIC_RELAXED);
+        const uint64_t g = (uint64_t)gen << 32;
+        const uint64_t prev = __atomic_exchange_n(&head[c], g | (uint32_t)k, __ATOMIC_RELAXED);
+        rec[k] = Rec{x, y, z, rr, (uint32_t)(prev >> 32) == gen ? (int32_t)(uint32_t)prev : -1, c};
+        // the row's bit, the row's word restarted if it is of an earlier
+        // generation
+        uint64_t old = __atomic_load_n(&rows[c >> 5], __ATOMIC_RELAXED);
+        for (;;) {
+          const uint64_t want =
+            ((uint32_t)(old >> 32) == gen ? old : g) | (1ull << (c & 31u));
+          if (want == old ||
+              __atomic_compare_exchange_n(&rows[c >> 5], &old, want, true, __ATOMIC_RELAXED,
+                __ATOMIC_RELAXED))
+            break;
+        }
       }
     });
     for (size_t k = base; k < base + n; k++)
@@ -235,12 +266,12 @@ struct RungVoids {
         const uint32_t wb = void_wrap(b);
         // a row of 32 cells is half an occupancy word: only its set cells
         const uint32_t row = wa * VOID_CELLS + wb;
-        uint32_t cells = (uint32_t)(occupied[row >> 1] >> ((row & 1u) * 32u)) & zmask;
+        uint32_t cells = row_cells(row) & zmask;
         while (cells) {
           const uint32_t wc = (uint32_t)__builtin_ctz(cells);
           cells &= cells - 1u;
           const uint32_t cell = row * VOID_CELLS + wc;
-          for (int32_t i = head[cell]; i != -1; i = rec[i].next)
+          for (int32_t i = first(cell); i != -1; i = rec[i].next)
             if (!visit((uint32_t)i, rec[i], wa, wb, wc))
               return;
         }
