@@ -887,13 +887,6 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if (e != SIF_OK)
       return e;
     timing_end(SIF_AMD_STAGE_PREFILTER);
-    // the survivors sorted now, on the compaction's count on the device (the
-    // candidates masked since are still among them), so the sort runs while
-    // the host finishes the rung under way
-    e = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), rs.nc,
-      rs.key_differ, compact2_u32_count());
-    if (e != SIF_OK)
-      return e;
     rs.step = 2;
     return SIF_OK;
   };
@@ -912,6 +905,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
         hipMemcpyDeviceToHost, stream()));
     }
     int e = compact2_u32_finish(&rs.ns);
+    if (e == SIF_OK)  // the survivors are a subset of the scanned candidates
+      e = sort_candidates(g_dev.skeys.as<uint32_t>(), g_dev.surv.as<uint32_t>(), rs.ns,
+        rs.key_differ);
     if (e != SIF_OK)
       return e;
     surv.resize(rs.ns);

@@ -82,9 +82,8 @@ int scan_candidates(const float* delta, const uint64_t* mask, uint64_t total,
 int scan_candidates_flat(const float* delta, const uint64_t* mask, uint64_t total,
   float t, uint32_t minima_n, uint32_t* keys, uint32_t* flat, uint64_t cap, uint64_t* count,
   uint32_t* key_differ = nullptr);
-// d_n (optional): the count on the device, n then its bound.
 int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n,
-  uint32_t key_differ = 0xffffffffu, const uint64_t* d_n = nullptr);
+  uint32_t key_differ = 0xffffffffu);
 // scan_candidates_flat (minima_n 0) from the lists a listing smoothing made
 // of the same field: the listed cells whose mask bit is clear.
 int scan_lists_flat(const CellLists& lists, const uint64_t* mask, uint64_t total,
@@ -234,9 +233,6 @@ int compact2_u32(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uin
 int compact2_u32_begin(const uint32_t* a, const uint32_t* b, const uint32_t* keep, uint64_t n,
   uint32_t* out_a, uint32_t* out_b);
 int compact2_u32_finish(uint64_t* count);
-// The count of the last compact2_u32_begin on the device, for work enqueued
-// after it on the library stream.
-const uint64_t* compact2_u32_count();
 
 // One exodus run on device-resident data (exodus.hip).
 struct RunSpec {

@@ -164,11 +164,8 @@ constexpr int RS_ITEMS = 8;
 constexpr uint32_t RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_RADIX = 16;
 
-// n: the count, or with d_n its bound (the count is then *d_n, on the device)
 __global__ void __launch_bounds__(RS_THREADS) radix_hist(const uint32_t* __restrict__ keys,
-  uint32_t n_arg, const uint64_t* __restrict__ d_n, int shift, uint32_t n_tiles,
-  uint32_t* __restrict__ hist) {
-  const uint32_t n = d_n ? (uint32_t)*d_n : n_arg;
+  uint32_t n, int shift, uint32_t n_tiles, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[RS_RADIX];
   if (threadIdx.x < RS_RADIX)
     h[threadIdx.x] = 0;
@@ -185,10 +182,9 @@ __global__ void __launch_bounds__(RS_THREADS) radix_hist(const uint32_t* __restr
 // Each thread owns RS_ITEMS consecutive items, so block order = thread-major
 // then item order, and the rank of an item among equal digits follows it.
 __global__ void __launch_bounds__(RS_THREADS) radix_scatter(const uint32_t* __restrict__ keys,
-  const uint32_t* __restrict__ vals, uint32_t n_arg, const uint64_t* __restrict__ d_n, int shift,
-  uint32_t n_tiles, const uint64_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
+  const uint32_t* __restrict__ vals, uint32_t n, int shift, uint32_t n_tiles,
+  const uint64_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
   uint32_t* __restrict__ vals_out) {
-  const uint32_t n = d_n ? (uint32_t)*d_n : n_arg;
   __shared__ uint32_t cnt[RS_RADIX][RS_THREADS];
   const uint32_t base = blockIdx.x * RS_TILE + threadIdx.x * RS_ITEMS;
   uint32_t k[RS_ITEMS], v[RS_ITEMS];
@@ -233,18 +229,7 @@ namespace {
 thread_local DeviceBuffer g_rs_hist, g_rs_offs, g_rs_k, g_rs_v;
 }
 
-__global__ void __launch_bounds__(256) copy2_u32_kernel(const uint32_t* __restrict__ ka,
-  const uint32_t* __restrict__ va, uint32_t n_arg, const uint64_t* __restrict__ d_n,
-  uint32_t* __restrict__ kb, uint32_t* __restrict__ vb) {
-  const uint32_t n = d_n ? (uint32_t)*d_n : n_arg;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    kb[i] = ka[i];
-    vb[i] = va[i];
-  }
-}
-
-int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t differ,
-  const uint64_t* d_n) {
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t differ) {
   if (n <= 1)
     return SIF_OK;
   const uint32_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
@@ -264,13 +249,13 @@ int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_
     const int shift = 4 * p;
     if (((differ >> shift) & (RS_RADIX - 1)) == 0)
       continue;  // one digit value in every key: the identity permutation
-    radix_hist<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, n, d_n, shift, n_tiles,
+    radix_hist<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, n, shift, n_tiles,
       g_rs_hist.as<uint32_t>());
     SIF_HIP(hipGetLastError());
     st = exclusive_scan_u32(g_rs_hist.as<uint32_t>(), hsize, g_rs_offs.as<uint64_t>(), nullptr);
     if (st != SIF_OK)
       return st;
-    radix_scatter<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, va, n, d_n, shift, n_tiles,
+    radix_scatter<<<n_tiles, RS_THREADS, 0, stream()>>>(ka, va, n, shift, n_tiles,
       g_rs_offs.as<uint64_t>(), kb, vb);
     SIF_HIP(hipGetLastError());
     uint32_t* t = ka;
@@ -281,14 +266,8 @@ int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_
     vb = t;
   }
   if (ka != keys) {
-    if (d_n) {  // the count alone, not its bound
-      copy2_u32_kernel<<<std::min<unsigned>(grid_for(n, 256), 1024), 256, 0, stream()>>>(ka, va,
-        n, d_n, keys, vals);
-      SIF_HIP(hipGetLastError());
-    } else {
-      SIF_HIP(hipMemcpyAsync(keys, ka, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
-      SIF_HIP(hipMemcpyAsync(vals, va, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
-    }
+    SIF_HIP(hipMemcpyAsync(keys, ka, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
+    SIF_HIP(hipMemcpyAsync(vals, va, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
   }
   return SIF_OK;
 }
@@ -362,10 +341,6 @@ int compact2_u32_begin(const uint32_t* a, const uint32_t* b, const uint32_t* kee
   SIF_HIP(hipMemcpyAsync(g_ccount_host, g_ccount.ptr, 8, hipMemcpyDeviceToHost, stream()));
   SIF_HIP(hipEventRecord(g_ccount_ready, stream()));
   return SIF_OK;
-}
-
-const uint64_t* compact2_u32_count() {
-  return g_ccount.as<uint64_t>();
 }
 
 int compact2_u32_finish(uint64_t* count) {

@@ -497,9 +497,7 @@ constexpr uint32_t SMALL_SORT = 8192;
 constexpr uint32_t SMALL_SORT_THREADS = 1024;
 
 __global__ void __launch_bounds__(SMALL_SORT_THREADS) small_sort_kernel(
-  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n_arg,
-  const uint64_t* __restrict__ d_n) {
-  const uint32_t n = d_n ? (uint32_t)*d_n : n_arg;
+  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n) {
   __shared__ uint64_t s[SMALL_SORT];
   uint32_t m = 1;
   while (m < n)
@@ -548,19 +546,18 @@ __global__ void __launch_bounds__(SMALL_SORT_THREADS) small_sort_kernel(
 // ordered by (depth, flat index). Bit 31 of a flat value rides along
 // untouched. key_differ: the bits on which the keys may differ (from the scan
 // that listed them, or a superset of them).
-int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n, uint32_t key_differ,
-  const uint64_t* d_n) {
+int sort_candidates(uint32_t* keys, uint32_t* flat, uint64_t n, uint32_t key_differ) {
   if (n == 0)
     return SIF_OK;
   timing_begin(SIF_AMD_STAGE_SORT);
   int st = SIF_OK;
   if (n <= SMALL_SORT) {
-    small_sort_kernel<<<1, SMALL_SORT_THREADS, 0, stream()>>>(keys, flat, (uint32_t)n, d_n);
+    small_sort_kernel<<<1, SMALL_SORT_THREADS, 0, stream()>>>(keys, flat, (uint32_t)n);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       st = hip_check(e, "small_sort_kernel");
   } else {
-    st = radix_sort_u32(keys, flat, (uint32_t)n, 32, key_differ, d_n);
+    st = radix_sort_u32(keys, flat, (uint32_t)n, 32, key_differ);
   }
   timing_end(SIF_AMD_STAGE_SORT);
   return st;
