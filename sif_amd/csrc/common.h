@@ -40,6 +40,13 @@ void timing_bytes(int stage, double bytes);  // algorithmic bytes of a stage
 // Checks a HIP call; on failure records SIF_ERR_ALLOC/INVALID and returns it.
 int hip_check(hipError_t e, const char* what);
 
+// The host's waits on the rung loop's critical path: the event (or all work
+// enqueued so far on the stream) polled until it completes, which wakes the
+// host as soon as the device is done instead of after a blocking wait's
+// sleep. The caller's thread spins meanwhile.
+int wait_event(hipEvent_t e);
+int wait_stream(hipStream_t s);
+
 #define SIF_HIP(call)                                                          \
   do {                                                                         \
     int _s = ::sif_amd::hip_check((call), #call);                              \

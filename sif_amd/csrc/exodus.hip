@@ -319,8 +319,8 @@ int append_voids(const float* x, const float* y, const float* z, const float* r,
   // rest of the stream, which may hold a growth launch in flight)
   if (!D.staged)
     SIF_HIP(hipEventCreateWithFlags(&D.staged, hipEventDisableTiming));
-  else
-    SIF_HIP(hipEventSynchronize(D.staged));
+  else if (int w = wait_event(D.staged))
+    return w;
   if (k > D.hcap) {
     for (float** p : {&D.hx, &D.hy, &D.hz, &D.hr})
       if (*p) {
@@ -914,7 +914,9 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     if (rs.ns)
       SIF_HIP(hipMemcpyAsync(surv.data(), g_dev.surv.ptr, rs.ns * 4, hipMemcpyDeviceToHost,
         stream()));
-    SIF_HIP(hipStreamSynchronize(stream()));
+    e = wait_stream(stream());
+    if (e != SIF_OK)
+      return e;
     if (went_ahead && rs.nc)
       for (int w = 0; w < 4; w++)
         rs.since[w] = g_dev.h_since[w];
@@ -1337,7 +1339,11 @@ int exodus_run(const RunSpec& R, sif_catalogue_t** out_cat, uint64_t** n_in_out,
     // effects of the candidates it has accepted.
     auto combine = [](int a, int b) { return (a == 1 || b == 1) ? 1 : (a | b); };
     auto commit = [&](BatchSlot& B, uint64_t* accepted_out) -> int {
-      SIF_HIP(hipEventSynchronize(B.done));
+      {
+        const int w = wait_event(B.done);
+        if (w != SIF_OK)
+          return w;
+      }
       const double t2 = now_ms();
       for (uint32_t b = 0; b < B.count; b++)
         if (B.h_reason[b] == 5) {

@@ -344,7 +344,8 @@ int compact2_u32_begin(const uint32_t* a, const uint32_t* b, const uint32_t* kee
 }
 
 int compact2_u32_finish(uint64_t* count) {
-  SIF_HIP(hipEventSynchronize(g_ccount_ready));
+  if (int w = wait_event(g_ccount_ready))
+    return w;
   *count = *g_ccount_host;
   return SIF_OK;
 }

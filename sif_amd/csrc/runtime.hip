@@ -200,6 +200,25 @@ __global__ void __launch_bounds__(256) copy_kernel(uint4* __restrict__ dst,
     dst[i] = src[i];
 }
 
+int wait_event(hipEvent_t e) {
+  for (;;) {
+    const hipError_t r = hipEventQuery(e);
+    if (r == hipSuccess)
+      return SIF_OK;
+    if (r != hipErrorNotReady)
+      return hip_check(r, "hipEventQuery");
+    __builtin_ia32_pause();
+  }
+}
+
+int wait_stream(hipStream_t s) {
+  thread_local hipEvent_t done = nullptr;
+  if (!done)
+    SIF_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  SIF_HIP(hipEventRecord(done, s));
+  return wait_event(done);
+}
+
 }  // namespace sif_amd
 
 using namespace sif_amd;

@@ -414,7 +414,8 @@ int scan_begin(ScanJob& J) {
 int scan_finish(ScanJob& J, uint32_t* keys, uint32_t* flat, uint64_t cap) {
   if (!J.begun)
     return SIF_OK;
-  SIF_HIP(hipEventSynchronize(g_back_ready));
+  if (int w = wait_event(g_back_ready))
+    return w;
   const uint64_t n = g_back[0];
   J.count = n;
   J.key_differ = (uint32_t)g_back[1] ^ (uint32_t)(g_back[1] >> 32);
