@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(RS_THREADS) radix_scatter(const uint32_t* __re
   const uint32_t* __restrict__ vals, uint32_t n, int shift, uint32_t n_tiles,
   const uint64_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
   uint32_t* __restrict__ vals_out) {
-  __shared__ uint32_t cnt[RS_RADIX][RS_THREADS];
+  __shared__ uint32_t cnt[RS_RADIX][RS_THREADS + 1];  // padded: the segment sums read across digits
   const uint32_t base = blockIdx.x * RS_TILE + threadIdx.x * RS_ITEMS;
   uint32_t k[RS_ITEMS], v[RS_ITEMS];
   uint32_t local[RS_RADIX];
@@ -201,13 +201,30 @@ __global__ void __launch_bounds__(RS_THREADS) radix_scatter(const uint32_t* __re
   for (int d = 0; d < RS_RADIX; d++)
     cnt[d][threadIdx.x] = local[d];
   __syncthreads();
-  // exclusive scan along threads for each digit: 16 digits, each by 16 lanes
-  // of work; simple and fixed-order.
-  if (threadIdx.x < RS_RADIX) {
-    uint32_t run = 0;
-    for (int t = 0; t < RS_THREADS; t++) {
-      const uint32_t c = cnt[threadIdx.x][t];
-      cnt[threadIdx.x][t] = run;
+  // exclusive scan along threads for each digit, in three short steps: thread
+  // (digit d, segment g) sums its 16 threads' counts, one thread a digit
+  // scans the 16 segment sums, and each segment writes its prefixes.
+  __shared__ uint32_t seg_sum[RS_RADIX][RS_THREADS / 16];
+  {
+    const int d = threadIdx.x & (RS_RADIX - 1), g = threadIdx.x >> 4;
+    uint32_t sum = 0;
+    for (int t = 0; t < 16; t++)
+      sum += cnt[d][g * 16 + t];
+    seg_sum[d][g] = sum;
+    __syncthreads();
+    if (threadIdx.x < RS_RADIX) {
+      uint32_t run = 0;
+      for (int q = 0; q < RS_THREADS / 16; q++) {
+        const uint32_t c = seg_sum[threadIdx.x][q];
+        seg_sum[threadIdx.x][q] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+    uint32_t run = seg_sum[d][g];
+    for (int t = 0; t < 16; t++) {
+      const uint32_t c = cnt[d][g * 16 + t];
+      cnt[d][g * 16 + t] = run;
       run += c;
     }
   }
