@@ -575,7 +575,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU figures are not part of "
-              "it (see DESIGN.md for how the path would shard)", file=sys.stderr)
+              "it; one box would shard by x-slab (CIC, scan) and pencil (FFT)", file=sys.stderr)
         sys.exit(2)
 
     from sif_amd import _native as N

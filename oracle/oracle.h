@@ -12,9 +12,12 @@
  * this library, and only as the checker or the timed CPU baseline. Nothing in
  * sif_amd/ links or calls it.
  *
- * Pinning (what stops a plausible mistake here from passing unseen) is listed
- * per function in DESIGN.md's oracle table; functions marked "parity unpinned"
- * there are checked only against the GPU.
+ * Each function is pinned by checks that a plausible mistake here would fail:
+ * the reference's own test vectors, closed forms, brute forces over every
+ * tracer or node, and invariants, named in the tests under tests/ that call
+ * it. Parity unpinned: FFTW's output bits (FFTW is absent; the smoothed field
+ * is checked against an FP64 transform instead), and so the end-to-end
+ * catalogue of the reference's float path with FFTW's bits.
  */
 #ifndef SIF_ORACLE_H
 #define SIF_ORACLE_H
