@@ -134,8 +134,57 @@ namespace {
 thread_local DeviceBuffer g_scan_scratch[4];
 }
 
+// The whole scan of a short array in one workgroup (one launch instead of
+// three): the array is staged in LDS with coalesced loads, each thread sums a
+// contiguous run of it, the runs' sums are scanned across the workgroup, and
+// each thread writes its run's prefixes.
+constexpr uint32_t ONE_SCAN_THREADS = 1024;
+constexpr uint64_t ONE_SCAN_MAX = 16384;
+
+__global__ void __launch_bounds__(ONE_SCAN_THREADS) one_block_scan_u32(
+  const uint32_t* __restrict__ in, uint32_t n, uint64_t* __restrict__ out,
+  uint64_t* __restrict__ total) {
+  __shared__ uint64_t wave_sums[ONE_SCAN_THREADS / 64];
+  __shared__ uint32_t vals[ONE_SCAN_MAX];
+  for (uint32_t i = threadIdx.x; i < n; i += ONE_SCAN_THREADS)
+    vals[i] = in[i];
+  __syncthreads();
+  const uint32_t per = (n + ONE_SCAN_THREADS - 1) / ONE_SCAN_THREADS;
+  const uint32_t b = min(threadIdx.x * per, n), e = min(b + per, n);
+  uint64_t v = 0;
+  for (uint32_t i = b; i < e; i++)
+    v += vals[i];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(x, off, 64);
+    if (lane >= off)
+      x += y;
+  }
+  if (lane == 63)
+    wave_sums[wave] = x;
+  __syncthreads();
+  uint64_t run = x - v, all = 0;
+  for (int w = 0; w < (int)(ONE_SCAN_THREADS / 64); w++) {
+    if (w < wave)
+      run += wave_sums[w];
+    all += wave_sums[w];
+  }
+  for (uint32_t i = b; i < e; i++) {
+    out[i] = run;
+    run += vals[i];
+  }
+  if (total && threadIdx.x == 0)
+    *total = all;
+}
+
 int exclusive_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out,
   uint64_t* total_dev) {
+  if (n > SCAN_TILE && n <= ONE_SCAN_MAX) {  // one tile is one launch already
+    one_block_scan_u32<<<1, ONE_SCAN_THREADS, 0, stream()>>>(in, (uint32_t)n, out, total_dev);
+    SIF_HIP(hipGetLastError());
+    return SIF_OK;
+  }
   const uint64_t n_tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   int st = g_scan_scratch[0].reserve(2 * (n_tiles + 1) * sizeof(uint64_t));
   if (st != SIF_OK)
