@@ -295,9 +295,122 @@ namespace {
 thread_local DeviceBuffer g_rs_hist, g_rs_offs, g_rs_k, g_rs_v;
 }
 
+// --- the same sort with 8-bit digits for short lists (half the passes) ---
+// A tile is R8_ROUNDS rounds of one item a thread, taken in index order; the
+// rank of an item among its tile's items of the same digit comes from the
+// items of earlier rounds (run), of earlier waves in its round (wave counts)
+// and of lower lanes in its wave (the lanes whose digit matches, found by
+// eight ballots).
+constexpr int R8_RADIX = 256;
+constexpr int R8_THREADS = 256;
+constexpr int R8_ROUNDS = 8;
+constexpr uint32_t R8_TILE = R8_THREADS * R8_ROUNDS;
+constexpr uint32_t R8_MAX_TILES = 64;  // 256 x 64 counts: a one-workgroup scan
+
+__global__ void __launch_bounds__(R8_THREADS) radix8_hist(const uint32_t* __restrict__ keys,
+  uint32_t n, int shift, uint32_t n_tiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[R8_RADIX];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * R8_TILE;
+  for (int j = 0; j < R8_ROUNDS; j++) {
+    const uint32_t i = base + j * R8_THREADS + threadIdx.x;
+    if (i < n)
+      atomicAdd(&h[(keys[i] >> shift) & (R8_RADIX - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(R8_THREADS) radix8_scatter(const uint32_t* __restrict__ keys,
+  const uint32_t* __restrict__ vals, uint32_t n, int shift, uint32_t n_tiles,
+  const uint64_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
+  uint32_t* __restrict__ vals_out) {
+  __shared__ uint32_t run[R8_RADIX];
+  __shared__ uint32_t wcount[R8_THREADS / 64][R8_RADIX];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  run[threadIdx.x] = (uint32_t)offsets[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
+  const uint32_t base = blockIdx.x * R8_TILE;
+  for (int j = 0; j < R8_ROUNDS; j++) {
+    const uint32_t i = base + j * R8_THREADS + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t k = 0, v = 0;
+    if (valid) {
+      k = keys[i];
+      v = vals[i];
+    }
+    const uint32_t d = (k >> shift) & (R8_RADIX - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const int bit = (d >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    // thread t clears the counts it summed into run[t] last round
+#pragma unroll
+    for (int w = 0; w < R8_THREADS / 64; w++)
+      wcount[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t rank = (uint32_t)__popcll(peers & below);
+    if (valid && rank == 0)
+      wcount[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int w = 0; w < wave; w++)
+        pos += wcount[w][d];
+      keys_out[pos] = k;
+      vals_out[pos] = v;
+    }
+    __syncthreads();  // every read of run before it moves on
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < R8_THREADS / 64; w++)
+      add += wcount[w][threadIdx.x];
+    run[threadIdx.x] += add;
+  }
+}
+
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t differ) {
   if (n <= 1)
     return SIF_OK;
+  const uint32_t n_tiles8 = (n + R8_TILE - 1) / R8_TILE;
+  if (n_tiles8 <= R8_MAX_TILES) {
+    const uint64_t hsize = (uint64_t)R8_RADIX * n_tiles8;
+    int st = g_rs_hist.reserve(hsize * sizeof(uint32_t));
+    if (st == SIF_OK)
+      st = g_rs_offs.reserve((hsize + 1) * sizeof(uint64_t));
+    if (st == SIF_OK)
+      st = g_rs_k.reserve((uint64_t)n * sizeof(uint32_t));
+    if (st == SIF_OK)
+      st = g_rs_v.reserve((uint64_t)n * sizeof(uint32_t));
+    if (st != SIF_OK)
+      return st;
+    uint32_t *ka = keys, *va = vals, *kb = g_rs_k.as<uint32_t>(), *vb = g_rs_v.as<uint32_t>();
+    for (int p = 0; p < (bits + 7) / 8; p++) {
+      const int shift = 8 * p;
+      if (((differ >> shift) & (R8_RADIX - 1)) == 0)
+        continue;  // one digit value in every key: the identity permutation
+      radix8_hist<<<n_tiles8, R8_THREADS, 0, stream()>>>(ka, n, shift, n_tiles8,
+        g_rs_hist.as<uint32_t>());
+      SIF_HIP(hipGetLastError());
+      st = exclusive_scan_u32(g_rs_hist.as<uint32_t>(), hsize, g_rs_offs.as<uint64_t>(), nullptr);
+      if (st != SIF_OK)
+        return st;
+      radix8_scatter<<<n_tiles8, R8_THREADS, 0, stream()>>>(ka, va, n, shift, n_tiles8,
+        g_rs_offs.as<uint64_t>(), kb, vb);
+      SIF_HIP(hipGetLastError());
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
+    if (ka != keys) {
+      SIF_HIP(hipMemcpyAsync(keys, ka, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
+      SIF_HIP(hipMemcpyAsync(vals, va, (uint64_t)n * 4, hipMemcpyDeviceToDevice, stream()));
+    }
+    return SIF_OK;
+  }
   const uint32_t n_tiles = (n + RS_TILE - 1) / RS_TILE;
   const uint64_t hsize = (uint64_t)RS_RADIX * n_tiles;
   int st = g_rs_hist.reserve(hsize * sizeof(uint32_t));
